@@ -1,0 +1,308 @@
+/*
+ * wpt.h - C ABI of the MI355X wavefront path tracer (libwpt.so).
+ *
+ * Plain C structs, plain pointers and sizes.  Every call returns a status
+ * code (WPT_OK == 0) instead of exiting the process; wpt_last_error() gives
+ * the message of the last failure on the calling thread.  Handles are opaque
+ * and owned by the caller, who destroys a scene only after every renderer
+ * created from it.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   uploadScene / destroyDeviceScene   src/scene/scene.cu:336-420, include/scene/scene.cuh:241
+ *   RenderDriver(...)::run             include/renderer/render_driver.cuh:52-59, src/renderer/render_driver.cu:451-482
+ *   RenderTaskRenderer::render         include/renderer/render_task.cuh:21-29, src/renderer/render_task.cu:26-114
+ *   traceRay / isOccluded              include/renderer/device_traversal.cuh:1035-1184
+ *   appendWarpCompacted                include/renderer/queue_compaction.cuh:8-29
+ */
+#ifndef WPT_H
+#define WPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WPT_API __attribute__((visibility("default")))
+
+#define WPT_ABI_VERSION 1
+
+/* Status codes. */
+#define WPT_OK 0
+#define WPT_ERROR_INVALID_ARGUMENT 1
+#define WPT_ERROR_DEVICE 2
+#define WPT_ERROR_OUT_OF_MEMORY 3
+#define WPT_ERROR_UNSUPPORTED 4
+#define WPT_ERROR_INVALID_TASK 5
+
+/* AccelerationMode, reference include/scene/scene.cuh:21. */
+#define WPT_ACCEL_BVH 0u
+#define WPT_ACCEL_DIRECT 1u
+
+/* MaterialType, reference include/scene/material.cuh:18-24. */
+#define WPT_MATERIAL_DIFFUSE 0u
+#define WPT_MATERIAL_METAL 1u
+#define WPT_MATERIAL_DIELECTRIC 2u
+#define WPT_MATERIAL_EMISSIVE 3u
+#define WPT_MATERIAL_SPECULAR_GLOSSINESS 4u
+
+/* RngStrategy, reference include/core/rng_types.hpp:5-8. */
+#define WPT_RNG_XORSHIFT32 0u
+#define WPT_RNG_PCG32 1u
+
+/* Scheduler: the hipGraph-captured staged wavefront or the persistent
+ * cooperative kernel (reference render_driver.cu:264-348). */
+#define WPT_MODE_GRAPH 0u
+#define WPT_MODE_PERSISTENT 1u
+
+/* Closest-hit categories, in the reference's test order
+ * (device_traversal.cuh:1129-1149). */
+#define WPT_HIT_NONE 0u
+#define WPT_HIT_SPHERE 1u
+#define WPT_HIT_STATIC_TRIANGLE 2u
+#define WPT_HIT_INSTANCE_TRIANGLE 3u
+#define WPT_HIT_FLAT_TRIANGLE 4u
+
+#define WPT_INVALID_INDEX 0xffffffffu
+
+/* Sphere + placement; reference Sphere (geometry.cuh:6-11) and SceneSphere. */
+typedef struct wpt_sphere {
+    float center[3];
+    float radius;
+    uint32_t material;
+    uint32_t acceleration; /* WPT_ACCEL_BVH | WPT_ACCEL_DIRECT */
+} wpt_sphere;
+
+/* Triangle; the geometry and shading subset of reference Triangle
+ * (geometry.cuh:13-49).  flags: bit0 hasVertexNormals, bit1 hasTexcoords. */
+typedef struct wpt_triangle {
+    float v0[3], v1[3], v2[3];
+    float n0[3], n1[3], n2[3];
+    float uv0[2], uv1[2], uv2[2];
+    float tangent[3];
+    float bitangent[3];
+    uint32_t material;
+    uint32_t flags;
+} wpt_triangle;
+
+#define WPT_TRIANGLE_HAS_NORMALS 1u
+#define WPT_TRIANGLE_HAS_TEXCOORDS 2u
+
+/* Mesh = a range of wpt_scene_desc.mesh_triangles (reference MeshAsset). */
+typedef struct wpt_mesh {
+    uint32_t first_triangle;
+    uint32_t triangle_count;
+} wpt_mesh;
+
+/* Affine instance transform with its explicit inverse basis
+ * (reference InstanceTransform, scene.cuh:29-55). */
+typedef struct wpt_transform {
+    float local_to_world[3][3]; /* columns X, Y, Z */
+    float world_to_local[3][3]; /* columns X, Y, Z */
+    float translation[3];
+} wpt_transform;
+
+/* Reference SceneInstance (scene.cuh:104-109). */
+typedef struct wpt_instance {
+    uint32_t mesh;
+    uint32_t material_override; /* WPT_INVALID_INDEX: keep triangle materials */
+    uint32_t acceleration;      /* WPT_ACCEL_BVH | WPT_ACCEL_DIRECT */
+    uint32_t reserved;
+    wpt_transform transform;
+} wpt_instance;
+
+/* Reference Material (material.cuh:31-70), untextured subset. */
+typedef struct wpt_material {
+    uint32_t type;
+    float albedo[3];
+    float emission[3];
+    float roughness;
+    float ior;
+    float metallic;
+    float opacity;
+    float alpha_cutoff;
+    uint32_t alpha_masked;
+    float attenuation_color[3];
+    float attenuation_distance;
+    float volume_density;
+    float specular[3];
+    float glossiness;
+} wpt_material;
+
+typedef struct wpt_scene_desc {
+    const wpt_sphere* spheres;
+    uint32_t sphere_count;
+    const wpt_triangle* static_triangles; /* world space, tested without a BVH */
+    uint32_t static_triangle_count;
+    const wpt_triangle* flat_triangles;   /* world space, own BVH2 (reference scene.triangles) */
+    uint32_t flat_triangle_count;
+    const wpt_triangle* mesh_triangles;   /* mesh-local space */
+    uint32_t mesh_triangle_count;
+    const wpt_mesh* meshes;
+    uint32_t mesh_count;
+    const wpt_instance* instances;
+    uint32_t instance_count;
+    const wpt_material* materials;
+    uint32_t material_count;
+    uint32_t black_background;            /* reference HostScene.blackBackground */
+    uint32_t ignore_direct_light_occlusion;
+} wpt_scene_desc;
+
+/* Reference Camera (include/scene/camera.cuh:7-27). */
+typedef struct wpt_camera {
+    float position[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float vertical_fov; /* radians */
+    float aspect_ratio;
+} wpt_camera;
+
+/* Reference RenderConfig render fields (include/config.hpp:11-37). */
+typedef struct wpt_render_config {
+    uint32_t width, height;
+    uint32_t tiled;
+    uint32_t tile_width, tile_height;
+    uint32_t max_depth;
+    uint32_t rr_start_depth;
+    uint32_t samples_per_pixel;
+    uint32_t block_size;
+    uint32_t rng_strategy;          /* WPT_RNG_* */
+    float indirect_clamp;           /* 0 disables */
+    uint32_t mode;                  /* WPT_MODE_* */
+    uint32_t persistent_blocks;     /* 0: occupancy-derived */
+    uint32_t intersection_debug;
+    uint32_t shading_normal_debug;
+    uint32_t profile_queues;
+} wpt_render_config;
+
+/* Reference RenderTask / RenderTaskId / DistributedTile
+ * (include/distributed/task.hpp:6-29). */
+typedef struct wpt_task {
+    uint64_t job_fingerprint;
+    uint32_t tile_ordinal;
+    uint32_t sample_start;
+    uint32_t sample_count;
+    uint32_t x, y, width, height;
+} wpt_task;
+
+/* Closest-hit record: a defined identity for every hit (the reference Hit
+ * carries none).  kind: WPT_HIT_*; instance: instance index for
+ * WPT_HIT_INSTANCE_TRIANGLE else WPT_INVALID_INDEX; primitive: sphere index,
+ * static/flat triangle index, or mesh-local triangle index. */
+typedef struct wpt_hit {
+    float t;
+    float u, v;
+    uint32_t kind;
+    uint32_t instance;
+    uint32_t primitive;
+    uint32_t material;
+    uint32_t light_index;
+} wpt_hit;
+
+/* Per-render counters (reference --profile-queues, render_driver.cu:393-400). */
+typedef struct wpt_stats {
+    uint64_t closest_queries;
+    uint64_t shadow_queries;
+    uint64_t samples;          /* pixels x spp rendered */
+    double trace_milliseconds; /* device time of the last render call */
+    uint32_t queue_counts[64]; /* last launch, per bounce */
+    uint32_t queue_count_depth;
+} wpt_stats;
+
+typedef struct wpt_scene wpt_scene;
+typedef struct wpt_renderer wpt_renderer;
+
+/* ---- library ---------------------------------------------------------- */
+WPT_API int wpt_abi_version(void);
+WPT_API const char* wpt_last_error(void);
+/* Number of visible gfx devices (0 without a GPU); never fails. */
+WPT_API int wpt_device_count(void);
+
+/* ---- scene: reference uploadScene / destroyDeviceScene ---------------- */
+/* Partitions spheres/instances by AccelerationMode, builds BLAS per mesh with
+ * a BVH instance and the TLAS over typed leaves (scene.cu:123-239), builds the
+ * triangle-light alias table (scene.cu:82-121) and uploads to the device. */
+WPT_API int wpt_scene_create(const wpt_scene_desc* desc, wpt_scene** out);
+/* Same as wpt_scene_create but host-only: no device upload.  Used to inspect
+ * partitions and BVHs on machines without a GPU. */
+WPT_API int wpt_scene_create_host(const wpt_scene_desc* desc, wpt_scene** out);
+WPT_API int wpt_scene_destroy(wpt_scene* scene);
+
+/* Scene inspection (partition lists, light table, BVH export). */
+typedef struct wpt_scene_info {
+    uint32_t sphere_count, direct_sphere_count;
+    uint32_t instance_count, direct_instance_count;
+    uint32_t tlas_leaf_count, tlas_node_count;
+    uint32_t blas_count;
+    uint32_t light_count;
+    uint32_t static_triangle_count, flat_triangle_count;
+    uint32_t flat_node_count;
+    uint32_t material_count;
+} wpt_scene_info;
+WPT_API int wpt_scene_info_get(const wpt_scene* scene, wpt_scene_info* info);
+/* which: 0 direct spheres, 1 direct instances, 2 TLAS leaves packed as
+ * (type << 31 | index) with type 1 = sphere, 0 = mesh instance. */
+WPT_API int wpt_scene_get_list(const wpt_scene* scene, uint32_t which, uint32_t* out, uint32_t capacity,
+                               uint32_t* count);
+/* BVH2 in the reference's node convention (NXB::BVH2::Node: float bmin[3],
+ * bmax[3], uint32 left, right; leaf iff left == 0xffffffff with right = the
+ * primitive index; root = node_count - 1).  bvh: 0 TLAS, 1 flat triangles,
+ * 2 + m BLAS of mesh m (node_count 0 when the mesh has no BLAS). */
+typedef struct wpt_bvh2_node {
+    float bmin[3];
+    float bmax[3];
+    uint32_t left, right;
+} wpt_bvh2_node;
+WPT_API int wpt_scene_export_bvh2(const wpt_scene* scene, uint32_t bvh, wpt_bvh2_node* out, uint32_t capacity,
+                                  uint32_t* node_count);
+/* Triangle lights (TriangleLight, geometry.cuh:51-59) as 16 floats each:
+ * v0,v1,v2,normal (12), area, material (bits), selectionPdf, pad; and the
+ * alias table as (probability, alias bits) pairs. */
+WPT_API int wpt_scene_export_lights(const wpt_scene* scene, float* lights16, float* alias2, uint32_t capacity,
+                                    uint32_t* count);
+/* Light offset per instance (reference MeshInstance.lightOffset) and light
+ * index per static/mesh triangle as assigned by the scene build. */
+WPT_API int wpt_scene_export_light_indices(const wpt_scene* scene, uint32_t* instance_light_offset,
+                                           uint32_t* static_light_index, uint32_t* mesh_light_index);
+
+/* ---- traversal: reference traceRay / isOccluded ----------------------- */
+/* Device pointers; rays are SoA: origin x,y,z then direction x,y,z, each an
+ * array of n floats (rays6 = 6 consecutive arrays of n).  stream may be NULL. */
+WPT_API int wpt_trace_closest(const wpt_scene* scene, const float* d_rays6, uint32_t n, wpt_hit* d_hits,
+                              void* stream);
+WPT_API int wpt_trace_occluded(const wpt_scene* scene, const float* d_rays6, const float* d_tmax, uint32_t n,
+                               uint8_t* d_occluded, void* stream);
+
+/* ---- queue compaction: reference appendWarpCompacted ------------------ */
+/* Order-preserving wave64 compaction: out[k] = items[i] for the k-th i with
+ * flags[i] != 0, writes bounded by capacity; *d_count receives the number of
+ * active items (may exceed capacity, as in the reference).  Device pointers. */
+WPT_API int wpt_compact(const uint32_t* d_items, const uint8_t* d_flags, uint32_t n, uint32_t* d_out,
+                        uint32_t capacity, uint32_t* d_count, void* stream);
+
+/* ---- renderer: reference RenderDriver / RenderTaskRenderer ------------ */
+WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, const wpt_render_config* config,
+                                uint64_t job_fingerprint, wpt_renderer** out);
+WPT_API int wpt_renderer_destroy(wpt_renderer* renderer);
+/* Clears the framebuffer and sample counter (resetRenderSession). */
+WPT_API int wpt_renderer_reset(wpt_renderer* renderer);
+/* Renders samples [first, first+count) over all tiles, accumulating into the
+ * device framebuffer (renderSamples, render_driver.cu:354-401). */
+WPT_API int wpt_renderer_render_samples(wpt_renderer* renderer, uint32_t first_sample, uint32_t count);
+/* Copies the unnormalized framebuffer (w*h*3 floats) to host memory and
+ * reports the accumulated sample count (writeFinalImage's source data). */
+WPT_API int wpt_renderer_read_framebuffer(wpt_renderer* renderer, float* host_rgb, uint32_t* sample_count);
+/* Full render: reset + all spp + read back (RenderDriver::run). */
+WPT_API int wpt_render_frame(wpt_renderer* renderer, float* host_rgb, uint32_t* sample_count);
+/* One task: validates fingerprint/tile/sample range, clears the tile,
+ * renders the sample range, returns unnormalized tile sums (w*h*3). */
+WPT_API int wpt_render_task(wpt_renderer* renderer, const wpt_task* task, float* host_tile_rgb);
+WPT_API int wpt_renderer_last_stats(const wpt_renderer* renderer, wpt_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WPT_H */

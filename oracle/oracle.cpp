@@ -1,0 +1,1332 @@
+// oracle.cpp - TEST INFRASTRUCTURE ONLY.  A scalar C++ restatement of the
+// reference wavefront path tracer's render path, used as the parity checker
+// for the gfx950 kernels.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it; the product library never links it.
+//
+// Each function names the reference file:line it restates.  It is written in
+// the reference's own structure (own-bounds BVH nodes re-tested on pop, hit
+// attributes computed at every improvement, one path traced depth-first) and
+// templated on the real type: R = float is the FP32 restatement ("oracle A"),
+// R = double the FP64 variant ("oracle B").
+//
+// Shared dependency: elementary functions (sin, cos, tan, exp, log, pow) come
+// from the product's deterministic wpt_math.h, because libm and the device
+// library differ in the last ulp; that header is pinned separately against
+// libm by tests/test_math.py.  Everything else here is independent code.
+//
+// Pinning summary (see tests/): RNG known answers and seed uniqueness; Fresnel
+// and GGX closed forms from the reference's material tests; traversal vs the
+// independent FP64 brute force on the reference's mt19937 ray set; Walker
+// alias invariants; whole-image energy against analytic furnace cases.
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "../wavefront-path-tracer_amd/csrc/common/wpt_math.h"
+
+namespace {
+
+constexpr uint32_t kInvalid = 0xffffffffu;
+
+// ---------------------------------------------------------------- vectors
+template <class R>
+struct V3 {
+    R x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(R a, R b, R c) : x(a), y(b), z(c) {}
+    V3 operator-() const { return V3(-x, -y, -z); }
+    V3 operator+(const V3& v) const { return V3(x + v.x, y + v.y, z + v.z); }
+    V3 operator-(const V3& v) const { return V3(x - v.x, y - v.y, z - v.z); }
+    V3 operator*(R s) const { return V3(x * s, y * s, z * s); }
+    V3 operator/(R s) const {
+        R inv = R(1) / s;
+        return *this * inv;
+    }
+    V3& operator+=(const V3& v) {
+        x += v.x;
+        y += v.y;
+        z += v.z;
+        return *this;
+    }
+    V3& operator*=(R s) {
+        x *= s;
+        y *= s;
+        z *= s;
+        return *this;
+    }
+};
+template <class R> V3<R> operator*(R s, const V3<R>& v) { return v * s; }
+template <class R> R dot(const V3<R>& a, const V3<R>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> V3<R> cross(const V3<R>& a, const V3<R>& b) {
+    return V3<R>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <class R> R lengthSquared(const V3<R>& v) { return dot(v, v); }
+template <class R> R length(const V3<R>& v) { return std::sqrt(lengthSquared(v)); }
+template <class R> V3<R> normalize(const V3<R>& v) {
+    R inv = R(1) / std::sqrt(dot(v, v));
+    return v * inv;
+}
+template <class R> V3<R> hadamard(const V3<R>& a, const V3<R>& b) { return V3<R>(a.x * b.x, a.y * b.y, a.z * b.z); }
+
+template <class R> struct V2 { R x, y; };
+
+template <class R> V3<R> vec(const float v[3]) { return V3<R>(R(v[0]), R(v[1]), R(v[2])); }
+
+template <class R>
+struct Ray {
+    V3<R> o, d;
+    V3<R> at(R t) const { return o + d * t; }
+};
+
+// ---------------------------------------------------------------- elementary functions
+inline float fsin(float x) { return wpt::sinF(x); }
+inline float fcos(float x) { return wpt::cosF(x); }
+inline double fsin(double x) { double s, c; wpt::detail::sinCosD(x, s, c); return s; }
+inline double fcos(double x) { double s, c; wpt::detail::sinCosD(x, s, c); return c; }
+inline float ftan(float x) { return wpt::tanF(x); }
+inline double ftan(double x) { return fsin(x) / fcos(x); }
+inline float fexp(float x) { return wpt::expF(x); }
+inline double fexp(double x) { return wpt::expD(x); }
+inline float flog(float x) { return wpt::logF(x); }
+inline double flog(double x) { return wpt::logD(x); }
+inline float fpow(float x, float y) { return wpt::powF(x, y); }
+inline double fpow(double x, double y) {
+    if (y == 0.0) return 1.0;
+    if (x == 0.0) return y > 0.0 ? 0.0 : INFINITY;
+    if (x == 1.0) return 1.0;
+    return wpt::expD(y * wpt::logD(x));
+}
+
+// ---------------------------------------------------------------- RNG (include/core/rng.cuh, src/core/rng.cu)
+inline uint64_t splitMix64(uint64_t v) {
+    v += 0x9e3779b97f4a7c15ull;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebull;
+    return v ^ (v >> 31);
+}
+
+struct RngState {
+    uint64_t state;
+    uint32_t strategy;
+};
+
+RngState makeRngState(uint64_t key, uint32_t strategy) {  // rng.cuh:16-29
+    RngState r;
+    r.strategy = strategy;
+    if (strategy == WPT_RNG_XORSHIFT32) {
+        uint32_t seed = static_cast<uint32_t>(key) * 747796405u + 2891336453u;
+        r.state = seed == 0 ? 1u : seed;
+    } else {
+        r.state = splitMix64(key);
+    }
+    return r;
+}
+
+uint64_t pixelRngKey(uint32_t pixel, uint32_t sample, uint32_t strategy) {  // rng.cuh:32-41
+    if (strategy == WPT_RNG_XORSHIFT32) return static_cast<uint64_t>(pixel ^ (sample * 0x9e3779b9u));
+    return (static_cast<uint64_t>(sample) << 32) | pixel;
+}
+
+float randomFloat(RngState& s) {  // rng.cu:30-41
+    uint32_t value;
+    if (s.strategy == WPT_RNG_XORSHIFT32) {
+        uint32_t x = static_cast<uint32_t>(s.state);
+        x ^= x << 13;
+        x ^= x >> 17;
+        x ^= x << 5;
+        s.state = x;
+        value = x;
+    } else {
+        uint64_t old = s.state;
+        s.state = old * 6364136223846793005ull + 1442695040888963407ull;
+        uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = static_cast<uint32_t>(old >> 59u);
+        value = (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+    return static_cast<float>(value >> 8) * 0x1.0p-24f;
+}
+
+template <class R> R rnd(RngState& s) { return R(randomFloat(s)); }
+
+// ---------------------------------------------------------------- scene tables
+struct TriangleLight {
+    V3<float> v0, v1, v2, normal;
+    float area;
+    uint32_t material;
+    float selectionPdf;
+};
+
+struct AliasEntry {
+    float probability;
+    uint32_t alias;
+};
+
+struct Node {  // NXB::BVH2::Node as used by device_traversal.cuh
+    float bmin[3], bmax[3];
+    uint32_t left, right;
+};
+
+struct BvhView {
+    const wpt_bvh2_node* nodes = nullptr;
+    uint32_t count = 0;
+};
+
+}  // namespace
+
+struct oracle_handle {
+    wpt_scene_desc d;
+    BvhView tlas, flat;
+    std::vector<BvhView> blas;
+    std::vector<uint32_t> directSpheres, directInstances, leaves;
+    std::vector<uint32_t> staticLight;  // light index per static triangle
+    std::vector<uint32_t> meshLight;    // per mesh triangle (mesh-local numbering)
+    std::vector<uint32_t> instanceLightOffset;
+    std::vector<TriangleLight> lights;
+    std::vector<float> lightWeights;
+    std::vector<AliasEntry> alias;
+};
+
+namespace {
+
+const wpt_material& mat(const oracle_handle& h, uint32_t i) { return h.d.materials[i]; }
+
+// Luminance power registration (scene_host.cpp:60-106).
+void registerLight(oracle_handle& h, const V3<float>& v0, const V3<float>& v1, const V3<float>& v2, uint32_t material) {
+    V3<float> normal = normalize(cross(v1 - v0, v2 - v0));
+    float area = 0.5f * length(cross(v1 - v0, v2 - v0));
+    const wpt_material& m = mat(h, material);
+    float power = area * (0.2126f * m.emission[0] + 0.7152f * m.emission[1] + 0.0722f * m.emission[2]);
+    h.lights.push_back(TriangleLight{v0, v1, v2, normal, area, material, 0.0f});
+    h.lightWeights.push_back(power);
+}
+
+V3<float> xform(const wpt_transform& t, const V3<float>& v) {  // transformVector, scene.cuh:38-41
+    return vec<float>(t.local_to_world[0]) * v.x + vec<float>(t.local_to_world[1]) * v.y +
+        vec<float>(t.local_to_world[2]) * v.z;
+}
+
+void buildTables(oracle_handle& h) {
+    const wpt_scene_desc& d = h.d;
+    // Mesh light numbering, then per-instance world lights (scene_host.cpp:77-106).
+    h.meshLight.assign(d.mesh_triangle_count, kInvalid);
+    for (uint32_t m = 0; m < d.mesh_count; ++m) {
+        uint32_t count = 0;
+        for (uint32_t t = 0; t < d.meshes[m].triangle_count; ++t) {
+            uint32_t gi = d.meshes[m].first_triangle + t;
+            if (mat(h, d.mesh_triangles[gi].material).type == WPT_MATERIAL_EMISSIVE) h.meshLight[gi] = count++;
+        }
+    }
+    uint32_t offset = 0;
+    for (uint32_t i = 0; i < d.instance_count; ++i) {  // scene.cu:156-166
+        h.instanceLightOffset.push_back(offset);
+        const wpt_mesh& m = d.meshes[d.instances[i].mesh];
+        for (uint32_t t = 0; t < m.triangle_count; ++t)
+            if (h.meshLight[m.first_triangle + t] != kInvalid) ++offset;
+    }
+    for (uint32_t i = 0; i < d.instance_count; ++i) {
+        const wpt_instance& inst = d.instances[i];
+        const wpt_mesh& m = d.meshes[inst.mesh];
+        for (uint32_t t = 0; t < m.triangle_count; ++t) {
+            uint32_t gi = m.first_triangle + t;
+            if (h.meshLight[gi] == kInvalid) continue;
+            const wpt_triangle& tri = d.mesh_triangles[gi];
+            V3<float> tr = vec<float>(inst.transform.translation);
+            registerLight(h, xform(inst.transform, vec<float>(tri.v0)) + tr, xform(inst.transform, vec<float>(tri.v1)) + tr,
+                xform(inst.transform, vec<float>(tri.v2)) + tr, tri.material);
+        }
+    }
+    h.staticLight.assign(d.static_triangle_count, kInvalid);
+    for (uint32_t i = 0; i < d.static_triangle_count; ++i) {
+        const wpt_triangle& tri = d.static_triangles[i];
+        if (mat(h, tri.material).type != WPT_MATERIAL_EMISSIVE) continue;
+        h.staticLight[i] = static_cast<uint32_t>(h.lights.size());
+        registerLight(h, vec<float>(tri.v0), vec<float>(tri.v1), vec<float>(tri.v2), tri.material);
+    }
+    // Walker alias (scene.cu:86-114).
+    size_t n = h.lights.size();
+    h.alias.assign(n, AliasEntry{0.0f, 0});
+    if (n) {
+        float total = 0.0f;
+        for (float w : h.lightWeights) total += w;
+        std::vector<float> scaled(n);
+        std::vector<uint32_t> small, large;
+        for (uint32_t i = 0; i < n; ++i) {
+            h.lights[i].selectionPdf = h.lightWeights[i] / total;
+            scaled[i] = h.lights[i].selectionPdf * n;
+            (scaled[i] < 1.0f ? small : large).push_back(i);
+        }
+        while (!small.empty() && !large.empty()) {
+            uint32_t low = small.back();
+            small.pop_back();
+            uint32_t high = large.back();
+            large.pop_back();
+            h.alias[low] = AliasEntry{scaled[low], high};
+            scaled[high] = scaled[high] + scaled[low] - 1.0f;
+            (scaled[high] < 1.0f ? small : large).push_back(high);
+        }
+        for (uint32_t i : small) h.alias[i] = AliasEntry{1.0f, i};
+        for (uint32_t i : large) h.alias[i] = AliasEntry{1.0f, i};
+    }
+    // Partition (scene.cu:175-239).
+    for (uint32_t i = 0; i < d.sphere_count; ++i) {
+        if (d.spheres[i].acceleration == WPT_ACCEL_DIRECT) h.directSpheres.push_back(i);
+        else h.leaves.push_back(0x80000000u | i);
+    }
+    for (uint32_t i = 0; i < d.instance_count; ++i) {
+        if (d.instances[i].acceleration == WPT_ACCEL_DIRECT) {
+            h.directInstances.push_back(i);
+            continue;
+        }
+        if (d.meshes[d.instances[i].mesh].triangle_count == 0) continue;
+        h.leaves.push_back(i);
+    }
+}
+
+// ---------------------------------------------------------------- intersection tests
+template <class R>
+bool intersectSphere(const Ray<R>& ray, const wpt_sphere& s, R& t) {  // device_traversal.cuh:24-53
+    V3<R> oc = ray.o - vec<R>(s.center);
+    R a = dot(ray.d, ray.d);
+    R b = R(2) * dot(oc, ray.d);
+    R c = dot(oc, oc) - R(s.radius) * R(s.radius);
+    R disc = b * b - R(4) * a * c;
+    if (disc < R(0)) return false;
+    R sq = std::sqrt(disc);
+    R t0 = (-b - sq) / (R(2) * a);
+    R t1 = (-b + sq) / (R(2) * a);
+    const R tMin = R(1.0e-6f);
+    if (t0 > tMin) {
+        t = t0;
+        return true;
+    }
+    if (t1 > tMin) {
+        t = t1;
+        return true;
+    }
+    return false;
+}
+
+template <class R>
+bool intersectTriangle(const Ray<R>& ray, const wpt_triangle& tri, R& t, R& u, R& v) {  // :55-91
+    V3<R> v0 = vec<R>(tri.v0), v1 = vec<R>(tri.v1), v2 = vec<R>(tri.v2);
+    V3<R> e1 = v1 - v0;
+    V3<R> e2 = v2 - v0;
+    V3<R> p = cross(ray.d, e2);
+    R det = dot(e1, p);
+    R scale = std::fabs(e1.x) * (std::fabs(ray.d.y * e2.z) + std::fabs(ray.d.z * e2.y)) +
+        std::fabs(e1.y) * (std::fabs(ray.d.z * e2.x) + std::fabs(ray.d.x * e2.z)) +
+        std::fabs(e1.z) * (std::fabs(ray.d.x * e2.y) + std::fabs(ray.d.y * e2.x));
+    if (!std::isfinite(det) || std::fabs(det) <= R(8) * std::numeric_limits<R>::epsilon() * scale) return false;
+    R inv = R(1) / det;
+    V3<R> ov = ray.o - v0;
+    u = dot(ov, p) * inv;
+    if (u < R(0) || u > R(1)) return false;
+    V3<R> q = cross(ov, e1);
+    v = dot(ray.d, q) * inv;
+    if (v < R(0) || u + v > R(1)) return false;
+    t = dot(e2, q) * inv;
+    return t > R(1.0e-6f);
+}
+
+template <class R>
+bool intersectAabb(const Ray<R>& ray, const V3<R>& inv, const wpt_bvh2_node& n, R closestT, R& nearT) {  // :182-196
+    R tx0 = (R(n.bmin[0]) - ray.o.x) * inv.x;
+    R tx1 = (R(n.bmax[0]) - ray.o.x) * inv.x;
+    R ty0 = (R(n.bmin[1]) - ray.o.y) * inv.y;
+    R ty1 = (R(n.bmax[1]) - ray.o.y) * inv.y;
+    R tz0 = (R(n.bmin[2]) - ray.o.z) * inv.z;
+    R tz1 = (R(n.bmax[2]) - ray.o.z) * inv.z;
+    R tmin = std::fmax(std::fmin(tx0, tx1), std::fmax(std::fmin(ty0, ty1), std::fmin(tz0, tz1)));
+    R tmax = std::fmin(std::fmax(tx0, tx1), std::fmin(std::fmax(ty0, ty1), std::fmax(tz0, tz1)));
+    nearT = tmin;
+    return tmax >= std::fmax(tmin, R(1.0e-6f)) && tmin < closestT;
+}
+
+// ---------------------------------------------------------------- closest hit (TraceHitState style)
+template <class R>
+struct HitState {
+    R closestT = R(1e30f);
+    int closestMaterial = -1;
+    V3<R> geometricNormal, normal;
+    V2<R> uv{R(0), R(0)};
+    uint32_t lightIndex = kInvalid;
+    uint32_t sphereIndex = kInvalid;
+    // identity of the current winner
+    uint32_t kind = WPT_HIT_NONE, instance = kInvalid, prim = kInvalid;
+    R u = R(0), v = R(0);
+};
+
+template <class R>
+V3<R> triNormal(const wpt_triangle& tri, const Ray<R>& ray, R u, R v, bool faceForward) {  // :100-111
+    V3<R> n;
+    if (tri.flags & WPT_TRIANGLE_HAS_NORMALS)
+        n = normalize((R(1) - u - v) * vec<R>(tri.n0) + u * vec<R>(tri.n1) + v * vec<R>(tri.n2));
+    else
+        n = normalize(cross(vec<R>(tri.v1) - vec<R>(tri.v0), vec<R>(tri.v2) - vec<R>(tri.v0)));
+    return faceForward && dot(n, ray.d) > R(0) ? -n : n;
+}
+
+template <class R>
+V3<R> triGeometricNormal(const wpt_triangle& tri, const Ray<R>& ray, bool faceForward) {  // :113-118
+    V3<R> n = normalize(cross(vec<R>(tri.v1) - vec<R>(tri.v0), vec<R>(tri.v2) - vec<R>(tri.v0)));
+    return faceForward && dot(n, ray.d) > R(0) ? -n : n;
+}
+
+template <class R>
+V2<R> triUv(const wpt_triangle& tri, R u, R v) {  // :120-132
+    if (!(tri.flags & WPT_TRIANGLE_HAS_TEXCOORDS)) return V2<R>{R(0), R(0)};
+    R w = R(1) - u - v;
+    return V2<R>{w * R(tri.uv0[0]) + u * R(tri.uv1[0]) + v * R(tri.uv2[0]), w * R(tri.uv0[1]) + u * R(tri.uv1[1]) + v * R(tri.uv2[1])};
+}
+
+bool cutout(const oracle_handle& h, uint32_t m) {  // alpha-masked and (untextured) opacity below cutoff
+    const wpt_material& x = mat(h, m);
+    return x.alpha_masked && x.opacity < x.alpha_cutoff;
+}
+
+// intersectTriangleHit, device_traversal.cuh:137-180.
+template <class R>
+bool triangleHit(const oracle_handle& h, const Ray<R>& ray, const wpt_triangle& tri, uint32_t triLight, HitState<R>& hs,
+    uint32_t materialOverride, uint32_t kind, uint32_t instance, uint32_t prim) {
+    R t, u, v;
+    if (!intersectTriangle(ray, tri, t, u, v) || t >= hs.closestT) return false;
+    V2<R> uv = triUv(tri, u, v);
+    uint32_t m = materialOverride == kInvalid ? tri.material : materialOverride;
+    if (cutout(h, m)) return false;
+    bool ff = mat(h, m).type != WPT_MATERIAL_DIELECTRIC;
+    hs.closestT = t;
+    hs.closestMaterial = static_cast<int>(m);
+    hs.geometricNormal = triGeometricNormal(tri, ray, ff);
+    hs.normal = triNormal(tri, ray, u, v, ff);
+    hs.uv = uv;
+    hs.lightIndex = materialOverride == kInvalid ? triLight : kInvalid;
+    hs.kind = kind;
+    hs.instance = instance;
+    hs.prim = prim;
+    hs.u = u;
+    hs.v = v;
+    return true;
+}
+
+template <class R>
+void sphereHit(const oracle_handle& h, const Ray<R>& ray, uint32_t index, HitState<R>& hs) {  // :789-800
+    R t;
+    if (!intersectSphere(ray, h.d.spheres[index], t) || t > hs.closestT) return;
+    if (t == hs.closestT && index >= hs.sphereIndex) return;
+    hs.closestT = t;
+    hs.closestMaterial = static_cast<int>(h.d.spheres[index].material);
+    hs.sphereIndex = index;
+    hs.kind = WPT_HIT_SPHERE;
+    hs.instance = kInvalid;
+    hs.prim = index;
+    hs.u = hs.v = R(0);
+}
+
+// intersectNexusBvh2 over triangles, device_traversal.cuh:239-313.
+template <class R>
+void bvhTriangles(const oracle_handle& h, const Ray<R>& ray, const BvhView& bvh, const wpt_triangle* tris,
+    const uint32_t* triLights, HitState<R>& hs, uint32_t materialOverride, uint32_t kind, uint32_t instance) {
+    V3<R> inv(R(1) / ray.d.x, R(1) / ray.d.y, R(1) / ray.d.z);
+    uint32_t stack[64];
+    uint32_t size = 1;
+    stack[0] = bvh.count - 1;
+    while (size > 0) {
+        const wpt_bvh2_node& node = bvh.nodes[stack[--size]];
+        R nearT;
+        if (!intersectAabb(ray, inv, node, hs.closestT, nearT)) continue;
+        if (node.left == kInvalid) {
+            uint32_t p = node.right;
+            triangleHit(h, ray, tris[p], triLights ? triLights[p] : kInvalid, hs, materialOverride, kind, instance, p);
+        } else {
+            R ln, rn;
+            bool hl = intersectAabb(ray, inv, bvh.nodes[node.left], hs.closestT, ln);
+            bool hr = intersectAabb(ray, inv, bvh.nodes[node.right], hs.closestT, rn);
+            if (hl && hr) {
+                uint32_t nearChild = ln < rn ? node.left : node.right;
+                uint32_t farChild = ln < rn ? node.right : node.left;
+                stack[size++] = farChild;
+                stack[size++] = nearChild;
+            } else if (hl) {
+                stack[size++] = node.left;
+            } else if (hr) {
+                stack[size++] = node.right;
+            }
+        }
+    }
+}
+
+template <class R>
+V3<R> inverseXform(const wpt_transform& t, const V3<R>& v) {  // scene.cuh:43-46
+    return vec<R>(t.world_to_local[0]) * v.x + vec<R>(t.world_to_local[1]) * v.y + vec<R>(t.world_to_local[2]) * v.z;
+}
+
+template <class R>
+V3<R> xformNormal(const wpt_transform& t, const V3<R>& n) {  // scene.cuh:48-55
+    return normalize(V3<R>(dot(vec<R>(t.world_to_local[0]), n), dot(vec<R>(t.world_to_local[1]), n),
+        dot(vec<R>(t.world_to_local[2]), n)));
+}
+
+template <class R>
+Ray<R> localRay(const wpt_transform& t, const Ray<R>& ray) {
+    Ray<R> l;
+    l.o = inverseXform(t, ray.o - vec<R>(t.translation));
+    l.d = inverseXform(t, ray.d);
+    return l;
+}
+
+// intersectMeshInstance, device_traversal.cuh:802-834.
+template <class R>
+void meshInstance(const oracle_handle& h, const Ray<R>& ray, uint32_t index, HitState<R>& hs, bool useBvh) {
+    const wpt_instance& inst = h.d.instances[index];
+    const wpt_mesh& mesh = h.d.meshes[inst.mesh];
+    const wpt_triangle* tris = h.d.mesh_triangles + mesh.first_triangle;
+    const uint32_t* lights = h.meshLight.data() + mesh.first_triangle;
+    Ray<R> local = localRay(inst.transform, ray);
+    R previous = hs.closestT;
+    const BvhView& bvh = h.blas[inst.mesh];
+    if (useBvh && bvh.count) {
+        bvhTriangles(h, local, bvh, tris, lights, hs, inst.material_override, WPT_HIT_INSTANCE_TRIANGLE, index);
+    } else {
+        for (uint32_t i = 0; i < mesh.triangle_count; ++i)
+            triangleHit(h, local, tris[i], lights[i], hs, inst.material_override, WPT_HIT_INSTANCE_TRIANGLE, index, i);
+    }
+    if (hs.closestT >= previous) return;
+    hs.geometricNormal = xformNormal(inst.transform, hs.geometricNormal);
+    hs.normal = xformNormal(inst.transform, hs.normal);
+    if (hs.lightIndex != kInvalid) hs.lightIndex += h.instanceLightOffset[index];
+    hs.sphereIndex = kInvalid;
+}
+
+template <class R>
+void sceneTlas(const oracle_handle& h, const Ray<R>& ray, HitState<R>& hs) {  // :836-878
+    if (!h.tlas.count) return;
+    V3<R> inv(R(1) / ray.d.x, R(1) / ray.d.y, R(1) / ray.d.z);
+    uint32_t stack[64];
+    uint32_t size = 1;
+    stack[0] = h.tlas.count - 1;
+    while (size > 0) {
+        const wpt_bvh2_node& node = h.tlas.nodes[stack[--size]];
+        R nearT;
+        if (!intersectAabb(ray, inv, node, hs.closestT, nearT)) continue;
+        if (node.left == kInvalid) {
+            uint32_t leaf = h.leaves[node.right];
+            if (leaf & 0x80000000u) sphereHit(h, ray, leaf & 0x7fffffffu, hs);
+            else meshInstance(h, ray, leaf, hs, true);
+        } else {
+            R ln, rn;
+            bool hl = intersectAabb(ray, inv, h.tlas.nodes[node.left], hs.closestT, ln);
+            bool hr = intersectAabb(ray, inv, h.tlas.nodes[node.right], hs.closestT, rn);
+            if (hl && hr) {
+                uint32_t nearChild = ln < rn ? node.left : node.right;
+                uint32_t farChild = ln < rn ? node.right : node.left;
+                stack[size++] = farChild;
+                stack[size++] = nearChild;
+            } else if (hl) {
+                stack[size++] = node.left;
+            } else if (hr) {
+                stack[size++] = node.right;
+            }
+        }
+    }
+}
+
+template <class R>
+void directTriangles(const oracle_handle& h, const Ray<R>& ray, HitState<R>& hs) {  // :880-1033
+    if (h.flat.count) {
+        bvhTriangles(h, ray, h.flat, h.d.flat_triangles, nullptr, hs, kInvalid, WPT_HIT_FLAT_TRIANGLE, kInvalid);
+    } else {
+        for (uint32_t i = 0; i < h.d.flat_triangle_count; ++i)
+            triangleHit(h, ray, h.d.flat_triangles[i], kInvalid, hs, kInvalid, WPT_HIT_FLAT_TRIANGLE, kInvalid, i);
+    }
+}
+
+template <class R>
+struct Hit {
+    R t;
+    V3<R> position, geometricNormal, normal;
+    V2<R> uv;
+    uint32_t material, lightIndex;
+};
+
+// traceRay, device_traversal.cuh:1115-1184.
+template <class R>
+bool traceRay(const oracle_handle& h, const Ray<R>& ray, Hit<R>& hit, HitState<R>& hs) {
+    for (uint32_t i = 0; i < h.directSpheres.size(); ++i) sphereHit(h, ray, h.directSpheres[i], hs);
+    for (uint32_t i = 0; i < h.directInstances.size(); ++i) meshInstance(h, ray, h.directInstances[i], hs, false);
+    for (uint32_t i = 0; i < h.d.static_triangle_count; ++i)
+        if (triangleHit(h, ray, h.d.static_triangles[i], h.staticLight[i], hs, kInvalid, WPT_HIT_STATIC_TRIANGLE, kInvalid, i))
+            hs.sphereIndex = kInvalid;
+    sceneTlas(h, ray, hs);
+    if (h.d.flat_triangle_count > 0) {
+        R previous = hs.closestT;
+        directTriangles(h, ray, hs);
+        if (hs.closestT < previous) hs.sphereIndex = kInvalid;
+    }
+    if (hs.sphereIndex != kInvalid) {
+        const wpt_sphere& s = h.d.spheres[hs.sphereIndex];
+        hs.normal = normalize(ray.at(hs.closestT) - vec<R>(s.center));
+        hs.geometricNormal = hs.normal;
+        hs.uv = V2<R>{R(0), R(0)};
+        hs.lightIndex = kInvalid;
+    }
+    if (hs.closestMaterial < 0) return false;
+    hit.t = hs.closestT;
+    hit.position = ray.at(hs.closestT);
+    hit.geometricNormal = hs.geometricNormal;
+    hit.normal = hs.normal;
+    hit.uv = hs.uv;
+    hit.material = static_cast<uint32_t>(hs.closestMaterial);
+    hit.lightIndex = hs.lightIndex;
+    return true;
+}
+
+// isOccluded, device_traversal.cuh:1035-1113.
+template <class R>
+bool isOccluded(const oracle_handle& h, const Ray<R>& ray, R maxDistance) {
+    for (uint32_t i : h.directSpheres) {
+        R t;
+        if (intersectSphere(ray, h.d.spheres[i], t) && t < maxDistance) return true;
+    }
+    for (uint32_t i : h.directInstances) {
+        HitState<R> hs;
+        hs.closestT = maxDistance;
+        meshInstance(h, ray, i, hs, false);
+        if (hs.closestMaterial >= 0) return true;
+    }
+    for (uint32_t i = 0; i < h.d.static_triangle_count; ++i) {
+        R t, u, v;
+        const wpt_triangle& tri = h.d.static_triangles[i];
+        if (intersectTriangle(ray, tri, t, u, v) && t < maxDistance && !cutout(h, tri.material)) return true;
+    }
+    if (h.tlas.count) {
+        V3<R> inv(R(1) / ray.d.x, R(1) / ray.d.y, R(1) / ray.d.z);
+        uint32_t stack[64];
+        uint32_t size = 1;
+        stack[0] = h.tlas.count - 1;
+        while (size > 0) {
+            const wpt_bvh2_node& node = h.tlas.nodes[stack[--size]];
+            R nearT;
+            if (!intersectAabb(ray, inv, node, maxDistance, nearT)) continue;
+            if (node.left == kInvalid) {
+                uint32_t leaf = h.leaves[node.right];
+                if (leaf & 0x80000000u) {
+                    R t;
+                    if (intersectSphere(ray, h.d.spheres[leaf & 0x7fffffffu], t) && t < maxDistance) return true;
+                } else {
+                    HitState<R> hs;
+                    hs.closestT = maxDistance;
+                    meshInstance(h, ray, leaf, hs, true);
+                    if (hs.closestMaterial >= 0) return true;
+                }
+            } else {
+                R ln, rn;
+                bool hl = intersectAabb(ray, inv, h.tlas.nodes[node.left], maxDistance, ln);
+                bool hr = intersectAabb(ray, inv, h.tlas.nodes[node.right], maxDistance, rn);
+                if (hl && hr) {
+                    uint32_t nearChild = ln < rn ? node.left : node.right;
+                    uint32_t farChild = ln < rn ? node.right : node.left;
+                    stack[size++] = farChild;
+                    stack[size++] = nearChild;
+                } else if (hl) {
+                    stack[size++] = node.left;
+                } else if (hr) {
+                    stack[size++] = node.right;
+                }
+            }
+        }
+    }
+    if (h.d.flat_triangle_count > 0) {
+        HitState<R> hs;
+        hs.closestT = maxDistance;
+        directTriangles(h, ray, hs);
+        if (hs.closestMaterial >= 0) return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- shading
+template <class R>
+struct PathState {  // path_state.cuh:6-24
+    Ray<R> ray;
+    V3<R> throughput, radiance;
+    uint32_t pixelIndex, depth;
+    RngState rng;
+    uint32_t mediumMaterial;
+    R mediumDensity;
+    R previousBsdfPdf;
+    bool specularBounce, active;
+    R etaScale;
+    R clamp;
+};
+
+template <class R>
+void addPathRadiance(PathState<R>& p, V3<R> c, bool emissionOrBackground = false) {  // path_state.cuh:33-43
+    bool indirect = p.depth > (emissionOrBackground ? 1u : 0u);
+    if (indirect && p.clamp > R(0)) {
+        R magnitude = std::fabs(c.x) / R(3) + std::fabs(c.y) / R(3) + std::fabs(c.z) / R(3);
+        if (magnitude > p.clamp) c *= p.clamp / magnitude;
+    }
+    p.radiance += c;
+}
+
+template <class R> R powerHeuristic(R a, R b) {
+    R a2 = a * a, b2 = b * b;
+    return a2 / (a2 + b2);
+}
+
+template <class R> V3<R> reflect(const V3<R>& i, const V3<R>& n) { return i - R(2) * dot(i, n) * n; }  // device_sampling.cuh:37-40
+
+template <class R> V3<R> refract(const V3<R>& i, const V3<R>& n, R eta) {  // :42-48
+    R c = std::fmin(dot(-i, n), R(1));
+    V3<R> perp = eta * (i + c * n);
+    V3<R> par = -std::sqrt(std::fmax(R(0), R(1) - lengthSquared(perp))) * n;
+    return perp + par;
+}
+
+template <class R> R dielectricReflectance(R cosine, R eta) {  // :59-68
+    if (eta == R(1)) return R(0);
+    cosine = std::fmin(R(1), std::fmax(R(0), cosine));
+    R s2 = eta * eta * std::fmax(R(0), R(1) - cosine * cosine);
+    if (s2 >= R(1)) return R(1);
+    R ct = std::sqrt(R(1) - s2);
+    R perp = (eta * cosine - ct) / (eta * cosine + ct);
+    R par = (cosine - eta * ct) / (cosine + eta * ct);
+    return R(0.5f) * (perp * perp + par * par);
+}
+
+template <class R>
+struct Interface {
+    V3<R> normal, reflection, transmission;
+    R eta, reflectance;
+    bool frontFace;
+};
+
+template <class R>
+Interface<R> dielectricInterface(const V3<R>& incident, const V3<R>& gn, V3<R> sn, R ior) {  // :71-93
+    bool front = dot(incident, gn) < R(0);
+    V3<R> facing = front ? gn : -gn;
+    if (dot(sn, gn) < R(0)) sn = -sn;
+    V3<R> n = front ? sn : -sn;
+    R eta = front ? R(1) / ior : ior;
+    if (dot(incident, n) >= R(0)) n = facing;
+    R refl = dielectricReflectance(dot(-incident, n), eta);
+    V3<R> r = reflect(incident, n);
+    V3<R> t = refract(incident, n, eta);
+    if (dot(r, facing) <= R(0) || (refl < R(1) && dot(t, facing) >= R(0))) {
+        n = facing;
+        refl = dielectricReflectance(dot(-incident, n), eta);
+        r = reflect(incident, n);
+        t = refract(incident, n, eta);
+    }
+    return Interface<R>{n, r, t, eta, refl, front};
+}
+
+template <class R>
+V3<R> sampleCosineHemisphere(const V3<R>& n, RngState& rng) {  // :6-25
+    const R twoPi = R(6.28318530718f);
+    R u1 = rnd<R>(rng);
+    R u2 = rnd<R>(rng);
+    R radius = std::sqrt(u1);
+    R phi = twoPi * u2;
+    R x = radius * fcos(phi);
+    R y = radius * fsin(phi);
+    R z = std::sqrt(R(1) - u1);
+    V3<R> helper = std::fabs(n.x) > R(0.1f) ? V3<R>(R(0), R(1), R(0)) : V3<R>(R(1), R(0), R(0));
+    V3<R> tangent = normalize(cross(helper, n));
+    V3<R> bitangent = cross(n, tangent);
+    return normalize(tangent * x + bitangent * y + n * z);
+}
+
+template <class R>
+V3<R> randomInUnitSphere(RngState& rng) {  // :27-35, draws taken x, y, z
+    while (true) {
+        R x = R(2) * rnd<R>(rng) - R(1);
+        R y = R(2) * rnd<R>(rng) - R(1);
+        R z = R(2) * rnd<R>(rng) - R(1);
+        V3<R> p(x, y, z);
+        if (lengthSquared(p) < R(1)) return p;
+    }
+}
+
+// specular_glossiness.cuh:7-80
+template <class R>
+struct SG {
+    V3<R> diffuse, f0;
+    R roughness;
+};
+template <class R> R sgMax(const V3<R>& v) { return std::fmax(v.x, std::fmax(v.y, v.z)); }
+template <class R> R sgProbability(const SG<R>& s) {
+    R spec = sgMax(s.f0);
+    R diff = sgMax(s.diffuse) * (R(1) - spec);
+    return std::fmin(R(0.95f), std::fmax(R(0.05f), spec / std::fmax(R(1.0e-6f), spec + diff)));
+}
+template <class R> R sgAlpha(const SG<R>& s) {
+    R r = std::fmax(R(0.02f), s.roughness);
+    return r * r;
+}
+template <class R> R sgD(R nh, R alpha) {
+    const R pi = R(3.14159265359f);
+    R a2 = alpha * alpha;
+    R den = (R(1) - nh * nh) + a2 * nh * nh;
+    return a2 / (pi * den * den);
+}
+template <class R> R sgG1(R c, R alpha) {
+    return R(2) * c / (c + std::sqrt(alpha * alpha + (R(1) - alpha * alpha) * c * c));
+}
+template <class R>
+V3<R> evalSG(const SG<R>& s, const V3<R>& n, const V3<R>& wo, const V3<R>& wi, R& pdf) {
+    const R invPi = R(0.31830988618f);
+    pdf = R(0);
+    R nv = dot(n, wo), nl = dot(n, wi);
+    V3<R> sum = wo + wi;
+    if (nv <= R(0) || nl <= R(0) || lengthSquared(sum) < R(1.0e-12f)) return V3<R>();
+    V3<R> half = normalize(sum);
+    R nh = std::fmin(R(1), std::fmax(R(0), dot(n, half)));
+    R vh = std::fmin(R(1), std::fmax(R(1.0e-8f), dot(wo, half)));
+    R alpha = sgAlpha(s);
+    R d = sgD(nh, alpha);
+    R prob = sgProbability(s);
+    pdf = (R(1) - prob) * nl * invPi + prob * d * nh / (R(4) * vh);
+    R schlick = fpow(R(1) - vh, R(5));
+    V3<R> fresnel = s.f0 + (V3<R>(R(1), R(1), R(1)) - s.f0) * schlick;
+    V3<R> diffuse = s.diffuse * ((R(1) - sgMax(s.f0)) * invPi);
+    R g = sgG1(nv, alpha) * sgG1(nl, alpha);
+    return diffuse + fresnel * (d * g / (R(4) * nv * nl));
+}
+template <class R>
+V3<R> sgBasis(const V3<R>& n, R x, R y, R z) {
+    V3<R> helper = std::fabs(n.x) > R(0.1f) ? V3<R>(R(0), R(1), R(0)) : V3<R>(R(1), R(0), R(0));
+    V3<R> tangent = normalize(cross(helper, n));
+    return normalize(x * tangent + y * cross(n, tangent) + z * n);
+}
+template <class R>
+V3<R> sampleSG(const SG<R>& s, const V3<R>& n, const V3<R>& wo, R choice, R u, R v) {
+    R phi = R(6.28318530718f) * v;
+    if (choice < sgProbability(s)) {
+        R alpha = sgAlpha(s);
+        R c = std::sqrt((R(1) - u) / (R(1) + (alpha * alpha - R(1)) * u));
+        R sn = std::sqrt(std::fmax(R(0), R(1) - c * c));
+        V3<R> half = sgBasis(n, sn * fcos(phi), sn * fsin(phi), c);
+        return R(2) * dot(wo, half) * half - wo;
+    }
+    R radius = std::sqrt(u);
+    return sgBasis(n, radius * fcos(phi), radius * fsin(phi), std::sqrt(R(1) - u));
+}
+
+template <class R> V3<R> albedoOf(const wpt_material& m) { return vec<R>(m.albedo); }
+
+bool metalEffective(const wpt_material& m) {
+    return m.type == WPT_MATERIAL_METAL || (m.type == WPT_MATERIAL_DIFFUSE && m.metallic > 0.5f);
+}
+
+// evaluateSurface / evaluateSurfaceCos (untextured), device_traversal.cuh:427-441, 519-527.
+template <class R>
+V3<R> evaluateSurfaceCos(const wpt_material& m, const V3<R>& n, const V3<R>& wo, const V3<R>& wi, R& pdf) {
+    V3<R> value;
+    if (m.type == WPT_MATERIAL_SPECULAR_GLOSSINESS) {
+        value = evalSG(SG<R>{albedoOf<R>(m), vec<R>(m.specular), R(m.roughness)}, n, wo, wi, pdf);
+    } else if (metalEffective(m)) {
+        if (R(m.roughness) <= R(0)) {
+            pdf = R(0);
+            value = V3<R>();
+        } else {
+            value = evalSG(SG<R>{V3<R>(), albedoOf<R>(m), R(m.roughness)}, n, wo, wi, pdf);
+        }
+    } else {
+        pdf = std::fmax(R(0), dot(n, wi)) * R(0.31830988618f);
+        value = albedoOf<R>(m) * R(0.31830988618f);
+    }
+    return value * std::fmax(R(0), dot(n, wi));
+}
+
+template <class R>
+V3<R> offsetRayOrigin(const V3<R>& p, const V3<R>& gn, const V3<R>& dir, R hitDistance) {  // ray.cuh:10-21
+    R magnitude = std::fmax(std::fmax(R(1), std::fabs(hitDistance)),
+        std::fmax(std::fabs(p.x), std::fmax(std::fabs(p.y), std::fabs(p.z))));
+    R sign = dot(dir, gn) >= R(0) ? R(1) : R(-1);
+    V3<R> offset = gn * (sign * R(32) * R(1.1920928955078125e-7f) * magnitude);
+    V3<R> r = p + offset;
+    const R inf = std::numeric_limits<R>::infinity();
+    if (offset.x != R(0)) r.x = std::nextafter(r.x, offset.x > R(0) ? inf : -inf);
+    if (offset.y != R(0)) r.y = std::nextafter(r.y, offset.y > R(0) ? inf : -inf);
+    if (offset.z != R(0)) r.z = std::nextafter(r.z, offset.z > R(0) ? inf : -inf);
+    return r;
+}
+
+struct Counters {
+    uint64_t closest = 0, shadow = 0;
+};
+
+// sampleDirectLight, device_traversal.cuh:529-569.
+template <class R>
+void sampleDirectLight(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const Hit<R>& shaded,
+    const wpt_material& material, Counters& cnt) {
+    uint32_t lightCount = static_cast<uint32_t>(h.lights.size());
+    if (lightCount == 0) return;
+    uint32_t candidate = std::min(static_cast<uint32_t>(randomFloat(path.rng) * lightCount), lightCount - 1);
+    uint32_t li = randomFloat(path.rng) < h.alias[candidate].probability ? candidate : h.alias[candidate].alias;
+    const TriangleLight& L = h.lights[li];
+    V3<R> lv0(R(L.v0.x), R(L.v0.y), R(L.v0.z)), lv1(R(L.v1.x), R(L.v1.y), R(L.v1.z)), lv2(R(L.v2.x), R(L.v2.y), R(L.v2.z));
+    V3<R> ln(R(L.normal.x), R(L.normal.y), R(L.normal.z));
+    R rootU = std::sqrt(rnd<R>(path.rng));
+    R v = rnd<R>(path.rng);
+    V3<R> lp = (R(1) - rootU) * lv0 + rootU * (R(1) - v) * lv1 + rootU * v * lv2;
+    V3<R> toLight = lp - hit.position;
+    R d2 = lengthSquared(toLight);
+    R dist = std::sqrt(d2);
+    V3<R> dir = toLight / dist;
+    R gc = std::fmax(R(0), dot(hit.geometricNormal, dir));
+    R sc = std::fmax(R(0), dot(shaded.normal, dir));
+    R lc = std::fmax(R(0), dot(ln, -dir));
+    if (gc == R(0) || sc == R(0) || lc == R(0)) return;
+    V3<R> so = offsetRayOrigin(hit.position, hit.geometricNormal, dir, hit.t);
+    V3<R> st = offsetRayOrigin(lp, ln, -dir, dist);
+    V3<R> seg = st - so;
+    R sd = length(seg);
+    if (!(sd > R(1.0e-6f))) return;
+    Ray<R> shadow{so, seg / sd};
+    if (!h.d.ignore_direct_light_occlusion) {
+        ++cnt.shadow;
+        if (isOccluded(h, shadow, sd - R(1.0e-6f))) return;
+    }
+    const wpt_material& lm = mat(h, L.material);
+    R area = R(L.area), pdfSel = R(L.selectionPdf);
+    R geometryTerm = lc * area / (d2 * pdfSel);
+    R lightPdf = pdfSel * d2 / (lc * area);
+    R bsdfPdf;
+    V3<R> bsdf = evaluateSurfaceCos(material, shaded.normal, -path.ray.d, dir, bsdfPdf);
+    R w = powerHeuristic(lightPdf, bsdfPdf);
+    V3<R> direct = hadamard(bsdf, vec<R>(lm.emission)) * (geometryTerm * w);
+    addPathRadiance(path, hadamard(path.throughput, direct));
+}
+
+// One path, shadePaths (device_path_shading.cuh:6-173) applied bounce by bounce.
+template <class R>
+V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, uint32_t rrStart, bool debug,
+    Counters& cnt) {
+    const R invPi = R(0.31830988618f);
+    for (uint32_t bounce = 0; bounce < maxDepth; ++bounce) {
+        Hit<R> hit;
+        HitState<R> hs;
+        ++cnt.closest;
+        bool didHit = traceRay(h, path.ray, hit, hs);
+        if (debug) {
+            V3<R> color(R(0.05f), R(0.10f), R(0.25f));
+            if (didHit) color = R(0.5f) * (hit.normal + V3<R>(R(1), R(1), R(1)));
+            return color;
+        }
+        if (!didHit) {
+            if (!h.d.black_background) {
+                R t = R(0.5f) * (path.ray.d.y + R(1));
+                V3<R> sky = (R(1) - t) * V3<R>(R(1), R(1), R(1)) + t * V3<R>(R(0.5f), R(0.7f), R(1.0f));
+                addPathRadiance(path, hadamard(path.throughput, sky), true);
+            }
+            return path.radiance;
+        }
+        const wpt_material& m = mat(h, hit.material);
+        if (path.mediumMaterial != kInvalid) {
+            const wpt_material& medium = mat(h, path.mediumMaterial);
+            if (medium.attenuation_distance > 0.0f) {
+                R dist = hit.t * path.mediumDensity;
+                R ad = R(medium.attenuation_distance);
+                V3<R> sigma(-flog(std::fmax(R(medium.attenuation_color[0]), R(0.0001f))) / ad,
+                    -flog(std::fmax(R(medium.attenuation_color[1]), R(0.0001f))) / ad,
+                    -flog(std::fmax(R(medium.attenuation_color[2]), R(0.0001f))) / ad);
+                path.throughput = hadamard(path.throughput, V3<R>(fexp(-sigma.x * dist), fexp(-sigma.y * dist), fexp(-sigma.z * dist)));
+            }
+        }
+        if (m.type == WPT_MATERIAL_EMISSIVE) {
+            R w = R(1);
+            if (hit.lightIndex != kInvalid) {
+                const TriangleLight& L = h.lights[hit.lightIndex];
+                R lc = std::fmax(R(0), dot(V3<R>(R(L.normal.x), R(L.normal.y), R(L.normal.z)), -path.ray.d));
+                if (lc == R(0)) {
+                    w = R(0);
+                } else if (path.depth > 0 && !path.specularBounce) {
+                    R lightPdf = R(L.selectionPdf) * hit.t * hit.t / (lc * R(L.area));
+                    w = powerHeuristic(path.previousBsdfPdf, lightPdf);
+                }
+            }
+            addPathRadiance(path, hadamard(path.throughput, vec<R>(m.emission)) * w, true);
+            return path.radiance;
+        }
+        Hit<R> shaded = hit;
+        if (dot(shaded.normal, hit.geometricNormal) < R(0)) shaded.normal = -shaded.normal;
+        V3<R> albedo = albedoOf<R>(m);
+        uint32_t type = (m.type == WPT_MATERIAL_DIFFUSE && m.metallic > 0.5f) ? WPT_MATERIAL_METAL : m.type;
+        R roughness = R(m.roughness);
+        V3<R> direction;
+        auto glossy = [&]() {  // continueGlossySurface, device_traversal.cuh:706-725
+            SG<R> s = m.type == WPT_MATERIAL_SPECULAR_GLOSSINESS ? SG<R>{albedo, vec<R>(m.specular), roughness}
+                                                                 : SG<R>{V3<R>(), albedo, roughness};
+            V3<R> wo = -path.ray.d;
+            R choice = rnd<R>(path.rng);
+            R u = rnd<R>(path.rng);
+            R v = rnd<R>(path.rng);
+            V3<R> dir = sampleSG(s, shaded.normal, wo, choice, u, v);
+            R pdf;
+            V3<R> f = evalSG(s, shaded.normal, wo, dir, pdf);
+            if (!(pdf > R(0)) || dot(dir, hit.geometricNormal) <= R(0)) {
+                path.active = false;
+                return dir;
+            }
+            path.throughput = hadamard(path.throughput, f) * (std::fmax(R(0), dot(shaded.normal, dir)) / pdf);
+            path.previousBsdfPdf = pdf;
+            path.specularBounce = false;
+            return dir;
+        };
+        switch (type) {
+        case WPT_MATERIAL_DIFFUSE: {
+            sampleDirectLight(h, path, hit, shaded, m, cnt);
+            // continueDiffuseSurface, device_traversal.cuh:685-704 (no environment)
+            direction = sampleCosineHemisphere(shaded.normal, path.rng);
+            R cosine = dot(shaded.normal, direction);
+            R pdf = std::fmax(R(0), dot(shaded.normal, direction)) * invPi;
+            if (!(pdf > R(0)) || cosine <= R(0) || dot(direction, hit.geometricNormal) <= R(0)) {
+                path.active = false;
+            } else {
+                path.throughput = hadamard(path.throughput, albedo) * (cosine * invPi / pdf);
+                path.previousBsdfPdf = pdf;
+                path.specularBounce = false;
+            }
+            break;
+        }
+        case WPT_MATERIAL_SPECULAR_GLOSSINESS:
+            sampleDirectLight(h, path, hit, shaded, m, cnt);
+            direction = glossy();
+            break;
+        case WPT_MATERIAL_METAL:
+            if (roughness > R(0)) {
+                sampleDirectLight(h, path, hit, shaded, m, cnt);
+                direction = glossy();
+                break;
+            }
+            path.throughput = hadamard(path.throughput, albedo);
+            {
+                V3<R> reflected = reflect(normalize(path.ray.d), shaded.normal);
+                direction = normalize(reflected + roughness * randomInUnitSphere<R>(path.rng));
+                if (dot(direction, shaded.normal) <= R(0) || dot(direction, hit.geometricNormal) <= R(0))
+                    direction = reflect(normalize(path.ray.d), hit.geometricNormal);
+            }
+            path.specularBounce = true;
+            path.previousBsdfPdf = R(0);
+            break;
+        case WPT_MATERIAL_DIELECTRIC: {  // continueDielectricSurface, device_traversal.cuh:589-635 (no environment)
+            path.throughput = hadamard(path.throughput, albedo);
+            Interface<R> b = dielectricInterface(normalize(path.ray.d), hit.geometricNormal, shaded.normal, R(m.ior));
+            R eta2 = b.eta * b.eta;
+            bool reflected = b.reflectance >= R(1) || b.reflectance > rnd<R>(path.rng);
+            if (!reflected) {
+                path.throughput *= eta2;
+                path.etaScale = path.etaScale / eta2;
+                if (b.frontFace) {
+                    path.mediumMaterial = hit.material;
+                    path.mediumDensity = R(m.volume_density) * R(1);
+                } else {
+                    path.mediumMaterial = kInvalid;
+                    path.mediumDensity = R(1);
+                }
+            }
+            path.specularBounce = true;
+            path.previousBsdfPdf = R(0);
+            direction = reflected ? b.reflection : b.transmission;
+            break;
+        }
+        default:
+            break;
+        }
+        if (!path.active) return path.radiance;
+        path.ray.o = offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t);
+        path.ray.d = direction;
+        ++path.depth;
+        if (path.depth >= maxDepth) return path.radiance;
+        if (path.depth >= rrStart) {
+            R survival = std::fmin(R(0.95f), std::fmax(R(0.05f), path.etaScale * std::fmax(path.throughput.x,
+                std::fmax(path.throughput.y, path.throughput.z))));
+            if (rnd<R>(path.rng) > survival) return path.radiance;
+            path.throughput *= R(1) / survival;
+        }
+    }
+    return path.radiance;
+}
+
+// generatePrimaryRays + Camera::generateRay (renderer.cu:10-42, camera.cuh:17-27).
+template <class R>
+PathState<R> primary(const wpt_camera& cam, uint32_t x, uint32_t y, uint32_t width, uint32_t height, uint32_t sample,
+    uint32_t strategy, float clamp) {
+    PathState<R> p{};
+    p.pixelIndex = y * width + x;
+    p.rng = makeRngState(pixelRngKey(p.pixelIndex, sample, strategy), strategy);
+    R u = (R(x) + rnd<R>(p.rng)) / R(width);
+    R v = (R(y) + rnd<R>(p.rng)) / R(height);
+    R tanHalf = ftan(R(cam.vertical_fov) * R(0.5f));
+    R px = (R(2) * u - R(1)) * R(cam.aspect_ratio) * tanHalf;
+    R py = (R(1) - R(2) * v) * tanHalf;
+    p.ray.o = vec<R>(cam.position);
+    p.ray.d = normalize(vec<R>(cam.forward) + vec<R>(cam.right) * px + vec<R>(cam.up) * py);
+    p.throughput = V3<R>(R(1), R(1), R(1));
+    p.radiance = V3<R>();
+    p.depth = 0;
+    p.mediumMaterial = kInvalid;
+    p.mediumDensity = R(1);
+    p.previousBsdfPdf = R(0);
+    p.specularBounce = true;
+    p.active = true;
+    p.etaScale = R(1);
+    p.clamp = R(clamp);
+    return p;
+}
+
+template <class R>
+void renderRows(const oracle_handle& h, const wpt_camera& cam, const wpt_render_config& cfg, uint32_t x0, uint32_t y0,
+    uint32_t w, uint32_t th, uint32_t s0, uint32_t sc, uint32_t first, uint32_t step, float* fb, Counters& cnt) {
+    bool debug = cfg.intersection_debug || cfg.shading_normal_debug;
+    for (uint32_t row = first; row < th; row += step) {
+        uint32_t y = y0 + row;
+        for (uint32_t x = x0; x < x0 + w; ++x) {
+            float* px = fb + 3 * (static_cast<size_t>(y) * cfg.width + x);
+            for (uint32_t s = s0; s < s0 + sc; ++s) {
+                PathState<R> p = primary<R>(cam, x, y, cfg.width, cfg.height, s, cfg.rng_strategy, cfg.indirect_clamp);
+                V3<R> L = tracePath(h, p, cfg.max_depth, cfg.rr_start_depth, debug, cnt);
+                px[0] += static_cast<float>(L.x);
+                px[1] += static_cast<float>(L.y);
+                px[2] += static_cast<float>(L.z);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_create(const oracle_scene* scene, oracle_handle** out) {
+    oracle_handle* h = new oracle_handle();
+    h->d = scene->desc;
+    h->tlas = BvhView{scene->tlas_nodes, scene->tlas_node_count};
+    h->flat = BvhView{scene->flat_nodes, scene->flat_node_count};
+    for (uint32_t m = 0; m < scene->desc.mesh_count; ++m) {
+        uint32_t count = scene->blas_node_count ? scene->blas_node_count[m] : 0;
+        h->blas.push_back(BvhView{count ? scene->blas_nodes + scene->blas_node_offset[m] : nullptr, count});
+    }
+    buildTables(*h);
+    *out = h;
+    return 0;
+}
+
+void oracle_destroy(oracle_handle* h) { delete h; }
+
+uint32_t oracle_list(const oracle_handle* h, uint32_t which, uint32_t* out, uint32_t capacity) {
+    const std::vector<uint32_t>& v = which == 0 ? h->directSpheres : (which == 1 ? h->directInstances : h->leaves);
+    if (out) std::memcpy(out, v.data(), sizeof(uint32_t) * std::min<size_t>(capacity, v.size()));
+    return static_cast<uint32_t>(v.size());
+}
+
+uint32_t oracle_lights(const oracle_handle* h, float* lights16, float* alias2, uint32_t capacity) {
+    uint32_t n = static_cast<uint32_t>(h->lights.size());
+    for (uint32_t i = 0; i < std::min(n, capacity); ++i) {
+        const TriangleLight& L = h->lights[i];
+        if (lights16) {
+            float* o = lights16 + 16 * i;
+            const V3<float> v[4] = {L.v0, L.v1, L.v2, L.normal};
+            for (int k = 0; k < 4; ++k) {
+                o[3 * k] = v[k].x;
+                o[3 * k + 1] = v[k].y;
+                o[3 * k + 2] = v[k].z;
+            }
+            o[12] = L.area;
+            std::memcpy(&o[13], &L.material, 4);
+            o[14] = L.selectionPdf;
+            o[15] = 0.0f;
+        }
+        if (alias2) {
+            alias2[2 * i] = h->alias[i].probability;
+            std::memcpy(&alias2[2 * i + 1], &h->alias[i].alias, 4);
+        }
+    }
+    return n;
+}
+
+void oracle_trace(const oracle_handle* h, const float* rays, uint32_t n, wpt_hit* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        Ray<float> ray{V3<float>(rays[i], rays[n + i], rays[2 * n + i]), V3<float>(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i])};
+        Hit<float> hit;
+        HitState<float> hs;
+        bool did = traceRay(*h, ray, hit, hs);
+        wpt_hit r{};
+        r.t = hs.closestT;
+        r.u = hs.u;
+        r.v = hs.v;
+        r.kind = did ? hs.kind : WPT_HIT_NONE;
+        r.instance = did ? hs.instance : kInvalid;
+        r.primitive = did ? hs.prim : kInvalid;
+        r.material = did ? hit.material : kInvalid;
+        r.light_index = did ? hit.lightIndex : kInvalid;
+        if (!did) r.t = 1e30f;
+        out[i] = r;
+    }
+}
+
+void oracle_occluded(const oracle_handle* h, const float* rays, const float* tmax, uint32_t n, uint8_t* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        Ray<float> ray{V3<float>(rays[i], rays[n + i], rays[2 * n + i]), V3<float>(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i])};
+        out[i] = isOccluded(*h, ray, tmax[i]) ? 1 : 0;
+    }
+}
+
+void oracle_brute_fp64(const oracle_handle* h, const float* rays, uint32_t n, double* tOut, uint32_t* matOut, float* nOut) {
+    const wpt_scene_desc& d = h->d;
+    for (uint32_t i = 0; i < n; ++i) {
+        V3<double> o(rays[i], rays[n + i], rays[2 * n + i]);
+        V3<double> dir(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        double best = 1e30;
+        uint32_t bestMat = kInvalid;
+        V3<double> bestN;
+        for (uint32_t s = 0; s < d.sphere_count; ++s) {
+            const wpt_sphere& sp = d.spheres[s];
+            V3<double> oc = o - vec<double>(sp.center);
+            double a = dot(dir, dir), b = dot(oc, dir), c = dot(oc, oc) - double(sp.radius) * sp.radius;
+            double disc = b * b - a * c;
+            if (disc < 0) continue;
+            double root = std::sqrt(disc);
+            double t = (-b - root) / a;
+            if (t <= 0.001) t = (-b + root) / a;
+            if (t <= 0.001 || t >= best) continue;
+            best = t;
+            bestMat = sp.material;
+            bestN = normalize(o + dir * t - vec<double>(sp.center));
+        }
+        auto tri = [&](const wpt_triangle& tr, const wpt_transform* xf, uint32_t override) {
+            V3<double> lo = o, ld = dir;
+            if (xf) {
+                lo = inverseXform(*xf, o - vec<double>(xf->translation));
+                ld = inverseXform(*xf, dir);
+            }
+            V3<double> v0 = vec<double>(tr.v0), e1 = vec<double>(tr.v1) - v0, e2 = vec<double>(tr.v2) - v0;
+            V3<double> p = cross(ld, e2);
+            double det = dot(e1, p);
+            if (std::fabs(det) < 1e-14) return;
+            V3<double> off = lo - v0;
+            double u = dot(off, p) / det;
+            V3<double> q = cross(off, e1);
+            double v = dot(ld, q) / det;
+            double t = dot(e2, q) / det;
+            if (u < 0 || v < 0 || u + v > 1 || t <= 0.001 || t >= best) return;
+            uint32_t m = override == kInvalid ? tr.material : override;
+            if (cutout(*h, m)) return;
+            V3<double> nn = (tr.flags & WPT_TRIANGLE_HAS_NORMALS)
+                ? normalize(vec<double>(tr.n0) * (1 - u - v) + vec<double>(tr.n1) * u + vec<double>(tr.n2) * v)
+                : normalize(cross(e1, e2));
+            if (dot(nn, ld) > 0) nn = -nn;
+            best = t;
+            bestMat = m;
+            bestN = xf ? xformNormal(*xf, nn) : nn;
+        };
+        for (uint32_t t = 0; t < d.static_triangle_count; ++t) tri(d.static_triangles[t], nullptr, kInvalid);
+        for (uint32_t t = 0; t < d.flat_triangle_count; ++t) tri(d.flat_triangles[t], nullptr, kInvalid);
+        for (uint32_t k = 0; k < d.instance_count; ++k) {
+            const wpt_instance& inst = d.instances[k];
+            const wpt_mesh& m = d.meshes[inst.mesh];
+            for (uint32_t t = 0; t < m.triangle_count; ++t)
+                tri(d.mesh_triangles[m.first_triangle + t], &inst.transform, inst.material_override);
+        }
+        tOut[i] = best;
+        matOut[i] = bestMat;
+        nOut[3 * i] = static_cast<float>(bestN.x);
+        nOut[3 * i + 1] = static_cast<float>(bestN.y);
+        nOut[3 * i + 2] = static_cast<float>(bestN.z);
+    }
+}
+
+int oracle_render(const oracle_handle* h, const wpt_camera* camera, const wpt_render_config* config, uint32_t x,
+    uint32_t y, uint32_t w, uint32_t th, uint32_t s0, uint32_t sc, uint32_t precision, uint32_t threads, float* fb,
+    oracle_stats* stats) {
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    threads = std::min(threads, std::max(1u, th));
+    std::vector<Counters> counters(threads);
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < threads; ++t) {
+        pool.emplace_back([&, t] {
+            if (precision == 64)
+                renderRows<double>(*h, *camera, *config, x, y, w, th, s0, sc, t, threads, fb, counters[t]);
+            else
+                renderRows<float>(*h, *camera, *config, x, y, w, th, s0, sc, t, threads, fb, counters[t]);
+        });
+    }
+    for (std::thread& t : pool) t.join();
+    if (stats) {
+        *stats = oracle_stats{};
+        for (const Counters& c : counters) {
+            stats->closest_queries += c.closest;
+            stats->shadow_queries += c.shadow;
+        }
+        stats->paths = static_cast<uint64_t>(w) * th * sc;
+    }
+    return 0;
+}
+
+uint32_t oracle_compact(const uint32_t* items, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t capacity) {
+    uint32_t count = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!flags[i]) continue;
+        if (count < capacity) out[count] = items[i];
+        ++count;
+    }
+    return count;
+}
+
+uint32_t oracle_reference_rays(uint32_t seed, uint32_t nRandom, float* rays, uint32_t capacity) {
+    std::vector<Ray<float>> r = {
+        {V3<float>(0, 0, 0), V3<float>(0, 0, 1)}, {V3<float>(1, 0, 0), V3<float>(0, 0, 1)},
+        {V3<float>(0, 0, 3), V3<float>(1, 0, 0)}, {V3<float>(0, 0, 0), V3<float>(0, -1, 0)},
+        {V3<float>(0, 0, 0), V3<float>(0, 0, -1)}, {V3<float>(-3, 0, 0), V3<float>(0, 0, 1)},
+        {V3<float>(3, 0, 0), V3<float>(0, 0, 1)}, {V3<float>(0, 0, 8), V3<float>(0, 0, -1)}};
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<float> position(-4, 4);
+    std::uniform_real_distribution<float> direction(-1, 1);
+    for (uint32_t i = 0; i < nRandom; ++i) {
+        float ox = position(rng), oy = position(rng), oz = position(rng);
+        float dx = direction(rng), dy = direction(rng), dz = direction(rng);
+        r.push_back({V3<float>(ox, oy, oz), normalize(V3<float>(dx, dy, dz))});
+    }
+    uint32_t n = static_cast<uint32_t>(r.size());
+    if (rays && capacity >= n) {
+        for (uint32_t i = 0; i < n; ++i) {
+            rays[i] = r[i].o.x;
+            rays[n + i] = r[i].o.y;
+            rays[2 * n + i] = r[i].o.z;
+            rays[3 * n + i] = r[i].d.x;
+            rays[4 * n + i] = r[i].d.y;
+            rays[5 * n + i] = r[i].d.z;
+        }
+    }
+    return n;
+}
+
+uint64_t oracle_rng_stream(uint32_t pixel, uint32_t sample, uint32_t strategy, uint32_t count, float* out) {
+    RngState s = makeRngState(pixelRngKey(pixel, sample, strategy), strategy);
+    uint64_t seeded = s.state;
+    for (uint32_t i = 0; i < count; ++i) out[i] = randomFloat(s);
+    return seeded;
+}
+
+float oracle_dielectric_reflectance(float cosine, float eta) { return dielectricReflectance<float>(cosine, eta); }
+
+void oracle_eval_specgloss(const float f0[3], const float diffuse[3], float roughness, const float n[3], const float wo[3],
+    const float wi[3], float value[3], float* pdf) {
+    SG<float> s{vec<float>(diffuse), vec<float>(f0), roughness};
+    V3<float> r = evalSG(s, vec<float>(n), vec<float>(wo), vec<float>(wi), *pdf);
+    value[0] = r.x;
+    value[1] = r.y;
+    value[2] = r.z;
+}
+
+}  // extern "C"

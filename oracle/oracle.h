@@ -1,0 +1,90 @@
+/*
+ * oracle.h - CPU restatement of the reference render path, used only as the
+ * checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * It is never linked into the product library.
+ *
+ * It takes the same scene description as the product's C ABI (include/wpt.h)
+ * plus the product's BVHs exported in the reference's node convention, and
+ * re-derives everything else (AccelerationMode partition, light table and
+ * alias, hit attributes, shading) following the reference sources.
+ */
+#ifndef WPT_ORACLE_H
+#define WPT_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/wpt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_scene {
+    wpt_scene_desc desc;
+    /* BVHs in the reference convention (wpt_bvh2_node), from the product. */
+    const wpt_bvh2_node* tlas_nodes;
+    uint32_t tlas_node_count;
+    const wpt_bvh2_node* flat_nodes;
+    uint32_t flat_node_count;
+    const wpt_bvh2_node* blas_nodes;   /* all meshes concatenated */
+    const uint32_t* blas_node_offset;  /* per mesh */
+    const uint32_t* blas_node_count;   /* per mesh, 0 = no BLAS */
+} oracle_scene;
+
+typedef struct oracle_stats {
+    uint64_t closest_queries;
+    uint64_t shadow_queries;
+    uint64_t paths;
+} oracle_stats;
+
+typedef struct oracle_handle oracle_handle;
+
+/* Builds the oracle's own scene tables (partition, lights, alias). */
+int oracle_create(const oracle_scene* scene, oracle_handle** out);
+void oracle_destroy(oracle_handle* h);
+
+/* Partition lists: which 0 direct spheres, 1 direct instances, 2 TLAS leaves
+ * (type << 31 | index, type 1 = sphere). */
+uint32_t oracle_list(const oracle_handle* h, uint32_t which, uint32_t* out, uint32_t capacity);
+/* Light table as 16 floats per light and the alias table as 2 words. */
+uint32_t oracle_lights(const oracle_handle* h, float* lights16, float* alias2, uint32_t capacity);
+
+/* FP32 restatement of traceRay / isOccluded.  rays6: SoA (ox,oy,oz,dx,dy,dz). */
+void oracle_trace(const oracle_handle* h, const float* rays6, uint32_t n, wpt_hit* out);
+void oracle_occluded(const oracle_handle* h, const float* rays6, const float* tmax, uint32_t n, uint8_t* out);
+
+/* Independent FP64 brute force over every primitive, ignoring all placement
+ * policy and acceleration structures: nearest t > 1e-3 (the tolerance model
+ * of the reference's scene_acceleration test), material and shading normal. */
+void oracle_brute_fp64(const oracle_handle* h, const float* rays6, uint32_t n, double* t, uint32_t* material,
+                       float* normal3);
+
+/* Renders samples [sample_start, sample_start + sample_count) of the pixels
+ * of one tile, adding unnormalized radiance into framebuffer (width*height*3).
+ * precision 32 = FP32 restatement, 64 = FP64 variant.  threads 0 = all. */
+int oracle_render(const oracle_handle* h, const wpt_camera* camera, const wpt_render_config* config, uint32_t x,
+                  uint32_t y, uint32_t w, uint32_t tile_h, uint32_t sample_start, uint32_t sample_count,
+                  uint32_t precision, uint32_t threads, float* framebuffer, oracle_stats* stats);
+
+/* Sequential stable filter (the order-preserving compaction contract). */
+uint32_t oracle_compact(const uint32_t* items, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t capacity);
+
+/* The ray set of the reference's scene_acceleration test: 8 fixed rays then
+ * n_random rays from std::mt19937(seed) with origins U(-4,4) and normalized
+ * U(-1,1) directions (tests/scene_acceleration_tests.cu:184-201). */
+uint32_t oracle_reference_rays(uint32_t seed, uint32_t n_random, float* rays6, uint32_t capacity);
+
+/* Reference RNG known answers: the first `count` floats of the stream for
+ * (pixel, sample) under a strategy, and the seeded state. */
+uint64_t oracle_rng_stream(uint32_t pixel, uint32_t sample, uint32_t strategy, uint32_t count, float* out);
+
+/* Reference BSDF helpers for known-answer tests. */
+float oracle_dielectric_reflectance(float cosine, float eta);
+void oracle_eval_specgloss(const float f0[3], const float diffuse[3], float roughness, const float n[3],
+                           const float wo[3], const float wi[3], float value[3], float* pdf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,182 @@
+"""ctypes wrapper of liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker; the product never imports it.  The oracle receives the same scene
+description as the product and the product's BVHs exported in the reference
+node convention, and re-derives everything else itself.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "wavefront-path-tracer_amd", "python"))
+import wpt  # noqa: E402
+
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OracleScene(C.Structure):
+    _fields_ = [("desc", wpt.SceneDesc), ("tlas_nodes", C.c_void_p), ("tlas_node_count", C.c_uint32),
+                ("flat_nodes", C.c_void_p), ("flat_node_count", C.c_uint32), ("blas_nodes", C.c_void_p),
+                ("blas_node_offset", C.c_void_p), ("blas_node_count", C.c_void_p)]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("closest_queries", C.c_uint64), ("shadow_queries", C.c_uint64), ("paths", C.c_uint64)]
+
+
+_lib = None
+
+
+def library():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with `make -C {HERE}`")
+        lib = C.CDLL(LIB_PATH)
+        P, u32 = C.c_void_p, C.c_uint32
+        for name, args, res in [
+            ("oracle_create", [C.POINTER(OracleScene), C.POINTER(P)], C.c_int),
+            ("oracle_destroy", [P], None),
+            ("oracle_list", [P, u32, P, u32], u32),
+            ("oracle_lights", [P, P, P, u32], u32),
+            ("oracle_trace", [P, P, u32, P], None),
+            ("oracle_occluded", [P, P, P, u32, P], None),
+            ("oracle_brute_fp64", [P, P, u32, P, P, P], None),
+            ("oracle_render", [P, C.POINTER(wpt.Camera), C.POINTER(wpt.RenderConfig), u32, u32, u32, u32, u32, u32,
+                               u32, u32, P, C.POINTER(OracleStats)], C.c_int),
+            ("oracle_compact", [P, P, u32, P, u32], u32),
+            ("oracle_reference_rays", [u32, u32, P, u32], u32),
+            ("oracle_rng_stream", [u32, u32, u32, u32, P], C.c_uint64),
+            ("oracle_dielectric_reflectance", [C.c_float, C.c_float], C.c_float),
+            ("oracle_eval_specgloss", [P, P, C.c_float, P, P, P, P, P], None),
+        ]:
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def reference_rays(seed=12345, n_random=2048) -> np.ndarray:
+    n = library().oracle_reference_rays(seed, n_random, None, 0)
+    rays = np.zeros((6, n), dtype=np.float32)
+    library().oracle_reference_rays(seed, n_random, rays.ctypes.data, n)
+    return rays
+
+
+def rng_stream(pixel, sample, strategy, count):
+    out = np.zeros(count, dtype=np.float32)
+    seeded = library().oracle_rng_stream(pixel, sample, strategy, count, out.ctypes.data)
+    return int(seeded), out
+
+
+def compact(items: np.ndarray, flags: np.ndarray, capacity: int):
+    items = np.ascontiguousarray(items, dtype=np.uint32)
+    flags = np.ascontiguousarray(flags, dtype=np.uint8)
+    out = np.zeros(max(1, capacity), dtype=np.uint32)
+    count = library().oracle_compact(items.ctypes.data, flags.ctypes.data, items.size, out.ctypes.data, capacity)
+    return out[:min(count, capacity)], int(count)
+
+
+class Oracle:
+    """The oracle over the scene of a product Scene handle (host or device)."""
+
+    def __init__(self, scene: "wpt.Scene"):
+        lib = library()
+        self._scene = scene
+        d = scene._desc
+        tlas, ntlas = scene.export_bvh2(0)
+        flat, nflat = scene.export_bvh2(1)
+        blas = []
+        offsets, counts = [], []
+        total = 0
+        for m in range(d.mesh_count):
+            nodes, n = scene.export_bvh2(2 + m)
+            blas.append((nodes, n))
+            offsets.append(total)
+            counts.append(n)
+            total += n
+        all_blas = (wpt.Bvh2Node * max(1, total))()
+        for (nodes, n), off in zip(blas, offsets):
+            C.memmove(C.byref(all_blas, off * C.sizeof(wpt.Bvh2Node)), nodes, n * C.sizeof(wpt.Bvh2Node))
+        self._keep = [tlas, flat, all_blas]
+        self._offsets = np.array(offsets or [0], dtype=np.uint32)
+        self._counts = np.array(counts or [0], dtype=np.uint32)
+        os_ = OracleScene()
+        os_.desc = d
+        os_.tlas_nodes = C.cast(tlas, C.c_void_p)
+        os_.tlas_node_count = ntlas
+        os_.flat_nodes = C.cast(flat, C.c_void_p)
+        os_.flat_node_count = nflat
+        os_.blas_nodes = C.cast(all_blas, C.c_void_p)
+        os_.blas_node_offset = self._offsets.ctypes.data
+        os_.blas_node_count = self._counts.ctypes.data
+        self._os = os_
+        self._h = C.c_void_p()
+        lib.oracle_create(C.byref(os_), C.byref(self._h))
+
+    def list(self, which):
+        n = library().oracle_list(self._h, which, None, 0)
+        out = np.zeros(n, dtype=np.uint32)
+        library().oracle_list(self._h, which, out.ctypes.data, n)
+        return out
+
+    def lights(self):
+        n = library().oracle_lights(self._h, None, None, 0)
+        lights = np.zeros((n, 16), dtype=np.float32)
+        alias = np.zeros((n, 2), dtype=np.float32)
+        library().oracle_lights(self._h, lights.ctypes.data, alias.ctypes.data, n)
+        return lights, alias
+
+    def trace(self, rays6: np.ndarray) -> np.ndarray:
+        rays6 = np.ascontiguousarray(rays6, dtype=np.float32)
+        n = rays6.shape[1]
+        out = np.zeros(n, dtype=wpt.HIT_DTYPE)
+        library().oracle_trace(self._h, rays6.ctypes.data, n, out.ctypes.data)
+        return out
+
+    def occluded(self, rays6: np.ndarray, tmax: np.ndarray) -> np.ndarray:
+        rays6 = np.ascontiguousarray(rays6, dtype=np.float32)
+        tmax = np.ascontiguousarray(tmax, dtype=np.float32)
+        n = rays6.shape[1]
+        out = np.zeros(n, dtype=np.uint8)
+        library().oracle_occluded(self._h, rays6.ctypes.data, tmax.ctypes.data, n, out.ctypes.data)
+        return out
+
+    def brute_fp64(self, rays6: np.ndarray):
+        rays6 = np.ascontiguousarray(rays6, dtype=np.float32)
+        n = rays6.shape[1]
+        t = np.zeros(n, dtype=np.float64)
+        m = np.zeros(n, dtype=np.uint32)
+        nrm = np.zeros((n, 3), dtype=np.float32)
+        library().oracle_brute_fp64(self._h, rays6.ctypes.data, n, t.ctypes.data, m.ctypes.data, nrm.ctypes.data)
+        return t, m, nrm
+
+    def render(self, camera, config, tile=None, sample_start=0, sample_count=None, precision=32, threads=0,
+               framebuffer=None):
+        w, h = config.width, config.height
+        x, y, tw, th = tile if tile is not None else (0, 0, w, h)
+        if sample_count is None:
+            sample_count = config.samples_per_pixel
+        fb = framebuffer if framebuffer is not None else np.zeros((h, w, 3), dtype=np.float32)
+        stats = OracleStats()
+        library().oracle_render(self._h, C.byref(camera), C.byref(config), x, y, tw, th, sample_start, sample_count,
+                                precision, threads, fb.ctypes.data, C.byref(stats))
+        return fb, stats
+
+    def close(self):
+        if self._h:
+            library().oracle_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

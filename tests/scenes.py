@@ -1,0 +1,110 @@
+"""Scene cases of the reference's acceleration test
+(tests/scene_acceleration_tests.cu:253-354), built through the C ABI's scene
+description.  The reference's alpha-masked material uses a 1x1 texture with
+alpha 0; the untextured equivalent is opacity 0 below the 0.5 cutoff."""
+import copy
+
+import wpt
+
+
+def triangle_at(z, material):
+    t = wpt.make_triangle((-2, -1, z), (2, -1, z), (0, 2, z), material,
+                          uvs=((0, 0), (1, 0), (0.5, 1)), tangent=(1, 0, 0), bitangent=(0, 1, 0))
+    return t
+
+
+def base_scene():
+    s = wpt.SceneBuilder()
+    for _ in range(4):
+        s.material(wpt.make_material(wpt.DIFFUSE, (0.5, 0.5, 0.5)))
+    s.materials[3].alpha_masked = 1
+    s.materials[3].opacity = 0.0
+    return s
+
+
+def clone(s):
+    c = wpt.SceneBuilder()
+    c.spheres = list(copy.deepcopy(s.spheres))
+    c.static_triangles = list(copy.deepcopy(s.static_triangles))
+    c.flat_triangles = list(copy.deepcopy(s.flat_triangles))
+    c.mesh_triangles = list(copy.deepcopy(s.mesh_triangles))
+    c.meshes = list(copy.deepcopy(s.meshes))
+    c.instances = list(copy.deepcopy(s.instances))
+    c.materials = list(copy.deepcopy(s.materials))
+    c.black_background = s.black_background
+    return c
+
+
+def reference_cases():
+    cases = []
+    cases.append(("empty scene", base_scene()))
+    spheres = base_scene()
+    spheres.sphere((0, 0, 3), 1, 0)
+    spheres.sphere((-3, 0, 3), 0.5, 1, wpt.ACCEL_DIRECT)
+    spheres.sphere((0, -21, 0), 20, 2, wpt.ACCEL_DIRECT)
+    cases.append(("sphere partitions", clone(spheres)))
+    all_direct = clone(spheres)
+    for sp in all_direct.spheres:
+        sp.acceleration = wpt.ACCEL_DIRECT
+    cases.append(("all direct spheres", all_direct))
+    all_bvh = clone(spheres)
+    for sp in all_bvh.spheres:
+        sp.acceleration = wpt.ACCEL_BVH
+    cases.append(("all accelerated spheres", all_bvh))
+    spheres.sphere((0, 0, 3), 1, 1, wpt.ACCEL_DIRECT)
+    cases.append(("coincident spheres across partitions", clone(spheres)))
+
+    field = base_scene()
+    field.sphere((0, -21, 0), 20, 2, wpt.ACCEL_DIRECT)
+    for x in range(12):
+        for z in range(12):
+            for layer in range(2):
+                field.sphere((-3.3 + 0.6 * x, -0.3 + 0.6 * layer, 1.0 + 0.6 * z), 0.15, (x + z + layer) % 3)
+    cases.append(("large sphere hierarchy", field))
+
+    mixed = clone(spheres)
+    mesh = mixed.mesh([triangle_at(1, 3), triangle_at(5, 2)])
+    mixed.instance(mesh, wpt.scaled_transform((0, 0, 0), 1))
+    mixed.instance(mesh, wpt.scaled_transform((4, 0, 1), 0.5), 1, wpt.ACCEL_DIRECT)
+    mixed.static_triangles.append(triangle_at(0.5, 3))
+    cases.append(("mixed geometry and shared mesh", clone(mixed)))
+    mixed.flat_triangles.append(triangle_at(7, 2))
+    cases.append(("mesh TLAS with additional flat triangles", clone(mixed)))
+
+    direct_mesh = base_scene()
+    m = direct_mesh.mesh([triangle_at(1, 3), triangle_at(5, 2)])
+    direct_mesh.instance(m, wpt.scaled_transform((0, 0, 0), 1), 1, wpt.ACCEL_DIRECT)
+    cases.append(("direct mesh without BLAS or TLAS", clone(direct_mesh)))
+    direct_mesh.instances[0].acceleration = wpt.ACCEL_BVH
+    cases.append(("accelerated mesh without spheres", clone(direct_mesh)))
+
+    empty_mesh = base_scene()
+    m = empty_mesh.mesh([])
+    empty_mesh.instance(m, wpt.scaled_transform((0, 0, 0), 1))
+    cases.append(("empty accelerated mesh", clone(empty_mesh)))
+    empty_mesh.instances[0].acceleration = wpt.ACCEL_DIRECT
+    cases.append(("empty direct mesh", clone(empty_mesh)))
+
+    reset = base_scene()
+    reset.sphere((0, 0, 3), 1, 0)
+    m = reset.mesh([triangle_at(5, 2)])
+    reset.instance(m, wpt.scaled_transform((0, 0, 0), 1), wpt.INVALID, wpt.ACCEL_DIRECT)
+    cases.append(("sphere replaces direct mesh attributes", reset))
+
+    flat = clone(spheres)
+    flat.flat_triangles = [triangle_at(1, 3), triangle_at(5, 2)]
+    cases.append(("sphere TLAS with flat triangle BVH", flat))
+    return cases
+
+
+def scaled_instance_scene(scale):
+    """material_transport_tests.cu:13-47: a 2e-4 triangle under an anisotropic
+    scaled instance must be hit at t = 2 from both sides at every scale."""
+    s = wpt.SceneBuilder()
+    s.material(wpt.make_material(wpt.DIFFUSE, (0.5, 0.5, 0.5)))
+    small = wpt.make_triangle((-.0001, -.0001, 0), (.0001, -.0001, 0), (0, .0001, 0), 0,
+                              tangent=(1, 0, 0), bitangent=(0, 1, 0))
+    m = s.mesh([small])
+    s.instance(m, wpt.scaled_transform(axis_scales=(scale, scale * 0.5, scale * 2)), wpt.INVALID, wpt.ACCEL_DIRECT)
+    s.instance(m, wpt.scaled_transform(axis_scales=(scale, scale * 0.5, scale * 2)), wpt.INVALID, wpt.ACCEL_BVH)
+    return s

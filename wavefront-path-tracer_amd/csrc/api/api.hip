@@ -1,0 +1,636 @@
+// C ABI of libwpt.so: scene upload, renderer resources, hipGraph capture,
+// persistent launches, the task seam and traversal/compaction entry points.
+//
+// Reference seams: uploadScene (src/scene/scene.cu:336-420), RenderDriver
+// (src/renderer/render_driver.cu:105-482), RenderTaskRenderer::render
+// (src/renderer/render_task.cu:26-114), render session
+// (src/renderer/render_session.cu:16-48).  Errors are returned as status
+// codes with a thread-local message instead of exiting the process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../../include/wpt.h"
+#include "../kernels/wavefront.h"
+#include "../scene/scene_build.h"
+
+using namespace wpt;
+
+namespace {
+
+thread_local std::string gLastError;
+
+int fail(int code, const std::string& message) {
+    gLastError = message;
+    return code;
+}
+
+int hipFail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? WPT_ERROR_OUT_OF_MEMORY : WPT_ERROR_DEVICE,
+        std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define WPT_HIP(call)                                  \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hipFail(e_, #call); \
+    } while (0)
+
+struct DeviceArena {
+    std::vector<void*> blocks;
+    ~DeviceArena() {
+        for (void* p : blocks) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t upload(const std::vector<T>& v, const T** out) {
+        *out = nullptr;
+        if (v.empty()) return hipSuccess;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, sizeof(T) * v.size());
+        if (e != hipSuccess) return e;
+        blocks.push_back(p);
+        e = hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+        *out = static_cast<const T*>(p);
+        return e;
+    }
+    hipError_t alloc(size_t bytes, void** out) {
+        *out = nullptr;
+        if (bytes == 0) return hipSuccess;
+        hipError_t e = hipMalloc(out, bytes);
+        if (e == hipSuccess) blocks.push_back(*out);
+        return e;
+    }
+};
+
+Bvh deviceBvhView(const HostBvh& h, const BvhNode* nodes) {
+    Bvh b{};
+    b.nodes = nodes;
+    b.nodeCount = static_cast<uint32_t>(h.nodes.size());
+    b.primCount = h.primCount();
+    b.root = h.primCount() > 1 ? 0u : (kLeafBit | 0u);
+    for (int a = 0; a < 3; ++a) {
+        b.rootMin[a] = h.rootBounds.mn[a];
+        b.rootMax[a] = h.rootBounds.mx[a];
+    }
+    return b;
+}
+
+}  // namespace
+
+struct wpt_scene {
+    HostSceneData host;
+    bool onDevice = false;
+    DevScene dev{};
+    DeviceArena arena;
+};
+
+struct wpt_renderer {
+    wpt_scene* scene = nullptr;
+    wpt_render_config config{};
+    uint64_t fingerprint = 0;
+    uint32_t capacity = 0;
+    std::vector<RenderTile> tiles;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t start = nullptr, end = nullptr;
+    LaunchContext ctx{};
+    DeviceArena arena;
+    uint32_t currentTile = 0xffffffffu;
+    double lastMs = 0.0;
+    uint64_t lastSamples = 0;
+    ~wpt_renderer() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (start) (void)hipEventDestroy(start);
+        if (end) (void)hipEventDestroy(end);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+__global__ void kSetTile(RenderTile* dst, RenderTile tile) { *dst = tile; }
+__global__ void kSetSample(WavefrontCounters* c, uint32_t sample) { c->sampleIndex = sample; }
+__global__ void kClearStats(WavefrontCounters* c) {
+    c->closestQueries = 0;
+    c->shadowQueries = 0;
+    for (int i = 0; i < 64; ++i) c->queueCounts[i] = 0;
+}
+__global__ void kClearRect(float* fb, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h) {
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t n = static_cast<size_t>(w) * h * 3;
+    if (i >= n) return;
+    uint32_t px = static_cast<uint32_t>(i / 3);
+    uint32_t c = static_cast<uint32_t>(i % 3);
+    uint32_t y = y0 + px / w;
+    uint32_t x = x0 + px % w;
+    fb[(static_cast<size_t>(y) * width + x) * 3 + c] = 0.0f;
+}
+
+int validateConfig(const wpt_render_config& c) {
+    if (!std::isfinite(c.indirect_clamp) || c.indirect_clamp < 0.0f)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Indirect radiance clamp must be finite and nonnegative");
+    if (c.width == 0 || c.height == 0) return fail(WPT_ERROR_INVALID_ARGUMENT, "Render dimensions must be greater than zero");
+    if (c.samples_per_pixel == 0 || c.max_depth == 0)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Samples per pixel and maximum depth must be greater than zero");
+    if (c.block_size == 0 || c.block_size > 1024 || c.block_size % 64 != 0)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Block size must be a multiple of 64 between 64 and 1024");
+    if (c.tiled && (c.tile_width == 0 || c.tile_height == 0))
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Tile dimensions must be greater than zero");
+    if (c.rng_strategy > WPT_RNG_PCG32) return fail(WPT_ERROR_INVALID_ARGUMENT, "Unknown RNG strategy");
+    if (c.mode > WPT_MODE_PERSISTENT) return fail(WPT_ERROR_INVALID_ARGUMENT, "Unknown scheduler mode");
+    if (c.max_depth > 0x7fffffffu) return fail(WPT_ERROR_INVALID_ARGUMENT, "Maximum depth out of range");
+    uint64_t pixels = static_cast<uint64_t>(c.width) * c.height;
+    uint64_t cap = c.tiled ? static_cast<uint64_t>(c.tile_width) * c.tile_height : pixels;
+    if (pixels > std::numeric_limits<uint32_t>::max() || cap >= (1ull << 30))
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Render dimensions exceed the supported queue index range");
+    return WPT_OK;
+}
+
+std::vector<RenderTile> makeTiles(uint32_t width, uint32_t height, uint32_t tw, uint32_t th) {
+    std::vector<RenderTile> tiles;
+    for (uint32_t y = 0; y < height; y += th)
+        for (uint32_t x = 0; x < width; x += tw) tiles.push_back(RenderTile{x, y, std::min(tw, width - x), std::min(th, height - y), 0});
+    return tiles;
+}
+
+int ensureGraph(wpt_renderer* r) {
+    if (r->config.mode != WPT_MODE_GRAPH || r->exec) return WPT_OK;
+    WPT_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueueStagedSample(r->ctx, r->stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(r->stream, &graph);
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hipFail(e, "enqueueStagedSample");
+    }
+    if (e2 != hipSuccess) return hipFail(e2, "hipStreamEndCapture");
+    r->graph = graph;
+    WPT_HIP(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+    return WPT_OK;
+}
+
+int launchTileSample(wpt_renderer* r, uint32_t tileIndex, const RenderTile& tile) {
+    if (r->currentTile != tileIndex || r->tiles.size() != 1) {
+        hipLaunchKernelGGL(kSetTile, dim3(1), dim3(1), 0, r->stream, const_cast<RenderTile*>(r->ctx.tile), tile);
+        WPT_HIP(hipGetLastError());
+        r->currentTile = tileIndex;
+    }
+    if (r->config.mode == WPT_MODE_GRAPH) {
+        WPT_HIP(hipGraphLaunch(r->exec, r->stream));
+    } else {
+        WPT_HIP(enqueuePersistentSample(r->ctx, r->stream));
+    }
+    return WPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+WPT_API int wpt_abi_version(void) { return WPT_ABI_VERSION; }
+
+WPT_API const char* wpt_last_error(void) { return gLastError.c_str(); }
+
+WPT_API int wpt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static int createScene(const wpt_scene_desc* desc, wpt_scene** out, bool upload) {
+    if (!desc || !out) return fail(WPT_ERROR_INVALID_ARGUMENT, "null scene description or output");
+    *out = nullptr;
+    wpt_scene* s = new (std::nothrow) wpt_scene();
+    if (!s) return fail(WPT_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    std::string err = buildHostScene(*desc, s->host);
+    if (!err.empty()) {
+        delete s;
+        return fail(WPT_ERROR_INVALID_ARGUMENT, err);
+    }
+    if (s->host.maxStackDepth > PrivateStack::kCapacity) {
+        delete s;
+        return fail(WPT_ERROR_UNSUPPORTED, "acceleration structure deeper than the traversal stack");
+    }
+    if (upload) {
+        int status = WPT_OK;
+        auto check = [&](hipError_t e, const char* what) {
+            if (e != hipSuccess && status == WPT_OK) status = hipFail(e, what);
+        };
+        HostSceneData& h = s->host;
+        DevScene& d = s->dev;
+        DeviceArena& a = s->arena;
+        const float4* f4 = nullptr;
+        check(a.upload(h.spheres, &d.spheres), "spheres");
+        check(a.upload(h.sphereMaterial, &d.sphereMaterial), "sphere materials");
+        d.sphereCount = static_cast<uint32_t>(h.spheres.size());
+        check(a.upload(h.directSpheres, &d.directSpheres), "direct spheres");
+        d.directSphereCount = static_cast<uint32_t>(h.directSpheres.size());
+        check(a.upload(h.directInstances, &d.directInstances), "direct instances");
+        d.directInstanceCount = static_cast<uint32_t>(h.directInstances.size());
+        check(a.upload(h.staticTriangles, &d.staticTriangles), "static triangles");
+        d.staticCount = static_cast<uint32_t>(h.staticTriangles.size());
+        const BvhNode* nodes = nullptr;
+        check(a.upload(h.tlas.nodes, &nodes), "tlas nodes");
+        d.tlas = deviceBvhView(h.tlas, nodes);
+        check(a.upload(h.tlasLeavesSlot, &d.tlasLeaves), "tlas leaves");
+        check(a.upload(h.instances, &d.instances), "instances");
+        d.instanceCount = static_cast<uint32_t>(h.instances.size());
+        std::vector<Blas> blases(h.meshes.size());
+        for (size_t m = 0; m < h.meshes.size(); ++m) {
+            const HostMesh& mesh = h.meshes[m];
+            Blas& b = blases[m];
+            const BvhNode* bn = nullptr;
+            check(a.upload(mesh.bvh.nodes, &bn), "blas nodes");
+            b.bvh = deviceBvhView(mesh.bvh, bn);
+            check(a.upload(mesh.geometry, &f4), "blas geometry");
+            b.geometry = f4;
+            check(a.upload(mesh.triangles, &b.triangles), "mesh triangles");
+            b.triangleCount = static_cast<uint32_t>(mesh.triangles.size());
+        }
+        check(a.upload(blases, &d.blases), "blases");
+        d.blasCount = static_cast<uint32_t>(blases.size());
+        check(a.upload(h.flat.nodes, &nodes), "flat nodes");
+        d.flat = deviceBvhView(h.flat, nodes);
+        check(a.upload(h.flatGeometry, &f4), "flat geometry");
+        d.flatGeometry = f4;
+        check(a.upload(h.flatTriangles, &d.flatTriangles), "flat triangles");
+        d.flatCount = static_cast<uint32_t>(h.flatTriangles.size());
+        check(a.upload(h.materials, &d.materials), "materials");
+        d.materialCount = static_cast<uint32_t>(h.materials.size());
+        check(a.upload(h.materialCutout, &d.materialCutout), "material cutout");
+        check(a.upload(h.lights, &d.lights), "lights");
+        check(a.upload(h.lightAlias, &d.lightAlias), "light alias");
+        d.lightCount = static_cast<uint32_t>(h.lights.size());
+        d.blackBackground = h.blackBackground;
+        d.ignoreDirectLightOcclusion = h.ignoreDirectLightOcclusion;
+        if (status != WPT_OK) {
+            delete s;
+            return status;
+        }
+        s->onDevice = true;
+    }
+    *out = s;
+    return WPT_OK;
+}
+
+WPT_API int wpt_scene_create(const wpt_scene_desc* desc, wpt_scene** out) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(WPT_ERROR_DEVICE, "no gfx device visible: the wavefront renderer needs a GPU");
+    return createScene(desc, out, true);
+}
+
+WPT_API int wpt_scene_create_host(const wpt_scene_desc* desc, wpt_scene** out) { return createScene(desc, out, false); }
+
+WPT_API int wpt_scene_destroy(wpt_scene* scene) {
+    delete scene;
+    return WPT_OK;
+}
+
+WPT_API int wpt_scene_info_get(const wpt_scene* s, wpt_scene_info* info) {
+    if (!s || !info) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    const HostSceneData& h = s->host;
+    info->sphere_count = static_cast<uint32_t>(h.spheres.size());
+    info->direct_sphere_count = static_cast<uint32_t>(h.directSpheres.size());
+    info->instance_count = static_cast<uint32_t>(h.instances.size());
+    info->direct_instance_count = static_cast<uint32_t>(h.directInstances.size());
+    info->tlas_leaf_count = static_cast<uint32_t>(h.tlasLeavesOriginal.size());
+    info->tlas_node_count = static_cast<uint32_t>(exportBvh2(h.tlas).size());
+    uint32_t blases = 0;
+    for (const HostMesh& m : h.meshes) blases += m.bvh.primCount() != 0;
+    info->blas_count = blases;
+    info->light_count = static_cast<uint32_t>(h.lights.size());
+    info->static_triangle_count = static_cast<uint32_t>(h.staticTriangles.size());
+    info->flat_triangle_count = static_cast<uint32_t>(h.flatTriangles.size());
+    info->flat_node_count = h.flat.primCount() ? 2 * h.flat.primCount() - 1 : 0;
+    info->material_count = static_cast<uint32_t>(h.materials.size());
+    return WPT_OK;
+}
+
+WPT_API int wpt_scene_get_list(const wpt_scene* s, uint32_t which, uint32_t* out, uint32_t capacity, uint32_t* count) {
+    if (!s || !count) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    const std::vector<uint32_t>* list = nullptr;
+    if (which == 0) list = &s->host.directSpheres;
+    else if (which == 1) list = &s->host.directInstances;
+    else if (which == 2) list = &s->host.tlasLeavesOriginal;
+    else return fail(WPT_ERROR_INVALID_ARGUMENT, "unknown list");
+    *count = static_cast<uint32_t>(list->size());
+    if (out) std::memcpy(out, list->data(), sizeof(uint32_t) * std::min<size_t>(capacity, list->size()));
+    return WPT_OK;
+}
+
+WPT_API int wpt_scene_export_bvh2(const wpt_scene* s, uint32_t bvh, wpt_bvh2_node* out, uint32_t capacity,
+    uint32_t* nodeCount) {
+    if (!s || !nodeCount) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    const HostBvh* h = nullptr;
+    if (bvh == 0) h = &s->host.tlas;
+    else if (bvh == 1) h = &s->host.flat;
+    else if (bvh - 2 < s->host.meshes.size()) h = &s->host.meshes[bvh - 2].bvh;
+    else return fail(WPT_ERROR_INVALID_ARGUMENT, "unknown BVH");
+    std::vector<wpt_bvh2_node> nodes = exportBvh2(*h);
+    *nodeCount = static_cast<uint32_t>(nodes.size());
+    if (out) std::memcpy(out, nodes.data(), sizeof(wpt_bvh2_node) * std::min<size_t>(capacity, nodes.size()));
+    return WPT_OK;
+}
+
+WPT_API int wpt_scene_export_lights(const wpt_scene* s, float* lights16, float* alias2, uint32_t capacity,
+    uint32_t* count) {
+    if (!s || !count) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    const HostSceneData& h = s->host;
+    *count = static_cast<uint32_t>(h.lights.size());
+    uint32_t n = std::min<uint32_t>(capacity, *count);
+    for (uint32_t i = 0; i < n; ++i) {
+        const TriangleLight& l = h.lights[i];
+        if (lights16) {
+            float* o = lights16 + 16 * i;
+            const Vec3 v[4] = {l.v0, l.v1, l.v2, l.normal};
+            for (int k = 0; k < 4; ++k) {
+                o[3 * k] = v[k].x;
+                o[3 * k + 1] = v[k].y;
+                o[3 * k + 2] = v[k].z;
+            }
+            o[12] = l.area;
+            std::memcpy(&o[13], &l.material, 4);
+            o[14] = l.selectionPdf;
+            o[15] = 0.0f;
+        }
+        if (alias2) {
+            alias2[2 * i] = h.lightAlias[i].probability;
+            std::memcpy(&alias2[2 * i + 1], &h.lightAlias[i].alias, 4);
+        }
+    }
+    return WPT_OK;
+}
+
+WPT_API int wpt_scene_export_light_indices(const wpt_scene* s, uint32_t* instanceOffset, uint32_t* staticIndex,
+    uint32_t* meshIndex) {
+    if (!s) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    const HostSceneData& h = s->host;
+    if (instanceOffset)
+        for (size_t i = 0; i < h.instances.size(); ++i) instanceOffset[i] = h.instances[i].lightOffset;
+    if (staticIndex)
+        for (size_t i = 0; i < h.staticTriangles.size(); ++i) staticIndex[i] = h.staticTriangles[i].lightIndex;
+    if (meshIndex) std::memcpy(meshIndex, h.meshLightIndex.data(), sizeof(uint32_t) * h.meshLightIndex.size());
+    return WPT_OK;
+}
+
+WPT_API int wpt_trace_closest(const wpt_scene* s, const float* rays, uint32_t n, wpt_hit* hits, void* stream) {
+    if (!s || !s->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not resident on a device");
+    if (n && (!rays || !hits)) return fail(WPT_ERROR_INVALID_ARGUMENT, "null ray or hit buffer");
+    WPT_HIP(launchTraceClosest(s->dev, rays, n, hits, static_cast<hipStream_t>(stream)));
+    return WPT_OK;
+}
+
+WPT_API int wpt_trace_occluded(const wpt_scene* s, const float* rays, const float* tmax, uint32_t n, uint8_t* out,
+    void* stream) {
+    if (!s || !s->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not resident on a device");
+    if (n && (!rays || !tmax || !out)) return fail(WPT_ERROR_INVALID_ARGUMENT, "null ray, distance or output buffer");
+    WPT_HIP(launchTraceOccluded(s->dev, rays, tmax, n, out, static_cast<hipStream_t>(stream)));
+    return WPT_OK;
+}
+
+WPT_API int wpt_compact(const uint32_t* items, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t capacity,
+    uint32_t* count, void* stream) {
+    if (!count || (n && (!items || !flags)) || (capacity && !out))
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "null compaction buffer");
+    if (n >= (1u << 30)) return fail(WPT_ERROR_INVALID_ARGUMENT, "compaction input too large");
+    uint32_t words = compactScratchWords(n);
+    void* scratch = nullptr;
+    WPT_HIP(hipMalloc(&scratch, sizeof(uint32_t) * words));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = launchCompact(items, flags, n, out, capacity, count, static_cast<uint32_t*>(scratch), words, st);
+    hipError_t e2 = hipStreamSynchronize(st);
+    (void)hipFree(scratch);
+    if (e != hipSuccess) return hipFail(e, "launchCompact");
+    if (e2 != hipSuccess) return hipFail(e2, "hipStreamSynchronize");
+    return WPT_OK;
+}
+
+WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, const wpt_render_config* config,
+    uint64_t fingerprint, wpt_renderer** out) {
+    if (!scene || !camera || !config || !out) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (!scene->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not resident on a device");
+    int v = validateConfig(*config);
+    if (v != WPT_OK) return v;
+    wpt_renderer* r = new (std::nothrow) wpt_renderer();
+    if (!r) return fail(WPT_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    r->scene = scene;
+    r->config = *config;
+    r->fingerprint = fingerprint;
+    const wpt_render_config& c = r->config;
+    uint32_t cap = c.tiled ? c.tile_width * c.tile_height : c.width * c.height;
+    r->capacity = cap;
+    r->tiles = c.tiled ? makeTiles(c.width, c.height, c.tile_width, c.tile_height)
+                       : makeTiles(c.width, c.height, c.width, c.height);
+    auto cleanup = [&](int status) {
+        delete r;
+        return status;
+    };
+    hipError_t e;
+#define R_HIP(call)                                            \
+    do {                                                       \
+        e = (call);                                            \
+        if (e != hipSuccess) return cleanup(hipFail(e, #call)); \
+    } while (0)
+    R_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    R_HIP(hipEventCreate(&r->start));
+    R_HIP(hipEventCreate(&r->end));
+    LaunchContext& ctx = r->ctx;
+    ctx.scene = scene->dev;
+    ctx.camera.position = Vec3(camera->position[0], camera->position[1], camera->position[2]);
+    ctx.camera.forward = Vec3(camera->forward[0], camera->forward[1], camera->forward[2]);
+    ctx.camera.right = Vec3(camera->right[0], camera->right[1], camera->right[2]);
+    ctx.camera.up = Vec3(camera->up[0], camera->up[1], camera->up[2]);
+    ctx.camera.verticalFov = camera->vertical_fov;
+    ctx.camera.aspectRatio = camera->aspect_ratio;
+    ctx.shade.maxDepth = c.max_depth;
+    ctx.shade.rrStart = c.rr_start_depth;
+    ctx.shade.rngStrategy = c.rng_strategy;
+    ctx.shade.clamp = c.indirect_clamp;
+    ctx.shade.intersectionDebug = c.intersection_debug;
+    ctx.shade.shadingNormalDebug = c.shading_normal_debug;
+    ctx.width = c.width;
+    ctx.height = c.height;
+    ctx.blockSize = c.block_size;
+    ctx.gridBlocks = (cap + c.block_size - 1) / c.block_size;
+    ctx.profileQueues = 1;
+    void* p = nullptr;
+    for (int q = 0; q < 2; ++q) {
+        R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kQueueFieldCount) * cap, &p));
+        ctx.queue[q].words = static_cast<uint32_t*>(p);
+        ctx.queue[q].capacity = cap;
+    }
+    if (c.mode == WPT_MODE_GRAPH) {
+        R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(cap) + sizeof(uint32_t) * 4 * static_cast<size_t>(cap), &p));
+        float* f = static_cast<float*>(p);
+        ctx.hits.t = f;
+        ctx.hits.u = f + cap;
+        ctx.hits.v = f + 2 * static_cast<size_t>(cap);
+        uint32_t* u = reinterpret_cast<uint32_t*>(f + 3 * static_cast<size_t>(cap));
+        ctx.hits.kind = u;
+        ctx.hits.instance = u + cap;
+        ctx.hits.prim = u + 2 * static_cast<size_t>(cap);
+        ctx.hits.material = u + 3 * static_cast<size_t>(cap);
+    }
+    R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height, &p));
+    ctx.framebuffer = static_cast<float*>(p);
+    R_HIP(r->arena.alloc(sizeof(uint32_t) * (ctx.gridBlocks + 1), &p));
+    ctx.chunkFlags = static_cast<uint32_t*>(p);
+    R_HIP(r->arena.alloc(sizeof(WavefrontCounters), &p));
+    ctx.counters = static_cast<WavefrontCounters*>(p);
+    R_HIP(hipMemset(p, 0, sizeof(WavefrontCounters)));
+    R_HIP(r->arena.alloc(sizeof(RenderTile), &p));
+    ctx.tile = static_cast<const RenderTile*>(p);
+    R_HIP(hipMemset(ctx.framebuffer, 0, sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height));
+    if (c.mode == WPT_MODE_PERSISTENT) {
+        int perCu = 0;
+        R_HIP(persistentOccupancy(c.block_size, &perCu));
+        if (perCu <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the persistent kernel"));
+        int device = 0, cus = 0, coop = 0;
+        R_HIP(hipGetDevice(&device));
+        R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        R_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
+        if (!coop) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "device lacks cooperative launch"));
+        uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(cus);
+        uint32_t blocks = c.persistent_blocks ? std::min(c.persistent_blocks, resident) : resident;
+        ctx.persistentBlocks = std::max(1u, std::min(blocks, ctx.gridBlocks));
+    }
+    R_HIP(hipDeviceSynchronize());
+#undef R_HIP
+    *out = r;
+    return WPT_OK;
+}
+
+WPT_API int wpt_renderer_destroy(wpt_renderer* r) {
+    delete r;
+    return WPT_OK;
+}
+
+WPT_API int wpt_renderer_reset(wpt_renderer* r) {
+    if (!r) return fail(WPT_ERROR_INVALID_ARGUMENT, "null renderer");
+    const wpt_render_config& c = r->config;
+    size_t n = 3 * static_cast<size_t>(c.width) * c.height;
+    hipLaunchKernelGGL(kClearRect, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, r->stream,
+        r->ctx.framebuffer, c.width, 0u, 0u, c.width, c.height);
+    hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, 0u);
+    WPT_HIP(hipGetLastError());
+    WPT_HIP(hipStreamSynchronize(r->stream));
+    return WPT_OK;
+}
+
+WPT_API int wpt_renderer_render_samples(wpt_renderer* r, uint32_t first, uint32_t count) {
+    if (!r) return fail(WPT_ERROR_INVALID_ARGUMENT, "null renderer");
+    int status = ensureGraph(r);
+    if (status != WPT_OK) return status;
+    hipLaunchKernelGGL(kClearStats, dim3(1), dim3(1), 0, r->stream, r->ctx.counters);
+    hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, first);
+    WPT_HIP(hipGetLastError());
+    WPT_HIP(hipEventRecord(r->start, r->stream));
+    for (uint32_t s = 0; s < count; ++s) {
+        for (uint32_t t = 0; t < r->tiles.size(); ++t) {
+            RenderTile tile = r->tiles[t];
+            tile.advanceSample = t + 1 == r->tiles.size() ? 1u : 0u;
+            status = launchTileSample(r, t, tile);
+            if (status != WPT_OK) return status;
+        }
+    }
+    WPT_HIP(hipEventRecord(r->end, r->stream));
+    WPT_HIP(hipEventSynchronize(r->end));
+    float ms = 0.0f;
+    WPT_HIP(hipEventElapsedTime(&ms, r->start, r->end));
+    r->lastMs = ms;
+    r->lastSamples = static_cast<uint64_t>(count) * r->config.width * r->config.height;
+    return WPT_OK;
+}
+
+WPT_API int wpt_renderer_read_framebuffer(wpt_renderer* r, float* host, uint32_t* sampleCount) {
+    if (!r || !host) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    WPT_HIP(hipStreamSynchronize(r->stream));
+    size_t n = 3 * static_cast<size_t>(r->config.width) * r->config.height;
+    WPT_HIP(hipMemcpy(host, r->ctx.framebuffer, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (sampleCount) {
+        WavefrontCounters c;
+        WPT_HIP(hipMemcpy(&c, r->ctx.counters, sizeof c, hipMemcpyDeviceToHost));
+        *sampleCount = c.sampleIndex;
+    }
+    return WPT_OK;
+}
+
+WPT_API int wpt_render_frame(wpt_renderer* r, float* host, uint32_t* sampleCount) {
+    if (!r || !host) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    int s = wpt_renderer_reset(r);
+    if (s != WPT_OK) return s;
+    s = wpt_renderer_render_samples(r, 0, r->config.samples_per_pixel);
+    if (s != WPT_OK) return s;
+    return wpt_renderer_read_framebuffer(r, host, sampleCount);
+}
+
+WPT_API int wpt_render_task(wpt_renderer* r, const wpt_task* task, float* hostTile) {
+    if (!r || !task || !hostTile) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    const wpt_render_config& c = r->config;
+    if (task->job_fingerprint != r->fingerprint)
+        return fail(WPT_ERROR_INVALID_TASK, "Render task fingerprint does not match the prepared renderer");
+    if (task->width == 0 || task->height == 0 || task->x >= c.width || task->y >= c.height ||
+        task->width > c.width - task->x || task->height > c.height - task->y ||
+        static_cast<uint64_t>(task->width) * task->height > r->capacity)
+        return fail(WPT_ERROR_INVALID_TASK, "Render task tile is outside the prepared renderer bounds");
+    if (task->sample_count == 0 || task->sample_start > c.samples_per_pixel ||
+        task->sample_count > c.samples_per_pixel - task->sample_start)
+        return fail(WPT_ERROR_INVALID_TASK, "Render task sample range is outside the prepared renderer bounds");
+    int status = ensureGraph(r);
+    if (status != WPT_OK) return status;
+    size_t n = static_cast<size_t>(task->width) * task->height * 3;
+    hipLaunchKernelGGL(kClearRect, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, r->stream,
+        r->ctx.framebuffer, c.width, task->x, task->y, task->width, task->height);
+    hipLaunchKernelGGL(kClearStats, dim3(1), dim3(1), 0, r->stream, r->ctx.counters);
+    hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, task->sample_start);
+    WPT_HIP(hipGetLastError());
+    RenderTile tile{task->x, task->y, task->width, task->height, 1u};
+    r->currentTile = 0xfffffffeu;  // force the tile upload
+    WPT_HIP(hipEventRecord(r->start, r->stream));
+    for (uint32_t s = 0; s < task->sample_count; ++s) {
+        status = launchTileSample(r, 0xfffffffeu, tile);
+        if (status != WPT_OK) return status;
+    }
+    WPT_HIP(hipEventRecord(r->end, r->stream));
+    WPT_HIP(hipStreamSynchronize(r->stream));
+    r->currentTile = 0xffffffffu;
+    float ms = 0.0f;
+    WPT_HIP(hipEventElapsedTime(&ms, r->start, r->end));
+    r->lastMs = ms;
+    r->lastSamples = static_cast<uint64_t>(task->sample_count) * task->width * task->height;
+    size_t pitch = sizeof(float) * 3 * c.width;
+    const float* src = r->ctx.framebuffer + (static_cast<size_t>(task->y) * c.width + task->x) * 3;
+    WPT_HIP(hipMemcpy2D(hostTile, sizeof(float) * 3 * task->width, src, pitch, sizeof(float) * 3 * task->width,
+        task->height, hipMemcpyDeviceToHost));
+    return WPT_OK;
+}
+
+WPT_API int wpt_renderer_last_stats(const wpt_renderer* r, wpt_stats* stats) {
+    if (!r || !stats) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    WavefrontCounters c;
+    WPT_HIP(hipStreamSynchronize(r->stream));
+    WPT_HIP(hipMemcpy(&c, r->ctx.counters, sizeof c, hipMemcpyDeviceToHost));
+    std::memset(stats, 0, sizeof *stats);
+    stats->closest_queries = c.closestQueries;
+    stats->shadow_queries = c.shadowQueries;
+    stats->samples = r->lastSamples;
+    stats->trace_milliseconds = r->lastMs;
+    uint32_t depth = std::min<uint32_t>(64u, r->config.max_depth);
+    for (uint32_t i = 0; i < depth; ++i) stats->queue_counts[i] = c.queueCounts[i];
+    stats->queue_count_depth = depth;
+    return WPT_OK;
+}
+
+}  // extern "C"

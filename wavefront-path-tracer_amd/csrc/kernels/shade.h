@@ -1,0 +1,586 @@
+// Path shading: surface attributes of the winning hit, next-event estimation,
+// BSDF continuation, Russian roulette and radiance accumulation.
+//
+// Reference: include/renderer/device_path_shading.cuh:6-173 (shadePaths),
+// device_traversal.cuh:100-180 (hit attributes), :319-441 (material lookup,
+// surface evaluation), :529-569 (sampleDirectLight), :571-635 (sky and
+// dielectric continuation), :685-725 (diffuse/glossy continuation),
+// device_sampling.cuh:6-93 (sampling, Fresnel), specular_glossiness.cuh:13-80
+// (GGX spec-gloss), path_state.cuh:33-43 (indirect clamp).
+//
+// The random numbers are drawn in the reference's order; where the reference
+// leaves the order unspecified (the three draws of randomInUnitSphere are
+// constructor arguments) they are drawn x, y, z.  Materials here are
+// untextured: texture fallbacks of 1.0 reduce the reference's texture terms
+// to the material constants exactly.
+#pragma once
+
+#include "../common/rng.h"
+#include "traverse.h"
+
+namespace wpt {
+
+constexpr float kInvPi = 0.31830988618f;
+constexpr float kTwoPi = 6.28318530718f;
+
+struct PathState {
+    Ray ray;
+    Vec3 throughput;
+    Vec3 radiance;
+    uint32_t pixel;
+    uint32_t depth;
+    uint64_t rng;
+    uint32_t mediumMaterial;
+    float mediumDensity;
+    float previousBsdfPdf;
+    uint32_t specularBounce;
+    uint32_t active;
+    float etaScale;
+};
+
+struct ShadeParams {
+    uint32_t maxDepth;
+    uint32_t rrStart;
+    uint32_t rngStrategy;
+    float clamp;
+    uint32_t intersectionDebug;
+    uint32_t shadingNormalDebug;
+};
+
+// Reference Hit (geometry.cuh:68-82), the fields the untextured path reads.
+struct SurfaceHit {
+    float t;
+    Vec3 position;
+    Vec3 geometricNormal;
+    Vec3 normal;
+    Vec2 uv;
+    uint32_t material;
+    uint32_t lightIndex;
+};
+
+WPT_HD float rand01(PathState& p, uint32_t strategy) { return rngNextFloat(p.rng, strategy); }
+
+WPT_HD void addPathRadiance(PathState& path, Vec3 contribution, float clamp, bool emissionOrBackground) {
+    bool indirect = path.depth > (emissionOrBackground ? 1u : 0u);
+    if (indirect && clamp > 0.0f) {
+        float magnitude = fabsf(contribution.x) / 3.0f + fabsf(contribution.y) / 3.0f + fabsf(contribution.z) / 3.0f;
+        if (magnitude > clamp) contribution *= clamp / magnitude;
+    }
+    path.radiance += contribution;
+}
+
+WPT_HD float powerHeuristic(float a, float b) {
+    float a2 = a * a;
+    float b2 = b * b;
+    return a2 / (a2 + b2);
+}
+
+// ---- surface attributes of the winner (device_traversal.cuh:100-180, 1150-1183)
+WPT_HD Vec3 triangleShadingNormal(const TriShade& tri, const Ray& ray, float u, float v, bool faceForward) {
+    Vec3 normal;
+    if (tri.flags & kTriHasNormals)
+        normal = normalize((1.0f - u - v) * tri.n0 + u * tri.n1 + v * tri.n2);
+    else
+        normal = normalize(cross(tri.v1 - tri.v0, tri.v2 - tri.v0));
+    return faceForward && dot(normal, ray.direction) > 0.0f ? -normal : normal;
+}
+
+WPT_HD Vec3 triangleGeometricNormal(const TriShade& tri, const Ray& ray, bool faceForward) {
+    Vec3 normal = normalize(cross(tri.v1 - tri.v0, tri.v2 - tri.v0));
+    return faceForward && dot(normal, ray.direction) > 0.0f ? -normal : normal;
+}
+
+WPT_HD Vec2 triangleUv(const TriShade& tri, float u, float v) {
+    if (!(tri.flags & kTriHasTexcoords)) return Vec2{0.0f, 0.0f};
+    float w = 1.0f - u - v;
+    return Vec2{w * tri.uv0.x + u * tri.uv1.x + v * tri.uv2.x, w * tri.uv0.y + u * tri.uv1.y + v * tri.uv2.y};
+}
+
+WPT_HD SurfaceHit surfaceAttributes(const DevScene& s, const Ray& ray, const HitRecord& h) {
+    SurfaceHit hit;
+    hit.t = h.t;
+    hit.material = h.material;
+    hit.uv = Vec2{0.0f, 0.0f};
+    hit.lightIndex = kInvalid;
+    if (h.kind == kHitSphere) {
+        float4 sp = s.spheres[h.prim];
+        hit.normal = normalize(ray.at(h.t) - xyz(sp));
+        hit.geometricNormal = hit.normal;
+    } else {
+        const TriShade* tri;
+        Ray local = ray;
+        const Instance* inst = nullptr;
+        if (h.kind == kHitInstance) {
+            inst = &s.instances[h.instance];
+            tri = &s.blases[inst->mesh].triangles[h.prim];
+            local = toLocal(inst->transform, ray);
+        } else if (h.kind == kHitStatic) {
+            tri = &s.staticTriangles[h.prim];
+        } else {
+            tri = &s.flatTriangles[h.prim];
+        }
+        bool faceForward = s.materials[h.material].type != kDielectric;
+        hit.uv = triangleUv(*tri, h.u, h.v);
+        hit.geometricNormal = triangleGeometricNormal(*tri, local, faceForward);
+        hit.normal = triangleShadingNormal(*tri, local, h.u, h.v, faceForward);
+        if (inst) {
+            hit.lightIndex = inst->materialOverride == kInvalid ? tri->lightIndex : kInvalid;
+            hit.geometricNormal = transformNormal(inst->transform, hit.geometricNormal);
+            hit.normal = transformNormal(inst->transform, hit.normal);
+            if (hit.lightIndex != kInvalid) hit.lightIndex += inst->lightOffset;
+        } else {
+            hit.lightIndex = h.kind == kHitFlat ? kInvalid : tri->lightIndex;
+        }
+    }
+    hit.position = ray.at(h.t);
+    return hit;
+}
+
+// ---- sampling and BSDFs (device_sampling.cuh, specular_glossiness.cuh)
+WPT_HD Vec3 reflect(const Vec3& incident, const Vec3& normal) {
+    return incident - 2.0f * dot(incident, normal) * normal;
+}
+
+WPT_HD Vec3 refract(const Vec3& incident, const Vec3& normal, float eta) {
+    float cosTheta = fminf(dot(-incident, normal), 1.0f);
+    Vec3 perpendicular = eta * (incident + cosTheta * normal);
+    Vec3 parallel = -sqrtf(fmaxf(0.0f, 1.0f - lengthSquared(perpendicular))) * normal;
+    return perpendicular + parallel;
+}
+
+WPT_HD float dielectricReflectance(float cosine, float eta) {
+    if (eta == 1.0f) return 0.0f;
+    cosine = fminf(1.0f, fmaxf(0.0f, cosine));
+    float sin2Transmitted = eta * eta * fmaxf(0.0f, 1.0f - cosine * cosine);
+    if (sin2Transmitted >= 1.0f) return 1.0f;
+    float transmittedCosine = sqrtf(1.0f - sin2Transmitted);
+    float perpendicular = (eta * cosine - transmittedCosine) / (eta * cosine + transmittedCosine);
+    float parallel = (cosine - eta * transmittedCosine) / (cosine + eta * transmittedCosine);
+    return 0.5f * (perpendicular * perpendicular + parallel * parallel);
+}
+
+struct DielectricInterface {
+    Vec3 normal;
+    Vec3 reflection;
+    Vec3 transmission;
+    float eta;
+    float reflectance;
+    bool frontFace;
+};
+
+WPT_HD DielectricInterface dielectricInterface(const Vec3& incident, const Vec3& geometricNormal, Vec3 shadingNormal,
+    float ior) {
+    bool frontFace = dot(incident, geometricNormal) < 0.0f;
+    Vec3 geometricFacing = frontFace ? geometricNormal : -geometricNormal;
+    if (dot(shadingNormal, geometricNormal) < 0.0f) shadingNormal = -shadingNormal;
+    Vec3 normal = frontFace ? shadingNormal : -shadingNormal;
+    float eta = frontFace ? 1.0f / ior : ior;
+    if (dot(incident, normal) >= 0.0f) normal = geometricFacing;
+    float reflectance = dielectricReflectance(dot(-incident, normal), eta);
+    Vec3 reflection = reflect(incident, normal);
+    Vec3 transmission = refract(incident, normal, eta);
+    if (dot(reflection, geometricFacing) <= 0.0f ||
+        (reflectance < 1.0f && dot(transmission, geometricFacing) >= 0.0f)) {
+        normal = geometricFacing;
+        reflectance = dielectricReflectance(dot(-incident, normal), eta);
+        reflection = reflect(incident, normal);
+        transmission = refract(incident, normal, eta);
+    }
+    DielectricInterface r;
+    r.normal = normal;
+    r.reflection = reflection;
+    r.transmission = transmission;
+    r.eta = eta;
+    r.reflectance = reflectance;
+    r.frontFace = frontFace;
+    return r;
+}
+
+WPT_HD Vec3 sampleCosineHemisphere(const Vec3& normal, PathState& p, uint32_t strategy) {
+    float u1 = rand01(p, strategy);
+    float u2 = rand01(p, strategy);
+    float radius = sqrtf(u1);
+    float phi = kTwoPi * u2;
+    float sinPhi, cosPhi;
+    sinCosF(phi, sinPhi, cosPhi);
+    float x = radius * cosPhi;
+    float y = radius * sinPhi;
+    float z = sqrtf(1.0f - u1);
+    Vec3 helper = fabsf(normal.x) > 0.1f ? Vec3(0.0f, 1.0f, 0.0f) : Vec3(1.0f, 0.0f, 0.0f);
+    Vec3 tangent = normalize(cross(helper, normal));
+    Vec3 bitangent = cross(normal, tangent);
+    return normalize(tangent * x + bitangent * y + normal * z);
+}
+
+WPT_HD Vec3 randomInUnitSphere(PathState& p, uint32_t strategy) {
+    while (true) {
+        float x = 2.0f * rand01(p, strategy) - 1.0f;
+        float y = 2.0f * rand01(p, strategy) - 1.0f;
+        float z = 2.0f * rand01(p, strategy) - 1.0f;
+        Vec3 point(x, y, z);
+        if (lengthSquared(point) < 1.0f) return point;
+    }
+}
+
+struct SpecGlossSurface {
+    Vec3 diffuse;
+    Vec3 f0;
+    float roughness;
+};
+
+WPT_HD float specGlossMax(const Vec3& v) { return fmaxf(v.x, fmaxf(v.y, v.z)); }
+
+WPT_HD float specGlossProbability(const SpecGlossSurface& s) {
+    float specular = specGlossMax(s.f0);
+    float diffuse = specGlossMax(s.diffuse) * (1.0f - specular);
+    return fminf(0.95f, fmaxf(0.05f, specular / fmaxf(1.0e-6f, specular + diffuse)));
+}
+
+WPT_HD float specGlossAlpha(const SpecGlossSurface& s) {
+    float roughness = fmaxf(0.02f, s.roughness);
+    return roughness * roughness;
+}
+
+WPT_HD float specGlossD(float nh, float alpha) {
+    const float pi = 3.14159265359f;
+    float a2 = alpha * alpha;
+    float denominator = (1.0f - nh * nh) + a2 * nh * nh;
+    return a2 / (pi * denominator * denominator);
+}
+
+WPT_HD float specGlossG1(float cosine, float alpha) {
+    return 2.0f * cosine / (cosine + sqrtf(alpha * alpha + (1.0f - alpha * alpha) * cosine * cosine));
+}
+
+WPT_HD Vec3 evaluateSpecGloss(const SpecGlossSurface& s, const Vec3& n, const Vec3& outgoing, const Vec3& incoming,
+    float& pdf) {
+    pdf = 0.0f;
+    float nv = dot(n, outgoing), nl = dot(n, incoming);
+    Vec3 sum = outgoing + incoming;
+    if (nv <= 0.0f || nl <= 0.0f || lengthSquared(sum) < 1.0e-12f) return Vec3();
+    Vec3 half = normalize(sum);
+    float nh = fminf(1.0f, fmaxf(0.0f, dot(n, half)));
+    float vh = fminf(1.0f, fmaxf(1.0e-8f, dot(outgoing, half)));
+    float alpha = specGlossAlpha(s);
+    float d = specGlossD(nh, alpha);
+    float probability = specGlossProbability(s);
+    pdf = (1.0f - probability) * nl * kInvPi + probability * d * nh / (4.0f * vh);
+    float schlick = powF(1.0f - vh, 5.0f);
+    Vec3 fresnel = s.f0 + (Vec3(1.0f, 1.0f, 1.0f) - s.f0) * schlick;
+    Vec3 diffuse = s.diffuse * ((1.0f - specGlossMax(s.f0)) * kInvPi);
+    float g = specGlossG1(nv, alpha) * specGlossG1(nl, alpha);
+    return diffuse + fresnel * (d * g / (4.0f * nv * nl));
+}
+
+WPT_HD Vec3 specGlossBasis(const Vec3& n, float x, float y, float z) {
+    Vec3 helper = fabsf(n.x) > 0.1f ? Vec3(0.0f, 1.0f, 0.0f) : Vec3(1.0f, 0.0f, 0.0f);
+    Vec3 tangent = normalize(cross(helper, n));
+    return normalize(x * tangent + y * cross(n, tangent) + z * n);
+}
+
+WPT_HD Vec3 sampleSpecGloss(const SpecGlossSurface& s, const Vec3& n, const Vec3& outgoing, float choice, float u,
+    float v) {
+    float phi = kTwoPi * v;
+    float sinPhi, cosPhi;
+    sinCosF(phi, sinPhi, cosPhi);
+    if (choice < specGlossProbability(s)) {
+        float alpha = specGlossAlpha(s);
+        float cosine = sqrtf((1.0f - u) / (1.0f + (alpha * alpha - 1.0f) * u));
+        float sine = sqrtf(fmaxf(0.0f, 1.0f - cosine * cosine));
+        Vec3 half = specGlossBasis(n, sine * cosPhi, sine * sinPhi, cosine);
+        return 2.0f * dot(outgoing, half) * half - outgoing;
+    }
+    float radius = sqrtf(u);
+    return specGlossBasis(n, radius * cosPhi, radius * sinPhi, sqrtf(1.0f - u));
+}
+
+// ---- material evaluation (device_traversal.cuh:427-441, 519-527)
+WPT_HD bool isMetalLike(const Material& m) { return m.type == kMetal || (m.type == kDiffuse && m.metallic > 0.5f); }
+
+WPT_HD SpecGlossSurface glossySurface(const Material& m) {
+    SpecGlossSurface s;
+    if (m.type == kSpecularGlossiness) {
+        s.diffuse = m.albedo;
+        s.f0 = m.specular;
+    } else {
+        s.diffuse = Vec3();
+        s.f0 = m.albedo;
+    }
+    s.roughness = m.roughness;
+    return s;
+}
+
+WPT_HD Vec3 evaluateSurfaceCos(const Material& m, const Vec3& shadingNormal, const Vec3& view, const Vec3& light,
+    float& pdf) {
+    Vec3 value;
+    if (m.type == kSpecularGlossiness) {
+        value = evaluateSpecGloss(glossySurface(m), shadingNormal, view, light, pdf);
+    } else if (isMetalLike(m)) {
+        if (m.roughness <= 0.0f) {
+            pdf = 0.0f;
+            value = Vec3();
+        } else {
+            SpecGlossSurface s;
+            s.diffuse = Vec3();
+            s.f0 = m.albedo;
+            s.roughness = m.roughness;
+            value = evaluateSpecGloss(s, shadingNormal, view, light, pdf);
+        }
+    } else {
+        pdf = fmaxf(0.0f, dot(shadingNormal, light)) * kInvPi;
+        value = m.albedo * kInvPi;
+    }
+    return value * fmaxf(0.0f, dot(shadingNormal, light));
+}
+
+// ---- next-event estimation toward triangle lights (device_traversal.cuh:529-569)
+template <class Stack>
+WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
+    const Material& material, const ShadeParams& sp, Stack& stack, uint32_t& shadowQueries) {
+    if (s.lightCount == 0) return;
+    uint32_t candidate = static_cast<uint32_t>(rand01(path, sp.rngStrategy) * static_cast<float>(s.lightCount));
+    if (candidate > s.lightCount - 1) candidate = s.lightCount - 1;
+    LightAlias entry = s.lightAlias[candidate];
+    uint32_t lightIndex = rand01(path, sp.rngStrategy) < entry.probability ? candidate : entry.alias;
+    const TriangleLight& light = s.lights[lightIndex];
+    float rootU = sqrtf(rand01(path, sp.rngStrategy));
+    float v = rand01(path, sp.rngStrategy);
+    Vec3 lightPosition = (1.0f - rootU) * light.v0 + rootU * (1.0f - v) * light.v1 + rootU * v * light.v2;
+    Vec3 toLight = lightPosition - hit.position;
+    float distanceSquared = lengthSquared(toLight);
+    float distance = sqrtf(distanceSquared);
+    Vec3 direction = toLight / distance;
+    float geometricCosine = fmaxf(0.0f, dot(hit.geometricNormal, direction));
+    float shadingCosine = fmaxf(0.0f, dot(shadingNormal, direction));
+    float lightCosine = fmaxf(0.0f, dot(light.normal, -direction));
+    if (geometricCosine == 0.0f || shadingCosine == 0.0f || lightCosine == 0.0f) return;
+    Vec3 shadowOrigin = offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t);
+    Vec3 shadowTarget = offsetRayOrigin(lightPosition, light.normal, -direction, distance);
+    Vec3 segment = shadowTarget - shadowOrigin;
+    float shadowDistance = length(segment);
+    if (!(shadowDistance > kRayMinDistance)) return;
+    Ray shadow(shadowOrigin, segment / shadowDistance);
+    if (!s.ignoreDirectLightOcclusion) {
+        ++shadowQueries;
+        if (traceOccluded(s, shadow, shadowDistance - kRayMinDistance, stack)) return;
+    }
+    const Material& lightMaterial = s.materials[light.material];
+    float geometryTerm = lightCosine * light.area / (distanceSquared * light.selectionPdf);
+    float lightPdf = light.selectionPdf * distanceSquared / (lightCosine * light.area);
+    float bsdfPdf;
+    Vec3 bsdf = evaluateSurfaceCos(material, shadingNormal, -path.ray.direction, direction, bsdfPdf);
+    float misWeight = powerHeuristic(lightPdf, bsdfPdf);
+    Vec3 directLighting = hadamard(bsdf, lightMaterial.emission) * (geometryTerm * misWeight);
+    addPathRadiance(path, hadamard(path.throughput, directLighting), sp.clamp, false);
+}
+
+// ---- continuations
+WPT_HD Vec3 continueDiffuse(PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal, const Material& m,
+    uint32_t strategy) {
+    Vec3 direction = sampleCosineHemisphere(shadingNormal, path, strategy);
+    float cosine = dot(shadingNormal, direction);
+    float pdf = fmaxf(0.0f, dot(shadingNormal, direction)) * kInvPi;
+    if (!(pdf > 0.0f) || cosine <= 0.0f || dot(direction, hit.geometricNormal) <= 0.0f) {
+        path.active = 0;
+        return direction;
+    }
+    path.throughput = hadamard(path.throughput, m.albedo) * (cosine * kInvPi / pdf);
+    path.previousBsdfPdf = pdf;
+    path.specularBounce = 0;
+    return direction;
+}
+
+WPT_HD Vec3 continueGlossy(PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal, const Material& m,
+    uint32_t strategy) {
+    SpecGlossSurface surface = glossySurface(m);
+    Vec3 outgoing = -path.ray.direction;
+    float choice = rand01(path, strategy);
+    float u = rand01(path, strategy);
+    float v = rand01(path, strategy);
+    Vec3 direction = sampleSpecGloss(surface, shadingNormal, outgoing, choice, u, v);
+    float pdf;
+    Vec3 bsdf = evaluateSpecGloss(surface, shadingNormal, outgoing, direction, pdf);
+    if (!(pdf > 0.0f) || dot(direction, hit.geometricNormal) <= 0.0f) {
+        path.active = 0;
+        return direction;
+    }
+    path.throughput = hadamard(path.throughput, bsdf) * (fmaxf(0.0f, dot(shadingNormal, direction)) / pdf);
+    path.previousBsdfPdf = pdf;
+    path.specularBounce = 0;
+    return direction;
+}
+
+WPT_HD Vec3 continueDielectric(PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal, const Material& m,
+    uint32_t strategy) {
+    path.throughput = hadamard(path.throughput, m.albedo);
+    DielectricInterface boundary = dielectricInterface(normalize(path.ray.direction), hit.geometricNormal,
+        shadingNormal, m.ior);
+    float etaSquared = boundary.eta * boundary.eta;
+    bool reflected = boundary.reflectance >= 1.0f || boundary.reflectance > rand01(path, strategy);
+    if (!reflected) {
+        path.throughput *= etaSquared;
+        path.etaScale /= etaSquared;
+        if (boundary.frontFace) {
+            path.mediumMaterial = hit.material;
+            path.mediumDensity = m.volumeDensity * 1.0f;
+        } else {
+            path.mediumMaterial = kInvalid;
+            path.mediumDensity = 1.0f;
+        }
+    }
+    path.specularBounce = 1;
+    path.previousBsdfPdf = 0.0f;
+    return reflected ? boundary.reflection : boundary.transmission;
+}
+
+// One bounce of shading.  Returns true when the path continues with
+// path.ray set to the continuation; on termination *done receives the
+// accumulated radiance to add to the pixel.
+template <class Stack>
+WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp, Stack& stack,
+    Vec3& done, uint32_t& shadowQueries) {
+    bool didHit = h.kind != kHitNone;
+    if (sp.intersectionDebug || sp.shadingNormalDebug) {
+        Vec3 color(0.05f, 0.10f, 0.25f);
+        if (didHit) {
+            SurfaceHit hit = surfaceAttributes(s, path.ray, h);
+            color = 0.5f * (hit.normal + Vec3(1.0f, 1.0f, 1.0f));
+        }
+        path.active = 0;
+        done = color;
+        return false;
+    }
+    if (!didHit) {
+        if (!s.blackBackground) {
+            float t = 0.5f * (path.ray.direction.y + 1.0f);
+            Vec3 sky = (1.0f - t) * Vec3(1.0f, 1.0f, 1.0f) + t * Vec3(0.5f, 0.7f, 1.0f);
+            addPathRadiance(path, hadamard(path.throughput, sky), sp.clamp, true);
+        }
+        path.active = 0;
+        done = path.radiance;
+        return false;
+    }
+    SurfaceHit hit = surfaceAttributes(s, path.ray, h);
+    const Material& material = s.materials[hit.material];
+    if (path.mediumMaterial != kInvalid) {
+        const Material& medium = s.materials[path.mediumMaterial];
+        if (medium.attenuationDistance > 0.0f) {
+            float distance = hit.t * path.mediumDensity;
+            Vec3 sigmaA(-logF(fmaxf(medium.attenuationColor.x, 0.0001f)) / medium.attenuationDistance,
+                -logF(fmaxf(medium.attenuationColor.y, 0.0001f)) / medium.attenuationDistance,
+                -logF(fmaxf(medium.attenuationColor.z, 0.0001f)) / medium.attenuationDistance);
+            path.throughput = hadamard(path.throughput,
+                Vec3(expF(-sigmaA.x * distance), expF(-sigmaA.y * distance), expF(-sigmaA.z * distance)));
+        }
+    }
+    if (material.type == kEmissive) {
+        float misWeight = 1.0f;
+        if (hit.lightIndex != kInvalid) {
+            const TriangleLight& light = s.lights[hit.lightIndex];
+            float lightCosine = fmaxf(0.0f, dot(light.normal, -path.ray.direction));
+            if (lightCosine == 0.0f) {
+                misWeight = 0.0f;
+            } else if (path.depth > 0 && !path.specularBounce) {
+                float lightPdf = light.selectionPdf * hit.t * hit.t / (lightCosine * light.area);
+                misWeight = powerHeuristic(path.previousBsdfPdf, lightPdf);
+            }
+        }
+        addPathRadiance(path, hadamard(path.throughput, material.emission) * misWeight, sp.clamp, true);
+        path.active = 0;
+        done = path.radiance;
+        return false;
+    }
+    Vec3 shadingNormal = hit.normal;
+    if (dot(shadingNormal, hit.geometricNormal) < 0.0f) shadingNormal = -shadingNormal;
+    uint32_t type = (material.type == kDiffuse && material.metallic > 0.5f) ? kMetal : material.type;
+    float roughness = material.roughness;
+    Vec3 direction;
+    switch (type) {
+    case kDiffuse:
+        sampleDirectLight(s, path, hit, shadingNormal, material, sp, stack, shadowQueries);
+        direction = continueDiffuse(path, hit, shadingNormal, material, sp.rngStrategy);
+        break;
+    case kSpecularGlossiness:
+        sampleDirectLight(s, path, hit, shadingNormal, material, sp, stack, shadowQueries);
+        direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
+        break;
+    case kMetal:
+        if (roughness > 0.0f) {
+            sampleDirectLight(s, path, hit, shadingNormal, material, sp, stack, shadowQueries);
+            direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
+            break;
+        }
+        {
+            path.throughput = hadamard(path.throughput, material.albedo);
+            Vec3 reflected = reflect(normalize(path.ray.direction), shadingNormal);
+            Vec3 jitter = randomInUnitSphere(path, sp.rngStrategy);
+            direction = normalize(reflected + roughness * jitter);
+            if (dot(direction, shadingNormal) <= 0.0f || dot(direction, hit.geometricNormal) <= 0.0f)
+                direction = reflect(normalize(path.ray.direction), hit.geometricNormal);
+            path.specularBounce = 1;
+            path.previousBsdfPdf = 0.0f;
+        }
+        break;
+    case kDielectric:
+        direction = continueDielectric(path, hit, shadingNormal, material, sp.rngStrategy);
+        break;
+    default:
+        break;
+    }
+    if (path.active) {
+        path.ray.origin = offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t);
+        path.ray.direction = direction;
+        ++path.depth;
+        if (path.depth >= sp.maxDepth) {
+            path.active = 0;
+            done = path.radiance;
+            return false;
+        }
+        if (path.depth >= sp.rrStart) {
+            float survival = fminf(0.95f, fmaxf(0.05f, path.etaScale * fmaxf(path.throughput.x,
+                fmaxf(path.throughput.y, path.throughput.z))));
+            if (rand01(path, sp.rngStrategy) > survival) {
+                path.active = 0;
+                done = path.radiance;
+                return false;
+            }
+            path.throughput *= 1.0f / survival;
+        }
+        return true;
+    }
+    done = path.radiance;
+    return false;
+}
+
+// Primary ray and fresh path state for one pixel (renderer.cu:25-41,
+// camera.cuh:17-27).
+struct CameraParams {
+    Vec3 position, forward, right, up;
+    float verticalFov;
+    float aspectRatio;
+};
+
+WPT_HD PathState primaryPath(const CameraParams& cam, uint32_t x, uint32_t y, uint32_t width, uint32_t height,
+    uint32_t sample, uint32_t strategy) {
+    PathState p;
+    p.pixel = y * width + x;
+    p.rng = rngSeed(pixelRngKey(p.pixel, sample, strategy), strategy);
+    float u = (static_cast<float>(x) + rand01(p, strategy)) / static_cast<float>(width);
+    float v = (static_cast<float>(y) + rand01(p, strategy)) / static_cast<float>(height);
+    float tanHalfFov = tanF(cam.verticalFov * 0.5f);
+    float px = (2.0f * u - 1.0f) * cam.aspectRatio * tanHalfFov;
+    float py = (1.0f - 2.0f * v) * tanHalfFov;
+    p.ray = Ray(cam.position, normalize(cam.forward + cam.right * px + cam.up * py));
+    p.throughput = Vec3(1.0f, 1.0f, 1.0f);
+    p.radiance = Vec3(0.0f, 0.0f, 0.0f);
+    p.depth = 0;
+    p.mediumMaterial = kInvalid;
+    p.mediumDensity = 1.0f;
+    p.previousBsdfPdf = 0.0f;
+    p.specularBounce = 1;
+    p.active = 1;
+    p.etaScale = 1.0f;
+    return p;
+}
+
+}  // namespace wpt

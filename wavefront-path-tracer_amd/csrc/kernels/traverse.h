@@ -1,0 +1,461 @@
+// Closest-hit and shadow any-hit traversal over the device scene.
+//
+// Reference: include/renderer/device_traversal.cuh:24-91 (sphere/triangle
+// tests), :182-222 (slab test), :239-313 (BLAS BVH2), :789-878 (sphere leaf,
+// mesh instance, TLAS), :1035-1184 (isOccluded, traceRay).
+//
+// The node layout differs from the reference (64 B nodes holding both child
+// boxes, see device_scene.h) but every traversal decision is the same:
+//  * the reference pushes far then near and immediately pops near, whose
+//    re-test cannot fail (closestT is unchanged), so near is taken directly;
+//  * a popped entry is re-tested against the current closestT; the slab
+//    test's tmax part cannot change, so the re-test is tmin < closestT with
+//    the tmin stored at push time (bit-identical to recomputing it);
+//  * leaf children are visited exactly when the reference would pop them.
+// With the same BVH and IEEE FP32 arithmetic in the same order, hit
+// identities and distances are bit-identical to the reference algorithm.
+#pragma once
+
+#include "../common/vec.h"
+#include "../scene/device_scene.h"
+
+namespace wpt {
+
+constexpr float kNoHit = 1e30f;
+
+// Winning primitive of a closest-hit query.
+struct HitRecord {
+    float t;
+    float u, v;
+    uint32_t kind;      // WPT_HIT_* values
+    uint32_t instance;  // instance index for instance triangles
+    uint32_t prim;      // sphere / triangle index in the caller's order
+    uint32_t material;  // resolved (override applied)
+};
+
+constexpr uint32_t kHitNone = 0, kHitSphere = 1, kHitStatic = 2, kHitInstance = 3, kHitFlat = 4;
+
+WPT_HD HitRecord emptyHit() {
+    HitRecord h;
+    h.t = kNoHit;
+    h.u = h.v = 0.0f;
+    h.kind = kHitNone;
+    h.instance = kInvalid;
+    h.prim = kInvalid;
+    h.material = kInvalid;
+    return h;
+}
+
+// Quadratic with unnormalized b; nearest root beyond the cutoff.
+WPT_HD bool intersectSphere(const Ray& ray, const Vec3& center, float radius, float& t) {
+    Vec3 oc = ray.origin - center;
+    float a = dot(ray.direction, ray.direction);
+    float b = 2.0f * dot(oc, ray.direction);
+    float c = dot(oc, oc) - radius * radius;
+    float discriminant = b * b - 4.0f * a * c;
+    if (discriminant < 0.0f) return false;
+    float sqrtD = sqrtf(discriminant);
+    float t0 = (-b - sqrtD) / (2.0f * a);
+    float t1 = (-b + sqrtD) / (2.0f * a);
+    if (t0 > kRayMinDistance) {
+        t = t0;
+        return true;
+    }
+    if (t1 > kRayMinDistance) {
+        t = t1;
+        return true;
+    }
+    return false;
+}
+
+// Moller-Trumbore with the determinant guard relative to its own products.
+WPT_HD bool intersectTriangle(const Ray& ray, const Vec3& v0, const Vec3& v1, const Vec3& v2, float& t, float& u,
+    float& v) {
+    Vec3 edge1 = v1 - v0;
+    Vec3 edge2 = v2 - v0;
+    Vec3 p = cross(ray.direction, edge2);
+    float determinant = dot(edge1, p);
+    const Vec3& d = ray.direction;
+    float scale = fabsf(edge1.x) * (fabsf(d.y * edge2.z) + fabsf(d.z * edge2.y)) +
+        fabsf(edge1.y) * (fabsf(d.z * edge2.x) + fabsf(d.x * edge2.z)) +
+        fabsf(edge1.z) * (fabsf(d.x * edge2.y) + fabsf(d.y * edge2.x));
+    if (!isFinite(determinant) || fabsf(determinant) <= 8.0f * kFltEpsilon * scale) return false;
+    float inverseDeterminant = 1.0f / determinant;
+    Vec3 originToVertex = ray.origin - v0;
+    u = dot(originToVertex, p) * inverseDeterminant;
+    if (u < 0.0f || u > 1.0f) return false;
+    Vec3 q = cross(originToVertex, edge1);
+    v = dot(ray.direction, q) * inverseDeterminant;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(edge2, q) * inverseDeterminant;
+    return t > kRayMinDistance;
+}
+
+struct RayInv {
+    Ray ray;
+    Vec3 inv;
+};
+
+WPT_HD RayInv makeRayInv(const Ray& r) {
+    RayInv ri;
+    ri.ray = r;
+    ri.inv = Vec3(1.0f / r.direction.x, 1.0f / r.direction.y, 1.0f / r.direction.z);
+    return ri;
+}
+
+WPT_HD bool slab(const RayInv& r, float mnx, float mny, float mnz, float mxx, float mxy, float mxz, float closestT,
+    float& nearT) {
+    float tx0 = (mnx - r.ray.origin.x) * r.inv.x;
+    float tx1 = (mxx - r.ray.origin.x) * r.inv.x;
+    float ty0 = (mny - r.ray.origin.y) * r.inv.y;
+    float ty1 = (mxy - r.ray.origin.y) * r.inv.y;
+    float tz0 = (mnz - r.ray.origin.z) * r.inv.z;
+    float tz1 = (mxz - r.ray.origin.z) * r.inv.z;
+    float tmin = fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1)));
+    float tmax = fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1)));
+    nearT = tmin;
+    return tmax >= fmaxf(tmin, kRayMinDistance) && tmin < closestT;
+}
+
+WPT_HD Vec3 xyz(const float4& f) { return Vec3(f.x, f.y, f.z); }
+WPT_HD uint32_t wbits(const float4& f) { return bitsOf(f.w); }
+
+// Private per-lane traversal stack (node pointer + entry distance).  The
+// scene build rejects trees whose combined TLAS+BLAS depth exceeds it.
+struct PrivateStack {
+    static constexpr uint32_t kCapacity = 64;
+    uint32_t node[kCapacity];
+    float tmin[kCapacity];
+    uint32_t size = 0;
+    WPT_HD void push(uint32_t n, float t) {
+        node[size] = n;
+        tmin[size] = t;
+        ++size;
+    }
+    WPT_HD void pop(uint32_t& n, float& t) {
+        --size;
+        n = node[size];
+        t = tmin[size];
+    }
+};
+
+// Pops until an entry still closer than closestT; false when the traversal
+// level (entries above `base`) is exhausted.
+template <class Stack>
+WPT_HD bool popLive(Stack& stack, uint32_t base, float closestT, uint32_t& cur) {
+    while (stack.size > base) {
+        float t;
+        stack.pop(cur, t);
+        if (t < closestT) return true;
+    }
+    return false;
+}
+
+// Closest triangle of one BVH (BLAS in local space, or the flat-triangle BVH).
+template <class Stack>
+WPT_HD void closestInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry, const uint8_t* cutout,
+    uint32_t materialOverride, uint32_t kind, uint32_t instance, HitRecord& best, Stack& stack) {
+    float nearT;
+    if (!slab(r, bvh.rootMin[0], bvh.rootMin[1], bvh.rootMin[2], bvh.rootMax[0], bvh.rootMax[1], bvh.rootMax[2],
+            best.t, nearT))
+        return;
+    uint32_t base = stack.size;
+    uint32_t cur = bvh.root;
+    while (true) {
+        if (cur & kLeafBit) {
+            uint32_t slot = cur & ~kLeafBit;
+            float4 a = geometry[3 * slot + 0];
+            float4 b = geometry[3 * slot + 1];
+            float4 c = geometry[3 * slot + 2];
+            float t, u, v;
+            if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < best.t) {
+                uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
+                if (!cutout[material]) {
+                    best.t = t;
+                    best.u = u;
+                    best.v = v;
+                    best.kind = kind;
+                    best.instance = instance;
+                    best.prim = wbits(a);
+                    best.material = material;
+                }
+            }
+        } else {
+            const BvhNode& n = bvh.nodes[cur];
+            float tl, tr;
+            bool hl = slab(r, n.lminx, n.lminy, n.lminz, n.lmaxx, n.lmaxy, n.lmaxz, best.t, tl);
+            bool hr = slab(r, n.rminx, n.rminy, n.rminz, n.rmaxx, n.rmaxy, n.rmaxz, best.t, tr);
+            if (hl && hr) {
+                bool leftNear = tl < tr;
+                stack.push(leftNear ? n.right : n.left, leftNear ? tr : tl);
+                cur = leftNear ? n.left : n.right;
+                continue;
+            }
+            if (hl) {
+                cur = n.left;
+                continue;
+            }
+            if (hr) {
+                cur = n.right;
+                continue;
+            }
+        }
+        if (!popLive(stack, base, best.t, cur)) return;
+    }
+}
+
+// First triangle of one BVH closer than maxDistance (shadow rays).
+template <class Stack>
+WPT_HD bool anyInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry, const uint8_t* cutout,
+    uint32_t materialOverride, float maxDistance, Stack& stack) {
+    float nearT;
+    if (!slab(r, bvh.rootMin[0], bvh.rootMin[1], bvh.rootMin[2], bvh.rootMax[0], bvh.rootMax[1], bvh.rootMax[2],
+            maxDistance, nearT))
+        return false;
+    uint32_t base = stack.size;
+    uint32_t cur = bvh.root;
+    while (true) {
+        if (cur & kLeafBit) {
+            uint32_t slot = cur & ~kLeafBit;
+            float4 a = geometry[3 * slot + 0];
+            float4 b = geometry[3 * slot + 1];
+            float4 c = geometry[3 * slot + 2];
+            float t, u, v;
+            if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < maxDistance) {
+                uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
+                if (!cutout[material]) {
+                    stack.size = base;
+                    return true;
+                }
+            }
+        } else {
+            const BvhNode& n = bvh.nodes[cur];
+            float tl, tr;
+            bool hl = slab(r, n.lminx, n.lminy, n.lminz, n.lmaxx, n.lmaxy, n.lmaxz, maxDistance, tl);
+            bool hr = slab(r, n.rminx, n.rminy, n.rminz, n.rmaxx, n.rmaxy, n.rmaxz, maxDistance, tr);
+            if (hl && hr) {
+                bool leftNear = tl < tr;
+                stack.push(leftNear ? n.right : n.left, leftNear ? tr : tl);
+                cur = leftNear ? n.left : n.right;
+                continue;
+            }
+            if (hl) {
+                cur = n.left;
+                continue;
+            }
+            if (hr) {
+                cur = n.right;
+                continue;
+            }
+        }
+        if (!popLive(stack, base, maxDistance, cur)) return false;
+    }
+}
+
+// Brute-force closest over original-order triangles (direct instances,
+// static triangles, flat triangles without a BVH).
+WPT_HD void closestBrute(const Ray& ray, const TriShade* tris, uint32_t count, const uint8_t* cutout,
+    uint32_t materialOverride, uint32_t kind, uint32_t instance, HitRecord& best) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const TriShade& tri = tris[i];
+        float t, u, v;
+        if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !(t < best.t)) continue;
+        uint32_t material = materialOverride == kInvalid ? tri.material : materialOverride;
+        if (cutout[material]) continue;
+        best.t = t;
+        best.u = u;
+        best.v = v;
+        best.kind = kind;
+        best.instance = instance;
+        best.prim = i;
+        best.material = material;
+    }
+}
+
+WPT_HD bool anyBrute(const Ray& ray, const TriShade* tris, uint32_t count, const uint8_t* cutout,
+    uint32_t materialOverride, float maxDistance) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const TriShade& tri = tris[i];
+        float t, u, v;
+        if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !(t < maxDistance)) continue;
+        uint32_t material = materialOverride == kInvalid ? tri.material : materialOverride;
+        if (!cutout[material]) return true;
+    }
+    return false;
+}
+
+WPT_HD Ray toLocal(const InstanceTransform& tr, const Ray& ray) {
+    Ray local;
+    local.origin = inverseTransformVector(tr, ray.origin - tr.translation);
+    local.direction = inverseTransformVector(tr, ray.direction);
+    return local;
+}
+
+// Sphere leaf: nearer root wins; on an exact tie the lower sphere index wins
+// (reference device_traversal.cuh:789-800).
+WPT_HD void sphereLeaf(const DevScene& s, const Ray& ray, uint32_t index, HitRecord& best) {
+    float4 sp = s.spheres[index];
+    float t;
+    if (!intersectSphere(ray, xyz(sp), sp.w, t) || t > best.t) return;
+    uint32_t currentSphere = best.kind == kHitSphere ? best.prim : kInvalid;
+    if (t == best.t && index >= currentSphere) return;
+    best.t = t;
+    best.u = best.v = 0.0f;
+    best.kind = kHitSphere;
+    best.instance = kInvalid;
+    best.prim = index;
+    best.material = s.sphereMaterial[index];
+}
+
+template <class Stack>
+WPT_HD void instanceClosest(const DevScene& s, const Ray& ray, uint32_t index, bool useBvh, HitRecord& best,
+    Stack& stack) {
+    const Instance& inst = s.instances[index];
+    const Blas& blas = s.blases[inst.mesh];
+    Ray local = toLocal(inst.transform, ray);
+    if (useBvh && blas.bvh.primCount != 0) {
+        RayInv ri = makeRayInv(local);
+        closestInBvh(ri, blas.bvh, blas.geometry, s.materialCutout, inst.materialOverride, kHitInstance, index, best,
+            stack);
+    } else {
+        closestBrute(local, blas.triangles, blas.triangleCount, s.materialCutout, inst.materialOverride,
+            kHitInstance, index, best);
+    }
+}
+
+// Closest hit in the reference's category order: direct spheres, direct
+// instances, static triangles, TLAS, flat triangles.
+template <class Stack>
+WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
+    HitRecord best = emptyHit();
+    for (uint32_t i = 0; i < s.directSphereCount; ++i) sphereLeaf(s, ray, s.directSpheres[i], best);
+    for (uint32_t i = 0; i < s.directInstanceCount; ++i) instanceClosest(s, ray, s.directInstances[i], false, best, stack);
+    closestBrute(ray, s.staticTriangles, s.staticCount, s.materialCutout, kInvalid, kHitStatic, kInvalid, best);
+    if (s.tlas.primCount != 0) {
+        RayInv r = makeRayInv(ray);
+        float nearT;
+        if (slab(r, s.tlas.rootMin[0], s.tlas.rootMin[1], s.tlas.rootMin[2], s.tlas.rootMax[0], s.tlas.rootMax[1],
+                s.tlas.rootMax[2], best.t, nearT)) {
+            uint32_t base = stack.size;
+            uint32_t cur = s.tlas.root;
+            while (true) {
+                if (cur & kLeafBit) {
+                    uint32_t leaf = s.tlasLeaves[cur & ~kLeafBit];
+                    if (leaf & kTlasSphereBit)
+                        sphereLeaf(s, ray, leaf & ~kTlasSphereBit, best);
+                    else
+                        instanceClosest(s, ray, leaf, true, best, stack);
+                } else {
+                    const BvhNode& n = s.tlas.nodes[cur];
+                    float tl, tr;
+                    bool hl = slab(r, n.lminx, n.lminy, n.lminz, n.lmaxx, n.lmaxy, n.lmaxz, best.t, tl);
+                    bool hr = slab(r, n.rminx, n.rminy, n.rminz, n.rmaxx, n.rmaxy, n.rmaxz, best.t, tr);
+                    if (hl && hr) {
+                        bool leftNear = tl < tr;
+                        stack.push(leftNear ? n.right : n.left, leftNear ? tr : tl);
+                        cur = leftNear ? n.left : n.right;
+                        continue;
+                    }
+                    if (hl) {
+                        cur = n.left;
+                        continue;
+                    }
+                    if (hr) {
+                        cur = n.right;
+                        continue;
+                    }
+                }
+                if (!popLive(stack, base, best.t, cur)) break;
+            }
+        }
+    }
+    if (s.flatCount != 0) {
+        if (s.flat.primCount != 0) {
+            RayInv r = makeRayInv(ray);
+            closestInBvh(r, s.flat, s.flatGeometry, s.materialCutout, kInvalid, kHitFlat, kInvalid, best, stack);
+        } else {
+            closestBrute(ray, s.flatTriangles, s.flatCount, s.materialCutout, kInvalid, kHitFlat, kInvalid, best);
+        }
+    }
+    return best;
+}
+
+// Shadow query: any blocker with t < maxDistance.
+template <class Stack>
+WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, Stack& stack) {
+    for (uint32_t i = 0; i < s.directSphereCount; ++i) {
+        float4 sp = s.spheres[s.directSpheres[i]];
+        float t;
+        if (intersectSphere(ray, xyz(sp), sp.w, t) && t < maxDistance) return true;
+    }
+    for (uint32_t i = 0; i < s.directInstanceCount; ++i) {
+        const Instance& inst = s.instances[s.directInstances[i]];
+        const Blas& blas = s.blases[inst.mesh];
+        Ray local = toLocal(inst.transform, ray);
+        if (anyBrute(local, blas.triangles, blas.triangleCount, s.materialCutout, inst.materialOverride, maxDistance))
+            return true;
+    }
+    if (anyBrute(ray, s.staticTriangles, s.staticCount, s.materialCutout, kInvalid, maxDistance)) return true;
+    if (s.tlas.primCount != 0) {
+        RayInv r = makeRayInv(ray);
+        float nearT;
+        if (slab(r, s.tlas.rootMin[0], s.tlas.rootMin[1], s.tlas.rootMin[2], s.tlas.rootMax[0], s.tlas.rootMax[1],
+                s.tlas.rootMax[2], maxDistance, nearT)) {
+            uint32_t base = stack.size;
+            uint32_t cur = s.tlas.root;
+            while (true) {
+                if (cur & kLeafBit) {
+                    uint32_t leaf = s.tlasLeaves[cur & ~kLeafBit];
+                    if (leaf & kTlasSphereBit) {
+                        float4 sp = s.spheres[leaf & ~kTlasSphereBit];
+                        float t;
+                        if (intersectSphere(ray, xyz(sp), sp.w, t) && t < maxDistance) {
+                            stack.size = base;
+                            return true;
+                        }
+                    } else {
+                        const Instance& inst = s.instances[leaf];
+                        const Blas& blas = s.blases[inst.mesh];
+                        RayInv local = makeRayInv(toLocal(inst.transform, ray));
+                        if (anyInBvh(local, blas.bvh, blas.geometry, s.materialCutout, inst.materialOverride,
+                                maxDistance, stack)) {
+                            stack.size = base;
+                            return true;
+                        }
+                    }
+                } else {
+                    const BvhNode& n = s.tlas.nodes[cur];
+                    float tl, tr;
+                    bool hl = slab(r, n.lminx, n.lminy, n.lminz, n.lmaxx, n.lmaxy, n.lmaxz, maxDistance, tl);
+                    bool hr = slab(r, n.rminx, n.rminy, n.rminz, n.rmaxx, n.rmaxy, n.rmaxz, maxDistance, tr);
+                    if (hl && hr) {
+                        bool leftNear = tl < tr;
+                        stack.push(leftNear ? n.right : n.left, leftNear ? tr : tl);
+                        cur = leftNear ? n.left : n.right;
+                        continue;
+                    }
+                    if (hl) {
+                        cur = n.left;
+                        continue;
+                    }
+                    if (hr) {
+                        cur = n.right;
+                        continue;
+                    }
+                }
+                if (!popLive(stack, base, maxDistance, cur)) break;
+            }
+        }
+    }
+    if (s.flatCount != 0) {
+        if (s.flat.primCount != 0) {
+            RayInv r = makeRayInv(ray);
+            if (anyInBvh(r, s.flat, s.flatGeometry, s.materialCutout, kInvalid, maxDistance, stack)) return true;
+        } else if (anyBrute(ray, s.flatTriangles, s.flatCount, s.materialCutout, kInvalid, maxDistance)) {
+            return true;
+        }
+    }
+    return false;
+}
+
+}  // namespace wpt

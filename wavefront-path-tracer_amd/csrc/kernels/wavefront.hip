@@ -1,0 +1,427 @@
+// gfx950 wavefront kernels.
+//
+// Reference kernels and their MI355X counterparts:
+//   generatePrimaryRays (src/renderer/renderer.cu:10-42) ...... kPrimary
+//   intersectScene (device_traversal.cuh:1186-1194) ........... kTrace
+//   shadePaths + appendWarpCompacted (device_path_shading.cuh:6-173,
+//     queue_compaction.cuh:8-29) ............................... kShade
+//   advanceSampleIndex (renderer.cu:44-48) .................... kAdvance
+//   persistentWavefrontTrace (renderer.cu:55-125) ............. kPersistent
+//
+// Queue compaction is order preserving: a wave computes its survivors' ranks
+// with a 64-bit ballot and mbcnt, the workgroup adds per-wave offsets from
+// LDS, and workgroups take chunk tickets in order and chain their totals with
+// a decoupled look-back (one flag word per chunk).  The continuation queue is
+// therefore the stable filter of the input queue: identical on every run and
+// identical to a sequential filter, where the reference's warp atomics only
+// guarantee the same set.  A workgroup only waits for chunks whose tickets
+// were taken earlier by workgroups that are already running, so the wait
+// always ends.
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+#include "../../../include/wpt.h"
+#include "wavefront.h"
+
+namespace wpt {
+
+namespace cg = cooperative_groups;
+
+namespace {
+
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kMaxWaves = 16;  // 1024-thread workgroups
+
+__device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void storeAgent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct CompactShared {
+    uint32_t waveOffset[kMaxWaves];
+    uint32_t base;
+    uint32_t total;
+    uint32_t chunk;
+    uint32_t shadow;
+};
+
+// Slot of each surviving lane in the ordered output; every thread of the
+// workgroup must call it.  Returns the workgroup's inclusive prefix.
+__device__ uint32_t orderedSlots(bool alive, uint32_t chunk, uint32_t* flags, CompactShared& sh, uint32_t& slot) {
+    uint32_t lane = __lane_id();
+    uint32_t wave = threadIdx.x >> 6;
+    unsigned long long mask = __ballot(alive);
+    uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+    if (lane == 0) sh.waveOffset[wave] = static_cast<uint32_t>(__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t waves = blockDim.x >> 6;
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            uint32_t c = sh.waveOffset[w];
+            sh.waveOffset[w] = sum;
+            sum += c;
+        }
+        uint32_t exclusive = 0;
+        if (chunk == 0) {
+            storeAgent(&flags[0], kFlagPrefix | sum);
+        } else {
+            storeAgent(&flags[chunk], kFlagAggregate | sum);
+            uint32_t p = chunk - 1;
+            while (true) {
+                uint32_t f = loadAgent(&flags[p]);
+                uint32_t state = f & ~kValueMask;
+                if (state == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                exclusive += f & kValueMask;
+                if (state == kFlagPrefix) break;
+                --p;
+            }
+            storeAgent(&flags[chunk], kFlagPrefix | (exclusive + sum));
+        }
+        sh.base = exclusive;
+        sh.total = sum;
+    }
+    __syncthreads();
+    slot = alive ? sh.base + sh.waveOffset[wave] + rank : kInvalid;
+    return sh.base + sh.total;
+}
+
+__device__ __forceinline__ float qf(const PathQueue& q, uint32_t field, uint32_t i) {
+    return __uint_as_float(q.words[static_cast<size_t>(field) * q.capacity + i]);
+}
+__device__ __forceinline__ uint32_t qu(const PathQueue& q, uint32_t field, uint32_t i) {
+    return q.words[static_cast<size_t>(field) * q.capacity + i];
+}
+__device__ __forceinline__ void sf(const PathQueue& q, uint32_t field, uint32_t i, float v) {
+    q.words[static_cast<size_t>(field) * q.capacity + i] = __float_as_uint(v);
+}
+__device__ __forceinline__ void su(const PathQueue& q, uint32_t field, uint32_t i, uint32_t v) {
+    q.words[static_cast<size_t>(field) * q.capacity + i] = v;
+}
+
+__device__ Ray loadRay(const PathQueue& q, uint32_t i) {
+    return Ray(Vec3(qf(q, kOx, i), qf(q, kOy, i), qf(q, kOz, i)), Vec3(qf(q, kDx, i), qf(q, kDy, i), qf(q, kDz, i)));
+}
+
+__device__ PathState loadPath(const PathQueue& q, uint32_t i) {
+    PathState p;
+    p.ray = loadRay(q, i);
+    p.throughput = Vec3(qf(q, kTx, i), qf(q, kTy, i), qf(q, kTz, i));
+    p.radiance = Vec3(qf(q, kRx, i), qf(q, kRy, i), qf(q, kRz, i));
+    p.pixel = qu(q, kPixel, i);
+    uint32_t df = qu(q, kDepthFlags, i);
+    p.depth = df & 0x7fffffffu;
+    p.specularBounce = df >> 31;
+    p.rng = static_cast<uint64_t>(qu(q, kRngLo, i)) | (static_cast<uint64_t>(qu(q, kRngHi, i)) << 32);
+    p.mediumMaterial = qu(q, kMedium, i);
+    p.mediumDensity = qf(q, kMediumDensity, i);
+    p.previousBsdfPdf = qf(q, kPrevPdf, i);
+    p.etaScale = qf(q, kEtaScale, i);
+    p.active = 1;
+    return p;
+}
+
+__device__ void storePath(const PathQueue& q, uint32_t i, const PathState& p) {
+    sf(q, kOx, i, p.ray.origin.x);
+    sf(q, kOy, i, p.ray.origin.y);
+    sf(q, kOz, i, p.ray.origin.z);
+    sf(q, kDx, i, p.ray.direction.x);
+    sf(q, kDy, i, p.ray.direction.y);
+    sf(q, kDz, i, p.ray.direction.z);
+    sf(q, kTx, i, p.throughput.x);
+    sf(q, kTy, i, p.throughput.y);
+    sf(q, kTz, i, p.throughput.z);
+    sf(q, kRx, i, p.radiance.x);
+    sf(q, kRy, i, p.radiance.y);
+    sf(q, kRz, i, p.radiance.z);
+    su(q, kPixel, i, p.pixel);
+    su(q, kDepthFlags, i, (p.depth & 0x7fffffffu) | (p.specularBounce ? 0x80000000u : 0u));
+    su(q, kRngLo, i, static_cast<uint32_t>(p.rng));
+    su(q, kRngHi, i, static_cast<uint32_t>(p.rng >> 32));
+    su(q, kMedium, i, p.mediumMaterial);
+    sf(q, kMediumDensity, i, p.mediumDensity);
+    sf(q, kPrevPdf, i, p.previousBsdfPdf);
+    sf(q, kEtaScale, i, p.etaScale);
+}
+
+// Tile-local index -> pixel.  Rows are walked in strips of 8 and each strip
+// column-major, so a wave's 64 consecutive paths cover an 8x8 pixel block.
+__device__ __forceinline__ void tilePixel(const RenderTile& tile, uint32_t i, uint32_t& x, uint32_t& y) {
+    uint32_t stripPixels = 8u * tile.width;
+    uint32_t strip = i / stripPixels;
+    uint32_t j = i - strip * stripPixels;
+    uint32_t rows = tile.height - strip * 8u;
+    rows = rows < 8u ? rows : 8u;
+    x = tile.x + j / rows;
+    y = tile.y + strip * 8u + j % rows;
+}
+
+__device__ __forceinline__ void accumulate(float* fb, uint32_t pixel, const Vec3& v) {
+    float* p = fb + 3u * static_cast<size_t>(pixel);
+    p[0] += v.x;
+    p[1] += v.y;
+    p[2] += v.z;
+}
+
+__global__ void kPrimary(LaunchContext ctx) {
+    RenderTile tile = *ctx.tile;
+    uint32_t pixels = tile.width * tile.height;
+    uint32_t count = pixels < ctx.queue[0].capacity ? pixels : ctx.queue[0].capacity;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) ctx.counters->count[0] = count;
+    if (i >= count) return;
+    uint32_t x, y;
+    tilePixel(tile, i, x, y);
+    PathState p = primaryPath(ctx.camera, x, y, ctx.width, ctx.height, ctx.counters->sampleIndex, ctx.shade.rngStrategy);
+    storePath(ctx.queue[0], i, p);
+}
+
+__global__ void kReset(LaunchContext ctx, uint32_t nextSel, uint32_t flagCount) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctx.counters->count[nextSel] = 0;
+        ctx.counters->ticket = 0;
+    }
+    for (uint32_t j = i; j < flagCount; j += gridDim.x * blockDim.x) ctx.chunkFlags[j] = 0;
+}
+
+__global__ void kTrace(LaunchContext ctx, uint32_t bounce, uint32_t sel) {
+    uint32_t count = ctx.counters->count[sel];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        atomicAdd(&ctx.counters->closestQueries, static_cast<unsigned long long>(count));
+        if (ctx.profileQueues && bounce < 64) ctx.counters->queueCounts[bounce] = count;
+    }
+    if (i >= count) return;
+    Ray ray = loadRay(ctx.queue[sel], i);
+    PrivateStack stack;
+    HitRecord h = traceClosest(ctx.scene, ray, stack);
+    ctx.hits.t[i] = h.t;
+    ctx.hits.u[i] = h.u;
+    ctx.hits.v[i] = h.v;
+    ctx.hits.kind[i] = h.kind;
+    ctx.hits.instance[i] = h.instance;
+    ctx.hits.prim[i] = h.prim;
+    ctx.hits.material[i] = h.material;
+}
+
+// Shades one chunk of the queue and appends survivors in order.
+__device__ void shadeChunk(const LaunchContext& ctx, uint32_t sel, uint32_t count, uint32_t chunk, bool fused,
+    CompactShared& sh) {
+    const PathQueue& in = ctx.queue[sel];
+    const PathQueue& out = ctx.queue[sel ^ 1u];
+    uint32_t i = chunk * blockDim.x + threadIdx.x;
+    bool alive = false;
+    PathState path;
+    uint32_t shadow = 0;
+    if (i < count) {
+        path = loadPath(in, i);
+        PrivateStack stack;
+        HitRecord h;
+        if (fused) {
+            h = traceClosest(ctx.scene, path.ray, stack);
+        } else {
+            h.t = ctx.hits.t[i];
+            h.u = ctx.hits.u[i];
+            h.v = ctx.hits.v[i];
+            h.kind = ctx.hits.kind[i];
+            h.instance = ctx.hits.instance[i];
+            h.prim = ctx.hits.prim[i];
+            h.material = ctx.hits.material[i];
+        }
+        Vec3 done;
+        alive = shadePath(ctx.scene, path, h, ctx.shade, stack, done, shadow);
+        if (!alive) accumulate(ctx.framebuffer, path.pixel, done);
+    }
+    if (threadIdx.x == 0) sh.shadow = 0;
+    __syncthreads();
+    if (shadow) atomicAdd(&sh.shadow, shadow);
+    uint32_t slot;
+    uint32_t inclusive = orderedSlots(alive, chunk, ctx.chunkFlags, sh, slot);
+    if (alive && slot < out.capacity) storePath(out, slot, path);
+    if (threadIdx.x == 0) {
+        if (sh.shadow) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(sh.shadow));
+        uint32_t chunks = (count + blockDim.x - 1) / blockDim.x;
+        if (chunk + 1 == chunks) storeAgent(&ctx.counters->count[sel ^ 1u], inclusive);
+    }
+}
+
+__global__ void kShade(LaunchContext ctx, uint32_t sel) {
+    __shared__ CompactShared sh;
+    uint32_t count = ctx.counters->count[sel];
+    if (threadIdx.x == 0) sh.chunk = atomicAdd(&ctx.counters->ticket, 1u);
+    __syncthreads();
+    uint32_t chunk = sh.chunk;
+    if (chunk * blockDim.x >= count) return;
+    shadeChunk(ctx, sel, count, chunk, false, sh);
+}
+
+__global__ void kAdvance(LaunchContext ctx) {
+    if (ctx.tile->advanceSample) ++ctx.counters->sampleIndex;
+}
+
+__global__ void kPersistent(LaunchContext ctx) {
+    __shared__ CompactShared sh;
+    cg::grid_group grid = cg::this_grid();
+    RenderTile tile = *ctx.tile;
+    uint32_t pixels = tile.width * tile.height;
+    uint32_t capacity = ctx.queue[0].capacity;
+    uint32_t count0 = pixels < capacity ? pixels : capacity;
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t sample = loadAgent(&ctx.counters->sampleIndex);
+    for (uint32_t i = gid; i < count0; i += stride) {
+        uint32_t x, y;
+        tilePixel(tile, i, x, y);
+        storePath(ctx.queue[0], i, primaryPath(ctx.camera, x, y, ctx.width, ctx.height, sample, ctx.shade.rngStrategy));
+    }
+    if (gid == 0) storeAgent(&ctx.counters->count[0], count0);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    grid.sync();
+    uint32_t sel = 0;
+    for (uint32_t bounce = 0; bounce < ctx.shade.maxDepth; ++bounce) {
+        uint32_t count = loadAgent(&ctx.counters->count[sel]);
+        if (count == 0) break;
+        uint32_t chunks = (count + blockDim.x - 1) / blockDim.x;
+        if (gid == 0) {
+            atomicAdd(&ctx.counters->closestQueries, static_cast<unsigned long long>(count));
+            if (ctx.profileQueues && bounce < 64) ctx.counters->queueCounts[bounce] = count;
+            storeAgent(&ctx.counters->count[sel ^ 1u], 0u);
+            storeAgent(&ctx.counters->ticket, 0u);
+        }
+        for (uint32_t j = gid; j < chunks; j += stride) storeAgent(&ctx.chunkFlags[j], 0u);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        grid.sync();
+        while (true) {
+            if (threadIdx.x == 0) sh.chunk = atomicAdd(&ctx.counters->ticket, 1u);
+            __syncthreads();
+            uint32_t chunk = sh.chunk;
+            if (chunk >= chunks) break;
+            shadeChunk(ctx, sel, count, chunk, true, sh);
+            __syncthreads();
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        grid.sync();
+        sel ^= 1u;
+    }
+}
+
+__global__ void kTraceClosestAbi(DevScene scene, const float* rays, uint32_t n, wpt_hit* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ray ray(Vec3(rays[i], rays[n + i], rays[2 * n + i]), Vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+    PrivateStack stack;
+    HitRecord h = traceClosest(scene, ray, stack);
+    wpt_hit r;
+    r.t = h.t;
+    r.u = h.u;
+    r.v = h.v;
+    r.kind = h.kind;
+    r.instance = h.instance;
+    r.primitive = h.prim;
+    r.material = h.material;
+    r.light_index = kInvalid;
+    if (h.kind != kHitNone) r.light_index = surfaceAttributes(scene, ray, h).lightIndex;
+    out[i] = r;
+}
+
+__global__ void kTraceOccludedAbi(DevScene scene, const float* rays, const float* tmax, uint32_t n, uint8_t* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ray ray(Vec3(rays[i], rays[n + i], rays[2 * n + i]), Vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+    PrivateStack stack;
+    out[i] = traceOccluded(scene, ray, tmax[i], stack) ? 1 : 0;
+}
+
+__global__ void kCompactReset(uint32_t* scratch, uint32_t words, uint32_t* count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count = 0;
+    for (uint32_t j = i; j < words; j += gridDim.x * blockDim.x) scratch[j] = 0;
+}
+
+__global__ void kCompactAbi(const uint32_t* items, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t capacity,
+    uint32_t* count, uint32_t* scratch) {
+    __shared__ CompactShared sh;
+    if (threadIdx.x == 0) sh.chunk = atomicAdd(&scratch[0], 1u);
+    __syncthreads();
+    uint32_t chunk = sh.chunk;
+    if (chunk * blockDim.x >= n) return;
+    uint32_t i = chunk * blockDim.x + threadIdx.x;
+    bool alive = i < n && flags[i] != 0;
+    uint32_t item = i < n ? items[i] : 0u;
+    uint32_t slot;
+    uint32_t inclusive = orderedSlots(alive, chunk, scratch + 1, sh, slot);
+    if (alive && slot < capacity) out[slot] = item;
+    uint32_t chunks = (n + blockDim.x - 1) / blockDim.x;
+    if (threadIdx.x == 0 && chunk + 1 == chunks) *count = inclusive;
+}
+
+inline uint32_t ceilDiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream) {
+    uint32_t capacity = ctx.queue[0].capacity;
+    uint32_t flagCount = ceilDiv(capacity, ctx.blockSize);
+    uint32_t resetBlocks = ceilDiv(flagCount, 256u);
+    hipLaunchKernelGGL(kPrimary, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx);
+    for (uint32_t bounce = 0; bounce < ctx.shade.maxDepth; ++bounce) {
+        uint32_t sel = bounce & 1u;
+        hipLaunchKernelGGL(kReset, dim3(resetBlocks), dim3(256), 0, stream, ctx, sel ^ 1u, flagCount);
+        hipLaunchKernelGGL(kTrace, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx, bounce, sel);
+        hipLaunchKernelGGL(kShade, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel);
+    }
+    hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
+    return hipGetLastError();
+}
+
+hipError_t persistentOccupancy(uint32_t blockSize, int* blocksPerCu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kPersistent, static_cast<int>(blockSize), 0);
+}
+
+hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream) {
+    LaunchContext local = ctx;
+    void* args[] = {&local};
+    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kPersistent), dim3(ctx.persistentBlocks),
+        dim3(ctx.blockSize), args, 0, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
+    return hipGetLastError();
+}
+
+hipError_t launchTraceClosest(const DevScene& scene, const float* rays6, uint32_t n, void* hitsOut,
+    hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kTraceClosestAbi, dim3(ceilDiv(n, 128u)), dim3(128), 0, stream, scene, rays6, n,
+        static_cast<wpt_hit*>(hitsOut));
+    return hipGetLastError();
+}
+
+hipError_t launchTraceOccluded(const DevScene& scene, const float* rays6, const float* tmax, uint32_t n,
+    uint8_t* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kTraceOccludedAbi, dim3(ceilDiv(n, 128u)), dim3(128), 0, stream, scene, rays6, tmax, n, out);
+    return hipGetLastError();
+}
+
+uint32_t compactScratchWords(uint32_t n) { return 1u + ceilDiv(n > 0 ? n : 1u, 256u); }
+
+hipError_t launchCompact(const uint32_t* items, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t capacity,
+    uint32_t* count, uint32_t* scratch, uint32_t scratchWords, hipStream_t stream) {
+    hipLaunchKernelGGL(kCompactReset, dim3(ceilDiv(scratchWords, 256u)), dim3(256), 0, stream, scratch, scratchWords,
+        count);
+    if (n == 0) return hipGetLastError();
+    hipLaunchKernelGGL(kCompactAbi, dim3(ceilDiv(n, 256u)), dim3(256), 0, stream, items, flags, n, out, capacity,
+        count, scratch);
+    return hipGetLastError();
+}
+
+}  // namespace wpt

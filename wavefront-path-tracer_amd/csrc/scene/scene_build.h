@@ -1,0 +1,78 @@
+// Host-side scene build: BVH2 construction, AccelerationMode partition and the
+// triangle-light table, producing the arrays that are uploaded to HBM.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/wpt.h"
+#include "device_scene.h"
+
+namespace wpt {
+
+struct Aabb {
+    float mn[3];
+    float mx[3];
+};
+
+// A BVH2 with single-primitive leaves in the 64 B child-bounds layout.
+// order[slot] is the primitive stored at leaf slot `slot`.
+struct HostBvh {
+    std::vector<BvhNode> nodes;
+    std::vector<uint32_t> order;
+    std::vector<Aabb> primBounds;  // original primitive bounds (for export)
+    uint32_t root = 0;
+    Aabb rootBounds{};
+    uint32_t depth = 0;  // longest root-to-leaf edge count
+    uint32_t primCount() const { return static_cast<uint32_t>(order.size()); }
+};
+
+// Binned-SAH build over primitive bounds.  The node numbering is fixed by the
+// primitive partition alone (a subtree over n primitives owns n-1 consecutive
+// interior nodes), so the result is identical for any thread count.
+HostBvh buildBvh2(const std::vector<Aabb>& primBounds, unsigned threads = 0);
+
+// Export in the reference node convention (NXB::BVH2: own bounds per node,
+// leaf iff left == 0xffffffff, root = count - 1).
+std::vector<wpt_bvh2_node> exportBvh2(const HostBvh& bvh);
+
+Aabb triangleBounds(const Vec3& a, const Vec3& b, const Vec3& c);
+
+struct HostMesh {
+    std::vector<TriShade> triangles;  // mesh-local, original order
+    HostBvh bvh;                      // empty unless an instance uses AccelerationMode::Bvh
+    std::vector<float4> geometry;     // slot order
+};
+
+// Everything uploadScene (reference src/scene/scene.cu:336-389) resolves on
+// the host, in device layout.
+struct HostSceneData {
+    std::vector<float4> spheres;
+    std::vector<uint32_t> sphereMaterial;
+    std::vector<uint32_t> directSpheres;
+    std::vector<uint32_t> directInstances;
+    std::vector<uint32_t> tlasLeavesOriginal;  // reference SceneLeaf order
+    std::vector<uint32_t> tlasLeavesSlot;      // BVH slot order
+    HostBvh tlas;
+    std::vector<Instance> instances;
+    std::vector<HostMesh> meshes;
+    std::vector<TriShade> staticTriangles;
+    std::vector<TriShade> flatTriangles;
+    HostBvh flat;
+    std::vector<float4> flatGeometry;
+    std::vector<Material> materials;
+    std::vector<uint8_t> materialCutout;
+    std::vector<TriangleLight> lights;
+    std::vector<LightAlias> lightAlias;
+    std::vector<float> lightWeights;
+    std::vector<uint32_t> meshLightIndex;  // concatenated mesh triangles
+    uint32_t blackBackground = 0;
+    uint32_t ignoreDirectLightOcclusion = 0;
+    uint32_t maxStackDepth = 0;  // worst-case traversal stack use
+};
+
+// Returns an empty string on success, else the validation error.
+std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsigned threads = 0);
+
+}  // namespace wpt

@@ -1,0 +1,40 @@
+// Scene assembly for presets and procedural scenes (reference
+// src/scene/scene_host.cpp, src/scene/scene_factory.cpp, src/scene/camera.cu).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/wpt.h"
+#include "config.hpp"
+
+namespace wpt {
+
+// Host-owned scene arrays behind a wpt_scene_desc (reference HostScene,
+// include/scene/scene_host.hpp:19-33).
+struct HostScene {
+    std::vector<wpt_sphere> spheres;
+    std::vector<wpt_triangle> staticTriangles;
+    std::vector<wpt_triangle> flatTriangles;
+    std::vector<wpt_triangle> meshTriangles;
+    std::vector<wpt_mesh> meshes;
+    std::vector<wpt_instance> instances;
+    std::vector<wpt_material> materials;
+    bool blackBackground = false;
+    bool ignoreDirectLightOcclusion = false;
+
+    wpt_scene_desc desc() const;
+};
+
+wpt_material makeMaterial(uint32_t type, float r, float g, float b, float er = 0.0f, float eg = 0.0f, float eb = 0.0f);
+wpt_triangle makeTriangle(const float v0[3], const float v1[3], const float v2[3], uint32_t material);
+wpt_transform makeScaledTransform(float tx, float ty, float tz, float scale);
+
+// Displaced geodesic icosphere: 20 * frequency^2 triangles, smooth normals.
+std::vector<wpt_triangle> displacedIcosphere(uint32_t frequency, uint32_t seed, float amplitude);
+
+// Returns false with *error set for presets that need external assets.
+bool createSceneFromConfig(const SceneConfig& config, HostScene& out, std::string* error);
+wpt_camera createSceneCamera(const SceneConfig& config, uint32_t width, uint32_t height);
+
+}  // namespace wpt

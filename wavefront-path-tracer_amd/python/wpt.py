@@ -1,0 +1,508 @@
+"""ctypes binding of libwpt.so (include/wpt.h, include/wpt_host.h).
+
+Used by the tests, the benchmark and the graft entry points.  Device buffers
+for the traversal/compaction entry points come from PyTorch tensors; PyTorch
+is plumbing only, all rendering work runs in the gfx950 kernels of libwpt.so.
+The library must exist (built by ``make`` in this package); nothing here falls
+back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+PACKAGE_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PACKAGE_DIR, "lib", "libwpt.so")
+
+OK = 0
+ACCEL_BVH, ACCEL_DIRECT = 0, 1
+DIFFUSE, METAL, DIELECTRIC, EMISSIVE, SPECULAR_GLOSSINESS = 0, 1, 2, 3, 4
+RNG_XORSHIFT32, RNG_PCG32 = 0, 1
+MODE_GRAPH, MODE_PERSISTENT = 0, 1
+HIT_NONE, HIT_SPHERE, HIT_STATIC, HIT_INSTANCE, HIT_FLAT = 0, 1, 2, 3, 4
+INVALID = 0xFFFFFFFF
+TRI_HAS_NORMALS, TRI_HAS_TEXCOORDS = 1, 2
+
+F3 = C.c_float * 3
+F2 = C.c_float * 2
+
+
+class Sphere(C.Structure):
+    _fields_ = [("center", F3), ("radius", C.c_float), ("material", C.c_uint32), ("acceleration", C.c_uint32)]
+
+
+class Triangle(C.Structure):
+    _fields_ = [("v0", F3), ("v1", F3), ("v2", F3), ("n0", F3), ("n1", F3), ("n2", F3),
+                ("uv0", F2), ("uv1", F2), ("uv2", F2), ("tangent", F3), ("bitangent", F3),
+                ("material", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("first_triangle", C.c_uint32), ("triangle_count", C.c_uint32)]
+
+
+class Transform(C.Structure):
+    _fields_ = [("local_to_world", F3 * 3), ("world_to_local", F3 * 3), ("translation", F3)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("mesh", C.c_uint32), ("material_override", C.c_uint32), ("acceleration", C.c_uint32),
+                ("reserved", C.c_uint32), ("transform", Transform)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("albedo", F3), ("emission", F3), ("roughness", C.c_float),
+                ("ior", C.c_float), ("metallic", C.c_float), ("opacity", C.c_float),
+                ("alpha_cutoff", C.c_float), ("alpha_masked", C.c_uint32), ("attenuation_color", F3),
+                ("attenuation_distance", C.c_float), ("volume_density", C.c_float), ("specular", F3),
+                ("glossiness", C.c_float)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("spheres", C.POINTER(Sphere)), ("sphere_count", C.c_uint32),
+                ("static_triangles", C.POINTER(Triangle)), ("static_triangle_count", C.c_uint32),
+                ("flat_triangles", C.POINTER(Triangle)), ("flat_triangle_count", C.c_uint32),
+                ("mesh_triangles", C.POINTER(Triangle)), ("mesh_triangle_count", C.c_uint32),
+                ("meshes", C.POINTER(Mesh)), ("mesh_count", C.c_uint32),
+                ("instances", C.POINTER(Instance)), ("instance_count", C.c_uint32),
+                ("materials", C.POINTER(Material)), ("material_count", C.c_uint32),
+                ("black_background", C.c_uint32), ("ignore_direct_light_occlusion", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", F3), ("forward", F3), ("right", F3), ("up", F3),
+                ("vertical_fov", C.c_float), ("aspect_ratio", C.c_float)]
+
+
+class RenderConfig(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("tiled", C.c_uint32),
+                ("tile_width", C.c_uint32), ("tile_height", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_start_depth", C.c_uint32), ("samples_per_pixel", C.c_uint32),
+                ("block_size", C.c_uint32), ("rng_strategy", C.c_uint32), ("indirect_clamp", C.c_float),
+                ("mode", C.c_uint32), ("persistent_blocks", C.c_uint32), ("intersection_debug", C.c_uint32),
+                ("shading_normal_debug", C.c_uint32), ("profile_queues", C.c_uint32)]
+
+
+class Task(C.Structure):
+    _fields_ = [("job_fingerprint", C.c_uint64), ("tile_ordinal", C.c_uint32), ("sample_start", C.c_uint32),
+                ("sample_count", C.c_uint32), ("x", C.c_uint32), ("y", C.c_uint32),
+                ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("kind", C.c_uint32),
+                ("instance", C.c_uint32), ("primitive", C.c_uint32), ("material", C.c_uint32),
+                ("light_index", C.c_uint32)]
+
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("kind", "<u4"), ("instance", "<u4"),
+                      ("primitive", "<u4"), ("material", "<u4"), ("light_index", "<u4")])
+
+
+class Stats(C.Structure):
+    _fields_ = [("closest_queries", C.c_uint64), ("shadow_queries", C.c_uint64), ("samples", C.c_uint64),
+                ("trace_milliseconds", C.c_double), ("queue_counts", C.c_uint32 * 64),
+                ("queue_count_depth", C.c_uint32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [(name, C.c_uint32) for name in (
+        "sphere_count", "direct_sphere_count", "instance_count", "direct_instance_count", "tlas_leaf_count",
+        "tlas_node_count", "blas_count", "light_count", "static_triangle_count", "flat_triangle_count",
+        "flat_node_count", "material_count")]
+
+
+class Bvh2Node(C.Structure):
+    _fields_ = [("bmin", F3), ("bmax", F3), ("left", C.c_uint32), ("right", C.c_uint32)]
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Loads libwpt.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {PACKAGE_DIR}`")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    u32, u64 = C.c_uint32, C.c_uint64
+    sigs = {
+        "wpt_abi_version": ([], C.c_int),
+        "wpt_last_error": ([], C.c_char_p),
+        "wpt_device_count": ([], C.c_int),
+        "wpt_scene_create": ([C.POINTER(SceneDesc), C.POINTER(P)], C.c_int),
+        "wpt_scene_create_host": ([C.POINTER(SceneDesc), C.POINTER(P)], C.c_int),
+        "wpt_scene_destroy": ([P], C.c_int),
+        "wpt_scene_info_get": ([P, C.POINTER(SceneInfo)], C.c_int),
+        "wpt_scene_get_list": ([P, u32, P, u32, C.POINTER(u32)], C.c_int),
+        "wpt_scene_export_bvh2": ([P, u32, P, u32, C.POINTER(u32)], C.c_int),
+        "wpt_scene_export_lights": ([P, P, P, u32, C.POINTER(u32)], C.c_int),
+        "wpt_scene_export_light_indices": ([P, P, P, P], C.c_int),
+        "wpt_trace_closest": ([P, P, u32, P, P], C.c_int),
+        "wpt_trace_occluded": ([P, P, P, u32, P, P], C.c_int),
+        "wpt_compact": ([P, P, u32, P, u32, P, P], C.c_int),
+        "wpt_renderer_create": ([P, C.POINTER(Camera), C.POINTER(RenderConfig), u64, C.POINTER(P)], C.c_int),
+        "wpt_renderer_destroy": ([P], C.c_int),
+        "wpt_renderer_reset": ([P], C.c_int),
+        "wpt_renderer_render_samples": ([P, u32, u32], C.c_int),
+        "wpt_renderer_read_framebuffer": ([P, P, C.POINTER(u32)], C.c_int),
+        "wpt_render_frame": ([P, P, C.POINTER(u32)], C.c_int),
+        "wpt_render_task": ([P, C.POINTER(Task), P], C.c_int),
+        "wpt_renderer_last_stats": ([P, C.POINTER(Stats)], C.c_int),
+        "wpt_host_scene_create": ([C.c_char_p, u32, u32, C.POINTER(P)], C.c_int),
+        "wpt_host_scene_get_desc": ([P, C.POINTER(SceneDesc)], C.c_int),
+        "wpt_host_scene_get_camera": ([P, u32, u32, C.POINTER(Camera)], C.c_int),
+        "wpt_host_scene_destroy": ([P], C.c_int),
+        "wpt_host_parse_command_line": ([C.c_int, C.POINTER(C.c_char_p), C.POINTER(RenderConfig), C.c_char_p, u32,
+                                         C.POINTER(u64)], C.c_int),
+        "wpt_host_create_tasks": ([u32, u32, u32, u32, u32, u32, u64, P, u32], u32),
+        "wpt_host_task_key": ([u64, C.POINTER(Task)], u64),
+        "wpt_host_sha256": ([P, C.c_size_t, C.c_char_p], None),
+        "wpt_host_main": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+class WptError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        msg = library().wpt_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed with status {status}: {msg}")
+        self.status = status
+
+
+def check(status: int, where: str) -> None:
+    if status != OK:
+        raise WptError(status, where)
+
+
+def device_count() -> int:
+    return int(library().wpt_device_count())
+
+
+# ---------------------------------------------------------------- scene construction
+def vec3(v) -> F3:
+    return F3(*[float(x) for x in v])
+
+
+def make_material(kind=DIFFUSE, albedo=(0.5, 0.5, 0.5), emission=(0, 0, 0), roughness=0.0, ior=1.0, metallic=0.0,
+                  opacity=1.0, alpha_cutoff=0.5, alpha_masked=False, attenuation_color=(1, 1, 1),
+                  attenuation_distance=0.0, volume_density=1.0, specular=(1, 1, 1), glossiness=1.0) -> Material:
+    return Material(kind, vec3(albedo), vec3(emission), roughness, ior, metallic, opacity, alpha_cutoff,
+                    1 if alpha_masked else 0, vec3(attenuation_color), attenuation_distance, volume_density,
+                    vec3(specular), glossiness)
+
+
+def make_triangle(v0, v1, v2, material, normals=None, uvs=None, tangent=(0, 0, 0), bitangent=(0, 0, 0)) -> Triangle:
+    t = Triangle()
+    t.v0, t.v1, t.v2 = vec3(v0), vec3(v1), vec3(v2)
+    flags = 0
+    if normals is not None:
+        t.n0, t.n1, t.n2 = vec3(normals[0]), vec3(normals[1]), vec3(normals[2])
+        flags |= TRI_HAS_NORMALS
+    if uvs is not None:
+        t.uv0, t.uv1, t.uv2 = F2(*uvs[0]), F2(*uvs[1]), F2(*uvs[2])
+        flags |= TRI_HAS_TEXCOORDS
+    t.tangent, t.bitangent = vec3(tangent), vec3(bitangent)
+    t.material = material
+    t.flags = flags
+    return t
+
+
+def scaled_transform(translation=(0, 0, 0), scale=1.0, axis_scales=None) -> Transform:
+    sx, sy, sz = axis_scales if axis_scales is not None else (scale, scale, scale)
+    t = Transform()
+    for i, s in enumerate((sx, sy, sz)):
+        t.local_to_world[i][i] = s
+        t.world_to_local[i][i] = np.float32(1.0) / np.float32(s)
+    t.translation = vec3(translation)
+    return t
+
+
+class SceneBuilder:
+    """Owns the arrays behind a wpt_scene_desc assembled from Python."""
+
+    def __init__(self):
+        self.spheres: List[Sphere] = []
+        self.static_triangles: List[Triangle] = []
+        self.flat_triangles: List[Triangle] = []
+        self.mesh_triangles: List[Triangle] = []
+        self.meshes: List[Mesh] = []
+        self.instances: List[Instance] = []
+        self.materials: List[Material] = []
+        self.black_background = False
+        self.ignore_direct_light_occlusion = False
+        self._keep = []
+
+    def sphere(self, center, radius, material, acceleration=ACCEL_BVH):
+        self.spheres.append(Sphere(vec3(center), radius, material, acceleration))
+        return len(self.spheres) - 1
+
+    def mesh(self, triangles: Sequence[Triangle]) -> int:
+        self.meshes.append(Mesh(len(self.mesh_triangles), len(triangles)))
+        self.mesh_triangles.extend(triangles)
+        return len(self.meshes) - 1
+
+    def instance(self, mesh, transform=None, material_override=INVALID, acceleration=ACCEL_BVH):
+        self.instances.append(Instance(mesh, material_override, acceleration, 0, transform or scaled_transform()))
+        return len(self.instances) - 1
+
+    def material(self, m: Material) -> int:
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def _array(self, ctype, items):
+        if not items:
+            return None, 0
+        arr = (ctype * len(items))(*items)
+        self._keep.append(arr)
+        return C.cast(arr, C.POINTER(ctype)), len(items)
+
+    def desc(self) -> SceneDesc:
+        d = SceneDesc()
+        d.spheres, d.sphere_count = self._array(Sphere, self.spheres)
+        d.static_triangles, d.static_triangle_count = self._array(Triangle, self.static_triangles)
+        d.flat_triangles, d.flat_triangle_count = self._array(Triangle, self.flat_triangles)
+        d.mesh_triangles, d.mesh_triangle_count = self._array(Triangle, self.mesh_triangles)
+        d.meshes, d.mesh_count = self._array(Mesh, self.meshes)
+        d.instances, d.instance_count = self._array(Instance, self.instances)
+        d.materials, d.material_count = self._array(Material, self.materials)
+        d.black_background = 1 if self.black_background else 0
+        d.ignore_direct_light_occlusion = 1 if self.ignore_direct_light_occlusion else 0
+        self._desc = d
+        return d
+
+
+class HostScene:
+    """A preset or manifest scene assembled by the C++ host layer."""
+
+    def __init__(self, name: str, seed: int = 0, icosphere_frequency: int = 0):
+        lib = library()
+        self._h = C.c_void_p()
+        check(lib.wpt_host_scene_create(name.encode(), seed, icosphere_frequency, C.byref(self._h)),
+              f"wpt_host_scene_create({name})")
+        self._desc = SceneDesc()
+        check(lib.wpt_host_scene_get_desc(self._h, C.byref(self._desc)), "wpt_host_scene_get_desc")
+
+    def desc(self) -> SceneDesc:
+        return self._desc
+
+    def camera(self, width: int, height: int) -> Camera:
+        cam = Camera()
+        check(library().wpt_host_scene_get_camera(self._h, width, height, C.byref(cam)), "wpt_host_scene_get_camera")
+        return cam
+
+    def close(self):
+        if self._h:
+            library().wpt_host_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """A scene handle: device resident (default) or host-only (inspection)."""
+
+    def __init__(self, desc: SceneDesc, device: bool = True, owner=None):
+        lib = library()
+        self._owner = owner  # keeps the description's arrays alive
+        self._desc = desc
+        self._h = C.c_void_p()
+        fn = lib.wpt_scene_create if device else lib.wpt_scene_create_host
+        check(fn(C.byref(desc), C.byref(self._h)), "wpt_scene_create" if device else "wpt_scene_create_host")
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> SceneInfo:
+        info = SceneInfo()
+        check(library().wpt_scene_info_get(self._h, C.byref(info)), "wpt_scene_info_get")
+        return info
+
+    def list(self, which: int) -> np.ndarray:
+        n = C.c_uint32()
+        check(library().wpt_scene_get_list(self._h, which, None, 0, C.byref(n)), "wpt_scene_get_list")
+        out = np.zeros(n.value, dtype=np.uint32)
+        check(library().wpt_scene_get_list(self._h, which, out.ctypes.data, n.value, C.byref(n)), "wpt_scene_get_list")
+        return out
+
+    def export_bvh2(self, which: int) -> np.ndarray:
+        n = C.c_uint32()
+        check(library().wpt_scene_export_bvh2(self._h, which, None, 0, C.byref(n)), "wpt_scene_export_bvh2")
+        out = (Bvh2Node * max(1, n.value))()
+        check(library().wpt_scene_export_bvh2(self._h, which, C.cast(out, C.c_void_p), n.value, C.byref(n)),
+              "wpt_scene_export_bvh2")
+        return out, n.value
+
+    def lights(self):
+        n = C.c_uint32()
+        check(library().wpt_scene_export_lights(self._h, None, None, 0, C.byref(n)), "wpt_scene_export_lights")
+        lights = np.zeros((n.value, 16), dtype=np.float32)
+        alias = np.zeros((n.value, 2), dtype=np.float32)
+        check(library().wpt_scene_export_lights(self._h, lights.ctypes.data, alias.ctypes.data, n.value, C.byref(n)),
+              "wpt_scene_export_lights")
+        return lights, alias
+
+    def light_indices(self):
+        d = self._desc
+        inst = np.zeros(d.instance_count, dtype=np.uint32)
+        static = np.zeros(d.static_triangle_count, dtype=np.uint32)
+        mesh = np.zeros(d.mesh_triangle_count, dtype=np.uint32)
+        check(library().wpt_scene_export_light_indices(self._h, inst.ctypes.data, static.ctypes.data,
+                                                        mesh.ctypes.data), "wpt_scene_export_light_indices")
+        return inst, static, mesh
+
+    def trace_closest(self, rays6, stream=None):
+        """rays6: float32 CUDA tensor (6, n). Returns a uint8 tensor of n wpt_hit records."""
+        import torch
+        n = rays6.shape[1]
+        hits = torch.empty((max(n, 1), HIT_DTYPE.itemsize), dtype=torch.uint8, device=rays6.device)
+        check(library().wpt_trace_closest(self._h, C.c_void_p(rays6.data_ptr()), n, C.c_void_p(hits.data_ptr()),
+                                          C.c_void_p(stream or 0)), "wpt_trace_closest")
+        return hits[:n]
+
+    def trace_occluded(self, rays6, tmax, stream=None):
+        import torch
+        n = rays6.shape[1]
+        out = torch.empty(max(n, 1), dtype=torch.uint8, device=rays6.device)
+        check(library().wpt_trace_occluded(self._h, C.c_void_p(rays6.data_ptr()), C.c_void_p(tmax.data_ptr()), n,
+                                           C.c_void_p(out.data_ptr()), C.c_void_p(stream or 0)), "wpt_trace_occluded")
+        return out[:n]
+
+    def close(self):
+        if self._h:
+            library().wpt_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hits_to_numpy(hits_u8) -> np.ndarray:
+    return hits_u8.cpu().numpy().view(HIT_DTYPE).reshape(-1)
+
+
+def compact(items, flags, capacity):
+    """Order-preserving compaction on the device (wpt_compact)."""
+    import torch
+    n = items.numel()
+    out = torch.full((max(capacity, 1),), 0xCDCDCDCD - (1 << 32), dtype=torch.int32, device=items.device)
+    count = torch.zeros(1, dtype=torch.int32, device=items.device)
+    check(library().wpt_compact(C.c_void_p(items.data_ptr()), C.c_void_p(flags.data_ptr()), n,
+                                C.c_void_p(out.data_ptr()), capacity, C.c_void_p(count.data_ptr()), None),
+          "wpt_compact")
+    return out, int(count.item())
+
+
+def render_config(width, height, spp=1, max_depth=8, rr_start=3, block_size=256, rng=RNG_PCG32, clamp=10.0,
+                  mode=MODE_GRAPH, tile=None, persistent_blocks=0, intersection_debug=False,
+                  profile_queues=True) -> RenderConfig:
+    rc = RenderConfig()
+    rc.width, rc.height = width, height
+    rc.tiled = 1 if tile else 0
+    rc.tile_width, rc.tile_height = tile if tile else (width, height)
+    rc.max_depth, rc.rr_start_depth, rc.samples_per_pixel = max_depth, rr_start, spp
+    rc.block_size, rc.rng_strategy, rc.indirect_clamp, rc.mode = block_size, rng, clamp, mode
+    rc.persistent_blocks = persistent_blocks
+    rc.intersection_debug = 1 if intersection_debug else 0
+    rc.shading_normal_debug = 0
+    rc.profile_queues = 1 if profile_queues else 0
+    return rc
+
+
+class Renderer:
+    def __init__(self, scene: Scene, camera: Camera, config: RenderConfig, fingerprint: int = 0):
+        self.scene, self.camera, self.config = scene, camera, config
+        self.fingerprint = fingerprint
+        self._h = C.c_void_p()
+        check(library().wpt_renderer_create(scene.handle, C.byref(camera), C.byref(config), fingerprint,
+                                            C.byref(self._h)), "wpt_renderer_create")
+
+    def reset(self):
+        check(library().wpt_renderer_reset(self._h), "wpt_renderer_reset")
+
+    def render_samples(self, first: int, count: int):
+        check(library().wpt_renderer_render_samples(self._h, first, count), "wpt_renderer_render_samples")
+
+    def framebuffer(self):
+        c = self.config
+        out = np.zeros((c.height, c.width, 3), dtype=np.float32)
+        n = C.c_uint32()
+        check(library().wpt_renderer_read_framebuffer(self._h, out.ctypes.data, C.byref(n)),
+              "wpt_renderer_read_framebuffer")
+        return out, n.value
+
+    def render_frame(self):
+        c = self.config
+        out = np.zeros((c.height, c.width, 3), dtype=np.float32)
+        n = C.c_uint32()
+        check(library().wpt_render_frame(self._h, out.ctypes.data, C.byref(n)), "wpt_render_frame")
+        return out, n.value
+
+    def render_task(self, task: Task):
+        out = np.zeros((task.height, task.width, 3), dtype=np.float32)
+        check(library().wpt_render_task(self._h, C.byref(task), out.ctypes.data), "wpt_render_task")
+        return out
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(library().wpt_renderer_last_stats(self._h, C.byref(s)), "wpt_renderer_last_stats")
+        return s
+
+    def close(self):
+        if self._h:
+            library().wpt_renderer_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint) -> List[Task]:
+    lib = library()
+    n = lib.wpt_host_create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint, None, 0)
+    arr = (Task * max(1, n))()
+    lib.wpt_host_create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint, C.cast(arr, C.c_void_p), n)
+    return [arr[i] for i in range(n)]
+
+
+def task_key(fingerprint: int, task: Task) -> int:
+    return int(library().wpt_host_task_key(fingerprint, C.byref(task)))
+
+
+def parse_command_line(args: Iterable[str]):
+    argv = ["pathtracer"] + list(args)
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    rc = RenderConfig()
+    name = C.create_string_buffer(128)
+    fp = C.c_uint64()
+    status = library().wpt_host_parse_command_line(len(argv), arr, C.byref(rc), name, 128, C.byref(fp))
+    return status, rc, name.value.decode(), fp.value
+
+
+def sha256(buf: np.ndarray) -> str:
+    out = C.create_string_buffer(65)
+    library().wpt_host_sha256(buf.ctypes.data, buf.nbytes, out)
+    return out.value.decode()
