@@ -300,6 +300,12 @@ WPT_API int wpt_render_frame(wpt_renderer* renderer, float* host_rgb, uint32_t* 
  * renders the sample range, returns unnormalized tile sums (w*h*3). */
 WPT_API int wpt_render_task(wpt_renderer* renderer, const wpt_task* task, float* host_tile_rgb);
 WPT_API int wpt_renderer_last_stats(const wpt_renderer* renderer, wpt_stats* stats);
+/* Renders samples [first, first+count) with every kernel launch bracketed by
+ * HIP events on the renderer's stream (no graph), accumulating device time
+ * per kernel into kernel_ms[5] and launch counts into launches[5]: 0 primary
+ * rays, 1 queue reset, 2 closest-hit trace, 3 shade+compact, 4 persistent. */
+WPT_API int wpt_renderer_profile_kernels(wpt_renderer* renderer, uint32_t first_sample, uint32_t count,
+                                         double* kernel_ms, uint32_t* launches);
 
 #ifdef __cplusplus
 }

@@ -290,9 +290,22 @@ void buildTables(oracle_handle& h) {
     }
 }
 
+// ---------------------------------------------------------------- work counters
+// Per-query work of the reference algorithm (node bound tests, triangle and
+// sphere tests, instance entries), used for the roofline's byte model.
+struct WorkCounts {
+    uint64_t nodes = 0, triangles = 0, spheres = 0, instances = 0;
+};
+thread_local WorkCounts* tlWork = nullptr;
+inline void countNode() { if (tlWork) ++tlWork->nodes; }
+inline void countTriangle() { if (tlWork) ++tlWork->triangles; }
+inline void countSphere() { if (tlWork) ++tlWork->spheres; }
+inline void countInstance() { if (tlWork) ++tlWork->instances; }
+
 // ---------------------------------------------------------------- intersection tests
 template <class R>
 bool intersectSphere(const Ray<R>& ray, const wpt_sphere& s, R& t) {  // device_traversal.cuh:24-53
+    countSphere();
     V3<R> oc = ray.o - vec<R>(s.center);
     R a = dot(ray.d, ray.d);
     R b = R(2) * dot(oc, ray.d);
@@ -316,6 +329,7 @@ bool intersectSphere(const Ray<R>& ray, const wpt_sphere& s, R& t) {  // device_
 
 template <class R>
 bool intersectTriangle(const Ray<R>& ray, const wpt_triangle& tri, R& t, R& u, R& v) {  // :55-91
+    countTriangle();
     V3<R> v0 = vec<R>(tri.v0), v1 = vec<R>(tri.v1), v2 = vec<R>(tri.v2);
     V3<R> e1 = v1 - v0;
     V3<R> e2 = v2 - v0;
@@ -338,6 +352,7 @@ bool intersectTriangle(const Ray<R>& ray, const wpt_triangle& tri, R& t, R& u, R
 
 template <class R>
 bool intersectAabb(const Ray<R>& ray, const V3<R>& inv, const wpt_bvh2_node& n, R closestT, R& nearT) {  // :182-196
+    countNode();
     R tx0 = (R(n.bmin[0]) - ray.o.x) * inv.x;
     R tx1 = (R(n.bmax[0]) - ray.o.x) * inv.x;
     R ty0 = (R(n.bmin[1]) - ray.o.y) * inv.y;
@@ -485,6 +500,7 @@ Ray<R> localRay(const wpt_transform& t, const Ray<R>& ray) {
 // intersectMeshInstance, device_traversal.cuh:802-834.
 template <class R>
 void meshInstance(const oracle_handle& h, const Ray<R>& ray, uint32_t index, HitState<R>& hs, bool useBvh) {
+    countInstance();
     const wpt_instance& inst = h.d.instances[index];
     const wpt_mesh& mesh = h.d.meshes[inst.mesh];
     const wpt_triangle* tris = h.d.mesh_triangles + mesh.first_triangle;
@@ -863,6 +879,7 @@ V3<R> offsetRayOrigin(const V3<R>& p, const V3<R>& gn, const V3<R>& dir, R hitDi
 
 struct Counters {
     uint64_t closest = 0, shadow = 0;
+    WorkCounts closestWork, shadowWork;
 };
 
 // sampleDirectLight, device_traversal.cuh:529-569.
@@ -895,7 +912,10 @@ void sampleDirectLight(const oracle_handle& h, PathState<R>& path, const Hit<R>&
     Ray<R> shadow{so, seg / sd};
     if (!h.d.ignore_direct_light_occlusion) {
         ++cnt.shadow;
-        if (isOccluded(h, shadow, sd - R(1.0e-6f))) return;
+        tlWork = &cnt.shadowWork;
+        bool blocked = isOccluded(h, shadow, sd - R(1.0e-6f));
+        tlWork = nullptr;
+        if (blocked) return;
     }
     const wpt_material& lm = mat(h, L.material);
     R area = R(L.area), pdfSel = R(L.selectionPdf);
@@ -917,7 +937,9 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
         Hit<R> hit;
         HitState<R> hs;
         ++cnt.closest;
+        tlWork = &cnt.closestWork;
         bool didHit = traceRay(h, path.ray, hit, hs);
+        tlWork = nullptr;
         if (debug) {
             V3<R> color(R(0.05f), R(0.10f), R(0.25f));
             if (didHit) color = R(0.5f) * (hit.normal + V3<R>(R(1), R(1), R(1)));
@@ -1267,6 +1289,14 @@ int oracle_render(const oracle_handle* h, const wpt_camera* camera, const wpt_re
         for (const Counters& c : counters) {
             stats->closest_queries += c.closest;
             stats->shadow_queries += c.shadow;
+            stats->closest_node_tests += c.closestWork.nodes;
+            stats->closest_triangle_tests += c.closestWork.triangles;
+            stats->closest_sphere_tests += c.closestWork.spheres;
+            stats->closest_instance_entries += c.closestWork.instances;
+            stats->shadow_node_tests += c.shadowWork.nodes;
+            stats->shadow_triangle_tests += c.shadowWork.triangles;
+            stats->shadow_sphere_tests += c.shadowWork.spheres;
+            stats->shadow_instance_entries += c.shadowWork.instances;
         }
         stats->paths = static_cast<uint64_t>(w) * th * sc;
     }

@@ -35,6 +35,10 @@ typedef struct oracle_stats {
     uint64_t closest_queries;
     uint64_t shadow_queries;
     uint64_t paths;
+    /* Work of the reference algorithm, per query type: node bound tests,
+     * triangle tests, sphere tests and mesh-instance entries. */
+    uint64_t closest_node_tests, closest_triangle_tests, closest_sphere_tests, closest_instance_entries;
+    uint64_t shadow_node_tests, shadow_triangle_tests, shadow_sphere_tests, shadow_instance_entries;
 } oracle_stats;
 
 typedef struct oracle_handle oracle_handle;

@@ -27,7 +27,9 @@ class OracleScene(C.Structure):
 
 
 class OracleStats(C.Structure):
-    _fields_ = [("closest_queries", C.c_uint64), ("shadow_queries", C.c_uint64), ("paths", C.c_uint64)]
+    _fields_ = [("closest_queries", C.c_uint64), ("shadow_queries", C.c_uint64), ("paths", C.c_uint64)] + [
+        (f"{kind}_{what}", C.c_uint64) for kind in ("closest", "shadow")
+        for what in ("node_tests", "triangle_tests", "sphere_tests", "instance_entries")]
 
 
 _lib = None

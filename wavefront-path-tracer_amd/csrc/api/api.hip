@@ -617,6 +617,26 @@ WPT_API int wpt_render_task(wpt_renderer* r, const wpt_task* task, float* hostTi
     return WPT_OK;
 }
 
+WPT_API int wpt_renderer_profile_kernels(wpt_renderer* r, uint32_t first, uint32_t count, double* kernelMs,
+    uint32_t* launches) {
+    if (!r || !kernelMs || !launches) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    hipLaunchKernelGGL(kClearStats, dim3(1), dim3(1), 0, r->stream, r->ctx.counters);
+    hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, first);
+    WPT_HIP(hipGetLastError());
+    bool persistent = r->config.mode == WPT_MODE_PERSISTENT;
+    for (uint32_t s = 0; s < count; ++s) {
+        for (uint32_t t = 0; t < r->tiles.size(); ++t) {
+            RenderTile tile = r->tiles[t];
+            tile.advanceSample = t + 1 == r->tiles.size() ? 1u : 0u;
+            hipLaunchKernelGGL(kSetTile, dim3(1), dim3(1), 0, r->stream, const_cast<RenderTile*>(r->ctx.tile), tile);
+            WPT_HIP(hipGetLastError());
+            WPT_HIP(profileSample(r->ctx, r->stream, persistent, kernelMs, launches));
+        }
+    }
+    r->currentTile = 0xffffffffu;
+    return WPT_OK;
+}
+
 WPT_API int wpt_renderer_last_stats(const wpt_renderer* r, wpt_stats* stats) {
     if (!r || !stats) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
     WavefrontCounters c;

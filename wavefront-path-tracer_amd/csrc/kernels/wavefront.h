@@ -75,6 +75,11 @@ hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream);
 // Persistent cooperative kernel for one tile sample, then advance.
 hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream);
 hipError_t persistentOccupancy(uint32_t blockSize, int* blocksPerCu);
+// One sample of the current tile with every launch bracketed by events:
+// ms/launches accumulate per kernel (0 primary, 1 reset, 2 trace, 3 shade,
+// 4 persistent).  Synchronizes the stream.
+hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[5],
+    uint32_t launches[5]);
 
 // Standalone traversal for the C ABI.
 hipError_t launchTraceClosest(const DevScene& scene, const float* rays6, uint32_t n, void* hitsOut,

@@ -20,6 +20,8 @@
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../../include/wpt.h"
 #include "wavefront.h"
 
@@ -364,23 +366,87 @@ __global__ void kCompactAbi(const uint32_t* items, const uint8_t* flags, uint32_
     if (threadIdx.x == 0 && chunk + 1 == chunks) *count = inclusive;
 }
 
+}  // namespace
+
 inline uint32_t ceilDiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+namespace {
+
+// One staged sample; `timer` (optional) brackets every launch with events.
+template <class Timer>
+hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& timer) {
+    uint32_t capacity = ctx.queue[0].capacity;
+    uint32_t flagCount = ceilDiv(capacity, ctx.blockSize);
+    uint32_t resetBlocks = ceilDiv(flagCount, 256u);
+    hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx); });
+    for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {
+        uint32_t sel = bounce & 1u;
+        e = timer(1, [&] { hipLaunchKernelGGL(kReset, dim3(resetBlocks), dim3(256), 0, stream, ctx, sel ^ 1u, flagCount); });
+        if (e == hipSuccess)
+            e = timer(2, [&] { hipLaunchKernelGGL(kTrace, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx, bounce, sel); });
+        if (e == hipSuccess)
+            e = timer(3, [&] { hipLaunchKernelGGL(kShade, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel); });
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
+    return hipGetLastError();
+}
 
 }  // namespace
 
 hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream) {
-    uint32_t capacity = ctx.queue[0].capacity;
-    uint32_t flagCount = ceilDiv(capacity, ctx.blockSize);
-    uint32_t resetBlocks = ceilDiv(flagCount, 256u);
-    hipLaunchKernelGGL(kPrimary, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx);
-    for (uint32_t bounce = 0; bounce < ctx.shade.maxDepth; ++bounce) {
-        uint32_t sel = bounce & 1u;
-        hipLaunchKernelGGL(kReset, dim3(resetBlocks), dim3(256), 0, stream, ctx, sel ^ 1u, flagCount);
-        hipLaunchKernelGGL(kTrace, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx, bounce, sel);
-        hipLaunchKernelGGL(kShade, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel);
+    return stagedSample(ctx, stream, [](int, auto&& launch) {
+        launch();
+        return hipGetLastError();
+    });
+}
+
+hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[5],
+    uint32_t launches[5]) {
+    struct Span {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Span> spans;
+    auto timer = [&](int kind, auto&& launch) -> hipError_t {
+        Span sp{kind, nullptr, nullptr};
+        hipError_t e = hipEventCreate(&sp.a);
+        if (e == hipSuccess) e = hipEventCreate(&sp.b);
+        if (e != hipSuccess) return e;
+        spans.push_back(sp);
+        e = hipEventRecord(sp.a, stream);
+        if (e != hipSuccess) return e;
+        launch();
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return hipEventRecord(sp.b, stream);
+    };
+    hipError_t e;
+    if (persistent) {
+        e = timer(4, [&] {
+            LaunchContext local = ctx;
+            void* args[] = {&local};
+            (void)hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kPersistent), dim3(ctx.persistentBlocks),
+                dim3(ctx.blockSize), args, 0, stream);
+        });
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
+            e = hipGetLastError();
+        }
+    } else {
+        e = stagedSample(ctx, stream, timer);
     }
-    hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
-    return hipGetLastError();
+    hipError_t s = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = s;
+    for (Span& sp : spans) {
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, sp.a, sp.b);
+        ms[sp.kind] += t;
+        launches[sp.kind] += 1;
+        (void)hipEventDestroy(sp.a);
+        (void)hipEventDestroy(sp.b);
+    }
+    return e;
 }
 
 hipError_t persistentOccupancy(uint32_t blockSize, int* blocksPerCu) {

@@ -155,6 +155,7 @@ def library() -> C.CDLL:
         "wpt_render_frame": ([P, P, C.POINTER(u32)], C.c_int),
         "wpt_render_task": ([P, C.POINTER(Task), P], C.c_int),
         "wpt_renderer_last_stats": ([P, C.POINTER(Stats)], C.c_int),
+        "wpt_renderer_profile_kernels": ([P, u32, u32, P, P], C.c_int),
         "wpt_host_scene_create": ([C.c_char_p, u32, u32, C.POINTER(P)], C.c_int),
         "wpt_host_scene_get_desc": ([P, C.POINTER(SceneDesc)], C.c_int),
         "wpt_host_scene_get_camera": ([P, u32, u32, C.POINTER(Camera)], C.c_int),
@@ -462,6 +463,14 @@ class Renderer:
         out = np.zeros((task.height, task.width, 3), dtype=np.float32)
         check(library().wpt_render_task(self._h, C.byref(task), out.ctypes.data), "wpt_render_task")
         return out
+
+    def profile_kernels(self, first: int, count: int):
+        """Per-kernel device milliseconds and launch counts (event-bracketed)."""
+        ms = np.zeros(5, dtype=np.float64)
+        launches = np.zeros(5, dtype=np.uint32)
+        check(library().wpt_renderer_profile_kernels(self._h, first, count, ms.ctypes.data, launches.ctypes.data),
+              "wpt_renderer_profile_kernels")
+        return ms, launches
 
     def stats(self) -> Stats:
         s = Stats()
