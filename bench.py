@@ -133,7 +133,8 @@ def main():
     trace_launches = int(launches[2] if mode == wpt.MODE_GRAPH else launches[4])
     log(f"kernels over {args.profile_spp} spp: primary {kernel_ms[0]:.2f} ms, reset {kernel_ms[1]:.2f} ms, "
         f"trace {kernel_ms[2]:.2f} ms ({int(launches[2])} launches), shade {kernel_ms[3]:.2f} ms, "
-        f"persistent {kernel_ms[4]:.2f} ms")
+        f"persistent {kernel_ms[4]:.2f} ms, shadow {kernel_ms[5]:.2f} ms ({int(launches[5])} launches), "
+        f"compact {kernel_ms[6]:.2f} ms")
 
     persistent = None
     if not args.skip_persistent and mode == wpt.MODE_GRAPH:

@@ -167,7 +167,7 @@ typedef struct wpt_render_config {
     uint32_t max_depth;
     uint32_t rr_start_depth;
     uint32_t samples_per_pixel;
-    uint32_t block_size;
+    uint32_t block_size;            /* multiple of 64, at most 512 */
     uint32_t rng_strategy;          /* WPT_RNG_* */
     float indirect_clamp;           /* 0 disables */
     uint32_t mode;                  /* WPT_MODE_* */
@@ -302,8 +302,9 @@ WPT_API int wpt_render_task(wpt_renderer* renderer, const wpt_task* task, float*
 WPT_API int wpt_renderer_last_stats(const wpt_renderer* renderer, wpt_stats* stats);
 /* Renders samples [first, first+count) with every kernel launch bracketed by
  * HIP events on the renderer's stream (no graph), accumulating device time
- * per kernel into kernel_ms[5] and launch counts into launches[5]: 0 primary
- * rays, 1 queue reset, 2 closest-hit trace, 3 shade+compact, 4 persistent. */
+ * per kernel into kernel_ms[7] and launch counts into launches[7]: 0 primary
+ * rays, 1 queue reset, 2 closest-hit trace, 3 shade, 4 persistent, 5 shadow
+ * rays, 6 queue compaction. */
 WPT_API int wpt_renderer_profile_kernels(wpt_renderer* renderer, uint32_t first_sample, uint32_t count,
                                          double* kernel_ms, uint32_t* launches);
 

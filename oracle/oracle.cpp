@@ -1348,6 +1348,19 @@ uint64_t oracle_rng_stream(uint32_t pixel, uint32_t sample, uint32_t strategy, u
     return seeded;
 }
 
+void oracle_elementary(uint32_t op, const float* x, const float* y, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (op) {
+        case 0: out[i] = fsin(x[i]); break;
+        case 1: out[i] = fcos(x[i]); break;
+        case 2: out[i] = ftan(x[i]); break;
+        case 3: out[i] = fexp(x[i]); break;
+        case 4: out[i] = flog(x[i]); break;
+        default: out[i] = fpow(x[i], y[i]); break;
+        }
+    }
+}
+
 float oracle_dielectric_reflectance(float cosine, float eta) { return dielectricReflectance<float>(cosine, eta); }
 
 void oracle_eval_specgloss(const float f0[3], const float diffuse[3], float roughness, const float n[3], const float wo[3],

@@ -82,6 +82,10 @@ uint32_t oracle_reference_rays(uint32_t seed, uint32_t n_random, float* rays6, u
  * (pixel, sample) under a strategy, and the seeded state. */
 uint64_t oracle_rng_stream(uint32_t pixel, uint32_t sample, uint32_t strategy, uint32_t count, float* out);
 
+/* Elementary functions the render path uses (op: 0 sin, 1 cos, 2 tan, 3 exp,
+ * 4 log, 5 pow(x, y)), as computed by both the kernels and this oracle. */
+void oracle_elementary(uint32_t op, const float* x, const float* y, uint32_t n, float* out);
+
 /* Reference BSDF helpers for known-answer tests. */
 float oracle_dielectric_reflectance(float cosine, float eta);
 void oracle_eval_specgloss(const float f0[3], const float diffuse[3], float roughness, const float n[3],

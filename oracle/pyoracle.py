@@ -56,6 +56,7 @@ def library():
             ("oracle_reference_rays", [u32, u32, P, u32], u32),
             ("oracle_rng_stream", [u32, u32, u32, u32, P], C.c_uint64),
             ("oracle_dielectric_reflectance", [C.c_float, C.c_float], C.c_float),
+            ("oracle_elementary", [u32, P, P, u32, P], None),
             ("oracle_eval_specgloss", [P, P, C.c_float, P, P, P, P, P], None),
         ]:
             fn = getattr(lib, name)
@@ -76,6 +77,28 @@ def rng_stream(pixel, sample, strategy, count):
     out = np.zeros(count, dtype=np.float32)
     seeded = library().oracle_rng_stream(pixel, sample, strategy, count, out.ctypes.data)
     return int(seeded), out
+
+
+def elementary(op: int, x: np.ndarray, y: np.ndarray = None) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y if y is not None else np.zeros_like(x), dtype=np.float32)
+    out = np.zeros_like(x)
+    library().oracle_elementary(op, x.ctypes.data, y.ctypes.data, x.size, out.ctypes.data)
+    return out
+
+
+def dielectric_reflectance(cosine: float, eta: float) -> float:
+    return float(library().oracle_dielectric_reflectance(cosine, eta))
+
+
+def eval_specgloss(f0, diffuse, roughness, n, wo, wi):
+    arr = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    f0, diffuse, n, wo, wi = arr(f0), arr(diffuse), arr(n), arr(wo), arr(wi)
+    value = np.zeros(3, dtype=np.float32)
+    pdf = C.c_float()
+    library().oracle_eval_specgloss(f0.ctypes.data, diffuse.ctypes.data, roughness, n.ctypes.data, wo.ctypes.data,
+                                    wi.ctypes.data, value.ctypes.data, C.byref(pdf))
+    return value, pdf.value
 
 
 def compact(items: np.ndarray, flags: np.ndarray, capacity: int):

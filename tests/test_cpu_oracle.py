@@ -1,0 +1,135 @@
+"""CPU tests that pin the oracle: RNG golden vectors and seed uniqueness,
+elementary functions against libm, BSDF/Fresnel known answers from the
+reference's material tests, and the oracle's traversal against an
+independent FP64 brute force on the reference's acceleration-test scenes."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import scenes
+import wpt
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_rng_streams_match_golden(oracle_mod):
+    # tests/golden/make_rng_golden.py: published PCG32 / xorshift32 /
+    # SplitMix64 definitions with the reference's keys and float mapping.
+    data = json.load(open(os.path.join(GOLDEN, "rng_streams.json")))
+    for case in data["cases"]:
+        seeded, floats = oracle_mod.rng_stream(case["pixel"], case["sample"], case["strategy"], len(case["floats"]))
+        assert seeded == case["seeded_state"], case
+        assert np.array_equal(floats, np.array(case["floats"], dtype=np.float32)), case
+
+
+def test_rng_values_in_unit_interval(oracle_mod):
+    for strategy in (0, 1):
+        _, f = oracle_mod.rng_stream(77, 3, strategy, 4096)
+        assert f.min() >= 0.0 and f.max() < 1.0
+        assert abs(f.mean() - 0.5) < 0.02
+
+
+def test_pcg_seed_uniqueness():
+    # tests/rng_tests.cu:14-63: no seed collision over 64x64 pixels x 256 samples.
+    pixels = np.arange(64 * 64, dtype=np.uint64)
+    samples = np.arange(256, dtype=np.uint64)
+    keys = (samples[:, None] << np.uint64(32)) | pixels[None, :]
+    with np.errstate(over="ignore"):
+        z = keys.reshape(-1) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    assert np.unique(z).size == z.size
+
+
+def ulps(got, want):
+    want32 = want.astype(np.float32)
+    return np.abs(got.astype(np.float64) - want) / np.spacing(np.abs(want32)).astype(np.float64)
+
+
+@pytest.mark.parametrize("op,fn,lo,hi", [(0, np.sin, 0.0, 6.2831855), (1, np.cos, 0.0, 6.2831855),
+                                         (0, np.sin, -100.0, 100.0), (1, np.cos, -100.0, 100.0),
+                                         (2, np.tan, -1.5, 1.5), (3, np.exp, -87.0, 88.0), (4, np.log, 1e-30, 1e30)])
+def test_elementary_functions_correctly_rounded(oracle_mod, op, fn, lo, hi):
+    x = np.random.default_rng(op * 7 + int(hi)).uniform(lo, hi, 400_000).astype(np.float32)
+    if op == 4:
+        x = np.exp(np.random.default_rng(9).uniform(np.log(lo), np.log(hi), 400_000)).astype(np.float32)
+    got = oracle_mod.elementary(op, x)
+    assert ulps(got, fn(x.astype(np.float64))).max() <= 1.0
+
+
+def test_pow_correctly_rounded(oracle_mod):
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.0, 1.0, 200_000).astype(np.float32)
+    for y in (5.0, 2.2, 2.4):
+        got = oracle_mod.elementary(5, x, np.full_like(x, y))
+        want = np.power(x.astype(np.float64), np.float64(np.float32(y)))
+        nz = want > 1e-30
+        assert ulps(got[nz], want[nz]).max() <= 1.0
+    assert oracle_mod.elementary(5, np.array([0.0, 1.0], np.float32), np.array([5.0, 2.2], np.float32)).tolist() == [0.0, 1.0]
+
+
+def test_fresnel_known_answers(oracle_mod):
+    # tests/material_transport_tests.cu:118-128
+    f = oracle_mod.dielectric_reflectance
+    assert abs(f(1.0, np.float32(1 / 1.5)) - 0.04) <= 1e-6
+    assert abs(f(0.767, 1.5) - 0.23811942) <= 2e-6
+    assert f(0.7, 1.5) == 1.0
+    assert f(0.0, 1.0) == 0.0
+    cosine, eta = np.float32(0.8), np.float32(1 / 1.5)
+    transmitted = np.sqrt(np.float32(1) - eta * eta * (np.float32(1) - cosine * cosine))
+    assert abs(f(cosine, eta) - f(transmitted, np.float32(1) / eta)) <= 1e-6
+
+
+def test_ggx_normal_incidence_closed_form(oracle_mod):
+    # tests/material_transport_tests.cu:100-108: at normal incidence
+    # f = F0 / (4 pi r^4), G = 1; the sampler mixes 95% GGX and 5% cosine.
+    albedo = np.array([0.9, 0.8, 0.7], np.float32)
+    r = np.float32(0.47)
+    n = np.array([0, 0, 1], np.float32)
+    value, pdf = oracle_mod.eval_specgloss(albedo, np.zeros(3), r, n, n, n)
+    lobe = 1.0 / (4 * np.pi * float(r) ** 4)
+    assert np.allclose(value, albedo * lobe, atol=1e-5)
+    assert abs(pdf - (0.95 * lobe + 0.05 / np.pi)) <= 1e-5
+
+
+@pytest.mark.parametrize("case", [c[0] for c in scenes.reference_cases()])
+def test_oracle_traversal_matches_fp64_brute_force(oracle_mod, case):
+    # tests/scene_acceleration_tests.cu:181-251 tolerances: material exact,
+    # |dt| < 3e-4 max(1, t), |dn| < 3e-3, sphere attributes zeroed.
+    builder = dict(scenes.reference_cases())[case]
+    scene = wpt.Scene(builder.desc(), device=False, owner=builder)
+    oracle = oracle_mod.Oracle(scene)
+    rays = oracle_mod.reference_rays(12345, 2048)
+    hits = oracle.trace(rays)
+    t, mat, normal = oracle.brute_fp64(rays)
+    expected_hit = mat != wpt.INVALID
+    assert np.array_equal(hits["kind"] != wpt.HIT_NONE, expected_hit)
+    h = expected_hit
+    assert np.array_equal(hits["material"][h], mat[h])
+    assert np.all(np.abs(hits["t"][h] - t[h]) < 3e-4 * np.maximum(1.0, t[h]))
+    sphere = hits["kind"] == wpt.HIT_SPHERE
+    assert np.all(hits["u"][sphere] == 0) and np.all(hits["v"][sphere] == 0)
+    assert np.all(hits["light_index"][sphere] == wpt.INVALID)
+    # Shadow semantics: nothing blocks before the closest hit, it blocks after.
+    limits = np.where(h, t, 1000.0).astype(np.float32)
+    assert not oracle.occluded(rays, limits - 0.02).any()
+    assert np.array_equal(oracle.occluded(rays, limits + 0.02).astype(bool), h)
+
+
+def test_brute_force_normals_are_unit_geometric(oracle_mod):
+    # The brute force reports the outward geometric normal (not face-forwarded,
+    # as in tests/scene_acceleration_tests.cu:216-232): unit length, and rays
+    # that start inside the mesh-free room see both orientations.
+    builder = dict(scenes.reference_cases())["mixed geometry and shared mesh"]
+    scene = wpt.Scene(builder.desc(), device=False, owner=builder)
+    oracle = oracle_mod.Oracle(scene)
+    rays = oracle_mod.reference_rays(12345, 2048)
+    t, mat, normal = oracle.brute_fp64(rays)
+    h = mat != wpt.INVALID
+    assert h.sum() > 100
+    assert np.allclose(np.linalg.norm(normal[h].astype(np.float64), axis=1), 1.0, atol=1e-5)
+    facing = np.einsum("ij,ij->i", normal[h].astype(np.float64), rays[3:6].T[h].astype(np.float64))
+    assert (facing < 0).any() and (facing > 0).any()

@@ -89,6 +89,7 @@ struct wpt_scene {
     bool onDevice = false;
     DevScene dev{};
     DeviceArena arena;
+    uint2* abiSpill = nullptr;  // traversal stack spill of the C ABI trace calls
 };
 
 struct wpt_renderer {
@@ -142,8 +143,8 @@ int validateConfig(const wpt_render_config& c) {
     if (c.width == 0 || c.height == 0) return fail(WPT_ERROR_INVALID_ARGUMENT, "Render dimensions must be greater than zero");
     if (c.samples_per_pixel == 0 || c.max_depth == 0)
         return fail(WPT_ERROR_INVALID_ARGUMENT, "Samples per pixel and maximum depth must be greater than zero");
-    if (c.block_size == 0 || c.block_size > 1024 || c.block_size % 64 != 0)
-        return fail(WPT_ERROR_INVALID_ARGUMENT, "Block size must be a multiple of 64 between 64 and 1024");
+    if (c.block_size == 0 || c.block_size > 512 || c.block_size % 64 != 0)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Block size must be a multiple of 64 between 64 and 512");
     if (c.tiled && (c.tile_width == 0 || c.tile_height == 0))
         return fail(WPT_ERROR_INVALID_ARGUMENT, "Tile dimensions must be greater than zero");
     if (c.rng_strategy > WPT_RNG_PCG32) return fail(WPT_ERROR_INVALID_ARGUMENT, "Unknown RNG strategy");
@@ -217,7 +218,7 @@ static int createScene(const wpt_scene_desc* desc, wpt_scene** out, bool upload)
         delete s;
         return fail(WPT_ERROR_INVALID_ARGUMENT, err);
     }
-    if (s->host.maxStackDepth > PrivateStack::kCapacity) {
+    if (s->host.maxStackDepth > kMaxStackDepth) {
         delete s;
         return fail(WPT_ERROR_UNSUPPORTED, "acceleration structure deeper than the traversal stack");
     }
@@ -384,18 +385,33 @@ WPT_API int wpt_scene_export_light_indices(const wpt_scene* s, uint32_t* instanc
     return WPT_OK;
 }
 
-WPT_API int wpt_trace_closest(const wpt_scene* s, const float* rays, uint32_t n, wpt_hit* hits, void* stream) {
-    if (!s || !s->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not resident on a device");
-    if (n && (!rays || !hits)) return fail(WPT_ERROR_INVALID_ARGUMENT, "null ray or hit buffer");
-    WPT_HIP(launchTraceClosest(s->dev, rays, n, hits, static_cast<hipStream_t>(stream)));
+static int ensureAbiSpill(wpt_scene* s) {
+    if (s->abiSpill) return WPT_OK;
+    void* p = nullptr;
+    size_t bytes = sizeof(uint2) * (kMaxStackDepth - kLdsStackEntries) * static_cast<size_t>(abiTraceThreads());
+    WPT_HIP(s->arena.alloc(bytes, &p));
+    s->abiSpill = static_cast<uint2*>(p);
     return WPT_OK;
 }
 
-WPT_API int wpt_trace_occluded(const wpt_scene* s, const float* rays, const float* tmax, uint32_t n, uint8_t* out,
+WPT_API int wpt_trace_closest(const wpt_scene* cs, const float* rays, uint32_t n, wpt_hit* hits, void* stream) {
+    wpt_scene* s = const_cast<wpt_scene*>(cs);
+    if (!s || !s->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not resident on a device");
+    if (n && (!rays || !hits)) return fail(WPT_ERROR_INVALID_ARGUMENT, "null ray or hit buffer");
+    int status = ensureAbiSpill(s);
+    if (status != WPT_OK) return status;
+    WPT_HIP(launchTraceClosest(s->dev, rays, n, hits, s->abiSpill, static_cast<hipStream_t>(stream)));
+    return WPT_OK;
+}
+
+WPT_API int wpt_trace_occluded(const wpt_scene* cs, const float* rays, const float* tmax, uint32_t n, uint8_t* out,
     void* stream) {
+    wpt_scene* s = const_cast<wpt_scene*>(cs);
     if (!s || !s->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not resident on a device");
     if (n && (!rays || !tmax || !out)) return fail(WPT_ERROR_INVALID_ARGUMENT, "null ray, distance or output buffer");
-    WPT_HIP(launchTraceOccluded(s->dev, rays, tmax, n, out, static_cast<hipStream_t>(stream)));
+    int status = ensureAbiSpill(s);
+    if (status != WPT_OK) return status;
+    WPT_HIP(launchTraceOccluded(s->dev, rays, tmax, n, out, s->abiSpill, static_cast<hipStream_t>(stream)));
     return WPT_OK;
 }
 
@@ -493,18 +509,36 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     R_HIP(r->arena.alloc(sizeof(RenderTile), &p));
     ctx.tile = static_cast<const RenderTile*>(p);
     R_HIP(hipMemset(ctx.framebuffer, 0, sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height));
+    // Resident grids: the trace/shadow kernels claim work per wave and the
+    // shade kernel per ordered chunk, so one resident wave of workgroups
+    // covers any queue length.
+    Occupancy occ{};
+    R_HIP(kernelOccupancy(c.block_size, &occ));
+    int device = 0, cus = 0, coop = 0;
+    R_HIP(hipGetDevice(&device));
+    R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (occ.trace <= 0 || occ.shade <= 0 || occ.shadow <= 0 || occ.compact <= 0)
+        return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the wavefront kernels"));
+    uint32_t cu = static_cast<uint32_t>(cus);
+    ctx.traceBlocks = std::max(1u, std::min(ctx.gridBlocks, static_cast<uint32_t>(std::max(occ.trace, occ.shadow)) * cu));
+    ctx.shadeBlocks = std::max(1u, std::min(ctx.gridBlocks, static_cast<uint32_t>(std::min(occ.shade, occ.compact)) * cu));
     if (c.mode == WPT_MODE_PERSISTENT) {
-        int perCu = 0;
-        R_HIP(persistentOccupancy(c.block_size, &perCu));
-        if (perCu <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the persistent kernel"));
-        int device = 0, cus = 0, coop = 0;
-        R_HIP(hipGetDevice(&device));
-        R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        if (occ.persistent <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the persistent kernel"));
         R_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
         if (!coop) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "device lacks cooperative launch"));
-        uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(cus);
+        uint32_t resident = static_cast<uint32_t>(occ.persistent) * cu;
         uint32_t blocks = c.persistent_blocks ? std::min(c.persistent_blocks, resident) : resident;
         ctx.persistentBlocks = std::max(1u, std::min(blocks, ctx.gridBlocks));
+    }
+    ctx.spillThreads = std::max(ctx.traceBlocks, ctx.persistentBlocks) * c.block_size;
+    R_HIP(r->arena.alloc(sizeof(uint2) * (kMaxStackDepth - kLdsStackEntries) * static_cast<size_t>(ctx.spillThreads), &p));
+    ctx.spill = static_cast<uint2*>(p);
+    R_HIP(r->arena.alloc(cap, &p));
+    ctx.alive = static_cast<uint8_t*>(p);
+    if (c.mode == WPT_MODE_GRAPH) {
+        R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kShadowFieldCount) * cap, &p));
+        ctx.shadow.words = static_cast<uint32_t*>(p);
+        ctx.shadow.capacity = cap;
     }
     R_HIP(hipDeviceSynchronize());
 #undef R_HIP

@@ -334,9 +334,22 @@ WPT_HD Vec3 evaluateSurfaceCos(const Material& m, const Vec3& shadingNormal, con
 }
 
 // ---- next-event estimation toward triangle lights (device_traversal.cuh:529-569)
-template <class Stack>
+//
+// The shadow test is deferred: the request carries the segment and the
+// clamped contribution, and whoever resolves it adds the contribution to the
+// path radiance when the segment is unoccluded.  NEE is the last radiance
+// addition of a bounce, so resolving it after the continuation is sampled
+// keeps the reference's order of floating-point additions.
+struct ShadowRequest {
+    Vec3 origin;
+    Vec3 direction;
+    float tmax;
+    Vec3 contribution;
+    uint32_t valid;
+};
+
 WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
-    const Material& material, const ShadeParams& sp, Stack& stack, uint32_t& shadowQueries) {
+    const Material& material, const ShadeParams& sp, ShadowRequest& request) {
     if (s.lightCount == 0) return;
     uint32_t candidate = static_cast<uint32_t>(rand01(path, sp.rngStrategy) * static_cast<float>(s.lightCount));
     if (candidate > s.lightCount - 1) candidate = s.lightCount - 1;
@@ -359,11 +372,6 @@ WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceH
     Vec3 segment = shadowTarget - shadowOrigin;
     float shadowDistance = length(segment);
     if (!(shadowDistance > kRayMinDistance)) return;
-    Ray shadow(shadowOrigin, segment / shadowDistance);
-    if (!s.ignoreDirectLightOcclusion) {
-        ++shadowQueries;
-        if (traceOccluded(s, shadow, shadowDistance - kRayMinDistance, stack)) return;
-    }
     const Material& lightMaterial = s.materials[light.material];
     float geometryTerm = lightCosine * light.area / (distanceSquared * light.selectionPdf);
     float lightPdf = light.selectionPdf * distanceSquared / (lightCosine * light.area);
@@ -371,7 +379,21 @@ WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceH
     Vec3 bsdf = evaluateSurfaceCos(material, shadingNormal, -path.ray.direction, direction, bsdfPdf);
     float misWeight = powerHeuristic(lightPdf, bsdfPdf);
     Vec3 directLighting = hadamard(bsdf, lightMaterial.emission) * (geometryTerm * misWeight);
-    addPathRadiance(path, hadamard(path.throughput, directLighting), sp.clamp, false);
+    Vec3 contribution = hadamard(path.throughput, directLighting);
+    if (s.ignoreDirectLightOcclusion) {
+        addPathRadiance(path, contribution, sp.clamp, false);
+        return;
+    }
+    // addPathRadiance's clamp applied up front; the addition happens on resolve.
+    if (path.depth > 0u && sp.clamp > 0.0f) {
+        float magnitude = fabsf(contribution.x) / 3.0f + fabsf(contribution.y) / 3.0f + fabsf(contribution.z) / 3.0f;
+        if (magnitude > sp.clamp) contribution *= sp.clamp / magnitude;
+    }
+    request.origin = shadowOrigin;
+    request.direction = segment / shadowDistance;
+    request.tmax = shadowDistance - kRayMinDistance;
+    request.contribution = contribution;
+    request.valid = 1;
 }
 
 // ---- continuations
@@ -430,15 +452,36 @@ WPT_HD Vec3 continueDielectric(PathState& path, const SurfaceHit& hit, const Vec
     }
     path.specularBounce = 1;
     path.previousBsdfPdf = 0.0f;
-    return reflected ? boundary.reflection : boundary.transmission;
+    // Per-component selects keep the interface in registers.
+    return Vec3(reflected ? boundary.reflection.x : boundary.transmission.x,
+        reflected ? boundary.reflection.y : boundary.transmission.y,
+        reflected ? boundary.reflection.z : boundary.transmission.z);
 }
 
 // One bounce of shading.  Returns true when the path continues with
-// path.ray set to the continuation; on termination *done receives the
-// accumulated radiance to add to the pixel.
-template <class Stack>
-WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp, Stack& stack,
-    Vec3& done, uint32_t& shadowQueries) {
+// path.ray set to the continuation.  A terminated path adds `debugColor` to
+// its pixel when `isDebug`, else its radiance after `shadow` is resolved.
+struct ShadeOutcome {
+    bool alive;
+    bool isDebug;
+    Vec3 debugColor;
+    ShadowRequest shadow;
+};
+
+WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp,
+    ShadeOutcome& out);
+
+WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp,
+    ShadeOutcome& out) {
+    out.isDebug = false;
+    out.shadow.valid = 0;
+    bool alive = shadeBounce(s, path, h, sp, out);
+    out.alive = alive;
+    return alive;
+}
+
+WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp,
+    ShadeOutcome& out) {
     bool didHit = h.kind != kHitNone;
     if (sp.intersectionDebug || sp.shadingNormalDebug) {
         Vec3 color(0.05f, 0.10f, 0.25f);
@@ -447,7 +490,8 @@ WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, co
             color = 0.5f * (hit.normal + Vec3(1.0f, 1.0f, 1.0f));
         }
         path.active = 0;
-        done = color;
+        out.isDebug = true;
+        out.debugColor = color;
         return false;
     }
     if (!didHit) {
@@ -457,7 +501,6 @@ WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, co
             addPathRadiance(path, hadamard(path.throughput, sky), sp.clamp, true);
         }
         path.active = 0;
-        done = path.radiance;
         return false;
     }
     SurfaceHit hit = surfaceAttributes(s, path.ray, h);
@@ -487,7 +530,6 @@ WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, co
         }
         addPathRadiance(path, hadamard(path.throughput, material.emission) * misWeight, sp.clamp, true);
         path.active = 0;
-        done = path.radiance;
         return false;
     }
     Vec3 shadingNormal = hit.normal;
@@ -497,16 +539,16 @@ WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, co
     Vec3 direction;
     switch (type) {
     case kDiffuse:
-        sampleDirectLight(s, path, hit, shadingNormal, material, sp, stack, shadowQueries);
+        sampleDirectLight(s, path, hit, shadingNormal, material, sp, out.shadow);
         direction = continueDiffuse(path, hit, shadingNormal, material, sp.rngStrategy);
         break;
     case kSpecularGlossiness:
-        sampleDirectLight(s, path, hit, shadingNormal, material, sp, stack, shadowQueries);
+        sampleDirectLight(s, path, hit, shadingNormal, material, sp, out.shadow);
         direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
         break;
     case kMetal:
         if (roughness > 0.0f) {
-            sampleDirectLight(s, path, hit, shadingNormal, material, sp, stack, shadowQueries);
+            sampleDirectLight(s, path, hit, shadingNormal, material, sp, out.shadow);
             direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
             break;
         }
@@ -533,7 +575,6 @@ WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, co
         ++path.depth;
         if (path.depth >= sp.maxDepth) {
             path.active = 0;
-            done = path.radiance;
             return false;
         }
         if (path.depth >= sp.rrStart) {
@@ -541,14 +582,12 @@ WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, co
                 fmaxf(path.throughput.y, path.throughput.z))));
             if (rand01(path, sp.rngStrategy) > survival) {
                 path.active = 0;
-                done = path.radiance;
                 return false;
             }
             path.throughput *= 1.0f / survival;
         }
         return true;
     }
-    done = path.radiance;
     return false;
 }
 

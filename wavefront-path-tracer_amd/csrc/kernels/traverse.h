@@ -120,10 +120,14 @@ WPT_HD bool slab(const RayInv& r, float mnx, float mny, float mnz, float mxx, fl
 WPT_HD Vec3 xyz(const float4& f) { return Vec3(f.x, f.y, f.z); }
 WPT_HD uint32_t wbits(const float4& f) { return bitsOf(f.w); }
 
-// Private per-lane traversal stack (node pointer + entry distance).  The
-// scene build rejects trees whose combined TLAS+BLAS depth exceeds it.
+// Traversal stacks hold (node pointer, entry distance) pairs and expose
+// `size`, push, pop and truncate.  This private-array stack serves host-side
+// builds of the traversal code; the kernels use an LDS-resident stack with
+// the same interface.  The scene build rejects trees whose combined
+// TLAS+BLAS depth exceeds kMaxStackDepth.
+constexpr uint32_t kMaxStackDepth = 64;
 struct PrivateStack {
-    static constexpr uint32_t kCapacity = 64;
+    static constexpr uint32_t kCapacity = kMaxStackDepth;
     uint32_t node[kCapacity];
     float tmin[kCapacity];
     uint32_t size = 0;
@@ -137,6 +141,7 @@ struct PrivateStack {
         n = node[size];
         t = tmin[size];
     }
+    WPT_HD void truncate(uint32_t base) { size = base; }
 };
 
 // Pops until an entry still closer than closestT; false when the traversal
@@ -224,7 +229,7 @@ WPT_HD bool anyInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry, co
             if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < maxDistance) {
                 uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
                 if (!cutout[material]) {
-                    stack.size = base;
+                    stack.truncate(base);
                     return true;
                 }
             }
@@ -410,7 +415,7 @@ WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, 
                         float4 sp = s.spheres[leaf & ~kTlasSphereBit];
                         float t;
                         if (intersectSphere(ray, xyz(sp), sp.w, t) && t < maxDistance) {
-                            stack.size = base;
+                            stack.truncate(base);
                             return true;
                         }
                     } else {
@@ -419,7 +424,7 @@ WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, 
                         RayInv local = makeRayInv(toLocal(inst.transform, ray));
                         if (anyInBvh(local, blas.bvh, blas.geometry, s.materialCutout, inst.materialOverride,
                                 maxDistance, stack)) {
-                            stack.size = base;
+                            stack.truncate(base);
                             return true;
                         }
                     }

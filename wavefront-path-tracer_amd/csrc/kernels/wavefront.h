@@ -22,8 +22,25 @@ enum QueueField : uint32_t {
     kQueueFieldCount
 };
 
+// Deferred shadow rays of one bounce (SoA).  target is the path's slot in
+// the queue being shaded (paths are updated in place before compaction), or
+// kShadowTerminated for a path that ended this bounce, whose pre-NEE radiance
+// and pixel travel with the request.
+enum ShadowField : uint32_t {
+    kSOx, kSOy, kSOz, kSDx, kSDy, kSDz, kSTmax,
+    kSCx, kSCy, kSCz,
+    kSTarget, kSPixel, kSRx, kSRy, kSRz,
+    kShadowFieldCount
+};
+constexpr uint32_t kShadowTerminated = 0xffffffffu;
+
 struct PathQueue {
     uint32_t* words;  // kQueueFieldCount * capacity
+    uint32_t capacity;
+};
+
+struct ShadowQueue {
+    uint32_t* words;  // kShadowFieldCount * capacity
     uint32_t capacity;
 };
 
@@ -45,47 +62,72 @@ struct RenderTile {
 // Device counters shared by the launches of one renderer.
 struct WavefrontCounters {
     uint32_t count[2];       // live paths in queue 0 / 1
-    uint32_t ticket;         // ordered chunk tickets of the running shade pass
+    uint32_t ticket;         // ordered chunk tickets of the running compaction pass
+    uint32_t traceCursor;    // next unclaimed ray of the running trace pass
+    uint32_t shadowCount;    // shadow requests of the current bounce
+    uint32_t shadowCursor;   // next unclaimed shadow request
     uint32_t sampleIndex;
+    uint32_t shadeCursor;    // next unclaimed path of the running shade pass
     unsigned long long closestQueries;
     unsigned long long shadowQueries;
     uint32_t queueCounts[64];
 };
+
+// Traversal stack entries kept in LDS per lane; deeper entries spill to a
+// per-thread column of `spill`.
+constexpr uint32_t kLdsStackEntries = 16;
 
 struct LaunchContext {
     DevScene scene;
     CameraParams camera;
     ShadeParams shade;
     PathQueue queue[2];
+    ShadowQueue shadow;
     HitBuffer hits;
+    uint8_t* alive;          // per slot: the path continues after this bounce
     float* framebuffer;      // width*height*3
     uint32_t width, height;
     uint32_t* chunkFlags;    // look-back state per chunk
     WavefrontCounters* counters;
     const RenderTile* tile;  // device copy of the current tile
+    uint2* spill;            // (kMaxStackDepth - kLdsStackEntries) x spillThreads
+    uint32_t spillThreads;
     uint32_t blockSize;
-    uint32_t gridBlocks;     // staged launches: ceil(capacity / blockSize)
+    uint32_t gridBlocks;       // ceil(capacity / blockSize)
+    uint32_t traceBlocks;      // resident grid of the trace / shadow kernels
+    uint32_t shadeBlocks;      // resident grid of the shade and compaction kernels
     uint32_t persistentBlocks;
     uint32_t profileQueues;
 };
 
-// Staged wavefront: records primary + maxDepth x (reset, trace, shade) +
-// advance on `stream` (captured once into a hipGraph by the caller).
+// LDS bytes of the traversal stack for a workgroup of `blockSize` lanes.
+inline size_t stackLdsBytes(uint32_t blockSize) { return sizeof(uint2) * kLdsStackEntries * blockSize; }
+
+// Staged wavefront: records primary + maxDepth x (reset, trace, shade,
+// shadow, compact) + advance on `stream` (captured once into a hipGraph by
+// the caller).
 hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream);
 // Persistent cooperative kernel for one tile sample, then advance.
 hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream);
-hipError_t persistentOccupancy(uint32_t blockSize, int* blocksPerCu);
+// Resident workgroups per CU of each kernel at `blockSize`.
+struct Occupancy {
+    int trace, shade, shadow, compact, persistent;
+};
+hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out);
 // One sample of the current tile with every launch bracketed by events:
 // ms/launches accumulate per kernel (0 primary, 1 reset, 2 trace, 3 shade,
-// 4 persistent).  Synchronizes the stream.
-hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[5],
-    uint32_t launches[5]);
+// 4 persistent, 5 shadow, 6 compact).  Synchronizes the stream.
+constexpr int kProfileSlots = 7;
+hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[kProfileSlots],
+    uint32_t launches[kProfileSlots]);
 
-// Standalone traversal for the C ABI.
-hipError_t launchTraceClosest(const DevScene& scene, const float* rays6, uint32_t n, void* hitsOut,
+// Standalone traversal for the C ABI; `spill` holds
+// (kMaxStackDepth - kLdsStackEntries) x abiTraceThreads() entries.
+uint32_t abiTraceThreads();
+hipError_t launchTraceClosest(const DevScene& scene, const float* rays6, uint32_t n, void* hitsOut, uint2* spill,
     hipStream_t stream);
 hipError_t launchTraceOccluded(const DevScene& scene, const float* rays6, const float* tmax, uint32_t n,
-    uint8_t* out, hipStream_t stream);
+    uint8_t* out, uint2* spill, hipStream_t stream);
 // Order-preserving compaction for the C ABI; scratch holds ticket + flags.
 hipError_t launchCompact(const uint32_t* items, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t capacity,
     uint32_t* count, uint32_t* scratch, uint32_t scratchWords, hipStream_t stream);

@@ -3,20 +3,28 @@
 // Reference kernels and their MI355X counterparts:
 //   generatePrimaryRays (src/renderer/renderer.cu:10-42) ...... kPrimary
 //   intersectScene (device_traversal.cuh:1186-1194) ........... kTrace
-//   shadePaths + appendWarpCompacted (device_path_shading.cuh:6-173,
-//     queue_compaction.cuh:8-29) ............................... kShade
+//   shadePaths (device_path_shading.cuh:6-173) ................ kShade
+//   isOccluded calls made inside shadePaths ................... kShadow
+//   appendWarpCompacted (queue_compaction.cuh:8-29) ........... kCompact
 //   advanceSampleIndex (renderer.cu:44-48) .................... kAdvance
 //   persistentWavefrontTrace (renderer.cu:55-125) ............. kPersistent
 //
-// Queue compaction is order preserving: a wave computes its survivors' ranks
-// with a 64-bit ballot and mbcnt, the workgroup adds per-wave offsets from
-// LDS, and workgroups take chunk tickets in order and chain their totals with
-// a decoupled look-back (one flag word per chunk).  The continuation queue is
-// therefore the stable filter of the input queue: identical on every run and
-// identical to a sequential filter, where the reference's warp atomics only
-// guarantee the same set.  A workgroup only waits for chunks whose tickets
-// were taken earlier by workgroups that are already running, so the wait
-// always ends.
+// Traversal stacks live in LDS (kLdsStackEntries per lane, lane-interleaved
+// so a wave's accesses at equal depth hit distinct banks); deeper entries
+// spill to a per-thread column in global memory.  Trace and shadow kernels
+// run a resident-sized grid in which each wave claims 64 rays at a time from
+// an atomic cursor, so divergent waves do not idle behind a static split.
+//
+// Shading updates each path in place and sets its alive flag; compaction is
+// a separate, light pass so that no workgroup waits on another's shading.
+// It is order preserving: a wave ranks its survivors with a 64-bit ballot
+// and mbcnt, the workgroup adds per-wave offsets from LDS, and workgroups
+// take chunk tickets in order and chain their totals with a decoupled
+// look-back (one flag word per chunk).  The continuation queue is the stable
+// filter of the input queue: identical on every run and identical to a
+// sequential filter, where the reference's warp atomics only guarantee the
+// same set.  A workgroup only waits for chunks whose tickets were taken
+// earlier by workgroups that are already running, so every wait ends.
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
@@ -34,13 +42,62 @@ namespace {
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
-constexpr uint32_t kMaxWaves = 16;  // 1024-thread workgroups
+constexpr uint32_t kMaxWaves = 16;  // 1024-lane workgroups
+constexpr uint32_t kAbiBlock = 128;
+constexpr uint32_t kAbiBlocks = 2048;
+constexpr uint32_t kSpillEntries = kMaxStackDepth - kLdsStackEntries;
 
 __device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void storeAgent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+extern __shared__ uint2 gLdsStack[];
+
+// Ring of the newest kLdsStackEntries entries in LDS, older ones in global
+// memory.  Entry e lives in ring slot e % K while e >= size - K.
+struct LdsStack {
+    uint2* ring;      // this lane's column: ring[slot * stride]
+    uint32_t stride;  // workgroup size
+    uint2* spill;     // this thread's column: spill[e * spillStride]
+    uint32_t spillStride;
+    uint32_t size;
+
+    __device__ void push(uint32_t n, float t) {
+        uint32_t slot = size & (kLdsStackEntries - 1u);
+        if (size >= kLdsStackEntries) spill[(size - kLdsStackEntries) * spillStride] = ring[slot * stride];
+        ring[slot * stride] = make_uint2(n, __float_as_uint(t));
+        ++size;
+    }
+    __device__ void pop(uint32_t& n, float& t) {
+        --size;
+        uint32_t slot = size & (kLdsStackEntries - 1u);
+        uint2 e = ring[slot * stride];
+        if (size >= kLdsStackEntries) ring[slot * stride] = spill[(size - kLdsStackEntries) * spillStride];
+        n = e.x;
+        t = __uint_as_float(e.y);
+    }
+    __device__ void truncate(uint32_t base) {
+        while (size > base) {
+            uint32_t n;
+            float t;
+            pop(n, t);
+        }
+    }
+};
+static_assert((kLdsStackEntries & (kLdsStackEntries - 1u)) == 0, "LDS stack depth must be a power of two");
+
+__device__ __forceinline__ LdsStack makeStack(uint2* spill, uint32_t spillThreads) {
+    LdsStack s;
+    s.ring = gLdsStack + threadIdx.x;
+    s.stride = blockDim.x;
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    s.spill = spill + g;
+    s.spillStride = spillThreads;
+    s.size = 0;
+    return s;
 }
 
 struct CompactShared {
@@ -96,17 +153,34 @@ __device__ uint32_t orderedSlots(bool alive, uint32_t chunk, uint32_t* flags, Co
     return sh.base + sh.total;
 }
 
-__device__ __forceinline__ float qf(const PathQueue& q, uint32_t field, uint32_t i) {
-    return __uint_as_float(q.words[static_cast<size_t>(field) * q.capacity + i]);
+// Wave-aggregated append: one atomic per wave, returns this lane's index.
+__device__ __forceinline__ uint32_t waveAppend(bool active, uint32_t* counter) {
+    unsigned long long mask = __ballot(active);
+    uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+    uint32_t base = 0;
+    if (__lane_id() == 0 && mask) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __builtin_amdgcn_readfirstlane(base);
+    return base + rank;
 }
-__device__ __forceinline__ uint32_t qu(const PathQueue& q, uint32_t field, uint32_t i) {
-    return q.words[static_cast<size_t>(field) * q.capacity + i];
+
+// Wave-level work claim: 64 consecutive items per call.
+__device__ __forceinline__ uint32_t claimWaveBatch(uint32_t* cursor) {
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(cursor, 64u);
+    return __builtin_amdgcn_readfirstlane(base);
 }
-__device__ __forceinline__ void sf(const PathQueue& q, uint32_t field, uint32_t i, float v) {
-    q.words[static_cast<size_t>(field) * q.capacity + i] = __float_as_uint(v);
+
+__device__ __forceinline__ uint32_t* field(const PathQueue& q, uint32_t f) {
+    return q.words + static_cast<size_t>(f) * q.capacity;
 }
-__device__ __forceinline__ void su(const PathQueue& q, uint32_t field, uint32_t i, uint32_t v) {
-    q.words[static_cast<size_t>(field) * q.capacity + i] = v;
+__device__ __forceinline__ float qf(const PathQueue& q, uint32_t f, uint32_t i) { return __uint_as_float(field(q, f)[i]); }
+__device__ __forceinline__ uint32_t qu(const PathQueue& q, uint32_t f, uint32_t i) { return field(q, f)[i]; }
+__device__ __forceinline__ void sf(const PathQueue& q, uint32_t f, uint32_t i, float v) { field(q, f)[i] = __float_as_uint(v); }
+__device__ __forceinline__ void su(const PathQueue& q, uint32_t f, uint32_t i, uint32_t v) { field(q, f)[i] = v; }
+
+__device__ __forceinline__ uint32_t* sfield(const ShadowQueue& q, uint32_t f) {
+    return q.words + static_cast<size_t>(f) * q.capacity;
 }
 
 __device__ Ray loadRay(const PathQueue& q, uint32_t i) {
@@ -191,85 +265,181 @@ __global__ void kReset(LaunchContext ctx, uint32_t nextSel, uint32_t flagCount) 
     if (i == 0) {
         ctx.counters->count[nextSel] = 0;
         ctx.counters->ticket = 0;
+        ctx.counters->traceCursor = 0;
+        ctx.counters->shadeCursor = 0;
+        ctx.counters->shadowCount = 0;
+        ctx.counters->shadowCursor = 0;
     }
     for (uint32_t j = i; j < flagCount; j += gridDim.x * blockDim.x) ctx.chunkFlags[j] = 0;
 }
 
 __global__ void kTrace(LaunchContext ctx, uint32_t bounce, uint32_t sel) {
     uint32_t count = ctx.counters->count[sel];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&ctx.counters->closestQueries, static_cast<unsigned long long>(count));
         if (ctx.profileQueues && bounce < 64) ctx.counters->queueCounts[bounce] = count;
     }
-    if (i >= count) return;
-    Ray ray = loadRay(ctx.queue[sel], i);
-    PrivateStack stack;
-    HitRecord h = traceClosest(ctx.scene, ray, stack);
-    ctx.hits.t[i] = h.t;
-    ctx.hits.u[i] = h.u;
-    ctx.hits.v[i] = h.v;
-    ctx.hits.kind[i] = h.kind;
-    ctx.hits.instance[i] = h.instance;
-    ctx.hits.prim[i] = h.prim;
-    ctx.hits.material[i] = h.material;
+    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+    const PathQueue& q = ctx.queue[sel];
+    while (true) {
+        uint32_t base = claimWaveBatch(&ctx.counters->traceCursor);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
+        if (i >= count) continue;
+        Ray ray = loadRay(q, i);
+        HitRecord h = traceClosest(ctx.scene, ray, stack);
+        ctx.hits.t[i] = h.t;
+        ctx.hits.u[i] = h.u;
+        ctx.hits.v[i] = h.v;
+        ctx.hits.kind[i] = h.kind;
+        ctx.hits.instance[i] = h.instance;
+        ctx.hits.prim[i] = h.prim;
+        ctx.hits.material[i] = h.material;
+    }
 }
 
-// Shades one chunk of the queue and appends survivors in order.
-__device__ void shadeChunk(const LaunchContext& ctx, uint32_t sel, uint32_t count, uint32_t chunk, bool fused,
+__device__ __forceinline__ HitRecord loadHit(const HitBuffer& hits, uint32_t i) {
+    HitRecord h;
+    h.t = hits.t[i];
+    h.u = hits.u[i];
+    h.v = hits.v[i];
+    h.kind = hits.kind[i];
+    h.instance = hits.instance[i];
+    h.prim = hits.prim[i];
+    h.material = hits.material[i];
+    return h;
+}
+
+__device__ void storeShadow(const ShadowQueue& q, uint32_t s, const ShadowRequest& r, uint32_t target,
+    const PathState& path) {
+    sfield(q, kSOx)[s] = __float_as_uint(r.origin.x);
+    sfield(q, kSOy)[s] = __float_as_uint(r.origin.y);
+    sfield(q, kSOz)[s] = __float_as_uint(r.origin.z);
+    sfield(q, kSDx)[s] = __float_as_uint(r.direction.x);
+    sfield(q, kSDy)[s] = __float_as_uint(r.direction.y);
+    sfield(q, kSDz)[s] = __float_as_uint(r.direction.z);
+    sfield(q, kSTmax)[s] = __float_as_uint(r.tmax);
+    sfield(q, kSCx)[s] = __float_as_uint(r.contribution.x);
+    sfield(q, kSCy)[s] = __float_as_uint(r.contribution.y);
+    sfield(q, kSCz)[s] = __float_as_uint(r.contribution.z);
+    sfield(q, kSTarget)[s] = target;
+    sfield(q, kSPixel)[s] = path.pixel;
+    sfield(q, kSRx)[s] = __float_as_uint(path.radiance.x);
+    sfield(q, kSRy)[s] = __float_as_uint(path.radiance.y);
+    sfield(q, kSRz)[s] = __float_as_uint(path.radiance.z);
+}
+
+// Staged shading: each path is updated in place with its alive flag; shadow
+// requests go to the shadow queue; finished paths without a pending shadow
+// ray reach the framebuffer.
+__global__ void kShade(LaunchContext ctx, uint32_t sel) {
+    uint32_t count = ctx.counters->count[sel];
+    const PathQueue& q = ctx.queue[sel];
+    while (true) {
+        uint32_t base = claimWaveBatch(&ctx.counters->shadeCursor);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
+        bool valid = i < count;
+        PathState path;
+        ShadeOutcome outcome;
+        outcome.alive = false;
+        outcome.shadow.valid = 0;
+        if (valid) {
+            path = loadPath(q, i);
+            shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, outcome);
+            ctx.alive[i] = outcome.alive ? 1 : 0;
+            if (outcome.alive) storePath(q, i, path);
+        }
+        bool wantsShadow = valid && outcome.shadow.valid;
+        uint32_t s = waveAppend(wantsShadow, &ctx.counters->shadowCount);
+        if (wantsShadow) {
+            storeShadow(ctx.shadow, s, outcome.shadow, outcome.alive ? i : kShadowTerminated, path);
+        } else if (valid && !outcome.alive) {
+            accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
+        }
+    }
+}
+
+__device__ void copyPath(const PathQueue& from, uint32_t i, const PathQueue& to, uint32_t j) {
+#pragma unroll
+    for (uint32_t f = 0; f < kQueueFieldCount; ++f) field(to, f)[j] = field(from, f)[i];
+}
+
+// Ordered compaction of one chunk of flagged paths into the next queue.
+__device__ void compactChunk(const LaunchContext& ctx, uint32_t sel, uint32_t count, uint32_t chunk,
     CompactShared& sh) {
     const PathQueue& in = ctx.queue[sel];
     const PathQueue& out = ctx.queue[sel ^ 1u];
     uint32_t i = chunk * blockDim.x + threadIdx.x;
-    bool alive = false;
-    PathState path;
-    uint32_t shadow = 0;
-    if (i < count) {
-        path = loadPath(in, i);
-        PrivateStack stack;
-        HitRecord h;
-        if (fused) {
-            h = traceClosest(ctx.scene, path.ray, stack);
-        } else {
-            h.t = ctx.hits.t[i];
-            h.u = ctx.hits.u[i];
-            h.v = ctx.hits.v[i];
-            h.kind = ctx.hits.kind[i];
-            h.instance = ctx.hits.instance[i];
-            h.prim = ctx.hits.prim[i];
-            h.material = ctx.hits.material[i];
-        }
-        Vec3 done;
-        alive = shadePath(ctx.scene, path, h, ctx.shade, stack, done, shadow);
-        if (!alive) accumulate(ctx.framebuffer, path.pixel, done);
-    }
-    if (threadIdx.x == 0) sh.shadow = 0;
-    __syncthreads();
-    if (shadow) atomicAdd(&sh.shadow, shadow);
+    bool alive = i < count && ctx.alive[i] != 0;
     uint32_t slot;
     uint32_t inclusive = orderedSlots(alive, chunk, ctx.chunkFlags, sh, slot);
-    if (alive && slot < out.capacity) storePath(out, slot, path);
+    if (alive && slot < out.capacity) copyPath(in, i, out, slot);
     if (threadIdx.x == 0) {
-        if (sh.shadow) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(sh.shadow));
         uint32_t chunks = (count + blockDim.x - 1) / blockDim.x;
         if (chunk + 1 == chunks) storeAgent(&ctx.counters->count[sel ^ 1u], inclusive);
     }
 }
 
-__global__ void kShade(LaunchContext ctx, uint32_t sel) {
+__global__ void kCompact(LaunchContext ctx, uint32_t sel) {
     __shared__ CompactShared sh;
     uint32_t count = ctx.counters->count[sel];
-    if (threadIdx.x == 0) sh.chunk = atomicAdd(&ctx.counters->ticket, 1u);
-    __syncthreads();
-    uint32_t chunk = sh.chunk;
-    if (chunk * blockDim.x >= count) return;
-    shadeChunk(ctx, sel, count, chunk, false, sh);
+    uint32_t chunks = (count + blockDim.x - 1) / blockDim.x;
+    while (true) {
+        if (threadIdx.x == 0) sh.chunk = atomicAdd(&ctx.counters->ticket, 1u);
+        __syncthreads();
+        uint32_t chunk = sh.chunk;
+        __syncthreads();
+        if (chunk >= chunks) break;
+        compactChunk(ctx, sel, count, chunk, sh);
+    }
+}
+
+// Resolves the bounce's shadow requests: unoccluded contributions are added
+// to the continuing path's radiance, and finished paths reach the pixel.
+__global__ void kShadow(LaunchContext ctx, uint32_t sel) {
+    uint32_t count = ctx.counters->shadowCount;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(count));
+    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+    const ShadowQueue& q = ctx.shadow;
+    const PathQueue& next = ctx.queue[sel];
+    while (true) {
+        uint32_t base = claimWaveBatch(&ctx.counters->shadowCursor);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
+        if (i >= count) continue;
+        Ray ray(Vec3(__uint_as_float(sfield(q, kSOx)[i]), __uint_as_float(sfield(q, kSOy)[i]),
+                    __uint_as_float(sfield(q, kSOz)[i])),
+            Vec3(__uint_as_float(sfield(q, kSDx)[i]), __uint_as_float(sfield(q, kSDy)[i]),
+                __uint_as_float(sfield(q, kSDz)[i])));
+        bool occluded = traceOccluded(ctx.scene, ray, __uint_as_float(sfield(q, kSTmax)[i]), stack);
+        Vec3 c(__uint_as_float(sfield(q, kSCx)[i]), __uint_as_float(sfield(q, kSCy)[i]),
+            __uint_as_float(sfield(q, kSCz)[i]));
+        uint32_t target = sfield(q, kSTarget)[i];
+        if (target != kShadowTerminated) {
+            if (!occluded) {
+                Vec3 r(qf(next, kRx, target), qf(next, kRy, target), qf(next, kRz, target));
+                r += c;
+                sf(next, kRx, target, r.x);
+                sf(next, kRy, target, r.y);
+                sf(next, kRz, target, r.z);
+            }
+        } else {
+            Vec3 r(__uint_as_float(sfield(q, kSRx)[i]), __uint_as_float(sfield(q, kSRy)[i]),
+                __uint_as_float(sfield(q, kSRz)[i]));
+            if (!occluded) r += c;
+            accumulate(ctx.framebuffer, sfield(q, kSPixel)[i], r);
+        }
+    }
 }
 
 __global__ void kAdvance(LaunchContext ctx) {
     if (ctx.tile->advanceSample) ++ctx.counters->sampleIndex;
 }
 
+// Persistent kernel: per bounce, a fused trace/shade/shadow pass updates
+// paths in place, then the ordered compaction pass builds the next queue.
 __global__ void kPersistent(LaunchContext ctx) {
     __shared__ CompactShared sh;
     cg::grid_group grid = cg::this_grid();
@@ -286,6 +456,7 @@ __global__ void kPersistent(LaunchContext ctx) {
         storePath(ctx.queue[0], i, primaryPath(ctx.camera, x, y, ctx.width, ctx.height, sample, ctx.shade.rngStrategy));
     }
     if (gid == 0) storeAgent(&ctx.counters->count[0], count0);
+    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     grid.sync();
     uint32_t sel = 0;
@@ -298,17 +469,43 @@ __global__ void kPersistent(LaunchContext ctx) {
             if (ctx.profileQueues && bounce < 64) ctx.counters->queueCounts[bounce] = count;
             storeAgent(&ctx.counters->count[sel ^ 1u], 0u);
             storeAgent(&ctx.counters->ticket, 0u);
+            storeAgent(&ctx.counters->shadeCursor, 0u);
         }
         for (uint32_t j = gid; j < chunks; j += stride) storeAgent(&ctx.chunkFlags[j], 0u);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        grid.sync();
+        const PathQueue& q = ctx.queue[sel];
+        uint32_t shadowQueries = 0;
+        while (true) {
+            uint32_t base = claimWaveBatch(&ctx.counters->shadeCursor);
+            if (base >= count) break;
+            uint32_t i = base + __lane_id();
+            if (i >= count) continue;
+            PathState path = loadPath(q, i);
+            HitRecord h = traceClosest(ctx.scene, path.ray, stack);
+            ShadeOutcome outcome;
+            shadePath(ctx.scene, path, h, ctx.shade, outcome);
+            if (outcome.shadow.valid) {
+                ++shadowQueries;
+                Ray ray(outcome.shadow.origin, outcome.shadow.direction);
+                if (!traceOccluded(ctx.scene, ray, outcome.shadow.tmax, stack)) path.radiance += outcome.shadow.contribution;
+            }
+            ctx.alive[i] = outcome.alive ? 1 : 0;
+            if (outcome.alive)
+                storePath(q, i, path);
+            else
+                accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
+        }
+        if (shadowQueries) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(shadowQueries));
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
         grid.sync();
         while (true) {
             if (threadIdx.x == 0) sh.chunk = atomicAdd(&ctx.counters->ticket, 1u);
             __syncthreads();
             uint32_t chunk = sh.chunk;
-            if (chunk >= chunks) break;
-            shadeChunk(ctx, sel, count, chunk, true, sh);
             __syncthreads();
+            if (chunk >= chunks) break;
+            compactChunk(ctx, sel, count, chunk, sh);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
         grid.sync();
@@ -316,31 +513,32 @@ __global__ void kPersistent(LaunchContext ctx) {
     }
 }
 
-__global__ void kTraceClosestAbi(DevScene scene, const float* rays, uint32_t n, wpt_hit* out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Ray ray(Vec3(rays[i], rays[n + i], rays[2 * n + i]), Vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
-    PrivateStack stack;
-    HitRecord h = traceClosest(scene, ray, stack);
-    wpt_hit r;
-    r.t = h.t;
-    r.u = h.u;
-    r.v = h.v;
-    r.kind = h.kind;
-    r.instance = h.instance;
-    r.primitive = h.prim;
-    r.material = h.material;
-    r.light_index = kInvalid;
-    if (h.kind != kHitNone) r.light_index = surfaceAttributes(scene, ray, h).lightIndex;
-    out[i] = r;
+__global__ void kTraceClosestAbi(DevScene scene, const float* rays, uint32_t n, wpt_hit* out, uint2* spill) {
+    LdsStack stack = makeStack(spill, gridDim.x * blockDim.x);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        Ray ray(Vec3(rays[i], rays[n + i], rays[2 * n + i]), Vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+        HitRecord h = traceClosest(scene, ray, stack);
+        wpt_hit r;
+        r.t = h.t;
+        r.u = h.u;
+        r.v = h.v;
+        r.kind = h.kind;
+        r.instance = h.instance;
+        r.primitive = h.prim;
+        r.material = h.material;
+        r.light_index = kInvalid;
+        if (h.kind != kHitNone) r.light_index = surfaceAttributes(scene, ray, h).lightIndex;
+        out[i] = r;
+    }
 }
 
-__global__ void kTraceOccludedAbi(DevScene scene, const float* rays, const float* tmax, uint32_t n, uint8_t* out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Ray ray(Vec3(rays[i], rays[n + i], rays[2 * n + i]), Vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
-    PrivateStack stack;
-    out[i] = traceOccluded(scene, ray, tmax[i], stack) ? 1 : 0;
+__global__ void kTraceOccludedAbi(DevScene scene, const float* rays, const float* tmax, uint32_t n, uint8_t* out,
+    uint2* spill) {
+    LdsStack stack = makeStack(spill, gridDim.x * blockDim.x);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        Ray ray(Vec3(rays[i], rays[n + i], rays[2 * n + i]), Vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+        out[i] = traceOccluded(scene, ray, tmax[i], stack) ? 1 : 0;
+    }
 }
 
 __global__ void kCompactReset(uint32_t* scratch, uint32_t words, uint32_t* count) {
@@ -378,18 +576,30 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
     uint32_t capacity = ctx.queue[0].capacity;
     uint32_t flagCount = ceilDiv(capacity, ctx.blockSize);
     uint32_t resetBlocks = ceilDiv(flagCount, 256u);
+    size_t lds = stackLdsBytes(ctx.blockSize);
     hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx); });
     for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {
         uint32_t sel = bounce & 1u;
         e = timer(1, [&] { hipLaunchKernelGGL(kReset, dim3(resetBlocks), dim3(256), 0, stream, ctx, sel ^ 1u, flagCount); });
         if (e == hipSuccess)
-            e = timer(2, [&] { hipLaunchKernelGGL(kTrace, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx, bounce, sel); });
+            e = timer(2, [&] { hipLaunchKernelGGL(kTrace, dim3(ctx.traceBlocks), dim3(ctx.blockSize), lds, stream, ctx, bounce, sel); });
         if (e == hipSuccess)
-            e = timer(3, [&] { hipLaunchKernelGGL(kShade, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel); });
+            e = timer(3, [&] { hipLaunchKernelGGL(kShade, dim3(ctx.shadeBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel); });
+        if (e == hipSuccess && ctx.scene.lightCount != 0 && !ctx.scene.ignoreDirectLightOcclusion)
+            e = timer(5, [&] { hipLaunchKernelGGL(kShadow, dim3(ctx.traceBlocks), dim3(ctx.blockSize), lds, stream, ctx, sel); });
+        if (e == hipSuccess)
+            e = timer(6, [&] { hipLaunchKernelGGL(kCompact, dim3(ctx.shadeBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel); });
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
     return hipGetLastError();
+}
+
+hipError_t launchPersistent(const LaunchContext& ctx, hipStream_t stream) {
+    LaunchContext local = ctx;
+    void* args[] = {&local};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kPersistent), dim3(ctx.persistentBlocks),
+        dim3(ctx.blockSize), args, stackLdsBytes(ctx.blockSize), stream);
 }
 
 }  // namespace
@@ -401,8 +611,26 @@ hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream) {
     });
 }
 
-hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[5],
-    uint32_t launches[5]) {
+hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out) {
+    size_t lds = stackLdsBytes(blockSize);
+    int b = static_cast<int>(blockSize);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace, b, lds);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade, b, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow, b, lds);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->compact, kCompact, b, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent, b, lds);
+    return e;
+}
+
+hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream) {
+    hipError_t e = launchPersistent(ctx, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
+    return hipGetLastError();
+}
+
+hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[kProfileSlots],
+    uint32_t launches[kProfileSlots]) {
     struct Span {
         int kind;
         hipEvent_t a, b;
@@ -423,12 +651,9 @@ hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool pers
     };
     hipError_t e;
     if (persistent) {
-        e = timer(4, [&] {
-            LaunchContext local = ctx;
-            void* args[] = {&local};
-            (void)hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kPersistent), dim3(ctx.persistentBlocks),
-                dim3(ctx.blockSize), args, 0, stream);
-        });
+        hipError_t launchError = hipSuccess;
+        e = timer(4, [&] { launchError = launchPersistent(ctx, stream); });
+        if (e == hipSuccess) e = launchError;
         if (e == hipSuccess) {
             hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
             e = hipGetLastError();
@@ -449,32 +674,21 @@ hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool pers
     return e;
 }
 
-hipError_t persistentOccupancy(uint32_t blockSize, int* blocksPerCu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kPersistent, static_cast<int>(blockSize), 0);
-}
+uint32_t abiTraceThreads() { return kAbiBlock * kAbiBlocks; }
 
-hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream) {
-    LaunchContext local = ctx;
-    void* args[] = {&local};
-    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kPersistent), dim3(ctx.persistentBlocks),
-        dim3(ctx.blockSize), args, 0, stream);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
-    return hipGetLastError();
-}
-
-hipError_t launchTraceClosest(const DevScene& scene, const float* rays6, uint32_t n, void* hitsOut,
+hipError_t launchTraceClosest(const DevScene& scene, const float* rays6, uint32_t n, void* hitsOut, uint2* spill,
     hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(kTraceClosestAbi, dim3(ceilDiv(n, 128u)), dim3(128), 0, stream, scene, rays6, n,
-        static_cast<wpt_hit*>(hitsOut));
+    hipLaunchKernelGGL(kTraceClosestAbi, dim3(kAbiBlocks), dim3(kAbiBlock), stackLdsBytes(kAbiBlock), stream, scene,
+        rays6, n, static_cast<wpt_hit*>(hitsOut), spill);
     return hipGetLastError();
 }
 
 hipError_t launchTraceOccluded(const DevScene& scene, const float* rays6, const float* tmax, uint32_t n,
-    uint8_t* out, hipStream_t stream) {
+    uint8_t* out, uint2* spill, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(kTraceOccludedAbi, dim3(ceilDiv(n, 128u)), dim3(128), 0, stream, scene, rays6, tmax, n, out);
+    hipLaunchKernelGGL(kTraceOccludedAbi, dim3(kAbiBlocks), dim3(kAbiBlock), stackLdsBytes(kAbiBlock), stream, scene,
+        rays6, tmax, n, out, spill);
     return hipGetLastError();
 }
 

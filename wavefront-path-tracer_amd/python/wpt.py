@@ -466,8 +466,8 @@ class Renderer:
 
     def profile_kernels(self, first: int, count: int):
         """Per-kernel device milliseconds and launch counts (event-bracketed)."""
-        ms = np.zeros(5, dtype=np.float64)
-        launches = np.zeros(5, dtype=np.uint32)
+        ms = np.zeros(7, dtype=np.float64)
+        launches = np.zeros(7, dtype=np.uint32)
         check(library().wpt_renderer_profile_kernels(self._h, first, count, ms.ctypes.data, launches.ctypes.data),
               "wpt_renderer_profile_kernels")
         return ms, launches
