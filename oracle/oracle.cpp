@@ -12,12 +12,14 @@
 // Shared dependency: elementary functions (sin, cos, tan, exp, log, pow) come
 // from the product's deterministic wpt_math.h, because libm and the device
 // library differ in the last ulp; that header is pinned separately against
-// libm by tests/test_math.py.  Everything else here is independent code.
+// libm (tests/test_cpu_oracle.py).  Everything else here is independent code.
 //
 // Pinning summary (see tests/): RNG known answers and seed uniqueness; Fresnel
 // and GGX closed forms from the reference's material tests; traversal vs the
 // independent FP64 brute force on the reference's mt19937 ray set; Walker
-// alias invariants; whole-image energy against analytic furnace cases.
+// alias invariants; whole images against closed forms (an emissive
+// enclosure, a mirror under an emitting plane, a Lambertian floor under a
+// uniform emitting hemisphere).
 #include "oracle.h"
 
 #include <algorithm>

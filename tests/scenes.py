@@ -108,3 +108,52 @@ def scaled_instance_scene(scale):
     s.instance(m, wpt.scaled_transform(axis_scales=(scale, scale * 0.5, scale * 2)), wpt.INVALID, wpt.ACCEL_DIRECT)
     s.instance(m, wpt.scaled_transform(axis_scales=(scale, scale * 0.5, scale * 2)), wpt.INVALID, wpt.ACCEL_BVH)
     return s
+
+
+def quad(builder, a, b, c, d, material):
+    """Two triangles a-b-c, a-c-d; the face normal is (b - a) x (c - a)."""
+    builder.static_triangles.append(wpt.make_triangle(a, b, c, material))
+    builder.static_triangles.append(wpt.make_triangle(a, c, d, material))
+
+
+def emissive_enclosure(le=(1.5, 0.75, 0.25)):
+    """A closed box of emissive walls whose normals face inward, around the
+    camera at the origin: every camera ray's radiance is exactly Le."""
+    b = wpt.SceneBuilder()
+    b.black_background = True
+    m = b.material(wpt.make_material(wpt.EMISSIVE, emission=le))
+    s = 2.0
+    p = [(x, y, z) for x in (-s, s) for y in (-s, s) for z in (-s, s)]
+    # Corner index = 4*ix + 2*iy + iz.
+    quad(b, p[0], p[4], p[6], p[2], m)   # z = -s, normal +z
+    quad(b, p[1], p[3], p[7], p[5], m)   # z = +s, normal -z
+    quad(b, p[0], p[1], p[5], p[4], m)   # y = -s, normal +y
+    quad(b, p[2], p[6], p[7], p[3], m)   # y = +s, normal -y
+    quad(b, p[0], p[2], p[3], p[1], m)   # x = -s, normal +x
+    quad(b, p[4], p[5], p[7], p[6], m)   # x = +s, normal -x
+    return b
+
+
+def floor_under_emitter(floor_material, le=(1.0, 1.0, 1.0), half=500.0):
+    """A floor at y = 0 under a parallel emitting plane at y = 1 (normal -y),
+    both 1000 x 1000, black background.  Seen from the middle, the emitter
+    fills the upper hemisphere to within 4e-6 of its solid-angle weight."""
+    b = wpt.SceneBuilder()
+    b.black_background = True
+    f = b.material(floor_material)
+    e = b.material(wpt.make_material(wpt.EMISSIVE, emission=le))
+    h = half
+    quad(b, (-h, 0, -h), (-h, 0, h), (h, 0, h), (h, 0, -h), f)  # normal +y
+    quad(b, (-h, 1, -h), (h, 1, -h), (h, 1, h), (-h, 1, h), e)  # normal -y
+    return b
+
+
+def camera(position, forward, up_hint=(0, 1, 0), fov=0.6, aspect=1.0):
+    """Pinhole camera; fov is the vertical field of view in radians."""
+    import numpy as np
+    f = np.asarray(forward, np.float64)
+    f /= np.linalg.norm(f)
+    r = np.cross(f, np.asarray(up_hint, np.float64))
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    return wpt.Camera(wpt.vec3(position), wpt.vec3(f), wpt.vec3(r), wpt.vec3(u), fov, aspect)

@@ -133,3 +133,53 @@ def test_brute_force_normals_are_unit_geometric(oracle_mod):
     assert np.allclose(np.linalg.norm(normal[h].astype(np.float64), axis=1), 1.0, atol=1e-5)
     facing = np.einsum("ij,ij->i", normal[h].astype(np.float64), rays[3:6].T[h].astype(np.float64))
     assert (facing < 0).any() and (facing > 0).any()
+
+
+def render_oracle(oracle_mod, builder, cam, size, spp, max_depth, rr_start, clamp, precision=32):
+    scene = wpt.Scene(builder.desc(), device=False, owner=builder)
+    oracle = oracle_mod.Oracle(scene)
+    cfg = wpt.render_config(size, size, spp=spp, max_depth=max_depth, rr_start=rr_start, clamp=clamp)
+    fb, _ = oracle.render(cam, cfg, precision=precision)
+    return fb / spp
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_emissive_enclosure_is_exactly_le(oracle_mod, precision):
+    # Camera rays end on an emitter at depth 0 with MIS weight 1
+    # (device_path_shading.cuh:59-76): radiance = throughput 1 x Le.
+    le = (1.5, 0.75, 0.25)
+    img = render_oracle(oracle_mod, scenes.emissive_enclosure(le), scenes.camera((0.3, -0.2, 0.1), (1, 0.4, 0.2)),
+                        16, 4, 8, 3, 10.0, precision)
+    assert np.array_equal(img.reshape(-1, 3), np.tile(np.array(le, np.float32), (16 * 16, 1)))
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_mirror_floor_reflects_emitter_exactly(oracle_mod, precision):
+    # Metal with roughness 0 (device_path_shading.cuh:115-131): throughput x
+    # albedo, specular bounce, the emitter is then hit with MIS weight 1.
+    albedo = (0.75, 0.5, 0.25)
+    b = scenes.floor_under_emitter(wpt.make_material(wpt.METAL, albedo, roughness=0.0), le=(2.0, 2.0, 2.0))
+    img = render_oracle(oracle_mod, b, scenes.camera((0, 0.5, 0), (0, -1, 0), up_hint=(0, 0, 1)), 16, 4, 8, 3, 10.0,
+                        precision)
+    want = np.float32(2.0) * np.array(albedo, np.float32)
+    assert np.array_equal(img.reshape(-1, 3), np.tile(want, (16 * 16, 1)))
+
+
+@pytest.mark.parametrize("kind,roughness", [(wpt.DIFFUSE, 0.0), (wpt.METAL, 0.3)])
+def test_floor_under_uniform_emitter_energy(oracle_mod, kind, roughness):
+    # A surface under a uniform emitting hemisphere reflects albedo x Le for
+    # any energy-conserving BRDF whose directional albedo is the albedo;
+    # for the Lambertian floor that is exact, and light sampling, BSDF
+    # sampling and their MIS weights must add up to it.  A dropped MIS weight
+    # doubles the result, a dropped cosine or 1/pi changes it by pi.
+    rho = 0.6
+    b = scenes.floor_under_emitter(wpt.make_material(kind, (rho, rho, rho), roughness=roughness))
+    img = render_oracle(oracle_mod, b, scenes.camera((0, 0.5, 0), (0, -1, 0), up_hint=(0, 0, 1), fov=0.3), 32, 32, 4,
+                        8, 0.0)
+    mean = img.mean(axis=(0, 1))
+    sem = img.reshape(-1, 3).std(axis=0) / np.sqrt(32 * 32)
+    if kind == wpt.DIFFUSE:
+        assert np.all(np.abs(mean - rho) < max(0.002, 5 * sem.max())), (mean, sem)
+    else:
+        # GGX reflectance at roughness 0.3 loses energy to masking, never gains.
+        assert np.all(mean < rho + 5 * sem.max()) and np.all(mean > 0.5 * rho), (mean, sem)

@@ -104,3 +104,29 @@ def test_progressive_samples_accumulate(torch_cuda):
     split, n = r.framebuffer()
     assert n == 4
     assert np.array_equal(full, split)
+
+
+@pytest.mark.parametrize("case", ["enclosure", "mirror", "lambert"])
+def test_closed_form_scenes(torch_cuda, oracle_mod, case):
+    # The analytic scenes that pin the oracle, rendered on the device: equal
+    # to the oracle image bit for bit and to the closed form.
+    import scenes
+    if case == "enclosure":
+        b, cam, want = scenes.emissive_enclosure((1.5, 0.75, 0.25)), \
+            scenes.camera((0.3, -0.2, 0.1), (1, 0.4, 0.2)), np.array([1.5, 0.75, 0.25], np.float32)
+    elif case == "mirror":
+        b = scenes.floor_under_emitter(wpt.make_material(wpt.METAL, (0.75, 0.5, 0.25)), le=(2, 2, 2))
+        cam, want = scenes.camera((0, 0.5, 0), (0, -1, 0), up_hint=(0, 0, 1)), np.array([1.5, 1.0, 0.5], np.float32)
+    else:
+        b = scenes.floor_under_emitter(wpt.make_material(wpt.DIFFUSE, (0.6, 0.6, 0.6)))
+        cam, want = scenes.camera((0, 0.5, 0), (0, -1, 0), up_hint=(0, 0, 1), fov=0.3), None
+    scene = wpt.Scene(b.desc(), device=True, owner=b)
+    cfg = wpt.render_config(32, 32, spp=16, max_depth=4, rr_start=8, clamp=0.0)
+    img, n = wpt.Renderer(scene, cam, cfg).render_frame()
+    ref, _ = oracle_mod.Oracle(scene).render(cam, cfg)
+    assert np.array_equal(img, ref)
+    mean = img / n
+    if want is not None:
+        assert np.array_equal(mean.reshape(-1, 3), np.tile(want, (32 * 32, 1)))
+    else:
+        assert np.all(np.abs(mean.mean(axis=(0, 1)) - 0.6) < 0.004)

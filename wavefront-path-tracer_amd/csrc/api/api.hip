@@ -501,27 +501,24 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     }
     R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height, &p));
     ctx.framebuffer = static_cast<float*>(p);
-    R_HIP(r->arena.alloc(sizeof(uint32_t) * (ctx.gridBlocks + 1), &p));
-    ctx.chunkFlags = static_cast<uint32_t*>(p);
     R_HIP(r->arena.alloc(sizeof(WavefrontCounters), &p));
     ctx.counters = static_cast<WavefrontCounters*>(p);
     R_HIP(hipMemset(p, 0, sizeof(WavefrontCounters)));
     R_HIP(r->arena.alloc(sizeof(RenderTile), &p));
     ctx.tile = static_cast<const RenderTile*>(p);
     R_HIP(hipMemset(ctx.framebuffer, 0, sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height));
-    // Resident grids: the trace/shadow kernels claim work per wave and the
-    // shade kernel per ordered chunk, so one resident wave of workgroups
-    // covers any queue length.
+    // Resident grids: the trace, shade and shadow kernels claim work per
+    // wave, so one resident wave of workgroups covers any queue length.
     Occupancy occ{};
     R_HIP(kernelOccupancy(c.block_size, &occ));
     int device = 0, cus = 0, coop = 0;
     R_HIP(hipGetDevice(&device));
     R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    if (occ.trace <= 0 || occ.shade <= 0 || occ.shadow <= 0 || occ.compact <= 0)
+    if (occ.trace <= 0 || occ.shade <= 0 || occ.shadow <= 0)
         return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the wavefront kernels"));
     uint32_t cu = static_cast<uint32_t>(cus);
     ctx.traceBlocks = std::max(1u, std::min(ctx.gridBlocks, static_cast<uint32_t>(std::max(occ.trace, occ.shadow)) * cu));
-    ctx.shadeBlocks = std::max(1u, std::min(ctx.gridBlocks, static_cast<uint32_t>(std::min(occ.shade, occ.compact)) * cu));
+    ctx.shadeBlocks = std::max(1u, std::min(ctx.gridBlocks, static_cast<uint32_t>(occ.shade) * cu));
     if (c.mode == WPT_MODE_PERSISTENT) {
         if (occ.persistent <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the persistent kernel"));
         R_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
@@ -533,8 +530,6 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     ctx.spillThreads = std::max(ctx.traceBlocks, ctx.persistentBlocks) * c.block_size;
     R_HIP(r->arena.alloc(sizeof(uint2) * (kMaxStackDepth - kLdsStackEntries) * static_cast<size_t>(ctx.spillThreads), &p));
     ctx.spill = static_cast<uint2*>(p);
-    R_HIP(r->arena.alloc(cap, &p));
-    ctx.alive = static_cast<uint8_t*>(p);
     if (c.mode == WPT_MODE_GRAPH) {
         R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kShadowFieldCount) * cap, &p));
         ctx.shadow.words = static_cast<uint32_t*>(p);

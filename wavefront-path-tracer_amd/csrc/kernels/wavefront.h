@@ -22,10 +22,9 @@ enum QueueField : uint32_t {
     kQueueFieldCount
 };
 
-// Deferred shadow rays of one bounce (SoA).  target is the path's slot in
-// the queue being shaded (paths are updated in place before compaction), or
-// kShadowTerminated for a path that ended this bounce, whose pre-NEE radiance
-// and pixel travel with the request.
+// Deferred shadow rays of one bounce (SoA).  target is the continuing path's
+// slot in the next queue, or kShadowTerminated for a path that ended this
+// bounce, whose pre-NEE radiance and pixel travel with the request.
 enum ShadowField : uint32_t {
     kSOx, kSOy, kSOz, kSDx, kSDy, kSDz, kSTmax,
     kSCx, kSCy, kSCz,
@@ -61,8 +60,9 @@ struct RenderTile {
 
 // Device counters shared by the launches of one renderer.
 struct WavefrontCounters {
-    uint32_t count[2];       // live paths in queue 0 / 1
-    uint32_t ticket;         // ordered chunk tickets of the running compaction pass
+    uint32_t count[3];       // live paths: staged mode uses [0]/[1] per queue;
+                             // the persistent kernel rotates all three by bounce
+    uint32_t cursor[3];      // persistent kernel: work cursor per bounce (mod 3)
     uint32_t traceCursor;    // next unclaimed ray of the running trace pass
     uint32_t shadowCount;    // shadow requests of the current bounce
     uint32_t shadowCursor;   // next unclaimed shadow request
@@ -84,10 +84,8 @@ struct LaunchContext {
     PathQueue queue[2];
     ShadowQueue shadow;
     HitBuffer hits;
-    uint8_t* alive;          // per slot: the path continues after this bounce
     float* framebuffer;      // width*height*3
     uint32_t width, height;
-    uint32_t* chunkFlags;    // look-back state per chunk
     WavefrontCounters* counters;
     const RenderTile* tile;  // device copy of the current tile
     uint2* spill;            // (kMaxStackDepth - kLdsStackEntries) x spillThreads
@@ -95,7 +93,7 @@ struct LaunchContext {
     uint32_t blockSize;
     uint32_t gridBlocks;       // ceil(capacity / blockSize)
     uint32_t traceBlocks;      // resident grid of the trace / shadow kernels
-    uint32_t shadeBlocks;      // resident grid of the shade and compaction kernels
+    uint32_t shadeBlocks;      // resident grid of the shade kernel
     uint32_t persistentBlocks;
     uint32_t profileQueues;
 };
@@ -104,19 +102,20 @@ struct LaunchContext {
 inline size_t stackLdsBytes(uint32_t blockSize) { return sizeof(uint2) * kLdsStackEntries * blockSize; }
 
 // Staged wavefront: records primary + maxDepth x (reset, trace, shade,
-// shadow, compact) + advance on `stream` (captured once into a hipGraph by
-// the caller).
+// shadow) + advance on `stream` (captured once into a hipGraph by the
+// caller).
 hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream);
 // Persistent cooperative kernel for one tile sample, then advance.
 hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream);
 // Resident workgroups per CU of each kernel at `blockSize`.
 struct Occupancy {
-    int trace, shade, shadow, compact, persistent;
+    int trace, shade, shadow, persistent;
 };
 hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out);
 // One sample of the current tile with every launch bracketed by events:
 // ms/launches accumulate per kernel (0 primary, 1 reset, 2 trace, 3 shade,
-// 4 persistent, 5 shadow, 6 compact).  Synchronizes the stream.
+// 4 persistent, 5 shadow, 6 compact: unused by the renderer, whose shade
+// kernel appends continuations itself).  Synchronizes the stream.
 constexpr int kProfileSlots = 7;
 hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[kProfileSlots],
     uint32_t launches[kProfileSlots]);

@@ -5,26 +5,28 @@
 //   intersectScene (device_traversal.cuh:1186-1194) ........... kTrace
 //   shadePaths (device_path_shading.cuh:6-173) ................ kShade
 //   isOccluded calls made inside shadePaths ................... kShadow
-//   appendWarpCompacted (queue_compaction.cuh:8-29) ........... kCompact
+//   appendWarpCompacted (queue_compaction.cuh:8-29) ........... waveAppend in
+//                                                               kShade / kPersistent,
+//                                                               kCompactAbi (C ABI)
 //   advanceSampleIndex (renderer.cu:44-48) .................... kAdvance
 //   persistentWavefrontTrace (renderer.cu:55-125) ............. kPersistent
 //
 // Traversal stacks live in LDS (kLdsStackEntries per lane, lane-interleaved
 // so a wave's accesses at equal depth hit distinct banks); deeper entries
-// spill to a per-thread column in global memory.  Trace and shadow kernels
-// run a resident-sized grid in which each wave claims 64 rays at a time from
-// an atomic cursor, so divergent waves do not idle behind a static split.
+// spill to a per-thread column in global memory.  Trace, shade and shadow
+// kernels run a resident-sized grid in which each wave claims 64 items at a
+// time from an atomic cursor, so divergent waves do not idle behind a static
+// split.
 //
-// Shading updates each path in place and sets its alive flag; compaction is
-// a separate, light pass so that no workgroup waits on another's shading.
-// It is order preserving: a wave ranks its survivors with a 64-bit ballot
-// and mbcnt, the workgroup adds per-wave offsets from LDS, and workgroups
-// take chunk tickets in order and chain their totals with a decoupled
-// look-back (one flag word per chunk).  The continuation queue is the stable
-// filter of the input queue: identical on every run and identical to a
-// sequential filter, where the reference's warp atomics only guarantee the
-// same set.  A workgroup only waits for chunks whose tickets were taken
-// earlier by workgroups that are already running, so every wait ends.
+// Continuations are appended to the next queue with one atomic per wave (a
+// 64-bit ballot ranks the survivors, mbcnt gives each lane its offset).
+// Queue order only affects coherence, never results: a pixel receives one
+// path per sample and each path's arithmetic depends on its own state alone.
+// The C ABI's compaction (kCompactAbi) is order preserving instead, so its
+// output is a stable filter of the input: workgroups take chunk tickets in
+// order and chain their totals with a decoupled look-back (one flag word per
+// chunk).  A workgroup only waits for chunks whose tickets were taken earlier
+// by workgroups that are already running, so every wait ends.
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
@@ -260,17 +262,14 @@ __global__ void kPrimary(LaunchContext ctx) {
     storePath(ctx.queue[0], i, p);
 }
 
-__global__ void kReset(LaunchContext ctx, uint32_t nextSel, uint32_t flagCount) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
+__global__ void kReset(LaunchContext ctx, uint32_t nextSel) {
+    if (threadIdx.x == 0) {
         ctx.counters->count[nextSel] = 0;
-        ctx.counters->ticket = 0;
         ctx.counters->traceCursor = 0;
         ctx.counters->shadeCursor = 0;
         ctx.counters->shadowCount = 0;
         ctx.counters->shadowCursor = 0;
     }
-    for (uint32_t j = i; j < flagCount; j += gridDim.x * blockDim.x) ctx.chunkFlags[j] = 0;
 }
 
 __global__ void kTrace(LaunchContext ctx, uint32_t bounce, uint32_t sel) {
@@ -329,12 +328,13 @@ __device__ void storeShadow(const ShadowQueue& q, uint32_t s, const ShadowReques
     sfield(q, kSRz)[s] = __float_as_uint(path.radiance.z);
 }
 
-// Staged shading: each path is updated in place with its alive flag; shadow
-// requests go to the shadow queue; finished paths without a pending shadow
-// ray reach the framebuffer.
+// Staged shading: continuing paths are appended to the next queue, shadow
+// requests to the shadow queue; finished paths without a pending shadow ray
+// reach the framebuffer.
 __global__ void kShade(LaunchContext ctx, uint32_t sel) {
     uint32_t count = ctx.counters->count[sel];
     const PathQueue& q = ctx.queue[sel];
+    const PathQueue& next = ctx.queue[sel ^ 1u];
     while (true) {
         uint32_t base = claimWaveBatch(&ctx.counters->shadeCursor);
         if (base >= count) break;
@@ -347,51 +347,17 @@ __global__ void kShade(LaunchContext ctx, uint32_t sel) {
         if (valid) {
             path = loadPath(q, i);
             shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, outcome);
-            ctx.alive[i] = outcome.alive ? 1 : 0;
-            if (outcome.alive) storePath(q, i, path);
         }
+        bool alive = valid && outcome.alive;
+        uint32_t slot = waveAppend(alive, &ctx.counters->count[sel ^ 1u]);
+        if (alive) storePath(next, slot, path);
         bool wantsShadow = valid && outcome.shadow.valid;
         uint32_t s = waveAppend(wantsShadow, &ctx.counters->shadowCount);
         if (wantsShadow) {
-            storeShadow(ctx.shadow, s, outcome.shadow, outcome.alive ? i : kShadowTerminated, path);
-        } else if (valid && !outcome.alive) {
+            storeShadow(ctx.shadow, s, outcome.shadow, alive ? slot : kShadowTerminated, path);
+        } else if (valid && !alive) {
             accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
         }
-    }
-}
-
-__device__ void copyPath(const PathQueue& from, uint32_t i, const PathQueue& to, uint32_t j) {
-#pragma unroll
-    for (uint32_t f = 0; f < kQueueFieldCount; ++f) field(to, f)[j] = field(from, f)[i];
-}
-
-// Ordered compaction of one chunk of flagged paths into the next queue.
-__device__ void compactChunk(const LaunchContext& ctx, uint32_t sel, uint32_t count, uint32_t chunk,
-    CompactShared& sh) {
-    const PathQueue& in = ctx.queue[sel];
-    const PathQueue& out = ctx.queue[sel ^ 1u];
-    uint32_t i = chunk * blockDim.x + threadIdx.x;
-    bool alive = i < count && ctx.alive[i] != 0;
-    uint32_t slot;
-    uint32_t inclusive = orderedSlots(alive, chunk, ctx.chunkFlags, sh, slot);
-    if (alive && slot < out.capacity) copyPath(in, i, out, slot);
-    if (threadIdx.x == 0) {
-        uint32_t chunks = (count + blockDim.x - 1) / blockDim.x;
-        if (chunk + 1 == chunks) storeAgent(&ctx.counters->count[sel ^ 1u], inclusive);
-    }
-}
-
-__global__ void kCompact(LaunchContext ctx, uint32_t sel) {
-    __shared__ CompactShared sh;
-    uint32_t count = ctx.counters->count[sel];
-    uint32_t chunks = (count + blockDim.x - 1) / blockDim.x;
-    while (true) {
-        if (threadIdx.x == 0) sh.chunk = atomicAdd(&ctx.counters->ticket, 1u);
-        __syncthreads();
-        uint32_t chunk = sh.chunk;
-        __syncthreads();
-        if (chunk >= chunks) break;
-        compactChunk(ctx, sel, count, chunk, sh);
     }
 }
 
@@ -403,7 +369,7 @@ __global__ void kShadow(LaunchContext ctx, uint32_t sel) {
         atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(count));
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const ShadowQueue& q = ctx.shadow;
-    const PathQueue& next = ctx.queue[sel];
+    const PathQueue& next = ctx.queue[sel ^ 1u];
     while (true) {
         uint32_t base = claimWaveBatch(&ctx.counters->shadowCursor);
         if (base >= count) break;
@@ -438,78 +404,80 @@ __global__ void kAdvance(LaunchContext ctx) {
     if (ctx.tile->advanceSample) ++ctx.counters->sampleIndex;
 }
 
-// Persistent kernel: per bounce, a fused trace/shade/shadow pass updates
-// paths in place, then the ordered compaction pass builds the next queue.
+// Persistent kernel: one grid-wide barrier per bounce.  Each wave claims 64
+// paths, traces, shades and resolves the shadow ray inline, and appends the
+// continuations to the other queue.  Bounce b reads count[b % 3], appends to
+// count[(b + 1) % 3] and clears count[(b + 2) % 3] and cursor[(b + 1) % 3],
+// which no block touches again before the barrier that ends bounce b.
 __global__ void kPersistent(LaunchContext ctx) {
-    __shared__ CompactShared sh;
     cg::grid_group grid = cg::this_grid();
+    WavefrontCounters* counters = ctx.counters;
     RenderTile tile = *ctx.tile;
     uint32_t pixels = tile.width * tile.height;
     uint32_t capacity = ctx.queue[0].capacity;
     uint32_t count0 = pixels < capacity ? pixels : capacity;
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t sample = loadAgent(&ctx.counters->sampleIndex);
+    uint32_t sample = loadAgent(&counters->sampleIndex);
     for (uint32_t i = gid; i < count0; i += stride) {
         uint32_t x, y;
         tilePixel(tile, i, x, y);
         storePath(ctx.queue[0], i, primaryPath(ctx.camera, x, y, ctx.width, ctx.height, sample, ctx.shade.rngStrategy));
     }
-    if (gid == 0) storeAgent(&ctx.counters->count[0], count0);
+    if (gid == 0) {
+        storeAgent(&counters->count[0], count0);
+        storeAgent(&counters->count[1], 0u);
+        storeAgent(&counters->cursor[0], 0u);
+    }
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     grid.sync();
-    uint32_t sel = 0;
+    uint32_t c = 0;
     for (uint32_t bounce = 0; bounce < ctx.shade.maxDepth; ++bounce) {
-        uint32_t count = loadAgent(&ctx.counters->count[sel]);
+        uint32_t sel = bounce & 1u;
+        uint32_t c1 = c == 2u ? 0u : c + 1u;
+        uint32_t c2 = c1 == 2u ? 0u : c1 + 1u;
+        uint32_t count = loadAgent(&counters->count[c]);
         if (count == 0) break;
-        uint32_t chunks = (count + blockDim.x - 1) / blockDim.x;
         if (gid == 0) {
-            atomicAdd(&ctx.counters->closestQueries, static_cast<unsigned long long>(count));
-            if (ctx.profileQueues && bounce < 64) ctx.counters->queueCounts[bounce] = count;
-            storeAgent(&ctx.counters->count[sel ^ 1u], 0u);
-            storeAgent(&ctx.counters->ticket, 0u);
-            storeAgent(&ctx.counters->shadeCursor, 0u);
+            atomicAdd(&counters->closestQueries, static_cast<unsigned long long>(count));
+            if (ctx.profileQueues && bounce < 64) counters->queueCounts[bounce] = count;
+            storeAgent(&counters->count[c2], 0u);
+            storeAgent(&counters->cursor[c1], 0u);
         }
-        for (uint32_t j = gid; j < chunks; j += stride) storeAgent(&ctx.chunkFlags[j], 0u);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-        grid.sync();
         const PathQueue& q = ctx.queue[sel];
+        const PathQueue& next = ctx.queue[sel ^ 1u];
         uint32_t shadowQueries = 0;
         while (true) {
-            uint32_t base = claimWaveBatch(&ctx.counters->shadeCursor);
+            uint32_t base = claimWaveBatch(&counters->cursor[c]);
             if (base >= count) break;
             uint32_t i = base + __lane_id();
-            if (i >= count) continue;
-            PathState path = loadPath(q, i);
-            HitRecord h = traceClosest(ctx.scene, path.ray, stack);
+            bool valid = i < count;
+            PathState path;
             ShadeOutcome outcome;
-            shadePath(ctx.scene, path, h, ctx.shade, outcome);
-            if (outcome.shadow.valid) {
-                ++shadowQueries;
-                Ray ray(outcome.shadow.origin, outcome.shadow.direction);
-                if (!traceOccluded(ctx.scene, ray, outcome.shadow.tmax, stack)) path.radiance += outcome.shadow.contribution;
+            outcome.alive = false;
+            if (valid) {
+                path = loadPath(q, i);
+                HitRecord h = traceClosest(ctx.scene, path.ray, stack);
+                shadePath(ctx.scene, path, h, ctx.shade, outcome);
+                if (outcome.shadow.valid) {
+                    ++shadowQueries;
+                    Ray ray(outcome.shadow.origin, outcome.shadow.direction);
+                    if (!traceOccluded(ctx.scene, ray, outcome.shadow.tmax, stack))
+                        path.radiance += outcome.shadow.contribution;
+                }
             }
-            ctx.alive[i] = outcome.alive ? 1 : 0;
-            if (outcome.alive)
-                storePath(q, i, path);
-            else
+            bool alive = valid && outcome.alive;
+            uint32_t slot = waveAppend(alive, &counters->count[c1]);
+            if (alive)
+                storePath(next, slot, path);
+            else if (valid)
                 accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
         }
-        if (shadowQueries) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(shadowQueries));
+        if (shadowQueries) atomicAdd(&counters->shadowQueries, static_cast<unsigned long long>(shadowQueries));
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
         grid.sync();
-        while (true) {
-            if (threadIdx.x == 0) sh.chunk = atomicAdd(&ctx.counters->ticket, 1u);
-            __syncthreads();
-            uint32_t chunk = sh.chunk;
-            __syncthreads();
-            if (chunk >= chunks) break;
-            compactChunk(ctx, sel, count, chunk, sh);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-        grid.sync();
-        sel ^= 1u;
+        c = c1;
     }
 }
 
@@ -573,22 +541,17 @@ namespace {
 // One staged sample; `timer` (optional) brackets every launch with events.
 template <class Timer>
 hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& timer) {
-    uint32_t capacity = ctx.queue[0].capacity;
-    uint32_t flagCount = ceilDiv(capacity, ctx.blockSize);
-    uint32_t resetBlocks = ceilDiv(flagCount, 256u);
     size_t lds = stackLdsBytes(ctx.blockSize);
     hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx); });
     for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {
         uint32_t sel = bounce & 1u;
-        e = timer(1, [&] { hipLaunchKernelGGL(kReset, dim3(resetBlocks), dim3(256), 0, stream, ctx, sel ^ 1u, flagCount); });
+        e = timer(1, [&] { hipLaunchKernelGGL(kReset, dim3(1), dim3(64), 0, stream, ctx, sel ^ 1u); });
         if (e == hipSuccess)
             e = timer(2, [&] { hipLaunchKernelGGL(kTrace, dim3(ctx.traceBlocks), dim3(ctx.blockSize), lds, stream, ctx, bounce, sel); });
         if (e == hipSuccess)
             e = timer(3, [&] { hipLaunchKernelGGL(kShade, dim3(ctx.shadeBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel); });
         if (e == hipSuccess && ctx.scene.lightCount != 0 && !ctx.scene.ignoreDirectLightOcclusion)
             e = timer(5, [&] { hipLaunchKernelGGL(kShadow, dim3(ctx.traceBlocks), dim3(ctx.blockSize), lds, stream, ctx, sel); });
-        if (e == hipSuccess)
-            e = timer(6, [&] { hipLaunchKernelGGL(kCompact, dim3(ctx.shadeBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel); });
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
@@ -617,7 +580,6 @@ hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out) {
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace, b, lds);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade, b, 0);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow, b, lds);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->compact, kCompact, b, 0);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent, b, lds);
     return e;
 }
