@@ -70,6 +70,92 @@ def cpu_baseline(scene, camera, threads, sample_spp):
     }, st
 
 
+def batch_workload(n_tasks, seed=2024):
+    """Batching workload: four resident scenes -- the Cornell box, the sphere
+    field with its 1,003,520-triangle icosphere, the dielectric prism room
+    (standing in for the shading-heavy scene, whose textures, environment and
+    photon map are not built) and a sphere-field variant whose icosphere has
+    200,000 triangles (frequency 100) -- and `n_tasks` square tiles of side
+    {32, 64, 128, 256, 512} at seeded positions of a 1920x1080 frame, with
+    spp uniform in [1, 64], mixed across the scenes."""
+    specs = [("cornell-box", 0, 8), ("sphere-field", 0, 12), ("prism", 0, 10), ("sphere-field", 100, 12)]
+    hosts = [wpt.HostScene(name, icosphere_frequency=freq) for name, freq, _ in specs]
+    scenes = [wpt.Scene(h.desc(), device=True, owner=h) for h in hosts]
+    cams = [h.camera(WIDTH, HEIGHT) for h in hosts]
+    rng = np.random.default_rng(seed)
+    tasks = []
+    for _ in range(n_tasks):
+        s = int(rng.integers(len(specs)))
+        side = int(rng.choice([32, 64, 128, 256, 512]))
+        x, y = int(rng.integers(0, WIDTH - side + 1)), int(rng.integers(0, HEIGHT - side + 1))
+        spp = int(rng.integers(1, 65))
+        tasks.append(wpt.batch_task(s, cams[s], WIDTH, HEIGHT, x, y, side, side, 0, spp, max_depth=specs[s][2],
+                                    rr_start=RR_START, rng=wpt.RNG_PCG32, clamp=CLAMP))
+    return hosts, scenes, tasks
+
+
+def measure_batch(n_tasks):
+    hosts, scenes, tasks = batch_workload(n_tasks)
+    samples = sum(t.tile_width * t.tile_height * t.sample_count for t in tasks)
+    batch = wpt.Batch(scenes)
+    # Autotune on the first 32 tasks at up to 4 spp, then render everything.
+    probe = [wpt.batch_task(t.scene, t.camera, t.width, t.height, t.x, t.y, t.tile_width, t.tile_height, 0,
+                            min(4, t.sample_count), t.max_depth, t.rr_start_depth, t.rng_strategy, t.indirect_clamp)
+             for t in tasks[:32]]
+    ta = time.perf_counter()
+    chosen = batch.autotune(probe)
+    tune_s = time.perf_counter() - ta
+    log(f"batch: autotuned in {tune_s:.1f} s -> block {chosen.block_size}, grid {chosen.blocks}")
+    batch.render(tasks[:8])  # warm-up
+    tb = time.perf_counter()
+    outs, lat = batch.render(tasks, latencies=True)
+    batch_s = time.perf_counter() - tb
+    bst = batch.stats()
+    log(f"batch: {n_tasks} tasks, {samples / 1e6:.1f} M samples in {batch_s * 1000:.1f} ms "
+        f"(device {bst.milliseconds:.1f} ms)")
+
+    # The same tasks one by one through wpt_render_task, one renderer per scene.
+    renderers = []
+    for i, sc in enumerate(scenes):
+        depth = max([t.max_depth for t in tasks if t.scene == i] or [8])
+        cfg = wpt.render_config(WIDTH, HEIGHT, spp=64, max_depth=depth, rr_start=RR_START, clamp=CLAMP)
+        renderers.append(wpt.Renderer(sc, hosts[i].camera(WIDTH, HEIGHT), cfg))
+    warm = wpt.Task(0, 0, 0, 1, 0, 0, 32, 32)
+    for r in renderers:
+        r.render_task(warm)
+    done = []
+    identical = True
+    tp = time.perf_counter()
+    for i, t in enumerate(tasks):
+        out = renderers[t.scene].render_task(wpt.Task(0, 0, t.sample_start, t.sample_count, t.x, t.y, t.tile_width,
+                                                      t.tile_height))
+        done.append((time.perf_counter() - tp) * 1000.0)
+        identical &= bool(np.array_equal(out, outs[i]))
+    plain_s = time.perf_counter() - tp
+    for r in renderers:
+        r.close()
+    lat_plain = np.array(done)
+    result = {
+        "workload": f"{n_tasks} tasks over 4 resident scenes (cornell-box, sphere-field 1.0M-tri icosphere, prism, "
+                    f"sphere-field 200k-tri icosphere); square tiles {{32..512}}, spp U[1,64], seed 2024",
+        "samples": int(samples),
+        "batched_msamples_per_s": round(samples / batch_s / 1e6, 2),
+        "plain_msamples_per_s": round(samples / plain_s / 1e6, 2),
+        "speedup": round(plain_s / batch_s, 3),
+        "batched_ms": round(batch_s * 1000.0, 2),
+        "plain_ms": round(plain_s * 1000.0, 2),
+        "batched_latency_ms_p50_p99": [round(float(np.percentile(lat, 50)), 2), round(float(np.percentile(lat, 99)), 2)],
+        "plain_latency_ms_p50_p99": [round(float(np.percentile(lat_plain, 50)), 2),
+                                     round(float(np.percentile(lat_plain, 99)), 2)],
+        "autotuned": {"block_size": int(chosen.block_size), "blocks": int(chosen.blocks),
+                      "tuning_s": round(tune_s, 2)},
+        "identical_to_plain": identical,
+    }
+    log(f"batch vs plain: {result}")
+    batch.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +167,11 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--cpu-spp", type=int, default=8, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--profile-spp", type=int, default=4, help="samples rendered with per-kernel events")
+    ap.add_argument("--skip-batch", action="store_true", help="do not measure batched vs one-by-one tasks")
+    ap.add_argument("--batch-tasks", type=int, default=256, help="tasks of the batching workload")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB three times: the reference kernel of "
+                         "known traffic against which the FETCH_SIZE/WRITE_SIZE counters are scaled")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -90,6 +181,17 @@ def main():
     if wpt.device_count() == 0:
         print("bench.py needs a gfx950 device", file=sys.stderr)
         sys.exit(1)
+
+    if args.calibration_copy_mib:
+        import torch
+        n = args.calibration_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: 3 x {args.calibration_copy_mib} MiB device-to-device")
+        del src, dst
 
     t0 = time.perf_counter()
     host = wpt.HostScene("sphere-field")
@@ -165,6 +267,10 @@ def main():
         cpu, work = cpu_baseline(scene, camera, threads, args.cpu_spp)
         log(f"cpu baseline: {cpu}")
 
+    batch = None
+    if not args.skip_batch:
+        batch = measure_batch(args.batch_tasks)
+
     roofline = None
     if work is not None and trace_ms > 0:
         per_query = (NODE_B * work.closest_node_tests + TRI_B * work.closest_triangle_tests +
@@ -207,6 +313,7 @@ def main():
         "queries_per_frame": int(queries),
         "device_ms_per_frame": round(float(np.mean(device_ms)), 2),
         "persistent": persistent,
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -304,9 +304,112 @@ WPT_API int wpt_renderer_last_stats(const wpt_renderer* renderer, wpt_stats* sta
  * HIP events on the renderer's stream (no graph), accumulating device time
  * per kernel into kernel_ms[7] and launch counts into launches[7]: 0 primary
  * rays, 1 queue reset, 2 closest-hit trace, 3 shade, 4 persistent, 5 shadow
- * rays, 6 queue compaction. */
+ * rays, 6 queue compaction (unused by the renderer, whose shade kernel
+ * appends continuations itself). */
 WPT_API int wpt_renderer_profile_kernels(wpt_renderer* renderer, uint32_t first_sample, uint32_t count,
                                          double* kernel_ms, uint32_t* launches);
+
+/* ---- batch: many tasks of several scenes in one persistent launch -------
+ * No reference counterpart: the reference renders its tasks one at a time
+ * (RenderTaskRenderer::render, render_task.cu:26-114); a batch takes the
+ * same tasks, of uneven tile size, sample range and scene, in one launch. */
+typedef struct wpt_batch wpt_batch;
+
+typedef struct wpt_batch_task {
+    uint32_t scene;            /* index into the scenes of wpt_batch_create */
+    wpt_camera camera;
+    uint32_t width, height;    /* frame the tile belongs to (pixel ids, RNG keys) */
+    uint32_t x, y, tile_width, tile_height;
+    uint32_t sample_start, sample_count;
+    uint32_t max_depth, rr_start_depth, rng_strategy;
+    float indirect_clamp;
+} wpt_batch_task;
+
+typedef struct wpt_batch_config {
+    uint32_t block_size;       /* multiple of 64, at most 512 */
+    uint32_t blocks;           /* persistent grid; 0 = every resident workgroup */
+} wpt_batch_config;
+
+typedef struct wpt_batch_stats {
+    uint64_t closest_queries;
+    uint64_t shadow_queries;
+    uint64_t samples;          /* pixel samples rendered */
+    double milliseconds;       /* device time of the launch */
+    uint32_t block_size, blocks;
+} wpt_batch_stats;
+
+/* The scenes must outlive the batch. */
+WPT_API int wpt_batch_create(wpt_scene* const* scenes, uint32_t scene_count, const wpt_batch_config* config,
+                             wpt_batch** out);
+WPT_API int wpt_batch_destroy(wpt_batch* batch);
+/* Renders every task in one launch.  outs[i] (host memory) receives task i's
+ * unnormalized tile sums, tile_width*tile_height*3 floats row-major, bit
+ * identical to wpt_render_task on the same task.  task_latency_ms, when not
+ * NULL, receives per task the time from the launch's start to the end of
+ * its last pixel. */
+WPT_API int wpt_batch_render(wpt_batch* batch, const wpt_batch_task* tasks, uint32_t n, float* const* outs,
+                             double* task_latency_ms);
+WPT_API int wpt_batch_last_stats(const wpt_batch* batch, wpt_batch_stats* stats);
+/* Autotuner: renders the tasks under each candidate workgroup size (64, 128,
+ * 256, 512) and persistent grid size (1/4, 1/2, 3/4 and all of the resident
+ * workgroups), keeps the fastest configuration for later renders and
+ * reports it in *chosen. */
+WPT_API int wpt_batch_autotune(wpt_batch* batch, const wpt_batch_task* tasks, uint32_t n, wpt_batch_config* chosen);
+
+/* ---- differentiable rendering: forward + path-replay adjoint -----------
+ * No reference counterpart.  Parameters form one float vector laid out as
+ *   [ texture: texture_height x texture_width x RGB (row j at v = (j+0.5)/h) |
+ *     roughness: one GGX alpha per reflective static triangle, in order |
+ *     emission: one scale per emissive static triangle, in order ].
+ * The texture is the albedo of the static triangles of `textured_material`
+ * (bilinear, clamp to edge).  Reflective static triangles are
+ * albedo/pi + a GGX lobe (F0 0.04) sampled by cosine; other diffuse surfaces
+ * are Lambertian; light selection and MIS do not depend on the parameters;
+ * no Russian roulette, no clamp.  Image: width x height x RGB, the mean over
+ * samples_per_pixel paths per pixel. */
+typedef struct wpt_diff wpt_diff;
+
+typedef struct wpt_diff_config {
+    uint32_t width, height;
+    uint32_t samples_per_pixel;
+    uint32_t max_depth;
+    uint32_t rng_strategy;
+    uint32_t texture_width, texture_height;
+    uint32_t textured_material;  /* WPT_INVALID_INDEX: no texture */
+    uint32_t block_size;         /* multiple of 64, at most 512 */
+} wpt_diff_config;
+
+typedef struct wpt_diff_layout {
+    uint32_t texture_floats;
+    uint32_t roughness_count;
+    uint32_t emission_count;
+    uint32_t parameter_count;
+} wpt_diff_layout;
+
+typedef struct wpt_diff_stats {
+    double forward_milliseconds;   /* device time of the last forward */
+    double adjoint_milliseconds;   /* device time of the last adjoint */
+    uint64_t paths;
+} wpt_diff_stats;
+
+/* The scene must be on the device and outlive the object. */
+WPT_API int wpt_diff_create(wpt_scene* scene, const wpt_camera* camera, const wpt_diff_config* config,
+                            wpt_diff** out);
+WPT_API int wpt_diff_destroy(wpt_diff* diff);
+WPT_API int wpt_diff_get_layout(const wpt_diff* diff, wpt_diff_layout* layout);
+/* Renders samples [sample_start, sample_start + spp) of every pixel with the
+ * parameters at d_params (device); writes the mean image to d_image
+ * (device) and keeps every path's radiance for the adjoint. */
+WPT_API int wpt_diff_forward(wpt_diff* diff, const float* d_params, uint32_t sample_start, float* d_image);
+/* Path-replay adjoint of the last forward (same d_params, unchanged since):
+ * d_param_grad (device, parameter_count floats) = dLoss/dparams given
+ * d_image_grad = dLoss/dimage (device). */
+WPT_API int wpt_diff_adjoint(wpt_diff* diff, const float* d_params, const float* d_image_grad, float* d_param_grad);
+/* One Adam step (t = step, from 1) on device vectors, then each section is
+ * clamped: texture [0, 1], roughness [0.02, 1], emission [0, inf). */
+WPT_API int wpt_diff_adam(wpt_diff* diff, float* d_params, const float* d_grad, float* d_m, float* d_v, uint32_t step,
+                          float learning_rate, float beta1, float beta2, float epsilon);
+WPT_API int wpt_diff_last_stats(const wpt_diff* diff, wpt_diff_stats* stats);
 
 #ifdef __cplusplus
 }

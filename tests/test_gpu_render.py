@@ -94,6 +94,21 @@ def test_render_task_sums(torch_cuda, oracle_mod):
             r.render_task(bad)
 
 
+def test_render_task_on_untiled_renderer(torch_cuda, oracle_mod):
+    # A task's tile anywhere inside an untiled renderer's frame, before and
+    # after a full-frame render on the same renderer.
+    host, scene, cam, cfg = setup("cornell-box", 64, 48, 6, 8, 3)
+    r = wpt.Renderer(scene, cam, cfg)
+    oracle = oracle_mod.Oracle(scene)
+    for x, y, w, h, start, count in ((5, 7, 20, 13, 1, 3), (0, 0, 64, 48, 0, 6), (40, 30, 24, 18, 2, 4)):
+        got = r.render_task(wpt.Task(0, 0, start, count, x, y, w, h))
+        want, _ = oracle.render(cam, cfg, tile=(x, y, w, h), sample_start=start, sample_count=count)
+        assert np.array_equal(got, want[y:y + h, x:x + w]), (x, y, w, h)
+        if start == 0:
+            full, n = r.render_frame()
+            assert n == 6 and np.array_equal(full, want)
+
+
 def test_progressive_samples_accumulate(torch_cuda):
     host, scene, cam, cfg = setup("cornell-box", 64, 64, 4, 8, 3)
     r = wpt.Renderer(scene, cam, cfg)

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../../include/wpt.h"
+#include "../kernels/diff_kernels.h"
 #include "../kernels/wavefront.h"
 #include "../scene/scene_build.h"
 
@@ -626,7 +627,7 @@ WPT_API int wpt_render_task(wpt_renderer* r, const wpt_task* task, float* hostTi
     hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, task->sample_start);
     WPT_HIP(hipGetLastError());
     RenderTile tile{task->x, task->y, task->width, task->height, 1u};
-    r->currentTile = 0xfffffffeu;  // force the tile upload
+    r->currentTile = 0xffffffffu;  // no tile uploaded: the first sample uploads the task's tile
     WPT_HIP(hipEventRecord(r->start, r->stream));
     for (uint32_t s = 0; s < task->sample_count; ++s) {
         status = launchTileSample(r, 0xfffffffeu, tile);
@@ -679,6 +680,498 @@ WPT_API int wpt_renderer_last_stats(const wpt_renderer* r, wpt_stats* stats) {
     uint32_t depth = std::min<uint32_t>(64u, r->config.max_depth);
     for (uint32_t i = 0; i < depth; ++i) stats->queue_counts[i] = c.queueCounts[i];
     stats->queue_count_depth = depth;
+    return WPT_OK;
+}
+
+}  // extern "C"
+
+// ---- batch ---------------------------------------------------------------
+
+struct wpt_batch {
+    std::vector<wpt_scene*> scenes;
+    wpt_batch_config config{};
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, end = nullptr;
+    DevScene* scenesDev = nullptr;
+    BatchCounters* counters = nullptr;
+    BatchTask* tasks = nullptr;
+    BatchTaskState* state = nullptr;
+    uint32_t taskCapacity = 0;
+    float* out = nullptr;
+    size_t outCapacity = 0;  // floats
+    uint2* spill = nullptr;
+    uint32_t spillThreads = 0;
+    int cus = 0;
+    int wallClockKhz = 0;
+    wpt_batch_stats last{};
+    ~wpt_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : {static_cast<void*>(scenesDev), static_cast<void*>(counters), static_cast<void*>(tasks),
+                 static_cast<void*>(state), static_cast<void*>(out), static_cast<void*>(spill)})
+            if (p) (void)hipFree(p);
+        if (start) (void)hipEventDestroy(start);
+        if (end) (void)hipEventDestroy(end);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int validateBatchConfig(const wpt_batch_config& c) {
+    if (c.block_size == 0 || c.block_size > 512 || c.block_size % 64 != 0)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Block size must be a multiple of 64 between 64 and 512");
+    return WPT_OK;
+}
+
+// Grows a device buffer to hold `count` elements; contents are not kept.
+template <class T>
+int growDevice(T** p, size_t count, size_t* capacity) {
+    if (count <= *capacity) return WPT_OK;
+    if (*p) WPT_HIP(hipFree(*p));
+    *p = nullptr;
+    *capacity = 0;
+    WPT_HIP(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count));
+    *capacity = count;
+    return WPT_OK;
+}
+
+// Resident workgroups of the batch kernel at `blockSize`, and the grid a
+// configuration asks for.
+int batchGrid(wpt_batch* b, const wpt_batch_config& c, uint32_t* blocks) {
+    int perCu = 0;
+    WPT_HIP(batchOccupancy(c.block_size, &perCu));
+    if (perCu <= 0) return fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the batch kernel");
+    uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(b->cus);
+    *blocks = c.blocks ? std::min(c.blocks, resident) : resident;
+    return WPT_OK;
+}
+
+// Validates the tasks and uploads them; returns the float offsets of each
+// task's sums in b->out and the total.
+int stageBatch(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, std::vector<size_t>& offsets, uint64_t* samples) {
+    std::vector<BatchTask> host(n);
+    offsets.assign(n, 0);
+    size_t total = 0;
+    *samples = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const wpt_batch_task& t = tasks[i];
+        if (t.scene >= b->scenes.size()) return fail(WPT_ERROR_INVALID_TASK, "Batch task " + std::to_string(i) + ": scene index out of range");
+        if (t.width == 0 || t.height == 0 || static_cast<uint64_t>(t.width) * t.height > std::numeric_limits<uint32_t>::max())
+            return fail(WPT_ERROR_INVALID_TASK, "Batch task " + std::to_string(i) + ": invalid frame dimensions");
+        if (t.tile_width == 0 || t.tile_height == 0 || t.x >= t.width || t.y >= t.height ||
+            t.tile_width > t.width - t.x || t.tile_height > t.height - t.y)
+            return fail(WPT_ERROR_INVALID_TASK, "Batch task " + std::to_string(i) + ": tile is outside the frame");
+        if (t.max_depth == 0 || t.max_depth > 0x7fffffffu)
+            return fail(WPT_ERROR_INVALID_TASK, "Batch task " + std::to_string(i) + ": maximum depth out of range");
+        if (t.rng_strategy > WPT_RNG_PCG32) return fail(WPT_ERROR_INVALID_TASK, "Batch task " + std::to_string(i) + ": unknown RNG strategy");
+        if (!std::isfinite(t.indirect_clamp) || t.indirect_clamp < 0.0f)
+            return fail(WPT_ERROR_INVALID_TASK, "Batch task " + std::to_string(i) + ": indirect clamp must be finite and nonnegative");
+        if (static_cast<uint64_t>(t.sample_start) + t.sample_count > 0xffffffffull)
+            return fail(WPT_ERROR_INVALID_TASK, "Batch task " + std::to_string(i) + ": sample range out of range");
+        BatchTask& d = host[i];
+        d.camera.position = Vec3(t.camera.position[0], t.camera.position[1], t.camera.position[2]);
+        d.camera.forward = Vec3(t.camera.forward[0], t.camera.forward[1], t.camera.forward[2]);
+        d.camera.right = Vec3(t.camera.right[0], t.camera.right[1], t.camera.right[2]);
+        d.camera.up = Vec3(t.camera.up[0], t.camera.up[1], t.camera.up[2]);
+        d.camera.verticalFov = t.camera.vertical_fov;
+        d.camera.aspectRatio = t.camera.aspect_ratio;
+        d.shade = ShadeParams{t.max_depth, t.rr_start_depth, t.rng_strategy, t.indirect_clamp, 0u, 0u};
+        d.tile = RenderTile{t.x, t.y, t.tile_width, t.tile_height, 0u};
+        d.scene = t.scene;
+        d.width = t.width;
+        d.height = t.height;
+        d.sampleStart = t.sample_start;
+        d.sampleCount = t.sample_count;
+        uint32_t pixels = t.tile_width * t.tile_height;
+        d.pixelCount = t.sample_count ? pixels : 0u;
+        d.out = total;
+        offsets[i] = total;
+        total += 3ull * pixels;
+        *samples += static_cast<uint64_t>(pixels) * t.sample_count;
+    }
+    size_t cap = b->taskCapacity;
+    int s = growDevice(&b->tasks, std::max<size_t>(n, 1), &cap);
+    if (s != WPT_OK) return s;
+    cap = b->taskCapacity;
+    s = growDevice(&b->state, std::max<size_t>(n, 1), &cap);
+    if (s != WPT_OK) return s;
+    b->taskCapacity = static_cast<uint32_t>(cap);
+    s = growDevice(&b->out, std::max<size_t>(total, 1), &b->outCapacity);
+    if (s != WPT_OK) return s;
+    if (n) WPT_HIP(hipMemcpy(b->tasks, host.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice));
+    // Tasks without samples still report zero sums.
+    WPT_HIP(hipMemsetAsync(b->out, 0, sizeof(float) * std::max<size_t>(total, 1), b->stream));
+    return WPT_OK;
+}
+
+// One launch over the staged tasks under configuration `c`; device time in *ms.
+int runBatch(wpt_batch* b, uint32_t n, const wpt_batch_config& c, float* ms) {
+    uint32_t blocks = 0;
+    int s = batchGrid(b, c, &blocks);
+    if (s != WPT_OK) return s;
+    uint32_t threads = blocks * c.block_size;
+    if (threads > b->spillThreads) {
+        if (b->spill) WPT_HIP(hipFree(b->spill));
+        b->spill = nullptr;
+        b->spillThreads = 0;
+        WPT_HIP(hipMalloc(reinterpret_cast<void**>(&b->spill), sizeof(uint2) * kSpillEntriesTotal * static_cast<size_t>(threads)));
+        b->spillThreads = threads;
+    }
+    BatchContext ctx{};
+    ctx.scenes = b->scenesDev;
+    ctx.tasks = b->tasks;
+    ctx.state = b->state;
+    ctx.counters = b->counters;
+    ctx.taskCount = n;
+    ctx.out = b->out;
+    ctx.spill = b->spill;
+    ctx.spillThreads = b->spillThreads;
+    WPT_HIP(hipEventRecord(b->start, b->stream));
+    WPT_HIP(enqueueBatch(ctx, blocks, c.block_size, b->stream));
+    WPT_HIP(hipEventRecord(b->end, b->stream));
+    WPT_HIP(hipStreamSynchronize(b->stream));
+    WPT_HIP(hipEventElapsedTime(ms, b->start, b->end));
+    b->last.block_size = c.block_size;
+    b->last.blocks = blocks;
+    return WPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+WPT_API int wpt_batch_create(wpt_scene* const* scenes, uint32_t sceneCount, const wpt_batch_config* config,
+    wpt_batch** out) {
+    if (!out || !config || (sceneCount && !scenes)) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (sceneCount == 0) return fail(WPT_ERROR_INVALID_ARGUMENT, "a batch needs at least one scene");
+    int v = validateBatchConfig(*config);
+    if (v != WPT_OK) return v;
+    std::vector<DevScene> dev(sceneCount);
+    for (uint32_t i = 0; i < sceneCount; ++i) {
+        if (!scenes[i]) return fail(WPT_ERROR_INVALID_ARGUMENT, "null scene");
+        if (!scenes[i]->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not on a device");
+        dev[i] = scenes[i]->dev;
+    }
+    wpt_batch* b = new (std::nothrow) wpt_batch();
+    if (!b) return fail(WPT_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    b->scenes.assign(scenes, scenes + sceneCount);
+    b->config = *config;
+    auto cleanup = [&](int status) {
+        delete b;
+        return status;
+    };
+    hipError_t e;
+#define B_HIP(call)                                            \
+    do {                                                       \
+        e = (call);                                            \
+        if (e != hipSuccess) return cleanup(hipFail(e, #call)); \
+    } while (0)
+    int device = 0;
+    B_HIP(hipGetDevice(&device));
+    B_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
+    B_HIP(hipDeviceGetAttribute(&b->wallClockKhz, hipDeviceAttributeWallClockRate, device));
+    B_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    B_HIP(hipEventCreate(&b->start));
+    B_HIP(hipEventCreate(&b->end));
+    B_HIP(hipMalloc(reinterpret_cast<void**>(&b->scenesDev), sizeof(DevScene) * sceneCount));
+    B_HIP(hipMemcpy(b->scenesDev, dev.data(), sizeof(DevScene) * sceneCount, hipMemcpyHostToDevice));
+    B_HIP(hipMalloc(reinterpret_cast<void**>(&b->counters), sizeof(BatchCounters)));
+#undef B_HIP
+    uint32_t blocks = 0;
+    int s = batchGrid(b, b->config, &blocks);
+    if (s != WPT_OK) return cleanup(s);
+    *out = b;
+    return WPT_OK;
+}
+
+WPT_API int wpt_batch_destroy(wpt_batch* b) {
+    delete b;
+    return WPT_OK;
+}
+
+WPT_API int wpt_batch_render(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, float* const* outs,
+    double* latencyMs) {
+    if (!b || (n && (!tasks || !outs))) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    for (uint32_t i = 0; i < n; ++i)
+        if (!outs[i]) return fail(WPT_ERROR_INVALID_ARGUMENT, "null task output");
+    std::vector<size_t> offsets;
+    uint64_t samples = 0;
+    int s = stageBatch(b, tasks, n, offsets, &samples);
+    if (s != WPT_OK) return s;
+    float ms = 0.0f;
+    s = runBatch(b, n, b->config, &ms);
+    if (s != WPT_OK) return s;
+    BatchCounters counters{};
+    WPT_HIP(hipMemcpy(&counters, b->counters, sizeof counters, hipMemcpyDeviceToHost));
+    b->last.closest_queries = counters.closestQueries;
+    b->last.shadow_queries = counters.shadowQueries;
+    b->last.samples = samples;
+    b->last.milliseconds = ms;
+    size_t total = 0;
+    for (uint32_t i = 0; i < n; ++i) total += 3ull * tasks[i].tile_width * tasks[i].tile_height;
+    std::vector<float> host(total);
+    if (total) WPT_HIP(hipMemcpy(host.data(), b->out, sizeof(float) * total, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i)
+        std::memcpy(outs[i], host.data() + offsets[i], sizeof(float) * 3ull * tasks[i].tile_width * tasks[i].tile_height);
+    if (latencyMs && n) {
+        std::vector<BatchTaskState> state(n);
+        WPT_HIP(hipMemcpy(state.data(), b->state, sizeof(BatchTaskState) * n, hipMemcpyDeviceToHost));
+        double khz = b->wallClockKhz > 0 ? static_cast<double>(b->wallClockKhz) : 1.0;
+        for (uint32_t i = 0; i < n; ++i)
+            latencyMs[i] = state[i].finishClock > counters.startClock
+                               ? static_cast<double>(state[i].finishClock - counters.startClock) / khz
+                               : 0.0;
+    }
+    return WPT_OK;
+}
+
+WPT_API int wpt_batch_last_stats(const wpt_batch* b, wpt_batch_stats* stats) {
+    if (!b || !stats) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    *stats = b->last;
+    return WPT_OK;
+}
+
+WPT_API int wpt_batch_autotune(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, wpt_batch_config* chosen) {
+    if (!b || (n && !tasks)) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    std::vector<size_t> offsets;
+    uint64_t samples = 0;
+    int s = stageBatch(b, tasks, n, offsets, &samples);
+    if (s != WPT_OK) return s;
+    wpt_batch_config best = b->config;
+    float bestMs = std::numeric_limits<float>::infinity();
+    for (uint32_t block : {64u, 128u, 256u, 512u}) {
+        int perCu = 0;
+        WPT_HIP(batchOccupancy(block, &perCu));
+        if (perCu <= 0) continue;
+        uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(b->cus);
+        for (uint32_t quarter = 1; quarter <= 4; ++quarter) {
+            wpt_batch_config c{block, std::max(1u, resident * quarter / 4u)};
+            float ms = 0.0f;
+            s = runBatch(b, n, c, &ms);
+            if (s != WPT_OK) return s;
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = c;
+            }
+        }
+    }
+    b->config = best;
+    if (chosen) *chosen = best;
+    return WPT_OK;
+}
+
+}  // extern "C"
+
+// ---- differentiable rendering ---------------------------------------------
+
+struct wpt_diff {
+    wpt_scene* scene = nullptr;
+    wpt_diff_config config{};
+    hipStream_t stream = nullptr;
+    hipEvent_t marks[4] = {nullptr, nullptr, nullptr, nullptr};
+    DeviceArena arena;
+    DiffContext ctx{};
+    uint32_t forwardBlocks = 0, adjointBlocks = 0;
+    uint32_t textureFloats = 0, roughnessCount = 0, emissionCount = 0;
+    const float* forwardParams = nullptr;  // parameters of the kept path radiance
+    wpt_diff_stats stats{};
+    ~wpt_diff() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : marks)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+WPT_API int wpt_diff_create(wpt_scene* scene, const wpt_camera* camera, const wpt_diff_config* config, wpt_diff** out) {
+    if (!scene || !camera || !config || !out) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (!scene->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not on a device");
+    const wpt_diff_config& c = *config;
+    if (c.width == 0 || c.height == 0 || c.samples_per_pixel == 0 || c.max_depth == 0)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "image size, samples per pixel and depth must be greater than zero");
+    if (c.block_size == 0 || c.block_size > 512 || c.block_size % 64 != 0)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Block size must be a multiple of 64 between 64 and 512");
+    if (c.rng_strategy > WPT_RNG_PCG32) return fail(WPT_ERROR_INVALID_ARGUMENT, "Unknown RNG strategy");
+    uint64_t paths = static_cast<uint64_t>(c.width) * c.height * c.samples_per_pixel;
+    if (paths >= (1ull << 31)) return fail(WPT_ERROR_INVALID_ARGUMENT, "too many paths per forward pass");
+    bool textured = c.textured_material != WPT_INVALID_INDEX;
+    if (textured && (c.texture_width == 0 || c.texture_height == 0 || c.textured_material >= scene->host.materials.size()))
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "texture needs a size and an existing material");
+    if (textured && static_cast<uint64_t>(c.texture_width) * c.texture_height * 3 >= (1ull << 31))
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "texture too large");
+    wpt_diff* d = new (std::nothrow) wpt_diff();
+    if (!d) return fail(WPT_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    d->scene = scene;
+    d->config = c;
+    auto cleanup = [&](int status) {
+        delete d;
+        return status;
+    };
+    // Parameter slots of the static triangles and lights.
+    const HostSceneData& h = scene->host;
+    std::vector<uint32_t> staticSlot(h.staticTriangles.size(), kInvalid);
+    std::vector<uint32_t> lightSlot(std::max<size_t>(h.lights.size(), 1), kInvalid);
+    for (size_t i = 0; i < h.staticTriangles.size(); ++i) {
+        const TriShade& t = h.staticTriangles[i];
+        uint32_t type = h.materials[t.material].type;
+        if (type == kEmissive) {
+            staticSlot[i] = d->emissionCount;
+            if (t.lightIndex != kInvalid) lightSlot[t.lightIndex] = d->emissionCount;
+            ++d->emissionCount;
+        } else if (type != kDielectric) {
+            staticSlot[i] = d->roughnessCount++;
+        }
+    }
+    d->textureFloats = textured ? c.texture_width * c.texture_height * 3 : 0;
+    hipError_t e;
+#define D_HIP(call)                                            \
+    do {                                                       \
+        e = (call);                                            \
+        if (e != hipSuccess) return cleanup(hipFail(e, #call)); \
+    } while (0)
+    D_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    for (hipEvent_t& ev : d->marks) D_HIP(hipEventCreate(&ev));
+    DiffContext& x = d->ctx;
+    x.scene = scene->dev;
+    x.camera.position = Vec3(camera->position[0], camera->position[1], camera->position[2]);
+    x.camera.forward = Vec3(camera->forward[0], camera->forward[1], camera->forward[2]);
+    x.camera.right = Vec3(camera->right[0], camera->right[1], camera->right[2]);
+    x.camera.up = Vec3(camera->up[0], camera->up[1], camera->up[2]);
+    x.camera.verticalFov = camera->vertical_fov;
+    x.camera.aspectRatio = camera->aspect_ratio;
+    x.layout.texWidth = textured ? c.texture_width : 0;
+    x.layout.texHeight = textured ? c.texture_height : 0;
+    x.layout.texturedMaterial = textured ? c.textured_material : kInvalid;
+    x.layout.roughnessCount = d->roughnessCount;
+    x.layout.emissionCount = d->emissionCount;
+    const uint32_t* p = nullptr;
+    D_HIP(d->arena.upload(staticSlot, &p));
+    x.layout.staticSlot = p;
+    D_HIP(d->arena.upload(lightSlot, &p));
+    x.layout.lightSlot = p;
+    x.width = c.width;
+    x.height = c.height;
+    x.spp = c.samples_per_pixel;
+    x.maxDepth = c.max_depth;
+    x.rngStrategy = c.rng_strategy;
+    void* mem = nullptr;
+    D_HIP(d->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(paths), &mem));
+    x.pathRadiance = static_cast<float*>(mem);
+    D_HIP(d->arena.alloc(sizeof(uint32_t), &mem));
+    x.cursor = static_cast<uint32_t*>(mem);
+    int forwardPerCu = 0, adjointPerCu = 0, device = 0, cus = 0;
+    D_HIP(diffOccupancy(c.block_size, &forwardPerCu, &adjointPerCu));
+    if (forwardPerCu <= 0 || adjointPerCu <= 0)
+        return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the differentiable kernels"));
+    D_HIP(hipGetDevice(&device));
+    D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    uint64_t pathBlocks = (paths + c.block_size - 1) / c.block_size;
+    d->forwardBlocks = static_cast<uint32_t>(std::min<uint64_t>(pathBlocks, static_cast<uint64_t>(forwardPerCu) * cus));
+    d->adjointBlocks = static_cast<uint32_t>(std::min<uint64_t>(pathBlocks, static_cast<uint64_t>(adjointPerCu) * cus));
+    x.spillThreads = std::max(d->forwardBlocks, d->adjointBlocks) * c.block_size;
+    D_HIP(d->arena.alloc(sizeof(uint2) * kSpillEntriesTotal * static_cast<size_t>(x.spillThreads), &mem));
+    x.spill = static_cast<uint2*>(mem);
+#undef D_HIP
+    d->stats.paths = paths;
+    *out = d;
+    return WPT_OK;
+}
+
+WPT_API int wpt_diff_destroy(wpt_diff* d) {
+    delete d;
+    return WPT_OK;
+}
+
+WPT_API int wpt_diff_get_layout(const wpt_diff* d, wpt_diff_layout* layout) {
+    if (!d || !layout) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    layout->texture_floats = d->textureFloats;
+    layout->roughness_count = d->roughnessCount;
+    layout->emission_count = d->emissionCount;
+    layout->parameter_count = d->textureFloats + d->roughnessCount + d->emissionCount;
+    return WPT_OK;
+}
+
+WPT_API int wpt_diff_forward(wpt_diff* d, const float* params, uint32_t sampleStart, float* image) {
+    if (!d || !params || !image) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (static_cast<uint64_t>(sampleStart) + d->config.samples_per_pixel > 0xffffffffull)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "sample range out of range");
+    DiffContext c = d->ctx;
+    c.params.texture = params;
+    c.params.roughness = params + d->textureFloats;
+    c.params.emission = params + d->textureFloats + d->roughnessCount;
+    c.sampleStart = sampleStart;
+    c.image = image;
+    WPT_HIP(hipEventRecord(d->marks[0], d->stream));
+    WPT_HIP(enqueueDiffForward(c, d->forwardBlocks, d->config.block_size, d->stream));
+    WPT_HIP(hipEventRecord(d->marks[1], d->stream));
+    WPT_HIP(hipStreamSynchronize(d->stream));
+    float ms = 0.0f;
+    WPT_HIP(hipEventElapsedTime(&ms, d->marks[0], d->marks[1]));
+    d->stats.forward_milliseconds = ms;
+    d->forwardParams = params;
+    d->ctx.sampleStart = sampleStart;
+    return WPT_OK;
+}
+
+WPT_API int wpt_diff_adjoint(wpt_diff* d, const float* params, const float* imageGrad, float* paramGrad) {
+    if (!d || !params || !imageGrad || !paramGrad) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (params != d->forwardParams)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "the adjoint replays the last forward: run wpt_diff_forward with these parameters first");
+    DiffContext c = d->ctx;
+    c.params.texture = params;
+    c.params.roughness = params + d->textureFloats;
+    c.params.emission = params + d->textureFloats + d->roughnessCount;
+    c.imageGrad = imageGrad;
+    c.grads.texture = paramGrad;
+    c.grads.roughness = paramGrad + d->textureFloats;
+    c.grads.emission = paramGrad + d->textureFloats + d->roughnessCount;
+    size_t count = static_cast<size_t>(d->textureFloats) + d->roughnessCount + d->emissionCount;
+    WPT_HIP(hipEventRecord(d->marks[2], d->stream));
+    WPT_HIP(enqueueDiffAdjoint(c, d->adjointBlocks, d->config.block_size, count, d->stream));
+    WPT_HIP(hipEventRecord(d->marks[3], d->stream));
+    WPT_HIP(hipStreamSynchronize(d->stream));
+    float ms = 0.0f;
+    WPT_HIP(hipEventElapsedTime(&ms, d->marks[2], d->marks[3]));
+    d->stats.adjoint_milliseconds = ms;
+    return WPT_OK;
+}
+
+WPT_API int wpt_diff_adam(wpt_diff* d, float* params, const float* grad, float* m, float* v, uint32_t step, float lr,
+    float beta1, float beta2, float epsilon) {
+    if (!d || !params || !grad || !m || !v) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (step == 0) return fail(WPT_ERROR_INVALID_ARGUMENT, "Adam steps count from 1");
+    AdamStep a{};
+    a.params = params;
+    a.grad = grad;
+    a.m = m;
+    a.v = v;
+    a.count = static_cast<size_t>(d->textureFloats) + d->roughnessCount + d->emissionCount;
+    a.textureFloats = d->textureFloats;
+    a.roughnessCount = d->roughnessCount;
+    a.lr = lr;
+    a.beta1 = beta1;
+    a.beta2 = beta2;
+    a.epsilon = epsilon;
+    a.bias1 = static_cast<float>(1.0 - std::pow(static_cast<double>(beta1), static_cast<double>(step)));
+    a.bias2 = static_cast<float>(1.0 - std::pow(static_cast<double>(beta2), static_cast<double>(step)));
+    a.textureRange[0] = 0.0f;
+    a.textureRange[1] = 1.0f;
+    a.roughnessRange[0] = 0.02f;
+    a.roughnessRange[1] = 1.0f;
+    a.emissionRange[0] = 0.0f;
+    a.emissionRange[1] = std::numeric_limits<float>::infinity();
+    WPT_HIP(enqueueAdam(a, d->stream));
+    WPT_HIP(hipStreamSynchronize(d->stream));
+    return WPT_OK;
+}
+
+WPT_API int wpt_diff_last_stats(const wpt_diff* d, wpt_diff_stats* stats) {
+    if (!d || !stats) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    *stats = d->stats;
     return WPT_OK;
 }
 

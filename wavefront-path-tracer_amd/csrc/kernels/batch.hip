@@ -1,0 +1,157 @@
+// Batched persistent consumer: many tasks (tiles x sample ranges) of several
+// resident scenes rendered by one launch.  The reference renders tasks one
+// at a time (RenderTaskRenderer::render, src/renderer/render_task.cu:26-114);
+// this kernel has no reference counterpart.
+//
+// Each wave works on one task at a time, so the task's scene, camera and
+// path parameters are wave-uniform and read through scalar loads.  Idle
+// lanes claim tile pixels from the task's cursor with one atomic per wave.
+// A lane renders every sample of its pixel in order, one bounce per loop
+// iteration (closest hit, shading, shadow ray), and starts the pixel's next
+// sample in the same iteration its path ends, so lanes do not wait for the
+// longest path of the wave.  The pixel's sum is accumulated in sample order
+// from zero, the same additions wpt_render_task makes into its cleared tile,
+// which makes the two outputs bit-identical.  When the task has no unclaimed
+// pixels and the wave's lanes have finished theirs, the wave moves on to the
+// next task with unclaimed pixels; the kernel ends when none is left.
+#include <hip/hip_runtime.h>
+
+#include "device_queue.h"
+#include "wavefront.h"
+
+namespace wpt {
+
+namespace {
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ unsigned long long waveSum(unsigned long long v) {
+    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(512) kBatchReset(BatchContext ctx) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctx.counters->startClock = ~0ull;
+        ctx.counters->closestQueries = 0;
+        ctx.counters->shadowQueries = 0;
+    }
+    for (uint32_t t = i; t < ctx.taskCount; t += gridDim.x * blockDim.x) {
+        ctx.state[t].cursor = 0;
+        ctx.state[t].done = 0;
+        ctx.state[t].finishClock = 0;
+    }
+}
+
+__global__ void __launch_bounds__(512) kBatch(BatchContext ctx) {
+    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+    const uint32_t n = ctx.taskCount;
+    const uint32_t waveId = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    if (__lane_id() == 0) atomicMin(&ctx.counters->startClock, static_cast<unsigned long long>(wall_clock64()));
+    // Waves start spread evenly over the task list.
+    uint32_t task = uniform(static_cast<uint32_t>((static_cast<unsigned long long>(waveId) * n) / waves));
+    bool taskOpen = true;
+
+    bool busy = false;
+    uint32_t x = 0, y = 0, samplesDone = 0, local = 0;
+    Vec3 sum(0.0f, 0.0f, 0.0f);
+    PathState path;
+    unsigned long long closest = 0, shadow = 0;
+
+    while (true) {
+        const BatchTask& T = ctx.tasks[task];
+        if (taskOpen) {
+            bool need = !busy;
+            unsigned long long mask = __ballot(need);
+            if (mask) {
+                uint32_t want = static_cast<uint32_t>(__popcll(mask));
+                uint32_t base = 0;
+                if (__lane_id() == 0) base = atomicAdd(&ctx.state[task].cursor, want);
+                base = uniform(base);
+                if (base + want >= T.pixelCount) taskOpen = false;
+                uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+                uint32_t u = base + rank;
+                if (need && u < T.pixelCount) {
+                    busy = true;
+                    tilePixel(T.tile, u, x, y);
+                    local = (y - T.tile.y) * T.tile.width + (x - T.tile.x);
+                    samplesDone = 0;
+                    sum = Vec3(0.0f, 0.0f, 0.0f);
+                    path = primaryPath(T.camera, x, y, T.width, T.height, T.sampleStart, T.shade.rngStrategy);
+                }
+            }
+        }
+        if (__ballot(busy) == 0) {
+            // Only reached with the task closed: find one with unclaimed pixels.
+            bool found = false;
+            for (uint32_t k = 1; k <= n; ++k) {
+                uint32_t t = task + k;
+                if (t >= n) t -= n;
+                uint32_t claimed = uniform(loadAgent(&ctx.state[t].cursor));
+                if (claimed < ctx.tasks[t].pixelCount) {
+                    task = t;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) break;
+            taskOpen = true;
+            continue;
+        }
+        if (busy) {
+            const DevScene& scene = ctx.scenes[T.scene];
+            ++closest;
+            HitRecord h = traceClosest(scene, path.ray, stack);
+            ShadeOutcome outcome;
+            shadePath(scene, path, h, T.shade, outcome);
+            if (outcome.shadow.valid) {
+                ++shadow;
+                Ray ray(outcome.shadow.origin, outcome.shadow.direction);
+                if (!traceOccluded(scene, ray, outcome.shadow.tmax, stack)) path.radiance += outcome.shadow.contribution;
+            }
+            if (!outcome.alive) {
+                sum += outcome.isDebug ? outcome.debugColor : path.radiance;
+                ++samplesDone;
+                if (samplesDone < T.sampleCount) {
+                    path = primaryPath(T.camera, x, y, T.width, T.height, T.sampleStart + samplesDone,
+                        T.shade.rngStrategy);
+                } else {
+                    float* o = ctx.out + T.out + 3ull * local;
+                    o[0] = sum.x;
+                    o[1] = sum.y;
+                    o[2] = sum.z;
+                    busy = false;
+                    if (atomicAdd(&ctx.state[task].done, 1u) + 1u == T.pixelCount)
+                        ctx.state[task].finishClock = wall_clock64();
+                }
+            }
+        }
+    }
+    closest = waveSum(closest);
+    shadow = waveSum(shadow);
+    if (__lane_id() == 0) {
+        atomicAdd(&ctx.counters->closestQueries, closest);
+        atomicAdd(&ctx.counters->shadowQueries, shadow);
+    }
+}
+
+}  // namespace
+
+hipError_t enqueueBatch(const BatchContext& ctx, uint32_t blocks, uint32_t blockSize, hipStream_t stream) {
+    uint32_t resetBlocks = (ctx.taskCount + 255u) / 256u;
+    hipLaunchKernelGGL(kBatchReset, dim3(resetBlocks > 0 ? resetBlocks : 1u), dim3(256), 0, stream, ctx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || ctx.taskCount == 0) return e;
+    hipLaunchKernelGGL(kBatch, dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream, ctx);
+    return hipGetLastError();
+}
+
+hipError_t batchOccupancy(uint32_t blockSize, int* blocksPerCu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch, static_cast<int>(blockSize),
+        stackLdsBytes(blockSize));
+}
+
+}  // namespace wpt

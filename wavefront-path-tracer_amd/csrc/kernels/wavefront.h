@@ -76,6 +76,7 @@ struct WavefrontCounters {
 // Traversal stack entries kept in LDS per lane; deeper entries spill to a
 // per-thread column of `spill`.
 constexpr uint32_t kLdsStackEntries = 16;
+constexpr uint32_t kSpillEntriesTotal = kMaxStackDepth - kLdsStackEntries;  // per thread
 
 struct LaunchContext {
     DevScene scene;
@@ -119,6 +120,49 @@ hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out);
 constexpr int kProfileSlots = 7;
 hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[kProfileSlots],
     uint32_t launches[kProfileSlots]);
+
+// ---- batch consumer ---------------------------------------------------
+// One task of a batch: a tile of one scene's frame and a sample range, with
+// its own camera and path parameters.
+struct BatchTask {
+    CameraParams camera;
+    ShadeParams shade;
+    RenderTile tile;
+    uint32_t scene;
+    uint32_t width, height;  // frame: pixel ids and RNG keys
+    uint32_t sampleStart, sampleCount;
+    uint32_t pixelCount;     // tile pixels, 0 when the task renders nothing
+    unsigned long long out;  // float offset of the task's tile sums (w*h*3)
+};
+
+// Per-task progress of a running batch.
+struct BatchTaskState {
+    uint32_t cursor;  // next unclaimed tile pixel
+    uint32_t done;    // finished pixels
+    unsigned long long finishClock;
+};
+
+struct BatchCounters {
+    unsigned long long startClock;
+    unsigned long long closestQueries;
+    unsigned long long shadowQueries;
+};
+
+struct BatchContext {
+    const DevScene* scenes;
+    const BatchTask* tasks;
+    BatchTaskState* state;
+    BatchCounters* counters;
+    uint32_t taskCount;
+    float* out;
+    uint2* spill;
+    uint32_t spillThreads;
+};
+
+// Resets the per-task state and counters, then runs the persistent batch
+// consumer on `blocks` x `blockSize` threads.
+hipError_t enqueueBatch(const BatchContext& ctx, uint32_t blocks, uint32_t blockSize, hipStream_t stream);
+hipError_t batchOccupancy(uint32_t blockSize, int* blocksPerCu);
 
 // Standalone traversal for the C ABI; `spill` holds
 // (kMaxStackDepth - kLdsStackEntries) x abiTraceThreads() entries.

@@ -21,7 +21,7 @@ void printHelp() {
               << "  --help, -h              Show this help text\n"
               << "  --scene NAME             Select a scene preset\n"
               << "                           NAME: sponza, cornell, hurricane, prism, crystal, deer, rtweek, demo,\n"
-              << "                                 cornell-box, sphere-field\n"
+              << "                                 cornell-box, sphere-field, diff-cornell\n"
               << "  --scene-file FILE        Load a scene manifest JSON file\n"
               << "  --list-scenes            List available scene presets\n"
               << "  --caustics               Enable photon-mapped caustics (Cornell/Hurricane/Prism/Crystal)\n"
@@ -153,6 +153,7 @@ SceneConfig presetDefaults(ScenePreset preset) {
     case ScenePreset::Demo:
     case ScenePreset::CornellBox:
     case ScenePreset::SphereField:
+    case ScenePreset::DiffCornell:
         c.objectFilename.clear();
         break;
     default:
@@ -175,6 +176,7 @@ const char* presetName(ScenePreset preset) {
     case ScenePreset::RtWeek: return "rtweek";
     case ScenePreset::CornellBox: return "cornell-box";
     case ScenePreset::SphereField: return "sphere-field";
+    case ScenePreset::DiffCornell: return "diff-cornell";
     }
     return "unknown";
 }
@@ -182,7 +184,7 @@ const char* presetName(ScenePreset preset) {
 bool parseScenePreset(const char* v, ScenePreset& p) {
     static const ScenePreset all[] = {ScenePreset::Sponza, ScenePreset::Cornell, ScenePreset::Hurricane,
         ScenePreset::Prism, ScenePreset::Crystal, ScenePreset::Deer, ScenePreset::Demo, ScenePreset::RtWeek,
-        ScenePreset::CornellBox, ScenePreset::SphereField};
+        ScenePreset::CornellBox, ScenePreset::SphereField, ScenePreset::DiffCornell};
     for (ScenePreset candidate : all) {
         if (!std::strcmp(v, presetName(candidate))) {
             p = candidate;
@@ -329,7 +331,7 @@ bool parseCommandLine(int argc, char** argv, RenderConfig& render, SceneConfig& 
             continue;
         }
         if (!std::strcmp(a, "--list-scenes")) {
-            std::cout << "Scene presets: sponza, cornell, hurricane, prism, crystal, deer, rtweek, demo, cornell-box, sphere-field\n";
+            std::cout << "Scene presets: sponza, cornell, hurricane, prism, crystal, deer, rtweek, demo, cornell-box, sphere-field, diff-cornell\n";
             return false;
         }
         if (!std::strcmp(a, "--help") || !std::strcmp(a, "-h")) {

@@ -21,6 +21,7 @@ enum class ScenePreset {
     // Procedural scenes of the measured configurations (no reference preset).
     CornellBox,   // unit-scale Cornell box, 2 spheres, emissive quad
     SphereField,  // 484-sphere field + 3 instances of a displaced icosphere
+    DiffCornell,  // Cornell box tessellated for the differentiable renderer
 };
 
 enum class DistributedRole { Local, Coordinator, Worker };

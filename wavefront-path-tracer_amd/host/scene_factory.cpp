@@ -201,6 +201,66 @@ void cornellBoxScene(HostScene& s) {
     s.blackBackground = true;
 }
 
+// The unit Cornell box tessellated for the differentiable renderer: 64
+// reflective triangles first (back wall 2x2 quads with texture coordinates
+// over the wall, floor 4x2, ceiling 2x2, left and right walls 2x4), then the
+// light as 2x2 quads = 8 emissive triangles; the two spheres as in the
+// Cornell box.  Material 5 marks the back wall, whose albedo the
+// differentiable renderer reads from its texture.
+void diffCornellScene(HostScene& s) {
+    s.materials = {makeMaterial(WPT_MATERIAL_DIFFUSE, 0.73f, 0.73f, 0.73f),
+        makeMaterial(WPT_MATERIAL_DIFFUSE, 0.65f, 0.05f, 0.05f),
+        makeMaterial(WPT_MATERIAL_DIFFUSE, 0.12f, 0.45f, 0.15f),
+        makeMaterial(WPT_MATERIAL_EMISSIVE, 1.0f, 1.0f, 1.0f, 17.0f, 12.0f, 4.0f),
+        makeMaterial(WPT_MATERIAL_DIELECTRIC, 1.0f, 1.0f, 1.0f),
+        makeMaterial(WPT_MATERIAL_DIFFUSE, 0.73f, 0.73f, 0.73f)};
+    s.materials[4].ior = 1.5f;
+    // Quads a + U*i/nu + V*j/nv; U x V points into the room.
+    auto grid = [&](std::array<float, 3> a, std::array<float, 3> u, std::array<float, 3> v, uint32_t nu, uint32_t nv,
+                    uint32_t m, bool texcoords) {
+        auto at = [&](uint32_t i, uint32_t j, float out[3]) {
+            float fi = static_cast<float>(i) / static_cast<float>(nu), fj = static_cast<float>(j) / static_cast<float>(nv);
+            for (int k = 0; k < 3; ++k) out[k] = a[k] + u[k] * fi + v[k] * fj;
+        };
+        for (uint32_t j = 0; j < nv; ++j) {
+            for (uint32_t i = 0; i < nu; ++i) {
+                float p00[3], p10[3], p11[3], p01[3];
+                at(i, j, p00);
+                at(i + 1, j, p10);
+                at(i + 1, j + 1, p11);
+                at(i, j + 1, p01);
+                size_t first = s.staticTriangles.size();
+                addQuad(s.staticTriangles, p00, p10, p11, p01, m);
+                if (!texcoords) continue;
+                // Texture coordinates across the wall: u along U, v along V.
+                auto uv = [&](uint32_t ii, uint32_t jj, float out[2]) {
+                    out[0] = static_cast<float>(ii) / static_cast<float>(nu);
+                    out[1] = static_cast<float>(jj) / static_cast<float>(nv);
+                };
+                wpt_triangle& t0 = s.staticTriangles[first];
+                wpt_triangle& t1 = s.staticTriangles[first + 1];
+                uv(i, j, t0.uv0);
+                uv(i + 1, j, t0.uv1);
+                uv(i + 1, j + 1, t0.uv2);
+                uv(i, j, t1.uv0);
+                uv(i + 1, j + 1, t1.uv1);
+                uv(i, j + 1, t1.uv2);
+                t0.flags |= 2u;
+                t1.flags |= 2u;
+            }
+        }
+    };
+    grid({-1, -1, -1}, {2, 0, 0}, {0, 2, 0}, 2, 2, 5, true);     // back wall (textured)
+    grid({-1, -1, 1}, {2, 0, 0}, {0, 0, -2}, 4, 2, 0, false);    // floor
+    grid({-1, 1, -1}, {2, 0, 0}, {0, 0, 2}, 2, 2, 0, false);     // ceiling
+    grid({-1, -1, 1}, {0, 0, -2}, {0, 2, 0}, 2, 4, 1, false);    // left (red)
+    grid({1, -1, -1}, {0, 0, 2}, {0, 2, 0}, 2, 4, 2, false);     // right (green)
+    grid({-0.25f, 0.999f, -0.25f}, {0.5f, 0, 0}, {0, 0, 0.5f}, 2, 2, 3, false);  // light
+    s.spheres.push_back(sphere(-0.45f, -0.7f, -0.3f, 0.3f, 0));
+    s.spheres.push_back(sphere(0.45f, -0.7f, 0.3f, 0.3f, 4));
+    s.blackBackground = true;
+}
+
 // Reference demo scene (scene_host.cpp:116-163).
 void demoScene(HostScene& s) {
     s.spheres.push_back(sphere(0.0f, -100.5f, 0.0f, 100.0f, 3, WPT_ACCEL_DIRECT));
@@ -413,6 +473,9 @@ bool createSceneFromConfig(const SceneConfig& config, HostScene& out, std::strin
     case ScenePreset::CornellBox:
         cornellBoxScene(out);
         return true;
+    case ScenePreset::DiffCornell:
+        diffCornellScene(out);
+        return true;
     case ScenePreset::SphereField:
         sphereFieldScene(out, config, true);
         return true;
@@ -484,7 +547,8 @@ wpt_camera createSceneCamera(const SceneConfig& config, uint32_t w, uint32_t h) 
         const float p[3] = {0.0f, 1.2f, 6.5f}, t[3] = {0.0f, 0.8f, -0.6f};
         return lookAt(p, t, 42.0f, w, h, false);
     }
-    case ScenePreset::CornellBox: {
+    case ScenePreset::CornellBox:
+    case ScenePreset::DiffCornell: {
         const float p[3] = {0.0f, 0.0f, 3.4f}, t[3] = {0.0f, 0.0f, 0.0f};
         return lookAt(p, t, 40.0f, w, h, true);
     }

@@ -115,6 +115,37 @@ class SceneInfo(C.Structure):
         "flat_node_count", "material_count")]
 
 
+class BatchTask(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("x", C.c_uint32), ("y", C.c_uint32), ("tile_width", C.c_uint32), ("tile_height", C.c_uint32),
+                ("sample_start", C.c_uint32), ("sample_count", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_start_depth", C.c_uint32), ("rng_strategy", C.c_uint32), ("indirect_clamp", C.c_float)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("blocks", C.c_uint32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("closest_queries", C.c_uint64), ("shadow_queries", C.c_uint64), ("samples", C.c_uint64),
+                ("milliseconds", C.c_double), ("block_size", C.c_uint32), ("blocks", C.c_uint32)]
+
+
+class DiffConfig(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("samples_per_pixel", C.c_uint32),
+                ("max_depth", C.c_uint32), ("rng_strategy", C.c_uint32), ("texture_width", C.c_uint32),
+                ("texture_height", C.c_uint32), ("textured_material", C.c_uint32), ("block_size", C.c_uint32)]
+
+
+class DiffLayout(C.Structure):
+    _fields_ = [("texture_floats", C.c_uint32), ("roughness_count", C.c_uint32), ("emission_count", C.c_uint32),
+                ("parameter_count", C.c_uint32)]
+
+
+class DiffStats(C.Structure):
+    _fields_ = [("forward_milliseconds", C.c_double), ("adjoint_milliseconds", C.c_double), ("paths", C.c_uint64)]
+
+
 class Bvh2Node(C.Structure):
     _fields_ = [("bmin", F3), ("bmax", F3), ("left", C.c_uint32), ("right", C.c_uint32)]
 
@@ -156,6 +187,18 @@ def library() -> C.CDLL:
         "wpt_render_task": ([P, C.POINTER(Task), P], C.c_int),
         "wpt_renderer_last_stats": ([P, C.POINTER(Stats)], C.c_int),
         "wpt_renderer_profile_kernels": ([P, u32, u32, P, P], C.c_int),
+        "wpt_batch_create": ([C.POINTER(P), u32, C.POINTER(BatchConfig), C.POINTER(P)], C.c_int),
+        "wpt_batch_destroy": ([P], C.c_int),
+        "wpt_batch_render": ([P, C.POINTER(BatchTask), u32, C.POINTER(P), P], C.c_int),
+        "wpt_batch_last_stats": ([P, C.POINTER(BatchStats)], C.c_int),
+        "wpt_batch_autotune": ([P, C.POINTER(BatchTask), u32, C.POINTER(BatchConfig)], C.c_int),
+        "wpt_diff_create": ([P, C.POINTER(Camera), C.POINTER(DiffConfig), C.POINTER(P)], C.c_int),
+        "wpt_diff_destroy": ([P], C.c_int),
+        "wpt_diff_get_layout": ([P, C.POINTER(DiffLayout)], C.c_int),
+        "wpt_diff_forward": ([P, P, u32, P], C.c_int),
+        "wpt_diff_adjoint": ([P, P, P, P], C.c_int),
+        "wpt_diff_adam": ([P, P, P, P, P, u32, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
+        "wpt_diff_last_stats": ([P, C.POINTER(DiffStats)], C.c_int),
         "wpt_host_scene_create": ([C.c_char_p, u32, u32, C.POINTER(P)], C.c_int),
         "wpt_host_scene_get_desc": ([P, C.POINTER(SceneDesc)], C.c_int),
         "wpt_host_scene_get_camera": ([P, u32, u32, C.POINTER(Camera)], C.c_int),
@@ -487,6 +530,146 @@ class Renderer:
             self.close()
         except Exception:
             pass
+
+
+def batch_task(scene_index, camera, width, height, x, y, tile_w, tile_h, sample_start, sample_count, max_depth=8,
+               rr_start=3, rng=RNG_PCG32, clamp=10.0) -> BatchTask:
+    return BatchTask(scene_index, camera, width, height, x, y, tile_w, tile_h, sample_start, sample_count, max_depth,
+                     rr_start, rng, clamp)
+
+
+class Batch:
+    """Many tasks of several device scenes rendered by one persistent launch
+    (wpt_batch_*).  The scenes must outlive the batch."""
+
+    def __init__(self, scenes: Sequence["Scene"], block_size: int = 256, blocks: int = 0):
+        self._scenes = list(scenes)
+        handles = (C.c_void_p * len(self._scenes))(*[sc.handle for sc in self._scenes])
+        self._h = C.c_void_p()
+        cfg = BatchConfig(block_size, blocks)
+        check(library().wpt_batch_create(handles, len(self._scenes), C.byref(cfg), C.byref(self._h)),
+              "wpt_batch_create")
+
+    @staticmethod
+    def _tasks(tasks):
+        return (BatchTask * max(1, len(tasks)))(*tasks)
+
+    def render(self, tasks: Sequence[BatchTask], latencies: bool = False):
+        """Returns a list of (tile_height, tile_width, 3) float32 sums, and the
+        per-task latencies in ms when asked."""
+        arr = self._tasks(tasks)
+        outs = [np.zeros((t.tile_height, t.tile_width, 3), np.float32) for t in tasks]
+        ptrs = (C.c_void_p * max(1, len(tasks)))(*[o.ctypes.data for o in outs])
+        lat = np.zeros(max(1, len(tasks)), np.float64)
+        check(library().wpt_batch_render(self._h, arr, len(tasks), ptrs, lat.ctypes.data if latencies else None),
+              "wpt_batch_render")
+        return (outs, lat[:len(tasks)]) if latencies else outs
+
+    def autotune(self, tasks: Sequence[BatchTask]) -> BatchConfig:
+        arr = self._tasks(tasks)
+        chosen = BatchConfig()
+        check(library().wpt_batch_autotune(self._h, arr, len(tasks), C.byref(chosen)), "wpt_batch_autotune")
+        return chosen
+
+    def stats(self) -> BatchStats:
+        st = BatchStats()
+        check(library().wpt_batch_last_stats(self._h, C.byref(st)), "wpt_batch_last_stats")
+        return st
+
+    def close(self):
+        if self._h:
+            library().wpt_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DiffRenderer:
+    """Differentiable renderer (wpt_diff_*): forward image and path-replay
+    adjoint over one parameter vector [texture | roughness | emission].
+    Device buffers are torch tensors on cuda:0; every call synchronizes."""
+
+    def __init__(self, scene: "Scene", camera: Camera, width: int, height: int, spp: int, max_depth: int,
+                 texture_size=(0, 0), textured_material: int = INVALID, rng: int = RNG_PCG32, block_size: int = 256):
+        self.scene, self.camera = scene, camera
+        self.width, self.height, self.spp = width, height, spp
+        cfg = DiffConfig(width, height, spp, max_depth, rng, texture_size[0], texture_size[1], textured_material,
+                         block_size)
+        self._h = C.c_void_p()
+        check(library().wpt_diff_create(scene.handle, C.byref(camera), C.byref(cfg), C.byref(self._h)),
+              "wpt_diff_create")
+        self.layout = DiffLayout()
+        check(library().wpt_diff_get_layout(self._h, C.byref(self.layout)), "wpt_diff_get_layout")
+
+    @staticmethod
+    def _ptr(t):
+        import torch
+        assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+        return C.c_void_p(t.data_ptr())
+
+    def forward(self, params, sample_start: int = 0):
+        import torch
+        torch.cuda.synchronize()
+        image = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=params.device)
+        check(library().wpt_diff_forward(self._h, self._ptr(params), sample_start, self._ptr(image)),
+              "wpt_diff_forward")
+        return image
+
+    def adjoint(self, params, image_grad):
+        import torch
+        torch.cuda.synchronize()
+        grad = torch.empty_like(params)
+        check(library().wpt_diff_adjoint(self._h, self._ptr(params), self._ptr(image_grad.contiguous()),
+                                         self._ptr(grad)), "wpt_diff_adjoint")
+        return grad
+
+    def adam(self, params, grad, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        import torch
+        torch.cuda.synchronize()
+        check(library().wpt_diff_adam(self._h, self._ptr(params), self._ptr(grad), self._ptr(m), self._ptr(v), step,
+                                      lr, beta1, beta2, eps), "wpt_diff_adam")
+
+    def stats(self) -> DiffStats:
+        st = DiffStats()
+        check(library().wpt_diff_last_stats(self._h, C.byref(st)), "wpt_diff_last_stats")
+        return st
+
+    def close(self):
+        if self._h:
+            library().wpt_diff_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def diff_render_function():
+    """torch.autograd.Function over DiffRenderer: image = f(params), the
+    backward is the path-replay adjoint.  For checking the adjoint inside
+    torch; the renderer itself does not need torch."""
+    import torch
+
+    class DiffRender(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, params, renderer, sample_start):
+            params = params.detach().contiguous()
+            ctx.renderer = renderer
+            ctx.save_for_backward(params)
+            return renderer.forward(params, sample_start)
+
+        @staticmethod
+        def backward(ctx, image_grad):
+            (params,) = ctx.saved_tensors
+            return ctx.renderer.adjoint(params, image_grad.contiguous()), None, None
+
+    return DiffRender
 
 
 def create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint) -> List[Task]:
