@@ -114,25 +114,28 @@ def measure_batch(n_tasks):
     log(f"batch: {n_tasks} tasks, {samples / 1e6:.1f} M samples in {batch_s * 1000:.1f} ms "
         f"(device {bst.milliseconds:.1f} ms)")
 
-    # The same tasks one by one through wpt_render_task, one renderer per scene.
-    renderers = []
-    for i, sc in enumerate(scenes):
-        depth = max([t.max_depth for t in tasks if t.scene == i] or [8])
-        cfg = wpt.render_config(WIDTH, HEIGHT, spp=64, max_depth=depth, rr_start=RR_START, clamp=CLAMP)
-        renderers.append(wpt.Renderer(sc, hosts[i].camera(WIDTH, HEIGHT), cfg))
-    warm = wpt.Task(0, 0, 0, 1, 0, 0, 32, 32)
-    for r in renderers:
-        r.render_task(warm)
+    # The same tasks one by one through wpt_render_task, as the reference's
+    # RenderTaskRenderer renders them: one renderer per scene and tile size,
+    # its queues sized to the tile.
+    renderers = {}
+    for t in tasks:
+        key = (t.scene, t.tile_width)
+        if key not in renderers:
+            cfg = wpt.render_config(WIDTH, HEIGHT, spp=64, max_depth=t.max_depth, rr_start=RR_START, clamp=CLAMP,
+                                    tile=(t.tile_width, t.tile_height))
+            renderers[key] = wpt.Renderer(scenes[t.scene], hosts[t.scene].camera(WIDTH, HEIGHT), cfg)
+    for (_, side), r in renderers.items():
+        r.render_task(wpt.Task(0, 0, 0, 1, 0, 0, side, side))
     done = []
     identical = True
     tp = time.perf_counter()
     for i, t in enumerate(tasks):
-        out = renderers[t.scene].render_task(wpt.Task(0, 0, t.sample_start, t.sample_count, t.x, t.y, t.tile_width,
-                                                      t.tile_height))
+        out = renderers[(t.scene, t.tile_width)].render_task(
+            wpt.Task(0, 0, t.sample_start, t.sample_count, t.x, t.y, t.tile_width, t.tile_height))
         done.append((time.perf_counter() - tp) * 1000.0)
         identical &= bool(np.array_equal(out, outs[i]))
     plain_s = time.perf_counter() - tp
-    for r in renderers:
+    for r in renderers.values():
         r.close()
     lat_plain = np.array(done)
     result = {
@@ -156,6 +159,93 @@ def measure_batch(n_tasks):
     return result
 
 
+def diff_parameters(layout, seed, truth):
+    """Parameter vector [texture 512x512 RGB | 64 roughness | 8 emission].
+    The initial guess (truth=False) is a flat grey texture with noise; the
+    ground truth is a smooth pattern of seeded sinusoids."""
+    rng = np.random.default_rng(seed)
+    side = int(round((layout.texture_floats // 3) ** 0.5))
+    if truth:
+        yy, xx = np.mgrid[0:side, 0:side] / side
+        tex = np.zeros((side, side, 3))
+        for c in range(3):
+            acc = np.zeros((side, side))
+            for _ in range(4):
+                fx, fy, ph = rng.uniform(1, 6), rng.uniform(1, 6), rng.uniform(0, 2 * np.pi)
+                acc += np.sin(2 * np.pi * (fx * xx + fy * yy) + ph)
+            tex[..., c] = 0.5 + 0.1 * acc
+        tex = np.clip(tex, 0.05, 0.95)
+        rough = rng.uniform(0.05, 0.95, layout.roughness_count)
+        emit = rng.uniform(0.5, 1.5, layout.emission_count)
+    else:
+        tex = np.clip(0.5 + rng.uniform(-0.05, 0.05, (side, side, 3)), 0, 1)
+        rough = rng.uniform(0.3, 0.7, layout.roughness_count)
+        emit = rng.uniform(0.8, 1.2, layout.emission_count)
+    return np.concatenate([tex.reshape(-1), rough, emit]).astype(np.float32)
+
+
+def measure_diff(steps):
+    """Differentiation workload: the Cornell box tessellated into 64
+    reflective + 8 emissive triangles, a 512x512 RGB albedo texture on the
+    back wall, per-triangle GGX roughness and per-light emission scales.
+    Target: 512x512 at 256 spp from the seed-B parameters; optimisation:
+    L2 loss at 512x512, 16 spp, depth 6, path-replay adjoint, Adam from the
+    seed-A parameters."""
+    import torch
+    size, spp, depth = 512, 16, 6
+    host = wpt.HostScene("diff-cornell")
+    scene = wpt.Scene(host.desc(), device=True, owner=host)
+    cam = host.camera(size, size)
+    r = wpt.DiffRenderer(scene, cam, size, size, spp, depth, (512, 512), 5)
+    lay = r.layout
+    truth = torch.tensor(diff_parameters(lay, 0xB, True), device="cuda:0")
+    target_renderer = wpt.DiffRenderer(scene, cam, size, size, 256, depth, (512, 512), 5)
+    target = target_renderer.forward(truth, sample_start=1 << 20)
+    target_renderer.close()
+    params = torch.tensor(diff_parameters(lay, 0xA, False), device="cuda:0")
+    m = torch.zeros_like(params)
+    v = torch.zeros_like(params)
+    # Warm-up, then time forward alone and forward + adjoint.
+    img = r.forward(params)
+    r.adjoint(params, 2.0 * (img - target) / img.numel())
+    fwd_ms, adj_ms, losses = [], [], []
+    t0 = time.perf_counter()
+    for step in range(1, steps + 1):
+        img = r.forward(params, sample_start=step * spp)
+        g_img = 2.0 * (img - target) / img.numel()
+        losses.append(float((g_img * g_img).sum() * img.numel() / 4.0))
+        grad = r.adjoint(params, g_img)
+        st = r.stats()
+        fwd_ms.append(st.forward_milliseconds)
+        adj_ms.append(st.adjoint_milliseconds)
+        r.adam(params, grad, m, v, step, 0.01)
+    wall = time.perf_counter() - t0
+    err = lambda a, b: float((a - b).abs().mean())
+    t, rc = lay.texture_floats, lay.roughness_count
+    result = {
+        "workload": "diff-cornell 512x512, 16 spp, depth 6; 512x512 RGB albedo texture + 64 roughness + 8 emission; "
+                    "target 256 spp from seed B, Adam lr 0.01 from seed A",
+        "forward_ms": round(float(np.median(fwd_ms)), 3),
+        "adjoint_ms": round(float(np.median(adj_ms)), 3),
+        "fwd_adjoint_ms_per_iter": round(float(np.median(np.array(fwd_ms) + np.array(adj_ms))), 3),
+        "ratio_to_forward": round(float(np.median((np.array(fwd_ms) + np.array(adj_ms)) / np.array(fwd_ms))), 3),
+        "wall_ms_per_iter": round(wall * 1000.0 / steps, 3),
+        "adam_steps": steps,
+        "loss_first_last": [losses[0], losses[-1]],
+        "param_mae_start_end": {
+            "texture": [err(torch.tensor(diff_parameters(lay, 0xA, False)[:t]), truth[:t].cpu()),
+                        err(params[:t], truth[:t])],
+            "roughness": [err(torch.tensor(diff_parameters(lay, 0xA, False)[t:t + rc]), truth[t:t + rc].cpu()),
+                          err(params[t:t + rc], truth[t:t + rc])],
+            "emission": [err(torch.tensor(diff_parameters(lay, 0xA, False)[t + rc:]), truth[t + rc:].cpu()),
+                         err(params[t + rc:], truth[t + rc:])],
+        },
+    }
+    log(f"diff: {result}")
+    r.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +259,8 @@ def main():
     ap.add_argument("--profile-spp", type=int, default=4, help="samples rendered with per-kernel events")
     ap.add_argument("--skip-batch", action="store_true", help="do not measure batched vs one-by-one tasks")
     ap.add_argument("--batch-tasks", type=int, default=256, help="tasks of the batching workload")
+    ap.add_argument("--skip-diff", action="store_true", help="do not measure the differentiable renderer")
+    ap.add_argument("--diff-steps", type=int, default=200, help="Adam steps of the inverse-rendering workload")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB three times: the reference kernel of "
                          "known traffic against which the FETCH_SIZE/WRITE_SIZE counters are scaled")
@@ -182,6 +274,12 @@ def main():
         print("bench.py needs a gfx950 device", file=sys.stderr)
         sys.exit(1)
 
+    if args.calibration_copy_mib or not args.skip_diff:
+        # torch ships its own HIP runtime beside the one libwpt links; it
+        # must initialise the device before libwpt's first device call.
+        import torch
+        torch.zeros(1, device="cuda:0")
+        torch.cuda.synchronize()
     if args.calibration_copy_mib:
         import torch
         n = args.calibration_copy_mib * (1 << 20) // 4
@@ -271,6 +369,10 @@ def main():
     if not args.skip_batch:
         batch = measure_batch(args.batch_tasks)
 
+    diff = None
+    if not args.skip_diff:
+        diff = measure_diff(args.diff_steps)
+
     roofline = None
     if work is not None and trace_ms > 0:
         per_query = (NODE_B * work.closest_node_tests + TRI_B * work.closest_triangle_tests +
@@ -314,6 +416,7 @@ def main():
         "device_ms_per_frame": round(float(np.mean(device_ms)), 2),
         "persistent": persistent,
         "batch": batch,
+        "diff": diff,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -1128,6 +1128,392 @@ void renderRows(const oracle_handle& h, const wpt_camera& cam, const wpt_render_
     }
 }
 
+
+// ---------------------------------------------------------------- differentiable rendering
+// Restatement of the differentiable estimator of wpt_diff (no reference
+// counterpart): FP32 for parity with the device, FP64 for the host adjoint,
+// and dual numbers (value, derivative along one parameter direction) for an
+// independent forward-mode derivative.
+
+struct Dual {
+    double v, d;
+    Dual(double a = 0.0, double b = 0.0) : v(a), d(b) {}
+    Dual operator-() const { return Dual(-v, -d); }
+};
+inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline Dual operator/(Dual a, Dual b) { return Dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+inline Dual& operator*=(Dual& a, Dual b) { return a = a * b; }
+inline Dual dsqrt(Dual a) {
+    double s = std::sqrt(a.v);
+    return Dual(s, a.d / (2.0 * s));
+}
+inline float dsqrt(float a) { return std::sqrt(a); }
+inline double dsqrt(double a) { return std::sqrt(a); }
+inline double value(Dual a) { return a.v; }
+inline double value(double a) { return a; }
+inline float value(float a) { return a; }
+
+struct DiffSetup {
+    uint32_t texW = 0, texH = 0, texturedMaterial = kInvalid;
+    std::vector<uint32_t> staticSlot, lightSlot;
+    uint32_t roughCount = 0, emitCount = 0, texFloats = 0;
+    const float* params = nullptr;
+    const float* direction = nullptr;  // dual seed, parameter layout
+    uint32_t width = 0, height = 0, spp = 0, maxDepth = 0, strategy = 0, sampleStart = 0;
+};
+
+template <class P> P paramAt(const DiffSetup& s, size_t i) { return P(s.params[i]); }
+template <> Dual paramAt<Dual>(const DiffSetup& s, size_t i) {
+    return Dual(s.params[i], s.direction ? static_cast<double>(s.direction[i]) : 0.0);
+}
+
+DiffSetup diffSetup(const oracle_handle& h, const oracle_diff_config& c, const float* params) {
+    DiffSetup s;
+    bool textured = c.textured_material != kInvalid;
+    s.texW = textured ? c.texture_width : 0;
+    s.texH = textured ? c.texture_height : 0;
+    s.texturedMaterial = textured ? c.textured_material : kInvalid;
+    s.texFloats = s.texW * s.texH * 3;
+    s.staticSlot.assign(h.d.static_triangle_count, kInvalid);
+    s.lightSlot.assign(std::max<size_t>(h.lights.size(), 1), kInvalid);
+    for (uint32_t i = 0; i < h.d.static_triangle_count; ++i) {
+        uint32_t type = mat(h, h.d.static_triangles[i].material).type;
+        if (type == WPT_MATERIAL_EMISSIVE) {
+            s.staticSlot[i] = s.emitCount;
+            if (h.staticLight[i] != kInvalid) s.lightSlot[h.staticLight[i]] = s.emitCount;
+            ++s.emitCount;
+        } else if (type != WPT_MATERIAL_DIELECTRIC) {
+            s.staticSlot[i] = s.roughCount++;
+        }
+    }
+    s.params = params;
+    s.width = c.width;
+    s.height = c.height;
+    s.spp = c.samples_per_pixel;
+    s.maxDepth = c.max_depth;
+    s.strategy = c.rng_strategy;
+    return s;
+}
+
+template <class R>
+struct Footprint {
+    uint32_t index[4];
+    R weight[4];
+};
+
+template <class R>
+Footprint<R> footprint(const DiffSetup& s, const V2<R>& uv) {
+    R x = uv.x * R(s.texW) - R(0.5f);
+    R y = uv.y * R(s.texH) - R(0.5f);
+    R fx = std::floor(x), fy = std::floor(y);
+    R tx = x - fx, ty = y - fy;
+    int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+    int mx = static_cast<int>(s.texW) - 1, my = static_cast<int>(s.texH) - 1;
+    auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+    int x0 = clampi(ix, mx), x1 = clampi(ix + 1, mx), y0 = clampi(iy, my), y1 = clampi(iy + 1, my);
+    Footprint<R> f;
+    f.index[0] = static_cast<uint32_t>(y0) * s.texW + static_cast<uint32_t>(x0);
+    f.index[1] = static_cast<uint32_t>(y0) * s.texW + static_cast<uint32_t>(x1);
+    f.index[2] = static_cast<uint32_t>(y1) * s.texW + static_cast<uint32_t>(x0);
+    f.index[3] = static_cast<uint32_t>(y1) * s.texW + static_cast<uint32_t>(x1);
+    f.weight[0] = (R(1) - tx) * (R(1) - ty);
+    f.weight[1] = tx * (R(1) - ty);
+    f.weight[2] = (R(1) - tx) * ty;
+    f.weight[3] = tx * ty;
+    return f;
+}
+
+// GGX lobe F0 D G / (4 cos_i cos_o) with alpha = roughness, and d/dalpha.
+template <class R, class P>
+void ggxLobe(const V3<R>& n, const V3<R>& wo, const V3<R>& wi, P alpha, P& value, P& dAlpha) {
+    const R pi = R(3.14159265358979f);
+    value = P(0);
+    dAlpha = P(0);
+    R ci = dot(n, wi), co = dot(n, wo);
+    if (!(ci > R(0)) || !(co > R(0))) return;
+    V3<R> hv = normalize(wi + wo);
+    R nh = std::fmax(R(0), dot(n, hv));
+    P c = P(nh * nh);
+    P a2 = alpha * alpha;
+    P k = c * (a2 - P(1)) + P(1);
+    P d = a2 / (P(pi) * k * k);
+    P dD = P(2) * alpha * (k - P(2) * a2 * c) / (P(pi) * k * k * k);
+    auto g1 = [&](R mu, P& dG) {
+        P m = P(mu);
+        P s = dsqrt(a2 + (P(1) - a2) * m * m);
+        P denom = m + s;
+        dG = P(-2) * m * alpha * (P(1) - m * m) / (s * denom * denom);
+        return P(2) * m / denom;
+    };
+    P dGi, dGo;
+    P gi = g1(ci, dGi);
+    P go = g1(co, dGo);
+    P scale = P(R(0.04f)) / P(R(4) * ci * co);
+    value = scale * (d * (gi * go));
+    dAlpha = scale * (dD * (gi * go) + d * (dGi * go + gi * dGo));
+}
+
+// Record of one vertex for the FP64 reverse sweep.
+struct DiffVertex {
+    int kind = 0;  // 1 emitter, 2 diffuse surface, 3 dielectric
+    V3<double> T;
+    // emitter
+    uint32_t eslot = kInvalid;
+    V3<double> em;  // emission * MIS weight
+    // surface
+    bool textured = false;
+    Footprint<double> fp{};
+    uint32_t rslot = kInvalid;
+    bool nee = false;
+    V3<double> neeBase, neeF;  // contribution = T (neeF neeBase) e
+    double neeE = 1.0, neeDAlpha = 0.0;
+    uint32_t neeSlot = kInvalid;
+    bool cont = false;
+    V3<double> weight;  // continuation (or dielectric) throughput factor
+    double cosOverPdf = 0.0, contDAlpha = 0.0;
+};
+
+// One path of the differentiable estimator; R geometry, P parameter values.
+template <class R, class P>
+V3<P> diffPathOracle(const oracle_handle& h, const DiffSetup& s, const wpt_camera& cam, uint32_t x, uint32_t y,
+    uint32_t sample, std::vector<DiffVertex>* record) {
+    const R invPi = R(0.31830988618f);
+    PathState<R> path = primary<R>(cam, x, y, s.width, s.height, sample, s.strategy, 0.0f);
+    V3<P> T(P(1), P(1), P(1));
+    V3<P> acc(P(0), P(0), P(0));
+    for (uint32_t depth = 0; depth < s.maxDepth; ++depth) {
+        Hit<R> hit;
+        HitState<R> hs;
+        if (!traceRay(h, path.ray, hit, hs)) break;
+        const wpt_material& m = mat(h, hit.material);
+        DiffVertex vtx;
+        if (record) {
+            vtx.T = V3<double>(value(T.x), value(T.y), value(T.z));
+        }
+        if (m.type == WPT_MATERIAL_EMISSIVE) {
+            R w = R(1);
+            uint32_t slot = kInvalid;
+            if (hit.lightIndex != kInvalid) {
+                slot = s.lightSlot[hit.lightIndex];
+                const TriangleLight& L = h.lights[hit.lightIndex];
+                R lc = std::fmax(R(0), dot(V3<R>(R(L.normal.x), R(L.normal.y), R(L.normal.z)), -path.ray.d));
+                if (lc == R(0)) {
+                    w = R(0);
+                } else if (depth > 0 && !path.specularBounce) {
+                    R lightPdf = R(L.selectionPdf) * hit.t * hit.t / (lc * R(L.area));
+                    w = powerHeuristic(path.previousBsdfPdf, lightPdf);
+                }
+            }
+            V3<P> em(P(R(m.emission[0])), P(R(m.emission[1])), P(R(m.emission[2])));
+            V3<P> term = hadamard(T, em) * P(w);
+            P e = slot == kInvalid ? P(1) : paramAt<P>(s, s.texFloats + s.roughCount + slot);
+            acc += term * e;
+            if (record) {
+                vtx.kind = 1;
+                vtx.eslot = slot;
+                vtx.em = V3<double>(double(R(m.emission[0])) * double(w), double(R(m.emission[1])) * double(w),
+                    double(R(m.emission[2])) * double(w));
+                record->push_back(vtx);
+            }
+            break;
+        }
+        V3<R> n = hit.normal;
+        if (dot(n, hit.geometricNormal) < R(0)) n = -n;
+        V3<R> direction;
+        if (m.type == WPT_MATERIAL_DIELECTRIC) {
+            V3<P> albedo(P(R(m.albedo[0])), P(R(m.albedo[1])), P(R(m.albedo[2])));
+            T = hadamard(T, albedo);
+            Interface<R> b = dielectricInterface(normalize(path.ray.d), hit.geometricNormal, n, R(m.ior));
+            R eta2 = b.eta * b.eta;
+            bool reflected = b.reflectance >= R(1) || b.reflectance > rnd<R>(path.rng);
+            if (!reflected) T *= P(eta2);
+            path.specularBounce = true;
+            path.previousBsdfPdf = R(0);
+            direction = reflected ? b.reflection : b.transmission;
+            if (record) {
+                vtx.kind = 3;
+                double f = reflected ? 1.0 : double(eta2);
+                vtx.weight = V3<double>(double(R(m.albedo[0])) * f, double(R(m.albedo[1])) * f, double(R(m.albedo[2])) * f);
+                record->push_back(vtx);
+            }
+        } else {
+            V3<R> wo = -path.ray.d;
+            // Surface: albedo (texture or material), roughness slot.
+            V3<P> albedo(P(R(m.albedo[0])), P(R(m.albedo[1])), P(R(m.albedo[2])));
+            uint32_t rslot = hs.kind == WPT_HIT_STATIC_TRIANGLE ? s.staticSlot[hs.prim] : kInvalid;
+            P alpha = rslot != kInvalid ? paramAt<P>(s, s.texFloats + rslot) : P(0);
+            bool textured = hs.kind == WPT_HIT_STATIC_TRIANGLE && hit.material == s.texturedMaterial && s.texW > 0;
+            Footprint<R> fp{};
+            if (textured) {
+                fp = footprint(s, hit.uv);
+                albedo = V3<P>(P(0), P(0), P(0));
+                for (int k = 0; k < 4; ++k) {
+                    size_t base = 3u * static_cast<size_t>(fp.index[k]);
+                    V3<P> t(paramAt<P>(s, base), paramAt<P>(s, base + 1), paramAt<P>(s, base + 2));
+                    albedo = albedo + t * P(fp.weight[k]);
+                }
+            }
+            auto bsdf = [&](const V3<R>& wi, P& dAlpha) {
+                V3<P> f = albedo * P(invPi);
+                dAlpha = P(0);
+                if (rslot != kInvalid) {
+                    P value;
+                    ggxLobe<R, P>(n, wo, wi, alpha, value, dAlpha);
+                    f = f + V3<P>(value, value, value);
+                }
+                return f;
+            };
+            if (record) {
+                vtx.kind = 2;
+                vtx.textured = textured;
+                vtx.rslot = rslot;
+                for (int k = 0; k < 4; ++k) {
+                    vtx.fp.index[k] = fp.index[k];
+                    vtx.fp.weight[k] = double(fp.weight[k]);
+                }
+            }
+            uint32_t lightCount = static_cast<uint32_t>(h.lights.size());
+            if (lightCount != 0) {
+                uint32_t candidate = std::min(static_cast<uint32_t>(randomFloat(path.rng) * lightCount), lightCount - 1);
+                uint32_t li = randomFloat(path.rng) < h.alias[candidate].probability ? candidate : h.alias[candidate].alias;
+                const TriangleLight& L = h.lights[li];
+                V3<R> lv0(R(L.v0.x), R(L.v0.y), R(L.v0.z)), lv1(R(L.v1.x), R(L.v1.y), R(L.v1.z)),
+                    lv2(R(L.v2.x), R(L.v2.y), R(L.v2.z));
+                V3<R> ln(R(L.normal.x), R(L.normal.y), R(L.normal.z));
+                R rootU = std::sqrt(rnd<R>(path.rng));
+                R v = rnd<R>(path.rng);
+                V3<R> lp = (R(1) - rootU) * lv0 + rootU * (R(1) - v) * lv1 + rootU * v * lv2;
+                V3<R> toLight = lp - hit.position;
+                R d2 = lengthSquared(toLight);
+                R dist = std::sqrt(d2);
+                V3<R> wl = toLight / dist;
+                R gc = std::fmax(R(0), dot(hit.geometricNormal, wl));
+                R sc = std::fmax(R(0), dot(n, wl));
+                R lc = std::fmax(R(0), dot(ln, -wl));
+                if (gc != R(0) && sc != R(0) && lc != R(0)) {
+                    V3<R> so = offsetRayOrigin(hit.position, hit.geometricNormal, wl, hit.t);
+                    V3<R> st = offsetRayOrigin(lp, ln, -wl, dist);
+                    V3<R> seg = st - so;
+                    R sd = length(seg);
+                    if (sd > R(1.0e-6f) && !isOccluded(h, Ray<R>{so, seg / sd}, sd - R(1.0e-6f))) {
+                        R geometryTerm = lc * R(L.area) / (d2 * R(L.selectionPdf));
+                        R lightPdf = R(L.selectionPdf) * d2 / (lc * R(L.area));
+                        R bsdfPdf = sc * invPi;
+                        R g = geometryTerm * powerHeuristic(lightPdf, bsdfPdf);
+                        uint32_t slot = s.lightSlot[li];
+                        P e = slot == kInvalid ? P(1) : paramAt<P>(s, s.texFloats + s.roughCount + slot);
+                        P dAlpha;
+                        V3<P> f = bsdf(wl, dAlpha);
+                        const wpt_material& lm = mat(h, L.material);
+                        V3<P> le(P(R(lm.emission[0])), P(R(lm.emission[1])), P(R(lm.emission[2])));
+                        V3<P> base = hadamard(T, le) * P(g * sc);
+                        V3<P> fb = hadamard(base, f);
+                        acc += fb * e;
+                        if (record) {
+                            vtx.nee = true;
+                            R gs = g * sc;
+                            vtx.neeBase = V3<double>(double(R(lm.emission[0])) * double(gs),
+                                double(R(lm.emission[1])) * double(gs), double(R(lm.emission[2])) * double(gs));
+                            vtx.neeF = V3<double>(value(f.x), value(f.y), value(f.z));
+                            vtx.neeE = value(e);
+                            vtx.neeSlot = slot;
+                            vtx.neeDAlpha = value(dAlpha);
+                        }
+                    }
+                }
+            }
+            direction = sampleCosineHemisphere(n, path.rng);
+            R cosine = dot(n, direction);
+            R pdf = std::fmax(R(0), cosine) * invPi;
+            bool continues = pdf > R(0) && cosine > R(0) && dot(direction, hit.geometricNormal) > R(0);
+            if (continues) {
+                P dAlpha;
+                V3<P> f = bsdf(direction, dAlpha);
+                R cosOverPdf = cosine / pdf;
+                V3<P> weight = f * P(cosOverPdf);
+                T = hadamard(T, weight);
+                if (record) {
+                    vtx.cont = true;
+                    vtx.weight = V3<double>(value(weight.x), value(weight.y), value(weight.z));
+                    vtx.cosOverPdf = double(cosOverPdf);
+                    vtx.contDAlpha = value(dAlpha);
+                }
+                path.previousBsdfPdf = pdf;
+                path.specularBounce = false;
+            }
+            if (record) record->push_back(vtx);
+            if (!continues) break;
+        }
+        path.ray.o = offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t);
+        path.ray.d = direction;
+        path.depth = depth + 1;
+    }
+    return acc;
+}
+
+// Reverse sweep over the recorded vertices: S_k is the radiance from vertex k
+// on per unit throughput at k; gradients need no division.
+void diffReverse(const std::vector<DiffVertex>& vs, const V3<double>& delta, const DiffSetup& s, double* grad) {
+    V3<double> S;  // S_{k+1}
+    const double invPi = double(0.31830988618f);
+    double* gTex = grad;
+    double* gRough = grad + s.texFloats;
+    double* gEmit = grad + s.texFloats + s.roughCount;
+    for (size_t i = vs.size(); i-- > 0;) {
+        const DiffVertex& v = vs[i];
+        V3<double> next = S;
+        if (v.kind == 1) {
+            double e = v.eslot == kInvalid ? 1.0 : s.params[s.texFloats + s.roughCount + v.eslot];
+            if (v.eslot != kInvalid) gEmit[v.eslot] += dot(delta, hadamard(v.T, v.em));
+            S = v.em * e;
+        } else if (v.kind == 3) {
+            S = hadamard(v.weight, next);
+        } else {
+            V3<double> texGrad;
+            double roughGrad = 0.0;
+            V3<double> own;
+            if (v.nee) {
+                V3<double> fb = hadamard(v.neeF, v.neeBase);
+                own = fb * v.neeE;
+                if (v.neeSlot != kInvalid) gEmit[v.neeSlot] += dot(delta, hadamard(v.T, fb));
+                V3<double> db = hadamard(delta, hadamard(v.T, v.neeBase)) * v.neeE;
+                texGrad += db * invPi;
+                roughGrad += (db.x + db.y + db.z) * v.neeDAlpha;
+            }
+            if (v.cont) {
+                V3<double> dn = hadamard(delta, hadamard(v.T, next));
+                texGrad += dn * (v.cosOverPdf * invPi);
+                roughGrad += (dn.x + dn.y + dn.z) * v.contDAlpha * v.cosOverPdf;
+                own += hadamard(v.weight, next);
+            }
+            if (v.textured)
+                for (int k = 0; k < 4; ++k) {
+                    double* t = gTex + 3u * static_cast<size_t>(v.fp.index[k]);
+                    t[0] += texGrad.x * v.fp.weight[k];
+                    t[1] += texGrad.y * v.fp.weight[k];
+                    t[2] += texGrad.z * v.fp.weight[k];
+                }
+            if (v.rslot != kInvalid) gRough[v.rslot] += roughGrad;
+            S = own;
+        }
+    }
+}
+
+template <class F>
+void parallelRows(uint32_t height, uint32_t threads, F&& body) {
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    threads = std::min(threads, std::max(1u, height));
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] {
+            for (uint32_t y = t; y < height; y += threads) body(t, y);
+        });
+    for (std::thread& t : pool) t.join();
+}
+
 }  // namespace
 
 extern "C" {
@@ -1372,6 +1758,90 @@ void oracle_eval_specgloss(const float f0[3], const float diffuse[3], float roug
     value[0] = r.x;
     value[1] = r.y;
     value[2] = r.z;
+}
+
+
+uint32_t oracle_diff_layout(const oracle_handle* h, const oracle_diff_config* c, uint32_t* texture_floats,
+    uint32_t* roughness_count, uint32_t* emission_count) {
+    DiffSetup s = diffSetup(*h, *c, nullptr);
+    if (texture_floats) *texture_floats = s.texFloats;
+    if (roughness_count) *roughness_count = s.roughCount;
+    if (emission_count) *emission_count = s.emitCount;
+    return s.texFloats + s.roughCount + s.emitCount;
+}
+
+int oracle_diff_render(const oracle_handle* h, const wpt_camera* cam, const oracle_diff_config* c, const float* params,
+    uint32_t sample_start, uint32_t precision, uint32_t threads, float* image) {
+    DiffSetup s = diffSetup(*h, *c, params);
+    parallelRows(c->height, threads, [&](uint32_t, uint32_t y) {
+        for (uint32_t x = 0; x < c->width; ++x) {
+            float* px = image + 3 * (static_cast<size_t>(y) * c->width + x);
+            if (precision == 64) {
+                V3<double> sum;
+                for (uint32_t k = 0; k < c->samples_per_pixel; ++k)
+                    sum += diffPathOracle<double, double>(*h, s, *cam, x, y, sample_start + k, nullptr);
+                double inv = 1.0 / c->samples_per_pixel;
+                px[0] = static_cast<float>(sum.x * inv);
+                px[1] = static_cast<float>(sum.y * inv);
+                px[2] = static_cast<float>(sum.z * inv);
+            } else {
+                V3<float> sum;
+                for (uint32_t k = 0; k < c->samples_per_pixel; ++k)
+                    sum += diffPathOracle<float, float>(*h, s, *cam, x, y, sample_start + k, nullptr);
+                float inv = 1.0f / static_cast<float>(c->samples_per_pixel);
+                px[0] = sum.x * inv;
+                px[1] = sum.y * inv;
+                px[2] = sum.z * inv;
+            }
+        }
+    });
+    return 0;
+}
+
+int oracle_diff_adjoint64(const oracle_handle* h, const wpt_camera* cam, const oracle_diff_config* c,
+    const float* params, uint32_t sample_start, const float* image_grad, uint32_t threads, double* grad) {
+    DiffSetup s = diffSetup(*h, *c, params);
+    size_t count = static_cast<size_t>(s.texFloats) + s.roughCount + s.emitCount;
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    threads = std::min(threads, std::max(1u, c->height));
+    std::vector<std::vector<double>> partial(threads, std::vector<double>(count, 0.0));
+    parallelRows(c->height, threads, [&](uint32_t t, uint32_t y) {
+        std::vector<DiffVertex> vs;
+        for (uint32_t x = 0; x < c->width; ++x) {
+            const float* g = image_grad + 3 * (static_cast<size_t>(y) * c->width + x);
+            double inv = 1.0 / c->samples_per_pixel;
+            V3<double> delta(g[0] * inv, g[1] * inv, g[2] * inv);
+            for (uint32_t k = 0; k < c->samples_per_pixel; ++k) {
+                vs.clear();
+                diffPathOracle<double, double>(*h, s, *cam, x, y, sample_start + k, &vs);
+                diffReverse(vs, delta, s, partial[t].data());
+            }
+        }
+    });
+    for (size_t i = 0; i < count; ++i) {
+        double sum = 0.0;
+        for (uint32_t t = 0; t < threads; ++t) sum += partial[t][i];
+        grad[i] = sum;
+    }
+    return 0;
+}
+
+int oracle_diff_directional64(const oracle_handle* h, const wpt_camera* cam, const oracle_diff_config* c,
+    const float* params, const float* direction, uint32_t sample_start, uint32_t threads, double* dimage) {
+    DiffSetup s = diffSetup(*h, *c, params);
+    s.direction = direction;
+    parallelRows(c->height, threads, [&](uint32_t, uint32_t y) {
+        for (uint32_t x = 0; x < c->width; ++x) {
+            V3<Dual> sum(Dual(0), Dual(0), Dual(0));
+            for (uint32_t k = 0; k < c->samples_per_pixel; ++k)
+                sum += diffPathOracle<double, Dual>(*h, s, *cam, x, y, sample_start + k, nullptr);
+            double* px = dimage + 3 * (static_cast<size_t>(y) * c->width + x);
+            px[0] = sum.x.d / c->samples_per_pixel;
+            px[1] = sum.y.d / c->samples_per_pixel;
+            px[2] = sum.z.d / c->samples_per_pixel;
+        }
+    });
+    return 0;
 }
 
 }  // extern "C"

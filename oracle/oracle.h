@@ -91,6 +91,29 @@ float oracle_dielectric_reflectance(float cosine, float eta);
 void oracle_eval_specgloss(const float f0[3], const float diffuse[3], float roughness, const float n[3],
                            const float wo[3], const float wi[3], float value[3], float* pdf);
 
+/* Differentiable estimator of wpt_diff (no reference counterpart), with
+ * the parameter vector laid out as texture | roughness | emission. */
+typedef struct oracle_diff_config {
+    uint32_t width, height, samples_per_pixel, max_depth, rng_strategy;
+    uint32_t texture_width, texture_height, textured_material;
+} oracle_diff_config;
+
+uint32_t oracle_diff_layout(const oracle_handle* h, const oracle_diff_config* c, uint32_t* texture_floats,
+                            uint32_t* roughness_count, uint32_t* emission_count);
+/* Mean image (w*h*3) in FP32 (precision 32) or FP64 (64). */
+int oracle_diff_render(const oracle_handle* h, const wpt_camera* cam, const oracle_diff_config* c, const float* params,
+                       uint32_t sample_start, uint32_t precision, uint32_t threads, float* image);
+/* FP64 reverse-mode adjoint over recorded path vertices: grad = dLoss/dparams
+ * for dLoss/dimage = image_grad. */
+int oracle_diff_adjoint64(const oracle_handle* h, const wpt_camera* cam, const oracle_diff_config* c,
+                          const float* params, uint32_t sample_start, const float* image_grad, uint32_t threads,
+                          double* grad);
+/* FP64 forward-mode (dual number) derivative of the image along a parameter
+ * direction. */
+int oracle_diff_directional64(const oracle_handle* h, const wpt_camera* cam, const oracle_diff_config* c,
+                              const float* params, const float* direction, uint32_t sample_start, uint32_t threads,
+                              double* dimage);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,11 @@ class OracleScene(C.Structure):
                 ("blas_node_offset", C.c_void_p), ("blas_node_count", C.c_void_p)]
 
 
+class OracleDiffConfig(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("width", "height", "samples_per_pixel", "max_depth", "rng_strategy",
+                                          "texture_width", "texture_height", "textured_material")]
+
+
 class OracleStats(C.Structure):
     _fields_ = [("closest_queries", C.c_uint64), ("shadow_queries", C.c_uint64), ("paths", C.c_uint64)] + [
         (f"{kind}_{what}", C.c_uint64) for kind in ("closest", "shadow")
@@ -53,6 +58,13 @@ def library():
             ("oracle_render", [P, C.POINTER(wpt.Camera), C.POINTER(wpt.RenderConfig), u32, u32, u32, u32, u32, u32,
                                u32, u32, P, C.POINTER(OracleStats)], C.c_int),
             ("oracle_compact", [P, P, u32, P, u32], u32),
+            ("oracle_diff_layout", [P, C.POINTER(OracleDiffConfig), P, P, P], u32),
+            ("oracle_diff_render", [P, C.POINTER(wpt.Camera), C.POINTER(OracleDiffConfig), P, u32, u32, u32, P],
+             C.c_int),
+            ("oracle_diff_adjoint64", [P, C.POINTER(wpt.Camera), C.POINTER(OracleDiffConfig), P, u32, P, u32, P],
+             C.c_int),
+            ("oracle_diff_directional64", [P, C.POINTER(wpt.Camera), C.POINTER(OracleDiffConfig), P, P, u32, u32, P],
+             C.c_int),
             ("oracle_reference_rays", [u32, u32, P, u32], u32),
             ("oracle_rng_stream", [u32, u32, u32, u32, P], C.c_uint64),
             ("oracle_dielectric_reflectance", [C.c_float, C.c_float], C.c_float),
@@ -172,6 +184,41 @@ class Oracle:
         n = rays6.shape[1]
         out = np.zeros(n, dtype=np.uint8)
         library().oracle_occluded(self._h, rays6.ctypes.data, tmax.ctypes.data, n, out.ctypes.data)
+        return out
+
+    # ---- differentiable estimator (parameters: texture | roughness | emission)
+    @staticmethod
+    def diff_config(width, height, spp, max_depth, texture_size=(0, 0), textured_material=wpt.INVALID,
+                    rng=wpt.RNG_PCG32):
+        return OracleDiffConfig(width, height, spp, max_depth, rng, texture_size[0], texture_size[1],
+                                textured_material)
+
+    def diff_layout(self, cfg):
+        t, r, e = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        library().oracle_diff_layout(self._h, C.byref(cfg), C.byref(t), C.byref(r), C.byref(e))
+        return t.value, r.value, e.value
+
+    def diff_render(self, camera, cfg, params, sample_start=0, precision=32, threads=0):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        img = np.zeros((cfg.height, cfg.width, 3), dtype=np.float32)
+        library().oracle_diff_render(self._h, C.byref(camera), C.byref(cfg), params.ctypes.data, sample_start,
+                                     precision, threads, img.ctypes.data)
+        return img
+
+    def diff_adjoint64(self, camera, cfg, params, image_grad, sample_start=0, threads=0):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        image_grad = np.ascontiguousarray(image_grad, dtype=np.float32)
+        grad = np.zeros(params.size, dtype=np.float64)
+        library().oracle_diff_adjoint64(self._h, C.byref(camera), C.byref(cfg), params.ctypes.data, sample_start,
+                                        image_grad.ctypes.data, threads, grad.ctypes.data)
+        return grad
+
+    def diff_directional64(self, camera, cfg, params, direction, sample_start=0, threads=0):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        direction = np.ascontiguousarray(direction, dtype=np.float32)
+        out = np.zeros((cfg.height, cfg.width, 3), dtype=np.float64)
+        library().oracle_diff_directional64(self._h, C.byref(camera), C.byref(cfg), params.ctypes.data,
+                                            direction.ctypes.data, sample_start, threads, out.ctypes.data)
         return out
 
     def brute_fp64(self, rays6: np.ndarray):

@@ -183,3 +183,63 @@ def test_floor_under_uniform_emitter_energy(oracle_mod, kind, roughness):
     else:
         # GGX reflectance at roughness 0.3 loses energy to masking, never gains.
         assert np.all(mean < rho + 5 * sem.max()) and np.all(mean > 0.5 * rho), (mean, sem)
+
+
+# ---- differentiable estimator (no reference counterpart; pinned by calculus)
+
+def diff_setup(oracle_mod, size=24, spp=2, texture=(12, 12), seed=1):
+    host = wpt.HostScene("diff-cornell")
+    scene = wpt.Scene(host.desc(), device=False, owner=host)
+    oracle = oracle_mod.Oracle(scene)
+    cfg = oracle.diff_config(size, size, spp, 6, texture, 5)
+    t, r, e = oracle.diff_layout(cfg)
+    rng = np.random.default_rng(seed)
+    params = np.concatenate([rng.uniform(0.2, 0.8, t), rng.uniform(0.1, 0.9, r),
+                             rng.uniform(0.5, 1.5, e)]).astype(np.float32)
+    return host, scene, oracle, cfg, host.camera(size, size), params, (t, r, e)
+
+
+def test_diff_layout_of_the_parameter_study_scene(oracle_mod):
+    *_, (t, r, e) = diff_setup(oracle_mod, texture=(512, 512))
+    assert (t, r, e) == (512 * 512 * 3, 64, 8)
+
+
+def test_diff_reverse_adjoint_equals_forward_mode(oracle_mod):
+    # The FP64 reverse sweep (hand-derived GGX derivative, no division) and
+    # dual-number forward mode (derivative of the value expressions
+    # themselves) agree along random directions.
+    host, scene, oracle, cfg, cam, params, _ = diff_setup(oracle_mod)
+    img = oracle.diff_render(cam, cfg, params, precision=64)
+    g_img = (2.0 * (img - 0.9 * img) / img.size).astype(np.float32)
+    grad = oracle.diff_adjoint64(cam, cfg, params, g_img)
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        v = rng.normal(size=params.size).astype(np.float32)
+        dual = float((g_img.astype(np.float64) * oracle.diff_directional64(cam, cfg, params, v)).sum())
+        assert abs(grad @ v.astype(np.float64) - dual) <= 1e-9 * abs(dual)
+
+
+def test_diff_forward_mode_matches_central_differences(oracle_mod):
+    host, scene, oracle, cfg, cam, params, (t, r, e) = diff_setup(oracle_mod)
+    rng = np.random.default_rng(9)
+    for section in (slice(0, t), slice(t, t + r), slice(t + r, t + r + e)):
+        v = np.zeros(params.size, np.float32)
+        v[section] = rng.normal(size=v[section].size)
+        eps = 1e-3
+        plus = oracle.diff_render(cam, cfg, params + eps * v, precision=64).astype(np.float64)
+        minus = oracle.diff_render(cam, cfg, params - eps * v, precision=64).astype(np.float64)
+        fd = (plus - minus) / (2 * eps)
+        dual = oracle.diff_directional64(cam, cfg, params, v)
+        assert np.abs(fd - dual).sum() <= 2e-3 * np.abs(dual).sum()
+
+
+def test_diff_image_is_linear_in_emission(oracle_mod):
+    # Every path's radiance is a sum of terms that each carry exactly one
+    # emission scale, so the image is homogeneous of degree 1 in the scales
+    # and its derivative along the scales themselves is the image (Euler).
+    host, scene, oracle, cfg, cam, params, (t, r, e) = diff_setup(oracle_mod)
+    img = oracle.diff_render(cam, cfg, params, precision=64).astype(np.float64)
+    v = np.zeros(params.size, np.float32)
+    v[t + r:] = params[t + r:]
+    dual = oracle.diff_directional64(cam, cfg, params, v)
+    assert np.allclose(dual, img, rtol=1e-6, atol=1e-7)

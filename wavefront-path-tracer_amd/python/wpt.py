@@ -591,7 +591,9 @@ class Batch:
 class DiffRenderer:
     """Differentiable renderer (wpt_diff_*): forward image and path-replay
     adjoint over one parameter vector [texture | roughness | emission].
-    Device buffers are torch tensors on cuda:0; every call synchronizes."""
+    Device buffers are torch tensors on cuda:0; every call synchronizes.
+    torch ships its own HIP runtime: initialise torch's device (any CUDA
+    tensor) before this library's first device call in the process."""
 
     def __init__(self, scene: "Scene", camera: Camera, width: int, height: int, spp: int, max_depth: int,
                  texture_size=(0, 0), textured_material: int = INVALID, rng: int = RNG_PCG32, block_size: int = 256):
