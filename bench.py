@@ -270,16 +270,16 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is a design, "
               f"not a measured configuration", file=sys.stderr)
         sys.exit(2)
-    if wpt.device_count() == 0:
-        print("bench.py needs a gfx950 device", file=sys.stderr)
-        sys.exit(1)
-
     if args.calibration_copy_mib or not args.skip_diff:
         # torch ships its own HIP runtime beside the one libwpt links; it
         # must initialise the device before libwpt's first device call.
         import torch
         torch.zeros(1, device="cuda:0")
         torch.cuda.synchronize()
+    if wpt.device_count() == 0:
+        print("bench.py needs a gfx950 device", file=sys.stderr)
+        sys.exit(1)
+
     if args.calibration_copy_mib:
         import torch
         n = args.calibration_copy_mib * (1 << 20) // 4

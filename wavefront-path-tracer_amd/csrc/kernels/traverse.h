@@ -330,6 +330,13 @@ WPT_HD void instanceClosest(const DevScene& s, const Ray& ray, uint32_t index, b
 
 // Closest hit in the reference's category order: direct spheres, direct
 // instances, static triangles, TLAS, flat triangles.
+//
+// The TLAS and the BLASes of its instance leaves are walked by one loop: on
+// an instance leaf whose BLAS root box is hit, the loop switches its node
+// array, ray and stack floor to the BLAS; when the BLAS level is exhausted
+// it switches back and resumes popping the TLAS.  The visit order is that of
+// the nested walk (each BLAS finishes before the TLAS resumes), but lanes at
+// different levels now run the same node-visit code in the same iteration.
 template <class Stack>
 WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
     HitRecord best = emptyHit();
@@ -341,17 +348,62 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
         float nearT;
         if (slab(r, s.tlas.rootMin[0], s.tlas.rootMin[1], s.tlas.rootMin[2], s.tlas.rootMax[0], s.tlas.rootMax[1],
                 s.tlas.rootMax[2], best.t, nearT)) {
-            uint32_t base = stack.size;
+            const uint32_t base = stack.size;
+            uint32_t blasBase = base;
             uint32_t cur = s.tlas.root;
+            const BvhNode* nodes = s.tlas.nodes;
+            const float4* geometry = nullptr;
+            uint32_t instance = kInvalid;  // kInvalid while in the TLAS
+            uint32_t materialOverride = kInvalid;
             while (true) {
                 if (cur & kLeafBit) {
-                    uint32_t leaf = s.tlasLeaves[cur & ~kLeafBit];
-                    if (leaf & kTlasSphereBit)
-                        sphereLeaf(s, ray, leaf & ~kTlasSphereBit, best);
-                    else
-                        instanceClosest(s, ray, leaf, true, best, stack);
+                    uint32_t slot = cur & ~kLeafBit;
+                    if (instance != kInvalid) {
+                        float4 a = geometry[3 * slot + 0];
+                        float4 b = geometry[3 * slot + 1];
+                        float4 c = geometry[3 * slot + 2];
+                        float t, u, v;
+                        if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < best.t) {
+                            uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
+                            if (!s.materialCutout[material]) {
+                                best.t = t;
+                                best.u = u;
+                                best.v = v;
+                                best.kind = kHitInstance;
+                                best.instance = instance;
+                                best.prim = wbits(a);
+                                best.material = material;
+                            }
+                        }
+                    } else {
+                        uint32_t leaf = s.tlasLeaves[slot];
+                        if (leaf & kTlasSphereBit) {
+                            sphereLeaf(s, ray, leaf & ~kTlasSphereBit, best);
+                        } else {
+                            const Instance& inst = s.instances[leaf];
+                            const Blas& blas = s.blases[inst.mesh];
+                            Ray local = toLocal(inst.transform, ray);
+                            if (blas.bvh.primCount != 0) {
+                                RayInv rl = makeRayInv(local);
+                                if (slab(rl, blas.bvh.rootMin[0], blas.bvh.rootMin[1], blas.bvh.rootMin[2],
+                                        blas.bvh.rootMax[0], blas.bvh.rootMax[1], blas.bvh.rootMax[2], best.t, nearT)) {
+                                    r = rl;
+                                    instance = leaf;
+                                    nodes = blas.bvh.nodes;
+                                    geometry = blas.geometry;
+                                    materialOverride = inst.materialOverride;
+                                    blasBase = stack.size;
+                                    cur = blas.bvh.root;
+                                    continue;
+                                }
+                            } else {
+                                closestBrute(local, blas.triangles, blas.triangleCount, s.materialCutout,
+                                    inst.materialOverride, kHitInstance, leaf, best);
+                            }
+                        }
+                    }
                 } else {
-                    const BvhNode& n = s.tlas.nodes[cur];
+                    const BvhNode& n = nodes[cur];
                     float tl, tr;
                     bool hl = slab(r, n.lminx, n.lminy, n.lminz, n.lmaxx, n.lmaxy, n.lmaxz, best.t, tl);
                     bool hr = slab(r, n.rminx, n.rminy, n.rminz, n.rmaxx, n.rmaxy, n.rmaxz, best.t, tr);
@@ -369,6 +421,12 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
                         cur = n.right;
                         continue;
                     }
+                }
+                if (instance != kInvalid) {
+                    if (popLive(stack, blasBase, best.t, cur)) continue;
+                    instance = kInvalid;
+                    nodes = s.tlas.nodes;
+                    r = makeRayInv(ray);
                 }
                 if (!popLive(stack, base, best.t, cur)) break;
             }
@@ -406,30 +464,59 @@ WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, 
         float nearT;
         if (slab(r, s.tlas.rootMin[0], s.tlas.rootMin[1], s.tlas.rootMin[2], s.tlas.rootMax[0], s.tlas.rootMax[1],
                 s.tlas.rootMax[2], maxDistance, nearT)) {
-            uint32_t base = stack.size;
+            // One loop over the TLAS and its instances' BLASes, as in
+            // traceClosest; any blocker ends the query.
+            const uint32_t base = stack.size;
+            uint32_t blasBase = base;
             uint32_t cur = s.tlas.root;
+            const BvhNode* nodes = s.tlas.nodes;
+            const float4* geometry = nullptr;
+            uint32_t instance = kInvalid;
+            uint32_t materialOverride = kInvalid;
             while (true) {
                 if (cur & kLeafBit) {
-                    uint32_t leaf = s.tlasLeaves[cur & ~kLeafBit];
-                    if (leaf & kTlasSphereBit) {
-                        float4 sp = s.spheres[leaf & ~kTlasSphereBit];
-                        float t;
-                        if (intersectSphere(ray, xyz(sp), sp.w, t) && t < maxDistance) {
-                            stack.truncate(base);
-                            return true;
+                    uint32_t slot = cur & ~kLeafBit;
+                    if (instance != kInvalid) {
+                        float4 a = geometry[3 * slot + 0];
+                        float4 b = geometry[3 * slot + 1];
+                        float4 c = geometry[3 * slot + 2];
+                        float t, u, v;
+                        if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < maxDistance) {
+                            uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
+                            if (!s.materialCutout[material]) {
+                                stack.truncate(base);
+                                return true;
+                            }
                         }
                     } else {
-                        const Instance& inst = s.instances[leaf];
-                        const Blas& blas = s.blases[inst.mesh];
-                        RayInv local = makeRayInv(toLocal(inst.transform, ray));
-                        if (anyInBvh(local, blas.bvh, blas.geometry, s.materialCutout, inst.materialOverride,
-                                maxDistance, stack)) {
-                            stack.truncate(base);
-                            return true;
+                        uint32_t leaf = s.tlasLeaves[slot];
+                        if (leaf & kTlasSphereBit) {
+                            float4 sp = s.spheres[leaf & ~kTlasSphereBit];
+                            float t;
+                            if (intersectSphere(ray, xyz(sp), sp.w, t) && t < maxDistance) {
+                                stack.truncate(base);
+                                return true;
+                            }
+                        } else {
+                            const Instance& inst = s.instances[leaf];
+                            const Blas& blas = s.blases[inst.mesh];
+                            RayInv rl = makeRayInv(toLocal(inst.transform, ray));
+                            if (slab(rl, blas.bvh.rootMin[0], blas.bvh.rootMin[1], blas.bvh.rootMin[2],
+                                    blas.bvh.rootMax[0], blas.bvh.rootMax[1], blas.bvh.rootMax[2], maxDistance,
+                                    nearT)) {
+                                r = rl;
+                                instance = leaf;
+                                nodes = blas.bvh.nodes;
+                                geometry = blas.geometry;
+                                materialOverride = inst.materialOverride;
+                                blasBase = stack.size;
+                                cur = blas.bvh.root;
+                                continue;
+                            }
                         }
                     }
                 } else {
-                    const BvhNode& n = s.tlas.nodes[cur];
+                    const BvhNode& n = nodes[cur];
                     float tl, tr;
                     bool hl = slab(r, n.lminx, n.lminy, n.lminz, n.lmaxx, n.lmaxy, n.lmaxz, maxDistance, tl);
                     bool hr = slab(r, n.rminx, n.rminy, n.rminz, n.rmaxx, n.rmaxy, n.rmaxz, maxDistance, tr);
@@ -447,6 +534,12 @@ WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, 
                         cur = n.right;
                         continue;
                     }
+                }
+                if (instance != kInvalid) {
+                    if (popLive(stack, blasBase, maxDistance, cur)) continue;
+                    instance = kInvalid;
+                    nodes = s.tlas.nodes;
+                    r = makeRayInv(ray);
                 }
                 if (!popLive(stack, base, maxDistance, cur)) break;
             }
