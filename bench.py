@@ -9,14 +9,18 @@ geodesic icosphere of 1,003,520 triangles.  1920x1080, 64 spp, depth 12,
 Russian roulette from depth 3, PCG32 streams, indirect clamp 10.
 
 One step = one full frame (64 spp) rendered into the device framebuffer by
-the hipGraph-captured staged wavefront; the scene is resident in HBM before
-the timed region.  `value` is Mrays/s = (closest-hit + shadow queries) / s;
-Msamples/s and the persistent-kernel rate are reported beside it.
+the chosen scheduler (default: the persistent kernel with path
+regeneration); the scene is resident in HBM before the timed region.
+`value` is Mrays/s = (closest-hit + shadow queries) / s; Msamples/s and the
+rates of the other two schedulers (hipGraph-captured staged wavefront,
+cooperative persistent kernel), with a check that their images are
+identical, are reported beside it.
 
-The roofline object is for the dominant kernel (closest-hit trace), timed with
-HIP events on the renderer's stream; its algorithmic bytes per query follow
-the reference algorithm's work counts (node tests, triangle/sphere tests,
-instance entries) measured by the CPU oracle on the same scene and BVH.
+The roofline object is for the dominant kernel (the regenerating kernel, or
+the closest-hit trace of the staged wavefront), timed with HIP events on the
+renderer's stream; its algorithmic bytes per query follow the reference
+algorithm's work counts (node tests, triangle/sphere tests, instance
+entries) measured by the CPU oracle on the same scene and BVH.
 The CPU baseline is the oracle (the scalar FP32 restatement of the reference
 traversal and shading) on the host cores, over a bounded sample.
 """
@@ -252,8 +256,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=SPP)
-    ap.add_argument("--mode", choices=["graph", "persistent"], default="graph")
-    ap.add_argument("--skip-persistent", action="store_true", help="do not time the persistent kernel beside")
+    ap.add_argument("--mode", choices=["regenerate", "graph", "persistent"], default="regenerate",
+                    help="scheduler of the timed frames; the other two are timed beside it")
+    ap.add_argument("--skip-persistent", action="store_true", help="do not time the other schedulers beside")
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--cpu-spp", type=int, default=8, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--profile-spp", type=int, default=4, help="samples rendered with per-kernel events")
@@ -301,7 +306,8 @@ def main():
         f"{host.desc().mesh_triangle_count} BLAS triangles x {info.instance_count} instances, "
         f"{info.sphere_count} spheres ({info.direct_sphere_count} direct)")
     camera = host.camera(WIDTH, HEIGHT)
-    mode = wpt.MODE_GRAPH if args.mode == "graph" else wpt.MODE_PERSISTENT
+    modes = {"regenerate": wpt.MODE_REGENERATE, "graph": wpt.MODE_GRAPH, "persistent": wpt.MODE_PERSISTENT}
+    mode = modes[args.mode]
     cfg = wpt.render_config(WIDTH, HEIGHT, spp=args.spp, max_depth=DEPTH, rr_start=RR_START, clamp=CLAMP, mode=mode)
     renderer = wpt.Renderer(scene, camera, cfg)
 
@@ -323,40 +329,46 @@ def main():
     value = queries * args.steps / wall / 1e6
     msamples = WIDTH * HEIGHT * args.spp * args.steps / wall / 1e6
     image, samples = renderer.framebuffer()
-    log(f"graph: {value:.1f} Mrays/s, {msamples:.1f} Msamples/s, {ms_per_step:.1f} ms/frame, "
+    log(f"{args.mode}: {value:.1f} Mrays/s, {msamples:.1f} Msamples/s, {ms_per_step:.1f} ms/frame, "
         f"{queries / (WIDTH * HEIGHT * args.spp):.3f} queries/path, mean radiance {image.mean() / samples:.4f}")
 
-    # Per-kernel device time (events around each launch, no graph).
+    # Per-kernel device time (events around each launch; no graph).  The
+    # regenerating and persistent schedulers are one launch per sample range.
     kernel_ms, launches = renderer.profile_kernels(0, args.profile_spp)
     prof_stats = renderer.stats()
-    trace_ms = kernel_ms[2] if mode == wpt.MODE_GRAPH else kernel_ms[4]
-    trace_launches = int(launches[2] if mode == wpt.MODE_GRAPH else launches[4])
+    trace_slot = 2 if mode == wpt.MODE_GRAPH else 4
+    trace_ms = kernel_ms[trace_slot]
+    trace_launches = int(launches[trace_slot])
     log(f"kernels over {args.profile_spp} spp: primary {kernel_ms[0]:.2f} ms, reset {kernel_ms[1]:.2f} ms, "
         f"trace {kernel_ms[2]:.2f} ms ({int(launches[2])} launches), shade {kernel_ms[3]:.2f} ms, "
-        f"persistent {kernel_ms[4]:.2f} ms, shadow {kernel_ms[5]:.2f} ms ({int(launches[5])} launches), "
-        f"compact {kernel_ms[6]:.2f} ms")
+        f"persistent/regenerate {kernel_ms[4]:.2f} ms ({int(launches[4])} launches), shadow {kernel_ms[5]:.2f} ms "
+        f"({int(launches[5])} launches)")
+    renderer.close()
 
-    persistent = None
-    if not args.skip_persistent and mode == wpt.MODE_GRAPH:
-        pcfg = wpt.render_config(WIDTH, HEIGHT, spp=args.spp, max_depth=DEPTH, rr_start=RR_START, clamp=CLAMP,
-                                 mode=wpt.MODE_PERSISTENT)
-        pr = wpt.Renderer(scene, camera, pcfg)
-        pr.reset()
-        pr.render_samples(0, args.spp)
-        tp = time.perf_counter()
-        pr.reset()
-        pr.render_samples(0, args.spp)
-        pwall = time.perf_counter() - tp
-        pst = pr.stats()
-        pimg, _ = pr.framebuffer()
-        persistent = {
-            "mrays_per_s": round((pst.closest_queries + pst.shadow_queries) / pwall / 1e6, 2),
-            "msamples_per_s": round(WIDTH * HEIGHT * args.spp / pwall / 1e6, 2),
-            "ms_per_frame": round(pwall * 1000.0, 2),
-            "image_identical_to_graph": bool(np.array_equal(pimg, image)),
-        }
-        log(f"persistent: {persistent}")
-        pr.close()
+    others = {}
+    if not args.skip_persistent:
+        for name, m in modes.items():
+            if m == mode:
+                continue
+            ocfg = wpt.render_config(WIDTH, HEIGHT, spp=args.spp, max_depth=DEPTH, rr_start=RR_START, clamp=CLAMP,
+                                     mode=m)
+            orr = wpt.Renderer(scene, camera, ocfg)
+            orr.reset()
+            orr.render_samples(0, args.spp)
+            tp = time.perf_counter()
+            orr.reset()
+            orr.render_samples(0, args.spp)
+            owall = time.perf_counter() - tp
+            ost = orr.stats()
+            oimg, _ = orr.framebuffer()
+            others[name] = {
+                "mrays_per_s": round((ost.closest_queries + ost.shadow_queries) / owall / 1e6, 2),
+                "msamples_per_s": round(WIDTH * HEIGHT * args.spp / owall / 1e6, 2),
+                "ms_per_frame": round(owall * 1000.0, 2),
+                "image_identical": bool(np.array_equal(oimg, image)),
+            }
+            log(f"{name}: {others[name]}")
+            orr.close()
 
     cpu = None
     work = None
@@ -375,11 +387,21 @@ def main():
 
     roofline = None
     if work is not None and trace_ms > 0:
-        per_query = (NODE_B * work.closest_node_tests + TRI_B * work.closest_triangle_tests +
-                     SPHERE_B * work.closest_sphere_tests + INSTANCE_B * work.closest_instance_entries) / \
-            max(1, work.closest_queries) + RAY_IN_B + HIT_OUT_B
-        prof_queries = prof_stats.closest_queries
-        achieved = per_query * prof_queries / (trace_ms / 1000.0) / 1e9
+        def traversal_bytes(nodes, tris, spheres, instances, count):
+            return (NODE_B * nodes + TRI_B * tris + SPHERE_B * spheres + INSTANCE_B * instances) / max(1, count)
+        closest_b = traversal_bytes(work.closest_node_tests, work.closest_triangle_tests, work.closest_sphere_tests,
+                                    work.closest_instance_entries, work.closest_queries)
+        shadow_b = traversal_bytes(work.shadow_node_tests, work.shadow_triangle_tests, work.shadow_sphere_tests,
+                                   work.shadow_instance_entries, work.shadow_queries)
+        if mode == wpt.MODE_GRAPH:
+            # kTrace: closest-hit traversal plus its ray in and hit out.
+            per_query = closest_b + RAY_IN_B + HIT_OUT_B
+            algorithmic = per_query * prof_stats.closest_queries
+        else:
+            # One kernel does every query; paths stay in registers.
+            per_query = closest_b
+            algorithmic = closest_b * prof_stats.closest_queries + shadow_b * prof_stats.shadow_queries
+        achieved = algorithmic / (trace_ms / 1000.0) / 1e9
         roofline = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -387,8 +409,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "kTrace" if mode == wpt.MODE_GRAPH else "kPersistent",
-            "bytes_per_query": round(per_query, 1),
+            "kernel": {wpt.MODE_GRAPH: "kTrace", wpt.MODE_PERSISTENT: "kPersistent",
+                       wpt.MODE_REGENERATE: "kBatch"}[mode],
+            "bytes_per_closest_query": round(per_query, 1),
             "avg_launch_ms": round(trace_ms / max(1, trace_launches), 4),
             "launches": trace_launches,
         }
@@ -409,12 +432,12 @@ def main():
         "config": {
             "workload": "sphere-field 1920x1080 64spp depth12 rr3 (3x 1,003,520-triangle icosphere + 485 spheres)",
             "width": WIDTH, "height": HEIGHT, "spp": args.spp, "max_depth": DEPTH, "rr_start": RR_START,
-            "rng": "pcg32", "clamp": CLAMP, "mode": args.mode,
+            "rng": "pcg32", "clamp": CLAMP, "scheduler": args.mode,
         },
         "msamples_per_s": round(msamples, 2),
         "queries_per_frame": int(queries),
         "device_ms_per_frame": round(float(np.mean(device_ms)), 2),
-        "persistent": persistent,
+        "other_schedulers": others,
         "batch": batch,
         "diff": diff,
         "roofline": roofline,

@@ -52,9 +52,14 @@ extern "C" {
 #define WPT_RNG_PCG32 1u
 
 /* Scheduler: the hipGraph-captured staged wavefront or the persistent
- * cooperative kernel (reference render_driver.cu:264-348). */
+ * cooperative kernel (reference render_driver.cu:264-348), or a persistent
+ * kernel with path regeneration and no barrier between bounces (no
+ * reference counterpart): each lane renders its pixel's samples one bounce
+ * per iteration and starts the next sample when a path ends.  All three
+ * produce bit-identical framebuffers. */
 #define WPT_MODE_GRAPH 0u
 #define WPT_MODE_PERSISTENT 1u
+#define WPT_MODE_REGENERATE 2u
 
 /* Closest-hit categories, in the reference's test order
  * (device_traversal.cuh:1129-1149). */

@@ -29,7 +29,10 @@ def setup(preset, width, height, spp, depth, rr, freq=0, mode=wpt.MODE_GRAPH, ti
     return host, scene, cam, cfg
 
 
-@pytest.mark.parametrize("mode", [wpt.MODE_GRAPH, wpt.MODE_PERSISTENT])
+MODES = [wpt.MODE_GRAPH, wpt.MODE_PERSISTENT, wpt.MODE_REGENERATE]
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_cornell_box_matches_oracle(torch_cuda, oracle_mod, mode):
     # Config 0 shape: procedural unit Cornell box, 16 spp, depth 8, RR from 3,
     # fixed seed; resolution reduced to 128^2 to keep the oracle fast.
@@ -50,7 +53,7 @@ def test_cornell_box_matches_oracle(torch_cuda, oracle_mod, mode):
     assert err64 < 5e-2
 
 
-@pytest.mark.parametrize("mode", [wpt.MODE_GRAPH, wpt.MODE_PERSISTENT])
+@pytest.mark.parametrize("mode", MODES)
 def test_sphere_field_matches_oracle(torch_cuda, oracle_mod, mode):
     host, scene, cam, cfg = setup("sphere-field", 160, 90, 2, 12, 3, freq=24, mode=mode)
     r = wpt.Renderer(scene, cam, cfg)
@@ -68,10 +71,11 @@ def test_other_presets_match_oracle(torch_cuda, oracle_mod, preset):
     assert np.array_equal(img, want)
 
 
-def test_tiled_equals_full_frame(torch_cuda):
+@pytest.mark.parametrize("mode", MODES)
+def test_tiled_equals_full_frame(torch_cuda, mode):
     host, scene, cam, cfg = setup("cornell-box", 100, 70, 3, 8, 3)
     full, _ = wpt.Renderer(scene, cam, cfg).render_frame()
-    tcfg = wpt.render_config(100, 70, spp=3, max_depth=8, rr_start=3, tile=(32, 24))
+    tcfg = wpt.render_config(100, 70, spp=3, max_depth=8, rr_start=3, tile=(32, 24), mode=mode)
     tiled, n = wpt.Renderer(scene, cam, tcfg).render_frame()
     assert n == 3
     assert np.array_equal(full, tiled)
@@ -94,10 +98,11 @@ def test_render_task_sums(torch_cuda, oracle_mod):
             r.render_task(bad)
 
 
-def test_render_task_on_untiled_renderer(torch_cuda, oracle_mod):
+@pytest.mark.parametrize("mode", MODES)
+def test_render_task_on_untiled_renderer(torch_cuda, oracle_mod, mode):
     # A task's tile anywhere inside an untiled renderer's frame, before and
     # after a full-frame render on the same renderer.
-    host, scene, cam, cfg = setup("cornell-box", 64, 48, 6, 8, 3)
+    host, scene, cam, cfg = setup("cornell-box", 64, 48, 6, 8, 3, mode=mode)
     r = wpt.Renderer(scene, cam, cfg)
     oracle = oracle_mod.Oracle(scene)
     for x, y, w, h, start, count in ((5, 7, 20, 13, 1, 3), (0, 0, 64, 48, 0, 6), (40, 30, 24, 18, 2, 4)):
@@ -109,8 +114,9 @@ def test_render_task_on_untiled_renderer(torch_cuda, oracle_mod):
             assert n == 6 and np.array_equal(full, want)
 
 
-def test_progressive_samples_accumulate(torch_cuda):
-    host, scene, cam, cfg = setup("cornell-box", 64, 64, 4, 8, 3)
+@pytest.mark.parametrize("mode", MODES)
+def test_progressive_samples_accumulate(torch_cuda, mode):
+    host, scene, cam, cfg = setup("cornell-box", 64, 64, 4, 8, 3, mode=mode)
     r = wpt.Renderer(scene, cam, cfg)
     full, _ = r.render_frame()
     r.reset()

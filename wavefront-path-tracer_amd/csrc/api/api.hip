@@ -108,6 +108,10 @@ struct wpt_renderer {
     uint32_t currentTile = 0xffffffffu;
     double lastMs = 0.0;
     uint64_t lastSamples = 0;
+    // Regenerate mode: the batch consumer over the renderer's tiles.
+    BatchContext regen{};
+    uint32_t regenBlocks = 0;
+    std::vector<BatchTask> regenTasks;
     ~wpt_renderer() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -149,7 +153,7 @@ int validateConfig(const wpt_render_config& c) {
     if (c.tiled && (c.tile_width == 0 || c.tile_height == 0))
         return fail(WPT_ERROR_INVALID_ARGUMENT, "Tile dimensions must be greater than zero");
     if (c.rng_strategy > WPT_RNG_PCG32) return fail(WPT_ERROR_INVALID_ARGUMENT, "Unknown RNG strategy");
-    if (c.mode > WPT_MODE_PERSISTENT) return fail(WPT_ERROR_INVALID_ARGUMENT, "Unknown scheduler mode");
+    if (c.mode > WPT_MODE_REGENERATE) return fail(WPT_ERROR_INVALID_ARGUMENT, "Unknown scheduler mode");
     if (c.max_depth > 0x7fffffffu) return fail(WPT_ERROR_INVALID_ARGUMENT, "Maximum depth out of range");
     uint64_t pixels = static_cast<uint64_t>(c.width) * c.height;
     uint64_t cap = c.tiled ? static_cast<uint64_t>(c.tile_width) * c.tile_height : pixels;
@@ -192,6 +196,36 @@ int launchTileSample(wpt_renderer* r, uint32_t tileIndex, const RenderTile& tile
     } else {
         WPT_HIP(enqueuePersistentSample(r->ctx, r->stream));
     }
+    return WPT_OK;
+}
+
+// Regenerate mode: one batch task per tile over samples [first, first +
+// count), accumulating into the framebuffer (or overwriting, for a task).
+BatchTask regenTask(const wpt_renderer* r, const RenderTile& tile, uint32_t first, uint32_t count, bool accumulate) {
+    const LaunchContext& c = r->ctx;
+    BatchTask t{};
+    t.camera = c.camera;
+    t.shade = c.shade;
+    t.tile = tile;
+    t.scene = 0;
+    t.width = c.width;
+    t.height = c.height;
+    t.sampleStart = first;
+    t.sampleCount = count;
+    t.pixelCount = count ? tile.width * tile.height : 0u;
+    t.outStride = c.width;
+    t.out = 3ull * (static_cast<unsigned long long>(tile.y) * c.width + tile.x);
+    t.accumulate = accumulate ? 1u : 0u;
+    return t;
+}
+
+int regenRun(wpt_renderer* r, const std::vector<BatchTask>& tasks) {
+    BatchContext ctx = r->regen;
+    ctx.taskCount = static_cast<uint32_t>(tasks.size());
+    r->regenTasks = tasks;  // the copy below may read the host vector after returning
+    WPT_HIP(hipMemcpyAsync(const_cast<BatchTask*>(ctx.tasks), r->regenTasks.data(), sizeof(BatchTask) * tasks.size(),
+        hipMemcpyHostToDevice, r->stream));
+    WPT_HIP(enqueueBatch(ctx, r->regenBlocks, r->config.block_size, r->stream));
     return WPT_OK;
 }
 
@@ -482,23 +516,59 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     ctx.blockSize = c.block_size;
     ctx.gridBlocks = (cap + c.block_size - 1) / c.block_size;
     ctx.profileQueues = 1;
+    ctx.capacity = cap;
+    // Trace, shade and shadow kernels run one item per thread over
+    // gridBlocks workgroups; the persistent kernel runs a resident grid.
+    Occupancy occ{};
+    R_HIP(kernelOccupancy(c.block_size, &occ));
+    int device = 0, cus = 0, coop = 0;
+    R_HIP(hipGetDevice(&device));
+    R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (occ.trace <= 0 || occ.shade <= 0 || occ.shadow <= 0)
+        return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the wavefront kernels"));
+    uint32_t cu = static_cast<uint32_t>(cus);
+    ctx.persistentBlocks = 1;
+    if (c.mode == WPT_MODE_PERSISTENT) {
+        if (occ.persistent <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the persistent kernel"));
+        R_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
+        if (!coop) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "device lacks cooperative launch"));
+        uint32_t resident = static_cast<uint32_t>(occ.persistent) * cu;
+        uint32_t blocks = c.persistent_blocks ? std::min(c.persistent_blocks, resident) : resident;
+        ctx.persistentBlocks = std::max(1u, std::min(blocks, ctx.gridBlocks));
+    }
+    // Segment capacity: workgroup b appends to segment b % kSegments, so a
+    // segment receives at most (its workgroups) x (items per workgroup).
+    uint32_t B = c.block_size;
+    uint64_t stagedSegment = static_cast<uint64_t>((ctx.gridBlocks + kSegments - 1) / kSegments) * B;
+    uint32_t P = ctx.persistentBlocks;
+    uint64_t rounds = (static_cast<uint64_t>(cap) + static_cast<uint64_t>(P) * B - 1) / (static_cast<uint64_t>(P) * B);
+    uint64_t persistentSegment = static_cast<uint64_t>((P + kSegments - 1) / kSegments) * rounds * B;
+    uint64_t segment = std::max(stagedSegment, persistentSegment);
+    uint64_t qcap = segment * kSegments;
+    if (qcap >= (1ull << 31)) return cleanup(fail(WPT_ERROR_INVALID_ARGUMENT, "queue capacity out of range"));
     void* p = nullptr;
-    for (int q = 0; q < 2; ++q) {
-        R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kQueueFieldCount) * cap, &p));
+    bool queues = c.mode != WPT_MODE_REGENERATE;
+    for (int q = 0; queues && q < 2; ++q) {
+        R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kQueueFieldCount) * qcap, &p));
         ctx.queue[q].words = static_cast<uint32_t*>(p);
-        ctx.queue[q].capacity = cap;
+        ctx.queue[q].capacity = static_cast<uint32_t>(qcap);
+        ctx.queue[q].segmentCapacity = static_cast<uint32_t>(segment);
     }
     if (c.mode == WPT_MODE_GRAPH) {
-        R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(cap) + sizeof(uint32_t) * 4 * static_cast<size_t>(cap), &p));
+        R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(qcap) + sizeof(uint32_t) * 4 * static_cast<size_t>(qcap), &p));
         float* f = static_cast<float*>(p);
         ctx.hits.t = f;
-        ctx.hits.u = f + cap;
-        ctx.hits.v = f + 2 * static_cast<size_t>(cap);
-        uint32_t* u = reinterpret_cast<uint32_t*>(f + 3 * static_cast<size_t>(cap));
+        ctx.hits.u = f + qcap;
+        ctx.hits.v = f + 2 * static_cast<size_t>(qcap);
+        uint32_t* u = reinterpret_cast<uint32_t*>(f + 3 * static_cast<size_t>(qcap));
         ctx.hits.kind = u;
-        ctx.hits.instance = u + cap;
-        ctx.hits.prim = u + 2 * static_cast<size_t>(cap);
-        ctx.hits.material = u + 3 * static_cast<size_t>(cap);
+        ctx.hits.instance = u + qcap;
+        ctx.hits.prim = u + 2 * static_cast<size_t>(qcap);
+        ctx.hits.material = u + 3 * static_cast<size_t>(qcap);
+        R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kShadowFieldCount) * qcap, &p));
+        ctx.shadow.words = static_cast<uint32_t*>(p);
+        ctx.shadow.capacity = static_cast<uint32_t>(qcap);
+        ctx.shadow.segmentCapacity = static_cast<uint32_t>(segment);
     }
     R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height, &p));
     ctx.framebuffer = static_cast<float*>(p);
@@ -508,34 +578,32 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     R_HIP(r->arena.alloc(sizeof(RenderTile), &p));
     ctx.tile = static_cast<const RenderTile*>(p);
     R_HIP(hipMemset(ctx.framebuffer, 0, sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height));
-    // Resident grids: the trace, shade and shadow kernels claim work per
-    // wave, so one resident wave of workgroups covers any queue length.
-    Occupancy occ{};
-    R_HIP(kernelOccupancy(c.block_size, &occ));
-    int device = 0, cus = 0, coop = 0;
-    R_HIP(hipGetDevice(&device));
-    R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    if (occ.trace <= 0 || occ.shade <= 0 || occ.shadow <= 0)
-        return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the wavefront kernels"));
-    uint32_t cu = static_cast<uint32_t>(cus);
-    ctx.traceBlocks = std::max(1u, std::min(ctx.gridBlocks, static_cast<uint32_t>(std::max(occ.trace, occ.shadow)) * cu));
-    ctx.shadeBlocks = std::max(1u, std::min(ctx.gridBlocks, static_cast<uint32_t>(occ.shade) * cu));
-    if (c.mode == WPT_MODE_PERSISTENT) {
-        if (occ.persistent <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the persistent kernel"));
-        R_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
-        if (!coop) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "device lacks cooperative launch"));
-        uint32_t resident = static_cast<uint32_t>(occ.persistent) * cu;
-        uint32_t blocks = c.persistent_blocks ? std::min(c.persistent_blocks, resident) : resident;
-        ctx.persistentBlocks = std::max(1u, std::min(blocks, ctx.gridBlocks));
+    if (c.mode == WPT_MODE_REGENERATE) {
+        int perCu = 0;
+        R_HIP(batchOccupancy(c.block_size, &perCu));
+        if (perCu <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the regenerating kernel"));
+        uint32_t resident = static_cast<uint32_t>(perCu) * cu;
+        r->regenBlocks = c.persistent_blocks ? std::min(c.persistent_blocks, resident) : resident;
+        BatchContext& b = r->regen;
+        R_HIP(r->arena.alloc(sizeof(DevScene), &p));
+        b.scenes = static_cast<const DevScene*>(p);
+        R_HIP(hipMemcpy(p, &scene->dev, sizeof(DevScene), hipMemcpyHostToDevice));
+        size_t tasks = r->tiles.size();
+        R_HIP(r->arena.alloc(sizeof(BatchTask) * tasks, &p));
+        b.tasks = static_cast<const BatchTask*>(p);
+        R_HIP(r->arena.alloc(sizeof(BatchTaskState) * tasks, &p));
+        b.state = static_cast<BatchTaskState*>(p);
+        R_HIP(r->arena.alloc(sizeof(BatchCounters), &p));
+        b.counters = static_cast<BatchCounters*>(p);
+        R_HIP(hipMemset(p, 0, sizeof(BatchCounters)));
+        b.out = ctx.framebuffer;
     }
-    ctx.spillThreads = std::max(ctx.traceBlocks, ctx.persistentBlocks) * c.block_size;
+    // One spill column per thread index of any grid.
+    ctx.spillThreads = std::max(std::max(ctx.gridBlocks, ctx.persistentBlocks), r->regenBlocks) * c.block_size;
     R_HIP(r->arena.alloc(sizeof(uint2) * (kMaxStackDepth - kLdsStackEntries) * static_cast<size_t>(ctx.spillThreads), &p));
     ctx.spill = static_cast<uint2*>(p);
-    if (c.mode == WPT_MODE_GRAPH) {
-        R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kShadowFieldCount) * cap, &p));
-        ctx.shadow.words = static_cast<uint32_t*>(p);
-        ctx.shadow.capacity = cap;
-    }
+    r->regen.spill = ctx.spill;
+    r->regen.spillThreads = ctx.spillThreads;
     R_HIP(hipDeviceSynchronize());
 #undef R_HIP
     *out = r;
@@ -561,6 +629,22 @@ WPT_API int wpt_renderer_reset(wpt_renderer* r) {
 
 WPT_API int wpt_renderer_render_samples(wpt_renderer* r, uint32_t first, uint32_t count) {
     if (!r) return fail(WPT_ERROR_INVALID_ARGUMENT, "null renderer");
+    if (r->config.mode == WPT_MODE_REGENERATE) {
+        std::vector<BatchTask> tasks;
+        for (const RenderTile& t : r->tiles) tasks.push_back(regenTask(r, t, first, count, true));
+        WPT_HIP(hipEventRecord(r->start, r->stream));
+        int s = regenRun(r, tasks);
+        if (s != WPT_OK) return s;
+        WPT_HIP(hipEventRecord(r->end, r->stream));
+        hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, first + count);
+        WPT_HIP(hipGetLastError());
+        WPT_HIP(hipEventSynchronize(r->end));
+        float ms = 0.0f;
+        WPT_HIP(hipEventElapsedTime(&ms, r->start, r->end));
+        r->lastMs = ms;
+        r->lastSamples = static_cast<uint64_t>(count) * r->config.width * r->config.height;
+        return WPT_OK;
+    }
     int status = ensureGraph(r);
     if (status != WPT_OK) return status;
     hipLaunchKernelGGL(kClearStats, dim3(1), dim3(1), 0, r->stream, r->ctx.counters);
@@ -618,24 +702,30 @@ WPT_API int wpt_render_task(wpt_renderer* r, const wpt_task* task, float* hostTi
     if (task->sample_count == 0 || task->sample_start > c.samples_per_pixel ||
         task->sample_count > c.samples_per_pixel - task->sample_start)
         return fail(WPT_ERROR_INVALID_TASK, "Render task sample range is outside the prepared renderer bounds");
-    int status = ensureGraph(r);
-    if (status != WPT_OK) return status;
-    size_t n = static_cast<size_t>(task->width) * task->height * 3;
-    hipLaunchKernelGGL(kClearRect, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, r->stream,
-        r->ctx.framebuffer, c.width, task->x, task->y, task->width, task->height);
-    hipLaunchKernelGGL(kClearStats, dim3(1), dim3(1), 0, r->stream, r->ctx.counters);
-    hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, task->sample_start);
-    WPT_HIP(hipGetLastError());
     RenderTile tile{task->x, task->y, task->width, task->height, 1u};
-    r->currentTile = 0xffffffffu;  // no tile uploaded: the first sample uploads the task's tile
     WPT_HIP(hipEventRecord(r->start, r->stream));
-    for (uint32_t s = 0; s < task->sample_count; ++s) {
-        status = launchTileSample(r, 0xfffffffeu, tile);
+    if (c.mode == WPT_MODE_REGENERATE) {
+        // Sums written over the tile: the same values as clearing it first.
+        int s = regenRun(r, {regenTask(r, tile, task->sample_start, task->sample_count, false)});
+        if (s != WPT_OK) return s;
+    } else {
+        int status = ensureGraph(r);
         if (status != WPT_OK) return status;
+        size_t n = static_cast<size_t>(task->width) * task->height * 3;
+        hipLaunchKernelGGL(kClearRect, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, r->stream,
+            r->ctx.framebuffer, c.width, task->x, task->y, task->width, task->height);
+        hipLaunchKernelGGL(kClearStats, dim3(1), dim3(1), 0, r->stream, r->ctx.counters);
+        hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, task->sample_start);
+        WPT_HIP(hipGetLastError());
+        r->currentTile = 0xffffffffu;  // no tile uploaded: the first sample uploads the task's tile
+        for (uint32_t s = 0; s < task->sample_count; ++s) {
+            status = launchTileSample(r, 0xfffffffeu, tile);
+            if (status != WPT_OK) return status;
+        }
+        r->currentTile = 0xffffffffu;
     }
     WPT_HIP(hipEventRecord(r->end, r->stream));
     WPT_HIP(hipStreamSynchronize(r->stream));
-    r->currentTile = 0xffffffffu;
     float ms = 0.0f;
     WPT_HIP(hipEventElapsedTime(&ms, r->start, r->end));
     r->lastMs = ms;
@@ -650,6 +740,14 @@ WPT_API int wpt_render_task(wpt_renderer* r, const wpt_task* task, float* hostTi
 WPT_API int wpt_renderer_profile_kernels(wpt_renderer* r, uint32_t first, uint32_t count, double* kernelMs,
     uint32_t* launches) {
     if (!r || !kernelMs || !launches) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (r->config.mode == WPT_MODE_REGENERATE) {
+        // One launch over all tiles and samples, timed as the persistent slot.
+        int s = wpt_renderer_render_samples(r, first, count);
+        if (s != WPT_OK) return s;
+        kernelMs[4] += r->lastMs;
+        launches[4] += 1;
+        return WPT_OK;
+    }
     hipLaunchKernelGGL(kClearStats, dim3(1), dim3(1), 0, r->stream, r->ctx.counters);
     hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, first);
     WPT_HIP(hipGetLastError());
@@ -673,6 +771,14 @@ WPT_API int wpt_renderer_last_stats(const wpt_renderer* r, wpt_stats* stats) {
     WPT_HIP(hipStreamSynchronize(r->stream));
     WPT_HIP(hipMemcpy(&c, r->ctx.counters, sizeof c, hipMemcpyDeviceToHost));
     std::memset(stats, 0, sizeof *stats);
+    if (r->config.mode == WPT_MODE_REGENERATE) {
+        // No queues: per-bounce counts are not recorded.
+        BatchCounters b;
+        WPT_HIP(hipMemcpy(&b, r->regen.counters, sizeof b, hipMemcpyDeviceToHost));
+        c.closestQueries = b.closestQueries;
+        c.shadowQueries = b.shadowQueries;
+        std::memset(c.queueCounts, 0, sizeof c.queueCounts);
+    }
     stats->closest_queries = c.closestQueries;
     stats->shadow_queries = c.shadowQueries;
     stats->samples = r->lastSamples;
@@ -784,6 +890,8 @@ int stageBatch(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, std::vecto
         d.sampleCount = t.sample_count;
         uint32_t pixels = t.tile_width * t.tile_height;
         d.pixelCount = t.sample_count ? pixels : 0u;
+        d.outStride = t.tile_width;
+        d.accumulate = 0;
         d.out = total;
         offsets[i] = total;
         total += 3ull * pixels;

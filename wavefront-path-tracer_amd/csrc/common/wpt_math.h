@@ -242,4 +242,79 @@ WPT_HD float tanF(float x) {
     return static_cast<float>(s / c);
 }
 
+// sin / cos in double for |x| < 2^20 (environment sampling math).
+WPT_HD double sinD(double x) {
+    double s, c;
+    detail::sinCosD(x, s, c);
+    return s;
+}
+WPT_HD double cosD(double x) {
+    double s, c;
+    detail::sinCosD(x, s, c);
+    return c;
+}
+
+namespace detail {
+constexpr double kPi = 3.141592653589793;
+constexpr double kPio2 = 1.5707963267948966;
+constexpr double kPio6 = 0.5235987755982989;
+constexpr double kSqrt3 = 1.7320508075688772;
+constexpr double kTanPi12 = 0.2679491924311228;  // tan(pi/12)
+
+// atan(t) for 0 <= t <= tan(pi/12): odd Taylor series through t^31.
+WPT_HD double atanReduced(double t) {
+    double t2 = t * t;
+    double p = 1.0 / 31.0;
+    p = 1.0 / 29.0 - t2 * p;
+    p = 1.0 / 27.0 - t2 * p;
+    p = 1.0 / 25.0 - t2 * p;
+    p = 1.0 / 23.0 - t2 * p;
+    p = 1.0 / 21.0 - t2 * p;
+    p = 1.0 / 19.0 - t2 * p;
+    p = 1.0 / 17.0 - t2 * p;
+    p = 1.0 / 15.0 - t2 * p;
+    p = 1.0 / 13.0 - t2 * p;
+    p = 1.0 / 11.0 - t2 * p;
+    p = 1.0 / 9.0 - t2 * p;
+    p = 1.0 / 7.0 - t2 * p;
+    p = 1.0 / 5.0 - t2 * p;
+    p = 1.0 / 3.0 - t2 * p;
+    return t - t * (t2 * p);
+}
+
+// atan(a) for a >= 0: a > 1 reflects through pi/2 - atan(1/a), and
+// a > tan(pi/12) shifts by pi/6 (atan a = pi/6 + atan((a sqrt3 - 1)/(a + sqrt3))).
+WPT_HD double atanPositive(double a) {
+    bool invert = a > 1.0;
+    if (invert) a = 1.0 / a;
+    double r;
+    if (a > kTanPi12)
+        r = kPio6 + atanReduced((a * kSqrt3 - 1.0) / (a + kSqrt3));
+    else
+        r = atanReduced(a);
+    return invert ? kPio2 - r : r;
+}
+}  // namespace detail
+
+// atan2 in double with IEEE signed-zero and infinity-free quadrant rules.
+WPT_HD double atan2D(double y, double x) {
+    if (y != y || x != x) return y + x;
+    bool yNeg = bitsOf(y) >> 63;
+    bool xNeg = bitsOf(x) >> 63;
+    double ay = yNeg ? -y : y, ax = xNeg ? -x : x;
+    double r;
+    if (ay == 0.0) {
+        r = xNeg ? detail::kPi : 0.0;
+    } else if (ax == 0.0) {
+        r = detail::kPio2;
+    } else {
+        r = detail::atanPositive(ay / ax);
+        if (xNeg) r = detail::kPi - r;
+    }
+    return yNeg ? -r : r;
+}
+WPT_HD float atan2F(float y, float x) {
+    return static_cast<float>(atan2D(static_cast<double>(y), static_cast<double>(x)));
+}
+
 }  // namespace wpt

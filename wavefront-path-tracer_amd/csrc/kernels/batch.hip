@@ -10,10 +10,13 @@
 // iteration (closest hit, shading, shadow ray), and starts the pixel's next
 // sample in the same iteration its path ends, so lanes do not wait for the
 // longest path of the wave.  The pixel's sum is accumulated in sample order
-// from zero, the same additions wpt_render_task makes into its cleared tile,
-// which makes the two outputs bit-identical.  When the task has no unclaimed
-// pixels and the wave's lanes have finished theirs, the wave moves on to the
-// next task with unclaimed pixels; the kernel ends when none is left.
+// from zero (or from the pixel's current sum when the task accumulates into
+// a framebuffer), the same additions the staged renderer makes into its
+// framebuffer one sample at a time, which makes the outputs bit-identical.
+// When the task has no unclaimed pixels and the wave's lanes have finished
+// theirs, the wave moves on to the next task with unclaimed pixels; the
+// kernel ends when none is left.  With no barrier between bounces, waves
+// that finish early keep taking new pixels while a few long traversals run.
 #include <hip/hip_runtime.h>
 
 #include "device_queue.h"
@@ -77,9 +80,13 @@ __global__ void __launch_bounds__(512) kBatch(BatchContext ctx) {
                 if (need && u < T.pixelCount) {
                     busy = true;
                     tilePixel(T.tile, u, x, y);
-                    local = (y - T.tile.y) * T.tile.width + (x - T.tile.x);
+                    local = (y - T.tile.y) * T.outStride + (x - T.tile.x);
                     samplesDone = 0;
                     sum = Vec3(0.0f, 0.0f, 0.0f);
+                    if (T.accumulate) {
+                        const float* o = ctx.out + T.out + 3ull * local;
+                        sum = Vec3(o[0], o[1], o[2]);
+                    }
                     path = primaryPath(T.camera, x, y, T.width, T.height, T.sampleStart, T.shade.rngStrategy);
                 }
             }

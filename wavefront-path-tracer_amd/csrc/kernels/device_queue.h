@@ -11,6 +11,14 @@ namespace wpt {
 
 constexpr uint32_t kSpillEntries = kMaxStackDepth - kLdsStackEntries;
 
+// Occupancy target of the traversal kernels (waves per SIMD), 0: compiler's
+// choice.  The register budget it implies must not force scratch spills.
+#if defined(WPT_TRACE_WAVES_PER_EU) && WPT_TRACE_WAVES_PER_EU > 0
+#define WPT_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WPT_TRACE_WAVES_PER_EU)))
+#else
+#define WPT_TRACE_OCCUPANCY
+#endif
+
 __device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -64,22 +72,95 @@ __device__ __forceinline__ LdsStack makeStack(uint2* spill, uint32_t spillThread
     return s;
 }
 
-// Wave-aggregated append: one atomic per wave, returns this lane's index.
-__device__ __forceinline__ uint32_t waveAppend(bool active, uint32_t* counter) {
-    unsigned long long mask = __ballot(active);
-    uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-    uint32_t base = 0;
-    if (__lane_id() == 0 && mask) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
-    base = __builtin_amdgcn_readfirstlane(base);
-    return base + rank;
+// ---- segmented queues
+// Flattened view of a queue's segments: segment s holds the items
+// [start[s], start[s + 1]) of the enumeration.
+struct SegmentView {
+    uint32_t start[kSegments + 1];
+};
+
+__device__ __forceinline__ SegmentView loadSegments(const uint32_t* counts) {
+    SegmentView v;
+    uint32_t acc = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < kSegments; ++s) {
+        v.start[s] = acc;
+        acc += __builtin_amdgcn_readfirstlane(counts[s]);
+    }
+    v.start[kSegments] = acc;
+    return v;
 }
 
-// Wave-level work claim: 64 consecutive items per call.
-__device__ __forceinline__ uint32_t claimWaveBatch(uint32_t* cursor) {
-    uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(cursor, 64u);
-    return __builtin_amdgcn_readfirstlane(base);
+// Queue slot of enumerated item g (< total).
+// Constant indices only (a select chain), so the view stays in scalar
+// registers.
+__device__ __forceinline__ uint32_t segmentSlot(const SegmentView& v, uint32_t segmentCapacity, uint32_t g) {
+    uint32_t s = 0, first = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < kSegments; ++k) {
+        bool beyond = g >= v.start[k];
+        s = beyond ? k : s;
+        first = beyond ? v.start[k] : first;
+    }
+    return s * segmentCapacity + (g - first);
+}
+
+// Layout of a freshly generated queue: item g belongs to virtual workgroup
+// g / B of size B and goes to that workgroup's segment, packed in order.
+WPT_HD uint32_t primarySlot(uint32_t g, uint32_t B, uint32_t segmentCapacity) {
+    uint32_t b = g / B;
+    return (b % kSegments) * segmentCapacity + (b / kSegments) * B + (g - b * B);
+}
+
+WPT_HD uint32_t primarySegmentCount(uint32_t total, uint32_t B, uint32_t s) {
+    uint32_t nb = (total + B - 1) / B;
+    if (nb <= s) return 0;
+    uint32_t blocks = (nb - 1 - s) / kSegments + 1;
+    uint32_t items = blocks * B;
+    if ((nb - 1) % kSegments == s) items -= nb * B - total;
+    return items;
+}
+
+// Workgroup-aggregated append of up to two kinds of items: one atomic per
+// kind per workgroup on this workgroup's segment counters.  Every thread of
+// the workgroup must call it.  Returns each active lane's index within its
+// segment.
+constexpr uint32_t kMaxWaves = 16;  // 1024-lane workgroups
+struct AppendShared {
+    uint32_t wave[kMaxWaves][2];
+    uint32_t base[2];
+};
+
+__device__ void blockAppend(bool a0, bool a1, uint32_t* counter0, uint32_t* counter1, AppendShared& sh,
+    uint32_t& index0, uint32_t& index1) {
+    unsigned long long m0 = __ballot(a0), m1 = __ballot(a1);
+    uint32_t r0 = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m0 >> 32),
+        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m0), 0u));
+    uint32_t r1 = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m1 >> 32),
+        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m1), 0u));
+    uint32_t w = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        sh.wave[w][0] = static_cast<uint32_t>(__popcll(m0));
+        sh.wave[w][1] = static_cast<uint32_t>(__popcll(m1));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t waves = blockDim.x >> 6;
+        uint32_t t0 = 0, t1 = 0;
+        for (uint32_t k = 0; k < waves; ++k) {
+            uint32_t c0 = sh.wave[k][0], c1 = sh.wave[k][1];
+            sh.wave[k][0] = t0;
+            sh.wave[k][1] = t1;
+            t0 += c0;
+            t1 += c1;
+        }
+        sh.base[0] = t0 ? atomicAdd(counter0, t0) : 0u;
+        sh.base[1] = t1 ? atomicAdd(counter1, t1) : 0u;
+    }
+    __syncthreads();
+    index0 = sh.base[0] + sh.wave[w][0] + r0;
+    index1 = sh.base[1] + sh.wave[w][1] + r1;
+    __syncthreads();
 }
 
 // Tile-local index -> pixel.  Rows are walked in strips of 8 and each strip

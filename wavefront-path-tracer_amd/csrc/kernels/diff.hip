@@ -1,11 +1,12 @@
 // Differentiable rendering kernels: forward paths, per-pixel resolve, the
 // path-replay adjoint and an Adam step.  No reference counterpart.
 //
-// One thread renders one (pixel, sample) path; waves claim 64 paths at a
-// time, consecutive paths covering 8x8 pixel blocks of one sample.  The
-// forward pass keeps every path's radiance (3 floats), which the adjoint
-// needs as L_path; the resolve kernel averages a pixel's samples in sample
-// order, so the image does not depend on scheduling.  The adjoint re-traces
+// One thread renders one (pixel, sample) path at a time, walking the paths
+// with a static grid stride (no shared cursor: device-scope atomics on one
+// address serialize across XCDs); consecutive paths cover 8x8 pixel blocks
+// of one sample.  The forward pass keeps every path's radiance (3 floats),
+// which the adjoint needs as L_path; the resolve kernel averages a pixel's
+// samples in sample order, so the image does not depend on scheduling.  The adjoint re-traces
 // each path with the same random numbers; texture gradients go to global
 // memory with hardware float atomics, roughness and emission gradients are
 // summed per workgroup in LDS first (a few dozen slots shared by every path).
@@ -33,11 +34,8 @@ __global__ void __launch_bounds__(512) kDiffForward(DiffContext c) {
     const uint32_t count = pixels * c.spp;
     NoGradients none;
     const Vec3 zero(0.0f, 0.0f, 0.0f);
-    while (true) {
-        uint32_t base = claimWaveBatch(c.cursor);
-        if (base >= count) break;
-        uint32_t i = base + __lane_id();
-        if (i >= count) continue;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         uint32_t sample, x, y;
         pathPixel(c, i, sample, x, y);
         Vec3 L = diffPath<false>(c.scene, c.layout, c.params, c.camera, x, y, c.width, c.height,
@@ -107,11 +105,8 @@ __global__ void __launch_bounds__(512) kDiffAdjoint(DiffContext c) {
     const uint32_t pixels = c.width * c.height;
     const uint32_t count = pixels * c.spp;
     const float invSpp = 1.0f / static_cast<float>(c.spp);
-    while (true) {
-        uint32_t base = claimWaveBatch(c.cursor);
-        if (base >= count) break;
-        uint32_t i = base + __lane_id();
-        if (i >= count) continue;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         uint32_t sample, x, y;
         pathPixel(c, i, sample, x, y);
         uint32_t p = y * c.width + x;

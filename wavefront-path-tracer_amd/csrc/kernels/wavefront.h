@@ -33,14 +33,22 @@ enum ShadowField : uint32_t {
 };
 constexpr uint32_t kShadowTerminated = 0xffffffffu;
 
+// A queue is split into kSegments segments of equal capacity.  Workgroup b
+// appends to segment b % kSegments, which is the XCD it runs on under the
+// round-robin dispatch of MI355X, so concurrent appends from different XCDs
+// hit different counters.  Consumers enumerate the segments' items in order.
+constexpr uint32_t kSegments = 8;
+
 struct PathQueue {
-    uint32_t* words;  // kQueueFieldCount * capacity
-    uint32_t capacity;
+    uint32_t* words;           // kQueueFieldCount * capacity
+    uint32_t capacity;         // kSegments * segmentCapacity
+    uint32_t segmentCapacity;
 };
 
 struct ShadowQueue {
-    uint32_t* words;  // kShadowFieldCount * capacity
+    uint32_t* words;           // kShadowFieldCount * capacity
     uint32_t capacity;
+    uint32_t segmentCapacity;
 };
 
 // Closest-hit records of a traced queue (SoA, staged mode only).
@@ -60,14 +68,11 @@ struct RenderTile {
 
 // Device counters shared by the launches of one renderer.
 struct WavefrontCounters {
-    uint32_t count[3];       // live paths: staged mode uses [0]/[1] per queue;
-                             // the persistent kernel rotates all three by bounce
-    uint32_t cursor[3];      // persistent kernel: work cursor per bounce (mod 3)
-    uint32_t traceCursor;    // next unclaimed ray of the running trace pass
-    uint32_t shadowCount;    // shadow requests of the current bounce
-    uint32_t shadowCursor;   // next unclaimed shadow request
+    uint32_t count[2][kSegments];    // staged mode: items per segment of queue 0 / 1
+    uint32_t shadowCount[kSegments];  // shadow requests per segment of the current bounce
+    uint32_t pcount[3][kSegments];   // persistent kernel: queues rotate through three count sets
     uint32_t sampleIndex;
-    uint32_t shadeCursor;    // next unclaimed path of the running shade pass
+    uint32_t pad;
     unsigned long long closestQueries;
     unsigned long long shadowQueries;
     uint32_t queueCounts[64];
@@ -75,7 +80,10 @@ struct WavefrontCounters {
 
 // Traversal stack entries kept in LDS per lane; deeper entries spill to a
 // per-thread column of `spill`.
-constexpr uint32_t kLdsStackEntries = 16;
+#ifndef WPT_LDS_STACK_ENTRIES
+#define WPT_LDS_STACK_ENTRIES 16
+#endif
+constexpr uint32_t kLdsStackEntries = WPT_LDS_STACK_ENTRIES;
 constexpr uint32_t kSpillEntriesTotal = kMaxStackDepth - kLdsStackEntries;  // per thread
 
 struct LaunchContext {
@@ -89,12 +97,11 @@ struct LaunchContext {
     uint32_t width, height;
     WavefrontCounters* counters;
     const RenderTile* tile;  // device copy of the current tile
-    uint2* spill;            // (kMaxStackDepth - kLdsStackEntries) x spillThreads
+    uint2* spill;            // (kMaxStackDepth - kLdsStackEntries) x spillThreads, one column per thread index
     uint32_t spillThreads;
+    uint32_t capacity;         // tile pixels: the most items any queue holds
     uint32_t blockSize;
-    uint32_t gridBlocks;       // ceil(capacity / blockSize)
-    uint32_t traceBlocks;      // resident grid of the trace / shadow kernels
-    uint32_t shadeBlocks;      // resident grid of the shade kernel
+    uint32_t gridBlocks;       // ceil(capacity / blockSize): one item per thread
     uint32_t persistentBlocks;
     uint32_t profileQueues;
 };
@@ -110,7 +117,7 @@ hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream);
 hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream);
 // Resident workgroups per CU of each kernel at `blockSize`.
 struct Occupancy {
-    int trace, shade, shadow, persistent;
+    int trace, shade, shadow, persistent;  // resident workgroups per CU
 };
 hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out);
 // One sample of the current tile with every launch bracketed by events:
@@ -132,7 +139,10 @@ struct BatchTask {
     uint32_t width, height;  // frame: pixel ids and RNG keys
     uint32_t sampleStart, sampleCount;
     uint32_t pixelCount;     // tile pixels, 0 when the task renders nothing
-    unsigned long long out;  // float offset of the task's tile sums (w*h*3)
+    uint32_t outStride;      // pixels per output row
+    unsigned long long out;  // float offset of the tile's first pixel in the output
+    uint32_t accumulate;     // 1: add to the output's current sums, 0: overwrite
+    uint32_t pad;
 };
 
 // Per-task progress of a running batch.

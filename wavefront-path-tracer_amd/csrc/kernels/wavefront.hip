@@ -5,7 +5,7 @@
 //   intersectScene (device_traversal.cuh:1186-1194) ........... kTrace
 //   shadePaths (device_path_shading.cuh:6-173) ................ kShade
 //   isOccluded calls made inside shadePaths ................... kShadow
-//   appendWarpCompacted (queue_compaction.cuh:8-29) ........... waveAppend in
+//   appendWarpCompacted (queue_compaction.cuh:8-29) ........... blockAppend in
 //                                                               kShade / kPersistent,
 //                                                               kCompactAbi (C ABI)
 //   advanceSampleIndex (renderer.cu:44-48) .................... kAdvance
@@ -13,15 +13,20 @@
 //
 // Traversal stacks live in LDS (kLdsStackEntries per lane, lane-interleaved
 // so a wave's accesses at equal depth hit distinct banks); deeper entries
-// spill to a per-thread column in global memory.  Trace, shade and shadow
-// kernels run a resident-sized grid in which each wave claims 64 items at a
-// time from an atomic cursor, so divergent waves do not idle behind a static
-// split.
+// spill to a per-thread column in global memory.
 //
-// Continuations are appended to the next queue with one atomic per wave (a
-// 64-bit ballot ranks the survivors, mbcnt gives each lane its offset).
-// Queue order only affects coherence, never results: a pixel receives one
-// path per sample and each path's arithmetic depends on its own state alone.
+// Work distribution uses no shared cursor: trace, shade and shadow kernels
+// take one item per thread over a grid sized to the queue's capacity, and
+// the hardware dispatcher hands out workgroups as earlier ones finish.
+// (Device-scope atomics on one address serialize across the 8 XCDs at tens
+// of nanoseconds each; a claim per wave cost more than the traversal.)
+// Continuations are appended with one atomic per workgroup on the counter of
+// the workgroup's queue segment (blockIdx % 8, its XCD): a 64-bit ballot
+// ranks a wave's survivors, mbcnt gives each lane its offset and the
+// workgroup adds per-wave offsets from LDS.  Queue order only affects
+// coherence, never results: a pixel receives one path per sample and each
+// path's arithmetic depends on its own state alone.
+//
 // The C ABI's compaction (kCompactAbi) is order preserving instead, so its
 // output is a stable filter of the input: workgroups take chunk tickets in
 // order and chain their totals with a decoupled look-back (one flag word per
@@ -45,7 +50,6 @@ namespace {
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
-constexpr uint32_t kMaxWaves = 16;  // 1024-lane workgroups
 constexpr uint32_t kAbiBlock = 128;
 constexpr uint32_t kAbiBlocks = 2048;
 
@@ -159,52 +163,49 @@ __device__ void storePath(const PathQueue& q, uint32_t i, const PathState& p) {
     sf(q, kEtaScale, i, p.etaScale);
 }
 
-__global__ void kPrimary(LaunchContext ctx) {
+__global__ void __launch_bounds__(512) kPrimary(LaunchContext ctx) {
     RenderTile tile = *ctx.tile;
     uint32_t pixels = tile.width * tile.height;
-    uint32_t count = pixels < ctx.queue[0].capacity ? pixels : ctx.queue[0].capacity;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) ctx.counters->count[0] = count;
-    if (i >= count) return;
+    uint32_t count = pixels < ctx.capacity ? pixels : ctx.capacity;
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < kSegments) ctx.counters->count[0][g] = primarySegmentCount(count, ctx.blockSize, g);
+    if (g >= count) return;
     uint32_t x, y;
-    tilePixel(tile, i, x, y);
+    tilePixel(tile, g, x, y);
     PathState p = primaryPath(ctx.camera, x, y, ctx.width, ctx.height, ctx.counters->sampleIndex, ctx.shade.rngStrategy);
-    storePath(ctx.queue[0], i, p);
+    storePath(ctx.queue[0], primarySlot(g, ctx.blockSize, ctx.queue[0].segmentCapacity), p);
 }
 
 __global__ void kReset(LaunchContext ctx, uint32_t nextSel) {
-    if (threadIdx.x == 0) {
-        ctx.counters->count[nextSel] = 0;
-        ctx.counters->traceCursor = 0;
-        ctx.counters->shadeCursor = 0;
-        ctx.counters->shadowCount = 0;
-        ctx.counters->shadowCursor = 0;
+    uint32_t s = threadIdx.x;
+    if (s < kSegments) {
+        ctx.counters->count[nextSel][s] = 0;
+        ctx.counters->shadowCount[s] = 0;
     }
 }
 
-__global__ void kTrace(LaunchContext ctx, uint32_t bounce, uint32_t sel) {
-    uint32_t count = ctx.counters->count[sel];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+// One ray per thread; the hardware dispatcher balances workgroups.
+__global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kTrace(LaunchContext ctx, uint32_t bounce, uint32_t sel) {
+    SegmentView v = loadSegments(ctx.counters->count[sel]);
+    uint32_t count = v.start[kSegments];
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) {
         atomicAdd(&ctx.counters->closestQueries, static_cast<unsigned long long>(count));
         if (ctx.profileQueues && bounce < 64) ctx.counters->queueCounts[bounce] = count;
     }
+    if (g >= count) return;
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const PathQueue& q = ctx.queue[sel];
-    while (true) {
-        uint32_t base = claimWaveBatch(&ctx.counters->traceCursor);
-        if (base >= count) break;
-        uint32_t i = base + __lane_id();
-        if (i >= count) continue;
-        Ray ray = loadRay(q, i);
-        HitRecord h = traceClosest(ctx.scene, ray, stack);
-        ctx.hits.t[i] = h.t;
-        ctx.hits.u[i] = h.u;
-        ctx.hits.v[i] = h.v;
-        ctx.hits.kind[i] = h.kind;
-        ctx.hits.instance[i] = h.instance;
-        ctx.hits.prim[i] = h.prim;
-        ctx.hits.material[i] = h.material;
-    }
+    uint32_t i = segmentSlot(v, q.segmentCapacity, g);
+    Ray ray = loadRay(q, i);
+    HitRecord h = traceClosest(ctx.scene, ray, stack);
+    ctx.hits.t[i] = h.t;
+    ctx.hits.u[i] = h.u;
+    ctx.hits.v[i] = h.v;
+    ctx.hits.kind[i] = h.kind;
+    ctx.hits.instance[i] = h.instance;
+    ctx.hits.prim[i] = h.prim;
+    ctx.hits.material[i] = h.material;
 }
 
 __device__ __forceinline__ HitRecord loadHit(const HitBuffer& hits, uint32_t i) {
@@ -238,75 +239,77 @@ __device__ void storeShadow(const ShadowQueue& q, uint32_t s, const ShadowReques
     sfield(q, kSRz)[s] = __float_as_uint(path.radiance.z);
 }
 
-// Staged shading: continuing paths are appended to the next queue, shadow
-// requests to the shadow queue; finished paths without a pending shadow ray
-// reach the framebuffer.
-__global__ void kShade(LaunchContext ctx, uint32_t sel) {
-    uint32_t count = ctx.counters->count[sel];
+// Staged shading, one path per thread: continuing paths are appended to the
+// workgroup's segment of the next queue, shadow requests to its segment of
+// the shadow queue; finished paths without a pending shadow ray reach the
+// framebuffer.
+__global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
+    __shared__ AppendShared sh;
+    SegmentView v = loadSegments(ctx.counters->count[sel]);
+    uint32_t count = v.start[kSegments];
+    if (blockIdx.x * blockDim.x >= count) return;  // whole workgroup
     const PathQueue& q = ctx.queue[sel];
     const PathQueue& next = ctx.queue[sel ^ 1u];
-    while (true) {
-        uint32_t base = claimWaveBatch(&ctx.counters->shadeCursor);
-        if (base >= count) break;
-        uint32_t i = base + __lane_id();
-        bool valid = i < count;
-        PathState path;
-        ShadeOutcome outcome;
-        outcome.alive = false;
-        outcome.shadow.valid = 0;
-        if (valid) {
-            path = loadPath(q, i);
-            shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, outcome);
-        }
-        bool alive = valid && outcome.alive;
-        uint32_t slot = waveAppend(alive, &ctx.counters->count[sel ^ 1u]);
-        if (alive) storePath(next, slot, path);
-        bool wantsShadow = valid && outcome.shadow.valid;
-        uint32_t s = waveAppend(wantsShadow, &ctx.counters->shadowCount);
-        if (wantsShadow) {
-            storeShadow(ctx.shadow, s, outcome.shadow, alive ? slot : kShadowTerminated, path);
-        } else if (valid && !alive) {
-            accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
-        }
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = g < count;
+    PathState path;
+    ShadeOutcome outcome;
+    outcome.alive = false;
+    outcome.shadow.valid = 0;
+    if (valid) {
+        uint32_t i = segmentSlot(v, q.segmentCapacity, g);
+        path = loadPath(q, i);
+        shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, outcome);
+    }
+    bool alive = valid && outcome.alive;
+    bool wantsShadow = valid && outcome.shadow.valid;
+    uint32_t seg = blockIdx.x % kSegments;
+    uint32_t a, s;
+    blockAppend(alive, wantsShadow, &ctx.counters->count[sel ^ 1u][seg], &ctx.counters->shadowCount[seg], sh, a, s);
+    uint32_t slot = seg * next.segmentCapacity + a;
+    if (alive) storePath(next, slot, path);
+    if (wantsShadow) {
+        storeShadow(ctx.shadow, seg * ctx.shadow.segmentCapacity + s, outcome.shadow, alive ? slot : kShadowTerminated,
+            path);
+    } else if (valid && !alive) {
+        accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
     }
 }
 
-// Resolves the bounce's shadow requests: unoccluded contributions are added
-// to the continuing path's radiance, and finished paths reach the pixel.
-__global__ void kShadow(LaunchContext ctx, uint32_t sel) {
-    uint32_t count = ctx.counters->shadowCount;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(count));
+// Resolves the bounce's shadow requests, one per thread: unoccluded
+// contributions are added to the continuing path's radiance, and finished
+// paths reach the pixel.
+__global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadow(LaunchContext ctx, uint32_t sel) {
+    SegmentView v = loadSegments(ctx.counters->shadowCount);
+    uint32_t count = v.start[kSegments];
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(count));
+    if (g >= count) return;
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const ShadowQueue& q = ctx.shadow;
     const PathQueue& next = ctx.queue[sel ^ 1u];
-    while (true) {
-        uint32_t base = claimWaveBatch(&ctx.counters->shadowCursor);
-        if (base >= count) break;
-        uint32_t i = base + __lane_id();
-        if (i >= count) continue;
-        Ray ray(Vec3(__uint_as_float(sfield(q, kSOx)[i]), __uint_as_float(sfield(q, kSOy)[i]),
-                    __uint_as_float(sfield(q, kSOz)[i])),
-            Vec3(__uint_as_float(sfield(q, kSDx)[i]), __uint_as_float(sfield(q, kSDy)[i]),
-                __uint_as_float(sfield(q, kSDz)[i])));
-        bool occluded = traceOccluded(ctx.scene, ray, __uint_as_float(sfield(q, kSTmax)[i]), stack);
-        Vec3 c(__uint_as_float(sfield(q, kSCx)[i]), __uint_as_float(sfield(q, kSCy)[i]),
-            __uint_as_float(sfield(q, kSCz)[i]));
-        uint32_t target = sfield(q, kSTarget)[i];
-        if (target != kShadowTerminated) {
-            if (!occluded) {
-                Vec3 r(qf(next, kRx, target), qf(next, kRy, target), qf(next, kRz, target));
-                r += c;
-                sf(next, kRx, target, r.x);
-                sf(next, kRy, target, r.y);
-                sf(next, kRz, target, r.z);
-            }
-        } else {
-            Vec3 r(__uint_as_float(sfield(q, kSRx)[i]), __uint_as_float(sfield(q, kSRy)[i]),
-                __uint_as_float(sfield(q, kSRz)[i]));
-            if (!occluded) r += c;
-            accumulate(ctx.framebuffer, sfield(q, kSPixel)[i], r);
+    uint32_t i = segmentSlot(v, q.segmentCapacity, g);
+    Ray ray(Vec3(__uint_as_float(sfield(q, kSOx)[i]), __uint_as_float(sfield(q, kSOy)[i]),
+                __uint_as_float(sfield(q, kSOz)[i])),
+        Vec3(__uint_as_float(sfield(q, kSDx)[i]), __uint_as_float(sfield(q, kSDy)[i]),
+            __uint_as_float(sfield(q, kSDz)[i])));
+    bool occluded = traceOccluded(ctx.scene, ray, __uint_as_float(sfield(q, kSTmax)[i]), stack);
+    Vec3 c(__uint_as_float(sfield(q, kSCx)[i]), __uint_as_float(sfield(q, kSCy)[i]),
+        __uint_as_float(sfield(q, kSCz)[i]));
+    uint32_t target = sfield(q, kSTarget)[i];
+    if (target != kShadowTerminated) {
+        if (!occluded) {
+            Vec3 r(qf(next, kRx, target), qf(next, kRy, target), qf(next, kRz, target));
+            r += c;
+            sf(next, kRx, target, r.x);
+            sf(next, kRy, target, r.y);
+            sf(next, kRz, target, r.z);
         }
+    } else {
+        Vec3 r(__uint_as_float(sfield(q, kSRx)[i]), __uint_as_float(sfield(q, kSRy)[i]),
+            __uint_as_float(sfield(q, kSRz)[i]));
+        if (!occluded) r += c;
+        accumulate(ctx.framebuffer, sfield(q, kSPixel)[i], r);
     }
 }
 
@@ -314,60 +317,70 @@ __global__ void kAdvance(LaunchContext ctx) {
     if (ctx.tile->advanceSample) ++ctx.counters->sampleIndex;
 }
 
-// Persistent kernel: one grid-wide barrier per bounce.  Each wave claims 64
-// paths, traces, shades and resolves the shadow ray inline, and appends the
-// continuations to the other queue.  Bounce b reads count[b % 3], appends to
-// count[(b + 1) % 3] and clears count[(b + 2) % 3] and cursor[(b + 1) % 3],
-// which no block touches again before the barrier that ends bounce b.
-__global__ void kPersistent(LaunchContext ctx) {
+// Persistent kernel: one grid-wide barrier per bounce.  The resident grid
+// walks the queue's items with a static stride (every workgroup runs the same
+// number of rounds, so the workgroup-wide appends line up); each thread
+// traces, shades and resolves its shadow ray inline, and continuations are
+// appended to the workgroup's segment of the other queue.  Bounce b reads
+// pcount[b % 3], appends to pcount[(b + 1) % 3] and clears pcount[(b + 2) % 3],
+// which no workgroup touches again before the barrier that ends bounce b.
+__global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
+    __shared__ AppendShared sh;
     cg::grid_group grid = cg::this_grid();
     WavefrontCounters* counters = ctx.counters;
     RenderTile tile = *ctx.tile;
     uint32_t pixels = tile.width * tile.height;
-    uint32_t capacity = ctx.queue[0].capacity;
-    uint32_t count0 = pixels < capacity ? pixels : capacity;
+    uint32_t count0 = pixels < ctx.capacity ? pixels : ctx.capacity;
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t sample = loadAgent(&counters->sampleIndex);
-    for (uint32_t i = gid; i < count0; i += stride) {
+    for (uint32_t g = gid; g < count0; g += stride) {
         uint32_t x, y;
-        tilePixel(tile, i, x, y);
-        storePath(ctx.queue[0], i, primaryPath(ctx.camera, x, y, ctx.width, ctx.height, sample, ctx.shade.rngStrategy));
+        tilePixel(tile, g, x, y);
+        storePath(ctx.queue[0], primarySlot(g, ctx.blockSize, ctx.queue[0].segmentCapacity),
+            primaryPath(ctx.camera, x, y, ctx.width, ctx.height, sample, ctx.shade.rngStrategy));
     }
-    if (gid == 0) {
-        storeAgent(&counters->count[0], count0);
-        storeAgent(&counters->count[1], 0u);
-        storeAgent(&counters->cursor[0], 0u);
+    if (gid < kSegments) {
+        storeAgent(&counters->pcount[0][gid], primarySegmentCount(count0, ctx.blockSize, gid));
+        storeAgent(&counters->pcount[1][gid], 0u);
     }
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     grid.sync();
     uint32_t c = 0;
+    const uint32_t seg = blockIdx.x % kSegments;
     for (uint32_t bounce = 0; bounce < ctx.shade.maxDepth; ++bounce) {
         uint32_t sel = bounce & 1u;
         uint32_t c1 = c == 2u ? 0u : c + 1u;
         uint32_t c2 = c1 == 2u ? 0u : c1 + 1u;
-        uint32_t count = loadAgent(&counters->count[c]);
+        SegmentView v;
+        {
+            uint32_t acc = 0;
+            for (uint32_t s = 0; s < kSegments; ++s) {
+                v.start[s] = acc;
+                acc += loadAgent(&counters->pcount[c][s]);
+            }
+            v.start[kSegments] = acc;
+        }
+        uint32_t count = v.start[kSegments];
         if (count == 0) break;
         if (gid == 0) {
             atomicAdd(&counters->closestQueries, static_cast<unsigned long long>(count));
             if (ctx.profileQueues && bounce < 64) counters->queueCounts[bounce] = count;
-            storeAgent(&counters->count[c2], 0u);
-            storeAgent(&counters->cursor[c1], 0u);
         }
+        if (gid < kSegments) storeAgent(&counters->pcount[c2][gid], 0u);
         const PathQueue& q = ctx.queue[sel];
         const PathQueue& next = ctx.queue[sel ^ 1u];
         uint32_t shadowQueries = 0;
-        while (true) {
-            uint32_t base = claimWaveBatch(&counters->cursor[c]);
-            if (base >= count) break;
-            uint32_t i = base + __lane_id();
-            bool valid = i < count;
+        uint32_t rounds = (count + stride - 1) / stride;
+        for (uint32_t round = 0; round < rounds; ++round) {
+            uint32_t g = round * stride + gid;
+            bool valid = g < count;
             PathState path;
             ShadeOutcome outcome;
             outcome.alive = false;
             if (valid) {
-                path = loadPath(q, i);
+                path = loadPath(q, segmentSlot(v, q.segmentCapacity, g));
                 HitRecord h = traceClosest(ctx.scene, path.ray, stack);
                 shadePath(ctx.scene, path, h, ctx.shade, outcome);
                 if (outcome.shadow.valid) {
@@ -378,9 +391,10 @@ __global__ void kPersistent(LaunchContext ctx) {
                 }
             }
             bool alive = valid && outcome.alive;
-            uint32_t slot = waveAppend(alive, &counters->count[c1]);
+            uint32_t a, unused;
+            blockAppend(alive, false, &counters->pcount[c1][seg], &counters->pcount[c1][seg], sh, a, unused);
             if (alive)
-                storePath(next, slot, path);
+                storePath(next, seg * next.segmentCapacity + a, path);
             else if (valid)
                 accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
         }
@@ -452,16 +466,17 @@ namespace {
 template <class Timer>
 hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& timer) {
     size_t lds = stackLdsBytes(ctx.blockSize);
-    hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, dim3(ctx.gridBlocks), dim3(ctx.blockSize), 0, stream, ctx); });
+    dim3 grid(ctx.gridBlocks), block(ctx.blockSize);
+    hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, grid, block, 0, stream, ctx); });
     for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {
         uint32_t sel = bounce & 1u;
         e = timer(1, [&] { hipLaunchKernelGGL(kReset, dim3(1), dim3(64), 0, stream, ctx, sel ^ 1u); });
         if (e == hipSuccess)
-            e = timer(2, [&] { hipLaunchKernelGGL(kTrace, dim3(ctx.traceBlocks), dim3(ctx.blockSize), lds, stream, ctx, bounce, sel); });
+            e = timer(2, [&] { hipLaunchKernelGGL(kTrace, grid, block, lds, stream, ctx, bounce, sel); });
         if (e == hipSuccess)
-            e = timer(3, [&] { hipLaunchKernelGGL(kShade, dim3(ctx.shadeBlocks), dim3(ctx.blockSize), 0, stream, ctx, sel); });
+            e = timer(3, [&] { hipLaunchKernelGGL(kShade, grid, block, 0, stream, ctx, sel); });
         if (e == hipSuccess && ctx.scene.lightCount != 0 && !ctx.scene.ignoreDirectLightOcclusion)
-            e = timer(5, [&] { hipLaunchKernelGGL(kShadow, dim3(ctx.traceBlocks), dim3(ctx.blockSize), lds, stream, ctx, sel); });
+            e = timer(5, [&] { hipLaunchKernelGGL(kShadow, grid, block, lds, stream, ctx, sel); });
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);

@@ -30,6 +30,7 @@ void printHelp() {
               << "  --no-caustics            Disable caustic photon tracing\n"
               << "  --no-caustic-gather      Trace photons but omit camera gathering\n"
               << "  --persistent-wavefront  Use the persistent cooperative wavefront kernel\n"
+              << "  --regenerate             Use the persistent kernel with path regeneration (no bounce barriers)\n"
               << "  --profile-queues        Print queue occupancy by bounce\n"
               << "  --resume                 Resume from the selected checkpoint\n"
               << "  --coordinator            Run as a distributed coordinator (local task leases)\n"
@@ -368,6 +369,7 @@ bool parseCommandLine(int argc, char** argv, RenderConfig& render, SceneConfig& 
         }
         if (!std::strcmp(a, "--no-caustic-gather")) { render.enableCausticGather = false; continue; }
         if (!std::strcmp(a, "--persistent-wavefront")) { render.persistentWavefront = true; continue; }
+        if (!std::strcmp(a, "--regenerate")) { render.regenerate = true; continue; }
         if (!std::strcmp(a, "--profile-queues")) { render.profileQueues = true; continue; }
         if (!std::strcmp(a, "--resume")) { render.resumeFromCheckpoint = true; continue; }
         if (!std::strcmp(a, "--intersection-debug")) { render.intersectionDebug = true; continue; }
@@ -558,7 +560,7 @@ wpt_render_config toRenderConfig(const RenderConfig& c) {
     r.block_size = c.blockSize;
     r.rng_strategy = c.rngStrategy;
     r.indirect_clamp = c.indirectRadianceClamp;
-    r.mode = c.persistentWavefront ? WPT_MODE_PERSISTENT : WPT_MODE_GRAPH;
+    r.mode = c.regenerate ? WPT_MODE_REGENERATE : (c.persistentWavefront ? WPT_MODE_PERSISTENT : WPT_MODE_GRAPH);
     r.persistent_blocks = c.persistentBlocks;
     r.intersection_debug = c.intersectionDebug ? 1u : 0u;
     r.shading_normal_debug = c.shadingNormalDebug ? 1u : 0u;

@@ -38,6 +38,7 @@ struct RenderConfig {
     uint32_t maxDepth = 32, russianRouletteStartDepth = 12, samplesPerPixel = 512, blockSize = 256;
     bool tiledRendering = false, intersectionDebug = false, shadingNormalDebug = false, resumeFromCheckpoint = false;
     bool persistentWavefront = false, profileQueues = false;
+    bool regenerate = false;  // extension: persistent kernel with path regeneration
     std::string checkpointFilename = "render.checkpoint";
     uint32_t checkpointProgressPercent = 25, checkpointBufferCount = 3;
     std::string outputFilename = "render.ppm";

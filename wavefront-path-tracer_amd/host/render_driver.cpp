@@ -169,8 +169,9 @@ int RenderDriver::run() {
     if (!prepare()) return 1;
     size_t tiles = config_.tiledRendering ? createRenderTiles(config_.width, config_.height, config_.tileWidth,
         config_.tileHeight).size() : 1;
-    std::cout << "Render mode: " << (config_.persistentWavefront ? "persistent device wavefront" :
-        (config_.tiledRendering ? "tiled" : "full frame")) << ", " << tiles << " work tiles\n";
+    std::cout << "Render mode: " << (config_.regenerate ? "persistent path regeneration" :
+        (config_.persistentWavefront ? "persistent device wavefront" :
+        (config_.tiledRendering ? "tiled" : "full frame"))) << ", " << tiles << " work tiles\n";
     if (wpt_renderer_reset(renderer_) != WPT_OK ||
         wpt_renderer_render_samples(renderer_, 0, config_.samplesPerPixel) != WPT_OK) {
         std::cerr << wpt_last_error() << '\n';

@@ -15,13 +15,13 @@ from typing import Iterable, List, Optional, Sequence
 import numpy as np
 
 PACKAGE_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PACKAGE_DIR, "lib", "libwpt.so")
+LIB_PATH = os.environ.get("WPT_LIBRARY") or os.path.join(PACKAGE_DIR, "lib", "libwpt.so")
 
 OK = 0
 ACCEL_BVH, ACCEL_DIRECT = 0, 1
 DIFFUSE, METAL, DIELECTRIC, EMISSIVE, SPECULAR_GLOSSINESS = 0, 1, 2, 3, 4
 RNG_XORSHIFT32, RNG_PCG32 = 0, 1
-MODE_GRAPH, MODE_PERSISTENT = 0, 1
+MODE_GRAPH, MODE_PERSISTENT, MODE_REGENERATE = 0, 1, 2
 HIT_NONE, HIT_SPHERE, HIT_STATIC, HIT_INSTANCE, HIT_FLAT = 0, 1, 2, 3, 4
 INVALID = 0xFFFFFFFF
 TRI_HAS_NORMALS, TRI_HAS_TEXCOORDS = 1, 2
