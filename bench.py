@@ -45,6 +45,11 @@ HBM_PEAK_GBS = 8000.0
 # (center, radius, material), a mesh-instance record (transforms + ids).
 NODE_B, TRI_B, SPHERE_B, INSTANCE_B = 32, 36, 20, 96
 RAY_IN_B, HIT_OUT_B = 24, 28
+# HBM bytes per launch of the headline kernel on this workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a 256 MiB copy kernel
+# (profiles/pmc_traffic_kBatch_sphere_field_1080p_64spp.md): one launch = one
+# 64-sample frame of the regenerating scheduler.
+MEASURED_TRAFFIC = {("kBatch", 64): 38.86e9}
 
 
 def log(msg):
@@ -332,17 +337,24 @@ def main():
     log(f"{args.mode}: {value:.1f} Mrays/s, {msamples:.1f} Msamples/s, {ms_per_step:.1f} ms/frame, "
         f"{queries / (WIDTH * HEIGHT * args.spp):.3f} queries/path, mean radiance {image.mean() / samples:.4f}")
 
-    # Per-kernel device time (events around each launch; no graph).  The
-    # regenerating and persistent schedulers are one launch per sample range.
-    kernel_ms, launches = renderer.profile_kernels(0, args.profile_spp)
-    prof_stats = renderer.stats()
-    trace_slot = 2 if mode == wpt.MODE_GRAPH else 4
-    trace_ms = kernel_ms[trace_slot]
-    trace_launches = int(launches[trace_slot])
-    log(f"kernels over {args.profile_spp} spp: primary {kernel_ms[0]:.2f} ms, reset {kernel_ms[1]:.2f} ms, "
-        f"trace {kernel_ms[2]:.2f} ms ({int(launches[2])} launches), shade {kernel_ms[3]:.2f} ms, "
-        f"persistent/regenerate {kernel_ms[4]:.2f} ms ({int(launches[4])} launches), shadow {kernel_ms[5]:.2f} ms "
-        f"({int(launches[5])} launches)")
+    # Device time of the dominant kernel.  The regenerating scheduler is one
+    # kBatch launch per frame: its HIP-event time is the frames' own.  The
+    # staged scheduler's kernels are timed one by one over a few samples.
+    if mode == wpt.MODE_REGENERATE:
+        trace_ms = float(np.sum(device_ms))
+        trace_launches = args.steps
+        prof_closest, prof_shadow = stats.closest_queries * args.steps, stats.shadow_queries * args.steps
+    else:
+        kernel_ms, launches = renderer.profile_kernels(0, args.profile_spp)
+        prof_stats = renderer.stats()
+        trace_slot = 2 if mode == wpt.MODE_GRAPH else 4
+        trace_ms = kernel_ms[trace_slot]
+        trace_launches = int(launches[trace_slot])
+        prof_closest, prof_shadow = prof_stats.closest_queries, prof_stats.shadow_queries
+        log(f"kernels over {args.profile_spp} spp: primary {kernel_ms[0]:.2f} ms, reset {kernel_ms[1]:.2f} ms, "
+            f"trace {kernel_ms[2]:.2f} ms ({int(launches[2])} launches), shade {kernel_ms[3]:.2f} ms, "
+            f"persistent {kernel_ms[4]:.2f} ms ({int(launches[4])} launches), shadow {kernel_ms[5]:.2f} ms "
+            f"({int(launches[5])} launches)")
     renderer.close()
 
     others = {}
@@ -370,12 +382,14 @@ def main():
             log(f"{name}: {others[name]}")
             orr.close()
 
+    threads = min(16, os.cpu_count() or 1)
     cpu = None
-    work = None
     if not args.skip_cpu:
-        threads = min(16, os.cpu_count() or 1)
         cpu, work = cpu_baseline(scene, camera, threads, args.cpu_spp)
         log(f"cpu baseline: {cpu}")
+    else:
+        # Work counts for the byte model from one sample of the frame.
+        _, work = cpu_baseline(scene, camera, threads, 1)
 
     batch = None
     if not args.skip_batch:
@@ -396,21 +410,21 @@ def main():
         if mode == wpt.MODE_GRAPH:
             # kTrace: closest-hit traversal plus its ray in and hit out.
             per_query = closest_b + RAY_IN_B + HIT_OUT_B
-            algorithmic = per_query * prof_stats.closest_queries
+            algorithmic = per_query * prof_closest
         else:
             # One kernel does every query; paths stay in registers.
             per_query = closest_b
-            algorithmic = closest_b * prof_stats.closest_queries + shadow_b * prof_stats.shadow_queries
+            algorithmic = closest_b * prof_closest + shadow_b * prof_shadow
         achieved = algorithmic / (trace_ms / 1000.0) / 1e9
+        kernel = {wpt.MODE_GRAPH: "kTrace", wpt.MODE_PERSISTENT: "kPersistent", wpt.MODE_REGENERATE: "kBatch"}[mode]
         roofline = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": {wpt.MODE_GRAPH: "kTrace", wpt.MODE_PERSISTENT: "kPersistent",
-                       wpt.MODE_REGENERATE: "kBatch"}[mode],
+            "kernel": kernel,
+            "traffic": MEASURED_TRAFFIC.get((kernel, args.spp)),
             "bytes_per_closest_query": round(per_query, 1),
             "avg_launch_ms": round(trace_ms / max(1, trace_launches), 4),
             "launches": trace_launches,
