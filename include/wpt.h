@@ -152,6 +152,16 @@ typedef struct wpt_scene_desc {
     uint32_t material_count;
     uint32_t black_background;            /* reference HostScene.blackBackground */
     uint32_t ignore_direct_light_occlusion;
+    /* Environment map (reference HostEnvironment, environment.cpp:10-64, and
+     * uploadEnvironmentMap, scene.cu:308-334): equirectangular RGB radiance,
+     * environment_width x environment_height, row 0 at +Y, column width/2 at
+     * +X, column 3*width/4 at +Z.  NULL: no map (sky or black background).
+     * Radiance is scaled by environment_intensity; the map turns about +Y by
+     * environment_rotation_degrees. */
+    const float* environment_rgb;
+    uint32_t environment_width, environment_height;
+    float environment_intensity;
+    float environment_rotation_degrees;
 } wpt_scene_desc;
 
 /* Reference Camera (include/scene/camera.cuh:7-27). */
@@ -269,6 +279,11 @@ WPT_API int wpt_scene_export_lights(const wpt_scene* scene, float* lights16, flo
                                     uint32_t* count);
 /* Light offset per instance (reference MeshInstance.lightOffset) and light
  * index per static/mesh triangle as assigned by the scene build. */
+/* Environment sampling tables (probabilities: width*height floats, alias:
+ * width*height x (probability, index) as 2 words); returns WPT_OK with
+ * *count = 0 when the scene has no map. */
+WPT_API int wpt_scene_export_environment(const wpt_scene* scene, float* probabilities, float* alias2,
+                                         uint32_t capacity, uint32_t* count);
 WPT_API int wpt_scene_export_light_indices(const wpt_scene* scene, uint32_t* instance_light_offset,
                                            uint32_t* static_light_index, uint32_t* mesh_light_index);
 

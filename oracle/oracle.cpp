@@ -194,6 +194,12 @@ struct oracle_handle {
     std::vector<TriangleLight> lights;
     std::vector<float> lightWeights;
     std::vector<AliasEntry> alias;
+    // Environment map: texels (3 floats each), importance table, alias table.
+    bool env = false;
+    uint32_t envW = 0, envH = 0;
+    float envIntensity = 1.0f, envRotation = 0.0f;
+    std::vector<float> envRgb, envProb;
+    std::vector<AliasEntry> envAlias;
 };
 
 namespace {
@@ -215,7 +221,75 @@ V3<float> xform(const wpt_transform& t, const V3<float>& v) {  // transformVecto
         vec<float>(t.local_to_world[2]) * v.z;
 }
 
+// Texel solid angle of row y, environment.cuh:31-35: the FP64 product
+// (4 pi / w) sin(pi (y + 1/2) / h) sin(pi / 2h), rounded to FP32.
+float envSolidAngle(uint32_t y, uint32_t w, uint32_t h) {
+    const double pi = 3.14159265358979323846;
+    return static_cast<float>((4.0 * pi / w) * fsin(pi * (y + 0.5) / h) * fsin(pi / (2.0 * h)));
+}
+
+// buildEnvironmentDistribution, src/scene/environment.cpp:10-64.  Weight of
+// a texel: luminance under the separable [1/8 3/4 1/8] tent (u wraps, v
+// clamps) times its solid angle, all in FP64; probabilities 0.99 weight
+// share + 0.01 solid-angle share rounded to FP32, renormalized by their FP64
+// sum; Walker alias over FP64 scaled masses with LIFO worklists.
+void buildEnvironment(oracle_handle& h) {
+    const wpt_scene_desc& d = h.d;
+    if (!d.environment_rgb) return;
+    const uint32_t w = d.environment_width, ht = d.environment_height, n = w * ht;
+    h.env = true;
+    h.envW = w;
+    h.envH = ht;
+    h.envIntensity = d.environment_intensity;
+    h.envRotation = std::fmod(d.environment_rotation_degrees, 360.0f) * (3.14159265359f / 180.0f);
+    h.envRgb.assign(d.environment_rgb, d.environment_rgb + 3ull * n);
+    const double tent[3] = {0.125, 0.75, 0.125};
+    std::vector<double> mass(n);
+    double total = 0.0;
+    for (uint32_t y = 0; y < ht; ++y) {
+        for (uint32_t x = 0; x < w; ++x) {
+            double lum = 0.0;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int yy = std::min<int>(std::max<int>(static_cast<int>(y) + dy, 0), static_cast<int>(ht) - 1);
+                for (int dx = -1; dx <= 1; ++dx) {
+                    uint32_t xx = static_cast<uint32_t>((static_cast<int64_t>(x) + w + dx) % w);
+                    const float* p = &h.envRgb[3ull * (static_cast<uint64_t>(yy) * w + xx)];
+                    lum += tent[dy + 1] * tent[dx + 1] *
+                        (0.2126 * double(p[0]) + 0.7152 * double(p[1]) + 0.0722 * double(p[2]));
+                }
+            }
+            mass[y * w + x] = lum * envSolidAngle(y, w, ht);
+            total += mass[y * w + x];
+        }
+    }
+    h.envProb.resize(n);
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double uniform = envSolidAngle(i / w, w, ht) / 12.5663706143591729538;
+        h.envProb[i] = static_cast<float>(total > 0 ? 0.99 * mass[i] / total + 0.01 * uniform : uniform);
+        sum += h.envProb[i];
+    }
+    std::vector<uint32_t> small, large;
+    for (uint32_t i = 0; i < n; ++i) {
+        h.envProb[i] = static_cast<float>(h.envProb[i] / sum);
+        mass[i] = double(h.envProb[i]) * n;
+        (mass[i] < 1.0 ? small : large).push_back(i);
+    }
+    h.envAlias.assign(n, AliasEntry{1.0f, 0});
+    while (!small.empty() && !large.empty()) {
+        uint32_t lo = small.back(), hi = large.back();
+        small.pop_back();
+        large.pop_back();
+        h.envAlias[lo] = AliasEntry{static_cast<float>(mass[lo]), hi};
+        mass[hi] += mass[lo] - 1.0;
+        (mass[hi] < 1.0 ? small : large).push_back(hi);
+    }
+    for (uint32_t i : small) h.envAlias[i] = AliasEntry{1.0f, i};
+    for (uint32_t i : large) h.envAlias[i] = AliasEntry{1.0f, i};
+}
+
 void buildTables(oracle_handle& h) {
+    buildEnvironment(h);
     const wpt_scene_desc& d = h.d;
     // Mesh light numbering, then per-instance world lights (scene_host.cpp:77-106).
     h.meshLight.assign(d.mesh_triangle_count, kInvalid);
@@ -695,6 +769,88 @@ void addPathRadiance(PathState<R>& p, V3<R> c, bool emissionOrBackground = false
     p.radiance += c;
 }
 
+// ---------------------------------------------------------------- environment lookups
+// include/scene/environment.cuh:17-70 and device_traversal.cuh:571-575.
+inline float fatan2(float y, float x) { return wpt::atan2F(y, x); }
+inline double fatan2(double y, double x) { return wpt::atan2D(y, x); }
+
+template <class R>
+V3<R> envRotate(const V3<R>& d, R angle) {  // right-handed turn about +Y
+    R c = fcos(angle), s = fsin(angle);
+    return V3<R>(c * d.x + s * d.z, d.y, -s * d.x + c * d.z);
+}
+
+// Panorama coordinates: +Y at v = 0, +X at u = 1/2, +Z at u = 3/4.
+template <class R>
+void envUv(const oracle_handle& h, const V3<R>& world, R& u, R& v) {
+    const R pi = R(3.14159265359f);
+    V3<R> d = envRotate(world, R(-h.envRotation));
+    u = fatan2(d.z, d.x) / (R(2) * pi) + R(0.5f);
+    u -= std::floor(u);
+    v = std::fmin(R(1), std::fmax(R(0), fatan2(std::sqrt(d.x * d.x + d.z * d.z), d.y) / pi));
+}
+
+template <class R>
+uint32_t envTexel(const oracle_handle& h, const V3<R>& dir) {
+    R u, v;
+    envUv(h, dir, u, v);
+    uint32_t x = static_cast<uint32_t>(u * R(h.envW)), y = static_cast<uint32_t>(v * R(h.envH));
+    return std::min(y, h.envH - 1) * h.envW + std::min(x, h.envW - 1);
+}
+
+template <class R>
+R envPdf(const oracle_handle& h, const V3<R>& dir) {
+    if (!h.env) return R(0);
+    uint32_t i = envTexel(h, dir);
+    return R(h.envProb[i]) / R(envSolidAngle(i / h.envW, h.envW, h.envH));
+}
+
+// Alias pick from a FP64 candidate and a coin, then a uniform longitude and
+// a uniform cos(latitude) inside the texel (all in FP64; the longitude
+// offset in FP32 as the reference writes it).
+template <class R>
+V3<R> envSample(const oracle_handle& h, double candidate, float coin, float lon, float lat, R& pdf) {
+    const double pi = 3.14159265358979323846;
+    uint32_t n = h.envW * h.envH;
+    uint32_t i = std::min(static_cast<uint32_t>(candidate * n), n - 1);
+    if (coin >= h.envAlias[i].probability) i = h.envAlias[i].alias;
+    uint32_t x = i % h.envW, y = i / h.envW;
+    float lonClamped = std::fmin(1.0f - 1.0e-7f, std::fmax(1.0e-7f, lon));
+    double u = (static_cast<float>(x) + lonClamped) / static_cast<float>(h.envW);
+    double t = std::fmin(1.0f - 1.0e-7f, std::fmax(1.0e-7f, lat));
+    double c = (1.0 - t) * fcos(pi * y / h.envH) + t * fcos(pi * (y + 1.0) / h.envH);
+    double sn = std::sqrt(std::fmax(0.0, 1.0 - c * c));
+    double phi = (u - 0.5) * 2 * pi;
+    V3<R> local(R(static_cast<float>(sn * fcos(phi))), R(static_cast<float>(c)), R(static_cast<float>(sn * fsin(phi))));
+    pdf = R(h.envProb[i]) / R(envSolidAngle(y, h.envW, h.envH));
+    return envRotate(local, R(h.envRotation));
+}
+
+// Bilinear reconstruction with texel centers at half-integers, u wrapping
+// and v clamped, times the intensity.  Exact weights (the reference's
+// texture unit quantizes them to 1/256).
+template <class R>
+V3<R> envRadiance(const oracle_handle& h, const V3<R>& dir) {
+    R u, v;
+    envUv(h, dir, u, v);
+    R fx = u * R(h.envW) - R(0.5f), fy = v * R(h.envH) - R(0.5f);
+    R x0 = std::floor(fx), y0 = std::floor(fy);
+    R ax = fx - x0, ay = fy - y0;
+    int w = static_cast<int>(h.envW), ht = static_cast<int>(h.envH);
+    int ix = static_cast<int>(x0), iy = static_cast<int>(y0);
+    auto texel = [&](int xx, int yy) {
+        xx = ((xx % w) + w) % w;
+        yy = std::min(std::max(yy, 0), ht - 1);
+        const float* p = &h.envRgb[3ull * (static_cast<uint64_t>(yy) * w + xx)];
+        return V3<R>(R(p[0]), R(p[1]), R(p[2]));
+    };
+    V3<R> a = texel(ix, iy), b = texel(ix + 1, iy), c = texel(ix, iy + 1), e = texel(ix + 1, iy + 1);
+    R w00 = (R(1) - ax) * (R(1) - ay), w10 = ax * (R(1) - ay), w01 = (R(1) - ax) * ay, w11 = ax * ay;
+    V3<R> r(a.x * w00 + b.x * w10 + c.x * w01 + e.x * w11, a.y * w00 + b.y * w10 + c.y * w01 + e.y * w11,
+        a.z * w00 + b.z * w10 + c.z * w01 + e.z * w11);
+    return r * R(h.envIntensity);
+}
+
 template <class R> R powerHeuristic(R a, R b) {
     R a2 = a * a, b2 = b * b;
     return a2 / (a2 + b2);
@@ -846,8 +1002,24 @@ bool metalEffective(const wpt_material& m) {
 }
 
 // evaluateSurface / evaluateSurfaceCos (untextured), device_traversal.cuh:427-441, 519-527.
+// Diffuse continuation density: half cosine, half environment when a
+// usable map exists (diffuseEnvironmentProbability / diffuseContinuationPdf,
+// device_traversal.cuh:403-418).
 template <class R>
-V3<R> evaluateSurfaceCos(const wpt_material& m, const V3<R>& n, const V3<R>& wo, const V3<R>& wi, R& pdf) {
+R envMixture(const oracle_handle& h) {
+    return h.env && h.envIntensity > 0.0f && h.envW && h.envH ? R(0.5f) : R(0);
+}
+
+template <class R>
+R diffusePdf(const oracle_handle& h, const V3<R>& n, const V3<R>& wi) {
+    R p = envMixture<R>(h);
+    R cosinePdf = std::fmax(R(0), dot(n, wi)) * R(0.31830988618f);
+    return p > R(0) ? (R(1) - p) * cosinePdf + p * envPdf(h, wi) : cosinePdf;
+}
+
+template <class R>
+V3<R> evaluateSurfaceCos(const oracle_handle& h, const wpt_material& m, const V3<R>& n, const V3<R>& wo,
+    const V3<R>& wi, R& pdf) {
     V3<R> value;
     if (m.type == WPT_MATERIAL_SPECULAR_GLOSSINESS) {
         value = evalSG(SG<R>{albedoOf<R>(m), vec<R>(m.specular), R(m.roughness)}, n, wo, wi, pdf);
@@ -859,7 +1031,7 @@ V3<R> evaluateSurfaceCos(const wpt_material& m, const V3<R>& n, const V3<R>& wo,
             value = evalSG(SG<R>{V3<R>(), albedoOf<R>(m), R(m.roughness)}, n, wo, wi, pdf);
         }
     } else {
-        pdf = std::fmax(R(0), dot(n, wi)) * R(0.31830988618f);
+        pdf = diffusePdf(h, n, wi);
         value = albedoOf<R>(m) * R(0.31830988618f);
     }
     return value * std::fmax(R(0), dot(n, wi));
@@ -924,10 +1096,54 @@ void sampleDirectLight(const oracle_handle& h, PathState<R>& path, const Hit<R>&
     R geometryTerm = lc * area / (d2 * pdfSel);
     R lightPdf = pdfSel * d2 / (lc * area);
     R bsdfPdf;
-    V3<R> bsdf = evaluateSurfaceCos(material, shaded.normal, -path.ray.d, dir, bsdfPdf);
+    V3<R> bsdf = evaluateSurfaceCos(h, material, shaded.normal, -path.ray.d, dir, bsdfPdf);
     R w = powerHeuristic(lightPdf, bsdfPdf);
     V3<R> direct = hadamard(bsdf, vec<R>(lm.emission)) * (geometryTerm * w);
     addPathRadiance(path, hadamard(path.throughput, direct));
+}
+
+// The five draws of an environment sample (samplePathEnvironment,
+// device_traversal.cuh:420-425): two for a FP64 alias candidate, coin,
+// longitude, latitude.
+template <class R>
+V3<R> envDraw(const oracle_handle& h, RngState& rng, R& pdf) {
+    float coarse = randomFloat(rng), fine = randomFloat(rng);
+    double candidate = double(coarse) + double(fine) / 16777216.0;
+    float coin = randomFloat(rng), lon = randomFloat(rng), lat = randomFloat(rng);
+    return envSample<R>(h, candidate, coin, lon, lat, pdf);
+}
+
+// sampleDirectEnvironment, device_traversal.cuh:637-651.
+template <class R>
+void sampleDirectEnvironment(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const Hit<R>& shaded,
+    const wpt_material& material, Counters& cnt) {
+    if (!h.env || h.envIntensity == 0.0f) return;
+    R pdf;
+    V3<R> dir = envDraw<R>(h, path.rng, pdf);
+    if (dot(shaded.normal, dir) <= R(0) || dot(hit.geometricNormal, dir) <= R(0) || !(pdf > R(0))) return;
+    Ray<R> shadow{offsetRayOrigin(hit.position, hit.geometricNormal, dir, hit.t), dir};
+    if (!h.d.ignore_direct_light_occlusion) {
+        ++cnt.shadow;
+        tlWork = &cnt.shadowWork;
+        bool blocked = isOccluded(h, shadow, R(1.0e30f));
+        tlWork = nullptr;
+        if (blocked) return;
+    }
+    R bsdfPdf;
+    V3<R> bsdf = evaluateSurfaceCos(h, material, shaded.normal, -path.ray.d, dir, bsdfPdf);
+    R w = powerHeuristic(pdf, bsdfPdf);
+    addPathRadiance(path, hadamard(path.throughput, hadamard(bsdf, envRadiance(h, dir)) * (w / pdf)));
+}
+
+// Escape test of a dielectric branch toward the environment.
+template <class R>
+bool escapes(const oracle_handle& h, const Hit<R>& hit, const V3<R>& dir, Counters& cnt) {
+    Ray<R> ray{offsetRayOrigin(hit.position, hit.geometricNormal, dir, hit.t), dir};
+    ++cnt.shadow;
+    tlWork = &cnt.shadowWork;
+    bool blocked = isOccluded(h, ray, R(1.0e30f));
+    tlWork = nullptr;
+    return !blocked;
 }
 
 // One path, shadePaths (device_path_shading.cuh:6-173) applied bounce by bounce.
@@ -947,8 +1163,12 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
             if (didHit) color = R(0.5f) * (hit.normal + V3<R>(R(1), R(1), R(1)));
             return color;
         }
-        if (!didHit) {
-            if (!h.d.black_background) {
+        if (!didHit) {  // addEnvironmentOrSky, device_traversal.cuh:577-587
+            if (h.env) {
+                R w = path.depth > 0 && !path.specularBounce ? powerHeuristic(path.previousBsdfPdf, envPdf(h, path.ray.d))
+                                                             : R(1);
+                addPathRadiance(path, hadamard(path.throughput, envRadiance(h, path.ray.d)) * w, true);
+            } else if (!h.d.black_background) {
                 R t = R(0.5f) * (path.ray.d.y + R(1));
                 V3<R> sky = (R(1) - t) * V3<R>(R(1), R(1), R(1)) + t * V3<R>(R(0.5f), R(0.7f), R(1.0f));
                 addPathRadiance(path, hadamard(path.throughput, sky), true);
@@ -1010,10 +1230,17 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
         switch (type) {
         case WPT_MATERIAL_DIFFUSE: {
             sampleDirectLight(h, path, hit, shaded, m, cnt);
-            // continueDiffuseSurface, device_traversal.cuh:685-704 (no environment)
-            direction = sampleCosineHemisphere(shaded.normal, path.rng);
+            sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
+            // continueDiffuseSurface, device_traversal.cuh:685-704
+            R mix = envMixture<R>(h);
+            if (mix > R(0) && rnd<R>(path.rng) < mix) {
+                R ignored;
+                direction = envDraw<R>(h, path.rng, ignored);
+            } else {
+                direction = sampleCosineHemisphere(shaded.normal, path.rng);
+            }
             R cosine = dot(shaded.normal, direction);
-            R pdf = std::fmax(R(0), dot(shaded.normal, direction)) * invPi;
+            R pdf = diffusePdf(h, shaded.normal, direction);
             if (!(pdf > R(0)) || cosine <= R(0) || dot(direction, hit.geometricNormal) <= R(0)) {
                 path.active = false;
             } else {
@@ -1025,11 +1252,13 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
         }
         case WPT_MATERIAL_SPECULAR_GLOSSINESS:
             sampleDirectLight(h, path, hit, shaded, m, cnt);
+            sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
             direction = glossy();
             break;
         case WPT_MATERIAL_METAL:
             if (roughness > R(0)) {
                 sampleDirectLight(h, path, hit, shaded, m, cnt);
+                sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
                 direction = glossy();
                 break;
             }
@@ -1043,10 +1272,19 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
             path.specularBounce = true;
             path.previousBsdfPdf = R(0);
             break;
-        case WPT_MATERIAL_DIELECTRIC: {  // continueDielectricSurface, device_traversal.cuh:589-635 (no environment)
+        case WPT_MATERIAL_DIELECTRIC: {  // continueDielectricSurface, device_traversal.cuh:589-635
             path.throughput = hadamard(path.throughput, albedo);
             Interface<R> b = dielectricInterface(normalize(path.ray.d), hit.geometricNormal, shaded.normal, R(m.ior));
             R eta2 = b.eta * b.eta;
+            // With a map, both branches that escape are integrated here.
+            bool reflectionOut = false, transmissionOut = false;
+            if (h.env && path.depth + 1 < maxDepth) {
+                if (b.reflectance > R(0) && (reflectionOut = escapes(h, hit, b.reflection, cnt)))
+                    addPathRadiance(path, hadamard(path.throughput, envRadiance(h, b.reflection)) * b.reflectance);
+                if (b.reflectance < R(1) && (transmissionOut = escapes(h, hit, b.transmission, cnt)))
+                    addPathRadiance(path, hadamard(path.throughput, envRadiance(h, b.transmission)) *
+                        ((R(1) - b.reflectance) * eta2));
+            }
             bool reflected = b.reflectance >= R(1) || b.reflectance > rnd<R>(path.rng);
             if (!reflected) {
                 path.throughput *= eta2;
@@ -1059,6 +1297,7 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
                     path.mediumDensity = R(1);
                 }
             }
+            if (reflected ? reflectionOut : transmissionOut) path.active = false;
             path.specularBounce = true;
             path.previousBsdfPdf = R(0);
             direction = reflected ? b.reflection : b.transmission;
@@ -1533,6 +1772,54 @@ int oracle_create(const oracle_scene* scene, oracle_handle** out) {
 }
 
 void oracle_destroy(oracle_handle* h) { delete h; }
+
+uint32_t oracle_env_tables(const oracle_handle* h, float* probabilities, float* alias2, uint32_t capacity) {
+    uint32_t n = static_cast<uint32_t>(h->envProb.size());
+    for (uint32_t i = 0; i < std::min(n, capacity); ++i) {
+        if (probabilities) probabilities[i] = h->envProb[i];
+        if (alias2) {
+            alias2[2 * i] = h->envAlias[i].probability;
+            std::memcpy(&alias2[2 * i + 1], &h->envAlias[i].alias, 4);
+        }
+    }
+    return n;
+}
+
+float oracle_env_solid_angle(uint32_t row, uint32_t width, uint32_t height) {
+    return envSolidAngle(row, width, height);
+}
+
+int oracle_env_eval(const oracle_handle* h, const float* dirs, uint32_t n, float* uv, float* pdf, float* rgb,
+    uint32_t* texel) {
+    if (!h->env) return 1;
+    for (uint32_t i = 0; i < n; ++i) {
+        V3<float> d(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        if (uv) envUv(*h, d, uv[2 * i], uv[2 * i + 1]);
+        if (pdf) pdf[i] = envPdf(*h, d);
+        if (rgb) {
+            V3<float> r = envRadiance(*h, d);
+            rgb[3 * i] = r.x;
+            rgb[3 * i + 1] = r.y;
+            rgb[3 * i + 2] = r.z;
+        }
+        if (texel) texel[i] = envTexel(*h, d);
+    }
+    return 0;
+}
+
+int oracle_env_sample(const oracle_handle* h, const double* candidates, const float* draws3, uint32_t n, float* dirs,
+    float* pdf) {
+    if (!h->env) return 1;
+    for (uint32_t i = 0; i < n; ++i) {
+        float p;
+        V3<float> d = envSample<float>(*h, candidates[i], draws3[3 * i], draws3[3 * i + 1], draws3[3 * i + 2], p);
+        dirs[3 * i] = d.x;
+        dirs[3 * i + 1] = d.y;
+        dirs[3 * i + 2] = d.z;
+        pdf[i] = p;
+    }
+    return 0;
+}
 
 uint32_t oracle_list(const oracle_handle* h, uint32_t which, uint32_t* out, uint32_t capacity) {
     const std::vector<uint32_t>& v = which == 0 ? h->directSpheres : (which == 1 ? h->directInstances : h->leaves);

@@ -45,6 +45,18 @@ typedef struct oracle_handle oracle_handle;
 
 /* Builds the oracle's own scene tables (partition, lights, alias). */
 int oracle_create(const oracle_scene* scene, oracle_handle** out);
+
+/* Environment map of the scene (FP32): importance and alias tables
+ * (alias2: probability, alias index bits per texel), texel solid angle, and
+ * per-direction uv, pdf, radiance and texel index (null outputs skipped).
+ * oracle_env_sample draws directions from FP64 alias candidates and
+ * (coin, longitude, latitude) triples.  Both return 1 without a map. */
+uint32_t oracle_env_tables(const oracle_handle* h, float* probabilities, float* alias2, uint32_t capacity);
+float oracle_env_solid_angle(uint32_t row, uint32_t width, uint32_t height);
+int oracle_env_eval(const oracle_handle* h, const float* dirs, uint32_t n, float* uv, float* pdf, float* rgb,
+    uint32_t* texel);
+int oracle_env_sample(const oracle_handle* h, const double* candidates, const float* draws3, uint32_t n, float* dirs,
+    float* pdf);
 void oracle_destroy(oracle_handle* h);
 
 /* Partition lists: which 0 direct spheres, 1 direct instances, 2 TLAS leaves

@@ -65,6 +65,10 @@ def library():
              C.c_int),
             ("oracle_diff_directional64", [P, C.POINTER(wpt.Camera), C.POINTER(OracleDiffConfig), P, P, u32, u32, P],
              C.c_int),
+            ("oracle_env_tables", [P, P, P, u32], u32),
+            ("oracle_env_solid_angle", [u32, u32, u32], C.c_float),
+            ("oracle_env_eval", [P, P, u32, P, P, P, P], C.c_int),
+            ("oracle_env_sample", [P, P, P, u32, P, P], C.c_int),
             ("oracle_reference_rays", [u32, u32, P, u32], u32),
             ("oracle_rng_stream", [u32, u32, u32, u32, P], C.c_uint64),
             ("oracle_dielectric_reflectance", [C.c_float, C.c_float], C.c_float),
@@ -220,6 +224,43 @@ class Oracle:
         library().oracle_diff_directional64(self._h, C.byref(camera), C.byref(cfg), params.ctypes.data,
                                             direction.ctypes.data, sample_start, threads, out.ctypes.data)
         return out
+
+    def env_tables(self):
+        """(probabilities, alias probability, alias index) of the scene's map."""
+        lib = library()
+        n = lib.oracle_env_tables(self._h, None, None, 0)
+        prob = np.zeros(n, dtype=np.float32)
+        alias = np.zeros((n, 2), dtype=np.float32)
+        lib.oracle_env_tables(self._h, prob.ctypes.data, alias.ctypes.data, n)
+        return prob, alias[:, 0].copy(), alias[:, 1].copy().view(np.uint32)
+
+    def env_eval(self, dirs: np.ndarray):
+        """uv (n, 2), pdf (n), radiance (n, 3), texel (n) for directions (n, 3)."""
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        n = d.shape[0]
+        uv = np.zeros((n, 2), np.float32)
+        pdf = np.zeros(n, np.float32)
+        rgb = np.zeros((n, 3), np.float32)
+        texel = np.zeros(n, np.uint32)
+        if library().oracle_env_eval(self._h, d.ctypes.data, n, uv.ctypes.data, pdf.ctypes.data, rgb.ctypes.data,
+                                     texel.ctypes.data):
+            raise RuntimeError("scene has no environment map")
+        return uv, pdf, rgb, texel
+
+    def env_sample(self, candidates: np.ndarray, draws3: np.ndarray):
+        """Directions (n, 3) and pdfs (n) for FP64 candidates and (coin, lon, lat) rows."""
+        c = np.ascontiguousarray(candidates, dtype=np.float64)
+        d3 = np.ascontiguousarray(draws3, dtype=np.float32).reshape(-1, 3)
+        n = c.shape[0]
+        dirs = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        if library().oracle_env_sample(self._h, c.ctypes.data, d3.ctypes.data, n, dirs.ctypes.data, pdf.ctypes.data):
+            raise RuntimeError("scene has no environment map")
+        return dirs, pdf
+
+    @staticmethod
+    def env_solid_angle(row, width, height) -> float:
+        return library().oracle_env_solid_angle(row, width, height)
 
     def brute_fp64(self, rays6: np.ndarray):
         rays6 = np.ascontiguousarray(rays6, dtype=np.float32)

@@ -157,3 +157,47 @@ def camera(position, forward, up_hint=(0, 1, 0), fov=0.6, aspect=1.0):
     r /= np.linalg.norm(r)
     u = np.cross(r, f)
     return wpt.Camera(wpt.vec3(position), wpt.vec3(f), wpt.vec3(r), wpt.vec3(u), fov, aspect)
+
+
+# ---- environment-map scenes (tests/environment_shading_tests.cu)
+
+def constant_map(value, width=16, height=8):
+    import numpy as np
+    return np.full((height, width, 3), value, np.float32)
+
+
+def sun_map(x=170, y=64, width=512, height=256, radiance=50000.0):
+    """Black panorama with one bright texel, the reference's HDR sun."""
+    import numpy as np
+    m = np.zeros((height, width, 3), np.float32)
+    m[y, x] = radiance
+    return m
+
+
+def env_floor(material, env, intensity=1.0, rotation_degrees=0.0, half=500.0):
+    """A floor at y = 0 (normal +y) lit only by the environment."""
+    b = wpt.SceneBuilder()
+    f = b.material(material)
+    h = half
+    quad(b, (-h, 0, -h), (-h, 0, h), (h, 0, h), (h, 0, -h), f)
+    b.environment(env, intensity, rotation_degrees)
+    return b
+
+
+def env_slab(floor_material, env, half=0.1):
+    """A small floor square at y = 0 under a glass slab (y = 1 to 1.01,
+    outward normals, 2000 x 2000), lit by the environment through the slab."""
+    b = wpt.SceneBuilder()
+    f = b.material(floor_material)
+    g = b.material(wpt.make_material(wpt.DIELECTRIC, (1, 1, 1), ior=1.5))
+    h = half
+    quad(b, (-h, 0, -h), (-h, 0, h), (h, 0, h), (h, 0, -h), f)
+    s = 1000.0
+    quad(b, (-s, 1.0, -s), (s, 1.0, -s), (s, 1.0, s), (-s, 1.0, s), g)      # normal -y
+    quad(b, (-s, 1.01, -s), (-s, 1.01, s), (s, 1.01, s), (s, 1.01, -s), g)  # normal +y
+    b.environment(env, 1.0, 0.0)
+    return b
+
+
+def look_down(height=0.5, fov=0.05):
+    return camera((0, height, 0), (0, -1, 0), up_hint=(0, 0, 1), fov=fov)

@@ -309,6 +309,16 @@ static int createScene(const wpt_scene_desc* desc, wpt_scene** out, bool upload)
         d.lightCount = static_cast<uint32_t>(h.lights.size());
         d.blackBackground = h.blackBackground;
         d.ignoreDirectLightOcclusion = h.ignoreDirectLightOcclusion;
+        d.environment = EnvironmentMap{};
+        if (!h.envTexels.empty()) {
+            check(a.upload(h.envTexels, &d.environment.texels), "environment texels");
+            check(a.upload(h.envProbabilities, &d.environment.probabilities), "environment probabilities");
+            check(a.upload(h.envAlias, &d.environment.alias), "environment alias");
+            d.environment.width = h.envWidth;
+            d.environment.height = h.envHeight;
+            d.environment.intensity = h.envIntensity;
+            d.environment.rotation = h.envRotation;
+        }
         if (status != WPT_OK) {
             delete s;
             return status;
@@ -376,6 +386,22 @@ WPT_API int wpt_scene_export_bvh2(const wpt_scene* s, uint32_t bvh, wpt_bvh2_nod
     std::vector<wpt_bvh2_node> nodes = exportBvh2(*h);
     *nodeCount = static_cast<uint32_t>(nodes.size());
     if (out) std::memcpy(out, nodes.data(), sizeof(wpt_bvh2_node) * std::min<size_t>(capacity, nodes.size()));
+    return WPT_OK;
+}
+
+WPT_API int wpt_scene_export_environment(const wpt_scene* s, float* probabilities, float* alias2, uint32_t capacity,
+    uint32_t* count) {
+    if (!s || !count) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    const HostSceneData& h = s->host;
+    *count = static_cast<uint32_t>(h.envProbabilities.size());
+    uint32_t n = std::min<uint32_t>(capacity, *count);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (probabilities) probabilities[i] = h.envProbabilities[i];
+        if (alias2) {
+            alias2[2 * i] = h.envAlias[i].probability;
+            std::memcpy(&alias2[2 * i + 1], &h.envAlias[i].alias, 4);
+        }
+    }
     return WPT_OK;
 }
 
@@ -1098,6 +1124,8 @@ WPT_API int wpt_diff_create(wpt_scene* scene, const wpt_camera* camera, const wp
     if (!scene || !camera || !config || !out) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (!scene->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not on a device");
+    if (scene->host.envTexels.size())
+        return fail(WPT_ERROR_UNSUPPORTED, "the differentiable renderer does not support environment maps");
     const wpt_diff_config& c = *config;
     if (c.width == 0 || c.height == 0 || c.samples_per_pixel == 0 || c.max_depth == 0)
         return fail(WPT_ERROR_INVALID_ARGUMENT, "image size, samples per pixel and depth must be greater than zero");

@@ -7,7 +7,7 @@
 // path parameters are wave-uniform and read through scalar loads.  Idle
 // lanes claim tile pixels from the task's cursor with one atomic per wave.
 // A lane renders every sample of its pixel in order, one bounce per loop
-// iteration (closest hit, shading, shadow ray), and starts the pixel's next
+// iteration (closest hit, then shading with its shadow rays traced inline), and starts the pixel's next
 // sample in the same iteration its path ends, so lanes do not wait for the
 // longest path of the wave.  The pixel's sum is accumulated in sample order
 // from zero (or from the pixel's current sum when the task accumulates into
@@ -113,12 +113,9 @@ __global__ void __launch_bounds__(512) kBatch(BatchContext ctx) {
             ++closest;
             HitRecord h = traceClosest(scene, path.ray, stack);
             ShadeOutcome outcome;
-            shadePath(scene, path, h, T.shade, outcome);
-            if (outcome.shadow.valid) {
-                ++shadow;
-                Ray ray(outcome.shadow.origin, outcome.shadow.direction);
-                if (!traceOccluded(scene, ray, outcome.shadow.tmax, stack)) path.radiance += outcome.shadow.contribution;
-            }
+            InlineOcclusion<LdsStack> occ{stack, 0u};
+            shadePath(scene, path, h, T.shade, occ, outcome);
+            shadow += occ.queries;
             if (!outcome.alive) {
                 sum += outcome.isDebug ? outcome.debugColor : path.radiance;
                 ++samplesDone;

@@ -131,7 +131,7 @@ struct AppendShared {
     uint32_t base[2];
 };
 
-__device__ void blockAppend(bool a0, bool a1, uint32_t* counter0, uint32_t* counter1, AppendShared& sh,
+__device__ __forceinline__ void blockAppend(bool a0, bool a1, uint32_t* counter0, uint32_t* counter1, AppendShared& sh,
     uint32_t& index0, uint32_t& index1) {
     unsigned long long m0 = __ballot(a0), m1 = __ballot(a1);
     uint32_t r0 = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m0 >> 32),

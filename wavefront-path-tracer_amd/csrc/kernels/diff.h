@@ -207,7 +207,11 @@ WPT_HD Vec3 diffPath(const DevScene& s, const DiffLayout& l, const DiffParams& p
         if (dot(n, hit.geometricNormal) < 0.0f) n = -n;
         Vec3 direction;
         if (m.type == kDielectric) {
-            direction = continueDielectric(path, hit, n, m, strategy);
+            // The differentiable estimator has no environment map (rejected
+            // at creation), so the interface needs no escape tests.
+            ShadeParams interfaceParams{maxDepth, maxDepth, strategy, 0.0f, 0u, 0u};
+            DeferOcclusion noEscapeTests;
+            direction = continueDielectric(s, path, hit, n, m, interfaceParams, noEscapeTests);
         } else {
             Vec3 wo = -path.ray.direction;
             DiffSurface surf = diffSurface(s, l, p, h, hit, m);

@@ -16,6 +16,7 @@
 #pragma once
 
 #include "../common/rng.h"
+#include "../scene/environment.h"
 #include "traverse.h"
 
 namespace wpt {
@@ -294,6 +295,63 @@ WPT_HD Vec3 sampleSpecGloss(const SpecGlossSurface& s, const Vec3& n, const Vec3
     return specGlossBasis(n, radius * cosPhi, radius * sinPhi, sqrtf(1.0f - u));
 }
 
+// ---- occlusion policies
+// Staged kernels defer area-light shadow rays to a later pass (the request
+// carries the segment and the clamped contribution; NEE is the last radiance
+// addition of a bounce, so resolving it later keeps the reference's order of
+// floating-point additions).  Kernels that can traverse inline, and every
+// scene with an environment map (whose environment NEE and dielectric
+// escape tests happen mid-bounce), trace each shadow ray where the reference
+// does.
+struct ShadowRequest {
+    Vec3 origin;
+    Vec3 direction;
+    float tmax;
+    Vec3 contribution;
+    uint32_t valid;
+};
+
+struct DeferOcclusion {
+    static constexpr bool kInline = false;
+    WPT_HD bool blocked(const DevScene&, const Ray&, float) { return false; }
+};
+
+template <class Stack>
+struct InlineOcclusion {
+    static constexpr bool kInline = true;
+    Stack& stack;
+    uint32_t queries;
+    WPT_HD bool blocked(const DevScene& s, const Ray& ray, float tmax) {
+        ++queries;
+        return traceOccluded(s, ray, tmax, stack);
+    }
+};
+
+// ---- environment sampling and MIS (device_traversal.cuh:403-425, 571-587,
+// 637-651)
+WPT_HD float diffuseEnvironmentProbability(const DevScene& s) {
+    const EnvironmentMap& m = s.environment;
+    return m.texels && m.intensity > 0.0f && m.width && m.height && m.probabilities && m.alias ? 0.5f : 0.0f;
+}
+
+// Mixture of cosine sampling and environment sampling (half each when a
+// usable map exists), the density the diffuse continuation samples.
+WPT_HD float diffuseContinuationPdf(const DevScene& s, const Vec3& normal, const Vec3& direction) {
+    float probability = diffuseEnvironmentProbability(s);
+    float cosinePdf = fmaxf(0.0f, dot(normal, direction)) * 0.31830988618f;
+    return probability > 0.0f ? (1.0f - probability) * cosinePdf +
+        probability * environmentPdf(s.environment, direction) : cosinePdf;
+}
+
+// Five draws: alias candidate (two, combined in double), coin, longitude,
+// latitude.
+WPT_HD EnvironmentSample samplePathEnvironment(const DevScene& s, PathState& path, uint32_t strategy) {
+    float coarse = rand01(path, strategy), fine = rand01(path, strategy);
+    double candidate = static_cast<double>(coarse) + static_cast<double>(fine) * (1.0 / 16777216.0);
+    float coin = rand01(path, strategy), u = rand01(path, strategy), v = rand01(path, strategy);
+    return sampleEnvironment(s.environment, candidate, coin, u, v);
+}
+
 // ---- material evaluation (device_traversal.cuh:427-441, 519-527)
 WPT_HD bool isMetalLike(const Material& m) { return m.type == kMetal || (m.type == kDiffuse && m.metallic > 0.5f); }
 
@@ -310,8 +368,8 @@ WPT_HD SpecGlossSurface glossySurface(const Material& m) {
     return s;
 }
 
-WPT_HD Vec3 evaluateSurfaceCos(const Material& m, const Vec3& shadingNormal, const Vec3& view, const Vec3& light,
-    float& pdf) {
+WPT_HD Vec3 evaluateSurfaceCos(const DevScene& sc, const Material& m, const Vec3& shadingNormal, const Vec3& view,
+    const Vec3& light, float& pdf) {
     Vec3 value;
     if (m.type == kSpecularGlossiness) {
         value = evaluateSpecGloss(glossySurface(m), shadingNormal, view, light, pdf);
@@ -327,29 +385,16 @@ WPT_HD Vec3 evaluateSurfaceCos(const Material& m, const Vec3& shadingNormal, con
             value = evaluateSpecGloss(s, shadingNormal, view, light, pdf);
         }
     } else {
-        pdf = fmaxf(0.0f, dot(shadingNormal, light)) * kInvPi;
+        pdf = diffuseContinuationPdf(sc, shadingNormal, light);
         value = m.albedo * kInvPi;
     }
     return value * fmaxf(0.0f, dot(shadingNormal, light));
 }
 
 // ---- next-event estimation toward triangle lights (device_traversal.cuh:529-569)
-//
-// The shadow test is deferred: the request carries the segment and the
-// clamped contribution, and whoever resolves it adds the contribution to the
-// path radiance when the segment is unoccluded.  NEE is the last radiance
-// addition of a bounce, so resolving it after the continuation is sampled
-// keeps the reference's order of floating-point additions.
-struct ShadowRequest {
-    Vec3 origin;
-    Vec3 direction;
-    float tmax;
-    Vec3 contribution;
-    uint32_t valid;
-};
-
+template <class Occluder>
 WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
-    const Material& material, const ShadeParams& sp, ShadowRequest& request) {
+    const Material& material, const ShadeParams& sp, Occluder& occ, ShadowRequest& request) {
     if (s.lightCount == 0) return;
     uint32_t candidate = static_cast<uint32_t>(rand01(path, sp.rngStrategy) * static_cast<float>(s.lightCount));
     if (candidate > s.lightCount - 1) candidate = s.lightCount - 1;
@@ -372,15 +417,18 @@ WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceH
     Vec3 segment = shadowTarget - shadowOrigin;
     float shadowDistance = length(segment);
     if (!(shadowDistance > kRayMinDistance)) return;
+    Ray shadow(shadowOrigin, segment / shadowDistance);
+    if (Occluder::kInline && !s.ignoreDirectLightOcclusion && occ.blocked(s, shadow, shadowDistance - kRayMinDistance))
+        return;
     const Material& lightMaterial = s.materials[light.material];
     float geometryTerm = lightCosine * light.area / (distanceSquared * light.selectionPdf);
     float lightPdf = light.selectionPdf * distanceSquared / (lightCosine * light.area);
     float bsdfPdf;
-    Vec3 bsdf = evaluateSurfaceCos(material, shadingNormal, -path.ray.direction, direction, bsdfPdf);
+    Vec3 bsdf = evaluateSurfaceCos(s, material, shadingNormal, -path.ray.direction, direction, bsdfPdf);
     float misWeight = powerHeuristic(lightPdf, bsdfPdf);
     Vec3 directLighting = hadamard(bsdf, lightMaterial.emission) * (geometryTerm * misWeight);
     Vec3 contribution = hadamard(path.throughput, directLighting);
-    if (s.ignoreDirectLightOcclusion) {
+    if (Occluder::kInline || s.ignoreDirectLightOcclusion) {
         addPathRadiance(path, contribution, sp.clamp, false);
         return;
     }
@@ -390,18 +438,41 @@ WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceH
         if (magnitude > sp.clamp) contribution *= sp.clamp / magnitude;
     }
     request.origin = shadowOrigin;
-    request.direction = segment / shadowDistance;
+    request.direction = shadow.direction;
     request.tmax = shadowDistance - kRayMinDistance;
     request.contribution = contribution;
     request.valid = 1;
 }
 
+// Next-event estimation toward the environment (device_traversal.cuh:637-651).
+template <class Occluder>
+WPT_HD void sampleDirectEnvironment(const DevScene& s, PathState& path, const SurfaceHit& hit,
+    const Vec3& shadingNormal, const Material& material, const ShadeParams& sp, Occluder& occ) {
+    const EnvironmentMap& m = s.environment;
+    if (!m.texels || m.intensity == 0.0f) return;
+    EnvironmentSample light = samplePathEnvironment(s, path, sp.rngStrategy);
+    float cosine = dot(shadingNormal, light.direction);
+    if (cosine <= 0.0f || dot(hit.geometricNormal, light.direction) <= 0.0f || !(light.pdf > 0.0f)) return;
+    Ray shadow(offsetRayOrigin(hit.position, hit.geometricNormal, light.direction, hit.t), light.direction);
+    if (!s.ignoreDirectLightOcclusion && occ.blocked(s, shadow, 1.0e30f)) return;
+    float bsdfPdf;
+    Vec3 bsdf = evaluateSurfaceCos(s, material, shadingNormal, -path.ray.direction, light.direction, bsdfPdf);
+    float weight = powerHeuristic(light.pdf, bsdfPdf);
+    Vec3 direct = hadamard(bsdf, environmentRadiance(m, light.direction)) * (weight / light.pdf);
+    addPathRadiance(path, hadamard(path.throughput, direct), sp.clamp, false);
+}
+
 // ---- continuations
-WPT_HD Vec3 continueDiffuse(PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal, const Material& m,
-    uint32_t strategy) {
-    Vec3 direction = sampleCosineHemisphere(shadingNormal, path, strategy);
+// Diffuse: cosine sampling, or half the time an environment sample when a
+// map exists (device_traversal.cuh:685-704).
+WPT_HD Vec3 continueDiffuse(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
+    const Material& m, uint32_t strategy) {
+    float probability = diffuseEnvironmentProbability(s);
+    Vec3 direction = probability > 0.0f && rand01(path, strategy) < probability
+        ? samplePathEnvironment(s, path, strategy).direction
+        : sampleCosineHemisphere(shadingNormal, path, strategy);
     float cosine = dot(shadingNormal, direction);
-    float pdf = fmaxf(0.0f, dot(shadingNormal, direction)) * kInvPi;
+    float pdf = diffuseContinuationPdf(s, shadingNormal, direction);
     if (!(pdf > 0.0f) || cosine <= 0.0f || dot(direction, hit.geometricNormal) <= 0.0f) {
         path.active = 0;
         return direction;
@@ -432,13 +503,37 @@ WPT_HD Vec3 continueGlossy(PathState& path, const SurfaceHit& hit, const Vec3& s
     return direction;
 }
 
-WPT_HD Vec3 continueDielectric(PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal, const Material& m,
-    uint32_t strategy) {
+// Dielectric: Fresnel coin between reflection and refraction
+// (device_traversal.cuh:589-635).  With an environment map, both branches'
+// escapes to the environment are integrated at the interface (each tested
+// for occlusion), and a path whose chosen branch escapes ends here.
+template <class Occluder>
+WPT_HD Vec3 continueDielectric(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
+    const Material& m, const ShadeParams& sp, Occluder& occ) {
     path.throughput = hadamard(path.throughput, m.albedo);
     DielectricInterface boundary = dielectricInterface(normalize(path.ray.direction), hit.geometricNormal,
         shadingNormal, m.ior);
     float etaSquared = boundary.eta * boundary.eta;
-    bool reflected = boundary.reflectance >= 1.0f || boundary.reflectance > rand01(path, strategy);
+    bool reflectionEscapes = false, transmissionEscapes = false;
+    if (s.environment.texels && path.depth + 1 < sp.maxDepth) {
+        if (boundary.reflectance > 0.0f) {
+            Vec3 direction = boundary.reflection;
+            Ray ray(offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t), direction);
+            reflectionEscapes = !occ.blocked(s, ray, 1.0e30f);
+            if (reflectionEscapes)
+                addPathRadiance(path, hadamard(path.throughput, environmentRadiance(s.environment, direction)) *
+                    boundary.reflectance, sp.clamp, false);
+        }
+        if (boundary.reflectance < 1.0f) {
+            Vec3 direction = boundary.transmission;
+            Ray ray(offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t), direction);
+            transmissionEscapes = !occ.blocked(s, ray, 1.0e30f);
+            if (transmissionEscapes)
+                addPathRadiance(path, hadamard(path.throughput, environmentRadiance(s.environment, direction)) *
+                    ((1.0f - boundary.reflectance) * etaSquared), sp.clamp, false);
+        }
+    }
+    bool reflected = boundary.reflectance >= 1.0f || boundary.reflectance > rand01(path, sp.rngStrategy);
     if (!reflected) {
         path.throughput *= etaSquared;
         path.etaScale /= etaSquared;
@@ -450,6 +545,7 @@ WPT_HD Vec3 continueDielectric(PathState& path, const SurfaceHit& hit, const Vec
             path.mediumDensity = 1.0f;
         }
     }
+    if (reflected ? reflectionEscapes : transmissionEscapes) path.active = 0;
     path.specularBounce = 1;
     path.previousBsdfPdf = 0.0f;
     // Per-component selects keep the interface in registers.
@@ -460,7 +556,8 @@ WPT_HD Vec3 continueDielectric(PathState& path, const SurfaceHit& hit, const Vec
 
 // One bounce of shading.  Returns true when the path continues with
 // path.ray set to the continuation.  A terminated path adds `debugColor` to
-// its pixel when `isDebug`, else its radiance after `shadow` is resolved.
+// its pixel when `isDebug`, else its radiance after `shadow` is resolved
+// (deferred occlusion only).
 struct ShadeOutcome {
     bool alive;
     bool isDebug;
@@ -468,20 +565,9 @@ struct ShadeOutcome {
     ShadowRequest shadow;
 };
 
+template <class Occluder>
 WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp,
-    ShadeOutcome& out);
-
-WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp,
-    ShadeOutcome& out) {
-    out.isDebug = false;
-    out.shadow.valid = 0;
-    bool alive = shadeBounce(s, path, h, sp, out);
-    out.alive = alive;
-    return alive;
-}
-
-WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp,
-    ShadeOutcome& out) {
+    Occluder& occ, ShadeOutcome& out) {
     bool didHit = h.kind != kHitNone;
     if (sp.intersectionDebug || sp.shadingNormalDebug) {
         Vec3 color(0.05f, 0.10f, 0.25f);
@@ -495,7 +581,14 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
         return false;
     }
     if (!didHit) {
-        if (!s.blackBackground) {
+        // addEnvironmentOrSky, device_traversal.cuh:577-587.
+        if (s.environment.texels) {
+            float weight = path.depth > 0 && !path.specularBounce
+                ? powerHeuristic(path.previousBsdfPdf, environmentPdf(s.environment, path.ray.direction))
+                : 1.0f;
+            addPathRadiance(path, hadamard(path.throughput, environmentRadiance(s.environment, path.ray.direction)) *
+                weight, sp.clamp, true);
+        } else if (!s.blackBackground) {
             float t = 0.5f * (path.ray.direction.y + 1.0f);
             Vec3 sky = (1.0f - t) * Vec3(1.0f, 1.0f, 1.0f) + t * Vec3(0.5f, 0.7f, 1.0f);
             addPathRadiance(path, hadamard(path.throughput, sky), sp.clamp, true);
@@ -539,16 +632,19 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
     Vec3 direction;
     switch (type) {
     case kDiffuse:
-        sampleDirectLight(s, path, hit, shadingNormal, material, sp, out.shadow);
-        direction = continueDiffuse(path, hit, shadingNormal, material, sp.rngStrategy);
+        sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
+        sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
+        direction = continueDiffuse(s, path, hit, shadingNormal, material, sp.rngStrategy);
         break;
     case kSpecularGlossiness:
-        sampleDirectLight(s, path, hit, shadingNormal, material, sp, out.shadow);
+        sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
+        sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
         direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
         break;
     case kMetal:
         if (roughness > 0.0f) {
-            sampleDirectLight(s, path, hit, shadingNormal, material, sp, out.shadow);
+            sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
+            sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
             direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
             break;
         }
@@ -564,7 +660,7 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
         }
         break;
     case kDielectric:
-        direction = continueDielectric(path, hit, shadingNormal, material, sp.rngStrategy);
+        direction = continueDielectric(s, path, hit, shadingNormal, material, sp, occ);
         break;
     default:
         break;
@@ -589,6 +685,16 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
         return true;
     }
     return false;
+}
+
+template <class Occluder>
+WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp, Occluder& occ,
+    ShadeOutcome& out) {
+    out.isDebug = false;
+    out.shadow.valid = 0;
+    bool alive = shadeBounce(s, path, h, sp, occ, out);
+    out.alive = alive;
+    return alive;
 }
 
 // Primary ray and fresh path state for one pixel (renderer.cu:25-41,

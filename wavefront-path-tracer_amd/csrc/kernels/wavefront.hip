@@ -118,11 +118,11 @@ __device__ __forceinline__ uint32_t* sfield(const ShadowQueue& q, uint32_t f) {
     return q.words + static_cast<size_t>(f) * q.capacity;
 }
 
-__device__ Ray loadRay(const PathQueue& q, uint32_t i) {
+__device__ __forceinline__ Ray loadRay(const PathQueue& q, uint32_t i) {
     return Ray(Vec3(qf(q, kOx, i), qf(q, kOy, i), qf(q, kOz, i)), Vec3(qf(q, kDx, i), qf(q, kDy, i), qf(q, kDz, i)));
 }
 
-__device__ PathState loadPath(const PathQueue& q, uint32_t i) {
+__device__ __forceinline__ PathState loadPath(const PathQueue& q, uint32_t i) {
     PathState p;
     p.ray = loadRay(q, i);
     p.throughput = Vec3(qf(q, kTx, i), qf(q, kTy, i), qf(q, kTz, i));
@@ -140,7 +140,7 @@ __device__ PathState loadPath(const PathQueue& q, uint32_t i) {
     return p;
 }
 
-__device__ void storePath(const PathQueue& q, uint32_t i, const PathState& p) {
+__device__ __forceinline__ void storePath(const PathQueue& q, uint32_t i, const PathState& p) {
     sf(q, kOx, i, p.ray.origin.x);
     sf(q, kOy, i, p.ray.origin.y);
     sf(q, kOz, i, p.ray.origin.z);
@@ -220,7 +220,7 @@ __device__ __forceinline__ HitRecord loadHit(const HitBuffer& hits, uint32_t i) 
     return h;
 }
 
-__device__ void storeShadow(const ShadowQueue& q, uint32_t s, const ShadowRequest& r, uint32_t target,
+__device__ __forceinline__ void storeShadow(const ShadowQueue& q, uint32_t s, const ShadowRequest& r, uint32_t target,
     const PathState& path) {
     sfield(q, kSOx)[s] = __float_as_uint(r.origin.x);
     sfield(q, kSOy)[s] = __float_as_uint(r.origin.y);
@@ -242,14 +242,19 @@ __device__ void storeShadow(const ShadowQueue& q, uint32_t s, const ShadowReques
 // Staged shading, one path per thread: continuing paths are appended to the
 // workgroup's segment of the next queue, shadow requests to its segment of
 // the shadow queue; finished paths without a pending shadow ray reach the
-// framebuffer.
+// framebuffer.  With `Inline` (scenes with an environment map, whose shading
+// traces shadow rays between its radiance additions) every shadow ray is
+// traced here and the shadow queue stays empty.
+template <bool Inline>
 __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
     __shared__ AppendShared sh;
     SegmentView v = loadSegments(ctx.counters->count[sel]);
     uint32_t count = v.start[kSegments];
     if (blockIdx.x * blockDim.x >= count) return;  // whole workgroup
-    const PathQueue& q = ctx.queue[sel];
-    const PathQueue& next = ctx.queue[sel ^ 1u];
+    // Queue descriptors by value: a dynamic index into the kernel argument
+    // makes the compiler copy the whole argument to scratch.
+    const PathQueue q = sel ? ctx.queue[1] : ctx.queue[0];
+    const PathQueue next = sel ? ctx.queue[0] : ctx.queue[1];
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = g < count;
     PathState path;
@@ -259,7 +264,15 @@ __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
     if (valid) {
         uint32_t i = segmentSlot(v, q.segmentCapacity, g);
         path = loadPath(q, i);
-        shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, outcome);
+        if constexpr (Inline) {
+            LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+            InlineOcclusion<LdsStack> occ{stack, 0u};
+            shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
+            if (occ.queries) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(occ.queries));
+        } else {
+            DeferOcclusion occ;
+            shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
+        }
     }
     bool alive = valid && outcome.alive;
     bool wantsShadow = valid && outcome.shadow.valid;
@@ -382,13 +395,9 @@ __global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
             if (valid) {
                 path = loadPath(q, segmentSlot(v, q.segmentCapacity, g));
                 HitRecord h = traceClosest(ctx.scene, path.ray, stack);
-                shadePath(ctx.scene, path, h, ctx.shade, outcome);
-                if (outcome.shadow.valid) {
-                    ++shadowQueries;
-                    Ray ray(outcome.shadow.origin, outcome.shadow.direction);
-                    if (!traceOccluded(ctx.scene, ray, outcome.shadow.tmax, stack))
-                        path.radiance += outcome.shadow.contribution;
-                }
+                InlineOcclusion<LdsStack> occ{stack, 0u};
+                shadePath(ctx.scene, path, h, ctx.shade, occ, outcome);
+                shadowQueries += occ.queries;
             }
             bool alive = valid && outcome.alive;
             uint32_t a, unused;
@@ -467,6 +476,7 @@ template <class Timer>
 hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& timer) {
     size_t lds = stackLdsBytes(ctx.blockSize);
     dim3 grid(ctx.gridBlocks), block(ctx.blockSize);
+    const bool inlineShadows = ctx.scene.environment.texels != nullptr;
     hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, grid, block, 0, stream, ctx); });
     for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {
         uint32_t sel = bounce & 1u;
@@ -474,8 +484,13 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
         if (e == hipSuccess)
             e = timer(2, [&] { hipLaunchKernelGGL(kTrace, grid, block, lds, stream, ctx, bounce, sel); });
         if (e == hipSuccess)
-            e = timer(3, [&] { hipLaunchKernelGGL(kShade, grid, block, 0, stream, ctx, sel); });
-        if (e == hipSuccess && ctx.scene.lightCount != 0 && !ctx.scene.ignoreDirectLightOcclusion)
+            e = timer(3, [&] {
+                if (inlineShadows)
+                    hipLaunchKernelGGL(kShade<true>, grid, block, lds, stream, ctx, sel);
+                else
+                    hipLaunchKernelGGL(kShade<false>, grid, block, 0, stream, ctx, sel);
+            });
+        if (e == hipSuccess && !inlineShadows && ctx.scene.lightCount != 0 && !ctx.scene.ignoreDirectLightOcclusion)
             e = timer(5, [&] { hipLaunchKernelGGL(kShadow, grid, block, lds, stream, ctx, sel); });
     }
     if (e != hipSuccess) return e;
@@ -503,7 +518,7 @@ hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out) {
     size_t lds = stackLdsBytes(blockSize);
     int b = static_cast<int>(blockSize);
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace, b, lds);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade, b, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade<false>, b, 0);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow, b, lds);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent, b, lds);
     return e;

@@ -137,6 +137,21 @@ struct LightAlias {
 };
 
 // Passed by value to every kernel (reference Scene, scene.cuh:117-161).
+struct EnvAlias {
+    float probability;
+    uint32_t alias;
+};
+
+// Equirectangular environment (texels == nullptr: none).
+struct EnvironmentMap {
+    const float4* texels;         // width * height RGB(+1), row 0 at +Y
+    const float* probabilities;   // importance per texel, sums to 1
+    const EnvAlias* alias;
+    uint32_t width, height;
+    float intensity;
+    float rotation;               // radians about +Y
+};
+
 struct DevScene {
     const float4* spheres;  // center xyz, radius
     const uint32_t* sphereMaterial;
@@ -166,6 +181,7 @@ struct DevScene {
     uint32_t blackBackground;
     uint32_t ignoreDirectLightOcclusion;
     uint32_t pad;
+    EnvironmentMap environment;
 };
 
 constexpr uint32_t kTlasSphereBit = 0x80000000u;

@@ -11,11 +11,13 @@
 // builder produces the same kind of tree (binary, one primitive per leaf) with
 // a host binned-SAH construction.
 #include "scene_build.h"
+#include "environment.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <future>
+#include <limits>
 #include <thread>
 
 namespace wpt {
@@ -386,6 +388,76 @@ std::vector<wpt_bvh2_node> exportBvh2(const HostBvh& bvh) {
     return out;
 }
 
+// Environment importance distribution, reference buildEnvironmentDistribution
+// (src/scene/environment.cpp:10-64): luminance averaged over the bilinear
+// tent of each texel cell (separable [1/8, 3/4, 1/8], wrap in u, clamp in v)
+// times the texel's solid angle; 99% of it mixed with 1% uniform solid angle
+// so black texels keep support; normalized in double; Walker alias with
+// LIFO small/large worklists.
+std::string buildEnvironment(const wpt_scene_desc& desc, HostSceneData& out) {
+    if (!desc.environment_rgb) return "";
+    uint64_t count = static_cast<uint64_t>(desc.environment_width) * desc.environment_height;
+    if (count == 0 || count > std::numeric_limits<uint32_t>::max()) return "environment has invalid dimensions";
+    if (!std::isfinite(desc.environment_intensity) || desc.environment_intensity < 0.0f ||
+        !std::isfinite(desc.environment_rotation_degrees))
+        return "environment intensity or rotation is invalid";
+    const uint32_t w = desc.environment_width, h = desc.environment_height;
+    out.envTexels.resize(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        const float* p = desc.environment_rgb + 3 * i;
+        if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]) || p[0] < 0 || p[1] < 0 || p[2] < 0)
+            return "environment contains negative or non-finite radiance";
+        out.envTexels[i] = float4{p[0], p[1], p[2], 1.0f};
+    }
+    std::vector<double> weights(count);
+    double total = 0.0;
+    const double filter[3] = {0.125, 0.75, 0.125};
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t x = i % w, y = i / w;
+        uint32_t columns[3] = {x ? x - 1 : w - 1, x, x + 1 < w ? x + 1 : 0};
+        uint32_t rows[3] = {y ? y - 1 : 0, y, y + 1 < h ? y + 1 : h - 1};
+        double average = 0.0;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                const float4& p = out.envTexels[static_cast<uint64_t>(rows[r]) * w + columns[c]];
+                average += filter[r] * filter[c] * (0.2126 * double(p.x) + 0.7152 * double(p.y) + 0.0722 * double(p.z));
+            }
+        weights[i] = average * environmentSolidAngle(y, w, h);
+        total += weights[i];
+    }
+    const double fourPi = 12.5663706143591729538;
+    out.envProbabilities.resize(count);
+    out.envAlias.resize(count);
+    double sum = 0.0;
+    for (uint32_t i = 0; i < count; ++i) {
+        double uniform = environmentSolidAngle(i / w, w, h) / fourPi;
+        out.envProbabilities[i] = static_cast<float>(total > 0 ? 0.99 * weights[i] / total + 0.01 * uniform : uniform);
+        sum += out.envProbabilities[i];
+    }
+    std::vector<uint32_t> small, large;
+    for (uint32_t i = 0; i < count; ++i) {
+        out.envProbabilities[i] /= sum;
+        weights[i] = double(out.envProbabilities[i]) * count;
+        (weights[i] < 1.0 ? small : large).push_back(i);
+    }
+    while (!small.empty() && !large.empty()) {
+        uint32_t low = small.back();
+        small.pop_back();
+        uint32_t high = large.back();
+        large.pop_back();
+        out.envAlias[low] = EnvAlias{static_cast<float>(weights[low]), high};
+        weights[high] += weights[low] - 1.0;
+        (weights[high] < 1.0 ? small : large).push_back(high);
+    }
+    for (uint32_t i : small) out.envAlias[i] = EnvAlias{1.0f, i};
+    for (uint32_t i : large) out.envAlias[i] = EnvAlias{1.0f, i};
+    out.envWidth = w;
+    out.envHeight = h;
+    out.envIntensity = desc.environment_intensity;
+    out.envRotation = std::fmod(desc.environment_rotation_degrees, 360.0f) * (3.14159265359f / 180.0f);
+    return "";
+}
+
 std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsigned threads) {
     if ((desc.sphere_count && !desc.spheres) || (desc.static_triangle_count && !desc.static_triangles) ||
         (desc.flat_triangle_count && !desc.flat_triangles) || (desc.mesh_triangle_count && !desc.mesh_triangles) ||
@@ -398,6 +470,8 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
     out = HostSceneData{};
     out.blackBackground = desc.black_background ? 1u : 0u;
     out.ignoreDirectLightOcclusion = desc.ignore_direct_light_occlusion ? 1u : 0u;
+    std::string envError = buildEnvironment(desc, out);
+    if (!envError.empty()) return envError;
     for (uint32_t i = 0; i < materialCount; ++i) {
         const wpt_material& m = desc.materials[i];
         if (m.type > WPT_MATERIAL_SPECULAR_GLOSSINESS) return "material has an unknown type";

@@ -69,6 +69,12 @@ struct HostSceneData {
     std::vector<uint32_t> meshLightIndex;  // concatenated mesh triangles
     uint32_t blackBackground = 0;
     uint32_t ignoreDirectLightOcclusion = 0;
+    // Environment map (empty texels: none).
+    std::vector<float4> envTexels;
+    std::vector<float> envProbabilities;
+    std::vector<EnvAlias> envAlias;
+    uint32_t envWidth = 0, envHeight = 0;
+    float envIntensity = 1.0f, envRotation = 0.0f;
     uint32_t maxStackDepth = 0;  // worst-case traversal stack use
 };
 

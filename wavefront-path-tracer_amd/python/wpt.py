@@ -69,7 +69,10 @@ class SceneDesc(C.Structure):
                 ("meshes", C.POINTER(Mesh)), ("mesh_count", C.c_uint32),
                 ("instances", C.POINTER(Instance)), ("instance_count", C.c_uint32),
                 ("materials", C.POINTER(Material)), ("material_count", C.c_uint32),
-                ("black_background", C.c_uint32), ("ignore_direct_light_occlusion", C.c_uint32)]
+                ("black_background", C.c_uint32), ("ignore_direct_light_occlusion", C.c_uint32),
+                ("environment_rgb", C.POINTER(C.c_float)), ("environment_width", C.c_uint32),
+                ("environment_height", C.c_uint32), ("environment_intensity", C.c_float),
+                ("environment_rotation_degrees", C.c_float)]
 
 
 class Camera(C.Structure):
@@ -174,6 +177,7 @@ def library() -> C.CDLL:
         "wpt_scene_get_list": ([P, u32, P, u32, C.POINTER(u32)], C.c_int),
         "wpt_scene_export_bvh2": ([P, u32, P, u32, C.POINTER(u32)], C.c_int),
         "wpt_scene_export_lights": ([P, P, P, u32, C.POINTER(u32)], C.c_int),
+        "wpt_scene_export_environment": ([P, P, P, u32, C.POINTER(u32)], C.c_int),
         "wpt_scene_export_light_indices": ([P, P, P, P], C.c_int),
         "wpt_trace_closest": ([P, P, u32, P, P], C.c_int),
         "wpt_trace_occluded": ([P, P, P, u32, P, P], C.c_int),
@@ -286,7 +290,19 @@ class SceneBuilder:
         self.materials: List[Material] = []
         self.black_background = False
         self.ignore_direct_light_occlusion = False
+        self.environment_rgb = None  # float32 (height, width, 3), row 0 at +Y
+        self.environment_intensity = 1.0
+        self.environment_rotation_degrees = 0.0
         self._keep = []
+
+    def environment(self, rgb, intensity=1.0, rotation_degrees=0.0):
+        """Equirectangular HDR map: float (height, width, 3) linear radiance."""
+        arr = np.ascontiguousarray(rgb, dtype=np.float32)
+        if arr.ndim != 3 or arr.shape[2] != 3:
+            raise ValueError("environment must be (height, width, 3)")
+        self.environment_rgb = arr
+        self.environment_intensity = float(intensity)
+        self.environment_rotation_degrees = float(rotation_degrees)
 
     def sphere(self, center, radius, material, acceleration=ACCEL_BVH):
         self.spheres.append(Sphere(vec3(center), radius, material, acceleration))
@@ -323,6 +339,13 @@ class SceneBuilder:
         d.materials, d.material_count = self._array(Material, self.materials)
         d.black_background = 1 if self.black_background else 0
         d.ignore_direct_light_occlusion = 1 if self.ignore_direct_light_occlusion else 0
+        if self.environment_rgb is not None:
+            env = self.environment_rgb
+            self._keep.append(env)
+            d.environment_rgb = env.ctypes.data_as(C.POINTER(C.c_float))
+            d.environment_height, d.environment_width = env.shape[0], env.shape[1]
+            d.environment_intensity = self.environment_intensity
+            d.environment_rotation_degrees = self.environment_rotation_degrees
         self._desc = d
         return d
 
@@ -402,6 +425,17 @@ class Scene:
         check(library().wpt_scene_export_lights(self._h, lights.ctypes.data, alias.ctypes.data, n.value, C.byref(n)),
               "wpt_scene_export_lights")
         return lights, alias
+
+    def environment_tables(self):
+        """(probabilities, alias probability, alias index) of the scene's map."""
+        n = C.c_uint32()
+        check(library().wpt_scene_export_environment(self._h, None, None, 0, C.byref(n)),
+              "wpt_scene_export_environment")
+        prob = np.zeros(n.value, dtype=np.float32)
+        alias = np.zeros((n.value, 2), dtype=np.float32)
+        check(library().wpt_scene_export_environment(self._h, prob.ctypes.data, alias.ctypes.data, n.value,
+                                                      C.byref(n)), "wpt_scene_export_environment")
+        return prob, alias[:, 0].copy(), alias[:, 1].copy().view(np.uint32)
 
     def light_indices(self):
         d = self._desc
