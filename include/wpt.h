@@ -89,10 +89,20 @@ typedef struct wpt_triangle {
     float bitangent[3];
     uint32_t material;
     uint32_t flags;
+    /* Second texture-coordinate set, per-vertex tangents with handedness
+     * signs, and the second set's face tangent frame (Triangle's uv1_*,
+     * vertexTangent*, vertexTangentSign*, tangent1, bitangent1). */
+    float uv1_0[2], uv1_1[2], uv1_2[2];
+    float vertex_tangent0[3], vertex_tangent1[3], vertex_tangent2[3];
+    float vertex_tangent_sign[3];
+    float tangent1[3];
+    float bitangent1[3];
 } wpt_triangle;
 
 #define WPT_TRIANGLE_HAS_NORMALS 1u
 #define WPT_TRIANGLE_HAS_TEXCOORDS 2u
+#define WPT_TRIANGLE_HAS_TEXCOORDS1 4u
+#define WPT_TRIANGLE_HAS_VERTEX_TANGENTS 8u
 
 /* Mesh = a range of wpt_scene_desc.mesh_triangles (reference MeshAsset). */
 typedef struct wpt_mesh {
@@ -135,6 +145,74 @@ typedef struct wpt_material {
     float glossiness;
 } wpt_material;
 
+/* RGBA8 texture (reference ObjTexture, obj_loader.hpp:9-13), rows top to
+ * bottom.  Read as normalized floats with bilinear filtering and wrapping
+ * in both directions; a lookup at (u, v) reads image position (u, 1 - v). */
+typedef struct wpt_texture {
+    uint32_t width, height;
+    const uint8_t* rgba; /* width * height * 4 bytes */
+} wpt_texture;
+
+/* Reference TextureBinding (material.cuh:14-21). */
+typedef struct wpt_texture_binding {
+    uint32_t texture;  /* index into wpt_scene_desc.textures; WPT_INVALID_INDEX: none */
+    uint32_t texcoord; /* 0: first set, 1: second set */
+    float offset[2];
+    float scale[2];
+    float rotation;    /* radians */
+    uint32_t has_transform;
+} wpt_texture_binding;
+
+/* Texture bindings and the glTF workflow fields of one material (the rest
+ * of reference Material, material.cuh:31-70).  Tints multiply the texel:
+ * albedo x albedo texture (sRGB 2.2, or the glTF sRGB curve in the
+ * specular-glossiness workflow; alpha x opacity is the coverage), emission
+ * x emissive texture, metallic x blue and roughness x green of the
+ * metallic-roughness texture, specular x specular-glossiness rgb and
+ * glossiness x its alpha.  The glTF workflow adds the KHR_materials_specular
+ * and KHR_materials_clearcoat lobes. */
+typedef struct wpt_material_ext {
+    wpt_texture_binding albedo_texture;
+    wpt_texture_binding metallic_roughness_texture;
+    wpt_texture_binding emissive_texture;
+    wpt_texture_binding normal_texture;
+    wpt_texture_binding thickness_texture;
+    wpt_texture_binding specular_glossiness_texture;
+    wpt_texture_binding specular_weight_texture;
+    wpt_texture_binding specular_color_texture;
+    wpt_texture_binding clearcoat_texture;
+    wpt_texture_binding clearcoat_roughness_texture;
+    wpt_texture_binding clearcoat_normal_texture;
+    float normal_scale;
+    uint32_t specular_glossiness_workflow;
+    uint32_t gltf_pbr_workflow;
+    float specular_weight;
+    float specular_color[3];
+    float clearcoat;
+    float clearcoat_roughness;
+    float clearcoat_normal_scale;
+    float dispersion;
+} wpt_material_ext;
+
+/* Punctual light (reference PunctualLight, punctual_light.cuh:6-16; glTF
+ * KHR_lights_punctual).  Directional lights shine along `direction`;
+ * positional ones fall off with the inverse square, a smooth range window
+ * (range 0: unlimited) and, for spots, a squared smoothstep between the
+ * cone cosines. */
+#define WPT_LIGHT_POINT 0u
+#define WPT_LIGHT_SPOT 1u
+#define WPT_LIGHT_DIRECTIONAL 2u
+
+typedef struct wpt_punctual_light {
+    uint32_t type;
+    float position[3];
+    float direction[3];  /* direction of emission */
+    float intensity[3];  /* linear RGB x intensity */
+    float range;
+    float inner_cone_cos;
+    float outer_cone_cos;
+} wpt_punctual_light;
+
 typedef struct wpt_scene_desc {
     const wpt_sphere* spheres;
     uint32_t sphere_count;
@@ -162,6 +240,19 @@ typedef struct wpt_scene_desc {
     uint32_t environment_width, environment_height;
     float environment_intensity;
     float environment_rotation_degrees;
+    /* Punctual lights in any order; the upload moves directional lights
+     * first, keeping the order within each kind (scene.cu:349-355). */
+    const wpt_punctual_light* punctual_lights;
+    uint32_t punctual_light_count;
+    /* Textures and per-material texture bindings (NULL: every material
+     * untextured; else material_count entries), and the normal-map switch
+     * and minimum tangent-space cosine (reference Scene.useNormalMaps,
+     * normalMapMinimumCosine; config defaults true and 0). */
+    const wpt_texture* textures;
+    uint32_t texture_count;
+    const wpt_material_ext* material_ext;
+    uint32_t use_normal_maps;
+    float normal_map_minimum_cosine;
 } wpt_scene_desc;
 
 /* Reference Camera (include/scene/camera.cuh:7-27). */

@@ -200,6 +200,9 @@ struct oracle_handle {
     float envIntensity = 1.0f, envRotation = 0.0f;
     std::vector<float> envRgb, envProb;
     std::vector<AliasEntry> envAlias;
+    // Punctual lights: directional ones first (scene.cu:349-355).
+    std::vector<wpt_punctual_light> punctual;
+    uint32_t directional = 0;
 };
 
 namespace {
@@ -290,6 +293,11 @@ void buildEnvironment(oracle_handle& h) {
 
 void buildTables(oracle_handle& h) {
     buildEnvironment(h);
+    for (uint32_t i = 0; i < h.d.punctual_light_count; ++i)
+        if (h.d.punctual_lights[i].type == WPT_LIGHT_DIRECTIONAL) h.punctual.push_back(h.d.punctual_lights[i]);
+    h.directional = static_cast<uint32_t>(h.punctual.size());
+    for (uint32_t i = 0; i < h.d.punctual_light_count; ++i)
+        if (h.d.punctual_lights[i].type != WPT_LIGHT_DIRECTIONAL) h.punctual.push_back(h.d.punctual_lights[i]);
     const wpt_scene_desc& d = h.d;
     // Mesh light numbering, then per-instance world lights (scene_host.cpp:77-106).
     h.meshLight.assign(d.mesh_triangle_count, kInvalid);
@@ -1102,6 +1110,79 @@ void sampleDirectLight(const oracle_handle& h, PathState<R>& path, const Hit<R>&
     addPathRadiance(path, hadamard(path.throughput, direct));
 }
 
+// samplePunctualLight, punctual_light.cuh:24-50: direction toward the
+// light, arriving intensity (inverse square, range window, spot cone) and
+// distance.
+template <class R>
+V3<R> punctualSample(const wpt_punctual_light& L, const V3<R>& p, V3<R>& incident, R& distance) {
+    distance = R(1.0e30f);
+    if (L.type == WPT_LIGHT_DIRECTIONAL) {
+        incident = vec<R>(L.intensity);
+        return -vec<R>(L.direction);
+    }
+    V3<R> offset = vec<R>(L.position) - p;
+    R d2 = lengthSquared(offset);
+    if (!(d2 > R(0))) {
+        incident = V3<R>();
+        return V3<R>();
+    }
+    distance = std::sqrt(d2);
+    V3<R> dir = offset / distance;
+    R att = R(1) / d2;
+    if (L.range > 0.0f) {
+        R ratio = distance / R(L.range);
+        att *= std::fmax(R(0), R(1) - ratio * ratio * ratio * ratio);
+    }
+    if (L.type == WPT_LIGHT_SPOT) {
+        R c = dot(vec<R>(L.direction), -dir);
+        R cone = std::fmin(R(1), std::fmax(R(0), (c - R(L.outer_cone_cos)) /
+            std::fmax(R(1.0e-6f), R(L.inner_cone_cos) - R(L.outer_cone_cos))));
+        att *= cone * cone;
+    }
+    incident = vec<R>(L.intensity) * att;
+    return dir;
+}
+
+// addPunctualLight, device_traversal.cuh:653-674 (no MIS: a delta light).
+template <class R>
+void addPunctual(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const Hit<R>& shaded,
+    const wpt_material& material, const wpt_punctual_light& L, R inverseSelectionPdf, Counters& cnt) {
+    V3<R> incident;
+    R distance;
+    V3<R> dir = punctualSample(L, hit.position, incident, distance);
+    if (dot(shaded.normal, dir) <= R(0) || dot(hit.geometricNormal, dir) <= R(0) || lengthSquared(incident) == R(0))
+        return;
+    Ray<R> shadow{offsetRayOrigin(hit.position, hit.geometricNormal, dir, hit.t), dir};
+    R reach = distance;
+    if (L.type != WPT_LIGHT_DIRECTIONAL) {
+        V3<R> segment = vec<R>(L.position) - shadow.o;
+        reach = length(segment);
+        if (!(reach > R(1.0e-6f))) return;
+        shadow.d = segment / reach;
+    }
+    if (!h.d.ignore_direct_light_occlusion) {
+        ++cnt.shadow;
+        tlWork = &cnt.shadowWork;
+        bool blocked = isOccluded(h, shadow, reach - R(1.0e-6f));
+        tlWork = nullptr;
+        if (blocked) return;
+    }
+    R ignored;
+    V3<R> bsdf = evaluateSurfaceCos(h, material, shaded.normal, -path.ray.d, dir, ignored);
+    addPathRadiance(path, hadamard(path.throughput, hadamard(bsdf, incident) * inverseSelectionPdf));
+}
+
+// sampleDirectPunctualLight, device_traversal.cuh:676-683.
+template <class R>
+void samplePunctual(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const Hit<R>& shaded,
+    const wpt_material& material, Counters& cnt) {
+    for (uint32_t i = 0; i < h.directional; ++i) addPunctual(h, path, hit, shaded, material, h.punctual[i], R(1), cnt);
+    uint32_t n = static_cast<uint32_t>(h.punctual.size()) - h.directional;
+    if (n == 0) return;
+    uint32_t i = h.directional + std::min(static_cast<uint32_t>(randomFloat(path.rng) * n), n - 1);
+    addPunctual(h, path, hit, shaded, material, h.punctual[i], R(static_cast<float>(n)), cnt);
+}
+
 // The five draws of an environment sample (samplePathEnvironment,
 // device_traversal.cuh:420-425): two for a FP64 alias candidate, coin,
 // longitude, latitude.
@@ -1230,6 +1311,7 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
         switch (type) {
         case WPT_MATERIAL_DIFFUSE: {
             sampleDirectLight(h, path, hit, shaded, m, cnt);
+            samplePunctual(h, path, hit, shaded, m, cnt);
             sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
             // continueDiffuseSurface, device_traversal.cuh:685-704
             R mix = envMixture<R>(h);
@@ -1252,12 +1334,14 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
         }
         case WPT_MATERIAL_SPECULAR_GLOSSINESS:
             sampleDirectLight(h, path, hit, shaded, m, cnt);
+            samplePunctual(h, path, hit, shaded, m, cnt);
             sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
             direction = glossy();
             break;
         case WPT_MATERIAL_METAL:
             if (roughness > R(0)) {
                 sampleDirectLight(h, path, hit, shaded, m, cnt);
+                samplePunctual(h, path, hit, shaded, m, cnt);
                 sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
                 direction = glossy();
                 break;
@@ -1783,6 +1867,18 @@ uint32_t oracle_env_tables(const oracle_handle* h, float* probabilities, float* 
         }
     }
     return n;
+}
+
+void oracle_punctual_sample(const wpt_punctual_light* light, const float* point, float* direction, float* incident,
+    float* distance) {
+    V3<float> in;
+    V3<float> d = punctualSample(*light, V3<float>(point[0], point[1], point[2]), in, *distance);
+    direction[0] = d.x;
+    direction[1] = d.y;
+    direction[2] = d.z;
+    incident[0] = in.x;
+    incident[1] = in.y;
+    incident[2] = in.z;
 }
 
 float oracle_env_solid_angle(uint32_t row, uint32_t width, uint32_t height) {

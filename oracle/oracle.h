@@ -51,6 +51,11 @@ int oracle_create(const oracle_scene* scene, oracle_handle** out);
  * per-direction uv, pdf, radiance and texel index (null outputs skipped).
  * oracle_env_sample draws directions from FP64 alias candidates and
  * (coin, longitude, latitude) triples.  Both return 1 without a map. */
+/* samplePunctualLight (FP32): direction toward the light, arriving
+ * intensity and distance (1e30 for directional lights) at `point`. */
+void oracle_punctual_sample(const wpt_punctual_light* light, const float* point, float* direction, float* incident,
+    float* distance);
+
 uint32_t oracle_env_tables(const oracle_handle* h, float* probabilities, float* alias2, uint32_t capacity);
 float oracle_env_solid_angle(uint32_t row, uint32_t width, uint32_t height);
 int oracle_env_eval(const oracle_handle* h, const float* dirs, uint32_t n, float* uv, float* pdf, float* rgb,

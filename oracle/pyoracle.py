@@ -65,6 +65,7 @@ def library():
              C.c_int),
             ("oracle_diff_directional64", [P, C.POINTER(wpt.Camera), C.POINTER(OracleDiffConfig), P, P, u32, u32, P],
              C.c_int),
+            ("oracle_punctual_sample", [C.POINTER(wpt.PunctualLight), P, P, P, P], None),
             ("oracle_env_tables", [P, P, P, u32], u32),
             ("oracle_env_solid_angle", [u32, u32, u32], C.c_float),
             ("oracle_env_eval", [P, P, u32, P, P, P, P], C.c_int),
@@ -101,6 +102,16 @@ def elementary(op: int, x: np.ndarray, y: np.ndarray = None) -> np.ndarray:
     out = np.zeros_like(x)
     library().oracle_elementary(op, x.ctypes.data, y.ctypes.data, x.size, out.ctypes.data)
     return out
+
+
+def punctual_sample(light, point):
+    """(direction, incident, distance) of samplePunctualLight at a point."""
+    p = np.asarray(point, np.float32)
+    d = np.zeros(3, np.float32)
+    inc = np.zeros(3, np.float32)
+    dist = C.c_float()
+    library().oracle_punctual_sample(C.byref(light), p.ctypes.data, d.ctypes.data, inc.ctypes.data, C.byref(dist))
+    return d, inc, dist.value
 
 
 def dielectric_reflectance(cosine: float, eta: float) -> float:

@@ -1,9 +1,9 @@
-"""GPU parity of environment-map rendering: images of scenes lit by an
-equirectangular map (with and without triangle lights, every material
-kind, rotated and scaled maps) from all three schedulers and the batch
-consumer, against the oracle's FP32 restatement bit for bit; and the
-reference's HDR-sun energy checks (tests/environment_shading_tests.cu) on
-the device."""
+"""GPU parity of environment-map and punctual-light rendering: images of
+scenes lit by an equirectangular map and by point, spot and directional
+lights (with and without triangle lights, every material kind, rotated and
+scaled maps) from all three schedulers and the batch consumer, against the
+oracle's FP32 restatement bit for bit; and the reference's HDR-sun energy
+checks (tests/environment_shading_tests.cu) on the device."""
 import math
 
 import numpy as np
@@ -26,16 +26,22 @@ def random_map(width=32, height=16, seed=3):
 
 
 class PresetWithMap:
-    """A host preset's scene description with an environment map attached."""
+    """A host preset's scene description with an environment map (None: no
+    map) and punctual lights attached."""
 
-    def __init__(self, preset, env, intensity, rotation_degrees, freq=0):
+    def __init__(self, preset, env, intensity, rotation_degrees, freq=0, lights=()):
         self.host = wpt.HostScene(preset, icosphere_frequency=freq)
-        self.env = np.ascontiguousarray(env, np.float32)
         d = wpt.SceneDesc.from_buffer_copy(self.host.desc())
-        d.environment_rgb = self.env.ctypes.data_as(wpt.C.POINTER(wpt.C.c_float))
-        d.environment_height, d.environment_width = self.env.shape[0], self.env.shape[1]
-        d.environment_intensity = intensity
-        d.environment_rotation_degrees = rotation_degrees
+        if env is not None:
+            self.env = np.ascontiguousarray(env, np.float32)
+            d.environment_rgb = self.env.ctypes.data_as(wpt.C.POINTER(wpt.C.c_float))
+            d.environment_height, d.environment_width = self.env.shape[0], self.env.shape[1]
+            d.environment_intensity = intensity
+            d.environment_rotation_degrees = rotation_degrees
+        if lights:
+            self.lights = (wpt.PunctualLight * len(lights))(*lights)
+            d.punctual_lights = wpt.C.cast(self.lights, wpt.C.POINTER(wpt.PunctualLight))
+            d.punctual_light_count = len(lights)
         self._desc = d
 
     def desc(self):
@@ -64,6 +70,29 @@ def test_preset_with_map_matches_oracle(torch_cuda, oracle_mod, mode, preset, fr
     st = r.stats()
     assert st.closest_queries == ost.closest_queries
     assert st.shadow_queries == ost.shadow_queries
+
+
+def mixed_lights(center=(0.5, 0.9, 0.5)):
+    c = np.asarray(center, np.float32)
+    return [wpt.make_punctual_light(wpt.LIGHT_POINT, c + (0.2, 0, 0.1), intensity=(0.6, 0.5, 0.4)),
+            wpt.make_punctual_light(wpt.LIGHT_DIRECTIONAL, direction=(0.3, -1, 0.2), intensity=(0.8, 0.8, 0.7)),
+            wpt.make_punctual_light(wpt.LIGHT_SPOT, c + (-0.2, 0, 0), direction=(0, -1, 0), intensity=(1.5, 1, 1),
+                                    range_=3.0, inner_cone_cos=0.95, outer_cone_cos=0.8),
+            wpt.make_punctual_light(wpt.LIGHT_DIRECTIONAL, direction=(-0.5, -1, -0.1), intensity=(0.2, 0.3, 0.5))]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("with_map", [False, True])
+def test_punctual_lights_match_oracle(torch_cuda, oracle_mod, mode, with_map):
+    src = PresetWithMap("cornell-box", random_map() if with_map else None, 0.5, 45.0, lights=mixed_lights())
+    scene = wpt.Scene(src.desc(), device=True, owner=src)
+    cam = src.camera(96, 64)
+    cfg = wpt.render_config(96, 64, spp=3, max_depth=8, rr_start=3, mode=mode, clamp=10.0)
+    r = wpt.Renderer(scene, cam, cfg)
+    img, _ = r.render_frame()
+    want, ost = oracle_mod.Oracle(scene).render(cam, cfg)
+    assert np.array_equal(img, want), f"max |diff| {np.abs(img - want).max()}"
+    assert r.stats().shadow_queries == ost.shadow_queries
 
 
 @pytest.mark.parametrize("mode", MODES)

@@ -319,6 +319,9 @@ static int createScene(const wpt_scene_desc* desc, wpt_scene** out, bool upload)
             d.environment.intensity = h.envIntensity;
             d.environment.rotation = h.envRotation;
         }
+        check(a.upload(h.punctual, &d.punctual), "punctual lights");
+        d.punctualCount = static_cast<uint32_t>(h.punctual.size());
+        d.directionalCount = h.directionalCount;
         if (status != WPT_OK) {
             delete s;
             return status;

@@ -444,6 +444,83 @@ WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceH
     request.valid = 1;
 }
 
+// ---- punctual lights (punctual_light.cuh:24-50, device_traversal.cuh:653-683)
+struct PunctualSample {
+    Vec3 direction;  // toward the light
+    Vec3 incident;   // radiant intensity arriving at the point
+    float distance;
+};
+
+WPT_HD PunctualSample samplePunctualLight(const PunctualLight& light, const Vec3& point) {
+    PunctualSample r;
+    r.distance = 1.0e30f;
+    if (light.type == kLightDirectional) {
+        r.direction = -light.direction;
+        r.incident = light.intensity;
+        return r;
+    }
+    Vec3 offset = light.position - point;
+    float distanceSquared = lengthSquared(offset);
+    if (!(distanceSquared > 0.0f)) {
+        r.direction = Vec3();
+        r.incident = Vec3();
+        return r;
+    }
+    r.distance = sqrtf(distanceSquared);
+    r.direction = offset / r.distance;
+    float attenuation = 1.0f / distanceSquared;
+    if (light.range > 0.0f) {
+        float ratio = r.distance / light.range;
+        attenuation *= fmaxf(0.0f, 1.0f - ratio * ratio * ratio * ratio);
+    }
+    if (light.type == kLightSpot) {
+        float cosine = dot(light.direction, -r.direction);
+        float cone = fminf(1.0f, fmaxf(0.0f, (cosine - light.outerConeCos) /
+            fmaxf(1.0e-6f, light.innerConeCos - light.outerConeCos)));
+        attenuation *= cone * cone;
+    }
+    r.incident = light.intensity * attenuation;
+    return r;
+}
+
+// One punctual connection: a delta light, so no MIS weight.
+template <class Occluder>
+WPT_HD void addPunctualLight(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
+    const Material& material, const ShadeParams& sp, Occluder& occ, const PunctualLight& source,
+    float inverseSelectionPdf) {
+    PunctualSample light = samplePunctualLight(source, hit.position);
+    if (dot(shadingNormal, light.direction) <= 0.0f || dot(hit.geometricNormal, light.direction) <= 0.0f ||
+        lengthSquared(light.incident) == 0.0f)
+        return;
+    Ray shadow(offsetRayOrigin(hit.position, hit.geometricNormal, light.direction, hit.t), light.direction);
+    float shadowDistance = light.distance;
+    if (source.type != kLightDirectional) {
+        Vec3 segment = source.position - shadow.origin;
+        shadowDistance = length(segment);
+        if (!(shadowDistance > kRayMinDistance)) return;
+        shadow.direction = segment / shadowDistance;
+    }
+    if (!s.ignoreDirectLightOcclusion && occ.blocked(s, shadow, shadowDistance - kRayMinDistance)) return;
+    float ignoredPdf;
+    Vec3 bsdf = evaluateSurfaceCos(s, material, shadingNormal, -path.ray.direction, light.direction, ignoredPdf);
+    Vec3 direct = hadamard(bsdf, light.incident) * inverseSelectionPdf;
+    addPathRadiance(path, hadamard(path.throughput, direct), sp.clamp, false);
+}
+
+// Every directional light, then one positional light chosen uniformly (one
+// draw when positional lights exist).
+template <class Occluder>
+WPT_HD void sampleDirectPunctualLight(const DevScene& s, PathState& path, const SurfaceHit& hit,
+    const Vec3& shadingNormal, const Material& material, const ShadeParams& sp, Occluder& occ) {
+    for (uint32_t i = 0; i < s.directionalCount; ++i)
+        addPunctualLight(s, path, hit, shadingNormal, material, sp, occ, s.punctual[i], 1.0f);
+    uint32_t count = s.punctualCount - s.directionalCount;
+    if (count == 0) return;
+    uint32_t pick = static_cast<uint32_t>(rand01(path, sp.rngStrategy) * static_cast<float>(count));
+    uint32_t index = s.directionalCount + (pick < count - 1 ? pick : count - 1);
+    addPunctualLight(s, path, hit, shadingNormal, material, sp, occ, s.punctual[index], static_cast<float>(count));
+}
+
 // Next-event estimation toward the environment (device_traversal.cuh:637-651).
 template <class Occluder>
 WPT_HD void sampleDirectEnvironment(const DevScene& s, PathState& path, const SurfaceHit& hit,
@@ -633,18 +710,21 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
     switch (type) {
     case kDiffuse:
         sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
+        sampleDirectPunctualLight(s, path, hit, shadingNormal, material, sp, occ);
         sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
         direction = continueDiffuse(s, path, hit, shadingNormal, material, sp.rngStrategy);
         break;
     case kSpecularGlossiness:
         sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
+        sampleDirectPunctualLight(s, path, hit, shadingNormal, material, sp, occ);
         sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
         direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
         break;
     case kMetal:
         if (roughness > 0.0f) {
             sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
-            sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
+            sampleDirectPunctualLight(s, path, hit, shadingNormal, material, sp, occ);
+        sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
             direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
             break;
         }

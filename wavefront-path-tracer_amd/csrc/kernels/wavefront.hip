@@ -242,9 +242,9 @@ __device__ __forceinline__ void storeShadow(const ShadowQueue& q, uint32_t s, co
 // Staged shading, one path per thread: continuing paths are appended to the
 // workgroup's segment of the next queue, shadow requests to its segment of
 // the shadow queue; finished paths without a pending shadow ray reach the
-// framebuffer.  With `Inline` (scenes with an environment map, whose shading
-// traces shadow rays between its radiance additions) every shadow ray is
-// traced here and the shadow queue stays empty.
+// framebuffer.  With `Inline` (scenes with an environment map or punctual
+// lights, whose shading adds radiance after the area-light NEE) every
+// shadow ray is traced here and the shadow queue stays empty.
 template <bool Inline>
 __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
     __shared__ AppendShared sh;
@@ -476,7 +476,7 @@ template <class Timer>
 hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& timer) {
     size_t lds = stackLdsBytes(ctx.blockSize);
     dim3 grid(ctx.gridBlocks), block(ctx.blockSize);
-    const bool inlineShadows = ctx.scene.environment.texels != nullptr;
+    const bool inlineShadows = shadesWithInlineShadows(ctx.scene);
     hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, grid, block, 0, stream, ctx); });
     for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {
         uint32_t sel = bounce & 1u;

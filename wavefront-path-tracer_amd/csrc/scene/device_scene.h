@@ -56,12 +56,24 @@ struct TriShade {
     Vec3 tangent, bitangent;
     uint32_t material;
     uint32_t lightIndex;
-    uint32_t flags;  // bit0 has vertex normals, bit1 has texcoords
-    uint32_t pad;
+    uint32_t flags;  // kTriHas*
+    uint32_t extra;  // index into DevScene.triExtra, kInvalid: none
 };
 
 constexpr uint32_t kTriHasNormals = 1u;
 constexpr uint32_t kTriHasTexcoords = 2u;
+constexpr uint32_t kTriHasTexcoords1 = 4u;
+constexpr uint32_t kTriHasVertexTangents = 8u;
+
+// Second texture-coordinate set and tangent data of a triangle (reference
+// Triangle uv1_*, vertexTangent*, vertexTangentSign*, tangent1, bitangent1),
+// kept out of TriShade because only textured triangles carry it.
+struct TriExtra {
+    Vec2 uv1_0, uv1_1, uv1_2;
+    Vec3 vertexTangent0, vertexTangent1, vertexTangent2;
+    float sign0, sign1, sign2;
+    Vec3 tangent1, bitangent1;
+};
 
 struct InstanceTransform {
     Vec3 localToWorldX, localToWorldY, localToWorldZ;
@@ -112,6 +124,38 @@ struct Material {
     float volumeDensity;
     Vec3 specular;
     float glossiness;
+    uint32_t ext;  // index into DevScene.materialExt, kInvalid: untextured
+};
+
+// Reference TextureBinding (material.cuh:14-21).
+struct TexBinding {
+    uint32_t texture;  // kInvalid: none
+    uint32_t texCoord;
+    Vec2 offset;
+    Vec2 scale;
+    float rotation;
+    uint32_t hasTransform;
+};
+
+// Texture bindings and glTF workflow fields (the rest of reference Material).
+struct MaterialExt {
+    TexBinding albedo, metallicRoughness, emissive, normal, thickness, specularGlossiness, specularWeight,
+        specularColor, clearcoat, clearcoatRoughness, clearcoatNormal;
+    float normalScale;
+    uint32_t specularGlossinessWorkflow;
+    uint32_t gltfPbrWorkflow;
+    float specularWeightFactor;
+    Vec3 specularColorFactor;
+    float clearcoatFactor;
+    float clearcoatRoughnessFactor;
+    float clearcoatNormalScale;
+    float dispersion;
+};
+
+// RGBA8 texels packed r | g << 8 | b << 16 | a << 24, rows top to bottom.
+struct DevTexture {
+    const uint32_t* texels;
+    uint32_t width, height;
 };
 
 enum MaterialType : uint32_t {
@@ -136,7 +180,32 @@ struct LightAlias {
     uint32_t alias;
 };
 
-// Passed by value to every kernel (reference Scene, scene.cuh:117-161).
+// Alpha-mask policy of a material in traversal (reference
+// intersectTriangleHit, device_traversal.cuh:148-151): opaque, never hit
+// (masked, untextured, opacity below the cutoff), or tested per hit against
+// opacity x albedo-texture alpha.
+constexpr uint8_t kCutNever = 0;
+constexpr uint8_t kCutAlways = 1;
+constexpr uint8_t kCutTexture = 2;
+
+// Reference PunctualLightType and PunctualLight (punctual_light.cuh:6-16);
+// the values are the C ABI's WPT_LIGHT_*.
+enum PunctualLightType : uint32_t {
+    kLightPoint = 0,
+    kLightSpot = 1,
+    kLightDirectional = 2,
+};
+
+struct PunctualLight {
+    uint32_t type;  // PunctualLightType
+    Vec3 position;
+    Vec3 direction;  // of emission
+    Vec3 intensity;
+    float range;     // 0: unlimited
+    float innerConeCos;
+    float outerConeCos;
+};
+
 struct EnvAlias {
     float probability;
     uint32_t alias;
@@ -152,6 +221,7 @@ struct EnvironmentMap {
     float rotation;               // radians about +Y
 };
 
+// Passed by value to every kernel (reference Scene, scene.cuh:117-161).
 struct DevScene {
     const float4* spheres;  // center xyz, radius
     const uint32_t* sphereMaterial;
@@ -174,7 +244,7 @@ struct DevScene {
     uint32_t flatCount;
     uint32_t materialCount;
     const Material* materials;
-    const uint8_t* materialCutout;  // alpha-masked and below cutoff: never hit
+    const uint8_t* materialCutout;  // kCut*: alpha-mask policy per material
     const TriangleLight* lights;
     const LightAlias* lightAlias;
     uint32_t lightCount;
@@ -182,7 +252,23 @@ struct DevScene {
     uint32_t ignoreDirectLightOcclusion;
     uint32_t pad;
     EnvironmentMap environment;
+    const PunctualLight* punctual;  // directional lights first
+    uint32_t punctualCount;
+    uint32_t directionalCount;
+    const MaterialExt* materialExt;
+    const DevTexture* textures;
+    const TriExtra* triExtra;
+    uint32_t textureCount;
+    uint32_t useNormalMaps;
+    float normalMapMinimumCosine;
+    uint32_t pad2;
 };
+
+// Shading traces its shadow rays where it makes them (rather than deferring
+// the area-light ray to a later pass) when radiance is added after the
+// area-light NEE within a bounce: environment and punctual NEE, and the
+// dielectric environment splits.
+WPT_HD bool shadesWithInlineShadows(const DevScene& s) { return s.environment.texels || s.punctualCount; }
 
 constexpr uint32_t kTlasSphereBit = 0x80000000u;
 

@@ -208,6 +208,7 @@ TriShade toShade(const wpt_triangle& t) {
     s.material = t.material;
     s.lightIndex = kInvalid;
     s.flags = t.flags & (kTriHasNormals | kTriHasTexcoords);
+    s.extra = kInvalid;
     return s;
 }
 
@@ -249,6 +250,7 @@ Material toMaterial(const wpt_material& m) {
     r.volumeDensity = m.volume_density;
     r.specular = toVec(m.specular);
     r.glossiness = m.glossiness;
+    r.ext = kInvalid;
     return r;
 }
 
@@ -458,6 +460,32 @@ std::string buildEnvironment(const wpt_scene_desc& desc, HostSceneData& out) {
     return "";
 }
 
+// Punctual lights, directional ones first in their given order, then the
+// positional ones (the stable partition of scene.cu:349-355): NEE visits
+// every directional light and one positional light chosen uniformly.
+static_assert(WPT_LIGHT_POINT == kLightPoint && WPT_LIGHT_SPOT == kLightSpot &&
+    WPT_LIGHT_DIRECTIONAL == kLightDirectional, "light types match the C ABI");
+
+std::string buildPunctualLights(const wpt_scene_desc& desc, HostSceneData& out) {
+    if (desc.punctual_light_count && !desc.punctual_lights) return "punctual light array is null";
+    auto finite3 = [](const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); };
+    for (int pass = 0; pass < 2; ++pass) {
+        for (uint32_t i = 0; i < desc.punctual_light_count; ++i) {
+            const wpt_punctual_light& l = desc.punctual_lights[i];
+            if (l.type > WPT_LIGHT_DIRECTIONAL) return "punctual light has an unknown type";
+            if (!finite3(l.position) || !finite3(l.direction) || !finite3(l.intensity) || !std::isfinite(l.range) ||
+                l.range < 0.0f || !std::isfinite(l.inner_cone_cos) || !std::isfinite(l.outer_cone_cos))
+                return "punctual light has a non-finite or negative parameter";
+            if ((l.type == WPT_LIGHT_DIRECTIONAL) != (pass == 0)) continue;
+            out.punctual.push_back(PunctualLight{l.type, Vec3(l.position[0], l.position[1], l.position[2]),
+                Vec3(l.direction[0], l.direction[1], l.direction[2]), Vec3(l.intensity[0], l.intensity[1], l.intensity[2]),
+                l.range, l.inner_cone_cos, l.outer_cone_cos});
+        }
+        if (pass == 0) out.directionalCount = static_cast<uint32_t>(out.punctual.size());
+    }
+    return "";
+}
+
 std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsigned threads) {
     if ((desc.sphere_count && !desc.spheres) || (desc.static_triangle_count && !desc.static_triangles) ||
         (desc.flat_triangle_count && !desc.flat_triangles) || (desc.mesh_triangle_count && !desc.mesh_triangles) ||
@@ -472,6 +500,8 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
     out.ignoreDirectLightOcclusion = desc.ignore_direct_light_occlusion ? 1u : 0u;
     std::string envError = buildEnvironment(desc, out);
     if (!envError.empty()) return envError;
+    std::string lightError = buildPunctualLights(desc, out);
+    if (!lightError.empty()) return lightError;
     for (uint32_t i = 0; i < materialCount; ++i) {
         const wpt_material& m = desc.materials[i];
         if (m.type > WPT_MATERIAL_SPECULAR_GLOSSINESS) return "material has an unknown type";

@@ -75,6 +75,8 @@ struct HostSceneData {
     std::vector<EnvAlias> envAlias;
     uint32_t envWidth = 0, envHeight = 0;
     float envIntensity = 1.0f, envRotation = 0.0f;
+    std::vector<PunctualLight> punctual;  // directional lights first
+    uint32_t directionalCount = 0;
     uint32_t maxStackDepth = 0;  // worst-case traversal stack use
 };
 

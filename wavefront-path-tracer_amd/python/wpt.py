@@ -37,7 +37,9 @@ class Sphere(C.Structure):
 class Triangle(C.Structure):
     _fields_ = [("v0", F3), ("v1", F3), ("v2", F3), ("n0", F3), ("n1", F3), ("n2", F3),
                 ("uv0", F2), ("uv1", F2), ("uv2", F2), ("tangent", F3), ("bitangent", F3),
-                ("material", C.c_uint32), ("flags", C.c_uint32)]
+                ("material", C.c_uint32), ("flags", C.c_uint32),
+                ("uv1_0", F2), ("uv1_1", F2), ("uv1_2", F2), ("vertex_tangent0", F3), ("vertex_tangent1", F3),
+                ("vertex_tangent2", F3), ("vertex_tangent_sign", F3), ("tangent1", F3), ("bitangent1", F3)]
 
 
 class Mesh(C.Structure):
@@ -61,6 +63,43 @@ class Material(C.Structure):
                 ("glossiness", C.c_float)]
 
 
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.c_void_p)]
+
+
+class TextureBinding(C.Structure):
+    _fields_ = [("texture", C.c_uint32), ("texcoord", C.c_uint32), ("offset", F2), ("scale", F2),
+                ("rotation", C.c_float), ("has_transform", C.c_uint32)]
+
+
+TEXTURE_SLOTS = ("albedo_texture", "metallic_roughness_texture", "emissive_texture", "normal_texture",
+                 "thickness_texture", "specular_glossiness_texture", "specular_weight_texture",
+                 "specular_color_texture", "clearcoat_texture", "clearcoat_roughness_texture",
+                 "clearcoat_normal_texture")
+
+
+class MaterialExt(C.Structure):
+    _fields_ = [(name, TextureBinding) for name in TEXTURE_SLOTS] + [
+        ("normal_scale", C.c_float), ("specular_glossiness_workflow", C.c_uint32),
+        ("gltf_pbr_workflow", C.c_uint32), ("specular_weight", C.c_float), ("specular_color", F3),
+        ("clearcoat", C.c_float), ("clearcoat_roughness", C.c_float), ("clearcoat_normal_scale", C.c_float),
+        ("dispersion", C.c_float)]
+
+
+LIGHT_POINT, LIGHT_SPOT, LIGHT_DIRECTIONAL = 0, 1, 2
+
+
+class PunctualLight(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("position", F3), ("direction", F3), ("intensity", F3), ("range", C.c_float),
+                ("inner_cone_cos", C.c_float), ("outer_cone_cos", C.c_float)]
+
+
+def make_punctual_light(kind=LIGHT_POINT, position=(0, 0, 0), direction=(0, 0, -1), intensity=(1, 1, 1), range_=0.0,
+                        inner_cone_cos=1.0, outer_cone_cos=0.70710678118) -> PunctualLight:
+    return PunctualLight(kind, vec3(position), vec3(direction), vec3(intensity), range_, inner_cone_cos,
+                         outer_cone_cos)
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("spheres", C.POINTER(Sphere)), ("sphere_count", C.c_uint32),
                 ("static_triangles", C.POINTER(Triangle)), ("static_triangle_count", C.c_uint32),
@@ -72,7 +111,11 @@ class SceneDesc(C.Structure):
                 ("black_background", C.c_uint32), ("ignore_direct_light_occlusion", C.c_uint32),
                 ("environment_rgb", C.POINTER(C.c_float)), ("environment_width", C.c_uint32),
                 ("environment_height", C.c_uint32), ("environment_intensity", C.c_float),
-                ("environment_rotation_degrees", C.c_float)]
+                ("environment_rotation_degrees", C.c_float),
+                ("punctual_lights", C.POINTER(PunctualLight)), ("punctual_light_count", C.c_uint32),
+                ("textures", C.POINTER(Texture)), ("texture_count", C.c_uint32),
+                ("material_ext", C.POINTER(MaterialExt)), ("use_normal_maps", C.c_uint32),
+                ("normal_map_minimum_cosine", C.c_float)]
 
 
 class Camera(C.Structure):
@@ -293,7 +336,12 @@ class SceneBuilder:
         self.environment_rgb = None  # float32 (height, width, 3), row 0 at +Y
         self.environment_intensity = 1.0
         self.environment_rotation_degrees = 0.0
+        self.punctual_lights: List[PunctualLight] = []
         self._keep = []
+
+    def punctual_light(self, light: PunctualLight) -> int:
+        self.punctual_lights.append(light)
+        return len(self.punctual_lights) - 1
 
     def environment(self, rgb, intensity=1.0, rotation_degrees=0.0):
         """Equirectangular HDR map: float (height, width, 3) linear radiance."""
@@ -339,6 +387,7 @@ class SceneBuilder:
         d.materials, d.material_count = self._array(Material, self.materials)
         d.black_background = 1 if self.black_background else 0
         d.ignore_direct_light_occlusion = 1 if self.ignore_direct_light_occlusion else 0
+        d.punctual_lights, d.punctual_light_count = self._array(PunctualLight, self.punctual_lights)
         if self.environment_rgb is not None:
             env = self.environment_rgb
             self._keep.append(env)
