@@ -281,6 +281,23 @@ typedef struct wpt_render_config {
     uint32_t intersection_debug;
     uint32_t shading_normal_debug;
     uint32_t profile_queues;
+    /* Photon-mapped caustics, built once when the renderer is created
+     * (reference RenderDriver::buildPhotonMap, render_driver.cu:180-234, and
+     * config.hpp:18-22).  0 photons: off.  The photons start on the
+     * triangle lights and are kept where they reach a diffuse surface after
+     * at least one mirror or dielectric bounce; diffuse vertices then add
+     * albedo x the gathered irradiance (caustic_gather).  Spectral photons
+     * carry one RGB channel each, for dispersive dielectrics.  A zero grid
+     * box or resolution means the reference's (-2.5,-1,-2.5)..(2.5,3,2.5)
+     * and 64. */
+    uint32_t caustic_photons;
+    uint32_t caustic_max_depth;      /* 0: 8 */
+    float caustic_gather_radius;     /* 0: 0.06 */
+    uint32_t caustic_gather;
+    uint32_t caustic_spectral;
+    uint32_t caustic_grid_resolution;
+    float caustic_grid_min[3];
+    float caustic_grid_max[3];
 } wpt_render_config;
 
 /* Reference RenderTask / RenderTaskId / DistributedTile
@@ -368,13 +385,14 @@ WPT_API int wpt_scene_export_bvh2(const wpt_scene* scene, uint32_t bvh, wpt_bvh2
  * alias table as (probability, alias bits) pairs. */
 WPT_API int wpt_scene_export_lights(const wpt_scene* scene, float* lights16, float* alias2, uint32_t capacity,
                                     uint32_t* count);
-/* Light offset per instance (reference MeshInstance.lightOffset) and light
- * index per static/mesh triangle as assigned by the scene build. */
-/* Environment sampling tables (probabilities: width*height floats, alias:
- * width*height x (probability, index) as 2 words); returns WPT_OK with
- * *count = 0 when the scene has no map. */
+/* Environment sampling tables (reference HostEnvironment probabilities and
+ * alias, environment.cpp:10-64): width*height floats, and width*height
+ * (probability, index bits) pairs; returns WPT_OK with *count = 0 when the
+ * scene has no map. */
 WPT_API int wpt_scene_export_environment(const wpt_scene* scene, float* probabilities, float* alias2,
                                          uint32_t capacity, uint32_t* count);
+/* Light offset per instance (reference MeshInstance.lightOffset) and light
+ * index per static/mesh triangle as assigned by the scene build. */
 WPT_API int wpt_scene_export_light_indices(const wpt_scene* scene, uint32_t* instance_light_offset,
                                            uint32_t* static_light_index, uint32_t* mesh_light_index);
 

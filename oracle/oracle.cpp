@@ -203,6 +203,11 @@ struct oracle_handle {
     // Punctual lights: directional ones first (scene.cu:349-355).
     std::vector<wpt_punctual_light> punctual;
     uint32_t directional = 0;
+    // Textures (RGBA8 rows) and every material's texture bindings (the
+    // reference's defaults where the scene gives none).
+    std::vector<std::vector<uint8_t>> tex;
+    std::vector<uint32_t> texW, texH;
+    std::vector<wpt_material_ext> ext;
 };
 
 namespace {
@@ -291,8 +296,37 @@ void buildEnvironment(oracle_handle& h) {
     for (uint32_t i : large) h.envAlias[i] = AliasEntry{1.0f, i};
 }
 
+// Reference TextureBinding / Material defaults (material.cuh:14-70).
+wpt_material_ext defaultExt() {
+    wpt_material_ext e{};
+    wpt_texture_binding* all[] = {&e.albedo_texture, &e.metallic_roughness_texture, &e.emissive_texture,
+        &e.normal_texture, &e.thickness_texture, &e.specular_glossiness_texture, &e.specular_weight_texture,
+        &e.specular_color_texture, &e.clearcoat_texture, &e.clearcoat_roughness_texture, &e.clearcoat_normal_texture};
+    for (wpt_texture_binding* b : all) {
+        b->texture = kInvalid;
+        b->scale[0] = b->scale[1] = 1.0f;
+    }
+    e.normal_scale = 1.0f;
+    e.specular_weight = 1.0f;
+    e.specular_color[0] = e.specular_color[1] = e.specular_color[2] = 1.0f;
+    e.clearcoat_normal_scale = 1.0f;
+    return e;
+}
+
+void buildTextures(oracle_handle& h) {
+    const wpt_scene_desc& d = h.d;
+    for (uint32_t i = 0; i < d.texture_count; ++i) {
+        const wpt_texture& t = d.textures[i];
+        h.tex.emplace_back(t.rgba, t.rgba + 4ull * t.width * t.height);
+        h.texW.push_back(t.width);
+        h.texH.push_back(t.height);
+    }
+    for (uint32_t m = 0; m < d.material_count; ++m) h.ext.push_back(d.material_ext ? d.material_ext[m] : defaultExt());
+}
+
 void buildTables(oracle_handle& h) {
     buildEnvironment(h);
+    buildTextures(h);
     for (uint32_t i = 0; i < h.d.punctual_light_count; ++i)
         if (h.d.punctual_lights[i].type == WPT_LIGHT_DIRECTIONAL) h.punctual.push_back(h.d.punctual_lights[i]);
     h.directional = static_cast<uint32_t>(h.punctual.size());
@@ -455,7 +489,8 @@ struct HitState {
     R closestT = R(1e30f);
     int closestMaterial = -1;
     V3<R> geometricNormal, normal;
-    V2<R> uv{R(0), R(0)};
+    V2<R> uv{R(0), R(0)}, uv1{R(0), R(0)};
+    V3<R> tangent, bitangent, tangent1, bitangent1;
     uint32_t lightIndex = kInvalid;
     uint32_t sphereIndex = kInvalid;
     // identity of the current winner
@@ -486,9 +521,20 @@ V2<R> triUv(const wpt_triangle& tri, R u, R v) {  // :120-132
     return V2<R>{w * R(tri.uv0[0]) + u * R(tri.uv1[0]) + v * R(tri.uv2[0]), w * R(tri.uv0[1]) + u * R(tri.uv1[1]) + v * R(tri.uv2[1])};
 }
 
-bool cutout(const oracle_handle& h, uint32_t m) {  // alpha-masked and (untextured) opacity below cutoff
+template <class R>
+V2<R> triUv1(const wpt_triangle& tri, R u, R v) {  // :120-132, second set
+    if (!(tri.flags & WPT_TRIANGLE_HAS_TEXCOORDS1)) return V2<R>{R(0), R(0)};
+    R w = R(1) - u - v;
+    return V2<R>{w * R(tri.uv1_0[0]) + u * R(tri.uv1_1[0]) + v * R(tri.uv1_2[0]),
+        w * R(tri.uv1_0[1]) + u * R(tri.uv1_1[1]) + v * R(tri.uv1_2[1])};
+}
+
+// Alpha mask at a candidate hit (:148-151): masked materials whose opacity
+// (x albedo-texture alpha) is below the cutoff are passed through.
+template <class R>
+bool cutout(const oracle_handle& h, uint32_t m, const wpt_triangle& tri, R u, R v) {
     const wpt_material& x = mat(h, m);
-    return x.alpha_masked && x.opacity < x.alpha_cutoff;
+    return x.alpha_masked && matOpacity(h, m, triUv(tri, u, v), triUv1(tri, u, v)) < R(x.alpha_cutoff);
 }
 
 // intersectTriangleHit, device_traversal.cuh:137-180.
@@ -499,13 +545,29 @@ bool triangleHit(const oracle_handle& h, const Ray<R>& ray, const wpt_triangle& 
     if (!intersectTriangle(ray, tri, t, u, v) || t >= hs.closestT) return false;
     V2<R> uv = triUv(tri, u, v);
     uint32_t m = materialOverride == kInvalid ? tri.material : materialOverride;
-    if (cutout(h, m)) return false;
+    if (cutout(h, m, tri, u, v)) return false;
     bool ff = mat(h, m).type != WPT_MATERIAL_DIELECTRIC;
     hs.closestT = t;
     hs.closestMaterial = static_cast<int>(m);
     hs.geometricNormal = triGeometricNormal(tri, ray, ff);
     hs.normal = triNormal(tri, ray, u, v, ff);
     hs.uv = uv;
+    hs.uv1 = triUv1(tri, u, v);
+    hs.tangent = vec<R>(tri.tangent);
+    hs.bitangent = vec<R>(tri.bitangent);
+    if (tri.flags & WPT_TRIANGLE_HAS_VERTEX_TANGENTS) {  // :159-173
+        R w = R(1) - u - v;
+        V3<R> tg = w * vec<R>(tri.vertex_tangent0) + u * vec<R>(tri.vertex_tangent1) + v * vec<R>(tri.vertex_tangent2);
+        tg = tg - hs.normal * dot(hs.normal, tg);
+        if (lengthSquared(tg) > R(0.00000001f)) {
+            hs.tangent = normalize(tg);
+            R sign = w * R(tri.vertex_tangent_sign[0]) + u * R(tri.vertex_tangent_sign[1]) +
+                v * R(tri.vertex_tangent_sign[2]);
+            hs.bitangent = (sign < R(0) ? R(-1) : R(1)) * normalize(cross(hs.normal, hs.tangent));
+        }
+    }
+    hs.tangent1 = vec<R>(tri.tangent1);
+    hs.bitangent1 = vec<R>(tri.bitangent1);
     hs.lightIndex = materialOverride == kInvalid ? triLight : kInvalid;
     hs.kind = kind;
     hs.instance = instance;
@@ -568,6 +630,11 @@ V3<R> inverseXform(const wpt_transform& t, const V3<R>& v) {  // scene.cuh:43-46
 }
 
 template <class R>
+V3<R> xformVector(const wpt_transform& t, const V3<R>& v) {  // scene.cuh:38-41
+    return vec<R>(t.local_to_world[0]) * v.x + vec<R>(t.local_to_world[1]) * v.y + vec<R>(t.local_to_world[2]) * v.z;
+}
+
+template <class R>
 V3<R> xformNormal(const wpt_transform& t, const V3<R>& n) {  // scene.cuh:48-55
     return normalize(V3<R>(dot(vec<R>(t.world_to_local[0]), n), dot(vec<R>(t.world_to_local[1]), n),
         dot(vec<R>(t.world_to_local[2]), n)));
@@ -601,6 +668,10 @@ void meshInstance(const oracle_handle& h, const Ray<R>& ray, uint32_t index, Hit
     if (hs.closestT >= previous) return;
     hs.geometricNormal = xformNormal(inst.transform, hs.geometricNormal);
     hs.normal = xformNormal(inst.transform, hs.normal);
+    hs.tangent = normalize(xformVector(inst.transform, hs.tangent));
+    hs.bitangent = normalize(xformVector(inst.transform, hs.bitangent));
+    hs.tangent1 = normalize(xformVector(inst.transform, hs.tangent1));
+    hs.bitangent1 = normalize(xformVector(inst.transform, hs.bitangent1));
     if (hs.lightIndex != kInvalid) hs.lightIndex += h.instanceLightOffset[index];
     hs.sphereIndex = kInvalid;
 }
@@ -652,7 +723,8 @@ template <class R>
 struct Hit {
     R t;
     V3<R> position, geometricNormal, normal;
-    V2<R> uv;
+    V2<R> uv, uv1;
+    V3<R> tangent, bitangent, tangent1, bitangent1;
     uint32_t material, lightIndex;
 };
 
@@ -674,7 +746,8 @@ bool traceRay(const oracle_handle& h, const Ray<R>& ray, Hit<R>& hit, HitState<R
         const wpt_sphere& s = h.d.spheres[hs.sphereIndex];
         hs.normal = normalize(ray.at(hs.closestT) - vec<R>(s.center));
         hs.geometricNormal = hs.normal;
-        hs.uv = V2<R>{R(0), R(0)};
+        hs.uv = hs.uv1 = V2<R>{R(0), R(0)};
+        hs.tangent = hs.bitangent = hs.tangent1 = hs.bitangent1 = V3<R>();
         hs.lightIndex = kInvalid;
     }
     if (hs.closestMaterial < 0) return false;
@@ -683,6 +756,11 @@ bool traceRay(const oracle_handle& h, const Ray<R>& ray, Hit<R>& hit, HitState<R
     hit.geometricNormal = hs.geometricNormal;
     hit.normal = hs.normal;
     hit.uv = hs.uv;
+    hit.uv1 = hs.uv1;
+    hit.tangent = hs.tangent;
+    hit.bitangent = hs.bitangent;
+    hit.tangent1 = hs.tangent1;
+    hit.bitangent1 = hs.bitangent1;
     hit.material = static_cast<uint32_t>(hs.closestMaterial);
     hit.lightIndex = hs.lightIndex;
     return true;
@@ -704,7 +782,7 @@ bool isOccluded(const oracle_handle& h, const Ray<R>& ray, R maxDistance) {
     for (uint32_t i = 0; i < h.d.static_triangle_count; ++i) {
         R t, u, v;
         const wpt_triangle& tri = h.d.static_triangles[i];
-        if (intersectTriangle(ray, tri, t, u, v) && t < maxDistance && !cutout(h, tri.material)) return true;
+        if (intersectTriangle(ray, tri, t, u, v) && t < maxDistance && !cutout(h, tri.material, tri, u, v)) return true;
     }
     if (h.tlas.count) {
         V3<R> inv(R(1) / ray.d.x, R(1) / ray.d.y, R(1) / ray.d.z);
@@ -1003,11 +1081,269 @@ V3<R> sampleSG(const SG<R>& s, const V3<R>& n, const V3<R>& wo, R choice, R u, R
     return sgBasis(n, radius * fcos(phi), radius * fsin(phi), std::sqrt(R(1) - u));
 }
 
+// ---------------------------------------------------------------- textures and materials
+// device_traversal.cuh:325-525 (texture lookups, material terms, normal
+// maps, glTF surface) and gltf_pbr.cuh.  Textures read normalized texels
+// with bilinear weights and wrapping; the weights are exact (the
+// reference's texture unit rounds them to 1/256).
+template <class R> struct T4 { R x, y, z, w; };
+
+template <class R>
+T4<R> texel(const oracle_handle& h, uint32_t t, uint32_t x, uint32_t y) {
+    const uint8_t* p = &h.tex[t][4ull * (static_cast<uint64_t>(y) * h.texW[t] + x)];
+    return T4<R>{R(p[0]) / R(255.0f), R(p[1]) / R(255.0f), R(p[2]) / R(255.0f), R(p[3]) / R(255.0f)};
+}
+
+template <class R>
+R unitWrap(R a) {
+    R w = a - std::floor(a);
+    return w >= R(0) && w <= R(1) ? w : R(0);  // non-finite coordinates read 0
+}
+
+template <class R>
+T4<R> texBilinear(const oracle_handle& h, uint32_t t, R u, R v) {
+    const int64_t W = h.texW[t], H = h.texH[t];
+    R x = unitWrap(u) * R(static_cast<float>(W)) - R(0.5f);
+    R y = unitWrap(v) * R(static_cast<float>(H)) - R(0.5f);
+    R fx = std::floor(x), fy = std::floor(y);
+    R ax = x - fx, ay = y - fy;
+    int64_t ix = static_cast<int64_t>(fx), iy = static_cast<int64_t>(fy);
+    auto wrap = [](int64_t i, int64_t n) { return static_cast<uint32_t>(((i % n) + n) % n); };
+    T4<R> a = texel<R>(h, t, wrap(ix, W), wrap(iy, H)), b = texel<R>(h, t, wrap(ix + 1, W), wrap(iy, H));
+    T4<R> c = texel<R>(h, t, wrap(ix, W), wrap(iy + 1, H)), e = texel<R>(h, t, wrap(ix + 1, W), wrap(iy + 1, H));
+    R w00 = (R(1) - ax) * (R(1) - ay), w10 = ax * (R(1) - ay), w01 = (R(1) - ax) * ay, w11 = ax * ay;
+    return T4<R>{a.x * w00 + b.x * w10 + c.x * w01 + e.x * w11, a.y * w00 + b.y * w10 + c.y * w01 + e.y * w11,
+        a.z * w00 + b.z * w10 + c.z * w01 + e.z * w11, a.w * w00 + b.w * w10 + c.w * w01 + e.w * w11};
+}
+
+// getTextureUv + sampleTexture (:325-347): the image is read at 1 - v.
+template <class R>
+T4<R> sampleTex(const oracle_handle& h, const wpt_texture_binding& b, const V2<R>& uv0, const V2<R>& uv1,
+    T4<R> fallback) {
+    if (b.texture == kInvalid || b.texture >= h.tex.size()) return fallback;
+    V2<R> uv = b.texcoord == 1 ? uv1 : uv0;
+    if (b.has_transform) {
+        uv.x *= R(b.scale[0]);
+        uv.y *= R(b.scale[1]);
+        R sn = fsin(R(b.rotation)), cs = fcos(R(b.rotation));
+        uv = V2<R>{cs * uv.x - sn * uv.y + R(b.offset[0]), sn * uv.x + cs * uv.y + R(b.offset[1])};
+    }
+    return texBilinear(h, b.texture, uv.x, R(1) - uv.y);
+}
+
+template <class R> T4<R> white() { return T4<R>{R(1), R(1), R(1), R(1)}; }
+template <class R> V3<R> srgb22(const T4<R>& t) {
+    return V3<R>(fpow(t.x, R(2.2f)), fpow(t.y, R(2.2f)), fpow(t.z, R(2.2f)));
+}
+template <class R> R gltfSrgb1(R c) {
+    return c <= R(0.04045f) ? c / R(12.92f) : fpow((c + R(0.055f)) / R(1.055f), R(2.4f));
+}
+template <class R> V3<R> gltfSrgb(const T4<R>& t) { return V3<R>(gltfSrgb1(t.x), gltfSrgb1(t.y), gltfSrgb1(t.z)); }
+
+// The texture-carrying fields of reference Hit (geometry.cuh:68-82).
+template <class R>
+struct TexHit {
+    V2<R> uv, uv1;
+    V3<R> normal, geometricNormal, tangent, bitangent, tangent1, bitangent1;
+};
+
+template <class R>
+V3<R> matAlbedo(const oracle_handle& h, uint32_t m, const TexHit<R>& th) {  // :359-363
+    const wpt_material_ext& e = h.ext[m];
+    T4<R> t = sampleTex(h, e.albedo_texture, th.uv, th.uv1, white<R>());
+    return hadamard(vec<R>(mat(h, m).albedo), e.specular_glossiness_workflow ? gltfSrgb(t) : srgb22(t));
+}
+
+template <class R>
+R matOpacity(const oracle_handle& h, uint32_t m, const V2<R>& uv, const V2<R>& uv1) {  // :365-375
+    const wpt_material_ext& e = h.ext[m];
+    if (e.albedo_texture.texture == kInvalid || e.albedo_texture.texture >= h.tex.size()) return R(mat(h, m).opacity);
+    return R(mat(h, m).opacity) * sampleTex(h, e.albedo_texture, uv, uv1, white<R>()).w;
+}
+
+template <class R>
+V3<R> matEmission(const oracle_handle& h, uint32_t m, const TexHit<R>& th) {  // :377-380
+    return hadamard(vec<R>(mat(h, m).emission), srgb22(sampleTex(h, h.ext[m].emissive_texture, th.uv, th.uv1,
+        white<R>())));
+}
+
+template <class R>
+R matMetallic(const oracle_handle& h, uint32_t m, const TexHit<R>& th) {  // :382-385
+    return R(mat(h, m).metallic) * sampleTex(h, h.ext[m].metallic_roughness_texture, th.uv, th.uv1, white<R>()).z;
+}
+
+template <class R>
+R matRoughness(const oracle_handle& h, uint32_t m, const TexHit<R>& th) {  // :387-393
+    const wpt_material_ext& e = h.ext[m];
+    if (e.specular_glossiness_workflow)
+        return R(1) - R(mat(h, m).glossiness) * sampleTex(h, e.specular_glossiness_texture, th.uv, th.uv1, white<R>()).w;
+    return R(mat(h, m).roughness) * sampleTex(h, e.metallic_roughness_texture, th.uv, th.uv1, white<R>()).y;
+}
+
+template <class R>
+R matThickness(const oracle_handle& h, uint32_t m, const TexHit<R>& th) {  // :443-449
+    const wpt_texture_binding& b = h.ext[m].thickness_texture;
+    if (b.texture == kInvalid || b.texture >= h.tex.size()) return R(1);
+    return sampleTex(h, b, th.uv, th.uv1, white<R>()).x;
+}
+
+template <class R> bool finiteV(const V3<R>& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+
+// getMaterialNormal (:451-500) for a binding and its scale.
+template <class R>
+V3<R> mapNormal(const oracle_handle& h, const wpt_texture_binding& b, float scale, const TexHit<R>& th) {
+    if (!h.d.use_normal_maps || b.texture == kInvalid || b.texture >= h.tex.size()) return th.normal;
+    V3<R> tg = b.texcoord == 1 ? th.tangent1 : th.tangent;
+    V3<R> bt = b.texcoord == 1 ? th.bitangent1 : th.bitangent;
+    if (!finiteV(tg) || !finiteV(bt) || lengthSquared(tg) < R(0.00000001f) || lengthSquared(bt) < R(0.00000001f))
+        return th.normal;
+    V3<R> t = normalize(tg - th.normal * dot(th.normal, tg));
+    V3<R> bb = normalize(cross(th.normal, t));
+    if (!finiteV(t) || !finiteV(bb)) return th.normal;
+    if (dot(bb, bt) < R(0)) bb = -bb;
+    T4<R> x = sampleTex(h, b, th.uv, th.uv1, T4<R>{R(0.5f), R(0.5f), R(1), R(1)});
+    V3<R> local((x.x * R(2) - R(1)) * R(scale), (x.y * R(2) - R(1)) * R(scale), x.z * R(2) - R(1));
+    local = normalize(local);
+    R minCos = R(h.d.normal_map_minimum_cosine);
+    if (minCos > R(0) && local.z < minCos) {
+        R mc = std::fmin(minCos, R(1));
+        R len = std::sqrt(std::fmax(R(0), local.x * local.x + local.y * local.y));
+        if (len > R(0)) {
+            R k = std::sqrt(std::fmax(R(0), R(1) - mc * mc)) / len;
+            local.x *= k;
+            local.y *= k;
+        } else {
+            local.x = R(0);
+            local.y = R(0);
+        }
+        local.z = mc;
+    }
+    V3<R> n = t * local.x + bb * local.y + th.normal * local.z;
+    if (!finiteV(n) || lengthSquared(n) < R(0.00000001f)) return th.normal;
+    return normalize(n);
+}
+
+// glTF metallic-roughness surface with specular and clearcoat (gltf_pbr.cuh).
+template <class R>
+struct GSurf {
+    V3<R> color, normal, coatNormal;
+    R metallic, roughness;
+    V3<R> f0;
+    R f90, coat, coatRoughness;
+    V3<R> reflectionNormal;
+};
+
+template <class R>
+V3<R> reflectionFrame(V3<R> n, V3<R> g, V3<R> view) {  // gltfReflectionNormal: 24 bisection steps
+    R gv = dot(g, view);
+    if (gv <= R(0)) return g;
+    R target = std::fmin(R(0.01f), R(0.9f) * gv);
+    V3<R> r = R(2) * dot(n, view) * n - view;
+    if (dot(g, r) >= target) return n;
+    R lo = R(0), hi = R(1);
+    for (int i = 0; i < 24; ++i) {
+        R t = (lo + hi) * R(0.5f);
+        V3<R> c = normalize(n * (R(1) - t) + g * t);
+        r = R(2) * dot(c, view) * c - view;
+        if (dot(g, r) >= target) hi = t;
+        else lo = t;
+    }
+    return normalize(n * (R(1) - hi) + g * hi);
+}
+template <class R> V3<R> specN(const GSurf<R>& s) { return lengthSquared(s.reflectionNormal) > R(0) ? s.reflectionNormal : s.normal; }
+template <class R> R sat(R x) { return std::fmin(R(1), std::fmax(R(0), x)); }
+template <class R> R schlick5(R c) {
+    R x = R(1) - sat(c);
+    return x * x * x * x * x;
+}
+template <class R> V3<R> dielF(const GSurf<R>& s, R c) {
+    return s.f0 + (V3<R>(s.f90, s.f90, s.f90) - s.f0) * schlick5(c);
+}
+template <class R> V3<R> baseF(const GSurf<R>& s, R c) {
+    V3<R> metal = s.color + (V3<R>(R(1), R(1), R(1)) - s.color) * schlick5(c);
+    return dielF(s, c) * (R(1) - s.metallic) + metal * s.metallic;
+}
+template <class R> void lobeWeights(const GSurf<R>& s, const V3<R>& view, R& wd, R& ws, R& wc) {
+    wc = s.coat * (R(0.04f) + R(0.96f) * schlick5(dot(s.coatNormal, view)));
+    R spec = s.metallic + (R(1) - s.metallic) * sgMax(dielF(s, dot(specN(s), view)));
+    wd = (R(1) - wc) * (R(1) - spec);
+    ws = (R(1) - wc) * spec;
+}
+template <class R> R gAlpha(R r) { return std::fmax(R(1.e-4f), r * r); }
+template <class R> R reflPdf(const V3<R>& n, const V3<R>& view, const V3<R>& light, R rough) {
+    if (rough <= R(0) || dot(n, view) <= R(0) || dot(n, light) <= R(0) || lengthSquared(view + light) < R(1.e-12f))
+        return R(0);
+    V3<R> half = normalize(view + light);
+    R vh = dot(view, half), nh = dot(n, half);
+    return vh > R(0) && nh > R(0) ? sgD(sat(nh), gAlpha(rough)) * nh / (R(4) * vh) : R(0);
+}
+template <class R> R reflKernel(const V3<R>& n, const V3<R>& view, const V3<R>& light, R rough) {
+    R nv = dot(n, view), nl = dot(n, light);
+    if (rough <= R(0) || nv <= R(0) || nl <= R(0) || lengthSquared(view + light) < R(1.e-12f)) return R(0);
+    R a = gAlpha(rough), nh = sat(dot(n, normalize(view + light)));
+    return sgD(nh, a) * sgG1(nv, a) * sgG1(nl, a) / (R(4) * nv * nl);
+}
+template <class R> V3<R> reflSample(const V3<R>& n, const V3<R>& view, R rough, R u, R v) {
+    if (rough <= R(0)) return R(2) * dot(n, view) * n - view;
+    R a = gAlpha(rough);
+    R c = std::sqrt((R(1) - u) / ((R(1) - u) + a * a * u));
+    R sn = std::sqrt(std::fmax(R(0), R(1) - c * c)), phi = R(6.28318530718f) * v;
+    V3<R> half = sgBasis(n, sn * fcos(phi), sn * fsin(phi), c);
+    return R(2) * dot(view, half) * half - view;
+}
+template <class R>
+V3<R> evalGltf(const GSurf<R>& s, const V3<R>& view, const V3<R>& light, R diffusePdf, R& pdf) {  // evaluateGltfPbrCos
+    R wd, ws, wc;
+    lobeWeights(s, view, wd, ws, wc);
+    R nl = std::fmax(R(0), dot(s.normal, light));
+    V3<R> rn = specN(s);
+    pdf = wd * diffusePdf + ws * reflPdf(rn, view, light, s.roughness) + wc * reflPdf(s.coatNormal, view, light,
+        s.coatRoughness);
+    V3<R> base;
+    if (lengthSquared(view + light) > R(1.e-12f)) {
+        R vh = sat(dot(view, normalize(view + light)));
+        R dw = (R(1) - s.metallic) * (R(1) - sgMax(dielF(s, vh)));
+        if (dot(s.normal, view) > R(0)) base = s.color * (dw * nl * R(0.31830988618f));
+        R sc = std::fmax(R(0), dot(rn, light));
+        base += baseF(s, vh) * (reflKernel(rn, view, light, s.roughness) * sc);
+    }
+    R cc = std::fmax(R(0), dot(s.coatNormal, light));
+    R cv = wc * reflKernel(s.coatNormal, view, light, s.coatRoughness) * cc;
+    return base * (R(1) - wc) + V3<R>(cv, cv, cv);
+}
+
+// getGltfPbrSurface (:502-517) with correctGltfReflectionNormals.
+template <class R>
+GSurf<R> gltfAt(const oracle_handle& h, uint32_t m, const TexHit<R>& th, const V3<R>& shadedNormal,
+    const V3<R>& view) {
+    const wpt_material_ext& e = h.ext[m];
+    R weight = R(e.specular_weight) * sampleTex(h, e.specular_weight_texture, th.uv, th.uv1, white<R>()).w;
+    V3<R> color = hadamard(vec<R>(e.specular_color), gltfSrgb(sampleTex(h, e.specular_color_texture, th.uv, th.uv1,
+        white<R>())));
+    R ior = R(mat(h, m).ior);
+    R f0 = (ior - R(1)) / (ior + R(1));
+    f0 *= f0;
+    GSurf<R> g;
+    g.f0 = V3<R>(std::fmin(R(1), f0 * color.x) * weight, std::fmin(R(1), f0 * color.y) * weight,
+        std::fmin(R(1), f0 * color.z) * weight);
+    V3<R> cn = mapNormal(h, e.clearcoat_normal_texture, e.clearcoat_normal_scale, th);
+    if (dot(cn, th.geometricNormal) < R(0)) cn = -cn;
+    g.color = matAlbedo(h, m, th);
+    g.normal = shadedNormal;
+    g.coatNormal = cn;
+    g.metallic = matMetallic(h, m, th);
+    g.roughness = matRoughness(h, m, th);
+    g.f90 = weight;
+    g.coat = R(e.clearcoat) * sampleTex(h, e.clearcoat_texture, th.uv, th.uv1, white<R>()).x;
+    g.coatRoughness = R(e.clearcoat_roughness) * sampleTex(h, e.clearcoat_roughness_texture, th.uv, th.uv1,
+        white<R>()).y;
+    g.reflectionNormal = reflectionFrame(g.normal, th.geometricNormal, view);
+    if (g.coat > R(0)) g.coatNormal = reflectionFrame(g.coatNormal, th.geometricNormal, view);
+    return g;
+}
+
 template <class R> V3<R> albedoOf(const wpt_material& m) { return vec<R>(m.albedo); }
 
-bool metalEffective(const wpt_material& m) {
-    return m.type == WPT_MATERIAL_METAL || (m.type == WPT_MATERIAL_DIFFUSE && m.metallic > 0.5f);
-}
 
 // evaluateSurface / evaluateSurfaceCos (untextured), device_traversal.cuh:427-441, 519-527.
 // Diffuse continuation density: half cosine, half environment when a
@@ -1025,24 +1361,48 @@ R diffusePdf(const oracle_handle& h, const V3<R>& n, const V3<R>& wi) {
     return p > R(0) ? (R(1) - p) * cosinePdf + p * envPdf(h, wi) : cosinePdf;
 }
 
+// The material terms shadePaths reads at a hit (device_path_shading.cuh:79-87).
 template <class R>
-V3<R> evaluateSurfaceCos(const oracle_handle& h, const wpt_material& m, const V3<R>& n, const V3<R>& wo,
+struct Vertex {
+    uint32_t material;
+    V3<R> n;        // shaded normal (normal-mapped, facing the geometric normal)
+    V3<R> albedo;
+    R roughness;
+    uint32_t type;  // after the metallic switch
+    bool gltf;
+    GSurf<R> g;     // glTF surface, gltf only
+};
+
+// evaluateSurfaceCos (:519-527) over evaluateSurface (:427-441).
+template <class R>
+V3<R> evaluateSurfaceCos(const oracle_handle& h, const Vertex<R>& vx, const TexHit<R>& th, const V3<R>& wo,
     const V3<R>& wi, R& pdf) {
+    if (vx.gltf) return evalGltf(vx.g, wo, wi, diffusePdf(h, vx.g.normal, wi), pdf);
+    const wpt_material& m = mat(h, vx.material);
+    const V3<R>& n = vx.n;
     V3<R> value;
     if (m.type == WPT_MATERIAL_SPECULAR_GLOSSINESS) {
-        value = evalSG(SG<R>{albedoOf<R>(m), vec<R>(m.specular), R(m.roughness)}, n, wo, wi, pdf);
-    } else if (metalEffective(m)) {
-        if (R(m.roughness) <= R(0)) {
+        V3<R> f0 = hadamard(vec<R>(m.specular), gltfSrgb(sampleTex(h, h.ext[vx.material].specular_glossiness_texture,
+            th.uv, th.uv1, white<R>())));
+        value = evalSG(SG<R>{vx.albedo, f0, vx.roughness}, n, wo, wi, pdf);
+    } else if (vx.type == WPT_MATERIAL_METAL) {
+        if (vx.roughness <= R(0)) {
             pdf = R(0);
             value = V3<R>();
         } else {
-            value = evalSG(SG<R>{V3<R>(), albedoOf<R>(m), R(m.roughness)}, n, wo, wi, pdf);
+            value = evalSG(SG<R>{V3<R>(), vx.albedo, vx.roughness}, n, wo, wi, pdf);
         }
     } else {
         pdf = diffusePdf(h, n, wi);
-        value = albedoOf<R>(m) * R(0.31830988618f);
+        value = vx.albedo * R(0.31830988618f);
     }
     return value * std::fmax(R(0), dot(n, wi));
+}
+
+template <class R>
+TexHit<R> texHitOf(const Hit<R>& hit) {
+    return TexHit<R>{hit.uv, hit.uv1, hit.normal, hit.geometricNormal, hit.tangent, hit.bitangent, hit.tangent1,
+        hit.bitangent1};
 }
 
 template <class R>
@@ -1067,7 +1427,7 @@ struct Counters {
 // sampleDirectLight, device_traversal.cuh:529-569.
 template <class R>
 void sampleDirectLight(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const Hit<R>& shaded,
-    const wpt_material& material, Counters& cnt) {
+    const Vertex<R>& vx, Counters& cnt) {
     uint32_t lightCount = static_cast<uint32_t>(h.lights.size());
     if (lightCount == 0) return;
     uint32_t candidate = std::min(static_cast<uint32_t>(randomFloat(path.rng) * lightCount), lightCount - 1);
@@ -1085,7 +1445,7 @@ void sampleDirectLight(const oracle_handle& h, PathState<R>& path, const Hit<R>&
     R gc = std::fmax(R(0), dot(hit.geometricNormal, dir));
     R sc = std::fmax(R(0), dot(shaded.normal, dir));
     R lc = std::fmax(R(0), dot(ln, -dir));
-    if (gc == R(0) || sc == R(0) || lc == R(0)) return;
+    if (gc == R(0) || (!vx.gltf && sc == R(0)) || lc == R(0)) return;
     V3<R> so = offsetRayOrigin(hit.position, hit.geometricNormal, dir, hit.t);
     V3<R> st = offsetRayOrigin(lp, ln, -dir, dist);
     V3<R> seg = st - so;
@@ -1104,7 +1464,7 @@ void sampleDirectLight(const oracle_handle& h, PathState<R>& path, const Hit<R>&
     R geometryTerm = lc * area / (d2 * pdfSel);
     R lightPdf = pdfSel * d2 / (lc * area);
     R bsdfPdf;
-    V3<R> bsdf = evaluateSurfaceCos(h, material, shaded.normal, -path.ray.d, dir, bsdfPdf);
+    V3<R> bsdf = evaluateSurfaceCos(h, vx, texHitOf(hit), -path.ray.d, dir, bsdfPdf);
     R w = powerHeuristic(lightPdf, bsdfPdf);
     V3<R> direct = hadamard(bsdf, vec<R>(lm.emission)) * (geometryTerm * w);
     addPathRadiance(path, hadamard(path.throughput, direct));
@@ -1146,11 +1506,12 @@ V3<R> punctualSample(const wpt_punctual_light& L, const V3<R>& p, V3<R>& inciden
 // addPunctualLight, device_traversal.cuh:653-674 (no MIS: a delta light).
 template <class R>
 void addPunctual(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const Hit<R>& shaded,
-    const wpt_material& material, const wpt_punctual_light& L, R inverseSelectionPdf, Counters& cnt) {
+    const Vertex<R>& vx, const wpt_punctual_light& L, R inverseSelectionPdf, Counters& cnt) {
     V3<R> incident;
     R distance;
     V3<R> dir = punctualSample(L, hit.position, incident, distance);
-    if (dot(shaded.normal, dir) <= R(0) || dot(hit.geometricNormal, dir) <= R(0) || lengthSquared(incident) == R(0))
+    if ((!vx.gltf && dot(shaded.normal, dir) <= R(0)) || dot(hit.geometricNormal, dir) <= R(0) ||
+        lengthSquared(incident) == R(0))
         return;
     Ray<R> shadow{offsetRayOrigin(hit.position, hit.geometricNormal, dir, hit.t), dir};
     R reach = distance;
@@ -1168,19 +1529,19 @@ void addPunctual(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, 
         if (blocked) return;
     }
     R ignored;
-    V3<R> bsdf = evaluateSurfaceCos(h, material, shaded.normal, -path.ray.d, dir, ignored);
+    V3<R> bsdf = evaluateSurfaceCos(h, vx, texHitOf(hit), -path.ray.d, dir, ignored);
     addPathRadiance(path, hadamard(path.throughput, hadamard(bsdf, incident) * inverseSelectionPdf));
 }
 
 // sampleDirectPunctualLight, device_traversal.cuh:676-683.
 template <class R>
 void samplePunctual(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const Hit<R>& shaded,
-    const wpt_material& material, Counters& cnt) {
-    for (uint32_t i = 0; i < h.directional; ++i) addPunctual(h, path, hit, shaded, material, h.punctual[i], R(1), cnt);
+    const Vertex<R>& vx, Counters& cnt) {
+    for (uint32_t i = 0; i < h.directional; ++i) addPunctual(h, path, hit, shaded, vx, h.punctual[i], R(1), cnt);
     uint32_t n = static_cast<uint32_t>(h.punctual.size()) - h.directional;
     if (n == 0) return;
     uint32_t i = h.directional + std::min(static_cast<uint32_t>(randomFloat(path.rng) * n), n - 1);
-    addPunctual(h, path, hit, shaded, material, h.punctual[i], R(static_cast<float>(n)), cnt);
+    addPunctual(h, path, hit, shaded, vx, h.punctual[i], R(static_cast<float>(n)), cnt);
 }
 
 // The five draws of an environment sample (samplePathEnvironment,
@@ -1197,11 +1558,12 @@ V3<R> envDraw(const oracle_handle& h, RngState& rng, R& pdf) {
 // sampleDirectEnvironment, device_traversal.cuh:637-651.
 template <class R>
 void sampleDirectEnvironment(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const Hit<R>& shaded,
-    const wpt_material& material, Counters& cnt) {
+    const Vertex<R>& vx, Counters& cnt) {
     if (!h.env || h.envIntensity == 0.0f) return;
     R pdf;
     V3<R> dir = envDraw<R>(h, path.rng, pdf);
-    if (dot(shaded.normal, dir) <= R(0) || dot(hit.geometricNormal, dir) <= R(0) || !(pdf > R(0))) return;
+    if ((!vx.gltf && dot(shaded.normal, dir) <= R(0)) || dot(hit.geometricNormal, dir) <= R(0) || !(pdf > R(0)))
+        return;
     Ray<R> shadow{offsetRayOrigin(hit.position, hit.geometricNormal, dir, hit.t), dir};
     if (!h.d.ignore_direct_light_occlusion) {
         ++cnt.shadow;
@@ -1211,7 +1573,7 @@ void sampleDirectEnvironment(const oracle_handle& h, PathState<R>& path, const H
         if (blocked) return;
     }
     R bsdfPdf;
-    V3<R> bsdf = evaluateSurfaceCos(h, material, shaded.normal, -path.ray.d, dir, bsdfPdf);
+    V3<R> bsdf = evaluateSurfaceCos(h, vx, texHitOf(hit), -path.ray.d, dir, bsdfPdf);
     R w = powerHeuristic(pdf, bsdfPdf);
     addPathRadiance(path, hadamard(path.throughput, hadamard(bsdf, envRadiance(h, dir)) * (w / pdf)));
 }
@@ -1227,9 +1589,84 @@ bool escapes(const oracle_handle& h, const Hit<R>& hit, const V3<R>& dir, Counte
     return !blocked;
 }
 
+// continueGltfPbrSurface, device_traversal.cuh:727-787.
+template <class R>
+V3<R> gltfContinue(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const GSurf<R>& g,
+    uint32_t maxDepth, Counters& cnt) {
+    V3<R> view = -path.ray.d;
+    R wd, ws, wc;
+    lobeWeights(g, view, wd, ws, wc);
+    (void)wd;
+    R choice = rnd<R>(path.rng);
+    bool coat = choice < wc;
+    bool spec = !coat && choice < wc + ws;
+    bool delta = (coat && g.coatRoughness <= R(0)) || (spec && g.roughness <= R(0));
+    bool baseOut = false, coatOut = false;
+    V3<R> baseCoef = baseF(g, dot(specN(g), view)) * (R(1) - wc);
+    if (h.env && path.depth + 1 < maxDepth) {  // delta lobes toward the map, integrated here
+        for (int lobe = 0; lobe < 2; ++lobe) {
+            bool isCoat = lobe == 1;
+            R mass = isCoat ? wc : ws;
+            R rough = isCoat ? g.coatRoughness : g.roughness;
+            if (!(mass > R(0)) || rough > R(0)) continue;
+            V3<R> n = isCoat ? g.coatNormal : specN(g);
+            V3<R> dir = reflSample(n, view, R(0), R(0), R(0));
+            if (dot(n, view) <= R(0) || dot(dir, hit.geometricNormal) <= R(0)) continue;
+            bool out = escapes(h, hit, dir, cnt);
+            if (isCoat) coatOut = out;
+            else baseOut = out;
+            if (out) {
+                V3<R> coef = isCoat ? V3<R>(wc, wc, wc) : baseCoef;
+                addPathRadiance(path, hadamard(path.throughput, hadamard(coef, envRadiance(h, dir))));
+            }
+        }
+    }
+    V3<R> dir;
+    if (coat || spec) {
+        R u = rnd<R>(path.rng), v = rnd<R>(path.rng);
+        V3<R> n = coat ? g.coatNormal : specN(g);
+        dir = reflSample(n, view, coat ? g.coatRoughness : g.roughness, u, v);
+        if (dot(n, view) <= R(0)) {
+            path.active = false;
+            return dir;
+        }
+    } else {
+        R mix = envMixture<R>(h);
+        if (mix > R(0) && rnd<R>(path.rng) < mix) {
+            R ignored;
+            dir = envDraw<R>(h, path.rng, ignored);
+        } else {
+            dir = sampleCosineHemisphere(g.normal, path.rng);
+        }
+    }
+    if (dot(dir, hit.geometricNormal) <= R(0)) {
+        path.active = false;
+        return dir;
+    }
+    if (delta) {
+        R mass = coat ? wc : ws;
+        V3<R> coef = coat ? V3<R>(wc, wc, wc) : baseCoef;
+        path.throughput = hadamard(path.throughput, coef) / mass;
+        path.specularBounce = true;
+        path.previousBsdfPdf = R(0);
+        if (coat ? coatOut : baseOut) path.active = false;
+    } else {
+        R pdf;
+        V3<R> value = evalGltf(g, view, dir, diffusePdf(h, g.normal, dir), pdf);
+        if (!(pdf > R(0))) {
+            path.active = false;
+            return dir;
+        }
+        path.throughput = hadamard(path.throughput, value) / pdf;
+        path.previousBsdfPdf = pdf;
+        path.specularBounce = false;
+    }
+    return dir;
+}
+
 // One path, shadePaths (device_path_shading.cuh:6-173) applied bounce by bounce.
 template <class R>
-V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, uint32_t rrStart, bool debug,
+V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, uint32_t rrStart, int debug,
     Counters& cnt) {
     const R invPi = R(0.31830988618f);
     for (uint32_t bounce = 0; bounce < maxDepth; ++bounce) {
@@ -1239,9 +1676,13 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
         tlWork = &cnt.closestWork;
         bool didHit = traceRay(h, path.ray, hit, hs);
         tlWork = nullptr;
-        if (debug) {
+        if (debug) {  // 1: intersection normals, 2: shading normals (:18-36)
             V3<R> color(R(0.05f), R(0.10f), R(0.25f));
-            if (didHit) color = R(0.5f) * (hit.normal + V3<R>(R(1), R(1), R(1)));
+            if (didHit) {
+                const wpt_material_ext& e = h.ext[hit.material];
+                V3<R> n = debug == 1 ? hit.normal : mapNormal(h, e.normal_texture, e.normal_scale, texHitOf(hit));
+                color = R(0.5f) * (n + V3<R>(R(1), R(1), R(1)));
+            }
             return color;
         }
         if (!didHit) {  // addEnvironmentOrSky, device_traversal.cuh:577-587
@@ -1280,18 +1721,25 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
                     w = powerHeuristic(path.previousBsdfPdf, lightPdf);
                 }
             }
-            addPathRadiance(path, hadamard(path.throughput, vec<R>(m.emission)) * w, true);
+            addPathRadiance(path, hadamard(path.throughput, matEmission(h, hit.material, texHitOf(hit))) * w, true);
             return path.radiance;
         }
+        const wpt_material_ext& ext = h.ext[hit.material];
+        TexHit<R> th = texHitOf(hit);
         Hit<R> shaded = hit;
+        shaded.normal = mapNormal(h, ext.normal_texture, ext.normal_scale, th);
         if (dot(shaded.normal, hit.geometricNormal) < R(0)) shaded.normal = -shaded.normal;
-        V3<R> albedo = albedoOf<R>(m);
-        uint32_t type = (m.type == WPT_MATERIAL_DIFFUSE && m.metallic > 0.5f) ? WPT_MATERIAL_METAL : m.type;
-        R roughness = R(m.roughness);
+        V3<R> albedo = matAlbedo(h, hit.material, th);
+        uint32_t type = (m.type == WPT_MATERIAL_DIFFUSE && matMetallic(h, hit.material, th) > R(0.5f))
+            ? WPT_MATERIAL_METAL : m.type;
+        R roughness = matRoughness(h, hit.material, th);
+        Vertex<R> vx{hit.material, shaded.normal, albedo, roughness, type, ext.gltf_pbr_workflow != 0, GSurf<R>{}};
         V3<R> direction;
         auto glossy = [&]() {  // continueGlossySurface, device_traversal.cuh:706-725
-            SG<R> s = m.type == WPT_MATERIAL_SPECULAR_GLOSSINESS ? SG<R>{albedo, vec<R>(m.specular), roughness}
-                                                                 : SG<R>{V3<R>(), albedo, roughness};
+            SG<R> s = m.type == WPT_MATERIAL_SPECULAR_GLOSSINESS
+                ? SG<R>{albedo, hadamard(vec<R>(m.specular), gltfSrgb(sampleTex(h, ext.specular_glossiness_texture,
+                      th.uv, th.uv1, white<R>()))), roughness}
+                : SG<R>{V3<R>(), albedo, roughness};
             V3<R> wo = -path.ray.d;
             R choice = rnd<R>(path.rng);
             R u = rnd<R>(path.rng);
@@ -1308,11 +1756,17 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
             path.specularBounce = false;
             return dir;
         };
-        switch (type) {
+        if (vx.gltf) {
+            vx.g = gltfAt(h, hit.material, th, shaded.normal, -path.ray.d);
+            sampleDirectLight(h, path, hit, shaded, vx, cnt);
+            samplePunctual(h, path, hit, shaded, vx, cnt);
+            sampleDirectEnvironment(h, path, hit, shaded, vx, cnt);
+            direction = gltfContinue(h, path, hit, vx.g, maxDepth, cnt);
+        } else switch (type) {
         case WPT_MATERIAL_DIFFUSE: {
-            sampleDirectLight(h, path, hit, shaded, m, cnt);
-            samplePunctual(h, path, hit, shaded, m, cnt);
-            sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
+            sampleDirectLight(h, path, hit, shaded, vx, cnt);
+            samplePunctual(h, path, hit, shaded, vx, cnt);
+            sampleDirectEnvironment(h, path, hit, shaded, vx, cnt);
             // continueDiffuseSurface, device_traversal.cuh:685-704
             R mix = envMixture<R>(h);
             if (mix > R(0) && rnd<R>(path.rng) < mix) {
@@ -1333,16 +1787,16 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
             break;
         }
         case WPT_MATERIAL_SPECULAR_GLOSSINESS:
-            sampleDirectLight(h, path, hit, shaded, m, cnt);
-            samplePunctual(h, path, hit, shaded, m, cnt);
-            sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
+            sampleDirectLight(h, path, hit, shaded, vx, cnt);
+            samplePunctual(h, path, hit, shaded, vx, cnt);
+            sampleDirectEnvironment(h, path, hit, shaded, vx, cnt);
             direction = glossy();
             break;
         case WPT_MATERIAL_METAL:
             if (roughness > R(0)) {
-                sampleDirectLight(h, path, hit, shaded, m, cnt);
-                samplePunctual(h, path, hit, shaded, m, cnt);
-                sampleDirectEnvironment(h, path, hit, shaded, m, cnt);
+                sampleDirectLight(h, path, hit, shaded, vx, cnt);
+                samplePunctual(h, path, hit, shaded, vx, cnt);
+                sampleDirectEnvironment(h, path, hit, shaded, vx, cnt);
                 direction = glossy();
                 break;
             }
@@ -1375,7 +1829,7 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
                 path.etaScale = path.etaScale / eta2;
                 if (b.frontFace) {
                     path.mediumMaterial = hit.material;
-                    path.mediumDensity = R(m.volume_density) * R(1);
+                    path.mediumDensity = R(m.volume_density) * matThickness(h, hit.material, th);
                 } else {
                     path.mediumMaterial = kInvalid;
                     path.mediumDensity = R(1);
@@ -1435,7 +1889,7 @@ PathState<R> primary(const wpt_camera& cam, uint32_t x, uint32_t y, uint32_t wid
 template <class R>
 void renderRows(const oracle_handle& h, const wpt_camera& cam, const wpt_render_config& cfg, uint32_t x0, uint32_t y0,
     uint32_t w, uint32_t th, uint32_t s0, uint32_t sc, uint32_t first, uint32_t step, float* fb, Counters& cnt) {
-    bool debug = cfg.intersection_debug || cfg.shading_normal_debug;
+    int debug = cfg.intersection_debug ? 1 : (cfg.shading_normal_debug ? 2 : 0);
     for (uint32_t row = first; row < th; row += step) {
         uint32_t y = y0 + row;
         for (uint32_t x = x0; x < x0 + w; ++x) {
@@ -1869,6 +2323,82 @@ uint32_t oracle_env_tables(const oracle_handle* h, float* probabilities, float* 
     return n;
 }
 
+// glTF lobes over a surface given as 23 floats: color[3], normal[3],
+// coatNormal[3], metallic, roughness, f0[3], f90, coat, coatRoughness,
+// reflectionNormal[3], unused[1].
+static GSurf<float> gsurfFrom(const float* p) {
+    GSurf<float> g;
+    g.color = V3<float>(p[0], p[1], p[2]);
+    g.normal = V3<float>(p[3], p[4], p[5]);
+    g.coatNormal = V3<float>(p[6], p[7], p[8]);
+    g.metallic = p[9];
+    g.roughness = p[10];
+    g.f0 = V3<float>(p[11], p[12], p[13]);
+    g.f90 = p[14];
+    g.coat = p[15];
+    g.coatRoughness = p[16];
+    g.reflectionNormal = V3<float>(p[17], p[18], p[19]);
+    return g;
+}
+
+void oracle_gltf_eval(const float* surface, const float* view, const float* light, float diffusePdf, float* value,
+    float* pdf, float* weights) {
+    GSurf<float> g = gsurfFrom(surface);
+    V3<float> v(view[0], view[1], view[2]), l(light[0], light[1], light[2]);
+    V3<float> f = evalGltf(g, v, l, diffusePdf, *pdf);
+    value[0] = f.x;
+    value[1] = f.y;
+    value[2] = f.z;
+    lobeWeights(g, v, weights[0], weights[1], weights[2]);
+}
+
+void oracle_gltf_sample_reflection(const float* normal, const float* view, float roughness, float u, float v,
+    float* out) {
+    V3<float> d = reflSample(V3<float>(normal[0], normal[1], normal[2]), V3<float>(view[0], view[1], view[2]),
+        roughness, u, v);
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}
+
+void oracle_gltf_reflection_normal(const float* normal, const float* geometric, const float* view, float* out) {
+    V3<float> d = reflectionFrame(V3<float>(normal[0], normal[1], normal[2]),
+        V3<float>(geometric[0], geometric[1], geometric[2]), V3<float>(view[0], view[1], view[2]));
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}
+
+// Material terms at a surface point (normal +z, tangent +x, bitangent +y):
+// out = albedo[3], opacity, emission[3], metallic, roughness, specular f0[3],
+// thickness, shading normal[3], and for glTF materials dielectricF0[3], F90,
+// coat, coatRoughness, coatNormal[3] (29 floats).
+void oracle_material_terms(const oracle_handle* h, uint32_t material, const float* uv, const float* uv1, float* out) {
+    TexHit<float> th;
+    th.uv = V2<float>{uv[0], uv[1]};
+    th.uv1 = V2<float>{uv1[0], uv1[1]};
+    th.normal = th.geometricNormal = V3<float>(0, 0, 1);
+    th.tangent = th.tangent1 = V3<float>(1, 0, 0);
+    th.bitangent = th.bitangent1 = V3<float>(0, 1, 0);
+    const wpt_material_ext& e = h->ext[material];
+    V3<float> a = matAlbedo(*h, material, th), em = matEmission(*h, material, th);
+    V3<float> f0 = hadamard(vec<float>(mat(*h, material).specular), gltfSrgb(sampleTex(*h,
+        e.specular_glossiness_texture, th.uv, th.uv1, white<float>())));
+    V3<float> n = mapNormal(*h, e.normal_texture, e.normal_scale, th);
+    float vals[18] = {a.x, a.y, a.z, matOpacity(*h, material, th.uv, th.uv1), em.x, em.y, em.z,
+        matMetallic(*h, material, th), matRoughness(*h, material, th), f0.x, f0.y, f0.z,
+        matThickness(*h, material, th), n.x, n.y, n.z, 0, 0};
+    for (int i = 0; i < 16; ++i) out[i] = vals[i];
+    if (e.gltf_pbr_workflow) {
+        GSurf<float> g = gltfAt(*h, material, th, n, V3<float>(0, 0, 1));
+        float gv[11] = {g.f0.x, g.f0.y, g.f0.z, g.f90, g.coat, g.coatRoughness, g.coatNormal.x, g.coatNormal.y,
+            g.coatNormal.z, 0, 0};
+        for (int i = 0; i < 9; ++i) out[16 + i] = gv[i];
+    } else {
+        for (int i = 16; i < 25; ++i) out[i] = 0.0f;
+    }
+}
+
 void oracle_punctual_sample(const wpt_punctual_light* light, const float* point, float* direction, float* incident,
     float* distance) {
     V3<float> in;
@@ -2014,7 +2544,7 @@ void oracle_brute_fp64(const oracle_handle* h, const float* rays, uint32_t n, do
             double t = dot(e2, q) / det;
             if (u < 0 || v < 0 || u + v > 1 || t <= 0.001 || t >= best) return;
             uint32_t m = override == kInvalid ? tr.material : override;
-            if (cutout(*h, m)) return;
+            if (cutout(*h, m, tr, u, v)) return;
             V3<double> nn = (tr.flags & WPT_TRIANGLE_HAS_NORMALS)
                 ? normalize(vec<double>(tr.n0) * (1 - u - v) + vec<double>(tr.n1) * u + vec<double>(tr.n2) * v)
                 : normalize(cross(e1, e2));

@@ -51,6 +51,23 @@ int oracle_create(const oracle_scene* scene, oracle_handle** out);
  * per-direction uv, pdf, radiance and texel index (null outputs skipped).
  * oracle_env_sample draws directions from FP64 alias candidates and
  * (coin, longitude, latitude) triples.  Both return 1 without a map. */
+/* glTF metallic-roughness lobes (FP32, gltf_pbr.cuh).  `surface` holds 20
+ * floats: color[3], normal[3], coatNormal[3], metallic, roughness, F0[3],
+ * F90, coat, coatRoughness, reflectionNormal[3] (zero: the normal).
+ * oracle_gltf_eval returns f x cos, the mixture pdf and the diffuse,
+ * specular and coat weights. */
+void oracle_gltf_eval(const float* surface, const float* view, const float* light, float diffuse_pdf, float* value,
+    float* pdf, float* weights);
+void oracle_gltf_sample_reflection(const float* normal, const float* view, float roughness, float u, float v,
+    float* out);
+void oracle_gltf_reflection_normal(const float* normal, const float* geometric, const float* view, float* out);
+/* Material terms of `material` at texture coordinates (uv, uv1) on a
+ * surface with normal +z, tangent +x, bitangent +y: albedo[3], opacity,
+ * emission[3], metallic, roughness, specular-glossiness F0[3], thickness,
+ * mapped normal[3]; for glTF materials dielectric F0[3], F90, coat, coat
+ * roughness, coat normal[3] (25 floats). */
+void oracle_material_terms(const oracle_handle* h, uint32_t material, const float* uv, const float* uv1, float* out);
+
 /* samplePunctualLight (FP32): direction toward the light, arriving
  * intensity and distance (1e30 for directional lights) at `point`. */
 void oracle_punctual_sample(const wpt_punctual_light* light, const float* point, float* direction, float* incident,

@@ -66,6 +66,10 @@ def library():
             ("oracle_diff_directional64", [P, C.POINTER(wpt.Camera), C.POINTER(OracleDiffConfig), P, P, u32, u32, P],
              C.c_int),
             ("oracle_punctual_sample", [C.POINTER(wpt.PunctualLight), P, P, P, P], None),
+            ("oracle_gltf_eval", [P, P, P, C.c_float, P, P, P], None),
+            ("oracle_gltf_sample_reflection", [P, P, C.c_float, C.c_float, C.c_float, P], None),
+            ("oracle_gltf_reflection_normal", [P, P, P, P], None),
+            ("oracle_material_terms", [P, u32, P, P, P], None),
             ("oracle_env_tables", [P, P, P, u32], u32),
             ("oracle_env_solid_angle", [u32, u32, u32], C.c_float),
             ("oracle_env_eval", [P, P, u32, P, P, P, P], C.c_int),
@@ -112,6 +116,41 @@ def punctual_sample(light, point):
     dist = C.c_float()
     library().oracle_punctual_sample(C.byref(light), p.ctypes.data, d.ctypes.data, inc.ctypes.data, C.byref(dist))
     return d, inc, dist.value
+
+
+def f32(v):
+    return np.ascontiguousarray(v, dtype=np.float32)
+
+
+def gltf_surface(color, normal, coat_normal, metallic, roughness, f0, f90, coat, coat_roughness,
+                 reflection_normal=(0, 0, 0)):
+    return f32(list(color) + list(normal) + list(coat_normal) + [metallic, roughness] + list(f0) +
+               [f90, coat, coat_roughness] + list(reflection_normal))
+
+
+def gltf_eval(surface, view, light, diffuse_pdf):
+    """(f x cos (3), pdf, (diffuse, specular, coat) weights) of evaluateGltfPbrCos."""
+    value = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    weights = np.zeros(3, np.float32)
+    s, v, l = f32(surface), f32(view), f32(light)
+    library().oracle_gltf_eval(s.ctypes.data, v.ctypes.data, l.ctypes.data, diffuse_pdf, value.ctypes.data,
+                               pdf.ctypes.data, weights.ctypes.data)
+    return value, float(pdf[0]), weights
+
+
+def gltf_sample_reflection(normal, view, roughness, u, v):
+    out = np.zeros(3, np.float32)
+    n, w = f32(normal), f32(view)
+    library().oracle_gltf_sample_reflection(n.ctypes.data, w.ctypes.data, roughness, u, v, out.ctypes.data)
+    return out
+
+
+def gltf_reflection_normal(normal, geometric, view):
+    out = np.zeros(3, np.float32)
+    n, g, v = f32(normal), f32(geometric), f32(view)
+    library().oracle_gltf_reflection_normal(n.ctypes.data, g.ctypes.data, v.ctypes.data, out.ctypes.data)
+    return out
 
 
 def dielectric_reflectance(cosine: float, eta: float) -> float:
@@ -235,6 +274,16 @@ class Oracle:
         library().oracle_diff_directional64(self._h, C.byref(camera), C.byref(cfg), params.ctypes.data,
                                             direction.ctypes.data, sample_start, threads, out.ctypes.data)
         return out
+
+    def material_terms(self, material, uv=(0.0, 0.0), uv1=(0.0, 0.0)):
+        """Texture-decoded terms of a material at (uv, uv1): a dict."""
+        out = np.zeros(25, np.float32)
+        a, b = f32(uv), f32(uv1)
+        library().oracle_material_terms(self._h, material, a.ctypes.data, b.ctypes.data, out.ctypes.data)
+        return {"albedo": out[0:3], "opacity": out[3], "emission": out[4:7], "metallic": out[7],
+                "roughness": out[8], "specular_f0": out[9:12], "thickness": out[12], "normal": out[13:16],
+                "dielectric_f0": out[16:19], "f90": out[19], "coat": out[20], "coat_roughness": out[21],
+                "coat_normal": out[22:25]}
 
     def env_tables(self):
         """(probabilities, alias probability, alias index) of the scene's map."""

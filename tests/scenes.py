@@ -201,3 +201,145 @@ def env_slab(floor_material, env, half=0.1):
 
 def look_down(height=0.5, fov=0.05):
     return camera((0, height, 0), (0, -1, 0), up_hint=(0, 0, 1), fov=fov)
+
+
+# ---- textured and glTF materials (device_traversal.cuh:325-525, gltf_pbr.cuh)
+
+def checker_texture(size=16, cells=4, alpha=True, seed=0):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:size, 0:size]
+    on = ((x * cells // size) + (y * cells // size)) % 2
+    t = np.zeros((size, size, 4), np.uint8)
+    t[..., 0] = np.where(on, 230, 40)
+    t[..., 1] = np.where(on, 180, 90)
+    t[..., 2] = rng.integers(0, 256, (size, size))
+    t[..., 3] = np.where((x // 4 + y // 4) % 3 == 0, 30, 255) if alpha else 255
+    return t
+
+
+def bump_normal_map(size=16):
+    import numpy as np
+    y, x = np.mgrid[0:size, 0:size]
+    nx = 0.4 * np.sin(2 * np.pi * x / size)
+    ny = 0.3 * np.cos(2 * np.pi * y / size)
+    n = np.stack([nx, ny, np.ones_like(nx)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    t = np.zeros((size, size, 4), np.uint8)
+    t[..., :3] = np.clip((n * 0.5 + 0.5) * 255 + 0.5, 0, 255).astype(np.uint8)
+    t[..., 3] = 255
+    return t
+
+
+def textured_quad(builder, a, b, c, d, material, uv_scale=1.0, tangent=(1, 0, 0), bitangent=(0, 0, 1), uvs1=False,
+                  vertex_tangents=False, normal=None):
+    """Quad a-b-c-d with texture coordinates (0,0),(1,0),(1,1),(0,1) x scale."""
+    s = uv_scale
+    uv = ((0, 0), (s, 0), (s, s), (0, s))
+    n = None if normal is None else (normal, normal, normal)
+    tris = []
+    for (i, j, k) in ((0, 1, 2), (0, 2, 3)):
+        p = (a, b, c, d)
+        kw = dict(normals=n, uvs=(uv[i], uv[j], uv[k]), tangent=tangent, bitangent=bitangent)
+        if uvs1:
+            kw["uvs1"] = ((uv[i][1], uv[i][0]), (uv[j][1], uv[j][0]), (uv[k][1], uv[k][0]))
+            kw["tangent1"], kw["bitangent1"] = bitangent, tangent
+        if vertex_tangents:
+            kw["vertex_tangents"] = (tangent, tangent, tangent)
+            kw["tangent_signs"] = (1.0, 1.0, -1.0) if (i, j, k) == (0, 2, 3) else (1.0, 1.0, 1.0)
+        tris.append(wpt.make_triangle(p[i], p[j], p[k], material, **kw))
+    builder.static_triangles.extend(tris)
+    return tris
+
+
+def textured_room(with_env=True, with_lights=True):
+    """A floor, walls and objects covering every texture slot and the glTF
+    lobes: tiled albedo with alpha, masked foliage, normal maps (first and
+    second texture sets, vertex tangents), emissive, metallic-roughness and
+    specular-glossiness textures, thickness under glass, clearcoat and
+    specular extensions, delta glTF lobes, and a textured instanced mesh."""
+    import numpy as np
+    b = wpt.SceneBuilder()
+    checker = b.texture(checker_texture(16, 4, alpha=True, seed=1))
+    opaque = b.texture(checker_texture(8, 2, alpha=False, seed=2))
+    bumps = b.texture(bump_normal_map(16))
+    rough = b.texture(checker_texture(4, 2, alpha=False, seed=3))
+    T = wpt.texture_binding
+    floor = b.material(wpt.make_material(wpt.DIFFUSE, (0.8, 0.8, 0.8)),
+                       wpt.make_material_ext(albedo_texture=T(opaque, scale=(3.0, 2.0), rotation=0.3, transform=True),
+                                             normal_texture=T(bumps), normal_scale=0.8))
+    leaves = b.material(wpt.make_material(wpt.DIFFUSE, (0.6, 0.9, 0.5), alpha_masked=True, alpha_cutoff=0.5),
+                        wpt.make_material_ext(albedo_texture=checker))
+    lamp = b.material(wpt.make_material(wpt.EMISSIVE, emission=(6.0, 5.0, 4.0)),
+                      wpt.make_material_ext(emissive_texture=opaque))
+    metal = b.material(wpt.make_material(wpt.DIFFUSE, (0.9, 0.7, 0.5), roughness=0.6, metallic=1.0),
+                       wpt.make_material_ext(metallic_roughness_texture=rough))
+    specgloss = b.material(wpt.make_material(wpt.SPECULAR_GLOSSINESS, (0.5, 0.6, 0.7), specular=(0.9, 0.8, 0.7),
+                                             glossiness=0.7),
+                           wpt.make_material_ext(specular_glossiness_workflow=True, albedo_texture=opaque,
+                                                 specular_glossiness_texture=checker))
+    glass = b.material(wpt.make_material(wpt.DIELECTRIC, (0.95, 0.9, 0.85), ior=1.45,
+                                         attenuation_color=(0.7, 0.8, 0.9), attenuation_distance=0.8,
+                                         volume_density=1.5),
+                       wpt.make_material_ext(thickness_texture=rough, albedo_texture=opaque))
+    coat = b.material(wpt.make_material(wpt.DIFFUSE, (0.7, 0.1, 0.1), roughness=0.45, ior=1.5),
+                      wpt.make_material_ext(gltf_pbr_workflow=True, clearcoat=0.8, clearcoat_roughness=0.2,
+                                            specular_weight=0.7, specular_color=(1.0, 0.9, 0.8),
+                                            clearcoat_normal_texture=bumps, albedo_texture=opaque,
+                                            specular_color_texture=checker, specular_weight_texture=rough))
+    chrome = b.material(wpt.make_material(wpt.DIFFUSE, (0.9, 0.9, 0.95), roughness=0.0, metallic=1.0, ior=1.5),
+                        wpt.make_material_ext(gltf_pbr_workflow=True, clearcoat=1.0, clearcoat_roughness=0.0))
+    paint = b.material(wpt.make_material(wpt.DIFFUSE, (0.2, 0.5, 0.3), roughness=0.7, metallic=0.3, ior=1.5),
+                       wpt.make_material_ext(gltf_pbr_workflow=True, metallic_roughness_texture=rough,
+                                             normal_texture=T(bumps, texcoord=1), normal_scale=1.2,
+                                             clearcoat_texture=checker, clearcoat=0.5, clearcoat_roughness=0.0,
+                                             clearcoat_roughness_texture=rough))
+    plain = b.material(wpt.make_material(wpt.DIFFUSE, (0.5, 0.5, 0.5)))
+    h = 3.0
+    textured_quad(b, (-h, 0, -h), (h, 0, -h), (h, 0, h), (-h, 0, h), floor, uv_scale=2.0, vertex_tangents=True)
+    textured_quad(b, (-h, 0, h), (h, 0, h), (h, 3, h), (-h, 3, h), coat, tangent=(1, 0, 0), bitangent=(0, 1, 0),
+                  uvs1=True)
+    textured_quad(b, (-h, 0, -h), (-h, 0, h), (-h, 3, h), (-h, 3, -h), paint, tangent=(0, 0, 1), bitangent=(0, 1, 0),
+                  uvs1=True, vertex_tangents=True)
+    textured_quad(b, (-1.0, 0.2, 0.5), (1.0, 0.2, 0.5), (1.0, 1.6, 0.5), (-1.0, 1.6, 0.5), leaves, uv_scale=1.5)
+    textured_quad(b, (-0.6, 2.9, -0.6), (0.6, 2.9, -0.6), (0.6, 2.9, 0.6), (-0.6, 2.9, 0.6), lamp)
+    b.sphere((1.4, 0.6, -0.8), 0.6, metal)
+    b.sphere((-1.3, 0.5, -1.0), 0.5, glass)
+    b.sphere((0.3, 0.35, -1.6), 0.35, specgloss)
+    b.sphere((2.2, 0.4, 1.2), 0.4, chrome)
+    # A textured mesh, instanced twice with a rotation and a scale.
+    tris = []
+    for (p, q, r) in (((0, 0, 0), (1, 0, 0), (0, 1, 0)), ((1, 0, 0), (1, 1, 0), (0, 1, 0))):
+        tris.append(wpt.make_triangle(p, q, r, paint, uvs=((p[0], p[1]), (q[0], q[1]), (r[0], r[1])),
+                                      tangent=(1, 0, 0), bitangent=(0, 1, 0), uvs1=((p[1], p[0]), (q[1], q[0]), (r[1], r[0])),
+                                      tangent1=(0, 1, 0), bitangent1=(1, 0, 0),
+                                      vertex_tangents=((1, 0.1, 0), (1, 0, 0.1), (0.9, 0.1, 0)),
+                                      tangent_signs=(1.0, -1.0, 1.0)))
+    m = b.mesh(tris)
+    c, s_ = np.cos(0.6), np.sin(0.6)
+    t1 = wpt.scaled_transform((0.5, 0.01, 1.8), 0.8)
+    rot = wpt.Transform()
+    for i in range(3):
+        for j in range(3):
+            rot.local_to_world[i][j] = [[c, 0, -s_], [0, 1, 0], [s_, 0, c]][i][j]
+            rot.world_to_local[i][j] = [[c, 0, s_], [0, 1, 0], [-s_, 0, c]][i][j]
+    rot.translation = wpt.vec3((-2.0, 0.3, 1.0))
+    b.instance(m, t1)
+    b.instance(m, rot, chrome)
+    b.instance(m, rot, wpt.INVALID, wpt.ACCEL_DIRECT)
+    if with_lights:
+        b.punctual_light(wpt.make_punctual_light(wpt.LIGHT_POINT, (0.5, 2.0, 1.0), intensity=(2, 2, 2)))
+        b.punctual_light(wpt.make_punctual_light(wpt.LIGHT_DIRECTIONAL, direction=(0.2, -1, 0.3),
+                                                 intensity=(0.5, 0.5, 0.5)))
+    if with_env:
+        env = np.zeros((16, 32, 3), np.float32) + 0.3
+        env[3, 10] = 60.0
+        b.environment(env, 1.0, 20.0)
+    else:
+        b.black_background = True
+    b.sphere((-2.4, 0.25, -2.4), 0.25, plain)
+    return b
+
+
+def room_camera(aspect=1.0):
+    return camera((0.2, 1.6, 2.7), (-0.05, -0.3, -1), fov=1.1, aspect=aspect)

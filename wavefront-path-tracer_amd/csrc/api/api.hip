@@ -322,6 +322,19 @@ static int createScene(const wpt_scene_desc* desc, wpt_scene** out, bool upload)
         check(a.upload(h.punctual, &d.punctual), "punctual lights");
         d.punctualCount = static_cast<uint32_t>(h.punctual.size());
         d.directionalCount = h.directionalCount;
+        std::vector<DevTexture> textures(h.textureTexels.size());
+        for (size_t t = 0; t < textures.size(); ++t) {
+            check(a.upload(h.textureTexels[t], &textures[t].texels), "texture texels");
+            textures[t].width = h.textureWidth[t];
+            textures[t].height = h.textureHeight[t];
+        }
+        check(a.upload(textures, &d.textures), "textures");
+        d.textureCount = static_cast<uint32_t>(textures.size());
+        check(a.upload(h.materialExt, &d.materialExt), "material texture bindings");
+        check(a.upload(h.triExtra, &d.triExtra), "triangle texture data");
+        d.useNormalMaps = h.useNormalMaps;
+        d.normalMapMinimumCosine = h.normalMapMinimumCosine;
+        d.fullFeatures = h.fullFeatures() ? 1u : 0u;
         if (status != WPT_OK) {
             delete s;
             return status;
@@ -549,7 +562,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     // Trace, shade and shadow kernels run one item per thread over
     // gridBlocks workgroups; the persistent kernel runs a resident grid.
     Occupancy occ{};
-    R_HIP(kernelOccupancy(c.block_size, &occ));
+    R_HIP(kernelOccupancy(c.block_size, scene->dev.fullFeatures != 0, &occ));
     int device = 0, cus = 0, coop = 0;
     R_HIP(hipGetDevice(&device));
     R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -609,7 +622,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     R_HIP(hipMemset(ctx.framebuffer, 0, sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height));
     if (c.mode == WPT_MODE_REGENERATE) {
         int perCu = 0;
-        R_HIP(batchOccupancy(c.block_size, &perCu));
+        R_HIP(batchOccupancy(c.block_size, scene->dev.fullFeatures != 0, &perCu));
         if (perCu <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the regenerating kernel"));
         uint32_t resident = static_cast<uint32_t>(perCu) * cu;
         r->regenBlocks = c.persistent_blocks ? std::min(c.persistent_blocks, resident) : resident;
@@ -626,6 +639,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
         b.counters = static_cast<BatchCounters*>(p);
         R_HIP(hipMemset(p, 0, sizeof(BatchCounters)));
         b.out = ctx.framebuffer;
+        b.full = scene->dev.fullFeatures;
     }
     // One spill column per thread index of any grid.
     ctx.spillThreads = std::max(std::max(ctx.gridBlocks, ctx.persistentBlocks), r->regenBlocks) * c.block_size;
@@ -836,6 +850,7 @@ struct wpt_batch {
     size_t outCapacity = 0;  // floats
     uint2* spill = nullptr;
     uint32_t spillThreads = 0;
+    bool full = false;  // some scene needs the full-feature kernel
     int cus = 0;
     int wallClockKhz = 0;
     wpt_batch_stats last{};
@@ -874,7 +889,7 @@ int growDevice(T** p, size_t count, size_t* capacity) {
 // configuration asks for.
 int batchGrid(wpt_batch* b, const wpt_batch_config& c, uint32_t* blocks) {
     int perCu = 0;
-    WPT_HIP(batchOccupancy(c.block_size, &perCu));
+    WPT_HIP(batchOccupancy(c.block_size, b->full, &perCu));
     if (perCu <= 0) return fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the batch kernel");
     uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(b->cus);
     *blocks = c.blocks ? std::min(c.blocks, resident) : resident;
@@ -963,6 +978,7 @@ int runBatch(wpt_batch* b, uint32_t n, const wpt_batch_config& c, float* ms) {
     ctx.out = b->out;
     ctx.spill = b->spill;
     ctx.spillThreads = b->spillThreads;
+    ctx.full = b->full ? 1u : 0u;
     WPT_HIP(hipEventRecord(b->start, b->stream));
     WPT_HIP(enqueueBatch(ctx, blocks, c.block_size, b->stream));
     WPT_HIP(hipEventRecord(b->end, b->stream));
@@ -993,6 +1009,7 @@ WPT_API int wpt_batch_create(wpt_scene* const* scenes, uint32_t sceneCount, cons
     wpt_batch* b = new (std::nothrow) wpt_batch();
     if (!b) return fail(WPT_ERROR_OUT_OF_MEMORY, "host allocation failed");
     b->scenes.assign(scenes, scenes + sceneCount);
+    for (uint32_t i = 0; i < sceneCount; ++i) b->full = b->full || scenes[i]->dev.fullFeatures;
     b->config = *config;
     auto cleanup = [&](int status) {
         delete b;
@@ -1079,7 +1096,7 @@ WPT_API int wpt_batch_autotune(wpt_batch* b, const wpt_batch_task* tasks, uint32
     float bestMs = std::numeric_limits<float>::infinity();
     for (uint32_t block : {64u, 128u, 256u, 512u}) {
         int perCu = 0;
-        WPT_HIP(batchOccupancy(block, &perCu));
+        WPT_HIP(batchOccupancy(block, b->full, &perCu));
         if (perCu <= 0) continue;
         uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(b->cus);
         for (uint32_t quarter = 1; quarter <= 4; ++quarter) {
@@ -1127,8 +1144,9 @@ WPT_API int wpt_diff_create(wpt_scene* scene, const wpt_camera* camera, const wp
     if (!scene || !camera || !config || !out) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (!scene->onDevice) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene is not on a device");
-    if (scene->host.envTexels.size())
-        return fail(WPT_ERROR_UNSUPPORTED, "the differentiable renderer does not support environment maps");
+    if (scene->dev.fullFeatures)
+        return fail(WPT_ERROR_UNSUPPORTED,
+            "the differentiable renderer supports neither environment maps, punctual lights nor texture bindings");
     const wpt_diff_config& c = *config;
     if (c.width == 0 || c.height == 0 || c.samples_per_pixel == 0 || c.max_depth == 0)
         return fail(WPT_ERROR_INVALID_ARGUMENT, "image size, samples per pixel and depth must be greater than zero");

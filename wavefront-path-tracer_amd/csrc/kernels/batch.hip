@@ -47,6 +47,7 @@ __global__ void __launch_bounds__(512) kBatchReset(BatchContext ctx) {
     }
 }
 
+template <bool Full>
 __global__ void __launch_bounds__(512) kBatch(BatchContext ctx) {
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const uint32_t n = ctx.taskCount;
@@ -111,10 +112,10 @@ __global__ void __launch_bounds__(512) kBatch(BatchContext ctx) {
         if (busy) {
             const DevScene& scene = ctx.scenes[T.scene];
             ++closest;
-            HitRecord h = traceClosest(scene, path.ray, stack);
+            HitRecord h = traceClosest<Full>(scene, path.ray, stack);
             ShadeOutcome outcome;
             InlineOcclusion<LdsStack> occ{stack, 0u};
-            shadePath(scene, path, h, T.shade, occ, outcome);
+            shadePath<Full>(scene, path, h, T.shade, occ, outcome);
             shadow += occ.queries;
             if (!outcome.alive) {
                 sum += outcome.isDebug ? outcome.debugColor : path.radiance;
@@ -149,13 +150,18 @@ hipError_t enqueueBatch(const BatchContext& ctx, uint32_t blocks, uint32_t block
     hipLaunchKernelGGL(kBatchReset, dim3(resetBlocks > 0 ? resetBlocks : 1u), dim3(256), 0, stream, ctx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ctx.taskCount == 0) return e;
-    hipLaunchKernelGGL(kBatch, dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream, ctx);
+    if (ctx.full)
+        hipLaunchKernelGGL(kBatch<true>, dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream, ctx);
+    else
+        hipLaunchKernelGGL(kBatch<false>, dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream, ctx);
     return hipGetLastError();
 }
 
-hipError_t batchOccupancy(uint32_t blockSize, int* blocksPerCu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch, static_cast<int>(blockSize),
-        stackLdsBytes(blockSize));
+hipError_t batchOccupancy(uint32_t blockSize, bool full, int* blocksPerCu) {
+    return full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch<true>, static_cast<int>(blockSize),
+                      stackLdsBytes(blockSize))
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch<false>,
+                      static_cast<int>(blockSize), stackLdsBytes(blockSize));
 }
 
 }  // namespace wpt

@@ -180,9 +180,9 @@ WPT_HD Vec3 diffPath(const DevScene& s, const DiffLayout& l, const DiffParams& p
     PathState path = primaryPath(cam, x, y, width, height, sample, strategy);
     Vec3 acc(0.0f, 0.0f, 0.0f);
     for (uint32_t depth = 0; depth < maxDepth; ++depth) {
-        HitRecord h = traceClosest(s, path.ray, stack);
+        HitRecord h = traceClosest<false>(s, path.ray, stack);
         if (h.kind == kHitNone) break;
-        SurfaceHit hit = surfaceAttributes(s, path.ray, h);
+        SurfaceHit hit = surfaceAttributes<false>(s, path.ray, h);
         const Material& m = s.materials[hit.material];
         if (m.type == kEmissive) {
             float misWeight = 1.0f;
@@ -211,7 +211,7 @@ WPT_HD Vec3 diffPath(const DevScene& s, const DiffLayout& l, const DiffParams& p
             // at creation), so the interface needs no escape tests.
             ShadeParams interfaceParams{maxDepth, maxDepth, strategy, 0.0f, 0u, 0u};
             DeferOcclusion noEscapeTests;
-            direction = continueDielectric(s, path, hit, n, m, interfaceParams, noEscapeTests);
+            direction = continueDielectric<false>(s, path, hit, n, m, m.albedo, interfaceParams, noEscapeTests);
         } else {
             Vec3 wo = -path.ray.direction;
             DiffSurface surf = diffSurface(s, l, p, h, hit, m);
@@ -240,7 +240,7 @@ WPT_HD Vec3 diffPath(const DevScene& s, const DiffLayout& l, const DiffParams& p
                     Vec3 segment = shadowTarget - shadowOrigin;
                     float shadowDistance = length(segment);
                     if (shadowDistance > kRayMinDistance &&
-                        !traceOccluded(s, Ray(shadowOrigin, segment / shadowDistance), shadowDistance - kRayMinDistance,
+                        !traceOccluded<false>(s, Ray(shadowOrigin, segment / shadowDistance), shadowDistance - kRayMinDistance,
                             stack)) {
                         float geometryTerm = lightCosine * light.area / (distanceSquared * light.selectionPdf);
                         float lightPdf = light.selectionPdf * distanceSquared / (lightCosine * light.area);

@@ -36,6 +36,21 @@ struct SurfaceHit {
     uint32_t lightIndex;
 };
 
+// Barycentric texture coordinates of the first and second sets
+// (getTriangleUv, device_traversal.cuh:120-132); (0, 0) when absent.
+WPT_HD Vec2 triangleUv(const TriShade& tri, float u, float v) {
+    if (!(tri.flags & kTriHasTexcoords)) return Vec2{0.0f, 0.0f};
+    float w = 1.0f - u - v;
+    return Vec2{w * tri.uv0.x + u * tri.uv1.x + v * tri.uv2.x, w * tri.uv0.y + u * tri.uv1.y + v * tri.uv2.y};
+}
+
+WPT_HD Vec2 triangleUv1(const DevScene& s, const TriShade& tri, float u, float v) {
+    if (!(tri.flags & kTriHasTexcoords1) || tri.extra == kInvalid) return Vec2{0.0f, 0.0f};
+    const TriExtra& e = s.triExtra[tri.extra];
+    float w = 1.0f - u - v;
+    return Vec2{w * e.uv1_0.x + u * e.uv1_1.x + v * e.uv1_2.x, w * e.uv1_0.y + u * e.uv1_1.y + v * e.uv1_2.y};
+}
+
 // ---- textures
 WPT_HD float4 unpackTexel(uint32_t p) {
     return float4{static_cast<float>(p & 255u) / 255.0f, static_cast<float>((p >> 8) & 255u) / 255.0f,
@@ -101,8 +116,9 @@ WPT_HD Vec3 gltfSrgbToLinear(const float4& t) {
 // ---- material lookups (device_traversal.cuh:359-500).  An untextured
 // material (ext == kInvalid) reads its constants: every texture term would be
 // the white fallback, which multiplies exactly to 1.
+template <bool Full = true>
 WPT_HD Vec3 materialAlbedo(const DevScene& s, const Material& m, const SurfaceHit& hit) {
-    if (m.ext == kInvalid) return m.albedo;
+    if (!Full || m.ext == kInvalid) return m.albedo;
     const MaterialExt& e = s.materialExt[m.ext];
     float4 texel = sampleTexture(s, e.albedo, hit.uv, hit.uv1, whiteTexel());
     return hadamard(m.albedo, e.specularGlossinessWorkflow ? gltfSrgbToLinear(texel) : srgbToLinear(texel));
@@ -115,33 +131,40 @@ WPT_HD float materialOpacity(const DevScene& s, const Material& m, const Vec2& u
     return m.opacity * sampleTexture(s, e.albedo, uv, uv1, whiteTexel()).w;
 }
 
+template <bool Full = true>
 WPT_HD Vec3 materialEmission(const DevScene& s, const Material& m, const SurfaceHit& hit) {
-    if (m.ext == kInvalid) return m.emission;
+    if (!Full || m.ext == kInvalid) return m.emission;
     return hadamard(m.emission, srgbToLinear(sampleTexture(s, s.materialExt[m.ext].emissive, hit.uv, hit.uv1,
         whiteTexel())));
 }
 
+template <bool Full = true>
 WPT_HD float materialMetallic(const DevScene& s, const Material& m, const SurfaceHit& hit) {
-    if (m.ext == kInvalid) return m.metallic;
+    if (!Full || m.ext == kInvalid) return m.metallic;
     return m.metallic * sampleTexture(s, s.materialExt[m.ext].metallicRoughness, hit.uv, hit.uv1, whiteTexel()).z;
 }
 
+template <bool Full = true>
 WPT_HD float materialRoughness(const DevScene& s, const Material& m, const SurfaceHit& hit) {
-    if (m.ext == kInvalid) return m.roughness;
+    if (!Full || m.ext == kInvalid) return m.roughness;
     const MaterialExt& e = s.materialExt[m.ext];
     if (e.specularGlossinessWorkflow)
         return 1.0f - m.glossiness * sampleTexture(s, e.specularGlossiness, hit.uv, hit.uv1, whiteTexel()).w;
     return m.roughness * sampleTexture(s, e.metallicRoughness, hit.uv, hit.uv1, whiteTexel()).y;
 }
 
+template <bool Full = true>
 WPT_HD float materialThickness(const DevScene& s, const Material& m, const SurfaceHit& hit) {
-    if (m.ext == kInvalid) return 1.0f;
+    if (!Full || m.ext == kInvalid) return 1.0f;
     const TexBinding& b = s.materialExt[m.ext].thickness;
     if (b.texture == kInvalid || b.texture >= s.textureCount) return 1.0f;
     return sampleTexture(s, b, hit.uv, hit.uv1, whiteTexel()).x;
 }
 
-WPT_HD bool finite3(const Vec3& v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+// Finite test that holds for host and device builds alike (false for NaN
+// and both infinities).
+WPT_HD bool finiteF(float x) { return fabsf(x) <= 3.402823466e38f; }
+WPT_HD bool finite3(const Vec3& v) { return finiteF(v.x) && finiteF(v.y) && finiteF(v.z); }
 
 // getMaterialNormal, device_traversal.cuh:452-500: the tangent-space normal
 // of `binding` (scaled in x and y) in the hit's re-orthogonalized tangent
@@ -179,8 +202,9 @@ WPT_HD Vec3 mappedNormal(const DevScene& s, const TexBinding& binding, float nor
     return normalize(normal);
 }
 
+template <bool Full = true>
 WPT_HD Vec3 materialNormal(const DevScene& s, const Material& m, const SurfaceHit& hit) {
-    if (m.ext == kInvalid) return hit.normal;
+    if (!Full || m.ext == kInvalid) return hit.normal;
     const MaterialExt& e = s.materialExt[m.ext];
     return mappedNormal(s, e.normal, e.normalScale, hit);
 }
@@ -260,10 +284,11 @@ WPT_HD Vec3 sampleSpecGloss(const SpecGlossSurface& s, const Vec3& n, const Vec3
 
 // getSpecGlossSurface, device_traversal.cuh:396-401, and the metal lobe of
 // evaluateSurface (:432-436).
+template <bool Full = true>
 WPT_HD SpecGlossSurface specGlossSurface(const DevScene& s, const Material& m, const SurfaceHit& hit,
     const Vec3& albedo, float roughness) {
     Vec3 f0 = m.specular;
-    if (m.ext != kInvalid)
+    if (Full && m.ext != kInvalid)
         f0 = hadamard(m.specular, gltfSrgbToLinear(sampleTexture(s, s.materialExt[m.ext].specularGlossiness, hit.uv,
             hit.uv1, whiteTexel())));
     return SpecGlossSurface{albedo, f0, roughness};

@@ -1,0 +1,130 @@
+// Photon map for caustics: the photons that reached a diffuse surface after
+// at least one specular bounce, binned in a uniform grid, and the gather
+// that turns them into irradiance at a shading point.
+//
+// Reference: src/renderer/photon_mapping.cu:4-163 (emitPhotons,
+// tracePhotons, buildPhotonGrid, gatherCaustics), render_driver.cu:180-234
+// (grid bounds and resolution, photon seed).
+//
+// The reference bins photons into per-cell linked lists with atomic
+// exchanges, so the order in which a cell's photons are visited (and with
+// it the summation order of the gather) varies from run to run.  Here the
+// photons are sorted by (cell, photon index) and each cell is a contiguous
+// range visited in photon order, which makes the gather deterministic.
+#pragma once
+
+#include "../scene/device_scene.h"
+
+namespace wpt {
+
+// A stored photon: where it landed, the direction back toward where it came
+// from, and its flux.
+struct Photon {
+    Vec3 position;
+    Vec3 direction;
+    Vec3 flux;
+    uint32_t valid;
+    uint32_t index;  // photon number, the secondary sort key
+    uint32_t pad;
+};
+
+// Grid over [minimum, maximum] with resolution^3 cells; photons sorted by
+// cell, cell c holding photons[cellStart[c] .. cellEnd[c]).
+struct PhotonGridView {
+    const Photon* photons;
+    const uint32_t* cellStart;
+    const uint32_t* cellEnd;
+    uint32_t resolution;
+    float radius;  // gather radius floor
+    Vec3 minimum;
+    Vec3 maximum;
+};
+
+// Cell of a position, clamped per axis; `inside` is false outside the box
+// (buildPhotonGrid, photon_mapping.cu:81-93).
+WPT_HD void photonCell(const PhotonGridView& g, const Vec3& p, int& x, int& y, int& z, bool& inside) {
+    Vec3 extent = g.maximum - g.minimum;
+    Vec3 offset = p - g.minimum;
+    Vec3 n(offset.x / extent.x, offset.y / extent.y, offset.z / extent.z);
+    int r = static_cast<int>(g.resolution);
+    x = static_cast<int>(n.x * static_cast<float>(g.resolution));
+    y = static_cast<int>(n.y * static_cast<float>(g.resolution));
+    z = static_cast<int>(n.z * static_cast<float>(g.resolution));
+    x = x < 0 ? 0 : (x > r - 1 ? r - 1 : x);
+    y = y < 0 ? 0 : (y > r - 1 ? r - 1 : y);
+    z = z < 0 ? 0 : (z > r - 1 ? r - 1 : z);
+    inside = !(n.x < 0.0f || n.y < 0.0f || n.z < 0.0f || n.x > 1.0f || n.y > 1.0f || n.z > 1.0f);
+}
+
+// Irradiance estimate at a point (gatherCaustics, photon_mapping.cu:96-163):
+// the up to 64 nearest photons within max(4 radius, 0.1) that arrive from
+// the normal's side, cone-filtered over the farthest kept photon's distance
+// (at least `radius`; the whole search radius below 16 photons).
+WPT_HD Vec3 gatherCaustics(const PhotonGridView& g, const Vec3& position, const Vec3& normal) {
+    constexpr uint32_t kNearest = 64;
+    constexpr uint32_t kMinimum = 16;
+    int cx, cy, cz;
+    bool inside;
+    photonCell(g, position, cx, cy, cz, inside);
+    Vec3 extent = g.maximum - g.minimum;
+    float searchRadius = fmaxf(g.radius * 4.0f, 0.1f);
+    int cellRadius = static_cast<int>(ceilf(searchRadius * static_cast<float>(g.resolution) / fmaxf(extent.x, 0.0001f)));
+    cellRadius = cellRadius > 1 ? cellRadius : 1;
+    float searchRadiusSquared = searchRadius * searchRadius;
+    float distances[kNearest];
+    uint32_t indices[kNearest];
+    uint32_t count = 0, farthest = 0;
+    float farthestSquared = 0.0f;
+    int res = static_cast<int>(g.resolution);
+    for (int z = cz - cellRadius; z <= cz + cellRadius; ++z)
+        for (int y = cy - cellRadius; y <= cy + cellRadius; ++y)
+            for (int x = cx - cellRadius; x <= cx + cellRadius; ++x) {
+                if (x < 0 || y < 0 || z < 0 || x >= res || y >= res || z >= res) continue;
+                uint32_t cell = (static_cast<uint32_t>(z) * g.resolution + static_cast<uint32_t>(y)) * g.resolution +
+                    static_cast<uint32_t>(x);
+                for (uint32_t i = g.cellStart[cell], end = g.cellEnd[cell]; i < end; ++i) {
+                    const Photon& photon = g.photons[i];
+                    Vec3 d = photon.position - position;
+                    float distanceSquared = dot(d, d);
+                    float cosine = dot(normal, photon.direction);
+                    if (distanceSquared > searchRadiusSquared || cosine <= 0.0f) continue;
+                    if (count < kNearest) {
+                        distances[count] = distanceSquared;
+                        indices[count] = i;
+                        if (distanceSquared > farthestSquared) {
+                            farthestSquared = distanceSquared;
+                            farthest = count;
+                        }
+                        ++count;
+                    } else if (distanceSquared < farthestSquared) {
+                        distances[farthest] = distanceSquared;
+                        indices[farthest] = i;
+                        farthest = 0;
+                        farthestSquared = distances[0];
+                        for (uint32_t k = 1; k < kNearest; ++k)
+                            if (distances[k] > farthestSquared) {
+                                farthestSquared = distances[k];
+                                farthest = k;
+                            }
+                    }
+                }
+            }
+    if (count == 0) return Vec3(0.0f, 0.0f, 0.0f);
+    float gatherSquared = searchRadiusSquared;
+    if (count >= kMinimum) {
+        gatherSquared = 0.0f;
+        for (uint32_t k = 0; k < count; ++k) gatherSquared = fmaxf(gatherSquared, distances[k]);
+        gatherSquared = fmaxf(gatherSquared, g.radius * g.radius);
+    }
+    float gatherRadius = sqrtf(gatherSquared);
+    Vec3 result(0.0f, 0.0f, 0.0f);
+    for (uint32_t k = 0; k < count; ++k) {
+        const Photon& photon = g.photons[indices[k]];
+        float distance = sqrtf(distances[k]);
+        float cone = fmaxf(0.0f, 1.0f - distance / gatherRadius);
+        result += photon.flux * (cone * dot(normal, photon.direction));
+    }
+    return result * (3.0f / (3.14159265359f * gatherSquared));
+}
+
+}  // namespace wpt

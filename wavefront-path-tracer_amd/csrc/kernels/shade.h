@@ -1,28 +1,27 @@
-// Path shading: surface attributes of the winning hit, next-event estimation,
-// BSDF continuation, Russian roulette and radiance accumulation.
+// Path shading: surface attributes of the winning hit, next-event estimation
+// toward triangle, punctual and environment lights, BSDF continuation,
+// Russian roulette and radiance accumulation.
 //
 // Reference: include/renderer/device_path_shading.cuh:6-173 (shadePaths),
-// device_traversal.cuh:100-180 (hit attributes), :319-441 (material lookup,
-// surface evaluation), :529-569 (sampleDirectLight), :571-635 (sky and
-// dielectric continuation), :685-725 (diffuse/glossy continuation),
-// device_sampling.cuh:6-93 (sampling, Fresnel), specular_glossiness.cuh:13-80
-// (GGX spec-gloss), path_state.cuh:33-43 (indirect clamp).
+// device_traversal.cuh:100-180 (hit attributes), :403-441 (environment
+// mixture, surface evaluation), :529-569 (sampleDirectLight), :571-635 (sky,
+// environment and dielectric continuation), :637-683 (environment and
+// punctual NEE), :685-787 (diffuse, glossy and glTF continuations),
+// device_sampling.cuh:6-93 (sampling, Fresnel), path_state.cuh:33-43
+// (indirect clamp).  Materials and their BSDF lobes are in material.h.
 //
 // The random numbers are drawn in the reference's order; where the reference
 // leaves the order unspecified (the three draws of randomInUnitSphere are
-// constructor arguments) they are drawn x, y, z.  Materials here are
-// untextured: texture fallbacks of 1.0 reduce the reference's texture terms
-// to the material constants exactly.
+// constructor arguments) they are drawn x, y, z.
 #pragma once
 
 #include "../common/rng.h"
 #include "../scene/environment.h"
+#include "material.h"
+#include "photon.h"
 #include "traverse.h"
 
 namespace wpt {
-
-constexpr float kInvPi = 0.31830988618f;
-constexpr float kTwoPi = 6.28318530718f;
 
 struct PathState {
     Ray ray;
@@ -46,17 +45,7 @@ struct ShadeParams {
     float clamp;
     uint32_t intersectionDebug;
     uint32_t shadingNormalDebug;
-};
-
-// Reference Hit (geometry.cuh:68-82), the fields the untextured path reads.
-struct SurfaceHit {
-    float t;
-    Vec3 position;
-    Vec3 geometricNormal;
-    Vec3 normal;
-    Vec2 uv;
-    uint32_t material;
-    uint32_t lightIndex;
+    const PhotonGridView* photons;  // caustic photon map gathered at diffuse vertices, or null
 };
 
 WPT_HD float rand01(PathState& p, uint32_t strategy) { return rngNextFloat(p.rng, strategy); }
@@ -91,17 +80,41 @@ WPT_HD Vec3 triangleGeometricNormal(const TriShade& tri, const Ray& ray, bool fa
     return faceForward && dot(normal, ray.direction) > 0.0f ? -normal : normal;
 }
 
-WPT_HD Vec2 triangleUv(const TriShade& tri, float u, float v) {
-    if (!(tri.flags & kTriHasTexcoords)) return Vec2{0.0f, 0.0f};
-    float w = 1.0f - u - v;
-    return Vec2{w * tri.uv0.x + u * tri.uv1.x + v * tri.uv2.x, w * tri.uv0.y + u * tri.uv1.y + v * tri.uv2.y};
+// Texture frames of a textured triangle hit, in the triangle's space
+// (intersectTriangleHit, device_traversal.cuh:159-176): interpolated vertex
+// tangents made orthogonal to the shading normal, with the handedness sign
+// giving the bitangent, else the face frame.
+WPT_HD void triangleFrames(const DevScene& s, const TriShade& tri, float u, float v, const Vec3& normal,
+    SurfaceHit& hit) {
+    hit.uv1 = triangleUv1(s, tri, u, v);
+    hit.tangent = tri.tangent;
+    hit.bitangent = tri.bitangent;
+    hit.tangent1 = Vec3();
+    hit.bitangent1 = Vec3();
+    if (tri.extra == kInvalid) return;
+    const TriExtra& e = s.triExtra[tri.extra];
+    if (tri.flags & kTriHasVertexTangents) {
+        float w = 1.0f - u - v;
+        Vec3 tangent = w * e.vertexTangent0 + u * e.vertexTangent1 + v * e.vertexTangent2;
+        tangent = tangent - normal * dot(normal, tangent);
+        if (lengthSquared(tangent) > 0.00000001f) {
+            hit.tangent = normalize(tangent);
+            float sign = w * e.sign0 + u * e.sign1 + v * e.sign2;
+            hit.bitangent = (sign < 0.0f ? -1.0f : 1.0f) * normalize(cross(normal, hit.tangent));
+        }
+    }
+    hit.tangent1 = e.tangent1;
+    hit.bitangent1 = e.bitangent1;
 }
 
+template <bool Full = true>
 WPT_HD SurfaceHit surfaceAttributes(const DevScene& s, const Ray& ray, const HitRecord& h) {
     SurfaceHit hit;
     hit.t = h.t;
     hit.material = h.material;
     hit.uv = Vec2{0.0f, 0.0f};
+    hit.uv1 = Vec2{0.0f, 0.0f};
+    hit.tangent = hit.bitangent = hit.tangent1 = hit.bitangent1 = Vec3();
     hit.lightIndex = kInvalid;
     if (h.kind == kHitSphere) {
         float4 sp = s.spheres[h.prim];
@@ -121,13 +134,21 @@ WPT_HD SurfaceHit surfaceAttributes(const DevScene& s, const Ray& ray, const Hit
             tri = &s.flatTriangles[h.prim];
         }
         bool faceForward = s.materials[h.material].type != kDielectric;
+        bool textured = Full && s.materials[h.material].ext != kInvalid;
         hit.uv = triangleUv(*tri, h.u, h.v);
         hit.geometricNormal = triangleGeometricNormal(*tri, local, faceForward);
         hit.normal = triangleShadingNormal(*tri, local, h.u, h.v, faceForward);
+        if (textured) triangleFrames(s, *tri, h.u, h.v, hit.normal, hit);
         if (inst) {
             hit.lightIndex = inst->materialOverride == kInvalid ? tri->lightIndex : kInvalid;
             hit.geometricNormal = transformNormal(inst->transform, hit.geometricNormal);
             hit.normal = transformNormal(inst->transform, hit.normal);
+            if (textured) {
+                hit.tangent = normalize(transformVector(inst->transform, hit.tangent));
+                hit.bitangent = normalize(transformVector(inst->transform, hit.bitangent));
+                hit.tangent1 = normalize(transformVector(inst->transform, hit.tangent1));
+                hit.bitangent1 = normalize(transformVector(inst->transform, hit.bitangent1));
+            }
             if (hit.lightIndex != kInvalid) hit.lightIndex += inst->lightOffset;
         } else {
             hit.lightIndex = h.kind == kHitFlat ? kInvalid : tri->lightIndex;
@@ -197,9 +218,9 @@ WPT_HD DielectricInterface dielectricInterface(const Vec3& incident, const Vec3&
     return r;
 }
 
-WPT_HD Vec3 sampleCosineHemisphere(const Vec3& normal, PathState& p, uint32_t strategy) {
-    float u1 = rand01(p, strategy);
-    float u2 = rand01(p, strategy);
+WPT_HD Vec3 cosineHemisphere(const Vec3& normal, uint64_t& rng, uint32_t strategy) {
+    float u1 = rngNextFloat(rng, strategy);
+    float u2 = rngNextFloat(rng, strategy);
     float radius = sqrtf(u1);
     float phi = kTwoPi * u2;
     float sinPhi, cosPhi;
@@ -213,6 +234,10 @@ WPT_HD Vec3 sampleCosineHemisphere(const Vec3& normal, PathState& p, uint32_t st
     return normalize(tangent * x + bitangent * y + normal * z);
 }
 
+WPT_HD Vec3 sampleCosineHemisphere(const Vec3& normal, PathState& p, uint32_t strategy) {
+    return cosineHemisphere(normal, p.rng, strategy);
+}
+
 WPT_HD Vec3 randomInUnitSphere(PathState& p, uint32_t strategy) {
     while (true) {
         float x = 2.0f * rand01(p, strategy) - 1.0f;
@@ -221,78 +246,6 @@ WPT_HD Vec3 randomInUnitSphere(PathState& p, uint32_t strategy) {
         Vec3 point(x, y, z);
         if (lengthSquared(point) < 1.0f) return point;
     }
-}
-
-struct SpecGlossSurface {
-    Vec3 diffuse;
-    Vec3 f0;
-    float roughness;
-};
-
-WPT_HD float specGlossMax(const Vec3& v) { return fmaxf(v.x, fmaxf(v.y, v.z)); }
-
-WPT_HD float specGlossProbability(const SpecGlossSurface& s) {
-    float specular = specGlossMax(s.f0);
-    float diffuse = specGlossMax(s.diffuse) * (1.0f - specular);
-    return fminf(0.95f, fmaxf(0.05f, specular / fmaxf(1.0e-6f, specular + diffuse)));
-}
-
-WPT_HD float specGlossAlpha(const SpecGlossSurface& s) {
-    float roughness = fmaxf(0.02f, s.roughness);
-    return roughness * roughness;
-}
-
-WPT_HD float specGlossD(float nh, float alpha) {
-    const float pi = 3.14159265359f;
-    float a2 = alpha * alpha;
-    float denominator = (1.0f - nh * nh) + a2 * nh * nh;
-    return a2 / (pi * denominator * denominator);
-}
-
-WPT_HD float specGlossG1(float cosine, float alpha) {
-    return 2.0f * cosine / (cosine + sqrtf(alpha * alpha + (1.0f - alpha * alpha) * cosine * cosine));
-}
-
-WPT_HD Vec3 evaluateSpecGloss(const SpecGlossSurface& s, const Vec3& n, const Vec3& outgoing, const Vec3& incoming,
-    float& pdf) {
-    pdf = 0.0f;
-    float nv = dot(n, outgoing), nl = dot(n, incoming);
-    Vec3 sum = outgoing + incoming;
-    if (nv <= 0.0f || nl <= 0.0f || lengthSquared(sum) < 1.0e-12f) return Vec3();
-    Vec3 half = normalize(sum);
-    float nh = fminf(1.0f, fmaxf(0.0f, dot(n, half)));
-    float vh = fminf(1.0f, fmaxf(1.0e-8f, dot(outgoing, half)));
-    float alpha = specGlossAlpha(s);
-    float d = specGlossD(nh, alpha);
-    float probability = specGlossProbability(s);
-    pdf = (1.0f - probability) * nl * kInvPi + probability * d * nh / (4.0f * vh);
-    float schlick = powF(1.0f - vh, 5.0f);
-    Vec3 fresnel = s.f0 + (Vec3(1.0f, 1.0f, 1.0f) - s.f0) * schlick;
-    Vec3 diffuse = s.diffuse * ((1.0f - specGlossMax(s.f0)) * kInvPi);
-    float g = specGlossG1(nv, alpha) * specGlossG1(nl, alpha);
-    return diffuse + fresnel * (d * g / (4.0f * nv * nl));
-}
-
-WPT_HD Vec3 specGlossBasis(const Vec3& n, float x, float y, float z) {
-    Vec3 helper = fabsf(n.x) > 0.1f ? Vec3(0.0f, 1.0f, 0.0f) : Vec3(1.0f, 0.0f, 0.0f);
-    Vec3 tangent = normalize(cross(helper, n));
-    return normalize(x * tangent + y * cross(n, tangent) + z * n);
-}
-
-WPT_HD Vec3 sampleSpecGloss(const SpecGlossSurface& s, const Vec3& n, const Vec3& outgoing, float choice, float u,
-    float v) {
-    float phi = kTwoPi * v;
-    float sinPhi, cosPhi;
-    sinCosF(phi, sinPhi, cosPhi);
-    if (choice < specGlossProbability(s)) {
-        float alpha = specGlossAlpha(s);
-        float cosine = sqrtf((1.0f - u) / (1.0f + (alpha * alpha - 1.0f) * u));
-        float sine = sqrtf(fmaxf(0.0f, 1.0f - cosine * cosine));
-        Vec3 half = specGlossBasis(n, sine * cosPhi, sine * sinPhi, cosine);
-        return 2.0f * dot(outgoing, half) * half - outgoing;
-    }
-    float radius = sqrtf(u);
-    return specGlossBasis(n, radius * cosPhi, radius * sinPhi, sqrtf(1.0f - u));
 }
 
 // ---- occlusion policies
@@ -329,15 +282,18 @@ struct InlineOcclusion {
 
 // ---- environment sampling and MIS (device_traversal.cuh:403-425, 571-587,
 // 637-651)
+template <bool Full = true>
 WPT_HD float diffuseEnvironmentProbability(const DevScene& s) {
+    if (!Full) return 0.0f;
     const EnvironmentMap& m = s.environment;
     return m.texels && m.intensity > 0.0f && m.width && m.height && m.probabilities && m.alias ? 0.5f : 0.0f;
 }
 
 // Mixture of cosine sampling and environment sampling (half each when a
 // usable map exists), the density the diffuse continuation samples.
+template <bool Full = true>
 WPT_HD float diffuseContinuationPdf(const DevScene& s, const Vec3& normal, const Vec3& direction) {
-    float probability = diffuseEnvironmentProbability(s);
+    float probability = diffuseEnvironmentProbability<Full>(s);
     float cosinePdf = fmaxf(0.0f, dot(normal, direction)) * 0.31830988618f;
     return probability > 0.0f ? (1.0f - probability) * cosinePdf +
         probability * environmentPdf(s.environment, direction) : cosinePdf;
@@ -352,49 +308,47 @@ WPT_HD EnvironmentSample samplePathEnvironment(const DevScene& s, PathState& pat
     return sampleEnvironment(s.environment, candidate, coin, u, v);
 }
 
-// ---- material evaluation (device_traversal.cuh:427-441, 519-527)
-WPT_HD bool isMetalLike(const Material& m) { return m.type == kMetal || (m.type == kDiffuse && m.metallic > 0.5f); }
-
-WPT_HD SpecGlossSurface glossySurface(const Material& m) {
-    SpecGlossSurface s;
-    if (m.type == kSpecularGlossiness) {
-        s.diffuse = m.albedo;
-        s.f0 = m.specular;
-    } else {
-        s.diffuse = Vec3();
-        s.f0 = m.albedo;
-    }
-    s.roughness = m.roughness;
-    return s;
-}
-
-WPT_HD Vec3 evaluateSurfaceCos(const DevScene& sc, const Material& m, const Vec3& shadingNormal, const Vec3& view,
-    const Vec3& light, float& pdf) {
-    Vec3 value;
-    if (m.type == kSpecularGlossiness) {
-        value = evaluateSpecGloss(glossySurface(m), shadingNormal, view, light, pdf);
-    } else if (isMetalLike(m)) {
-        if (m.roughness <= 0.0f) {
-            pdf = 0.0f;
-            value = Vec3();
+// ---- the BSDF at a vertex, as NEE evaluates it (evaluateSurfaceCos,
+// device_traversal.cuh:427-441, 519-527): f x cosine toward a light and
+// the density with which the continuation would have sampled it.
+//
+// Diffuse (Lambert with the environment-mixture density) and the GGX lobes
+// of metals and specular-glossiness materials; NEE skips lights below the
+// shading normal.
+template <bool Full>
+struct StandardBsdf {
+    static constexpr bool kOneSided = true;
+    static constexpr bool kFull = Full;
+    bool glossy;
+    Vec3 normal;
+    SpecGlossSurface sg;  // diffuse: sg.diffuse is the albedo
+    WPT_HD Vec3 evalCos(const DevScene& s, const Vec3& view, const Vec3& light, float& pdf) const {
+        Vec3 value;
+        if (glossy) {
+            value = evaluateSpecGloss(sg, normal, view, light, pdf);
         } else {
-            SpecGlossSurface s;
-            s.diffuse = Vec3();
-            s.f0 = m.albedo;
-            s.roughness = m.roughness;
-            value = evaluateSpecGloss(s, shadingNormal, view, light, pdf);
+            pdf = diffuseContinuationPdf<Full>(s, normal, light);
+            value = sg.diffuse * kInvPi;
         }
-    } else {
-        pdf = diffuseContinuationPdf(sc, shadingNormal, light);
-        value = m.albedo * kInvPi;
+        return value * fmaxf(0.0f, dot(normal, light));
     }
-    return value * fmaxf(0.0f, dot(shadingNormal, light));
-}
+};
+
+// glTF metallic-roughness with specular and clearcoat: evaluated for lights
+// below the shading normal too (the coat and reflection frames differ).
+struct GltfBsdf {
+    static constexpr bool kOneSided = false;
+    static constexpr bool kFull = true;
+    GltfSurface g;
+    WPT_HD Vec3 evalCos(const DevScene& s, const Vec3& view, const Vec3& light, float& pdf) const {
+        return evaluateGltfCos(g, view, light, diffuseContinuationPdf(s, g.normal, light), pdf);
+    }
+};
 
 // ---- next-event estimation toward triangle lights (device_traversal.cuh:529-569)
-template <class Occluder>
+template <class Occluder, class Bsdf>
 WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
-    const Material& material, const ShadeParams& sp, Occluder& occ, ShadowRequest& request) {
+    const Bsdf& bsdfAt, const ShadeParams& sp, Occluder& occ, ShadowRequest& request) {
     if (s.lightCount == 0) return;
     uint32_t candidate = static_cast<uint32_t>(rand01(path, sp.rngStrategy) * static_cast<float>(s.lightCount));
     if (candidate > s.lightCount - 1) candidate = s.lightCount - 1;
@@ -411,7 +365,7 @@ WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceH
     float geometricCosine = fmaxf(0.0f, dot(hit.geometricNormal, direction));
     float shadingCosine = fmaxf(0.0f, dot(shadingNormal, direction));
     float lightCosine = fmaxf(0.0f, dot(light.normal, -direction));
-    if (geometricCosine == 0.0f || shadingCosine == 0.0f || lightCosine == 0.0f) return;
+    if (geometricCosine == 0.0f || (Bsdf::kOneSided && shadingCosine == 0.0f) || lightCosine == 0.0f) return;
     Vec3 shadowOrigin = offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t);
     Vec3 shadowTarget = offsetRayOrigin(lightPosition, light.normal, -direction, distance);
     Vec3 segment = shadowTarget - shadowOrigin;
@@ -424,7 +378,7 @@ WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceH
     float geometryTerm = lightCosine * light.area / (distanceSquared * light.selectionPdf);
     float lightPdf = light.selectionPdf * distanceSquared / (lightCosine * light.area);
     float bsdfPdf;
-    Vec3 bsdf = evaluateSurfaceCos(s, material, shadingNormal, -path.ray.direction, direction, bsdfPdf);
+    Vec3 bsdf = bsdfAt.evalCos(s, -path.ray.direction, direction, bsdfPdf);
     float misWeight = powerHeuristic(lightPdf, bsdfPdf);
     Vec3 directLighting = hadamard(bsdf, lightMaterial.emission) * (geometryTerm * misWeight);
     Vec3 contribution = hadamard(path.throughput, directLighting);
@@ -484,12 +438,13 @@ WPT_HD PunctualSample samplePunctualLight(const PunctualLight& light, const Vec3
 }
 
 // One punctual connection: a delta light, so no MIS weight.
-template <class Occluder>
+template <class Occluder, class Bsdf>
 WPT_HD void addPunctualLight(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
-    const Material& material, const ShadeParams& sp, Occluder& occ, const PunctualLight& source,
+    const Bsdf& bsdfAt, const ShadeParams& sp, Occluder& occ, const PunctualLight& source,
     float inverseSelectionPdf) {
     PunctualSample light = samplePunctualLight(source, hit.position);
-    if (dot(shadingNormal, light.direction) <= 0.0f || dot(hit.geometricNormal, light.direction) <= 0.0f ||
+    if ((Bsdf::kOneSided && dot(shadingNormal, light.direction) <= 0.0f) ||
+        dot(hit.geometricNormal, light.direction) <= 0.0f ||
         lengthSquared(light.incident) == 0.0f)
         return;
     Ray shadow(offsetRayOrigin(hit.position, hit.geometricNormal, light.direction, hit.t), light.direction);
@@ -502,38 +457,42 @@ WPT_HD void addPunctualLight(const DevScene& s, PathState& path, const SurfaceHi
     }
     if (!s.ignoreDirectLightOcclusion && occ.blocked(s, shadow, shadowDistance - kRayMinDistance)) return;
     float ignoredPdf;
-    Vec3 bsdf = evaluateSurfaceCos(s, material, shadingNormal, -path.ray.direction, light.direction, ignoredPdf);
+    Vec3 bsdf = bsdfAt.evalCos(s, -path.ray.direction, light.direction, ignoredPdf);
     Vec3 direct = hadamard(bsdf, light.incident) * inverseSelectionPdf;
     addPathRadiance(path, hadamard(path.throughput, direct), sp.clamp, false);
 }
 
 // Every directional light, then one positional light chosen uniformly (one
 // draw when positional lights exist).
-template <class Occluder>
+template <class Occluder, class Bsdf>
 WPT_HD void sampleDirectPunctualLight(const DevScene& s, PathState& path, const SurfaceHit& hit,
-    const Vec3& shadingNormal, const Material& material, const ShadeParams& sp, Occluder& occ) {
+    const Vec3& shadingNormal, const Bsdf& bsdfAt, const ShadeParams& sp, Occluder& occ) {
+    if (!Bsdf::kFull) return;
     for (uint32_t i = 0; i < s.directionalCount; ++i)
-        addPunctualLight(s, path, hit, shadingNormal, material, sp, occ, s.punctual[i], 1.0f);
+        addPunctualLight(s, path, hit, shadingNormal, bsdfAt, sp, occ, s.punctual[i], 1.0f);
     uint32_t count = s.punctualCount - s.directionalCount;
     if (count == 0) return;
     uint32_t pick = static_cast<uint32_t>(rand01(path, sp.rngStrategy) * static_cast<float>(count));
     uint32_t index = s.directionalCount + (pick < count - 1 ? pick : count - 1);
-    addPunctualLight(s, path, hit, shadingNormal, material, sp, occ, s.punctual[index], static_cast<float>(count));
+    addPunctualLight(s, path, hit, shadingNormal, bsdfAt, sp, occ, s.punctual[index], static_cast<float>(count));
 }
 
 // Next-event estimation toward the environment (device_traversal.cuh:637-651).
-template <class Occluder>
+template <class Occluder, class Bsdf>
 WPT_HD void sampleDirectEnvironment(const DevScene& s, PathState& path, const SurfaceHit& hit,
-    const Vec3& shadingNormal, const Material& material, const ShadeParams& sp, Occluder& occ) {
+    const Vec3& shadingNormal, const Bsdf& bsdfAt, const ShadeParams& sp, Occluder& occ) {
+    if (!Bsdf::kFull) return;
     const EnvironmentMap& m = s.environment;
     if (!m.texels || m.intensity == 0.0f) return;
     EnvironmentSample light = samplePathEnvironment(s, path, sp.rngStrategy);
     float cosine = dot(shadingNormal, light.direction);
-    if (cosine <= 0.0f || dot(hit.geometricNormal, light.direction) <= 0.0f || !(light.pdf > 0.0f)) return;
+    if ((Bsdf::kOneSided && cosine <= 0.0f) || dot(hit.geometricNormal, light.direction) <= 0.0f ||
+        !(light.pdf > 0.0f))
+        return;
     Ray shadow(offsetRayOrigin(hit.position, hit.geometricNormal, light.direction, hit.t), light.direction);
     if (!s.ignoreDirectLightOcclusion && occ.blocked(s, shadow, 1.0e30f)) return;
     float bsdfPdf;
-    Vec3 bsdf = evaluateSurfaceCos(s, material, shadingNormal, -path.ray.direction, light.direction, bsdfPdf);
+    Vec3 bsdf = bsdfAt.evalCos(s, -path.ray.direction, light.direction, bsdfPdf);
     float weight = powerHeuristic(light.pdf, bsdfPdf);
     Vec3 direct = hadamard(bsdf, environmentRadiance(m, light.direction)) * (weight / light.pdf);
     addPathRadiance(path, hadamard(path.throughput, direct), sp.clamp, false);
@@ -542,27 +501,27 @@ WPT_HD void sampleDirectEnvironment(const DevScene& s, PathState& path, const Su
 // ---- continuations
 // Diffuse: cosine sampling, or half the time an environment sample when a
 // map exists (device_traversal.cuh:685-704).
+template <bool Full>
 WPT_HD Vec3 continueDiffuse(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
-    const Material& m, uint32_t strategy) {
-    float probability = diffuseEnvironmentProbability(s);
+    const Vec3& albedo, uint32_t strategy) {
+    float probability = diffuseEnvironmentProbability<Full>(s);
     Vec3 direction = probability > 0.0f && rand01(path, strategy) < probability
         ? samplePathEnvironment(s, path, strategy).direction
         : sampleCosineHemisphere(shadingNormal, path, strategy);
     float cosine = dot(shadingNormal, direction);
-    float pdf = diffuseContinuationPdf(s, shadingNormal, direction);
+    float pdf = diffuseContinuationPdf<Full>(s, shadingNormal, direction);
     if (!(pdf > 0.0f) || cosine <= 0.0f || dot(direction, hit.geometricNormal) <= 0.0f) {
         path.active = 0;
         return direction;
     }
-    path.throughput = hadamard(path.throughput, m.albedo) * (cosine * kInvPi / pdf);
+    path.throughput = hadamard(path.throughput, albedo) * (cosine * kInvPi / pdf);
     path.previousBsdfPdf = pdf;
     path.specularBounce = 0;
     return direction;
 }
 
-WPT_HD Vec3 continueGlossy(PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal, const Material& m,
-    uint32_t strategy) {
-    SpecGlossSurface surface = glossySurface(m);
+WPT_HD Vec3 continueGlossy(PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
+    const SpecGlossSurface& surface, uint32_t strategy) {
     Vec3 outgoing = -path.ray.direction;
     float choice = rand01(path, strategy);
     float u = rand01(path, strategy);
@@ -584,15 +543,15 @@ WPT_HD Vec3 continueGlossy(PathState& path, const SurfaceHit& hit, const Vec3& s
 // (device_traversal.cuh:589-635).  With an environment map, both branches'
 // escapes to the environment are integrated at the interface (each tested
 // for occlusion), and a path whose chosen branch escapes ends here.
-template <class Occluder>
+template <bool Full = true, class Occluder>
 WPT_HD Vec3 continueDielectric(const DevScene& s, PathState& path, const SurfaceHit& hit, const Vec3& shadingNormal,
-    const Material& m, const ShadeParams& sp, Occluder& occ) {
-    path.throughput = hadamard(path.throughput, m.albedo);
+    const Material& m, const Vec3& albedo, const ShadeParams& sp, Occluder& occ) {
+    path.throughput = hadamard(path.throughput, albedo);
     DielectricInterface boundary = dielectricInterface(normalize(path.ray.direction), hit.geometricNormal,
         shadingNormal, m.ior);
     float etaSquared = boundary.eta * boundary.eta;
     bool reflectionEscapes = false, transmissionEscapes = false;
-    if (s.environment.texels && path.depth + 1 < sp.maxDepth) {
+    if (Full && s.environment.texels && path.depth + 1 < sp.maxDepth) {
         if (boundary.reflectance > 0.0f) {
             Vec3 direction = boundary.reflection;
             Ray ray(offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t), direction);
@@ -616,7 +575,7 @@ WPT_HD Vec3 continueDielectric(const DevScene& s, PathState& path, const Surface
         path.etaScale /= etaSquared;
         if (boundary.frontFace) {
             path.mediumMaterial = hit.material;
-            path.mediumDensity = m.volumeDensity * 1.0f;
+            path.mediumDensity = m.volumeDensity * materialThickness<Full>(s, m, hit);
         } else {
             path.mediumMaterial = kInvalid;
             path.mediumDensity = 1.0f;
@@ -631,6 +590,85 @@ WPT_HD Vec3 continueDielectric(const DevScene& s, PathState& path, const Surface
         reflected ? boundary.reflection.z : boundary.transmission.z);
 }
 
+// glTF metallic-roughness continuation (continueGltfPbrSurface,
+// device_traversal.cuh:727-787): one draw picks coat, specular or diffuse
+// by their weights; zero-roughness lobes are deltas.  With an environment
+// map, visible delta reflections toward it are integrated at the vertex
+// (each tested for occlusion), and a path whose sampled delta lobe escapes
+// ends here.
+template <class Occluder>
+WPT_HD Vec3 continueGltf(const DevScene& s, PathState& path, const SurfaceHit& hit, const GltfSurface& g,
+    const ShadeParams& sp, Occluder& occ) {
+    Vec3 view = -path.ray.direction;
+    GltfWeights weights = pbrWeights(g, view);
+    float choice = rand01(path, sp.rngStrategy);
+    bool coat = choice < weights.coat;
+    bool specular = !coat && choice < weights.coat + weights.specular;
+    bool delta = (coat && g.coatRoughness <= 0.0f) || (specular && g.roughness <= 0.0f);
+    bool baseEscapes = false, coatEscapes = false;
+    Vec3 baseCoefficient = pbrBaseFresnel(g, dot(pbrSpecularNormal(g), view)) * (1.0f - weights.coat);
+    if (s.environment.texels && path.depth + 1 < sp.maxDepth) {
+        for (int lobe = 0; lobe < 2; ++lobe) {
+            bool isCoat = lobe == 1;
+            float mass = isCoat ? weights.coat : weights.specular;
+            float roughness = isCoat ? g.coatRoughness : g.roughness;
+            if (!(mass > 0.0f) || roughness > 0.0f) continue;
+            Vec3 normal = isCoat ? g.coatNormal : pbrSpecularNormal(g);
+            Vec3 direction = pbrSampleReflection(normal, view, 0.0f, 0.0f, 0.0f);
+            if (dot(normal, view) <= 0.0f || dot(direction, hit.geometricNormal) <= 0.0f) continue;
+            Ray ray(offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t), direction);
+            bool escapes = !occ.blocked(s, ray, 1.0e30f);
+            if (isCoat)
+                coatEscapes = escapes;
+            else
+                baseEscapes = escapes;
+            if (escapes) {
+                Vec3 coefficient = isCoat ? Vec3(weights.coat, weights.coat, weights.coat) : baseCoefficient;
+                addPathRadiance(path, hadamard(path.throughput, hadamard(coefficient,
+                    environmentRadiance(s.environment, direction))), sp.clamp, false);
+            }
+        }
+    }
+    Vec3 direction;
+    if (coat || specular) {
+        float u = rand01(path, sp.rngStrategy), v = rand01(path, sp.rngStrategy);
+        Vec3 normal = coat ? g.coatNormal : pbrSpecularNormal(g);
+        direction = pbrSampleReflection(normal, view, coat ? g.coatRoughness : g.roughness, u, v);
+        if (dot(normal, view) <= 0.0f) {
+            path.active = 0;
+            return direction;
+        }
+    } else {
+        float probability = diffuseEnvironmentProbability(s);
+        direction = probability > 0.0f && rand01(path, sp.rngStrategy) < probability
+            ? samplePathEnvironment(s, path, sp.rngStrategy).direction
+            : sampleCosineHemisphere(g.normal, path, sp.rngStrategy);
+    }
+    if (dot(direction, hit.geometricNormal) <= 0.0f) {
+        path.active = 0;
+        return direction;
+    }
+    if (delta) {
+        float mass = coat ? weights.coat : weights.specular;
+        Vec3 coefficient = coat ? Vec3(weights.coat, weights.coat, weights.coat) : baseCoefficient;
+        path.throughput = hadamard(path.throughput, coefficient) / mass;
+        path.specularBounce = 1;
+        path.previousBsdfPdf = 0.0f;
+        if (coat ? coatEscapes : baseEscapes) path.active = 0;
+    } else {
+        float pdf;
+        Vec3 value = evaluateGltfCos(g, view, direction, diffuseContinuationPdf(s, g.normal, direction), pdf);
+        if (!(pdf > 0.0f)) {
+            path.active = 0;
+            return direction;
+        }
+        path.throughput = hadamard(path.throughput, value) / pdf;
+        path.previousBsdfPdf = pdf;
+        path.specularBounce = 0;
+    }
+    return direction;
+}
+
 // One bounce of shading.  Returns true when the path continues with
 // path.ray set to the continuation.  A terminated path adds `debugColor` to
 // its pixel when `isDebug`, else its radiance after `shadow` is resolved
@@ -642,15 +680,16 @@ struct ShadeOutcome {
     ShadowRequest shadow;
 };
 
-template <class Occluder>
+template <bool Full, class Occluder>
 WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp,
     Occluder& occ, ShadeOutcome& out) {
     bool didHit = h.kind != kHitNone;
     if (sp.intersectionDebug || sp.shadingNormalDebug) {
         Vec3 color(0.05f, 0.10f, 0.25f);
         if (didHit) {
-            SurfaceHit hit = surfaceAttributes(s, path.ray, h);
-            color = 0.5f * (hit.normal + Vec3(1.0f, 1.0f, 1.0f));
+            SurfaceHit hit = surfaceAttributes<Full>(s, path.ray, h);
+            Vec3 normal = sp.intersectionDebug ? hit.normal : materialNormal<Full>(s, s.materials[hit.material], hit);
+            color = 0.5f * (normal + Vec3(1.0f, 1.0f, 1.0f));
         }
         path.active = 0;
         out.isDebug = true;
@@ -659,7 +698,7 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
     }
     if (!didHit) {
         // addEnvironmentOrSky, device_traversal.cuh:577-587.
-        if (s.environment.texels) {
+        if (Full && s.environment.texels) {
             float weight = path.depth > 0 && !path.specularBounce
                 ? powerHeuristic(path.previousBsdfPdf, environmentPdf(s.environment, path.ray.direction))
                 : 1.0f;
@@ -673,7 +712,7 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
         path.active = 0;
         return false;
     }
-    SurfaceHit hit = surfaceAttributes(s, path.ray, h);
+    SurfaceHit hit = surfaceAttributes<Full>(s, path.ray, h);
     const Material& material = s.materials[hit.material];
     if (path.mediumMaterial != kInvalid) {
         const Material& medium = s.materials[path.mediumMaterial];
@@ -698,38 +737,56 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
                 misWeight = powerHeuristic(path.previousBsdfPdf, lightPdf);
             }
         }
-        addPathRadiance(path, hadamard(path.throughput, material.emission) * misWeight, sp.clamp, true);
+        addPathRadiance(path, hadamard(path.throughput, materialEmission<Full>(s, material, hit)) * misWeight, sp.clamp,
+            true);
         path.active = 0;
         return false;
     }
-    Vec3 shadingNormal = hit.normal;
+    Vec3 shadingNormal = materialNormal<Full>(s, material, hit);
     if (dot(shadingNormal, hit.geometricNormal) < 0.0f) shadingNormal = -shadingNormal;
-    uint32_t type = (material.type == kDiffuse && material.metallic > 0.5f) ? kMetal : material.type;
-    float roughness = material.roughness;
+    Vec3 albedo = materialAlbedo<Full>(s, material, hit);
+    uint32_t type =
+        material.type == kDiffuse && materialMetallic<Full>(s, material, hit) > 0.5f ? kMetal : material.type;
+    float roughness = materialRoughness<Full>(s, material, hit);
     Vec3 direction;
-    switch (type) {
-    case kDiffuse:
-        sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
-        sampleDirectPunctualLight(s, path, hit, shadingNormal, material, sp, occ);
-        sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
-        direction = continueDiffuse(s, path, hit, shadingNormal, material, sp.rngStrategy);
+    if (Full && isGltf(s, material)) {
+        GltfBsdf bsdf{gltfSurface(s, material, hit, shadingNormal, albedo, roughness, -path.ray.direction)};
+        sampleDirectLight(s, path, hit, shadingNormal, bsdf, sp, occ, out.shadow);
+        sampleDirectPunctualLight(s, path, hit, shadingNormal, bsdf, sp, occ);
+        sampleDirectEnvironment(s, path, hit, shadingNormal, bsdf, sp, occ);
+        direction = continueGltf(s, path, hit, bsdf.g, sp, occ);
+    } else switch (type) {
+    case kDiffuse: {
+        StandardBsdf<Full> bsdf{false, shadingNormal, SpecGlossSurface{albedo, Vec3(), roughness}};
+        sampleDirectLight(s, path, hit, shadingNormal, bsdf, sp, occ, out.shadow);
+        sampleDirectPunctualLight(s, path, hit, shadingNormal, bsdf, sp, occ);
+        sampleDirectEnvironment(s, path, hit, shadingNormal, bsdf, sp, occ);
+        if (Full && sp.photons) {  // device_path_shading.cuh:99-102
+            Vec3 caustic = gatherCaustics(*sp.photons, hit.position, shadingNormal);
+            addPathRadiance(path, hadamard(path.throughput, hadamard(albedo, caustic)), sp.clamp, false);
+        }
+        direction = continueDiffuse<Full>(s, path, hit, shadingNormal, albedo, sp.rngStrategy);
         break;
-    case kSpecularGlossiness:
-        sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
-        sampleDirectPunctualLight(s, path, hit, shadingNormal, material, sp, occ);
-        sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
-        direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
+    }
+    case kSpecularGlossiness: {
+        StandardBsdf<Full> bsdf{true, shadingNormal, specGlossSurface<Full>(s, material, hit, albedo, roughness)};
+        sampleDirectLight(s, path, hit, shadingNormal, bsdf, sp, occ, out.shadow);
+        sampleDirectPunctualLight(s, path, hit, shadingNormal, bsdf, sp, occ);
+        sampleDirectEnvironment(s, path, hit, shadingNormal, bsdf, sp, occ);
+        direction = continueGlossy(path, hit, shadingNormal, bsdf.sg, sp.rngStrategy);
         break;
+    }
     case kMetal:
         if (roughness > 0.0f) {
-            sampleDirectLight(s, path, hit, shadingNormal, material, sp, occ, out.shadow);
-            sampleDirectPunctualLight(s, path, hit, shadingNormal, material, sp, occ);
-        sampleDirectEnvironment(s, path, hit, shadingNormal, material, sp, occ);
-            direction = continueGlossy(path, hit, shadingNormal, material, sp.rngStrategy);
+            StandardBsdf<Full> bsdf{true, shadingNormal, SpecGlossSurface{Vec3(), albedo, roughness}};
+            sampleDirectLight(s, path, hit, shadingNormal, bsdf, sp, occ, out.shadow);
+            sampleDirectPunctualLight(s, path, hit, shadingNormal, bsdf, sp, occ);
+            sampleDirectEnvironment(s, path, hit, shadingNormal, bsdf, sp, occ);
+            direction = continueGlossy(path, hit, shadingNormal, bsdf.sg, sp.rngStrategy);
             break;
         }
         {
-            path.throughput = hadamard(path.throughput, material.albedo);
+            path.throughput = hadamard(path.throughput, albedo);
             Vec3 reflected = reflect(normalize(path.ray.direction), shadingNormal);
             Vec3 jitter = randomInUnitSphere(path, sp.rngStrategy);
             direction = normalize(reflected + roughness * jitter);
@@ -740,7 +797,7 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
         }
         break;
     case kDielectric:
-        direction = continueDielectric(s, path, hit, shadingNormal, material, sp, occ);
+        direction = continueDielectric<Full>(s, path, hit, shadingNormal, material, albedo, sp, occ);
         break;
     default:
         break;
@@ -767,12 +824,12 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
     return false;
 }
 
-template <class Occluder>
+template <bool Full = true, class Occluder>
 WPT_HD bool shadePath(const DevScene& s, PathState& path, const HitRecord& h, const ShadeParams& sp, Occluder& occ,
     ShadeOutcome& out) {
     out.isDebug = false;
     out.shadow.valid = 0;
-    bool alive = shadeBounce(s, path, h, sp, occ, out);
+    bool alive = shadeBounce<Full>(s, path, h, sp, occ, out);
     out.alive = alive;
     return alive;
 }

@@ -18,6 +18,7 @@
 
 #include "../common/vec.h"
 #include "../scene/device_scene.h"
+#include "material.h"
 
 namespace wpt {
 
@@ -32,6 +33,24 @@ struct HitRecord {
     uint32_t prim;      // sphere / triangle index in the caller's order
     uint32_t material;  // resolved (override applied)
 };
+
+// Alpha mask of a candidate triangle hit (reference intersectTriangleHit,
+// device_traversal.cuh:146-151): masked materials below their cutoff are
+// passed through; textured ones test opacity x albedo alpha at the hit.
+WPT_HD bool textureCutAway(const DevScene& s, uint32_t material, const TriShade& tri, float u, float v) {
+    const Material& m = s.materials[material];
+    return materialOpacity(s, m, triangleUv(tri, u, v), triangleUv1(s, tri, u, v)) < m.alphaCutoff;
+}
+
+// `Full` kernels serve every scene; the others only scenes without
+// environment maps, punctual lights or texture bindings, and leave that code
+// out (it costs registers on every path).
+template <bool Full>
+WPT_HD bool cutAway(const DevScene& s, uint32_t material, const TriShade* tris, uint32_t prim, float u, float v) {
+    uint8_t policy = s.materialCutout[material];
+    if (policy == kCutNever) return false;
+    return policy == kCutAlways || (Full && textureCutAway(s, material, tris[prim], u, v));
+}
 
 constexpr uint32_t kHitNone = 0, kHitSphere = 1, kHitStatic = 2, kHitInstance = 3, kHitFlat = 4;
 
@@ -157,9 +176,10 @@ WPT_HD bool popLive(Stack& stack, uint32_t base, float closestT, uint32_t& cur) 
 }
 
 // Closest triangle of one BVH (BLAS in local space, or the flat-triangle BVH).
-template <class Stack>
-WPT_HD void closestInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry, const uint8_t* cutout,
-    uint32_t materialOverride, uint32_t kind, uint32_t instance, HitRecord& best, Stack& stack) {
+template <bool Full, class Stack>
+WPT_HD void closestInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, const float4* geometry,
+    const TriShade* tris, uint32_t materialOverride, uint32_t kind, uint32_t instance, HitRecord& best,
+    Stack& stack) {
     float nearT;
     if (!slab(r, bvh.rootMin[0], bvh.rootMin[1], bvh.rootMin[2], bvh.rootMax[0], bvh.rootMax[1], bvh.rootMax[2],
             best.t, nearT))
@@ -175,7 +195,7 @@ WPT_HD void closestInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry
             float t, u, v;
             if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < best.t) {
                 uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
-                if (!cutout[material]) {
+                if (!cutAway<Full>(s, material, tris, wbits(a), u, v)) {
                     best.t = t;
                     best.u = u;
                     best.v = v;
@@ -210,9 +230,9 @@ WPT_HD void closestInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry
 }
 
 // First triangle of one BVH closer than maxDistance (shadow rays).
-template <class Stack>
-WPT_HD bool anyInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry, const uint8_t* cutout,
-    uint32_t materialOverride, float maxDistance, Stack& stack) {
+template <bool Full, class Stack>
+WPT_HD bool anyInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, const float4* geometry,
+    const TriShade* tris, uint32_t materialOverride, float maxDistance, Stack& stack) {
     float nearT;
     if (!slab(r, bvh.rootMin[0], bvh.rootMin[1], bvh.rootMin[2], bvh.rootMax[0], bvh.rootMax[1], bvh.rootMax[2],
             maxDistance, nearT))
@@ -228,7 +248,7 @@ WPT_HD bool anyInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry, co
             float t, u, v;
             if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < maxDistance) {
                 uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
-                if (!cutout[material]) {
+                if (!cutAway<Full>(s, material, tris, wbits(a), u, v)) {
                     stack.truncate(base);
                     return true;
                 }
@@ -259,14 +279,15 @@ WPT_HD bool anyInBvh(const RayInv& r, const Bvh& bvh, const float4* geometry, co
 
 // Brute-force closest over original-order triangles (direct instances,
 // static triangles, flat triangles without a BVH).
-WPT_HD void closestBrute(const Ray& ray, const TriShade* tris, uint32_t count, const uint8_t* cutout,
+template <bool Full>
+WPT_HD void closestBrute(const DevScene& s, const Ray& ray, const TriShade* tris, uint32_t count,
     uint32_t materialOverride, uint32_t kind, uint32_t instance, HitRecord& best) {
     for (uint32_t i = 0; i < count; ++i) {
         const TriShade& tri = tris[i];
         float t, u, v;
         if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !(t < best.t)) continue;
         uint32_t material = materialOverride == kInvalid ? tri.material : materialOverride;
-        if (cutout[material]) continue;
+        if (cutAway<Full>(s, material, tris, i, u, v)) continue;
         best.t = t;
         best.u = u;
         best.v = v;
@@ -277,14 +298,15 @@ WPT_HD void closestBrute(const Ray& ray, const TriShade* tris, uint32_t count, c
     }
 }
 
-WPT_HD bool anyBrute(const Ray& ray, const TriShade* tris, uint32_t count, const uint8_t* cutout,
+template <bool Full>
+WPT_HD bool anyBrute(const DevScene& s, const Ray& ray, const TriShade* tris, uint32_t count,
     uint32_t materialOverride, float maxDistance) {
     for (uint32_t i = 0; i < count; ++i) {
         const TriShade& tri = tris[i];
         float t, u, v;
         if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !(t < maxDistance)) continue;
         uint32_t material = materialOverride == kInvalid ? tri.material : materialOverride;
-        if (!cutout[material]) return true;
+        if (!cutAway<Full>(s, material, tris, i, u, v)) return true;
     }
     return false;
 }
@@ -312,7 +334,7 @@ WPT_HD void sphereLeaf(const DevScene& s, const Ray& ray, uint32_t index, HitRec
     best.material = s.sphereMaterial[index];
 }
 
-template <class Stack>
+template <bool Full, class Stack>
 WPT_HD void instanceClosest(const DevScene& s, const Ray& ray, uint32_t index, bool useBvh, HitRecord& best,
     Stack& stack) {
     const Instance& inst = s.instances[index];
@@ -320,10 +342,10 @@ WPT_HD void instanceClosest(const DevScene& s, const Ray& ray, uint32_t index, b
     Ray local = toLocal(inst.transform, ray);
     if (useBvh && blas.bvh.primCount != 0) {
         RayInv ri = makeRayInv(local);
-        closestInBvh(ri, blas.bvh, blas.geometry, s.materialCutout, inst.materialOverride, kHitInstance, index, best,
+        closestInBvh<Full>(s, ri, blas.bvh, blas.geometry, blas.triangles, inst.materialOverride, kHitInstance, index, best,
             stack);
     } else {
-        closestBrute(local, blas.triangles, blas.triangleCount, s.materialCutout, inst.materialOverride,
+        closestBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride,
             kHitInstance, index, best);
     }
 }
@@ -337,12 +359,12 @@ WPT_HD void instanceClosest(const DevScene& s, const Ray& ray, uint32_t index, b
 // it switches back and resumes popping the TLAS.  The visit order is that of
 // the nested walk (each BLAS finishes before the TLAS resumes), but lanes at
 // different levels now run the same node-visit code in the same iteration.
-template <class Stack>
+template <bool Full = true, class Stack>
 WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
     HitRecord best = emptyHit();
     for (uint32_t i = 0; i < s.directSphereCount; ++i) sphereLeaf(s, ray, s.directSpheres[i], best);
-    for (uint32_t i = 0; i < s.directInstanceCount; ++i) instanceClosest(s, ray, s.directInstances[i], false, best, stack);
-    closestBrute(ray, s.staticTriangles, s.staticCount, s.materialCutout, kInvalid, kHitStatic, kInvalid, best);
+    for (uint32_t i = 0; i < s.directInstanceCount; ++i) instanceClosest<Full>(s, ray, s.directInstances[i], false, best, stack);
+    closestBrute<Full>(s, ray, s.staticTriangles, s.staticCount, kInvalid, kHitStatic, kInvalid, best);
     if (s.tlas.primCount != 0) {
         RayInv r = makeRayInv(ray);
         float nearT;
@@ -365,7 +387,8 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
                         float t, u, v;
                         if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < best.t) {
                             uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
-                            if (!s.materialCutout[material]) {
+                            if (s.materialCutout[material] == kCutNever ||
+                                !cutAway<Full>(s, material, s.blases[s.instances[instance].mesh].triangles, wbits(a), u, v)) {
                                 best.t = t;
                                 best.u = u;
                                 best.v = v;
@@ -397,8 +420,8 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
                                     continue;
                                 }
                             } else {
-                                closestBrute(local, blas.triangles, blas.triangleCount, s.materialCutout,
-                                    inst.materialOverride, kHitInstance, leaf, best);
+                                closestBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride,
+                                    kHitInstance, leaf, best);
                             }
                         }
                     }
@@ -435,16 +458,16 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
     if (s.flatCount != 0) {
         if (s.flat.primCount != 0) {
             RayInv r = makeRayInv(ray);
-            closestInBvh(r, s.flat, s.flatGeometry, s.materialCutout, kInvalid, kHitFlat, kInvalid, best, stack);
+            closestInBvh<Full>(s, r, s.flat, s.flatGeometry, s.flatTriangles, kInvalid, kHitFlat, kInvalid, best, stack);
         } else {
-            closestBrute(ray, s.flatTriangles, s.flatCount, s.materialCutout, kInvalid, kHitFlat, kInvalid, best);
+            closestBrute<Full>(s, ray, s.flatTriangles, s.flatCount, kInvalid, kHitFlat, kInvalid, best);
         }
     }
     return best;
 }
 
 // Shadow query: any blocker with t < maxDistance.
-template <class Stack>
+template <bool Full = true, class Stack>
 WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, Stack& stack) {
     for (uint32_t i = 0; i < s.directSphereCount; ++i) {
         float4 sp = s.spheres[s.directSpheres[i]];
@@ -455,10 +478,10 @@ WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, 
         const Instance& inst = s.instances[s.directInstances[i]];
         const Blas& blas = s.blases[inst.mesh];
         Ray local = toLocal(inst.transform, ray);
-        if (anyBrute(local, blas.triangles, blas.triangleCount, s.materialCutout, inst.materialOverride, maxDistance))
+        if (anyBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride, maxDistance))
             return true;
     }
-    if (anyBrute(ray, s.staticTriangles, s.staticCount, s.materialCutout, kInvalid, maxDistance)) return true;
+    if (anyBrute<Full>(s, ray, s.staticTriangles, s.staticCount, kInvalid, maxDistance)) return true;
     if (s.tlas.primCount != 0) {
         RayInv r = makeRayInv(ray);
         float nearT;
@@ -483,7 +506,8 @@ WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, 
                         float t, u, v;
                         if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < maxDistance) {
                             uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
-                            if (!s.materialCutout[material]) {
+                            if (s.materialCutout[material] == kCutNever ||
+                                !cutAway<Full>(s, material, s.blases[s.instances[instance].mesh].triangles, wbits(a), u, v)) {
                                 stack.truncate(base);
                                 return true;
                             }
@@ -548,8 +572,8 @@ WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, 
     if (s.flatCount != 0) {
         if (s.flat.primCount != 0) {
             RayInv r = makeRayInv(ray);
-            if (anyInBvh(r, s.flat, s.flatGeometry, s.materialCutout, kInvalid, maxDistance, stack)) return true;
-        } else if (anyBrute(ray, s.flatTriangles, s.flatCount, s.materialCutout, kInvalid, maxDistance)) {
+            if (anyInBvh<Full>(s, r, s.flat, s.flatGeometry, s.flatTriangles, kInvalid, maxDistance, stack)) return true;
+        } else if (anyBrute<Full>(s, ray, s.flatTriangles, s.flatCount, kInvalid, maxDistance)) {
             return true;
         }
     }

@@ -119,7 +119,9 @@ hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream)
 struct Occupancy {
     int trace, shade, shadow, persistent;  // resident workgroups per CU
 };
-hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out);
+// Occupancy of the kernels serving scenes with (full) or without
+// environment maps, punctual lights and texture bindings.
+hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out);
 // One sample of the current tile with every launch bracketed by events:
 // ms/launches accumulate per kernel (0 primary, 1 reset, 2 trace, 3 shade,
 // 4 persistent, 5 shadow, 6 compact: unused by the renderer, whose shade
@@ -167,12 +169,40 @@ struct BatchContext {
     float* out;
     uint2* spill;
     uint32_t spillThreads;
+    uint32_t full;  // some scene has an environment map, punctual lights or texture bindings
 };
+
+// Caustic photon map (photon.hip): settings, the renderer's buffers, and
+// the build: trace, key, sort, cell ranges.  `view` names the buffers the
+// gather reads (sorted photons and cell ranges) and the grid box.
+struct PhotonSettings {
+    uint32_t count;
+    uint32_t maxDepth;
+    uint32_t seed;
+    uint32_t strategy;
+    uint32_t spectral;
+};
+
+struct PhotonMapBuffers {
+    Photon* photons;  // emission order
+    Photon* sorted;   // cell order
+    uint64_t* keys;
+    uint64_t* sortedKeys;
+    void* temp;
+    size_t tempBytes;
+    uint32_t* cellStart;
+    uint32_t* cellEnd;
+    uint32_t* stored;  // photons kept inside the grid
+};
+
+hipError_t photonSortBytes(uint32_t count, uint32_t cells, size_t* bytes);
+hipError_t enqueuePhotonMap(const DevScene& scene, const PhotonSettings& settings, const PhotonMapBuffers& buffers,
+    const PhotonGridView& view, uint2* spill, uint32_t spillThreads, hipStream_t stream);
 
 // Resets the per-task state and counters, then runs the persistent batch
 // consumer on `blocks` x `blockSize` threads.
 hipError_t enqueueBatch(const BatchContext& ctx, uint32_t blocks, uint32_t blockSize, hipStream_t stream);
-hipError_t batchOccupancy(uint32_t blockSize, int* blocksPerCu);
+hipError_t batchOccupancy(uint32_t blockSize, bool full, int* blocksPerCu);
 
 // Standalone traversal for the C ABI; `spill` holds
 // (kMaxStackDepth - kLdsStackEntries) x abiTraceThreads() entries.

@@ -185,6 +185,7 @@ __global__ void kReset(LaunchContext ctx, uint32_t nextSel) {
 }
 
 // One ray per thread; the hardware dispatcher balances workgroups.
+template <bool Full>
 __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kTrace(LaunchContext ctx, uint32_t bounce, uint32_t sel) {
     SegmentView v = loadSegments(ctx.counters->count[sel]);
     uint32_t count = v.start[kSegments];
@@ -198,7 +199,7 @@ __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kTrace(LaunchContext 
     const PathQueue& q = ctx.queue[sel];
     uint32_t i = segmentSlot(v, q.segmentCapacity, g);
     Ray ray = loadRay(q, i);
-    HitRecord h = traceClosest(ctx.scene, ray, stack);
+    HitRecord h = traceClosest<Full>(ctx.scene, ray, stack);
     ctx.hits.t[i] = h.t;
     ctx.hits.u[i] = h.u;
     ctx.hits.v[i] = h.v;
@@ -245,7 +246,7 @@ __device__ __forceinline__ void storeShadow(const ShadowQueue& q, uint32_t s, co
 // framebuffer.  With `Inline` (scenes with an environment map or punctual
 // lights, whose shading adds radiance after the area-light NEE) every
 // shadow ray is traced here and the shadow queue stays empty.
-template <bool Inline>
+template <bool Inline, bool Full>
 __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
     __shared__ AppendShared sh;
     SegmentView v = loadSegments(ctx.counters->count[sel]);
@@ -267,11 +268,11 @@ __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
         if constexpr (Inline) {
             LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
             InlineOcclusion<LdsStack> occ{stack, 0u};
-            shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
+            shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
             if (occ.queries) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(occ.queries));
         } else {
             DeferOcclusion occ;
-            shadePath(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
+            shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
         }
     }
     bool alive = valid && outcome.alive;
@@ -292,6 +293,7 @@ __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
 // Resolves the bounce's shadow requests, one per thread: unoccluded
 // contributions are added to the continuing path's radiance, and finished
 // paths reach the pixel.
+template <bool Full>
 __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadow(LaunchContext ctx, uint32_t sel) {
     SegmentView v = loadSegments(ctx.counters->shadowCount);
     uint32_t count = v.start[kSegments];
@@ -306,7 +308,7 @@ __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadow(LaunchContext
                 __uint_as_float(sfield(q, kSOz)[i])),
         Vec3(__uint_as_float(sfield(q, kSDx)[i]), __uint_as_float(sfield(q, kSDy)[i]),
             __uint_as_float(sfield(q, kSDz)[i])));
-    bool occluded = traceOccluded(ctx.scene, ray, __uint_as_float(sfield(q, kSTmax)[i]), stack);
+    bool occluded = traceOccluded<Full>(ctx.scene, ray, __uint_as_float(sfield(q, kSTmax)[i]), stack);
     Vec3 c(__uint_as_float(sfield(q, kSCx)[i]), __uint_as_float(sfield(q, kSCy)[i]),
         __uint_as_float(sfield(q, kSCz)[i]));
     uint32_t target = sfield(q, kSTarget)[i];
@@ -337,6 +339,7 @@ __global__ void kAdvance(LaunchContext ctx) {
 // appended to the workgroup's segment of the other queue.  Bounce b reads
 // pcount[b % 3], appends to pcount[(b + 1) % 3] and clears pcount[(b + 2) % 3],
 // which no workgroup touches again before the barrier that ends bounce b.
+template <bool Full>
 __global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
     __shared__ AppendShared sh;
     cg::grid_group grid = cg::this_grid();
@@ -394,9 +397,9 @@ __global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
             outcome.alive = false;
             if (valid) {
                 path = loadPath(q, segmentSlot(v, q.segmentCapacity, g));
-                HitRecord h = traceClosest(ctx.scene, path.ray, stack);
+                HitRecord h = traceClosest<Full>(ctx.scene, path.ray, stack);
                 InlineOcclusion<LdsStack> occ{stack, 0u};
-                shadePath(ctx.scene, path, h, ctx.shade, occ, outcome);
+                shadePath<Full>(ctx.scene, path, h, ctx.shade, occ, outcome);
                 shadowQueries += occ.queries;
             }
             bool alive = valid && outcome.alive;
@@ -477,21 +480,34 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
     size_t lds = stackLdsBytes(ctx.blockSize);
     dim3 grid(ctx.gridBlocks), block(ctx.blockSize);
     const bool inlineShadows = shadesWithInlineShadows(ctx.scene);
+    const bool full = ctx.scene.fullFeatures != 0;
     hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, grid, block, 0, stream, ctx); });
     for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {
         uint32_t sel = bounce & 1u;
         e = timer(1, [&] { hipLaunchKernelGGL(kReset, dim3(1), dim3(64), 0, stream, ctx, sel ^ 1u); });
         if (e == hipSuccess)
-            e = timer(2, [&] { hipLaunchKernelGGL(kTrace, grid, block, lds, stream, ctx, bounce, sel); });
+            e = timer(2, [&] {
+                if (full)
+                    hipLaunchKernelGGL(kTrace<true>, grid, block, lds, stream, ctx, bounce, sel);
+                else
+                    hipLaunchKernelGGL(kTrace<false>, grid, block, lds, stream, ctx, bounce, sel);
+            });
         if (e == hipSuccess)
             e = timer(3, [&] {
                 if (inlineShadows)
-                    hipLaunchKernelGGL(kShade<true>, grid, block, lds, stream, ctx, sel);
+                    hipLaunchKernelGGL((kShade<true, true>), grid, block, lds, stream, ctx, sel);
+                else if (full)
+                    hipLaunchKernelGGL((kShade<false, true>), grid, block, 0, stream, ctx, sel);
                 else
-                    hipLaunchKernelGGL(kShade<false>, grid, block, 0, stream, ctx, sel);
+                    hipLaunchKernelGGL((kShade<false, false>), grid, block, 0, stream, ctx, sel);
             });
         if (e == hipSuccess && !inlineShadows && ctx.scene.lightCount != 0 && !ctx.scene.ignoreDirectLightOcclusion)
-            e = timer(5, [&] { hipLaunchKernelGGL(kShadow, grid, block, lds, stream, ctx, sel); });
+            e = timer(5, [&] {
+                if (full)
+                    hipLaunchKernelGGL(kShadow<true>, grid, block, lds, stream, ctx, sel);
+                else
+                    hipLaunchKernelGGL(kShadow<false>, grid, block, lds, stream, ctx, sel);
+            });
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
@@ -501,7 +517,9 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
 hipError_t launchPersistent(const LaunchContext& ctx, hipStream_t stream) {
     LaunchContext local = ctx;
     void* args[] = {&local};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kPersistent), dim3(ctx.persistentBlocks),
+    const void* kernel = ctx.scene.fullFeatures ? reinterpret_cast<const void*>(kPersistent<true>)
+                                                : reinterpret_cast<const void*>(kPersistent<false>);
+    return hipLaunchCooperativeKernel(kernel, dim3(ctx.persistentBlocks),
         dim3(ctx.blockSize), args, stackLdsBytes(ctx.blockSize), stream);
 }
 
@@ -514,13 +532,20 @@ hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream) {
     });
 }
 
-hipError_t kernelOccupancy(uint32_t blockSize, Occupancy* out) {
+hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out) {
     size_t lds = stackLdsBytes(blockSize);
     int b = static_cast<int>(blockSize);
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace, b, lds);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade<false>, b, 0);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow, b, lds);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent, b, lds);
+    hipError_t e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace<true>, b, lds)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace<false>, b, lds);
+    if (e == hipSuccess)
+        e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade<true, true>, b, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade<false, false>, b, 0);
+    if (e == hipSuccess)
+        e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<true>, b, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<false>, b, lds);
+    if (e == hipSuccess)
+        e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<true>, b, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<false>, b, lds);
     return e;
 }
 

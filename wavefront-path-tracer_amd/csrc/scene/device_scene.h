@@ -261,7 +261,7 @@ struct DevScene {
     uint32_t textureCount;
     uint32_t useNormalMaps;
     float normalMapMinimumCosine;
-    uint32_t pad2;
+    uint32_t fullFeatures;  // environment map, punctual lights or texture bindings present
 };
 
 // Shading traces its shadow rays where it makes them (rather than deferring

@@ -192,7 +192,16 @@ Aabb nodeChildBounds(const BvhNode& n, bool right) {
 
 Vec3 toVec(const float v[3]) { return Vec3(v[0], v[1], v[2]); }
 
-TriShade toShade(const wpt_triangle& t) {
+constexpr uint32_t kTriExtraFlags = WPT_TRIANGLE_HAS_TEXCOORDS1 | WPT_TRIANGLE_HAS_VERTEX_TANGENTS;
+static_assert(WPT_TRIANGLE_HAS_NORMALS == kTriHasNormals && WPT_TRIANGLE_HAS_TEXCOORDS == kTriHasTexcoords &&
+    WPT_TRIANGLE_HAS_TEXCOORDS1 == kTriHasTexcoords1 && WPT_TRIANGLE_HAS_VERTEX_TANGENTS == kTriHasVertexTangents,
+    "triangle flags match the C ABI");
+
+bool nonzero3(const float v[3]) { return v[0] != 0.0f || v[1] != 0.0f || v[2] != 0.0f; }
+
+// Shading record of a triangle; its second texture-coordinate set and
+// tangent data, when present, go to `extra`.
+TriShade toShade(const wpt_triangle& t, std::vector<TriExtra>& extra) {
     TriShade s{};
     s.v0 = toVec(t.v0);
     s.v1 = toVec(t.v1);
@@ -207,8 +216,24 @@ TriShade toShade(const wpt_triangle& t) {
     s.bitangent = toVec(t.bitangent);
     s.material = t.material;
     s.lightIndex = kInvalid;
-    s.flags = t.flags & (kTriHasNormals | kTriHasTexcoords);
+    s.flags = t.flags & (kTriHasNormals | kTriHasTexcoords | kTriHasTexcoords1 | kTriHasVertexTangents);
     s.extra = kInvalid;
+    if ((t.flags & kTriExtraFlags) || nonzero3(t.tangent1) || nonzero3(t.bitangent1)) {
+        TriExtra e{};
+        e.uv1_0 = Vec2{t.uv1_0[0], t.uv1_0[1]};
+        e.uv1_1 = Vec2{t.uv1_1[0], t.uv1_1[1]};
+        e.uv1_2 = Vec2{t.uv1_2[0], t.uv1_2[1]};
+        e.vertexTangent0 = toVec(t.vertex_tangent0);
+        e.vertexTangent1 = toVec(t.vertex_tangent1);
+        e.vertexTangent2 = toVec(t.vertex_tangent2);
+        e.sign0 = t.vertex_tangent_sign[0];
+        e.sign1 = t.vertex_tangent_sign[1];
+        e.sign2 = t.vertex_tangent_sign[2];
+        e.tangent1 = toVec(t.tangent1);
+        e.bitangent1 = toVec(t.bitangent1);
+        s.extra = static_cast<uint32_t>(extra.size());
+        extra.push_back(e);
+    }
     return s;
 }
 
@@ -463,6 +488,71 @@ std::string buildEnvironment(const wpt_scene_desc& desc, HostSceneData& out) {
 // Punctual lights, directional ones first in their given order, then the
 // positional ones (the stable partition of scene.cu:349-355): NEE visits
 // every directional light and one positional light chosen uniformly.
+// RGBA8 textures, packed one 32-bit word per texel.
+std::string buildTextures(const wpt_scene_desc& desc, HostSceneData& out) {
+    if (desc.texture_count && !desc.textures) return "texture array is null";
+    for (uint32_t i = 0; i < desc.texture_count; ++i) {
+        const wpt_texture& t = desc.textures[i];
+        uint64_t texels = static_cast<uint64_t>(t.width) * t.height;
+        if (!t.width || !t.height || !t.rgba || texels > (1ull << 31)) return "texture has invalid dimensions or data";
+        std::vector<uint32_t> packed(texels);
+        for (uint64_t k = 0; k < texels; ++k) {
+            const uint8_t* p = t.rgba + 4 * k;
+            packed[k] = static_cast<uint32_t>(p[0]) | (static_cast<uint32_t>(p[1]) << 8) |
+                (static_cast<uint32_t>(p[2]) << 16) | (static_cast<uint32_t>(p[3]) << 24);
+        }
+        out.textureTexels.push_back(std::move(packed));
+        out.textureWidth.push_back(t.width);
+        out.textureHeight.push_back(t.height);
+    }
+    return "";
+}
+
+std::vector<const wpt_texture_binding*> bindings(const wpt_material_ext& e) {
+    return {&e.albedo_texture, &e.metallic_roughness_texture, &e.emissive_texture, &e.normal_texture,
+        &e.thickness_texture, &e.specular_glossiness_texture, &e.specular_weight_texture, &e.specular_color_texture,
+        &e.clearcoat_texture, &e.clearcoat_roughness_texture, &e.clearcoat_normal_texture};
+}
+
+// A material needs its extension record when a texture is bound or a glTF
+// workflow changes its lookups; otherwise its constants are the exact result.
+bool needsExt(const wpt_material_ext& e) {
+    if (e.specular_glossiness_workflow || e.gltf_pbr_workflow) return true;
+    for (const wpt_texture_binding* b : bindings(e))
+        if (b->texture != WPT_INVALID_INDEX) return true;
+    return false;
+}
+
+TexBinding toBinding(const wpt_texture_binding& b) {
+    return TexBinding{b.texture, b.texcoord, Vec2{b.offset[0], b.offset[1]}, Vec2{b.scale[0], b.scale[1]}, b.rotation,
+        b.has_transform ? 1u : 0u};
+}
+
+MaterialExt toMaterialExt(const wpt_material_ext& e) {
+    MaterialExt r{};
+    r.albedo = toBinding(e.albedo_texture);
+    r.metallicRoughness = toBinding(e.metallic_roughness_texture);
+    r.emissive = toBinding(e.emissive_texture);
+    r.normal = toBinding(e.normal_texture);
+    r.thickness = toBinding(e.thickness_texture);
+    r.specularGlossiness = toBinding(e.specular_glossiness_texture);
+    r.specularWeight = toBinding(e.specular_weight_texture);
+    r.specularColor = toBinding(e.specular_color_texture);
+    r.clearcoat = toBinding(e.clearcoat_texture);
+    r.clearcoatRoughness = toBinding(e.clearcoat_roughness_texture);
+    r.clearcoatNormal = toBinding(e.clearcoat_normal_texture);
+    r.normalScale = e.normal_scale;
+    r.specularGlossinessWorkflow = e.specular_glossiness_workflow ? 1u : 0u;
+    r.gltfPbrWorkflow = e.gltf_pbr_workflow ? 1u : 0u;
+    r.specularWeightFactor = e.specular_weight;
+    r.specularColorFactor = toVec(e.specular_color);
+    r.clearcoatFactor = e.clearcoat;
+    r.clearcoatRoughnessFactor = e.clearcoat_roughness;
+    r.clearcoatNormalScale = e.clearcoat_normal_scale;
+    r.dispersion = e.dispersion;
+    return r;
+}
+
 static_assert(WPT_LIGHT_POINT == kLightPoint && WPT_LIGHT_SPOT == kLightSpot &&
     WPT_LIGHT_DIRECTIONAL == kLightDirectional, "light types match the C ABI");
 
@@ -502,12 +592,30 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
     if (!envError.empty()) return envError;
     std::string lightError = buildPunctualLights(desc, out);
     if (!lightError.empty()) return lightError;
+    std::string textureError = buildTextures(desc, out);
+    if (!textureError.empty()) return textureError;
     for (uint32_t i = 0; i < materialCount; ++i) {
         const wpt_material& m = desc.materials[i];
         if (m.type > WPT_MATERIAL_SPECULAR_GLOSSINESS) return "material has an unknown type";
-        out.materials.push_back(toMaterial(m));
-        out.materialCutout.push_back(m.alpha_masked && m.opacity < m.alpha_cutoff ? 1 : 0);
+        Material r = toMaterial(m);
+        bool albedoTexture = false;
+        if (desc.material_ext && needsExt(desc.material_ext[i])) {
+            const wpt_material_ext& e = desc.material_ext[i];
+            for (const wpt_texture_binding* b : bindings(e))
+                if (b->texcoord > 1) return "texture binding has an unknown coordinate set";
+            r.ext = static_cast<uint32_t>(out.materialExt.size());
+            out.materialExt.push_back(toMaterialExt(e));
+            albedoTexture = e.albedo_texture.texture < desc.texture_count;
+        }
+        out.materials.push_back(r);
+        // kCut* policy of intersectTriangleHit: textured masks test per hit.
+        out.materialCutout.push_back(!m.alpha_masked ? kCutNever
+            : albedoTexture                        ? kCutTexture
+            : m.opacity < m.alpha_cutoff           ? kCutAlways
+                                                   : kCutNever);
     }
+    out.useNormalMaps = desc.use_normal_maps ? 1u : 0u;
+    out.normalMapMinimumCosine = desc.normal_map_minimum_cosine;
 
     for (uint32_t i = 0; i < desc.sphere_count; ++i) {
         const wpt_sphere& s = desc.spheres[i];
@@ -524,11 +632,11 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
 
     for (uint32_t i = 0; i < desc.static_triangle_count; ++i) {
         if (badMaterial(desc.static_triangles[i].material)) return "static triangle material index out of range";
-        out.staticTriangles.push_back(toShade(desc.static_triangles[i]));
+        out.staticTriangles.push_back(toShade(desc.static_triangles[i], out.triExtra));
     }
     for (uint32_t i = 0; i < desc.flat_triangle_count; ++i) {
         if (badMaterial(desc.flat_triangles[i].material)) return "flat triangle material index out of range";
-        out.flatTriangles.push_back(toShade(desc.flat_triangles[i]));
+        out.flatTriangles.push_back(toShade(desc.flat_triangles[i], out.triExtra));
     }
 
     // Meshes: triangles and per-mesh light indices (scene_host.cpp:77-86).
@@ -541,7 +649,7 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
         for (uint32_t t = 0; t < mesh.triangle_count; ++t) {
             const wpt_triangle& src = desc.mesh_triangles[mesh.first_triangle + t];
             if (badMaterial(src.material)) return "mesh triangle material index out of range";
-            TriShade s = toShade(src);
+            TriShade s = toShade(src, out.triExtra);
             s.lightIndex = out.materials[s.material].type == kEmissive ? meshLights++ : kInvalid;
             out.meshes[m].triangles.push_back(s);
         }

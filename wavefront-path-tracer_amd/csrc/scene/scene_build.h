@@ -77,6 +77,19 @@ struct HostSceneData {
     float envIntensity = 1.0f, envRotation = 0.0f;
     std::vector<PunctualLight> punctual;  // directional lights first
     uint32_t directionalCount = 0;
+    // Textures (RGBA8 packed per texel), texture bindings of the textured
+    // materials, and the second-set / tangent data of textured triangles.
+    std::vector<std::vector<uint32_t>> textureTexels;
+    std::vector<uint32_t> textureWidth, textureHeight;
+    std::vector<MaterialExt> materialExt;
+    std::vector<TriExtra> triExtra;
+    uint32_t useNormalMaps = 0;
+    float normalMapMinimumCosine = 0.0f;
+    // An environment map, punctual lights or texture bindings: the scene
+    // needs the full-feature kernels.
+    bool fullFeatures() const {
+        return !envTexels.empty() || !punctual.empty() || !materialExt.empty() || !textureTexels.empty();
+    }
     uint32_t maxStackDepth = 0;  // worst-case traversal stack use
 };
 

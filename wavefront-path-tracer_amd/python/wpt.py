@@ -24,7 +24,7 @@ RNG_XORSHIFT32, RNG_PCG32 = 0, 1
 MODE_GRAPH, MODE_PERSISTENT, MODE_REGENERATE = 0, 1, 2
 HIT_NONE, HIT_SPHERE, HIT_STATIC, HIT_INSTANCE, HIT_FLAT = 0, 1, 2, 3, 4
 INVALID = 0xFFFFFFFF
-TRI_HAS_NORMALS, TRI_HAS_TEXCOORDS = 1, 2
+TRI_HAS_NORMALS, TRI_HAS_TEXCOORDS, TRI_HAS_TEXCOORDS1, TRI_HAS_VERTEX_TANGENTS = 1, 2, 4, 8
 
 F3 = C.c_float * 3
 F2 = C.c_float * 2
@@ -86,6 +86,38 @@ class MaterialExt(C.Structure):
         ("dispersion", C.c_float)]
 
 
+def texture_binding(texture=INVALID, texcoord=0, offset=(0.0, 0.0), scale=(1.0, 1.0), rotation=0.0,
+                    transform=False) -> TextureBinding:
+    return TextureBinding(texture, texcoord, F2(*offset), F2(*scale), rotation, 1 if transform else 0)
+
+
+def make_material_ext(normal_scale=1.0, specular_glossiness_workflow=False, gltf_pbr_workflow=False,
+                      specular_weight=1.0, specular_color=(1.0, 1.0, 1.0), clearcoat=0.0, clearcoat_roughness=0.0,
+                      clearcoat_normal_scale=1.0, dispersion=0.0, **bindings) -> MaterialExt:
+    """Texture bindings (keyword per TEXTURE_SLOTS name, a TextureBinding or
+    a texture index) and glTF workflow fields, with the reference's defaults."""
+    e = MaterialExt()
+    for name in TEXTURE_SLOTS:
+        b = bindings.pop(name, None)
+        if b is None:
+            b = texture_binding()
+        elif isinstance(b, int):
+            b = texture_binding(b)
+        setattr(e, name, b)
+    if bindings:
+        raise TypeError(f"unknown texture slots {sorted(bindings)}")
+    e.normal_scale = normal_scale
+    e.specular_glossiness_workflow = 1 if specular_glossiness_workflow else 0
+    e.gltf_pbr_workflow = 1 if gltf_pbr_workflow else 0
+    e.specular_weight = specular_weight
+    e.specular_color = vec3(specular_color)
+    e.clearcoat = clearcoat
+    e.clearcoat_roughness = clearcoat_roughness
+    e.clearcoat_normal_scale = clearcoat_normal_scale
+    e.dispersion = dispersion
+    return e
+
+
 LIGHT_POINT, LIGHT_SPOT, LIGHT_DIRECTIONAL = 0, 1, 2
 
 
@@ -129,7 +161,11 @@ class RenderConfig(C.Structure):
                 ("rr_start_depth", C.c_uint32), ("samples_per_pixel", C.c_uint32),
                 ("block_size", C.c_uint32), ("rng_strategy", C.c_uint32), ("indirect_clamp", C.c_float),
                 ("mode", C.c_uint32), ("persistent_blocks", C.c_uint32), ("intersection_debug", C.c_uint32),
-                ("shading_normal_debug", C.c_uint32), ("profile_queues", C.c_uint32)]
+                ("shading_normal_debug", C.c_uint32), ("profile_queues", C.c_uint32),
+                ("caustic_photons", C.c_uint32), ("caustic_max_depth", C.c_uint32),
+                ("caustic_gather_radius", C.c_float), ("caustic_gather", C.c_uint32),
+                ("caustic_spectral", C.c_uint32), ("caustic_grid_resolution", C.c_uint32),
+                ("caustic_grid_min", F3), ("caustic_grid_max", F3)]
 
 
 class Task(C.Structure):
@@ -294,7 +330,9 @@ def make_material(kind=DIFFUSE, albedo=(0.5, 0.5, 0.5), emission=(0, 0, 0), roug
                     vec3(specular), glossiness)
 
 
-def make_triangle(v0, v1, v2, material, normals=None, uvs=None, tangent=(0, 0, 0), bitangent=(0, 0, 0)) -> Triangle:
+def make_triangle(v0, v1, v2, material, normals=None, uvs=None, tangent=(0, 0, 0), bitangent=(0, 0, 0), uvs1=None,
+                  vertex_tangents=None, tangent_signs=(1.0, 1.0, 1.0), tangent1=(0, 0, 0),
+                  bitangent1=(0, 0, 0)) -> Triangle:
     t = Triangle()
     t.v0, t.v1, t.v2 = vec3(v0), vec3(v1), vec3(v2)
     flags = 0
@@ -304,6 +342,14 @@ def make_triangle(v0, v1, v2, material, normals=None, uvs=None, tangent=(0, 0, 0
     if uvs is not None:
         t.uv0, t.uv1, t.uv2 = F2(*uvs[0]), F2(*uvs[1]), F2(*uvs[2])
         flags |= TRI_HAS_TEXCOORDS
+    if uvs1 is not None:
+        t.uv1_0, t.uv1_1, t.uv1_2 = F2(*uvs1[0]), F2(*uvs1[1]), F2(*uvs1[2])
+        flags |= TRI_HAS_TEXCOORDS1
+    if vertex_tangents is not None:
+        t.vertex_tangent0, t.vertex_tangent1, t.vertex_tangent2 = (vec3(v) for v in vertex_tangents)
+        t.vertex_tangent_sign = vec3(tangent_signs)
+        flags |= TRI_HAS_VERTEX_TANGENTS
+    t.tangent1, t.bitangent1 = vec3(tangent1), vec3(bitangent1)
     t.tangent, t.bitangent = vec3(tangent), vec3(bitangent)
     t.material = material
     t.flags = flags
@@ -337,7 +383,19 @@ class SceneBuilder:
         self.environment_intensity = 1.0
         self.environment_rotation_degrees = 0.0
         self.punctual_lights: List[PunctualLight] = []
+        self.textures: List[np.ndarray] = []
+        self.material_exts: List[Optional[MaterialExt]] = []
+        self.use_normal_maps = True
+        self.normal_map_minimum_cosine = 0.0
         self._keep = []
+
+    def texture(self, rgba) -> int:
+        """An RGBA8 texture: uint8 (height, width, 4), rows top to bottom."""
+        arr = np.ascontiguousarray(rgba, dtype=np.uint8)
+        if arr.ndim != 3 or arr.shape[2] != 4:
+            raise ValueError("texture must be (height, width, 4) uint8")
+        self.textures.append(arr)
+        return len(self.textures) - 1
 
     def punctual_light(self, light: PunctualLight) -> int:
         self.punctual_lights.append(light)
@@ -365,8 +423,11 @@ class SceneBuilder:
         self.instances.append(Instance(mesh, material_override, acceleration, 0, transform or scaled_transform()))
         return len(self.instances) - 1
 
-    def material(self, m: Material) -> int:
+    def material(self, m: Material, ext: Optional[MaterialExt] = None) -> int:
         self.materials.append(m)
+        while len(self.material_exts) < len(self.materials) - 1:
+            self.material_exts.append(None)
+        self.material_exts.append(ext)
         return len(self.materials) - 1
 
     def _array(self, ctype, items):
@@ -388,6 +449,18 @@ class SceneBuilder:
         d.black_background = 1 if self.black_background else 0
         d.ignore_direct_light_occlusion = 1 if self.ignore_direct_light_occlusion else 0
         d.punctual_lights, d.punctual_light_count = self._array(PunctualLight, self.punctual_lights)
+        if self.textures:
+            views = []
+            for t in self.textures:
+                self._keep.append(t)
+                views.append(Texture(t.shape[1], t.shape[0], t.ctypes.data))
+            d.textures, d.texture_count = self._array(Texture, views)
+        if any(e is not None for e in self.material_exts):
+            exts = [self.material_exts[i] if i < len(self.material_exts) and self.material_exts[i] is not None
+                    else make_material_ext() for i in range(len(self.materials))]
+            d.material_ext, _ = self._array(MaterialExt, exts)
+        d.use_normal_maps = 1 if self.use_normal_maps else 0
+        d.normal_map_minimum_cosine = self.normal_map_minimum_cosine
         if self.environment_rgb is not None:
             env = self.environment_rgb
             self._keep.append(env)
@@ -542,7 +615,8 @@ def compact(items, flags, capacity):
 
 def render_config(width, height, spp=1, max_depth=8, rr_start=3, block_size=256, rng=RNG_PCG32, clamp=10.0,
                   mode=MODE_GRAPH, tile=None, persistent_blocks=0, intersection_debug=False,
-                  profile_queues=True) -> RenderConfig:
+                  profile_queues=True, caustic_photons=0, caustic_max_depth=8, caustic_radius=0.06,
+                  caustic_gather=True, caustic_spectral=False, caustic_grid=None) -> RenderConfig:
     rc = RenderConfig()
     rc.width, rc.height = width, height
     rc.tiled = 1 if tile else 0
@@ -553,6 +627,12 @@ def render_config(width, height, spp=1, max_depth=8, rr_start=3, block_size=256,
     rc.intersection_debug = 1 if intersection_debug else 0
     rc.shading_normal_debug = 0
     rc.profile_queues = 1 if profile_queues else 0
+    rc.caustic_photons, rc.caustic_max_depth, rc.caustic_gather_radius = caustic_photons, caustic_max_depth, caustic_radius
+    rc.caustic_gather = 1 if caustic_gather else 0
+    rc.caustic_spectral = 1 if caustic_spectral else 0
+    if caustic_grid is not None:  # (resolution, (min xyz), (max xyz))
+        rc.caustic_grid_resolution = caustic_grid[0]
+        rc.caustic_grid_min, rc.caustic_grid_max = vec3(caustic_grid[1]), vec3(caustic_grid[2])
     return rc
 
 
