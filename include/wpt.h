@@ -332,6 +332,8 @@ typedef struct wpt_stats {
     double trace_milliseconds; /* device time of the last render call */
     uint32_t queue_counts[64]; /* last launch, per bounce */
     uint32_t queue_count_depth;
+    uint32_t caustic_photons_stored;    /* photons in the caustic grid */
+    double caustic_build_milliseconds;  /* photon trace + grid build at creation */
 } wpt_stats;
 
 typedef struct wpt_scene wpt_scene;
@@ -423,6 +425,10 @@ WPT_API int wpt_renderer_render_samples(wpt_renderer* renderer, uint32_t first_s
 /* Copies the unnormalized framebuffer (w*h*3 floats) to host memory and
  * reports the accumulated sample count (writeFinalImage's source data). */
 WPT_API int wpt_renderer_read_framebuffer(wpt_renderer* renderer, float* host_rgb, uint32_t* sample_count);
+/* Restores a framebuffer of unnormalized sums (w*h*3 floats) and the sample
+ * counter, so that rendering resumes at sample_count (the device half of
+ * loadRenderSessionCheckpoint, render_checkpoint.cpp:187-192). */
+WPT_API int wpt_renderer_load_framebuffer(wpt_renderer* renderer, const float* host_rgb, uint32_t sample_count);
 /* Full render: reset + all spp + read back (RenderDriver::run). */
 WPT_API int wpt_render_frame(wpt_renderer* renderer, float* host_rgb, uint32_t* sample_count);
 /* One task: validates fingerprint/tile/sample range, clears the tile,

@@ -137,6 +137,11 @@ uint64_t pixelRngKey(uint32_t pixel, uint32_t sample, uint32_t strategy) {  // r
     return (static_cast<uint64_t>(sample) << 32) | pixel;
 }
 
+uint64_t photonRngKey(uint32_t photon, uint32_t seed, uint32_t strategy) {  // rng.cuh:43-51
+    if (strategy == WPT_RNG_XORSHIFT32) return static_cast<uint64_t>(seed ^ (photon * 0x9e3779b9u));
+    return (static_cast<uint64_t>(seed) << 32) | photon;
+}
+
 float randomFloat(RngState& s) {  // rng.cu:30-41
     uint32_t value;
     if (s.strategy == WPT_RNG_XORSHIFT32) {
@@ -1589,6 +1594,189 @@ bool escapes(const oracle_handle& h, const Hit<R>& hit, const V3<R>& dir, Counte
     return !blocked;
 }
 
+// ---------------------------------------------------------------- caustic photon map
+// photon_mapping.cu:4-163 with render_driver.cu:180-234's grid.  A cell's
+// photons are visited in photon order (the reference's linked lists are
+// filled by atomic exchanges, in no fixed order).
+template <class R>
+struct OPhoton {
+    V3<R> position, direction, flux;
+};
+
+template <class R>
+struct PhotonMapT {
+    std::vector<OPhoton<R>> photons;  // grouped by cell, photon order within a cell
+    std::vector<uint32_t> start, end; // per cell
+    uint32_t res = 64;
+    R radius = R(0.06f);
+    V3<R> lo, hi;
+    uint32_t stored = 0;
+};
+
+template <class R>
+bool photonCellOf(const PhotonMapT<R>& m, const V3<R>& p, int& x, int& y, int& z) {
+    V3<R> ext = m.hi - m.lo, off = p - m.lo;
+    R nx = off.x / ext.x, ny = off.y / ext.y, nz = off.z / ext.z;
+    int r = static_cast<int>(m.res);
+    auto clampi = [r](int v) { return std::min(std::max(v, 0), r - 1); };
+    x = clampi(static_cast<int>(nx * R(static_cast<float>(m.res))));
+    y = clampi(static_cast<int>(ny * R(static_cast<float>(m.res))));
+    z = clampi(static_cast<int>(nz * R(static_cast<float>(m.res))));
+    return !(nx < R(0) || ny < R(0) || nz < R(0) || nx > R(1) || ny > R(1) || nz > R(1));
+}
+
+// Bins stored photons (given in photon order) into the grid.
+template <class R>
+void binPhotons(PhotonMapT<R>& m, const std::vector<OPhoton<R>>& stored) {
+    uint32_t cells = m.res * m.res * m.res;
+    std::vector<std::vector<uint32_t>> bins(cells);
+    for (uint32_t i = 0; i < stored.size(); ++i) {
+        int x, y, z;
+        if (!photonCellOf(m, stored[i].position, x, y, z)) continue;
+        bins[(static_cast<uint32_t>(z) * m.res + static_cast<uint32_t>(y)) * m.res + static_cast<uint32_t>(x)].push_back(i);
+    }
+    m.start.assign(cells, 0);
+    m.end.assign(cells, 0);
+    m.photons.clear();
+    for (uint32_t c = 0; c < cells; ++c) {
+        m.start[c] = static_cast<uint32_t>(m.photons.size());
+        for (uint32_t i : bins[c]) m.photons.push_back(stored[i]);
+        m.end[c] = static_cast<uint32_t>(m.photons.size());
+    }
+    m.stored = static_cast<uint32_t>(m.photons.size());
+}
+
+// emitPhotons + tracePhotons for `count` photons; the survivors in photon order.
+template <class R>
+std::vector<OPhoton<R>> tracePhotonSet(const oracle_handle& h, uint32_t count, uint32_t maxDepth, uint32_t strategy,
+    bool spectral) {
+    std::vector<OPhoton<R>> out;
+    uint32_t lights = static_cast<uint32_t>(h.lights.size());
+    if (!lights) return out;
+    for (uint32_t i = 0; i < count; ++i) {
+        RngState rng = makeRngState(photonRngKey(i, 0x13579bdfu, strategy), strategy);
+        uint32_t candidate = std::min(static_cast<uint32_t>(randomFloat(rng) * lights), lights - 1);
+        uint32_t li = randomFloat(rng) < h.alias[candidate].probability ? candidate : h.alias[candidate].alias;
+        const TriangleLight& L = h.lights[li];
+        V3<R> v0(R(L.v0.x), R(L.v0.y), R(L.v0.z)), v1(R(L.v1.x), R(L.v1.y), R(L.v1.z)), v2(R(L.v2.x), R(L.v2.y), R(L.v2.z));
+        V3<R> ln(R(L.normal.x), R(L.normal.y), R(L.normal.z));
+        R rootU = std::sqrt(rnd<R>(rng));
+        R v = rnd<R>(rng);
+        V3<R> point = (R(1) - rootU) * v0 + rootU * ((R(1) - v) * v1 + v * v2);
+        V3<R> dir = sampleCosineHemisphere(ln, rng);
+        const wpt_material& em = mat(h, L.material);
+        uint32_t channel = 3;
+        V3<R> flux = vec<R>(em.emission) * (R(L.area) * R(3.14159265359f) / R(static_cast<float>(count)));
+        if (spectral) {
+            channel = std::min(static_cast<uint32_t>(randomFloat(rng) * 3.0f), 2u);
+            R cf = R(em.emission[channel]) * (R(L.area) * R(3.14159265359f) * R(3) / R(static_cast<float>(count)));
+            flux = V3<R>(channel == 0 ? cf : R(0), channel == 1 ? cf : R(0), channel == 2 ? cf : R(0));
+        }
+        Ray<R> ray{point + ln * R(0.001f), dir};
+        uint32_t specular = 0;
+        for (uint32_t depth = 0; depth < maxDepth; ++depth) {
+            Hit<R> hit;
+            HitState<R> hs;
+            if (!traceRay(h, ray, hit, hs) || hit.material >= h.d.material_count) break;
+            const wpt_material& m = mat(h, hit.material);
+            V3<R> n = dot(hit.normal, -ray.d) >= R(0) ? hit.normal : -hit.normal;
+            if (m.type == WPT_MATERIAL_DIFFUSE) {
+                if (specular) out.push_back(OPhoton<R>{hit.position, -ray.d, flux});
+                break;
+            }
+            V3<R> next;
+            if (m.type == WPT_MATERIAL_METAL) {
+                next = normalize(reflect(normalize(ray.d), n));
+            } else if (m.type == WPT_MATERIAL_DIELECTRIC) {
+                R ior = R(m.ior);
+                float dispersion = h.ext[hit.material].dispersion;
+                if (dispersion > 0.0f && channel < 3) ior += (R(static_cast<float>(channel)) - R(1)) * R(dispersion);
+                Interface<R> b = dielectricInterface(normalize(ray.d), hit.geometricNormal, hit.normal, ior);
+                bool reflected = b.reflectance >= R(1) || b.reflectance > rnd<R>(rng);
+                next = reflected ? b.reflection : b.transmission;
+            } else {
+                next = sampleCosineHemisphere(n, rng);
+            }
+            if (m.type == WPT_MATERIAL_METAL || m.type == WPT_MATERIAL_DIELECTRIC) ++specular;
+            flux = hadamard(flux, vec<R>(m.albedo));
+            ray = Ray<R>{offsetRayOrigin(hit.position, hit.geometricNormal, next, hit.t), next};
+        }
+    }
+    return out;
+}
+
+template <class R>
+void photonGrid(const wpt_render_config& c, PhotonMapT<R>& m) {
+    m.res = c.caustic_grid_resolution ? c.caustic_grid_resolution : 64u;
+    m.radius = R(c.caustic_gather_radius > 0.0f ? c.caustic_gather_radius : 0.06f);
+    bool box = c.caustic_grid_min[0] < c.caustic_grid_max[0] && c.caustic_grid_min[1] < c.caustic_grid_max[1] &&
+        c.caustic_grid_min[2] < c.caustic_grid_max[2];
+    m.lo = box ? vec<R>(c.caustic_grid_min) : V3<R>(R(-2.5f), R(-1.0f), R(-2.5f));
+    m.hi = box ? vec<R>(c.caustic_grid_max) : V3<R>(R(2.5f), R(3.0f), R(2.5f));
+}
+
+// gatherCaustics, photon_mapping.cu:96-163.
+template <class R>
+V3<R> gatherPhotons(const PhotonMapT<R>& m, const V3<R>& p, const V3<R>& n) {
+    const uint32_t kNearest = 64, kMinimum = 16;
+    int cx, cy, cz;
+    photonCellOf(m, p, cx, cy, cz);
+    V3<R> ext = m.hi - m.lo;
+    R search = std::fmax(m.radius * R(4), R(0.1f));
+    int cellRadius = std::max(1, static_cast<int>(std::ceil(search * R(static_cast<float>(m.res)) /
+        std::fmax(ext.x, R(0.0001f)))));
+    R search2 = search * search;
+    std::vector<R> dist;
+    std::vector<uint32_t> idx;
+    uint32_t far = 0;
+    R far2 = R(0);
+    int res = static_cast<int>(m.res);
+    for (int z = cz - cellRadius; z <= cz + cellRadius; ++z)
+        for (int y = cy - cellRadius; y <= cy + cellRadius; ++y)
+            for (int x = cx - cellRadius; x <= cx + cellRadius; ++x) {
+                if (x < 0 || y < 0 || z < 0 || x >= res || y >= res || z >= res) continue;
+                uint32_t cell = (static_cast<uint32_t>(z) * m.res + static_cast<uint32_t>(y)) * m.res + static_cast<uint32_t>(x);
+                for (uint32_t i = m.start[cell]; i < m.end[cell]; ++i) {
+                    V3<R> d = m.photons[i].position - p;
+                    R d2 = dot(d, d);
+                    if (d2 > search2 || dot(n, m.photons[i].direction) <= R(0)) continue;
+                    if (dist.size() < kNearest) {
+                        dist.push_back(d2);
+                        idx.push_back(i);
+                        if (d2 > far2) {
+                            far2 = d2;
+                            far = static_cast<uint32_t>(dist.size() - 1);
+                        }
+                    } else if (d2 < far2) {  // replace the farthest, then find the new farthest
+                        dist[far] = d2;
+                        idx[far] = i;
+                        far = 0;
+                        far2 = dist[0];
+                        for (uint32_t k = 1; k < kNearest; ++k)
+                            if (dist[k] > far2) {
+                                far2 = dist[k];
+                                far = k;
+                            }
+                    }
+                }
+            }
+    if (dist.empty()) return V3<R>();
+    R g2 = search2;
+    if (dist.size() >= kMinimum) {
+        g2 = R(0);
+        for (R d2 : dist) g2 = std::fmax(g2, d2);
+        g2 = std::fmax(g2, m.radius * m.radius);
+    }
+    R g = std::sqrt(g2);
+    V3<R> sum;
+    for (size_t k = 0; k < dist.size(); ++k) {
+        const OPhoton<R>& ph = m.photons[idx[k]];
+        R cone = std::fmax(R(0), R(1) - std::sqrt(dist[k]) / g);
+        sum += ph.flux * (cone * dot(n, ph.direction));
+    }
+    return sum * (R(3) / (R(3.14159265359f) * g2));
+}
+
 // continueGltfPbrSurface, device_traversal.cuh:727-787.
 template <class R>
 V3<R> gltfContinue(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit, const GSurf<R>& g,
@@ -1667,7 +1855,7 @@ V3<R> gltfContinue(const oracle_handle& h, PathState<R>& path, const Hit<R>& hit
 // One path, shadePaths (device_path_shading.cuh:6-173) applied bounce by bounce.
 template <class R>
 V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, uint32_t rrStart, int debug,
-    Counters& cnt) {
+    Counters& cnt, const PhotonMapT<R>* photons = nullptr) {
     const R invPi = R(0.31830988618f);
     for (uint32_t bounce = 0; bounce < maxDepth; ++bounce) {
         Hit<R> hit;
@@ -1767,6 +1955,9 @@ V3<R> tracePath(const oracle_handle& h, PathState<R> path, uint32_t maxDepth, ui
             sampleDirectLight(h, path, hit, shaded, vx, cnt);
             samplePunctual(h, path, hit, shaded, vx, cnt);
             sampleDirectEnvironment(h, path, hit, shaded, vx, cnt);
+            if (photons)  // caustics gathered at the diffuse vertex (device_path_shading.cuh:99-102)
+                addPathRadiance(path, hadamard(path.throughput, hadamard(albedo, gatherPhotons(*photons, hit.position,
+                    shaded.normal))));
             // continueDiffuseSurface, device_traversal.cuh:685-704
             R mix = envMixture<R>(h);
             if (mix > R(0) && rnd<R>(path.rng) < mix) {
@@ -1888,7 +2079,8 @@ PathState<R> primary(const wpt_camera& cam, uint32_t x, uint32_t y, uint32_t wid
 
 template <class R>
 void renderRows(const oracle_handle& h, const wpt_camera& cam, const wpt_render_config& cfg, uint32_t x0, uint32_t y0,
-    uint32_t w, uint32_t th, uint32_t s0, uint32_t sc, uint32_t first, uint32_t step, float* fb, Counters& cnt) {
+    uint32_t w, uint32_t th, uint32_t s0, uint32_t sc, uint32_t first, uint32_t step, float* fb, Counters& cnt,
+    const PhotonMapT<R>* photons) {
     int debug = cfg.intersection_debug ? 1 : (cfg.shading_normal_debug ? 2 : 0);
     for (uint32_t row = first; row < th; row += step) {
         uint32_t y = y0 + row;
@@ -1896,7 +2088,7 @@ void renderRows(const oracle_handle& h, const wpt_camera& cam, const wpt_render_
             float* px = fb + 3 * (static_cast<size_t>(y) * cfg.width + x);
             for (uint32_t s = s0; s < s0 + sc; ++s) {
                 PathState<R> p = primary<R>(cam, x, y, cfg.width, cfg.height, s, cfg.rng_strategy, cfg.indirect_clamp);
-                V3<R> L = tracePath(h, p, cfg.max_depth, cfg.rr_start_depth, debug, cnt);
+                V3<R> L = tracePath(h, p, cfg.max_depth, cfg.rr_start_depth, debug, cnt, photons);
                 px[0] += static_cast<float>(L.x);
                 px[1] += static_cast<float>(L.y);
                 px[2] += static_cast<float>(L.z);
@@ -2399,6 +2591,36 @@ void oracle_material_terms(const oracle_handle* h, uint32_t material, const floa
     }
 }
 
+uint32_t oracle_photon_map(const oracle_handle* h, const wpt_render_config* config, float* photons9, uint32_t capacity) {
+    PhotonMapT<float> m;
+    photonGrid(*config, m);
+    binPhotons(m, tracePhotonSet<float>(*h, config->caustic_photons,
+        config->caustic_max_depth ? config->caustic_max_depth : 8u, config->rng_strategy, config->caustic_spectral != 0));
+    for (uint32_t i = 0; i < std::min(capacity, m.stored); ++i) {
+        const OPhoton<float>& q = m.photons[i];
+        float row[9] = {q.position.x, q.position.y, q.position.z, q.direction.x, q.direction.y, q.direction.z,
+            q.flux.x, q.flux.y, q.flux.z};
+        std::memcpy(photons9 + 9ull * i, row, sizeof row);
+    }
+    return m.stored;
+}
+
+void oracle_gather_photons(const wpt_render_config* config, const float* photons9, uint32_t n, const float* point,
+    const float* normal, float* out) {
+    PhotonMapT<float> m;
+    photonGrid(*config, m);
+    std::vector<OPhoton<float>> set(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = photons9 + 9 * i;
+        set[i] = OPhoton<float>{V3<float>(q[0], q[1], q[2]), V3<float>(q[3], q[4], q[5]), V3<float>(q[6], q[7], q[8])};
+    }
+    binPhotons(m, set);
+    V3<float> e = gatherPhotons(m, V3<float>(point[0], point[1], point[2]), V3<float>(normal[0], normal[1], normal[2]));
+    out[0] = e.x;
+    out[1] = e.y;
+    out[2] = e.z;
+}
+
 void oracle_punctual_sample(const wpt_punctual_light* light, const float* point, float* direction, float* incident,
     float* distance) {
     V3<float> in;
@@ -2575,13 +2797,31 @@ int oracle_render(const oracle_handle* h, const wpt_camera* camera, const wpt_re
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     threads = std::min(threads, std::max(1u, th));
     std::vector<Counters> counters(threads);
+    // The caustic map, as the renderer builds it on creation.
+    PhotonMapT<float> map32;
+    PhotonMapT<double> map64;
+    bool gather = config->caustic_photons && config->caustic_gather;
+    if (config->caustic_photons) {
+        uint32_t depth = config->caustic_max_depth ? config->caustic_max_depth : 8u;
+        if (precision == 64) {
+            photonGrid(*config, map64);
+            binPhotons(map64, tracePhotonSet<double>(*h, config->caustic_photons, depth, config->rng_strategy,
+                config->caustic_spectral != 0));
+        } else {
+            photonGrid(*config, map32);
+            binPhotons(map32, tracePhotonSet<float>(*h, config->caustic_photons, depth, config->rng_strategy,
+                config->caustic_spectral != 0));
+        }
+    }
     std::vector<std::thread> pool;
     for (uint32_t t = 0; t < threads; ++t) {
         pool.emplace_back([&, t] {
             if (precision == 64)
-                renderRows<double>(*h, *camera, *config, x, y, w, th, s0, sc, t, threads, fb, counters[t]);
+                renderRows<double>(*h, *camera, *config, x, y, w, th, s0, sc, t, threads, fb, counters[t],
+                    gather ? &map64 : nullptr);
             else
-                renderRows<float>(*h, *camera, *config, x, y, w, th, s0, sc, t, threads, fb, counters[t]);
+                renderRows<float>(*h, *camera, *config, x, y, w, th, s0, sc, t, threads, fb, counters[t],
+                    gather ? &map32 : nullptr);
         });
     }
     for (std::thread& t : pool) t.join();
@@ -2600,6 +2840,7 @@ int oracle_render(const oracle_handle* h, const wpt_camera* camera, const wpt_re
             stats->shadow_instance_entries += c.shadowWork.instances;
         }
         stats->paths = static_cast<uint64_t>(w) * th * sc;
+        stats->caustic_photons_stored = precision == 64 ? map64.stored : map32.stored;
     }
     return 0;
 }

@@ -39,6 +39,7 @@ typedef struct oracle_stats {
      * triangle tests, sphere tests and mesh-instance entries. */
     uint64_t closest_node_tests, closest_triangle_tests, closest_sphere_tests, closest_instance_entries;
     uint64_t shadow_node_tests, shadow_triangle_tests, shadow_sphere_tests, shadow_instance_entries;
+    uint32_t caustic_photons_stored; /* photons in the caustic grid (config.caustic_photons > 0) */
 } oracle_stats;
 
 typedef struct oracle_handle oracle_handle;
@@ -67,6 +68,16 @@ void oracle_gltf_reflection_normal(const float* normal, const float* geometric, 
  * mapped normal[3]; for glTF materials dielectric F0[3], F90, coat, coat
  * roughness, coat normal[3] (25 floats). */
 void oracle_material_terms(const oracle_handle* h, uint32_t material, const float* uv, const float* uv1, float* out);
+
+/* Caustic gather (FP32) over a given photon set: `photons9` holds n rows
+ * of position[3], direction[3], flux[3] in photon order; they are binned in
+ * the grid of `config` (resolution, box, radius) and gathered at `point`
+ * with `normal`. */
+void oracle_gather_photons(const wpt_render_config* config, const float* photons9, uint32_t n, const float* point,
+    const float* normal, float* out);
+/* The caustic map of `config` (FP32): the stored photons in grid order as
+ * rows of position, direction, flux; returns their count. */
+uint32_t oracle_photon_map(const oracle_handle* h, const wpt_render_config* config, float* photons9, uint32_t capacity);
 
 /* samplePunctualLight (FP32): direction toward the light, arriving
  * intensity and distance (1e30 for directional lights) at `point`. */

@@ -34,7 +34,8 @@ class OracleDiffConfig(C.Structure):
 class OracleStats(C.Structure):
     _fields_ = [("closest_queries", C.c_uint64), ("shadow_queries", C.c_uint64), ("paths", C.c_uint64)] + [
         (f"{kind}_{what}", C.c_uint64) for kind in ("closest", "shadow")
-        for what in ("node_tests", "triangle_tests", "sphere_tests", "instance_entries")]
+        for what in ("node_tests", "triangle_tests", "sphere_tests", "instance_entries")] + [
+        ("caustic_photons_stored", C.c_uint32)]
 
 
 _lib = None
@@ -66,6 +67,8 @@ def library():
             ("oracle_diff_directional64", [P, C.POINTER(wpt.Camera), C.POINTER(OracleDiffConfig), P, P, u32, u32, P],
              C.c_int),
             ("oracle_punctual_sample", [C.POINTER(wpt.PunctualLight), P, P, P, P], None),
+            ("oracle_gather_photons", [C.POINTER(wpt.RenderConfig), P, u32, P, P, P], None),
+            ("oracle_photon_map", [P, C.POINTER(wpt.RenderConfig), P, u32], u32),
             ("oracle_gltf_eval", [P, P, P, C.c_float, P, P, P], None),
             ("oracle_gltf_sample_reflection", [P, P, C.c_float, C.c_float, C.c_float, P], None),
             ("oracle_gltf_reflection_normal", [P, P, P, P], None),
@@ -116,6 +119,16 @@ def punctual_sample(light, point):
     dist = C.c_float()
     library().oracle_punctual_sample(C.byref(light), p.ctypes.data, d.ctypes.data, inc.ctypes.data, C.byref(dist))
     return d, inc, dist.value
+
+
+def gather_photons(config, photons9, point, normal):
+    """gatherCaustics over a photon set (n, 9): position, direction, flux."""
+    ph = np.ascontiguousarray(photons9, dtype=np.float32).reshape(-1, 9)
+    out = np.zeros(3, np.float32)
+    p, n = f32(point), f32(normal)
+    library().oracle_gather_photons(C.byref(config), ph.ctypes.data, ph.shape[0], p.ctypes.data, n.ctypes.data,
+                                    out.ctypes.data)
+    return out
 
 
 def f32(v):
@@ -284,6 +297,14 @@ class Oracle:
                 "roughness": out[8], "specular_f0": out[9:12], "thickness": out[12], "normal": out[13:16],
                 "dielectric_f0": out[16:19], "f90": out[19], "coat": out[20], "coat_roughness": out[21],
                 "coat_normal": out[22:25]}
+
+    def photon_map(self, config):
+        """Stored caustic photons (n, 9): position, direction, flux, in grid order."""
+        lib = library()
+        n = lib.oracle_photon_map(self._h, C.byref(config), None, 0)
+        out = np.zeros((max(n, 1), 9), np.float32)
+        lib.oracle_photon_map(self._h, C.byref(config), out.ctypes.data, n)
+        return out[:n]
 
     def env_tables(self):
         """(probabilities, alias probability, alias index) of the scene's map."""

@@ -65,6 +65,32 @@ def test_command_line_flags():
     assert (rc.tiled, rc.tile_width, rc.tile_height) == (1, 64, 32)
 
 
+@pytest.mark.parametrize("scene,traced,spectral", [("prism", True, True), ("crystal", True, True),
+                                                    ("cornell", True, False), ("hurricane", True, False),
+                                                    ("cornell-box", False, False), ("sponza", False, False)])
+def test_caustic_flags_follow_the_preset(scene, traced, spectral):
+    # render_driver.cu:37-40, 180-196: photons only for the caustic presets,
+    # one spectral channel per photon for prism and crystal.
+    status, rc, _, _ = wpt.parse_command_line(["--scene", scene, "--caustics", "--caustic-photons", "4096",
+                                               "--caustic-radius", "0.03", "--no-caustic-gather"])
+    assert status == wpt.OK
+    if not traced:
+        assert rc.caustic_photons == 0
+        return
+    assert (rc.caustic_photons, rc.caustic_max_depth, rc.caustic_gather) == (4096, 8, 0)
+    assert abs(rc.caustic_gather_radius - 0.03) < 1e-7 and rc.caustic_spectral == int(spectral)
+    status, rc, _, _ = wpt.parse_command_line(["--scene", scene])
+    assert rc.caustic_photons == 0
+
+
+def test_caustic_flags_change_the_fingerprint():
+    _, _, _, a = wpt.parse_command_line(["--scene", "prism"])
+    _, _, _, b = wpt.parse_command_line(["--scene", "prism", "--caustics"])
+    _, _, _, c = wpt.parse_command_line(["--scene", "prism", "--caustics", "--no-caustic-gather"])
+    _, _, _, d = wpt.parse_command_line(["--scene", "prism", "--caustics", "--caustic-radius", "0.05"])
+    assert len({a, b, c, d}) == 4
+
+
 @pytest.mark.parametrize("args", [["--clamp-indirect", "-1"], ["--clamp-indirect", "nan"], ["--rng", "mt"],
                                   ["--scene", "nope"], ["--bogus"], ["--worker"], ["--coordinator", "--worker"],
                                   ["--listen-port", "70000"], ["--samples-per-task", "0"], ["--help"]])
@@ -81,6 +107,44 @@ def test_fingerprint_sensitivity():
     _, _, _, e = wpt.parse_command_line(["--scene", "sphere-field"])
     assert a == b
     assert len({a, c, d, e}) == 4
+
+
+def test_fingerprint_covers_the_reference_fields():
+    # config_tests.cpp:129-141: stable, and changed by the depth, the
+    # indirect clamp, the RNG and (render_fingerprint.cpp:134-138) the
+    # caustic settings.
+    def fp(*args):
+        status, _, _, value = wpt.parse_command_line(["--scene", "demo"] + list(args))
+        assert status == wpt.OK
+        return value
+    a = fp()
+    assert a == fp()
+    others = [fp("--max-depth", "33"), fp("--clamp-indirect", "0"), fp("--rng", "pcg32"), fp("--caustics"),
+              fp("--caustic-photons", "1000"), fp("--tile", "64x64"), fp("--persistent-wavefront")]
+    assert len({a, *others}) == 1 + len(others)
+
+
+def test_scene_fingerprint_ignores_the_name_but_not_the_shading(tmp_path):
+    # config_tests.cpp:163-172 and environment_tests.cpp:95-99: the scene
+    # identity leaves out the display name, and follows the normal-map
+    # switch, the environment's gain and rotation and the map's bytes.
+    hdr = tmp_path / "sky.hdr"
+    hdr.write_bytes(b"#?RADIANCE\n")
+    def fp(**fields):
+        doc = {"name": "a", "preset": "demo",
+               "environment": {"file": "sky.hdr", "intensity": 2.5, "rotation_degrees": 90}}
+        doc.update(fields)
+        return wpt.scene_fingerprint(write_manifest(tmp_path, doc))
+    base = fp()
+    assert fp(name="renamed") == base
+    assert fp(normal_maps={"enabled": False}) != base
+    assert fp(environment={"file": "sky.hdr", "intensity": 5.0, "rotation_degrees": 90}) != base
+    assert fp(environment={"file": "sky.hdr", "intensity": 2.5, "rotation_degrees": 91}) != base
+    hdr.write_bytes(b"#?RADIANCE\nchanged")
+    assert fp() != base
+    assert wpt.scene_fingerprint("demo") != base
+    with pytest.raises(wpt.WptError):
+        wpt.scene_fingerprint("no-such-preset")
 
 
 def write_manifest(tmp_path, obj, name="scene.json"):

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../../include/wpt.h"
+#include "last_error.h"
 #include "../kernels/diff_kernels.h"
 #include "../kernels/wavefront.h"
 #include "../scene/scene_build.h"
@@ -32,6 +33,13 @@ int fail(int code, const std::string& message) {
     gLastError = message;
     return code;
 }
+
+}  // namespace
+
+// Host-layer entry points report errors through the same message.
+int wpt::setLastError(int code, const std::string& message) { return fail(code, message); }
+
+namespace {
 
 int hipFail(hipError_t e, const char* what) {
     return fail(e == hipErrorOutOfMemory ? WPT_ERROR_OUT_OF_MEMORY : WPT_ERROR_DEVICE,
@@ -108,6 +116,8 @@ struct wpt_renderer {
     uint32_t currentTile = 0xffffffffu;
     double lastMs = 0.0;
     uint64_t lastSamples = 0;
+    double photonMs = 0.0;       // caustic photon map build
+    uint32_t photonsStored = 0;  // photons kept in the grid
     // Regenerate mode: the batch consumer over the renderer's tiles.
     BatchContext regen{};
     uint32_t regenBlocks = 0;
@@ -541,6 +551,15 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     R_HIP(hipEventCreate(&r->end));
     LaunchContext& ctx = r->ctx;
     ctx.scene = scene->dev;
+    // Caustic photons gathered at diffuse vertices need the full-feature
+    // kernels (render_driver.cu:180-234 builds the map per driver).
+    if (c.caustic_photons) {
+        if (c.caustic_photons > 0x7fffffffu) return cleanup(fail(WPT_ERROR_INVALID_ARGUMENT, "too many caustic photons"));
+        if (!(c.caustic_gather_radius >= 0.0f) || !std::isfinite(c.caustic_gather_radius) ||
+            c.caustic_grid_resolution > 1024)
+            return cleanup(fail(WPT_ERROR_INVALID_ARGUMENT, "invalid caustic gather radius or grid resolution"));
+        if (c.caustic_gather) ctx.scene.fullFeatures = 1;
+    }
     ctx.camera.position = Vec3(camera->position[0], camera->position[1], camera->position[2]);
     ctx.camera.forward = Vec3(camera->forward[0], camera->forward[1], camera->forward[2]);
     ctx.camera.right = Vec3(camera->right[0], camera->right[1], camera->right[2]);
@@ -562,7 +581,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     // Trace, shade and shadow kernels run one item per thread over
     // gridBlocks workgroups; the persistent kernel runs a resident grid.
     Occupancy occ{};
-    R_HIP(kernelOccupancy(c.block_size, scene->dev.fullFeatures != 0, &occ));
+    R_HIP(kernelOccupancy(c.block_size, ctx.scene.fullFeatures != 0, &occ));
     int device = 0, cus = 0, coop = 0;
     R_HIP(hipGetDevice(&device));
     R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -622,14 +641,14 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     R_HIP(hipMemset(ctx.framebuffer, 0, sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height));
     if (c.mode == WPT_MODE_REGENERATE) {
         int perCu = 0;
-        R_HIP(batchOccupancy(c.block_size, scene->dev.fullFeatures != 0, &perCu));
+        R_HIP(batchOccupancy(c.block_size, ctx.scene.fullFeatures != 0, &perCu));
         if (perCu <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the regenerating kernel"));
         uint32_t resident = static_cast<uint32_t>(perCu) * cu;
         r->regenBlocks = c.persistent_blocks ? std::min(c.persistent_blocks, resident) : resident;
         BatchContext& b = r->regen;
         R_HIP(r->arena.alloc(sizeof(DevScene), &p));
         b.scenes = static_cast<const DevScene*>(p);
-        R_HIP(hipMemcpy(p, &scene->dev, sizeof(DevScene), hipMemcpyHostToDevice));
+        R_HIP(hipMemcpy(p, &ctx.scene, sizeof(DevScene), hipMemcpyHostToDevice));
         size_t tasks = r->tiles.size();
         R_HIP(r->arena.alloc(sizeof(BatchTask) * tasks, &p));
         b.tasks = static_cast<const BatchTask*>(p);
@@ -639,7 +658,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
         b.counters = static_cast<BatchCounters*>(p);
         R_HIP(hipMemset(p, 0, sizeof(BatchCounters)));
         b.out = ctx.framebuffer;
-        b.full = scene->dev.fullFeatures;
+        b.full = ctx.scene.fullFeatures;
     }
     // One spill column per thread index of any grid.
     ctx.spillThreads = std::max(std::max(ctx.gridBlocks, ctx.persistentBlocks), r->regenBlocks) * c.block_size;
@@ -647,6 +666,53 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     ctx.spill = static_cast<uint2*>(p);
     r->regen.spill = ctx.spill;
     r->regen.spillThreads = ctx.spillThreads;
+    if (c.caustic_photons) {
+        PhotonSettings ps{c.caustic_photons, c.caustic_max_depth ? c.caustic_max_depth : 8u, 0x13579bdfu,
+            c.rng_strategy, c.caustic_spectral ? 1u : 0u};
+        PhotonGridView view{};
+        view.resolution = c.caustic_grid_resolution ? c.caustic_grid_resolution : 64u;
+        view.radius = c.caustic_gather_radius > 0.0f ? c.caustic_gather_radius : 0.06f;
+        bool box = c.caustic_grid_min[0] < c.caustic_grid_max[0] && c.caustic_grid_min[1] < c.caustic_grid_max[1] &&
+            c.caustic_grid_min[2] < c.caustic_grid_max[2];
+        view.minimum = box ? Vec3(c.caustic_grid_min[0], c.caustic_grid_min[1], c.caustic_grid_min[2])
+                           : Vec3(-2.5f, -1.0f, -2.5f);
+        view.maximum = box ? Vec3(c.caustic_grid_max[0], c.caustic_grid_max[1], c.caustic_grid_max[2])
+                           : Vec3(2.5f, 3.0f, 2.5f);
+        uint32_t cells = view.resolution * view.resolution * view.resolution;
+        PhotonMapBuffers pb{};
+        size_t n = c.caustic_photons;
+        R_HIP(r->arena.alloc(sizeof(Photon) * n, &p));
+        pb.photons = static_cast<Photon*>(p);
+        R_HIP(r->arena.alloc(sizeof(Photon) * n, &p));
+        pb.sorted = static_cast<Photon*>(p);
+        R_HIP(r->arena.alloc(sizeof(uint64_t) * n, &p));
+        pb.keys = static_cast<uint64_t*>(p);
+        R_HIP(r->arena.alloc(sizeof(uint64_t) * n, &p));
+        pb.sortedKeys = static_cast<uint64_t*>(p);
+        R_HIP(photonSortBytes(c.caustic_photons, cells, &pb.tempBytes));
+        R_HIP(r->arena.alloc(pb.tempBytes > 0 ? pb.tempBytes : 1, &p));
+        pb.temp = p;
+        R_HIP(r->arena.alloc(sizeof(uint32_t) * cells, &p));
+        pb.cellStart = static_cast<uint32_t*>(p);
+        R_HIP(r->arena.alloc(sizeof(uint32_t) * cells, &p));
+        pb.cellEnd = static_cast<uint32_t*>(p);
+        R_HIP(r->arena.alloc(sizeof(uint32_t), &p));
+        pb.stored = static_cast<uint32_t*>(p);
+        view.photons = pb.sorted;
+        view.cellStart = pb.cellStart;
+        view.cellEnd = pb.cellEnd;
+        R_HIP(r->arena.alloc(sizeof(PhotonGridView), &p));
+        R_HIP(hipMemcpy(p, &view, sizeof(PhotonGridView), hipMemcpyHostToDevice));
+        if (c.caustic_gather) ctx.shade.photons = static_cast<const PhotonGridView*>(p);
+        R_HIP(hipEventRecord(r->start, r->stream));
+        R_HIP(enqueuePhotonMap(ctx.scene, ps, pb, view, ctx.spill, ctx.spillThreads, r->stream));
+        R_HIP(hipEventRecord(r->end, r->stream));
+        R_HIP(hipEventSynchronize(r->end));
+        float ms = 0.0f;
+        R_HIP(hipEventElapsedTime(&ms, r->start, r->end));
+        r->photonMs = ms;
+        R_HIP(hipMemcpy(&r->photonsStored, pb.stored, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
     R_HIP(hipDeviceSynchronize());
 #undef R_HIP
     *out = r;
@@ -721,6 +787,17 @@ WPT_API int wpt_renderer_read_framebuffer(wpt_renderer* r, float* host, uint32_t
         WPT_HIP(hipMemcpy(&c, r->ctx.counters, sizeof c, hipMemcpyDeviceToHost));
         *sampleCount = c.sampleIndex;
     }
+    return WPT_OK;
+}
+
+WPT_API int wpt_renderer_load_framebuffer(wpt_renderer* r, const float* host, uint32_t sampleCount) {
+    if (!r || !host) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    WPT_HIP(hipStreamSynchronize(r->stream));
+    size_t n = 3 * static_cast<size_t>(r->config.width) * r->config.height;
+    WPT_HIP(hipMemcpy(r->ctx.framebuffer, host, sizeof(float) * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, sampleCount);
+    WPT_HIP(hipGetLastError());
+    WPT_HIP(hipStreamSynchronize(r->stream));
     return WPT_OK;
 }
 
@@ -829,6 +906,8 @@ WPT_API int wpt_renderer_last_stats(const wpt_renderer* r, wpt_stats* stats) {
     uint32_t depth = std::min<uint32_t>(64u, r->config.max_depth);
     for (uint32_t i = 0; i < depth; ++i) stats->queue_counts[i] = c.queueCounts[i];
     stats->queue_count_depth = depth;
+    stats->caustic_photons_stored = r->photonsStored;
+    stats->caustic_build_milliseconds = r->photonMs;
     return WPT_OK;
 }
 

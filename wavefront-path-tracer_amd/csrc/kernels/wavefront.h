@@ -5,6 +5,7 @@
 
 #include <cstdint>
 
+#include "photon_trace.h"
 #include "shade.h"
 
 namespace wpt {
@@ -172,17 +173,9 @@ struct BatchContext {
     uint32_t full;  // some scene has an environment map, punctual lights or texture bindings
 };
 
-// Caustic photon map (photon.hip): settings, the renderer's buffers, and
+// Caustic photon map (photon.hip, photon_trace.h): the renderer's buffers, and
 // the build: trace, key, sort, cell ranges.  `view` names the buffers the
 // gather reads (sorted photons and cell ranges) and the grid box.
-struct PhotonSettings {
-    uint32_t count;
-    uint32_t maxDepth;
-    uint32_t seed;
-    uint32_t strategy;
-    uint32_t spectral;
-};
-
 struct PhotonMapBuffers {
     Photon* photons;  // emission order
     Photon* sorted;   // cell order

@@ -479,7 +479,9 @@ template <class Timer>
 hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& timer) {
     size_t lds = stackLdsBytes(ctx.blockSize);
     dim3 grid(ctx.gridBlocks), block(ctx.blockSize);
-    const bool inlineShadows = shadesWithInlineShadows(ctx.scene);
+    // The caustic gather adds radiance after the area light's, so a
+    // deferred shadow ray would reorder the sum.
+    const bool inlineShadows = shadesWithInlineShadows(ctx.scene) || ctx.shade.photons != nullptr;
     const bool full = ctx.scene.fullFeatures != 0;
     hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, grid, block, 0, stream, ctx); });
     for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {

@@ -8,6 +8,7 @@
 #include <fstream>
 #include <iostream>
 #include <sstream>
+#include <vector>
 
 #include "json.hpp"
 
@@ -547,7 +548,7 @@ bool parseCommandLineText(const std::string& command, RenderConfig& render, Scen
     return parseCommandLine(static_cast<int>(argv.size()), argv.data(), render, scene);
 }
 
-wpt_render_config toRenderConfig(const RenderConfig& c) {
+wpt_render_config toRenderConfig(const RenderConfig& c, ScenePreset preset) {
     wpt_render_config r{};
     r.width = c.width;
     r.height = c.height;
@@ -565,6 +566,15 @@ wpt_render_config toRenderConfig(const RenderConfig& c) {
     r.intersection_debug = c.intersectionDebug ? 1u : 0u;
     r.shading_normal_debug = c.shadingNormalDebug ? 1u : 0u;
     r.profile_queues = c.profileQueues ? 1u : 0u;
+    bool causticPreset = preset == ScenePreset::Cornell || preset == ScenePreset::Hurricane ||
+        preset == ScenePreset::Prism || preset == ScenePreset::Crystal;
+    if (c.enableCaustics && causticPreset) {
+        r.caustic_photons = c.causticPhotonCount;
+        r.caustic_max_depth = c.causticMaxDepth;
+        r.caustic_gather_radius = c.causticGatherRadius;
+        r.caustic_gather = c.enableCausticGather ? 1u : 0u;
+        r.caustic_spectral = preset == ScenePreset::Prism || preset == ScenePreset::Crystal ? 1u : 0u;
+    }
     return r;
 }
 
@@ -594,11 +604,42 @@ struct Fnv {
 };
 }  // namespace
 
-uint64_t computeRenderFingerprint(const RenderConfig& render, const SceneConfig& scene) {
-    Fnv f;
+namespace {
+// Versions of the random streams and of the integrator that the reference
+// folds into every fingerprint (render_fingerprint.cpp:9,
+// render_fingerprint.hpp:6).  Images of this renderer match the
+// reference's integrator at that version.
+constexpr uint32_t kRngImplementationVersion = 1;
+constexpr uint32_t kIntegratorVersion = 12;
+
+// A file's contents: byte 0 if it cannot be opened, else byte 1 and every
+// byte of the file.
+void appendAsset(Fnv& f, const std::string& filename) {
+    std::ifstream in(filename, std::ios::binary);
+    if (!in) {
+        f.byte(0);
+        return;
+    }
+    f.byte(1);
+    std::vector<char> buffer(64 * 1024);
+    while (in.read(buffer.data(), static_cast<std::streamsize>(buffer.size())) || in.gcount() > 0)
+        for (std::streamsize i = 0; i < in.gcount(); ++i) f.byte(static_cast<uint8_t>(buffer[static_cast<size_t>(i)]));
+}
+
+// render_fingerprint.cpp:82-120.  The procedural presets that the
+// reference lacks add their generator seed and icosphere frequency last.
+void appendScene(Fnv& f, const SceneConfig& scene, bool includeName) {
     f.u32(static_cast<uint32_t>(scene.preset));
-    f.str(scene.name);
-    f.str(scene.objectFilename);
+    if (includeName) f.str(scene.name);
+    switch (scene.preset) {
+    case ScenePreset::Sponza: appendAsset(f, scene.gltfFilename); break;
+    case ScenePreset::Cornell:
+    case ScenePreset::Hurricane:
+    case ScenePreset::Crystal:
+    case ScenePreset::Deer:
+    case ScenePreset::RtWeek: appendAsset(f, scene.objectFilename); break;
+    default: break;
+    }
     f.f32(scene.gltfScale);
     f.f32(scene.objectScale);
     f.f32(scene.objectTranslation.x);
@@ -611,10 +652,25 @@ uint64_t computeRenderFingerprint(const RenderConfig& render, const SceneConfig&
     f.boolean(scene.removeBackdrop);
     f.boolean(scene.convertObjectMaterialsToDielectric);
     f.u32(static_cast<uint32_t>(scene.objectMaterialOverride));
+    f.boolean(scene.addSponzaTopLight);
+    f.boolean(scene.ignoreSponzaLightOcclusion);
     f.boolean(scene.useNormalMaps);
     f.f32(scene.normalMapMinimumCosine);
-    f.u32(scene.seed);
-    f.u32(scene.icosphereFrequency);
+    f.boolean(!scene.environmentFilename.empty());
+    if (!scene.environmentFilename.empty()) {
+        appendAsset(f, scene.environmentFilename);
+        f.f32(scene.environmentIntensity);
+        f.f32(scene.environmentRotationDegrees);
+    }
+    if (scene.preset == ScenePreset::CornellBox || scene.preset == ScenePreset::SphereField ||
+        scene.preset == ScenePreset::DiffCornell) {
+        f.u32(scene.seed);
+        f.u32(scene.icosphereFrequency);
+    }
+}
+
+// render_fingerprint.cpp:122-145.
+void appendRender(Fnv& f, const RenderConfig& render) {
     f.u32(render.width);
     f.u32(render.height);
     f.u32(render.tileWidth);
@@ -627,8 +683,28 @@ uint64_t computeRenderFingerprint(const RenderConfig& render, const SceneConfig&
     f.boolean(render.intersectionDebug);
     f.boolean(render.shadingNormalDebug);
     f.boolean(render.persistentWavefront);
+    f.boolean(render.enableCaustics);
+    f.boolean(render.enableCausticGather);
+    f.u32(render.causticPhotonCount);
+    f.u32(render.causticMaxDepth);
+    f.f32(render.causticGatherRadius);
     f.f32(render.indirectRadianceClamp);
     f.u32(render.rngStrategy);
+    f.u32(kRngImplementationVersion);
+    f.u32(kIntegratorVersion);
+}
+}  // namespace
+
+uint64_t computeRenderFingerprint(const RenderConfig& render, const SceneConfig& scene) {
+    Fnv f;
+    appendScene(f, scene, true);
+    appendRender(f, render);
+    return f.v;
+}
+
+uint64_t computeSceneFingerprint(const SceneConfig& scene) {
+    Fnv f;
+    appendScene(f, scene, false);
     return f.v;
 }
 

@@ -94,11 +94,19 @@ bool parseCommandLine(int argc, char** argv, RenderConfig& render, SceneConfig& 
 bool parseCommandLineText(const std::string& command, RenderConfig& render, SceneConfig& scene,
     std::vector<std::string>& argumentStorage);
 
-// The C ABI render configuration for a RenderConfig.
-wpt_render_config toRenderConfig(const RenderConfig& c);
+// The C ABI render configuration for a RenderConfig.  Caustic photons are
+// traced only for the caustic presets (Cornell, Hurricane, Prism, Crystal),
+// with one spectral channel per photon for Prism and Crystal
+// (render_driver.cu:37-40, 180-196).
+wpt_render_config toRenderConfig(const RenderConfig& c, ScenePreset preset);
 
-// FNV-1a over the output-affecting fields (reference
-// src/renderer/render_fingerprint.cpp:82-149 fingerprints the same roles).
+// 64-bit FNV-1a over the scene (with its name and the bytes of its asset
+// files) and the output-affecting render settings, in the field order of
+// src/renderer/render_fingerprint.cpp:82-163: the key of checkpoints and
+// tasks.
 uint64_t computeRenderFingerprint(const RenderConfig& render, const SceneConfig& scene);
+// The scene part alone, without the name (computeSceneFingerprint,
+// render_fingerprint.cpp:157-161).
+uint64_t computeSceneFingerprint(const SceneConfig& scene);
 
 }  // namespace wpt

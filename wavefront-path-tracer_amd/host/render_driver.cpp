@@ -1,5 +1,7 @@
 #include "render_driver.hpp"
 
+#include "checkpoint.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -156,7 +158,7 @@ RenderDriver::~RenderDriver() {
 
 bool RenderDriver::prepare() {
     if (renderer_) return true;
-    wpt_render_config rc = toRenderConfig(config_);
+    wpt_render_config rc = toRenderConfig(config_, sceneConfig_.preset);
     if (wpt_renderer_create(scene_, &camera_, &rc, fingerprint_, &renderer_) != WPT_OK) {
         std::cerr << wpt_last_error() << '\n';
         renderer_ = nullptr;
@@ -172,19 +174,71 @@ int RenderDriver::run() {
     std::cout << "Render mode: " << (config_.regenerate ? "persistent path regeneration" :
         (config_.persistentWavefront ? "persistent device wavefront" :
         (config_.tiledRendering ? "tiled" : "full frame"))) << ", " << tiles << " work tiles\n";
-    if (wpt_renderer_reset(renderer_) != WPT_OK ||
-        wpt_renderer_render_samples(renderer_, 0, config_.samplesPerPixel) != WPT_OK) {
+    const uint32_t spp = config_.samplesPerPixel;
+    uint32_t completed = 0;
+    if (config_.resumeFromCheckpoint) {
+        std::vector<float> saved;
+        std::string error;
+        if (!readRenderCheckpoint(config_.checkpointFilename, config_.width, config_.height, fingerprint_, saved,
+                completed, &error)) {
+            std::cerr << error << '\n';
+            return 1;
+        }
+        if (completed > spp) {
+            std::cerr << "Checkpoint has more samples than the current render target\n";
+            return 1;
+        }
+        if (wpt_renderer_load_framebuffer(renderer_, saved.data(), completed) != WPT_OK) {
+            std::cerr << wpt_last_error() << '\n';
+            return 1;
+        }
+        std::cout << "Resuming from " << completed << " completed samples\n";
+    } else if (wpt_renderer_reset(renderer_) != WPT_OK) {
         std::cerr << wpt_last_error() << '\n';
         return 1;
     }
     wpt_stats stats{};
     wpt_renderer_last_stats(renderer_, &stats);
-    std::cout << "Trace time (" << (config_.tiledRendering ? "tiled" : "full frame") << "): "
-              << stats.trace_milliseconds << " ms\n";
-    double seconds = stats.trace_milliseconds / 1000.0;
+    if (stats.caustic_build_milliseconds > 0.0)
+        std::cout << "Traced " << config_.causticPhotonCount << " photons; " << stats.caustic_photons_stored
+                  << " stored in the caustic grid; photon map build: " << stats.caustic_build_milliseconds << " ms\n";
+    // Samples run in steps of the checkpoint interval; a checkpoint is
+    // queued after each step short of the target (render_driver.cu:359-383).
+    const uint32_t step = std::max(1u, (spp * config_.checkpointProgressPercent + 99) / 100);
+    RenderCheckpointWriter checkpoints(config_.width, config_.height, config_.checkpointFilename,
+        config_.checkpointBufferCount, fingerprint_);
+    double traceMs = 0.0;
+    uint64_t queries = 0, samplesTraced = 0;
+    while (completed < spp) {
+        uint32_t count = std::min(step - completed % step, spp - completed);
+        if (wpt_renderer_render_samples(renderer_, completed, count) != WPT_OK) {
+            std::cerr << wpt_last_error() << '\n';
+            return 1;
+        }
+        wpt_renderer_last_stats(renderer_, &stats);
+        traceMs += stats.trace_milliseconds;
+        queries += stats.closest_queries + stats.shadow_queries;
+        samplesTraced += stats.samples;
+        completed += count;
+        std::cout << "Completed sample " << completed << " of " << spp << '\n';
+        if (completed % step == 0 && completed < spp) {
+            if (checkpoints.enqueue(renderer_, completed))
+                std::cout << "Queued checkpoint at " << completed << " of " << spp << " samples\n";
+            else
+                std::cout << "Skipped checkpoint at " << completed << " of " << spp
+                          << " samples because all checkpoint buffers are busy\n";
+        }
+    }
+    std::string checkpointError;
+    if (!checkpoints.finish(&checkpointError)) {
+        std::cerr << checkpointError << '\n';
+        return 1;
+    }
+    std::cout << "Trace time (" << (config_.tiledRendering ? "tiled" : "full frame") << "): " << traceMs << " ms\n";
+    double seconds = traceMs / 1000.0;
     if (seconds > 0)
-        std::cout << "Throughput: " << (stats.closest_queries + stats.shadow_queries) / seconds / 1e6 << " Mrays/s, "
-                  << stats.samples / seconds / 1e6 << " Msamples/s\n";
+        std::cout << "Throughput: " << queries / seconds / 1e6 << " Mrays/s, " << samplesTraced / seconds / 1e6
+                  << " Msamples/s\n";
     if (config_.profileQueues) {
         std::cout << "Queue occupancy (last launch):";
         for (uint32_t b = 0; b < stats.queue_count_depth; ++b) std::cout << ' ' << stats.queue_counts[b];

@@ -174,6 +174,16 @@ class Task(C.Structure):
                 ("width", C.c_uint32), ("height", C.c_uint32)]
 
 
+class LedgerJob(C.Structure):
+    _fields_ = [("job_fingerprint", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("tile_width", C.c_uint32), ("tile_height", C.c_uint32), ("samples_per_pixel", C.c_uint32),
+                ("samples_per_task", C.c_uint32)]
+
+
+LEDGER_ACCEPTED, LEDGER_DUPLICATE, LEDGER_REJECTED = 0, 1, 2
+TASK_PENDING, TASK_LEASED, TASK_COMPLETED = 0, 1, 2
+
+
 class Hit(C.Structure):
     _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("kind", C.c_uint32),
                 ("instance", C.c_uint32), ("primitive", C.c_uint32), ("material", C.c_uint32),
@@ -187,7 +197,8 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("kind", "<u4"),
 class Stats(C.Structure):
     _fields_ = [("closest_queries", C.c_uint64), ("shadow_queries", C.c_uint64), ("samples", C.c_uint64),
                 ("trace_milliseconds", C.c_double), ("queue_counts", C.c_uint32 * 64),
-                ("queue_count_depth", C.c_uint32)]
+                ("queue_count_depth", C.c_uint32), ("caustic_photons_stored", C.c_uint32),
+                ("caustic_build_milliseconds", C.c_double)]
 
 
 class SceneInfo(C.Structure):
@@ -290,6 +301,26 @@ def library() -> C.CDLL:
                                          C.POINTER(u64)], C.c_int),
         "wpt_host_create_tasks": ([u32, u32, u32, u32, u32, u32, u64, P, u32], u32),
         "wpt_host_task_key": ([u64, C.POINTER(Task)], u64),
+        "wpt_host_scene_fingerprint": ([C.c_char_p, C.POINTER(u64)], C.c_int),
+        "wpt_renderer_load_framebuffer": ([P, P, u32], C.c_int),
+        "wpt_host_render_checkpoint_write": ([C.c_char_p, P, u32, u32, u32, u64], C.c_int),
+        "wpt_host_render_checkpoint_read": ([C.c_char_p, u32, u32, u64, P, C.POINTER(u32)], C.c_int),
+        "wpt_ledger_create": ([C.POINTER(LedgerJob), P, u32, C.POINTER(P)], C.c_int),
+        "wpt_ledger_destroy": ([P], C.c_int),
+        "wpt_ledger_lease_next": ([P, C.c_char_p, u64, u64, C.POINTER(Task), C.POINTER(u64)], C.c_int),
+        "wpt_ledger_commit": ([P, C.POINTER(Task), u32, P, u64], C.c_int),
+        "wpt_ledger_renew_leases": ([P, C.c_char_p, u64, u64], u32),
+        "wpt_ledger_release_leases": ([P, C.c_char_p], u32),
+        "wpt_ledger_requeue_expired": ([P, u64], u32),
+        "wpt_ledger_task_state": ([P, C.POINTER(Task)], C.c_int),
+        "wpt_ledger_counts": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], C.c_int),
+        "wpt_ledger_read": ([P, P, P], C.c_int),
+        "wpt_ledger_write_checkpoint": ([P, C.c_char_p, u64], C.c_int),
+        "wpt_ledger_restore_checkpoint": ([P, C.c_char_p, C.POINTER(u64)], C.c_int),
+        "wpt_ledger_writer_create": ([C.c_char_p, C.POINTER(P)], C.c_int),
+        "wpt_ledger_writer_enqueue": ([P, P, u64], C.c_int),
+        "wpt_ledger_writer_flush": ([P], C.c_int),
+        "wpt_ledger_writer_destroy": ([P], C.c_int),
         "wpt_host_sha256": ([P, C.c_size_t, C.c_char_p], None),
         "wpt_host_main": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
     }
@@ -658,6 +689,12 @@ class Renderer:
               "wpt_renderer_read_framebuffer")
         return out, n.value
 
+    def load_framebuffer(self, rgb: np.ndarray, sample_count: int):
+        """Restore sums and the sample counter, to resume at sample_count."""
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        check(library().wpt_renderer_load_framebuffer(self._h, rgb.ctypes.data, sample_count),
+              "wpt_renderer_load_framebuffer")
+
     def render_frame(self):
         c = self.config
         out = np.zeros((c.height, c.width, 3), dtype=np.float32)
@@ -843,6 +880,119 @@ def create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint) -> L
     arr = (Task * max(1, n))()
     lib.wpt_host_create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint, C.cast(arr, C.c_void_p), n)
     return [arr[i] for i in range(n)]
+
+
+def write_render_checkpoint(filename: str, rgb: np.ndarray, sample_index: int, fingerprint: int) -> None:
+    """A "PTCHKPT1" render checkpoint of an (h, w, 3) float32 sum image."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w = rgb.shape[:2]
+    check(library().wpt_host_render_checkpoint_write(str(filename).encode(), rgb.ctypes.data, w, h, sample_index,
+                                                     fingerprint), "render checkpoint write")
+
+
+def read_render_checkpoint(filename: str, width: int, height: int, fingerprint: int):
+    """(sums (h, w, 3), completed samples); WptError on any mismatch."""
+    out = np.zeros((height, width, 3), np.float32)
+    n = C.c_uint32()
+    check(library().wpt_host_render_checkpoint_read(str(filename).encode(), width, height, fingerprint,
+                                                    out.ctypes.data, C.byref(n)), "render checkpoint read")
+    return out, n.value
+
+
+class Ledger:
+    """Task ledger of one job: leases, merged tile sums, checkpoints."""
+
+    def __init__(self, fingerprint, width, height, tile_w, tile_h, spp, per_task, tasks=None):
+        self.job = LedgerJob(fingerprint, width, height, tile_w, tile_h, spp, per_task)
+        if tasks is None:
+            tasks = create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint)
+        self.tasks = list(tasks)
+        arr = (Task * max(1, len(self.tasks)))(*self.tasks)
+        h = C.c_void_p()
+        check(library().wpt_ledger_create(C.byref(self.job), C.cast(arr, C.c_void_p), len(self.tasks), C.byref(h)),
+              "ledger create")
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            library().wpt_ledger_destroy(self.handle)
+            self.handle = None
+
+    def lease_next(self, worker: str, now_ms: int, lease_ms: int):
+        t, expires = Task(), C.c_uint64()
+        r = library().wpt_ledger_lease_next(self.handle, worker.encode(), now_ms, lease_ms, C.byref(t),
+                                            C.byref(expires))
+        if r < 0:
+            raise WptError(-r, "ledger lease")
+        return (t, expires.value) if r == 1 else None
+
+    def commit(self, task: Task, sample_count: int, radiance: np.ndarray) -> int:
+        rad = np.ascontiguousarray(radiance, np.float32)
+        return library().wpt_ledger_commit(self.handle, C.byref(task), sample_count, rad.ctypes.data, rad.size)
+
+    def renew(self, worker, now_ms, lease_ms):
+        return library().wpt_ledger_renew_leases(self.handle, worker.encode(), now_ms, lease_ms)
+
+    def release(self, worker):
+        return library().wpt_ledger_release_leases(self.handle, worker.encode())
+
+    def requeue_expired(self, now_ms):
+        return library().wpt_ledger_requeue_expired(self.handle, now_ms)
+
+    def state(self, task: Task) -> int:
+        return library().wpt_ledger_task_state(self.handle, C.byref(task))
+
+    def counts(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(library().wpt_ledger_counts(self.handle, C.byref(a), C.byref(b), C.byref(c)), "ledger counts")
+        return a.value, b.value, c.value
+
+    def image(self):
+        rad = np.zeros((self.job.height, self.job.width, 3), np.float32)
+        cnt = np.zeros((self.job.height, self.job.width), np.uint32)
+        check(library().wpt_ledger_read(self.handle, rad.ctypes.data, cnt.ctypes.data), "ledger read")
+        return rad, cnt
+
+    def write_checkpoint(self, filename, sequence):
+        check(library().wpt_ledger_write_checkpoint(self.handle, str(filename).encode(), sequence),
+              "ledger checkpoint write")
+
+    def restore_checkpoint(self, filename) -> int:
+        seq = C.c_uint64()
+        check(library().wpt_ledger_restore_checkpoint(self.handle, str(filename).encode(), C.byref(seq)),
+              "ledger checkpoint restore")
+        return seq.value
+
+
+class LedgerWriter:
+    """Background ledger checkpoint writer (newest waiting snapshot wins)."""
+
+    def __init__(self, filename):
+        h = C.c_void_p()
+        check(library().wpt_ledger_writer_create(str(filename).encode(), C.byref(h)), "ledger writer create")
+        self.handle = h
+
+    def enqueue(self, ledger: Ledger, sequence: int):
+        check(library().wpt_ledger_writer_enqueue(self.handle, ledger.handle, sequence), "ledger writer enqueue")
+
+    def flush(self):
+        check(library().wpt_ledger_writer_flush(self.handle), "ledger writer flush")
+
+    def close(self):
+        if self.handle:
+            h, self.handle = self.handle, None
+            check(library().wpt_ledger_writer_destroy(h), "ledger writer close")
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.close()
+
+
+def scene_fingerprint(preset_or_manifest: str) -> int:
+    """Dataset identity of a preset or manifest (no display name)."""
+    out = C.c_uint64()
+    check(library().wpt_host_scene_fingerprint(preset_or_manifest.encode(), C.byref(out)), "scene_fingerprint")
+    return out.value
 
 
 def task_key(fingerprint: int, task: Task) -> int:
