@@ -253,7 +253,17 @@ typedef struct wpt_scene_desc {
     const wpt_material_ext* material_ext;
     uint32_t use_normal_maps;
     float normal_map_minimum_cosine;
+    /* Builder of the mesh BLASes: WPT_BVH_BUILDER_HOST_SAH (0, binned SAH on
+     * the host) or WPT_BVH_BUILDER_DEVICE_PLOC (clustering on the GPU; the
+     * reference builds on the GPU with NexusBVH, scene.cu:123-239), with
+     * bvh_search_radius its neighbour radius (0: 16; at most 64).  The
+     * device builder needs a device scene (wpt_scene_create). */
+    uint32_t bvh_builder;
+    uint32_t bvh_search_radius;
 } wpt_scene_desc;
+
+#define WPT_BVH_BUILDER_HOST_SAH 0u
+#define WPT_BVH_BUILDER_DEVICE_PLOC 1u
 
 /* Reference Camera (include/scene/camera.cuh:7-27). */
 typedef struct wpt_camera {
@@ -365,6 +375,9 @@ typedef struct wpt_scene_info {
     uint32_t static_triangle_count, flat_triangle_count;
     uint32_t flat_node_count;
     uint32_t material_count;
+    uint32_t bvh_builder;           /* WPT_BVH_BUILDER_* of the mesh BLASes */
+    uint32_t max_blas_depth;        /* longest root-to-leaf path of any BLAS */
+    double blas_build_milliseconds; /* BLAS construction (host wall or device) */
 } wpt_scene_info;
 WPT_API int wpt_scene_info_get(const wpt_scene* scene, wpt_scene_info* info);
 /* which: 0 direct spheres, 1 direct instances, 2 TLAS leaves packed as

@@ -19,6 +19,7 @@
 
 #include "../../../include/wpt.h"
 #include "last_error.h"
+#include "../kernels/bvh_build.h"
 #include "../kernels/diff_kernels.h"
 #include "../kernels/wavefront.h"
 #include "../scene/scene_build.h"
@@ -93,8 +94,17 @@ Bvh deviceBvhView(const HostBvh& h, const BvhNode* nodes) {
 
 }  // namespace
 
+// A BLAS built on the device: where its nodes and slot order live.
+struct DeviceBlas {
+    const BvhNode* nodes = nullptr;
+    const uint32_t* order = nullptr;
+    uint32_t primCount = 0;
+    DeviceBvhBuild build{};
+};
+
 struct wpt_scene {
     HostSceneData host;
+    std::vector<DeviceBlas> deviceBlas;  // per mesh; primCount 0: host-built or none
     bool onDevice = false;
     DevScene dev{};
     DeviceArena arena;
@@ -267,6 +277,14 @@ static int createScene(const wpt_scene_desc* desc, wpt_scene** out, bool upload)
         delete s;
         return fail(WPT_ERROR_UNSUPPORTED, "acceleration structure deeper than the traversal stack");
     }
+    if (!upload) {
+        for (const HostMesh& m : s->host.meshes)
+            if (m.buildOnDevice) {
+                delete s;
+                return fail(WPT_ERROR_UNSUPPORTED, "the device BVH builder needs a device scene");
+            }
+    }
+    s->deviceBlas.resize(s->host.meshes.size());
     if (upload) {
         int status = WPT_OK;
         auto check = [&](hipError_t e, const char* what) {
@@ -292,16 +310,52 @@ static int createScene(const wpt_scene_desc* desc, wpt_scene** out, bool upload)
         check(a.upload(h.instances, &d.instances), "instances");
         d.instanceCount = static_cast<uint32_t>(h.instances.size());
         std::vector<Blas> blases(h.meshes.size());
+        double deviceBuildMs = 0.0;
         for (size_t m = 0; m < h.meshes.size(); ++m) {
             const HostMesh& mesh = h.meshes[m];
             Blas& b = blases[m];
+            check(a.upload(mesh.triangles, &b.triangles), "mesh triangles");
+            b.triangleCount = static_cast<uint32_t>(mesh.triangles.size());
+            if (mesh.buildOnDevice && status == WPT_OK) {
+                // Device construction straight into the scene's arena.
+                uint32_t n = b.triangleCount;
+                void *nodes = nullptr, *geometry = nullptr, *order = nullptr;
+                check(a.alloc(sizeof(BvhNode) * (n - 1), &nodes), "blas nodes");
+                check(a.alloc(sizeof(float4) * 3 * static_cast<size_t>(n), &geometry), "blas geometry");
+                check(a.alloc(sizeof(uint32_t) * n, &order), "blas order");
+                DeviceBlas& rec = s->deviceBlas[m];
+                if (status == WPT_OK)
+                    check(buildTriangleBvh(b.triangles, n, h.bvhSearchRadius, static_cast<BvhNode*>(nodes),
+                              static_cast<float4*>(geometry), static_cast<uint32_t*>(order), nullptr, &rec.build),
+                        "device BVH build");
+                rec.nodes = static_cast<const BvhNode*>(nodes);
+                rec.order = static_cast<const uint32_t*>(order);
+                rec.primCount = n;
+                b.bvh = Bvh{};
+                b.bvh.nodes = rec.nodes;
+                b.bvh.nodeCount = n - 1;
+                b.bvh.primCount = n;
+                b.bvh.root = n > 1 ? 0u : (kLeafBit | 0u);
+                for (int k = 0; k < 3; ++k) {
+                    b.bvh.rootMin[k] = rec.build.rootMin[k];
+                    b.bvh.rootMax[k] = rec.build.rootMax[k];
+                }
+                b.geometry = static_cast<const float4*>(geometry);
+                h.maxBlasDepth = std::max(h.maxBlasDepth, rec.build.depth);
+                deviceBuildMs += rec.build.milliseconds;
+                continue;
+            }
             const BvhNode* bn = nullptr;
             check(a.upload(mesh.bvh.nodes, &bn), "blas nodes");
             b.bvh = deviceBvhView(mesh.bvh, bn);
             check(a.upload(mesh.geometry, &f4), "blas geometry");
             b.geometry = f4;
-            check(a.upload(mesh.triangles, &b.triangles), "mesh triangles");
-            b.triangleCount = static_cast<uint32_t>(mesh.triangles.size());
+        }
+        if (h.bvhBuilder == WPT_BVH_BUILDER_DEVICE_PLOC) {
+            h.blasBuildMilliseconds = deviceBuildMs;
+            h.maxStackDepth = stackDepth(h);
+            if (status == WPT_OK && h.maxStackDepth > kMaxStackDepth)
+                status = fail(WPT_ERROR_UNSUPPORTED, "device-built BVH deeper than the traversal stack");
         }
         check(a.upload(blases, &d.blases), "blases");
         d.blasCount = static_cast<uint32_t>(blases.size());
@@ -379,8 +433,11 @@ WPT_API int wpt_scene_info_get(const wpt_scene* s, wpt_scene_info* info) {
     info->tlas_leaf_count = static_cast<uint32_t>(h.tlasLeavesOriginal.size());
     info->tlas_node_count = static_cast<uint32_t>(exportBvh2(h.tlas).size());
     uint32_t blases = 0;
-    for (const HostMesh& m : h.meshes) blases += m.bvh.primCount() != 0;
+    for (const HostMesh& m : h.meshes) blases += m.bvh.primCount() != 0 || m.buildOnDevice;
     info->blas_count = blases;
+    info->bvh_builder = h.bvhBuilder;
+    info->max_blas_depth = h.maxBlasDepth;
+    info->blas_build_milliseconds = h.blasBuildMilliseconds;
     info->light_count = static_cast<uint32_t>(h.lights.size());
     info->static_triangle_count = static_cast<uint32_t>(h.staticTriangles.size());
     info->flat_triangle_count = static_cast<uint32_t>(h.flatTriangles.size());
@@ -405,10 +462,31 @@ WPT_API int wpt_scene_export_bvh2(const wpt_scene* s, uint32_t bvh, wpt_bvh2_nod
     uint32_t* nodeCount) {
     if (!s || !nodeCount) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
     const HostBvh* h = nullptr;
+    HostBvh fetched;
     if (bvh == 0) h = &s->host.tlas;
     else if (bvh == 1) h = &s->host.flat;
     else if (bvh - 2 < s->host.meshes.size()) h = &s->host.meshes[bvh - 2].bvh;
     else return fail(WPT_ERROR_INVALID_ARGUMENT, "unknown BVH");
+    if (bvh >= 2 && s->deviceBlas[bvh - 2].primCount) {
+        // A device-built BLAS: nodes and slot order back from the device,
+        // leaf bounds recomputed from the triangles as the builder did.
+        const DeviceBlas& d = s->deviceBlas[bvh - 2];
+        const HostMesh& mesh = s->host.meshes[bvh - 2];
+        fetched.nodes.resize(d.primCount - 1);
+        fetched.order.resize(d.primCount);
+        if (d.primCount > 1)
+            WPT_HIP(hipMemcpy(fetched.nodes.data(), d.nodes, sizeof(BvhNode) * (d.primCount - 1),
+                hipMemcpyDeviceToHost));
+        WPT_HIP(hipMemcpy(fetched.order.data(), d.order, sizeof(uint32_t) * d.primCount, hipMemcpyDeviceToHost));
+        for (const TriShade& t : mesh.triangles) fetched.primBounds.push_back(triangleBounds(t.v0, t.v1, t.v2));
+        fetched.root = d.primCount > 1 ? 0u : (kLeafBit | 0u);
+        fetched.depth = d.build.depth;
+        for (int k = 0; k < 3; ++k) {
+            fetched.rootBounds.mn[k] = d.build.rootMin[k];
+            fetched.rootBounds.mx[k] = d.build.rootMax[k];
+        }
+        h = &fetched;
+    }
     std::vector<wpt_bvh2_node> nodes = exportBvh2(*h);
     *nodeCount = static_cast<uint32_t>(nodes.size());
     if (out) std::memcpy(out, nodes.data(), sizeof(wpt_bvh2_node) * std::min<size_t>(capacity, nodes.size()));

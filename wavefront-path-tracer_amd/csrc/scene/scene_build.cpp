@@ -14,6 +14,7 @@
 #include "environment.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <future>
@@ -351,6 +352,10 @@ Aabb triangleBounds(const Vec3& a, const Vec3& b, const Vec3& c) {
     return r;
 }
 
+uint32_t stackDepth(const HostSceneData& s) {
+    return std::max(s.tlas.depth + s.maxBlasDepth + 2, s.flat.depth + 1);
+}
+
 HostBvh buildBvh2(const std::vector<Aabb>& primBounds, unsigned threads) {
     HostBvh bvh;
     bvh.primBounds = primBounds;
@@ -663,10 +668,25 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
         if (inst.acceleration > WPT_ACCEL_DIRECT) return "instance has an unknown acceleration mode";
         if (inst.acceleration == WPT_ACCEL_BVH) needsBvh[inst.mesh] = true;
     }
+    if (desc.bvh_builder > WPT_BVH_BUILDER_DEVICE_PLOC) return "unknown BVH builder";
+    if (desc.bvh_search_radius > 64) return "BVH search radius above 64";
+    out.bvhBuilder = desc.bvh_builder;
+    out.bvhSearchRadius = desc.bvh_search_radius ? desc.bvh_search_radius : 16u;
     uint32_t maxBlasDepth = 0;
+    auto blasStart = std::chrono::steady_clock::now();
     for (uint32_t m = 0; m < desc.mesh_count; ++m) {
         HostMesh& mesh = out.meshes[m];
         if (!needsBvh[m] || mesh.triangles.empty()) continue;
+        if (desc.bvh_builder == WPT_BVH_BUILDER_DEVICE_PLOC) {
+            // Built, and counted in the depth, at upload.  The root box (the
+            // union of the triangles' boxes, as the device tree's root)
+            // places the instances in the TLAS.
+            mesh.buildOnDevice = true;
+            Aabb root = emptyBox();
+            for (const TriShade& t : mesh.triangles) grow(root, triangleBounds(t.v0, t.v1, t.v2));
+            mesh.bvh.rootBounds = root;
+            continue;
+        }
         std::vector<Aabb> bounds;
         bounds.reserve(mesh.triangles.size());
         for (const TriShade& t : mesh.triangles) bounds.push_back(triangleBounds(t.v0, t.v1, t.v2));
@@ -674,6 +694,9 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
         mesh.geometry = slotGeometry(mesh.triangles, mesh.bvh);
         maxBlasDepth = std::max(maxBlasDepth, mesh.bvh.depth);
     }
+    out.blasBuildMilliseconds =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - blasStart).count();
+    out.maxBlasDepth = maxBlasDepth;
 
     // Instances and cumulative light offsets (scene.cu:156-166).
     uint32_t lightOffset = 0;
@@ -766,7 +789,7 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
         out.flat = buildBvh2(bounds, threads);
         out.flatGeometry = slotGeometry(out.flatTriangles, out.flat);
     }
-    out.maxStackDepth = std::max(out.tlas.depth + maxBlasDepth + 2, out.flat.depth + 1);
+    out.maxStackDepth = stackDepth(out);
     return std::string();
 }
 

@@ -43,6 +43,7 @@ struct HostMesh {
     std::vector<TriShade> triangles;  // mesh-local, original order
     HostBvh bvh;                      // empty unless an instance uses AccelerationMode::Bvh
     std::vector<float4> geometry;     // slot order
+    bool buildOnDevice = false;       // BVH left to the device builder
 };
 
 // Everything uploadScene (reference src/scene/scene.cu:336-389) resolves on
@@ -91,9 +92,17 @@ struct HostSceneData {
         return !envTexels.empty() || !punctual.empty() || !materialExt.empty() || !textureTexels.empty();
     }
     uint32_t maxStackDepth = 0;  // worst-case traversal stack use
+    uint32_t maxBlasDepth = 0;   // of the BLASes built so far
+    uint32_t bvhBuilder = 0;     // WPT_BVH_BUILDER_*
+    uint32_t bvhSearchRadius = 16;
+    double blasBuildMilliseconds = 0.0;
 };
 
 // Returns an empty string on success, else the validation error.
 std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsigned threads = 0);
+
+// Worst-case traversal stack use of the TLAS over the deepest BLAS, or of
+// the flat BVH.
+uint32_t stackDepth(const HostSceneData& s);
 
 }  // namespace wpt

@@ -147,7 +147,8 @@ class SceneDesc(C.Structure):
                 ("punctual_lights", C.POINTER(PunctualLight)), ("punctual_light_count", C.c_uint32),
                 ("textures", C.POINTER(Texture)), ("texture_count", C.c_uint32),
                 ("material_ext", C.POINTER(MaterialExt)), ("use_normal_maps", C.c_uint32),
-                ("normal_map_minimum_cosine", C.c_float)]
+                ("normal_map_minimum_cosine", C.c_float), ("bvh_builder", C.c_uint32),
+                ("bvh_search_radius", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -181,6 +182,7 @@ class LedgerJob(C.Structure):
 
 
 LEDGER_ACCEPTED, LEDGER_DUPLICATE, LEDGER_REJECTED = 0, 1, 2
+BVH_HOST_SAH, BVH_DEVICE_PLOC = 0, 1
 TASK_PENDING, TASK_LEASED, TASK_COMPLETED = 0, 1, 2
 
 
@@ -205,7 +207,8 @@ class SceneInfo(C.Structure):
     _fields_ = [(name, C.c_uint32) for name in (
         "sphere_count", "direct_sphere_count", "instance_count", "direct_instance_count", "tlas_leaf_count",
         "tlas_node_count", "blas_count", "light_count", "static_triangle_count", "flat_triangle_count",
-        "flat_node_count", "material_count")]
+        "flat_node_count", "material_count", "bvh_builder", "max_blas_depth")] + [
+        ("blas_build_milliseconds", C.c_double)]
 
 
 class BatchTask(C.Structure):
