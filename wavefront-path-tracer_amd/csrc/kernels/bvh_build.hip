@@ -13,8 +13,10 @@
 //     2 exclusive scans: merge ranks (node numbers) and survivor positions
 //     kMerge         new interior nodes and the compacted cluster list
 //   kLeafDepth       tree depth from parent links
+//   kPreorder, kPermute  interior nodes renumbered depth first
 // Interior nodes are numbered downward from n-2 pass by pass, so the last
-// merge, the root, is node 0.
+// merge, the root, is node 0; the final depth-first renumbering keeps the
+// root at 0 and places every subtree's nodes in one contiguous range.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -225,7 +227,7 @@ __global__ void kTotal(const uint32_t* rank, const uint32_t* flag, uint32_t coun
 __global__ void kMerge(uint32_t count, const uint32_t* nn, const uint32_t* mergeFlag, const uint32_t* mergeRank,
     const uint32_t* keepFlag, const uint32_t* newPos, uint32_t nodeBase, const uint32_t* clusterId,
     const Box* clusterBox, uint32_t* outId, Box* outBox, BvhNode* nodes, uint32_t* leafParent,
-    uint32_t* nodeParent) {
+    uint32_t* nodeParent, uint32_t* leafCount) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count || !keepFlag[i]) return;
     uint32_t dst = newPos[i];
@@ -257,8 +259,37 @@ __global__ void kMerge(uint32_t count, const uint32_t* nn, const uint32_t* merge
     nodes[node] = r;
     if (a & kLeafBit) leafParent[a & ~kLeafBit] = node; else nodeParent[a] = node;
     if (b & kLeafBit) leafParent[b & ~kLeafBit] = node; else nodeParent[b] = node;
+    // Children were made in earlier passes, so their counts are final.
+    leafCount[node] = ((a & kLeafBit) ? 1u : leafCount[a]) + ((b & kLeafBit) ? 1u : leafCount[b]);
     outId[dst] = node;
     outBox[dst] = unite(A, B);
+}
+
+// Depth-first (preorder) number of each interior node: the root is 0, a
+// left child follows its parent, a right child follows the parent's whole
+// left subtree (leaves - 1 interior nodes).  Found by walking to the root.
+__global__ void kPreorder(const BvhNode* nodes, const uint32_t* nodeParent, const uint32_t* leafCount, uint32_t count,
+    uint32_t* preorder) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= count) return;
+    uint32_t index = 0, c = v, p = nodeParent[v];
+    for (uint32_t step = 0; p != kInvalid && step < count; ++step) {
+        uint32_t left = nodes[p].left;
+        index += c == left ? 1u : 1u + ((left & kLeafBit) ? 0u : leafCount[left] - 1u);
+        c = p;
+        p = nodeParent[p];
+    }
+    preorder[v] = index;
+}
+
+// Nodes moved to their preorder slots, interior child links renumbered.
+__global__ void kPermute(const BvhNode* nodes, const uint32_t* preorder, uint32_t count, BvhNode* out) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= count) return;
+    BvhNode r = nodes[v];
+    if (!(r.left & kLeafBit)) r.left = preorder[r.left];
+    if (!(r.right & kLeafBit)) r.right = preorder[r.right];
+    out[preorder[v]] = r;
 }
 
 // Edges from each leaf to the root; the block maximum goes to *depth.
@@ -329,7 +360,8 @@ hipError_t buildTriangleBvh(const TriShade* tris, uint32_t n, uint32_t radius, B
     Box *boxes, *partial, *bounds, *boxA, *boxB;
     uint64_t *keys, *sortedKeys;
     uint32_t *index, *sorted, *idA, *idB, *nn, *mergeFlag, *mergeRank, *keepFlag, *newPos, *leafParent, *nodeParent,
-        *scalars;
+        *leafCount, *preorder, *scalars;
+    BvhNode* staged;
     B_HIP(s.get(n, &boxes));
     B_HIP(s.get(blocks, &partial));
     B_HIP(s.get(1, &bounds));
@@ -348,6 +380,9 @@ hipError_t buildTriangleBvh(const TriShade* tris, uint32_t n, uint32_t radius, B
     B_HIP(s.get(n, &newPos));
     B_HIP(s.get(n, &leafParent));
     B_HIP(s.get(n, &nodeParent));
+    B_HIP(s.get(n, &leafCount));
+    B_HIP(s.get(n, &preorder));
+    B_HIP(s.get(n, &staged));
     B_HIP(s.get(2, &scalars));
 
     hipLaunchKernelGGL(kPrimBounds, dim3(blocks), dim3(kBlock), 0, stream, tris, n, boxes, partial);
@@ -386,7 +421,7 @@ hipError_t buildTriangleBvh(const TriShade* tris, uint32_t n, uint32_t radius, B
         B_HIP(hipStreamSynchronize(stream));
         if (merges == 0 || merges > remaining) return hipErrorUnknown;  // no mutual pair: cannot happen
         hipLaunchKernelGGL(kMerge, dim3(b), dim3(kBlock), 0, stream, count, nn, mergeFlag, mergeRank, keepFlag, newPos,
-            remaining - merges, idA, boxA, idB, boxB, nodes, leafParent, nodeParent);
+            remaining - merges, idA, boxA, idB, boxB, nodes, leafParent, nodeParent, leafCount);
         B_HIP(hipGetLastError());
         remaining -= merges;
         count -= merges;
@@ -397,6 +432,16 @@ hipError_t buildTriangleBvh(const TriShade* tris, uint32_t n, uint32_t radius, B
     B_HIP(hipMemsetAsync(scalars + 1, 0, sizeof(uint32_t), stream));
     hipLaunchKernelGGL(kLeafDepth, dim3(blocks), dim3(kBlock), 0, stream, leafParent, nodeParent, n, scalars + 1);
     B_HIP(hipGetLastError());
+    if (n > 2) {
+        // Clustering numbers nodes by pass; preorder puts each subtree's
+        // nodes together, as the host builder does, for cache locality.
+        uint32_t inner = n - 1;
+        hipLaunchKernelGGL(kPreorder, dim3(blocksFor(inner)), dim3(kBlock), 0, stream, nodes, nodeParent, leafCount,
+            inner, preorder);
+        hipLaunchKernelGGL(kPermute, dim3(blocksFor(inner)), dim3(kBlock), 0, stream, nodes, preorder, inner, staged);
+        B_HIP(hipGetLastError());
+        B_HIP(hipMemcpyAsync(nodes, staged, sizeof(BvhNode) * inner, hipMemcpyDeviceToDevice, stream));
+    }
     Box root;
     uint32_t depth = 0;
     B_HIP(hipMemcpyAsync(&root, boxA, sizeof root, hipMemcpyDeviceToHost, stream));
