@@ -255,6 +255,57 @@ def measure_diff(steps):
     return result
 
 
+def measure_terrain(spp, photons, frames):
+    """Shading-heavy workload with caustics: the terrain preset at full size
+    (2,006,680 triangles, 16 x 4096^2 RGBA8 textures, 512 emissive
+    triangles, a 2048 x 1024 HDR sky with a sun, dielectric crystals) with a
+    spectral photon map, 1920x1080, depth 10, regenerating scheduler; BLASes
+    built on the device."""
+    t0 = time.perf_counter()
+    host = wpt.HostScene("terrain")
+    t1 = time.perf_counter()
+    desc = wpt.SceneDesc.from_buffer_copy(host.desc())
+    desc.bvh_builder = wpt.BVH_DEVICE_PLOC
+    scene = wpt.Scene(desc, device=True, owner=host)
+    t2 = time.perf_counter()
+    info = scene.info()
+    log(f"terrain: generated in {t1 - t0:.2f} s, scene build + upload {t2 - t1:.2f} s "
+        f"(device BLAS build {info.blas_build_milliseconds:.1f} ms)")
+    status, rc, _, _ = wpt.parse_command_line(
+        ["--scene", "terrain", "--caustics", "--caustic-photons", str(photons), "--resolution",
+         f"{WIDTH}x{HEIGHT}", "--spp", str(spp), "--max-depth", "10", "--rr-depth", "3", "--rng", "pcg32",
+         "--regenerate"])
+    assert status == wpt.OK
+    r = wpt.Renderer(scene, host.camera(WIDTH, HEIGHT), rc)
+    photon = r.stats()
+    r.reset()
+    r.render_samples(0, spp)  # warmup
+    tp = time.perf_counter()
+    for _ in range(frames):
+        r.reset()
+        r.render_samples(0, spp)
+    wall = (time.perf_counter() - tp) / frames
+    st = r.stats()
+    img, n = r.framebuffer()
+    r.close()
+    out = {
+        "workload": f"terrain 1920x1080 {spp}spp depth10 rr3: 2,006,680 triangles (terrain + 4 rock meshes "
+                    f"+ foliage + crystals + 512 lamp triangles, 162 instances), 16 x 4096^2 RGBA8 textures, "
+                    f"2048x1024 HDR sky, {photons} spectral caustic photons",
+        "mrays_per_s": round((st.closest_queries + st.shadow_queries) / wall / 1e6, 2),
+        "msamples_per_s": round(WIDTH * HEIGHT * spp / wall / 1e6, 2),
+        "ms_per_frame": round(wall * 1000.0, 2),
+        "device_ms_per_frame": round(st.trace_milliseconds, 2),
+        "queries_per_path": round((st.closest_queries + st.shadow_queries) / (WIDTH * HEIGHT * spp), 3),
+        "photon_map_ms": round(photon.caustic_build_milliseconds, 2),
+        "photons_stored": int(photon.caustic_photons_stored),
+        "device_blas_build_ms": round(info.blas_build_milliseconds, 2),
+        "mean_radiance": round(float(img.mean() / max(1, n)), 5),
+    }
+    log(f"terrain: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +321,9 @@ def main():
     ap.add_argument("--skip-batch", action="store_true", help="do not measure batched vs one-by-one tasks")
     ap.add_argument("--batch-tasks", type=int, default=256, help="tasks of the batching workload")
     ap.add_argument("--skip-diff", action="store_true", help="do not measure the differentiable renderer")
+    ap.add_argument("--skip-terrain", action="store_true", help="do not measure the shading-heavy caustic workload")
+    ap.add_argument("--terrain-spp", type=int, default=64)
+    ap.add_argument("--terrain-photons", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--diff-steps", type=int, default=200, help="Adam steps of the inverse-rendering workload")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB three times: the reference kernel of "
@@ -399,6 +453,10 @@ def main():
     if not args.skip_diff:
         diff = measure_diff(args.diff_steps)
 
+    terrain = None
+    if not args.skip_terrain:
+        terrain = measure_terrain(args.terrain_spp, args.terrain_photons, 1)
+
     roofline = None
     if work is not None and trace_ms > 0:
         def traversal_bytes(nodes, tris, spheres, instances, count):
@@ -454,6 +512,7 @@ def main():
         "other_schedulers": others,
         "batch": batch,
         "diff": diff,
+        "shading_heavy": terrain,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -215,6 +215,33 @@ def test_pathtracer_binary_help_and_list():
     assert "Unknown option" in out.stderr
 
 
+def test_terrain_preset_shape(tmp_path):
+    # The shading-heavy workload at reduced size: terrain 2 r^2 triangles,
+    # 4 rocks of 20 (f/2)^2, a 96-triangle bush, a 24-triangle crystal and
+    # 256 lamp quads; 160 props of which 48 (30 %) are alpha-masked bushes.
+    p = write_manifest(tmp_path, {"preset": "terrain", "procedural": {
+        "seed": 3, "icosphere_frequency": 8, "terrain_resolution": 24, "texture_size": 32}})
+    host = wpt.HostScene(p)
+    d = host.desc()
+    assert d.mesh_triangle_count == 2 * 24 * 24 + 4 * 20 * 4 * 4 + 96 + 24 + 512
+    assert (d.mesh_count, d.instance_count, d.texture_count) == (8, 1 + 84 + 48 + 28 + 1, 16)
+    assert d.material_count == 9 + 256 and d.material_ext
+    assert (d.textures[0].width, d.environment_width, d.environment_height) == (32, 256, 128)
+    masked = [i for i in range(d.instance_count)
+              if d.instances[i].material_override != wpt.INVALID
+              and d.materials[d.instances[i].material_override].alpha_masked]
+    assert len(masked) == 48
+    emissive = [m for m in range(d.material_count) if d.materials[m].type == wpt.EMISSIVE]
+    assert len(emissive) == 256
+    radiance = np.array([d.materials[m].emission[0] / 6.0 for m in emissive])
+    assert 0.5 < np.median(radiance) < 2.0 and radiance.max() > 5 * radiance.min()  # log-normal spread
+    scene = wpt.Scene(d, device=False, owner=host)
+    assert scene.info().light_count == 512
+    full = write_manifest(tmp_path, {"preset": "terrain", "procedural": {"texture_size": 9000}}, "bad.json")
+    with pytest.raises(wpt.WptError):
+        wpt.HostScene(full)
+
+
 def test_procedural_scene_shapes():
     h0 = wpt.HostScene("cornell-box")
     c0 = h0.desc()

@@ -67,8 +67,7 @@ WPT_HD Photon tracePhoton(const DevScene& scene, const PhotonSettings& ps, uint3
             next = normalize(reflect(normalize(ray.direction), normal));
         } else if (m.type == kDielectric) {
             float ior = m.ior;
-            float dispersion = m.ext != kInvalid ? scene.materialExt[m.ext].dispersion : 0.0f;
-            if (dispersion > 0.0f && channel < 3) ior += (static_cast<float>(channel) - 1.0f) * dispersion;
+            if (m.dispersion > 0.0f && channel < 3) ior += (static_cast<float>(channel) - 1.0f) * m.dispersion;
             DielectricInterface b = dielectricInterface(normalize(ray.direction), hit.geometricNormal, hit.normal, ior);
             bool reflected = b.reflectance >= 1.0f || b.reflectance > rngNextFloat(rng, ps.strategy);
             next = reflected ? b.reflection : b.transmission;

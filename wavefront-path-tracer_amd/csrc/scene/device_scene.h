@@ -124,7 +124,8 @@ struct Material {
     float volumeDensity;
     Vec3 specular;
     float glossiness;
-    uint32_t ext;  // index into DevScene.materialExt, kInvalid: untextured
+    uint32_t ext;      // index into DevScene.materialExt, kInvalid: untextured
+    float dispersion;  // IOR step per spectral channel of caustic photons
 };
 
 // Reference TextureBinding (material.cuh:14-21).

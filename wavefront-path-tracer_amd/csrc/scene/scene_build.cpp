@@ -603,6 +603,12 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
         const wpt_material& m = desc.materials[i];
         if (m.type > WPT_MATERIAL_SPECULAR_GLOSSINESS) return "material has an unknown type";
         Material r = toMaterial(m);
+        // Dispersion (read by photon tracing only) lives on the material
+        // itself, so that an otherwise plain dielectric keeps the lean path.
+        if (desc.material_ext) {
+            r.dispersion = desc.material_ext[i].dispersion;
+            if (!(r.dispersion >= 0.0f) || !std::isfinite(r.dispersion)) return "material dispersion is invalid";
+        }
         bool albedoTexture = false;
         if (desc.material_ext && needsExt(desc.material_ext[i])) {
             const wpt_material_ext& e = desc.material_ext[i];

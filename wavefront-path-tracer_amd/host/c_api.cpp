@@ -301,6 +301,7 @@ WPT_API int wpt_host_main(int argc, char** argv) {
     }
     wpt_camera camera = wpt::createSceneCamera(sceneConfig, config.width, config.height);
     wpt_scene_desc desc = host.desc();
+    desc.bvh_builder = sceneConfig.bvhBuilder;
     wpt_scene* scene = nullptr;
     if (wpt_scene_create(&desc, &scene) != WPT_OK) {
         std::cerr << wpt_last_error() << '\n';

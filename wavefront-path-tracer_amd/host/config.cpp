@@ -32,6 +32,7 @@ void printHelp() {
               << "  --no-caustic-gather      Trace photons but omit camera gathering\n"
               << "  --persistent-wavefront  Use the persistent cooperative wavefront kernel\n"
               << "  --regenerate             Use the persistent kernel with path regeneration (no bounce barriers)\n"
+              << "  --bvh-builder host|device  Build mesh BLASes with host binned SAH (default) or on the GPU\n"
               << "  --profile-queues        Print queue occupancy by bounce\n"
               << "  --resume                 Resume from the selected checkpoint\n"
               << "  --coordinator            Run as a distributed coordinator (local task leases)\n"
@@ -156,6 +157,7 @@ SceneConfig presetDefaults(ScenePreset preset) {
     case ScenePreset::CornellBox:
     case ScenePreset::SphereField:
     case ScenePreset::DiffCornell:
+    case ScenePreset::Terrain:
         c.objectFilename.clear();
         break;
     default:
@@ -179,6 +181,7 @@ const char* presetName(ScenePreset preset) {
     case ScenePreset::CornellBox: return "cornell-box";
     case ScenePreset::SphereField: return "sphere-field";
     case ScenePreset::DiffCornell: return "diff-cornell";
+    case ScenePreset::Terrain: return "terrain";
     }
     return "unknown";
 }
@@ -186,7 +189,7 @@ const char* presetName(ScenePreset preset) {
 bool parseScenePreset(const char* v, ScenePreset& p) {
     static const ScenePreset all[] = {ScenePreset::Sponza, ScenePreset::Cornell, ScenePreset::Hurricane,
         ScenePreset::Prism, ScenePreset::Crystal, ScenePreset::Deer, ScenePreset::Demo, ScenePreset::RtWeek,
-        ScenePreset::CornellBox, ScenePreset::SphereField, ScenePreset::DiffCornell};
+        ScenePreset::CornellBox, ScenePreset::SphereField, ScenePreset::DiffCornell, ScenePreset::Terrain};
     for (ScenePreset candidate : all) {
         if (!std::strcmp(v, presetName(candidate))) {
             p = candidate;
@@ -305,6 +308,12 @@ bool loadSceneManifest(const char* filename, SceneConfig& scene, std::string* er
             loaded.seed = static_cast<uint32_t>(p.value("seed", static_cast<double>(loaded.seed)));
             loaded.icosphereFrequency =
                 static_cast<uint32_t>(p.value("icosphere_frequency", static_cast<double>(loaded.icosphereFrequency)));
+            loaded.terrainResolution =
+                static_cast<uint32_t>(p.value("terrain_resolution", static_cast<double>(loaded.terrainResolution)));
+            loaded.textureSize = static_cast<uint32_t>(p.value("texture_size", static_cast<double>(loaded.textureSize)));
+            if (loaded.icosphereFrequency < 2 || loaded.icosphereFrequency > 1024 || loaded.terrainResolution < 2 ||
+                loaded.terrainResolution > 4096 || loaded.textureSize < 4 || loaded.textureSize > 8192)
+                return setError("scene manifest procedural sizes are out of range");
         }
         scene = std::move(loaded);
         return true;
@@ -333,7 +342,7 @@ bool parseCommandLine(int argc, char** argv, RenderConfig& render, SceneConfig& 
             continue;
         }
         if (!std::strcmp(a, "--list-scenes")) {
-            std::cout << "Scene presets: sponza, cornell, hurricane, prism, crystal, deer, rtweek, demo, cornell-box, sphere-field, diff-cornell\n";
+            std::cout << "Scene presets: sponza, cornell, hurricane, prism, crystal, deer, rtweek, demo, cornell-box, sphere-field, diff-cornell, terrain\n";
             return false;
         }
         if (!std::strcmp(a, "--help") || !std::strcmp(a, "-h")) {
@@ -371,6 +380,16 @@ bool parseCommandLine(int argc, char** argv, RenderConfig& render, SceneConfig& 
         if (!std::strcmp(a, "--no-caustic-gather")) { render.enableCausticGather = false; continue; }
         if (!std::strcmp(a, "--persistent-wavefront")) { render.persistentWavefront = true; continue; }
         if (!std::strcmp(a, "--regenerate")) { render.regenerate = true; continue; }
+        if (!std::strcmp(a, "--bvh-builder") && i + 1 < argc) {
+            const char* v = next();
+            if (!std::strcmp(v, "host")) scene.bvhBuilder = WPT_BVH_BUILDER_HOST_SAH;
+            else if (!std::strcmp(v, "device")) scene.bvhBuilder = WPT_BVH_BUILDER_DEVICE_PLOC;
+            else {
+                std::cerr << "Unknown BVH builder: " << v << " (expected host or device)\n";
+                return false;
+            }
+            continue;
+        }
         if (!std::strcmp(a, "--profile-queues")) { render.profileQueues = true; continue; }
         if (!std::strcmp(a, "--resume")) { render.resumeFromCheckpoint = true; continue; }
         if (!std::strcmp(a, "--intersection-debug")) { render.intersectionDebug = true; continue; }
@@ -472,18 +491,21 @@ bool parseCommandLine(int argc, char** argv, RenderConfig& render, SceneConfig& 
                 std::cerr << "Unknown scene preset: " << v << '\n';
                 return false;
             }
-            uint32_t seed = scene.seed;
+            uint32_t seed = scene.seed, builder = scene.bvhBuilder;
             scene = scenePreset(preset);
             scene.seed = seed;
+            scene.bvhBuilder = builder;
             continue;
         }
         if (!std::strcmp(a, "--scene-file") && i + 1 < argc) {
             const char* f = next();
             std::string error;
+            uint32_t builder = scene.bvhBuilder;
             if (!loadSceneManifest(f, scene, &error)) {
                 std::cerr << "Could not load scene manifest '" << f << "': " << error << '\n';
                 return false;
             }
+            scene.bvhBuilder = builder;
             continue;
         }
         std::cerr << "Unknown option: " << a << "\n";
@@ -567,13 +589,14 @@ wpt_render_config toRenderConfig(const RenderConfig& c, ScenePreset preset) {
     r.shading_normal_debug = c.shadingNormalDebug ? 1u : 0u;
     r.profile_queues = c.profileQueues ? 1u : 0u;
     bool causticPreset = preset == ScenePreset::Cornell || preset == ScenePreset::Hurricane ||
-        preset == ScenePreset::Prism || preset == ScenePreset::Crystal;
+        preset == ScenePreset::Prism || preset == ScenePreset::Crystal || preset == ScenePreset::Terrain;
     if (c.enableCaustics && causticPreset) {
         r.caustic_photons = c.causticPhotonCount;
         r.caustic_max_depth = c.causticMaxDepth;
         r.caustic_gather_radius = c.causticGatherRadius;
         r.caustic_gather = c.enableCausticGather ? 1u : 0u;
-        r.caustic_spectral = preset == ScenePreset::Prism || preset == ScenePreset::Crystal ? 1u : 0u;
+        r.caustic_spectral =
+            preset == ScenePreset::Prism || preset == ScenePreset::Crystal || preset == ScenePreset::Terrain ? 1u : 0u;
     }
     return r;
 }
@@ -663,9 +686,13 @@ void appendScene(Fnv& f, const SceneConfig& scene, bool includeName) {
         f.f32(scene.environmentRotationDegrees);
     }
     if (scene.preset == ScenePreset::CornellBox || scene.preset == ScenePreset::SphereField ||
-        scene.preset == ScenePreset::DiffCornell) {
+        scene.preset == ScenePreset::DiffCornell || scene.preset == ScenePreset::Terrain) {
         f.u32(scene.seed);
         f.u32(scene.icosphereFrequency);
+    }
+    if (scene.preset == ScenePreset::Terrain) {
+        f.u32(scene.terrainResolution);
+        f.u32(scene.textureSize);
     }
 }
 

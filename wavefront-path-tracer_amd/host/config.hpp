@@ -22,6 +22,7 @@ enum class ScenePreset {
     CornellBox,   // unit-scale Cornell box, 2 spheres, emissive quad
     SphereField,  // 484-sphere field + 3 instances of a displaced icosphere
     DiffCornell,  // Cornell box tessellated for the differentiable renderer
+    Terrain,      // textured terrain, instanced props, 512 lamps, HDR sky, crystals with caustics
 };
 
 enum class DistributedRole { Local, Coordinator, Worker };
@@ -81,6 +82,11 @@ struct SceneConfig {
     // Procedural generators.
     uint32_t seed = 1;
     uint32_t icosphereFrequency = 224;  // 20 * f^2 triangles
+    uint32_t bvhBuilder = WPT_BVH_BUILDER_HOST_SAH;  // mesh BLAS builder (--bvh-builder)
+    // Terrain preset: grid cells per side (2 r^2 triangles; rocks use
+    // icosphereFrequency / 2) and the edge of its 16 square RGBA8 textures.
+    uint32_t terrainResolution = 708;
+    uint32_t textureSize = 4096;
 };
 
 const char* presetName(ScenePreset preset);

@@ -339,7 +339,35 @@ wpt_scene_desc HostScene::desc() const {
     d.material_count = static_cast<uint32_t>(materials.size());
     d.black_background = blackBackground ? 1u : 0u;
     d.ignore_direct_light_occlusion = ignoreDirectLightOcclusion ? 1u : 0u;
+    d.textures = textures.empty() ? nullptr : textures.data();
+    d.texture_count = static_cast<uint32_t>(textures.size());
+    d.material_ext = materialExt.empty() ? nullptr : materialExt.data();
+    if (!environment.empty()) {
+        d.environment_rgb = environment.data();
+        d.environment_width = environmentWidth;
+        d.environment_height = environmentHeight;
+        d.environment_intensity = environmentIntensity;
+        d.environment_rotation_degrees = environmentRotationDegrees;
+    }
+    d.use_normal_maps = useNormalMaps ? 1u : 0u;
+    d.normal_map_minimum_cosine = normalMapMinimumCosine;
     return d;
+}
+
+wpt_material_ext plainMaterialExt() {
+    wpt_material_ext e{};
+    wpt_texture_binding none{};
+    none.texture = WPT_INVALID_INDEX;
+    none.scale[0] = none.scale[1] = 1.0f;
+    e.albedo_texture = e.metallic_roughness_texture = e.emissive_texture = e.normal_texture = none;
+    e.thickness_texture = e.specular_glossiness_texture = e.specular_weight_texture = none;
+    e.specular_color_texture = e.clearcoat_texture = e.clearcoat_roughness_texture = none;
+    e.clearcoat_normal_texture = none;
+    e.normal_scale = 1.0f;
+    e.specular_weight = 1.0f;
+    e.specular_color[0] = e.specular_color[1] = e.specular_color[2] = 1.0f;
+    e.clearcoat_normal_scale = 1.0f;
+    return e;
 }
 
 wpt_material makeMaterial(uint32_t type, float r, float g, float b, float er, float eg, float eb) {
@@ -476,6 +504,11 @@ bool createSceneFromConfig(const SceneConfig& config, HostScene& out, std::strin
     case ScenePreset::DiffCornell:
         diffCornellScene(out);
         return true;
+    case ScenePreset::Terrain:
+        terrainScene(out, config);
+        out.useNormalMaps = config.useNormalMaps;
+        out.normalMapMinimumCosine = config.normalMapMinimumCosine;
+        return true;
     case ScenePreset::SphereField:
         sphereFieldScene(out, config, true);
         return true;
@@ -500,6 +533,9 @@ bool createSceneFromConfig(const SceneConfig& config, HostScene& out, std::strin
         out.instances.push_back(inst);
         referenceCornellRoom(out, config, config.convertObjectMaterialsToDielectric &&
             config.lightProfile == CornellLightProfile::Crystal, config.lightProfile == CornellLightProfile::Prism);
+        // The prism splits photons by channel (scene_host.cpp:251-255).
+        out.materialExt.assign(out.materials.size(), plainMaterialExt());
+        out.materialExt[5].dispersion = 0.035f;
         return true;
     }
     default:
@@ -551,6 +587,10 @@ wpt_camera createSceneCamera(const SceneConfig& config, uint32_t w, uint32_t h) 
     case ScenePreset::DiffCornell: {
         const float p[3] = {0.0f, 0.0f, 3.4f}, t[3] = {0.0f, 0.0f, 0.0f};
         return lookAt(p, t, 40.0f, w, h, true);
+    }
+    case ScenePreset::Terrain: {
+        const float p[3] = {-5.5f, 2.6f, 6.5f}, t[3] = {0.0f, 0.4f, 0.0f};
+        return lookAt(p, t, 50.0f, w, h, true);
     }
     default: {
         const float p[3] = {-10.0f, 0.35f, 0.8f}, t[3] = {6.0f, 3.2f, 2.4f};
