@@ -770,18 +770,13 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
         R_HIP(photonSortBytes(c.caustic_photons, cells, &pb.tempBytes));
         R_HIP(r->arena.alloc(pb.tempBytes > 0 ? pb.tempBytes : 1, &p));
         pb.temp = p;
-        R_HIP(r->arena.alloc(sizeof(uint32_t) * cells, &p));
-        pb.cellStart = static_cast<uint32_t*>(p);
-        R_HIP(r->arena.alloc(sizeof(uint32_t) * cells, &p));
-        pb.cellEnd = static_cast<uint32_t*>(p);
-        R_HIP(r->arena.alloc(sizeof(uint32_t), &p));
+        R_HIP(r->arena.alloc(sizeof(CellRange) * cells, &p));
+        pb.cells = static_cast<CellRange*>(p);
+        R_HIP(r->arena.alloc(sizeof(uint32_t) * 7, &p));
         pb.stored = static_cast<uint32_t*>(p);
+        pb.occupied = pb.stored + 1;
         view.photons = pb.sorted;
-        view.cellStart = pb.cellStart;
-        view.cellEnd = pb.cellEnd;
-        R_HIP(r->arena.alloc(sizeof(PhotonGridView), &p));
-        R_HIP(hipMemcpy(p, &view, sizeof(PhotonGridView), hipMemcpyHostToDevice));
-        if (c.caustic_gather) ctx.shade.photons = static_cast<const PhotonGridView*>(p);
+        view.cells = pb.cells;
         R_HIP(hipEventRecord(r->start, r->stream));
         R_HIP(enqueuePhotonMap(ctx.scene, ps, pb, view, ctx.spill, ctx.spillThreads, r->stream));
         R_HIP(hipEventRecord(r->end, r->stream));
@@ -789,7 +784,17 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
         float ms = 0.0f;
         R_HIP(hipEventElapsedTime(&ms, r->start, r->end));
         r->photonMs = ms;
-        R_HIP(hipMemcpy(&r->photonsStored, pb.stored, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        uint32_t counters[7];
+        R_HIP(hipMemcpy(counters, pb.stored, sizeof counters, hipMemcpyDeviceToHost));
+        r->photonsStored = counters[0];
+        for (int a = 0; a < 3; ++a) {
+            view.occupiedMin[a] = counters[1 + a];
+            view.occupiedMax[a] = counters[4 + a];
+        }
+        // The gather's view, with the occupied cell box, once the map exists.
+        R_HIP(r->arena.alloc(sizeof(PhotonGridView), &p));
+        R_HIP(hipMemcpy(p, &view, sizeof(PhotonGridView), hipMemcpyHostToDevice));
+        if (c.caustic_gather) ctx.shade.photons = static_cast<const PhotonGridView*>(p);
     }
     R_HIP(hipDeviceSynchronize());
 #undef R_HIP

@@ -28,16 +28,23 @@ struct Photon {
     uint32_t pad;
 };
 
+// The photons of one cell: sorted[start .. end).
+struct alignas(8) CellRange {
+    uint32_t start, end;
+};
+
 // Grid over [minimum, maximum] with resolution^3 cells; photons sorted by
-// cell, cell c holding photons[cellStart[c] .. cellEnd[c]).
+// cell.  occupiedMin/Max bound the cell coordinates of every non-empty cell
+// (min > max: no photon stored); cells outside that box are empty.
 struct PhotonGridView {
     const Photon* photons;
-    const uint32_t* cellStart;
-    const uint32_t* cellEnd;
+    const CellRange* cells;
     uint32_t resolution;
     float radius;  // gather radius floor
     Vec3 minimum;
     Vec3 maximum;
+    uint32_t occupiedMin[3];
+    uint32_t occupiedMax[3];
 };
 
 // Cell of a position, clamped per axis; `inside` is false outside the box
@@ -63,6 +70,7 @@ WPT_HD void photonCell(const PhotonGridView& g, const Vec3& p, int& x, int& y, i
 WPT_HD Vec3 gatherCaustics(const PhotonGridView& g, const Vec3& position, const Vec3& normal) {
     constexpr uint32_t kNearest = 64;
     constexpr uint32_t kMinimum = 16;
+    if (g.occupiedMin[0] > g.occupiedMax[0]) return Vec3(0.0f, 0.0f, 0.0f);
     int cx, cy, cz;
     bool inside;
     photonCell(g, position, cx, cy, cz, inside);
@@ -75,14 +83,22 @@ WPT_HD Vec3 gatherCaustics(const PhotonGridView& g, const Vec3& position, const 
     uint32_t indices[kNearest];
     uint32_t count = 0, farthest = 0;
     float farthestSquared = 0.0f;
-    int res = static_cast<int>(g.resolution);
-    for (int z = cz - cellRadius; z <= cz + cellRadius; ++z)
-        for (int y = cy - cellRadius; y <= cy + cellRadius; ++y)
-            for (int x = cx - cellRadius; x <= cx + cellRadius; ++x) {
-                if (x < 0 || y < 0 || z < 0 || x >= res || y >= res || z >= res) continue;
+    // The search cube clipped to the occupied cells: the cells left out are
+    // empty, so the photons visited, and their order, are unchanged.
+    int lo[3], hi[3];
+    const int c[3] = {cx, cy, cz};
+    for (int a = 0; a < 3; ++a) {
+        int omin = static_cast<int>(g.occupiedMin[a]), omax = static_cast<int>(g.occupiedMax[a]);
+        lo[a] = c[a] - cellRadius > omin ? c[a] - cellRadius : omin;
+        hi[a] = c[a] + cellRadius < omax ? c[a] + cellRadius : omax;
+    }
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x) {
                 uint32_t cell = (static_cast<uint32_t>(z) * g.resolution + static_cast<uint32_t>(y)) * g.resolution +
                     static_cast<uint32_t>(x);
-                for (uint32_t i = g.cellStart[cell], end = g.cellEnd[cell]; i < end; ++i) {
+                const CellRange range = g.cells[cell];
+                for (uint32_t i = range.start; i < range.end; ++i) {
                     const Photon& photon = g.photons[i];
                     Vec3 d = photon.position - position;
                     float distanceSquared = dot(d, d);

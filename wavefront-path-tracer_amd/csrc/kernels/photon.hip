@@ -27,25 +27,54 @@ __global__ void __launch_bounds__(256) kPhotonTrace(DevScene scene, PhotonSettin
 }
 
 // Sort keys: (cell << 32) | photon, with photons outside the grid or not
-// stored keyed past the last cell.
-__global__ void kPhotonKeys(PhotonGridView g, PhotonSettings ps, const Photon* photons, uint64_t* keys,
-    uint32_t* stored) {
+// stored keyed past the last cell; the stored count and the bounds of the
+// occupied cells, reduced per block before one atomic each.
+__global__ void __launch_bounds__(256) kPhotonKeys(PhotonGridView g, PhotonSettings ps, const Photon* photons,
+    uint64_t* keys, uint32_t* stored, uint32_t* occupied) {
+    __shared__ uint32_t box[6][256];
+    __shared__ uint32_t count[256];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ps.count) return;
-    const Photon& p = photons[i];
-    uint32_t cells = g.resolution * g.resolution * g.resolution;
-    uint32_t cell = cells;
-    if (p.valid) {
-        int x, y, z;
-        bool inside;
-        photonCell(g, p.position, x, y, z, inside);
-        if (inside) {
-            cell = (static_cast<uint32_t>(z) * g.resolution + static_cast<uint32_t>(y)) * g.resolution +
-                static_cast<uint32_t>(x);
-            atomicAdd(stored, 1u);
+    uint32_t b[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    uint32_t kept = 0;
+    if (i < ps.count) {
+        const Photon& p = photons[i];
+        uint32_t cells = g.resolution * g.resolution * g.resolution;
+        uint32_t cell = cells;
+        if (p.valid) {
+            int x, y, z;
+            bool inside;
+            photonCell(g, p.position, x, y, z, inside);
+            if (inside) {
+                cell = (static_cast<uint32_t>(z) * g.resolution + static_cast<uint32_t>(y)) * g.resolution +
+                    static_cast<uint32_t>(x);
+                kept = 1;
+                b[0] = b[3] = static_cast<uint32_t>(x);
+                b[1] = b[4] = static_cast<uint32_t>(y);
+                b[2] = b[5] = static_cast<uint32_t>(z);
+            }
+        }
+        keys[i] = (static_cast<uint64_t>(cell) << 32) | i;
+    }
+    for (int k = 0; k < 6; ++k) box[k][threadIdx.x] = b[k];
+    count[threadIdx.x] = kept;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            for (int k = 0; k < 3; ++k) {
+                box[k][threadIdx.x] = min(box[k][threadIdx.x], box[k][threadIdx.x + s]);
+                box[k + 3][threadIdx.x] = max(box[k + 3][threadIdx.x], box[k + 3][threadIdx.x + s]);
+            }
+            count[threadIdx.x] += count[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && count[0] != 0) {
+        atomicAdd(stored, count[0]);
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&occupied[k], box[k][0]);
+            atomicMax(&occupied[k + 3], box[k + 3][0]);
         }
     }
-    keys[i] = (static_cast<uint64_t>(cell) << 32) | i;
 }
 
 __global__ void kFillWords(uint32_t* p, uint32_t n, uint32_t value) {
@@ -54,15 +83,15 @@ __global__ void kFillWords(uint32_t* p, uint32_t n, uint32_t value) {
 
 // Cell ranges of the sorted keys, and the photons in sorted order.
 __global__ void kPhotonRanges(PhotonSettings ps, uint32_t cells, const uint64_t* sortedKeys, const Photon* photons,
-    Photon* sorted, uint32_t* cellStart, uint32_t* cellEnd) {
+    Photon* sorted, CellRange* ranges) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= ps.count) return;
     uint64_t key = sortedKeys[p];
     uint32_t cell = static_cast<uint32_t>(key >> 32);
     sorted[p] = photons[static_cast<uint32_t>(key)];
     if (cell >= cells) return;
-    if (p == 0 || static_cast<uint32_t>(sortedKeys[p - 1] >> 32) != cell) cellStart[cell] = p;
-    if (p + 1 == ps.count || static_cast<uint32_t>(sortedKeys[p + 1] >> 32) != cell) cellEnd[cell] = p + 1;
+    if (p == 0 || static_cast<uint32_t>(sortedKeys[p - 1] >> 32) != cell) ranges[cell].start = p;
+    if (p + 1 == ps.count || static_cast<uint32_t>(sortedKeys[p + 1] >> 32) != cell) ranges[cell].end = p + 1;
 }
 
 uint32_t keyBits(uint32_t cells) {
@@ -88,16 +117,19 @@ hipError_t enqueuePhotonMap(const DevScene& scene, const PhotonSettings& ps, con
     hipLaunchKernelGGL(kPhotonTrace, dim3(traceBlocks), dim3(256), stackLdsBytes(256), stream, scene, ps, b.photons,
         spill, spillThreads);
     hipLaunchKernelGGL(kFillWords, dim3(1), dim3(64), 0, stream, b.stored, 1u, 0u);
-    hipLaunchKernelGGL(kPhotonKeys, dim3(blocks), dim3(256), 0, stream, view, ps, b.photons, b.keys, b.stored);
+    hipLaunchKernelGGL(kFillWords, dim3(1), dim3(64), 0, stream, b.occupied, 3u, 0xffffffffu);
+    hipLaunchKernelGGL(kFillWords, dim3(1), dim3(64), 0, stream, b.occupied + 3, 3u, 0u);
+    hipLaunchKernelGGL(kPhotonKeys, dim3(blocks), dim3(256), 0, stream, view, ps, b.photons, b.keys, b.stored,
+        b.occupied);
     size_t bytes = b.tempBytes;
     hipError_t e = hipcub::DeviceRadixSort::SortKeys(b.temp, bytes, b.keys, b.sortedKeys, static_cast<int>(ps.count),
         0, static_cast<int>(keyBits(cells)), stream);
     if (e != hipSuccess) return e;
-    uint32_t fillBlocks = (cells + 255) / 256 < 4096 ? (cells + 255) / 256 : 4096;
-    hipLaunchKernelGGL(kFillWords, dim3(fillBlocks), dim3(256), 0, stream, b.cellStart, cells, 0u);
-    hipLaunchKernelGGL(kFillWords, dim3(fillBlocks), dim3(256), 0, stream, b.cellEnd, cells, 0u);
+    uint32_t fillBlocks = (2 * cells + 255) / 256 < 4096 ? (2 * cells + 255) / 256 : 4096;
+    hipLaunchKernelGGL(kFillWords, dim3(fillBlocks), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(b.cells),
+        2 * cells, 0u);
     hipLaunchKernelGGL(kPhotonRanges, dim3(blocks), dim3(256), 0, stream, ps, cells, b.sortedKeys, b.photons,
-        b.sorted, b.cellStart, b.cellEnd);
+        b.sorted, b.cells);
     return hipGetLastError();
 }
 

@@ -183,9 +183,9 @@ struct PhotonMapBuffers {
     uint64_t* sortedKeys;
     void* temp;
     size_t tempBytes;
-    uint32_t* cellStart;
-    uint32_t* cellEnd;
-    uint32_t* stored;  // photons kept inside the grid
+    CellRange* cells;
+    uint32_t* stored;    // photons kept inside the grid
+    uint32_t* occupied;  // min x, y, z then max x, y, z of the non-empty cells
 };
 
 hipError_t photonSortBytes(uint32_t count, uint32_t cells, size_t* bytes);
