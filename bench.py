@@ -23,6 +23,11 @@ algorithm's work counts (node tests, triangle/sphere tests, instance
 entries) measured by the CPU oracle on the same scene and BVH.
 The CPU baseline is the oracle (the scalar FP32 restatement of the reference
 traversal and shading) on the host cores, over a bounded sample.
+
+Beside the headline: the small Cornell box case (256x256, 16 spp, GPU and
+the full frame on the host cores), batches of uneven tasks against tasks
+one by one, the path-replay adjoint against the forward pass, and the
+shading-heavy terrain with 4M caustic photons.
 """
 import argparse
 import json
@@ -77,6 +82,47 @@ def cpu_baseline(scene, camera, threads, sample_spp):
                   f"{queries} queries, {dt:.2f} s), FP32 scalar restatement, {threads} std::threads",
         "msamples_per_s": round(st.paths / dt / 1e6, 4),
     }, st
+
+
+def measure_cornell(threads, frames=20):
+    """The small plain case: the unit Cornell box (12 triangles, emissive
+    quad, Lambertian and dielectric spheres), 256x256, 16 spp, depth 8,
+    Russian roulette from 3, on the GPU (regenerating scheduler) and in full
+    on the host cores (the oracle), with the images compared."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    host = wpt.HostScene("cornell-box")
+    scene = wpt.Scene(host.desc(), device=True, owner=host)
+    cam = host.camera(256, 256)
+    cfg = wpt.render_config(256, 256, spp=16, max_depth=8, rr_start=3, clamp=CLAMP, mode=wpt.MODE_REGENERATE)
+    r = wpt.Renderer(scene, cam, cfg)
+    r.render_frame()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        r.reset()
+        r.render_samples(0, 16)
+    wall = (time.perf_counter() - t0) / frames
+    st = r.stats()
+    img, _ = r.framebuffer()
+    r.close()
+    oracle = pyoracle.Oracle(scene)
+    t0 = time.perf_counter()
+    want, ost = oracle.render(cam, cfg, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    queries = st.closest_queries + st.shadow_queries
+    out = {
+        "workload": "cornell-box 256x256 16spp depth8 rr3 (12 triangles, 2 spheres), regenerating scheduler",
+        "mrays_per_s": round(queries / wall / 1e6, 2),
+        "msamples_per_s": round(256 * 256 * 16 / wall / 1e6, 2),
+        "ms_per_frame": round(wall * 1000.0, 3),
+        "cpu_mrays_per_s": round((ost.closest_queries + ost.shadow_queries) / cpu_s / 1e6, 3),
+        "cpu_ms_per_frame": round(cpu_s * 1000.0, 1),
+        "cpu_cores": threads,
+        "identical_to_cpu": bool(np.array_equal(img, want)),
+    }
+    log(f"cornell-box: {out}")
+    return out
 
 
 def batch_workload(n_tasks, seed=2024):
@@ -322,6 +368,7 @@ def main():
     ap.add_argument("--batch-tasks", type=int, default=256, help="tasks of the batching workload")
     ap.add_argument("--skip-diff", action="store_true", help="do not measure the differentiable renderer")
     ap.add_argument("--skip-terrain", action="store_true", help="do not measure the shading-heavy caustic workload")
+    ap.add_argument("--skip-cornell", action="store_true", help="do not measure the small Cornell box case")
     ap.add_argument("--terrain-spp", type=int, default=64)
     ap.add_argument("--terrain-photons", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--diff-steps", type=int, default=200, help="Adam steps of the inverse-rendering workload")
@@ -445,6 +492,10 @@ def main():
         # Work counts for the byte model from one sample of the frame.
         _, work = cpu_baseline(scene, camera, threads, 1)
 
+    cornell = None
+    if not args.skip_cornell:
+        cornell = measure_cornell(threads)
+
     batch = None
     if not args.skip_batch:
         batch = measure_batch(args.batch_tasks)
@@ -510,6 +561,7 @@ def main():
         "queries_per_frame": int(queries),
         "device_ms_per_frame": round(float(np.mean(device_ms)), 2),
         "other_schedulers": others,
+        "plain_small": cornell,
         "batch": batch,
         "diff": diff,
         "shading_heavy": terrain,

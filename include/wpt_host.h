@@ -12,6 +12,7 @@
  *   DistributedCoordinator (task ledger)        include/distributed/coordinator.hpp:29-79
  *   write/readDistributedCheckpoint             src/distributed/checkpoint.cpp:86-221
  *   DistributedCheckpointWriter                 include/distributed/checkpoint_writer.hpp:11-40
+ *   loadObjScene, loadEnvironmentMap            src/scene/obj_loader.cpp:47-198, src/scene/environment.cpp:66-82
  *   main (local render, coordinator role)       src/main.cu:11-53, src/distributed/coordinator_app.cpp:56-180
  */
 #ifndef WPT_HOST_H
@@ -51,6 +52,14 @@ WPT_API uint32_t wpt_host_create_tasks(uint32_t width, uint32_t height, uint32_t
                                        uint32_t samples_per_pixel, uint32_t samples_per_task, uint64_t fingerprint,
                                        wpt_task* out, uint32_t capacity);
 WPT_API uint64_t wpt_host_task_key(uint64_t fingerprint, const wpt_task* task);
+
+/* Decodes a Radiance .hdr file (the reference's loadEnvironmentMap over
+ * stb_image, src/scene/environment.cpp:66-82) into linear RGB floats, rows
+ * top to bottom.  With rgb NULL only the size is returned; else capacity
+ * (floats) must hold width*height*3.  Fails on unreadable, non-Radiance or
+ * truncated files. */
+WPT_API int wpt_host_load_hdr(const char* filename, float* rgb, uint64_t capacity, uint32_t* width,
+                              uint32_t* height);
 
 /* ---- render checkpoints ------------------------------------------------
  * Header {"PTCHKPT1", version 3, width, height, completed samples,

@@ -4,6 +4,7 @@ mirrors and dielectrics, the sorted grid) stores exactly the oracle's
 photons, and renders that gather them at diffuse vertices match the oracle
 bit for bit on every scheduler, for the spectral prism and for white-light
 caustics of a glass and a mirror sphere."""
+import json
 import os
 import subprocess
 
@@ -88,3 +89,44 @@ def test_pathtracer_binary_reports_the_photon_map(torch_cuda, tmp_path):
     assert out.returncode == 0, out.stderr
     assert "Traced 65536 photons" in out.stdout, out.stdout
     assert (tmp_path / "prism.ppm").stat().st_size > 64 * 48 * 3
+
+
+OCTAHEDRON = """v 1 0 0
+v -1 0 0
+v 0 1 0
+v 0 -1 0
+v 0 0 1
+v 0 0 -1
+f 1 3 5
+f 3 2 5
+f 2 4 5
+f 4 1 5
+f 3 1 6
+f 2 3 6
+f 4 2 6
+f 1 4 6
+"""
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_obj_crystal_room_caustics_match_oracle(torch_cuda, oracle_mod, tmp_path, mode):
+    # The crystal preset built from an OBJ file: the object becomes
+    # dispersive glass (ior 1.52, dispersion 0.02) under the crystal light.
+    (tmp_path / "gem.obj").write_text(OCTAHEDRON)
+    manifest = tmp_path / "crystal.json"
+    manifest.write_text(json.dumps({"preset": "crystal", "object": {
+        "source": "obj", "file": "gem.obj", "scale": 0.6, "translation": [0.0, 0.0, -0.5], "dielectric": True},
+        "room": {"neutral": True}, "lighting": {"profile": "crystal"}}))
+    host = wpt.HostScene(str(manifest))
+    scene = wpt.Scene(host.desc(), device=True, owner=host)
+    status, rc, _, _ = wpt.parse_command_line(["--scene-file", str(manifest), "--caustics", "--caustic-photons",
+                                               "30000", "--resolution", "48x32", "--spp", "2", "--max-depth", "8",
+                                               "--rr-depth", "3", "--rng", "pcg32"])
+    assert status == wpt.OK and rc.caustic_spectral == 1
+    rc.mode = mode
+    cam = host.camera(48, 32)
+    r = wpt.Renderer(scene, cam, rc)
+    img, _ = r.render_frame()
+    want, ost = oracle_mod.Oracle(scene).render(cam, rc)
+    assert r.stats().caustic_photons_stored == ost.caustic_photons_stored > 0
+    assert np.array_equal(img, want)

@@ -9,6 +9,7 @@
 #include "checkpoint.hpp"
 #include "config.hpp"
 #include "ledger.hpp"
+#include "loaders.hpp"
 #include "render_driver.hpp"
 #include "scene_factory.hpp"
 
@@ -133,6 +134,19 @@ WPT_API uint64_t wpt_host_task_key(uint64_t fingerprint, const wpt_task* task) {
 WPT_API void wpt_host_sha256(const void* data, size_t bytes, char out[65]) {
     std::string h = wpt::sha256Hex(data, bytes);
     std::memcpy(out, h.c_str(), 65);
+}
+
+WPT_API int wpt_host_load_hdr(const char* filename, float* rgb, uint64_t capacity, uint32_t* width,
+    uint32_t* height) {
+    if (!filename || !width || !height) return invalid("null argument");
+    std::vector<float> pixels;
+    std::string error;
+    if (!wpt::loadRadianceHdr(filename, pixels, *width, *height, &error)) return invalid(error);
+    if (rgb) {
+        if (capacity < pixels.size()) return invalid("HDR output buffer too small");
+        std::memcpy(rgb, pixels.data(), sizeof(float) * pixels.size());
+    }
+    return WPT_OK;
 }
 
 WPT_API int wpt_host_render_checkpoint_write(const char* filename, const float* rgb, uint32_t width, uint32_t height,

@@ -1,0 +1,41 @@
+// File loaders of the host layer: Wavefront OBJ/MTL meshes and Radiance HDR
+// environment maps, producing the scene description's arrays.
+//
+// OBJ follows the reference's loadObjScene (src/scene/obj_loader.cpp:47-198,
+// over tinyobjloader with triangulation): one mesh per object/group with
+// faces, material 0 the reference's default (Lambertian 0.65 0.3 0.18) and
+// MTL materials from 1 in file order, Lambertian or emissive (Ke != 0) with
+// Kd, Ke and Ni; every triangle gets vertex normals, the file's when all
+// three are given, else the normalized sum of the unnormalized face normals
+// around the vertex; texture coordinates when all three are given.
+// Polygons are split as fans from their first vertex.  Image textures
+// (map_Kd) are recorded by name but not decoded: this build has no image
+// codec, so such materials keep their Kd.
+//
+// HDR follows stb_image's Radiance decoder, which the reference's
+// loadEnvironmentMap uses (src/scene/environment.cpp:66-82): "#?RADIANCE" or
+// "#?RGBE" header, FORMAT=32-bit_rle_rgbe, "-Y h +X w" rows top to bottom,
+// flat or new-style run-length scanlines, and texel (r, g, b, e) ->
+// (r, g, b) * 2^(e - 136) (zero when e = 0).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wpt.h"
+
+namespace wpt {
+
+struct ObjScene {
+    std::vector<std::vector<wpt_triangle>> meshes;
+    std::vector<wpt_material> materials;   // [0] default, then the MTL's
+    std::vector<std::string> albedoMaps;   // per material; empty: none
+};
+
+bool loadObjScene(const std::string& filename, ObjScene& out, std::string* error);
+
+bool loadRadianceHdr(const std::string& filename, std::vector<float>& rgb, uint32_t& width, uint32_t& height,
+    std::string* error);
+
+}  // namespace wpt

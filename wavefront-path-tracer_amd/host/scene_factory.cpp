@@ -1,5 +1,7 @@
 #include "scene_factory.hpp"
 
+#include "loaders.hpp"
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -492,8 +494,91 @@ std::vector<wpt_triangle> displacedIcosphere(uint32_t n, uint32_t seed, float am
     return out;
 }
 
-bool createSceneFromConfig(const SceneConfig& config, HostScene& out, std::string* error) {
-    out = HostScene{};
+namespace {
+
+wpt_transform objectTransform(const SceneConfig& c) {
+    wpt_transform t = makeScaledTransform(c.objectTranslation.x, c.objectTranslation.y, c.objectTranslation.z,
+        c.objectScale);
+    if (c.lightProfile == CornellLightProfile::Prism) rotateY(t, -20.0f);
+    return t;
+}
+
+// The Cornell room around an OBJ object (assembleCornellScene's OBJ branch,
+// scene_host.cpp:242-355): one instance per OBJ mesh, object materials after
+// the room's five, optionally turned into dispersive glass.
+bool cornellObjScene(HostScene& s, const SceneConfig& c, std::string* error) {
+    ObjScene obj;
+    if (!loadObjScene(c.objectFilename, obj, error)) return false;
+    std::vector<float> dispersion(obj.materials.size(), 0.0f);
+    if (c.convertObjectMaterialsToDielectric) {
+        for (size_t i = 0; i < obj.materials.size(); ++i) {  // scene_host.cpp:273-282
+            wpt_material& m = obj.materials[i];
+            m.type = WPT_MATERIAL_DIELECTRIC;
+            set3(m.albedo, 1.0f, 1.0f, 1.0f);
+            set3(m.emission, 0.0f, 0.0f, 0.0f);
+            m.roughness = 0.0f;
+            m.ior = 1.52f;
+            dispersion[i] = 0.02f;
+        }
+    }
+    for (std::vector<wpt_triangle>& mesh : obj.meshes) {
+        uint32_t first = static_cast<uint32_t>(s.meshTriangles.size());
+        for (wpt_triangle& t : mesh) {
+            t.material = c.objectMaterialOverride >= 0 ? static_cast<uint32_t>(c.objectMaterialOverride) : t.material + 5;
+            s.meshTriangles.push_back(t);
+        }
+        s.meshes.push_back(wpt_mesh{first, static_cast<uint32_t>(mesh.size())});
+        wpt_instance inst{};
+        inst.mesh = static_cast<uint32_t>(s.meshes.size() - 1);
+        inst.material_override = WPT_INVALID_INDEX;
+        inst.acceleration = WPT_ACCEL_BVH;
+        inst.transform = objectTransform(c);
+        s.instances.push_back(inst);
+    }
+    s.materials = obj.materials;
+    referenceCornellRoom(s, c, c.convertObjectMaterialsToDielectric, c.lightProfile == CornellLightProfile::Prism);
+    s.materialExt.assign(s.materials.size(), plainMaterialExt());
+    for (size_t i = 0; i < dispersion.size(); ++i) s.materialExt[5 + i].dispersion = dispersion[i];
+    return true;
+}
+
+// An OBJ object over a ground sphere (assembleObjScene,
+// scene_host.cpp:391-418): the TLAS/BLAS policy instances the mesh, the
+// others bake the transform into directly tested triangles.
+bool objScene(HostScene& s, const SceneConfig& c, std::string* error) {
+    ObjScene obj;
+    if (!loadObjScene(c.objectFilename, obj, error)) return false;
+    s.spheres.push_back(sphere(0.0f, -100.5f, 0.0f, 100.0f, 0, WPT_ACCEL_DIRECT));
+    s.materials.push_back(makeMaterial(WPT_MATERIAL_DIFFUSE, 0.75f, 0.75f, 0.75f));
+    s.materials.insert(s.materials.end(), obj.materials.begin(), obj.materials.end());
+    std::vector<wpt_triangle> all;
+    for (const std::vector<wpt_triangle>& mesh : obj.meshes) all.insert(all.end(), mesh.begin(), mesh.end());
+    for (wpt_triangle& t : all) t.material += 1;
+    if (c.acceleration == ObjAccelerationPolicy::TlasBlasBvh2) {
+        s.meshTriangles = std::move(all);
+        s.meshes.push_back(wpt_mesh{0, static_cast<uint32_t>(s.meshTriangles.size())});
+        wpt_instance inst{};
+        inst.mesh = 0;
+        inst.material_override = WPT_INVALID_INDEX;
+        inst.acceleration = WPT_ACCEL_BVH;
+        inst.transform = makeScaledTransform(c.objectTranslation.x, c.objectTranslation.y, c.objectTranslation.z,
+            c.objectScale);
+        s.instances.push_back(inst);
+    } else {
+        const float tr[3] = {c.objectTranslation.x, c.objectTranslation.y, c.objectTranslation.z};
+        for (wpt_triangle& t : all) {
+            for (float* v : {t.v0, t.v1, t.v2})
+                for (int k = 0; k < 3; ++k) v[k] = c.objectScale * v[k] + tr[k];
+        }
+        s.flatTriangles = std::move(all);
+    }
+    return true;
+}
+
+}  // namespace
+
+namespace {
+bool buildPreset(const SceneConfig& config, HostScene& out, std::string* error) {
     switch (config.preset) {
     case ScenePreset::Demo:
         demoScene(out);
@@ -506,8 +591,6 @@ bool createSceneFromConfig(const SceneConfig& config, HostScene& out, std::strin
         return true;
     case ScenePreset::Terrain:
         terrainScene(out, config);
-        out.useNormalMaps = config.useNormalMaps;
-        out.normalMapMinimumCosine = config.normalMapMinimumCosine;
         return true;
     case ScenePreset::SphereField:
         sphereFieldScene(out, config, true);
@@ -531,19 +614,47 @@ bool createSceneFromConfig(const SceneConfig& config, HostScene& out, std::strin
             config.objectTranslation.z, config.objectScale);
         if (config.lightProfile == CornellLightProfile::Prism) rotateY(inst.transform, -20.0f);
         out.instances.push_back(inst);
-        referenceCornellRoom(out, config, config.convertObjectMaterialsToDielectric &&
-            config.lightProfile == CornellLightProfile::Crystal, config.lightProfile == CornellLightProfile::Prism);
+        // The light follows the dielectric switch first, then the profile
+        // (scene_host.cpp:322-326).
+        referenceCornellRoom(out, config, config.convertObjectMaterialsToDielectric,
+            config.lightProfile == CornellLightProfile::Prism);
         // The prism splits photons by channel (scene_host.cpp:251-255).
         out.materialExt.assign(out.materials.size(), plainMaterialExt());
         out.materialExt[5].dispersion = 0.035f;
         return true;
     }
+    case ScenePreset::Cornell:
+    case ScenePreset::Hurricane:
+    case ScenePreset::Crystal:
+        if (config.objectSource != CornellObjectSource::Obj) break;
+        return cornellObjScene(out, config, error);
+    case ScenePreset::Deer:
+        return objScene(out, config, error);
     default:
-        if (error)
-            *error = std::string("scene preset '") + presetName(config.preset) +
-                "' needs external OBJ/glTF assets, which this renderer does not load";
-        return false;
+        break;
     }
+    if (error)
+        *error = std::string("scene preset '") + presetName(config.preset) +
+            "' needs glTF or other external assets, which this renderer does not load";
+    return false;
+}
+}  // namespace
+
+bool createSceneFromConfig(const SceneConfig& config, HostScene& out, std::string* error) {
+    out = HostScene{};
+    if (!buildPreset(config, out, error)) return false;
+    out.useNormalMaps = config.useNormalMaps;
+    out.normalMapMinimumCosine = config.normalMapMinimumCosine;
+    if (!config.environmentFilename.empty()) {
+        // Any preset may light with a panorama (applySceneConfig,
+        // scene_factory.cpp:92-104).
+        if (!loadRadianceHdr(config.environmentFilename, out.environment, out.environmentWidth,
+                out.environmentHeight, error))
+            return false;
+        out.environmentIntensity = config.environmentIntensity;
+        out.environmentRotationDegrees = config.environmentRotationDegrees;
+    }
+    return true;
 }
 
 namespace {

@@ -306,6 +306,7 @@ def library() -> C.CDLL:
         "wpt_host_task_key": ([u64, C.POINTER(Task)], u64),
         "wpt_host_scene_fingerprint": ([C.c_char_p, C.POINTER(u64)], C.c_int),
         "wpt_renderer_load_framebuffer": ([P, P, u32], C.c_int),
+        "wpt_host_load_hdr": ([C.c_char_p, P, u64, C.POINTER(u32), C.POINTER(u32)], C.c_int),
         "wpt_host_render_checkpoint_write": ([C.c_char_p, P, u32, u32, u32, u64], C.c_int),
         "wpt_host_render_checkpoint_read": ([C.c_char_p, u32, u32, u64, P, C.POINTER(u32)], C.c_int),
         "wpt_ledger_create": ([C.POINTER(LedgerJob), P, u32, C.POINTER(P)], C.c_int),
@@ -883,6 +884,16 @@ def create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint) -> L
     arr = (Task * max(1, n))()
     lib.wpt_host_create_tasks(width, height, tile_w, tile_h, spp, per_task, fingerprint, C.cast(arr, C.c_void_p), n)
     return [arr[i] for i in range(n)]
+
+
+def load_hdr(filename) -> np.ndarray:
+    """A Radiance .hdr file as float32 (height, width, 3)."""
+    w, h = C.c_uint32(), C.c_uint32()
+    check(library().wpt_host_load_hdr(str(filename).encode(), None, 0, C.byref(w), C.byref(h)), "load_hdr")
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    check(library().wpt_host_load_hdr(str(filename).encode(), out.ctypes.data, out.size, C.byref(w), C.byref(h)),
+          "load_hdr")
+    return out
 
 
 def write_render_checkpoint(filename: str, rgb: np.ndarray, sample_index: int, fingerprint: int) -> None:
