@@ -485,7 +485,10 @@ bool intersectAabb(const Ray<R>& ray, const V3<R>& inv, const wpt_bvh2_node& n, 
     R tmin = std::fmax(std::fmin(tx0, tx1), std::fmax(std::fmin(ty0, ty1), std::fmin(tz0, tz1)));
     R tmax = std::fmin(std::fmax(tx0, tx1), std::fmin(std::fmax(ty0, ty1), std::fmax(tz0, tz1)));
     nearT = tmin;
-    return tmax >= std::fmax(tmin, R(1.0e-6f)) && tmin < closestT;
+    // Culled only beyond closestT * (1 + 2^-20): a box holding a hit at
+    // exactly closestT can have a computed entry a few ulps above it, and
+    // that hit must still take part in the tie order below.
+    return tmax >= std::fmax(tmin, R(1.0e-6f)) && tmin <= closestT * R(1.00000095f);
 }
 
 // ---------------------------------------------------------------- closest hit (TraceHitState style)
@@ -500,8 +503,25 @@ struct HitState {
     uint32_t sphereIndex = kInvalid;
     // identity of the current winner
     uint32_t kind = WPT_HIT_NONE, instance = kInvalid, prim = kInvalid;
+    uint32_t rank = kInvalid;
     R u = R(0), v = R(0);
 };
+
+// Order of equally distant hits: by category in the reference's traceRay
+// visit order (spheres first: device_traversal.cuh:789-800 lets a sphere
+// displace an equally distant triangle; then direct-instance, static,
+// TLAS-instance and flat triangles), then instance, then primitive.  Within
+// one BVH the reference keeps whichever tie its (unpinned) tree visits
+// first; a fixed order makes the result independent of the tree.
+enum : uint32_t { kRankSphere = 0, kRankDirectInstance = 1, kRankStatic = 2, kRankTlasInstance = 3, kRankFlat = 4 };
+
+template <class R>
+bool nearerHit(R t, uint32_t rank, uint32_t instance, uint32_t prim, const HitState<R>& hs) {
+    if (t != hs.closestT) return t < hs.closestT;
+    if (rank != hs.rank) return rank < hs.rank;
+    if (instance != hs.instance) return instance < hs.instance;
+    return prim < hs.prim;
+}
 
 template <class R>
 V3<R> triNormal(const wpt_triangle& tri, const Ray<R>& ray, R u, R v, bool faceForward) {  // :100-111
@@ -545,9 +565,9 @@ bool cutout(const oracle_handle& h, uint32_t m, const wpt_triangle& tri, R u, R 
 // intersectTriangleHit, device_traversal.cuh:137-180.
 template <class R>
 bool triangleHit(const oracle_handle& h, const Ray<R>& ray, const wpt_triangle& tri, uint32_t triLight, HitState<R>& hs,
-    uint32_t materialOverride, uint32_t kind, uint32_t instance, uint32_t prim) {
+    uint32_t materialOverride, uint32_t kind, uint32_t rank, uint32_t instance, uint32_t prim) {
     R t, u, v;
-    if (!intersectTriangle(ray, tri, t, u, v) || t >= hs.closestT) return false;
+    if (!intersectTriangle(ray, tri, t, u, v) || !nearerHit(t, rank, instance, prim, hs)) return false;
     V2<R> uv = triUv(tri, u, v);
     uint32_t m = materialOverride == kInvalid ? tri.material : materialOverride;
     if (cutout(h, m, tri, u, v)) return false;
@@ -575,6 +595,7 @@ bool triangleHit(const oracle_handle& h, const Ray<R>& ray, const wpt_triangle& 
     hs.bitangent1 = vec<R>(tri.bitangent1);
     hs.lightIndex = materialOverride == kInvalid ? triLight : kInvalid;
     hs.kind = kind;
+    hs.rank = rank;
     hs.instance = instance;
     hs.prim = prim;
     hs.u = u;
@@ -585,12 +606,12 @@ bool triangleHit(const oracle_handle& h, const Ray<R>& ray, const wpt_triangle& 
 template <class R>
 void sphereHit(const oracle_handle& h, const Ray<R>& ray, uint32_t index, HitState<R>& hs) {  // :789-800
     R t;
-    if (!intersectSphere(ray, h.d.spheres[index], t) || t > hs.closestT) return;
-    if (t == hs.closestT && index >= hs.sphereIndex) return;
+    if (!intersectSphere(ray, h.d.spheres[index], t) || !nearerHit(t, kRankSphere, kInvalid, index, hs)) return;
     hs.closestT = t;
     hs.closestMaterial = static_cast<int>(h.d.spheres[index].material);
     hs.sphereIndex = index;
     hs.kind = WPT_HIT_SPHERE;
+    hs.rank = kRankSphere;
     hs.instance = kInvalid;
     hs.prim = index;
     hs.u = hs.v = R(0);
@@ -599,7 +620,8 @@ void sphereHit(const oracle_handle& h, const Ray<R>& ray, uint32_t index, HitSta
 // intersectNexusBvh2 over triangles, device_traversal.cuh:239-313.
 template <class R>
 void bvhTriangles(const oracle_handle& h, const Ray<R>& ray, const BvhView& bvh, const wpt_triangle* tris,
-    const uint32_t* triLights, HitState<R>& hs, uint32_t materialOverride, uint32_t kind, uint32_t instance) {
+    const uint32_t* triLights, HitState<R>& hs, uint32_t materialOverride, uint32_t kind, uint32_t rank,
+    uint32_t instance) {
     V3<R> inv(R(1) / ray.d.x, R(1) / ray.d.y, R(1) / ray.d.z);
     uint32_t stack[64];
     uint32_t size = 1;
@@ -610,7 +632,8 @@ void bvhTriangles(const oracle_handle& h, const Ray<R>& ray, const BvhView& bvh,
         if (!intersectAabb(ray, inv, node, hs.closestT, nearT)) continue;
         if (node.left == kInvalid) {
             uint32_t p = node.right;
-            triangleHit(h, ray, tris[p], triLights ? triLights[p] : kInvalid, hs, materialOverride, kind, instance, p);
+            triangleHit(h, ray, tris[p], triLights ? triLights[p] : kInvalid, hs, materialOverride, kind, rank, instance,
+                p);
         } else {
             R ln, rn;
             bool hl = intersectAabb(ray, inv, bvh.nodes[node.left], hs.closestT, ln);
@@ -662,15 +685,20 @@ void meshInstance(const oracle_handle& h, const Ray<R>& ray, uint32_t index, Hit
     const wpt_triangle* tris = h.d.mesh_triangles + mesh.first_triangle;
     const uint32_t* lights = h.meshLight.data() + mesh.first_triangle;
     Ray<R> local = localRay(inst.transform, ray);
-    R previous = hs.closestT;
+    const uint32_t previousRank = hs.rank, previousInstance = hs.instance, previousPrim = hs.prim;
+    const R previous = hs.closestT;
     const BvhView& bvh = h.blas[inst.mesh];
+    const uint32_t rank = useBvh ? kRankTlasInstance : kRankDirectInstance;
     if (useBvh && bvh.count) {
-        bvhTriangles(h, local, bvh, tris, lights, hs, inst.material_override, WPT_HIT_INSTANCE_TRIANGLE, index);
+        bvhTriangles(h, local, bvh, tris, lights, hs, inst.material_override, WPT_HIT_INSTANCE_TRIANGLE, rank, index);
     } else {
         for (uint32_t i = 0; i < mesh.triangle_count; ++i)
-            triangleHit(h, local, tris[i], lights[i], hs, inst.material_override, WPT_HIT_INSTANCE_TRIANGLE, index, i);
+            triangleHit(h, local, tris[i], lights[i], hs, inst.material_override, WPT_HIT_INSTANCE_TRIANGLE, rank, index,
+                i);
     }
-    if (hs.closestT >= previous) return;
+    // Unchanged winner: nothing of this instance to transform.
+    if (hs.closestT == previous && hs.rank == previousRank && hs.instance == previousInstance && hs.prim == previousPrim)
+        return;
     hs.geometricNormal = xformNormal(inst.transform, hs.geometricNormal);
     hs.normal = xformNormal(inst.transform, hs.normal);
     hs.tangent = normalize(xformVector(inst.transform, hs.tangent));
@@ -717,10 +745,12 @@ void sceneTlas(const oracle_handle& h, const Ray<R>& ray, HitState<R>& hs) {  //
 template <class R>
 void directTriangles(const oracle_handle& h, const Ray<R>& ray, HitState<R>& hs) {  // :880-1033
     if (h.flat.count) {
-        bvhTriangles(h, ray, h.flat, h.d.flat_triangles, nullptr, hs, kInvalid, WPT_HIT_FLAT_TRIANGLE, kInvalid);
+        bvhTriangles(h, ray, h.flat, h.d.flat_triangles, nullptr, hs, kInvalid, WPT_HIT_FLAT_TRIANGLE, kRankFlat,
+            kInvalid);
     } else {
         for (uint32_t i = 0; i < h.d.flat_triangle_count; ++i)
-            triangleHit(h, ray, h.d.flat_triangles[i], kInvalid, hs, kInvalid, WPT_HIT_FLAT_TRIANGLE, kInvalid, i);
+            triangleHit(h, ray, h.d.flat_triangles[i], kInvalid, hs, kInvalid, WPT_HIT_FLAT_TRIANGLE, kRankFlat, kInvalid,
+                i);
     }
 }
 
@@ -739,13 +769,13 @@ bool traceRay(const oracle_handle& h, const Ray<R>& ray, Hit<R>& hit, HitState<R
     for (uint32_t i = 0; i < h.directSpheres.size(); ++i) sphereHit(h, ray, h.directSpheres[i], hs);
     for (uint32_t i = 0; i < h.directInstances.size(); ++i) meshInstance(h, ray, h.directInstances[i], hs, false);
     for (uint32_t i = 0; i < h.d.static_triangle_count; ++i)
-        if (triangleHit(h, ray, h.d.static_triangles[i], h.staticLight[i], hs, kInvalid, WPT_HIT_STATIC_TRIANGLE, kInvalid, i))
+        if (triangleHit(h, ray, h.d.static_triangles[i], h.staticLight[i], hs, kInvalid, WPT_HIT_STATIC_TRIANGLE,
+                kRankStatic, kInvalid, i))
             hs.sphereIndex = kInvalid;
     sceneTlas(h, ray, hs);
     if (h.d.flat_triangle_count > 0) {
-        R previous = hs.closestT;
         directTriangles(h, ray, hs);
-        if (hs.closestT < previous) hs.sphereIndex = kInvalid;
+        if (hs.kind == WPT_HIT_FLAT_TRIANGLE) hs.sphereIndex = kInvalid;
     }
     if (hs.sphereIndex != kInvalid) {
         const wpt_sphere& s = h.d.spheres[hs.sphereIndex];

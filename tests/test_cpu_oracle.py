@@ -243,3 +243,41 @@ def test_diff_image_is_linear_in_emission(oracle_mod):
     v[t + r:] = params[t + r:]
     dual = oracle.diff_directional64(cam, cfg, params, v)
     assert np.allclose(dual, img, rtol=1e-6, atol=1e-7)
+
+
+def tie_scene():
+    """Exact distance ties: a mesh holding each triangle twice (identical
+    vertices, so identical t, u, v), instanced twice at the same place."""
+    b = wpt.SceneBuilder()
+    m = [b.material(wpt.make_material(wpt.DIFFUSE, (0.1 * (i + 1), 0.5, 0.5))) for i in range(2)]
+    rng = np.random.default_rng(3)
+    tris = []
+    for k in range(40):
+        c = rng.uniform(-1, 1, 3) * np.array([1, 0, 1])
+        tris.append(wpt.make_triangle(c, c + (0.3, 0, 0), c + (0, 0, 0.3), m[1]))
+    first = [wpt.make_triangle(list(t.v0), list(t.v1), list(t.v2), m[0]) for t in tris]
+    mesh = b.mesh(first + tris)  # prim i and 40 + i coincide
+    b.instance(mesh)
+    b.instance(mesh)
+    return b
+
+
+def tie_rays(n=4096, seed=5):
+    rng = np.random.default_rng(seed)
+    o = np.concatenate([rng.uniform(-1.2, 1.2, (n, 1)), np.full((n, 1), 2.0), rng.uniform(-1.2, 1.2, (n, 1))], 1)
+    d = np.tile([0.0, -1.0, 0.0], (n, 1))
+    d[n // 2:, 0] = rng.uniform(-0.05, 0.05, n // 2)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.ascontiguousarray(np.concatenate([o, d], 1).T.astype(np.float32))
+
+
+def test_exact_ties_take_the_lowest_instance_and_primitive(oracle_mod):
+    # Whatever the tree visits first, an exact tie goes to the lower
+    # instance, then the lower primitive: instance 0, the first copy.
+    b = tie_scene()
+    scene = wpt.Scene(b.desc(), device=False, owner=b)
+    hits = oracle_mod.Oracle(scene).trace(tie_rays())
+    mesh_hits = hits["kind"] == wpt.HIT_INSTANCE
+    assert mesh_hits.sum() > 100
+    assert np.all(hits["instance"][mesh_hits] == 0) and np.all(hits["primitive"][mesh_hits] < 40)
+    assert np.all(hits["material"][mesh_hits] == 0)

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(512) kBatchReset(BatchContext ctx) {
 }
 
 template <bool Full>
-__global__ void __launch_bounds__(512) kBatch(BatchContext ctx) {
+__global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY kBatch(BatchContext ctx) {
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const uint32_t n = ctx.taskCount;
     const uint32_t waveId = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;

@@ -13,6 +13,11 @@ constexpr uint32_t kSpillEntries = kMaxStackDepth - kLdsStackEntries;
 
 // Occupancy target of the traversal kernels (waves per SIMD), 0: compiler's
 // choice.  The register budget it implies must not force scratch spills.
+#if defined(WPT_BATCH_WAVES_PER_EU) && WPT_BATCH_WAVES_PER_EU > 0
+#define WPT_BATCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WPT_BATCH_WAVES_PER_EU)))
+#else
+#define WPT_BATCH_OCCUPANCY
+#endif
 #if defined(WPT_TRACE_WAVES_PER_EU) && WPT_TRACE_WAVES_PER_EU > 0
 #define WPT_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WPT_TRACE_WAVES_PER_EU)))
 #else

@@ -32,7 +32,31 @@ struct HitRecord {
     uint32_t instance;  // instance index for instance triangles
     uint32_t prim;      // sphere / triangle index in the caller's order
     uint32_t material;  // resolved (override applied)
+    uint32_t rank;      // tie order of the hit's category (kRank*)
 };
+
+// Hits at exactly the same distance are ordered by category -- spheres,
+// direct-instance triangles, static triangles, TLAS-instance triangles,
+// flat triangles -- then by instance, then by primitive.  Across categories
+// this is the reference's outcome (traceRay visits the categories in that
+// order and a sphere displaces an equally distant triangle,
+// device_traversal.cuh:789-800,1115-1184).  Within a BVH the reference keeps
+// whichever tie its tree visits first; fixing the order instead makes the
+// nearest hit independent of tree shape and visit order, so that any BVH
+// (binary or wide, host- or device-built) gives the same hits.
+constexpr uint32_t kRankSphere = 0, kRankDirectInstance = 1, kRankStatic = 2, kRankTlasInstance = 3, kRankFlat = 4;
+
+// Boxes are culled only beyond tieReach(closestT): a box holding a hit at
+// exactly closestT can have a computed entry distance a few ulps above it,
+// and must still be opened for that hit to take part in the tie order.
+WPT_HD float tieReach(float closestT) { return closestT * 1.00000095f; }
+
+WPT_HD bool nearer(float t, uint32_t rank, uint32_t instance, uint32_t prim, const HitRecord& best) {
+    if (t != best.t) return t < best.t;
+    if (rank != best.rank) return rank < best.rank;
+    if (instance != best.instance) return instance < best.instance;
+    return prim < best.prim;
+}
 
 // Alpha mask of a candidate triangle hit (reference intersectTriangleHit,
 // device_traversal.cuh:146-151): masked materials below their cutoff are
@@ -62,6 +86,7 @@ WPT_HD HitRecord emptyHit() {
     h.instance = kInvalid;
     h.prim = kInvalid;
     h.material = kInvalid;
+    h.rank = kInvalid;
     return h;
 }
 
@@ -133,7 +158,7 @@ WPT_HD bool slab(const RayInv& r, float mnx, float mny, float mnz, float mxx, fl
     float tmin = fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1)));
     float tmax = fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1)));
     nearT = tmin;
-    return tmax >= fmaxf(tmin, kRayMinDistance) && tmin < closestT;
+    return tmax >= fmaxf(tmin, kRayMinDistance) && tmin <= tieReach(closestT);
 }
 
 WPT_HD Vec3 xyz(const float4& f) { return Vec3(f.x, f.y, f.z); }
@@ -170,7 +195,7 @@ WPT_HD bool popLive(Stack& stack, uint32_t base, float closestT, uint32_t& cur) 
     while (stack.size > base) {
         float t;
         stack.pop(cur, t);
-        if (t < closestT) return true;
+        if (t <= tieReach(closestT)) return true;
     }
     return false;
 }
@@ -178,7 +203,7 @@ WPT_HD bool popLive(Stack& stack, uint32_t base, float closestT, uint32_t& cur) 
 // Closest triangle of one BVH (BLAS in local space, or the flat-triangle BVH).
 template <bool Full, class Stack>
 WPT_HD void closestInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, const float4* geometry,
-    const TriShade* tris, uint32_t materialOverride, uint32_t kind, uint32_t instance, HitRecord& best,
+    const TriShade* tris, uint32_t materialOverride, uint32_t kind, uint32_t rank, uint32_t instance, HitRecord& best,
     Stack& stack) {
     float nearT;
     if (!slab(r, bvh.rootMin[0], bvh.rootMin[1], bvh.rootMin[2], bvh.rootMax[0], bvh.rootMax[1], bvh.rootMax[2],
@@ -193,7 +218,8 @@ WPT_HD void closestInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, con
             float4 b = geometry[3 * slot + 1];
             float4 c = geometry[3 * slot + 2];
             float t, u, v;
-            if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < best.t) {
+            if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) &&
+                nearer(t, rank, instance, wbits(a), best)) {
                 uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
                 if (!cutAway<Full>(s, material, tris, wbits(a), u, v)) {
                     best.t = t;
@@ -203,6 +229,7 @@ WPT_HD void closestInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, con
                     best.instance = instance;
                     best.prim = wbits(a);
                     best.material = material;
+                    best.rank = rank;
                 }
             }
         } else {
@@ -281,11 +308,11 @@ WPT_HD bool anyInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, const f
 // static triangles, flat triangles without a BVH).
 template <bool Full>
 WPT_HD void closestBrute(const DevScene& s, const Ray& ray, const TriShade* tris, uint32_t count,
-    uint32_t materialOverride, uint32_t kind, uint32_t instance, HitRecord& best) {
+    uint32_t materialOverride, uint32_t kind, uint32_t rank, uint32_t instance, HitRecord& best) {
     for (uint32_t i = 0; i < count; ++i) {
         const TriShade& tri = tris[i];
         float t, u, v;
-        if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !(t < best.t)) continue;
+        if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !nearer(t, rank, instance, i, best)) continue;
         uint32_t material = materialOverride == kInvalid ? tri.material : materialOverride;
         if (cutAway<Full>(s, material, tris, i, u, v)) continue;
         best.t = t;
@@ -295,6 +322,7 @@ WPT_HD void closestBrute(const DevScene& s, const Ray& ray, const TriShade* tris
         best.instance = instance;
         best.prim = i;
         best.material = material;
+        best.rank = rank;
     }
 }
 
@@ -323,15 +351,14 @@ WPT_HD Ray toLocal(const InstanceTransform& tr, const Ray& ray) {
 WPT_HD void sphereLeaf(const DevScene& s, const Ray& ray, uint32_t index, HitRecord& best) {
     float4 sp = s.spheres[index];
     float t;
-    if (!intersectSphere(ray, xyz(sp), sp.w, t) || t > best.t) return;
-    uint32_t currentSphere = best.kind == kHitSphere ? best.prim : kInvalid;
-    if (t == best.t && index >= currentSphere) return;
+    if (!intersectSphere(ray, xyz(sp), sp.w, t) || !nearer(t, kRankSphere, kInvalid, index, best)) return;
     best.t = t;
     best.u = best.v = 0.0f;
     best.kind = kHitSphere;
     best.instance = kInvalid;
     best.prim = index;
     best.material = s.sphereMaterial[index];
+    best.rank = kRankSphere;
 }
 
 template <bool Full, class Stack>
@@ -342,11 +369,11 @@ WPT_HD void instanceClosest(const DevScene& s, const Ray& ray, uint32_t index, b
     Ray local = toLocal(inst.transform, ray);
     if (useBvh && blas.bvh.primCount != 0) {
         RayInv ri = makeRayInv(local);
-        closestInBvh<Full>(s, ri, blas.bvh, blas.geometry, blas.triangles, inst.materialOverride, kHitInstance, index, best,
-            stack);
+        closestInBvh<Full>(s, ri, blas.bvh, blas.geometry, blas.triangles, inst.materialOverride, kHitInstance,
+            useBvh ? kRankTlasInstance : kRankDirectInstance, index, best, stack);
     } else {
-        closestBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride,
-            kHitInstance, index, best);
+        closestBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride, kHitInstance,
+            useBvh ? kRankTlasInstance : kRankDirectInstance, index, best);
     }
 }
 
@@ -364,7 +391,7 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
     HitRecord best = emptyHit();
     for (uint32_t i = 0; i < s.directSphereCount; ++i) sphereLeaf(s, ray, s.directSpheres[i], best);
     for (uint32_t i = 0; i < s.directInstanceCount; ++i) instanceClosest<Full>(s, ray, s.directInstances[i], false, best, stack);
-    closestBrute<Full>(s, ray, s.staticTriangles, s.staticCount, kInvalid, kHitStatic, kInvalid, best);
+    closestBrute<Full>(s, ray, s.staticTriangles, s.staticCount, kInvalid, kHitStatic, kRankStatic, kInvalid, best);
     if (s.tlas.primCount != 0) {
         RayInv r = makeRayInv(ray);
         float nearT;
@@ -385,7 +412,8 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
                         float4 b = geometry[3 * slot + 1];
                         float4 c = geometry[3 * slot + 2];
                         float t, u, v;
-                        if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < best.t) {
+                        if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) &&
+                            nearer(t, kRankTlasInstance, instance, wbits(a), best)) {
                             uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
                             if (s.materialCutout[material] == kCutNever ||
                                 !cutAway<Full>(s, material, s.blases[s.instances[instance].mesh].triangles, wbits(a), u, v)) {
@@ -396,6 +424,7 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
                                 best.instance = instance;
                                 best.prim = wbits(a);
                                 best.material = material;
+                                best.rank = kRankTlasInstance;
                             }
                         }
                     } else {
@@ -421,7 +450,7 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
                                 }
                             } else {
                                 closestBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride,
-                                    kHitInstance, leaf, best);
+                                    kHitInstance, kRankTlasInstance, leaf, best);
                             }
                         }
                     }
@@ -458,9 +487,10 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
     if (s.flatCount != 0) {
         if (s.flat.primCount != 0) {
             RayInv r = makeRayInv(ray);
-            closestInBvh<Full>(s, r, s.flat, s.flatGeometry, s.flatTriangles, kInvalid, kHitFlat, kInvalid, best, stack);
+            closestInBvh<Full>(s, r, s.flat, s.flatGeometry, s.flatTriangles, kInvalid, kHitFlat, kRankFlat, kInvalid,
+                best, stack);
         } else {
-            closestBrute<Full>(s, ray, s.flatTriangles, s.flatCount, kInvalid, kHitFlat, kInvalid, best);
+            closestBrute<Full>(s, ray, s.flatTriangles, s.flatCount, kInvalid, kHitFlat, kRankFlat, kInvalid, best);
         }
     }
     return best;
