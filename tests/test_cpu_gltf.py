@@ -1,0 +1,450 @@
+"""glTF import on the CPU: the PNG decoder behind texture import (every
+colour type, bit depth, filter and interlacing, checked against stb_image
+built from the reference's vendored copy when it is present, and against
+values fixed by the PNG format otherwise) and the glTF loader
+(src/scene/gltf_loader.cpp:307-503): primitives, accessors with strides
+and three index widths, node hierarchies, materials with their extensions
+and texture transforms, KHR_lights_punctual, embedded, binary-chunk and
+external data, and the sponza and Cornell-room assemblies
+(src/scene/scene_host.cpp:242-385)."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import wpt
+from gltf_fixtures import (ADAM7, encode_png, gltf_document, quat, trs, write_glb, write_gltf)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+STB = os.path.join(ROOT, "oracle", "_ref", "libref_stb.so")
+
+
+def stb_decode(data: bytes):
+    """stb_image's RGBA8 decode, or None when it rejects the data."""
+    lib = C.CDLL(STB)
+    lib.wpt_ref_stbi_rgba.restype = C.c_longlong
+    lib.wpt_ref_stbi_rgba.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_longlong, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]
+    w, h = C.c_int(), C.c_int()
+    n = lib.wpt_ref_stbi_rgba(data, len(data), None, 0, C.byref(w), C.byref(h))
+    if n < 0:
+        return None
+    out = np.zeros((h.value, w.value, 4), np.uint8)
+    lib.wpt_ref_stbi_rgba(data, len(data), out.ctypes.data, out.size, C.byref(w), C.byref(h))
+    return out
+
+
+needs_stb = pytest.mark.skipif(not os.path.exists(STB), reason="stb_image checker not built (no reference tree)")
+
+GREY_SCALE = {1: 255, 2: 85, 4: 17, 8: 1}
+COMBOS = [(0, 1), (0, 2), (0, 4), (0, 8), (0, 16), (2, 8), (2, 16), (3, 1), (3, 2), (3, 4), (3, 8),
+          (4, 8), (4, 16), (6, 8), (6, 16)]
+
+
+def image_case(color, depth, w=13, h=7, seed=0):
+    rng = np.random.default_rng(seed * 100 + color * 20 + depth)
+    ch = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[color]
+    palette = None
+    if color == 3:
+        n = min(1 << depth, 200)
+        samples = rng.integers(0, n, (h, w, 1))
+        palette = rng.integers(0, 256, (n, 3))
+    else:
+        samples = rng.integers(0, 1 << depth, (h, w, ch))
+    return samples, palette
+
+
+def expected_rgba(samples, color, depth, palette=None, palette_alpha=None, key=None):
+    s = samples.astype(np.int64)
+    to8 = (lambda v: v >> 8) if depth == 16 else (lambda v: v)
+    h, w = s.shape[:2]
+    out = np.zeros((h, w, 4), np.int64)
+    if color == 3:
+        pal = np.concatenate([np.asarray(palette), np.full((len(palette), 1), 255)], 1)
+        if palette_alpha is not None:
+            pal[:len(palette_alpha), 3] = palette_alpha
+        out = pal[s[..., 0]]
+    elif color in (0, 4):
+        g = s[..., 0] * (GREY_SCALE[depth] if depth < 8 else 1)
+        out[..., :3] = to8(g)[..., None]
+        out[..., 3] = to8(s[..., 1]) if color == 4 else 255
+        if key is not None:
+            out[..., 3] = np.where(s[..., 0] == key[0], 0, 255)
+    else:
+        out[..., :3] = to8(s[..., :3])
+        out[..., 3] = to8(s[..., 3]) if color == 6 else 255
+        if key is not None:
+            out[..., 3] = np.where(np.all(s[..., :3] == np.asarray(key), -1), 0, 255)
+    return out.astype(np.uint8)
+
+
+@pytest.mark.parametrize("interlace", [False, True])
+@pytest.mark.parametrize("color,depth", COMBOS)
+def test_png_colour_types_depths_filters_and_interlacing(color, depth, interlace):
+    samples, palette = image_case(color, depth)
+    png = encode_png(samples, color, depth, palette=palette, interlace=interlace)
+    got = wpt.decode_image(png)
+    assert np.array_equal(got, expected_rgba(samples, color, depth, palette))
+    if os.path.exists(STB):
+        assert np.array_equal(got, stb_decode(png))
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 9), (9, 1), (3, 3), (8, 8), (17, 5)])
+def test_png_adam7_pass_shapes(w, h):
+    # Small sizes leave some of the seven passes empty.
+    rng = np.random.default_rng(w * 31 + h)
+    samples = rng.integers(0, 4, (h, w, 1))
+    png = encode_png(samples, 0, 2, interlace=True)
+    assert np.array_equal(wpt.decode_image(png), expected_rgba(samples, 0, 2))
+    empty = sum(1 for x0, y0, _, _ in ADAM7 if x0 >= w or y0 >= h)
+    assert empty == sum(1 for x0, y0, dx, dy in ADAM7 if samples[y0::dy, x0::dx].size == 0)
+    if os.path.exists(STB):
+        assert np.array_equal(wpt.decode_image(png), stb_decode(png))
+
+
+def test_png_transparency_chunks():
+    rng = np.random.default_rng(7)
+    # Palette alphas shorter than the palette: the rest stay opaque.
+    idx = rng.integers(0, 6, (5, 6, 1))
+    pal = rng.integers(0, 256, (6, 3))
+    png = encode_png(idx, 3, 4, palette=pal, trns=bytes([0, 128, 255]))
+    want = expected_rgba(idx, 3, 4, pal, palette_alpha=[0, 128, 255])
+    assert np.array_equal(wpt.decode_image(png), want)
+    cases = [(0, 2, [2]), (0, 8, [77]), (0, 16, [4097]), (2, 8, [10, 20, 30]), (2, 16, [300, 600, 900])]
+    for color, depth, key in cases:
+        samples, _ = image_case(color, depth, seed=3)
+        samples[0, :3] = key  # guarantee matches
+        trns = b"".join(int(k).to_bytes(2, "big") for k in key)
+        png = encode_png(samples, color, depth, trns=trns)
+        want = expected_rgba(samples, color, depth, key=key)
+        got = wpt.decode_image(png)
+        assert np.array_equal(got, want), (color, depth)
+        assert (got[0, :3, 3] == 0).all()
+        if os.path.exists(STB):
+            assert np.array_equal(got, stb_decode(png)), (color, depth)
+
+
+@pytest.mark.parametrize("level", [0, 1, 6, 9])
+def test_png_deflate_block_kinds(level):
+    # Level 0 writes stored blocks, 1 fixed or dynamic Huffman codes, 9
+    # dynamic codes with long matches (repeated regions).
+    rng = np.random.default_rng(level)
+    px = rng.integers(0, 256, (64, 96, 4))
+    px[10:40] = px[10:11]
+    px[:, 50:90] = 17
+    png = encode_png(px, 6, 8, level=level, idat_parts=5)
+    got = wpt.decode_image(png)
+    assert np.array_equal(got, px.astype(np.uint8))
+    if os.path.exists(STB):
+        assert np.array_equal(got, stb_decode(png))
+
+
+@needs_stb
+def test_png_large_texture_matches_stb():
+    rng = np.random.default_rng(11)
+    base = rng.integers(0, 256, (256, 256, 4))
+    px = (base + np.arange(256)[:, None, None]) % 256
+    png = encode_png(px, 6, 8, level=9)
+    assert np.array_equal(wpt.decode_image(png), stb_decode(png))
+
+
+def corrupt(png: bytes, offset: int, value: int) -> bytes:
+    b = bytearray(png)
+    b[offset] = value
+    return bytes(b)
+
+
+def test_png_rejections():
+    samples, _ = image_case(2, 8)
+    png = encode_png(samples, 2, 8, level=0, idat_parts=1)
+    bad = [
+        png[:40],                                       # truncated in the image data
+        png[:8],                                        # signature only
+        b"\xff\xd8\xff\xe0" + b"\x00" * 64,             # JPEG: not decoded here
+        b"GIF89a" + b"\x00" * 64,                       # unknown format
+        encode_png(samples, 2, 8, trns=b"\x00" * 6)[:-12] + b"",  # no IEND
+    ]
+    for data in bad:
+        with pytest.raises(wpt.WptError):
+            wpt.decode_image(data)
+    # An invalid filter type byte (stored block: the first raw byte follows
+    # the IDAT header, the zlib header and the 5-byte stored-block header).
+    idat = png.index(b"IDAT")
+    with pytest.raises(wpt.WptError):
+        wpt.decode_image(corrupt(png, idat + 4 + 2 + 5, 9))
+    # A palette image without a palette.
+    idx = np.zeros((2, 2, 1), np.int64)
+    with pytest.raises(wpt.WptError):
+        wpt.decode_image(encode_png(idx, 3, 8))
+    # CRCs are not checked (stb_image ignores them too).
+    assert np.array_equal(wpt.decode_image(corrupt(png, len(png) - 1, 0x5A)), wpt.decode_image(png))
+
+
+# ---- glTF -------------------------------------------------------------------
+
+def load(tmp_path, manifest):
+    p = tmp_path / "scene.json"
+    p.write_text(json.dumps(manifest))
+    return wpt.HostScene(str(p))
+
+
+def sponza_manifest(file, scale=1.0, top_light=False, ignore_occlusion=False):
+    return {"preset": "sponza", "object": {"source": "gltf", "file": file}, "gltf_scale": scale,
+            "lighting": {"add_sponza_top_light": top_light, "ignore_sponza_light_occlusion": ignore_occlusion}}
+
+
+def arrays(desc):
+    """Everything a glTF import puts into the description, as bytes."""
+    def take(ptr, n, t):
+        return bytes(C.string_at(C.cast(ptr, C.c_void_p), n * C.sizeof(t))) if n else b""
+    out = {
+        "tris": take(desc.mesh_triangles, desc.mesh_triangle_count, wpt.Triangle),
+        "meshes": take(desc.meshes, desc.mesh_count, wpt.Mesh),
+        "instances": take(desc.instances, desc.instance_count, wpt.Instance),
+        "materials": take(desc.materials, desc.material_count, wpt.Material),
+        "ext": take(desc.material_ext, desc.material_count, wpt.MaterialExt),
+        "lights": take(desc.punctual_lights, desc.punctual_light_count, wpt.PunctualLight),
+        "static": take(desc.static_triangles, desc.static_triangle_count, wpt.Triangle),
+    }
+    for i in range(desc.texture_count):
+        t = desc.textures[i]
+        out[f"tex{i}"] = (t.width, t.height, C.string_at(t.rgba, t.width * t.height * 4))
+    return out
+
+
+def expected_worlds():
+    root = trs((1, 2, 3), quat([0, 1, 0], 30), (2, 2, 2))
+    child = root @ np.array([0, 0, -1, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0.5, -0.5, 0.25, 1.0]).reshape(4, 4).T
+    spot = root @ trs((0, 4, 0), quat([1, 0, 0], -90))
+    sun = trs(q=quat([1, 0, 0], -45))
+    bulb = trs((-1, 1, 0))
+    return root, child, spot, sun, bulb
+
+
+def check_instance(inst, world, scale=1.0, offset=(0, 0, 0)):
+    l2w = np.array(inst.transform.local_to_world, np.float64).T  # columns -> matrix
+    w2l = np.array(inst.transform.world_to_local, np.float64).T
+    assert np.allclose(l2w, scale * world[:3, :3], rtol=1e-6, atol=1e-6)
+    assert np.allclose(l2w @ w2l, np.eye(3), atol=2e-6)
+    assert np.allclose(inst.transform.translation, scale * world[:3, 3] + np.asarray(offset), rtol=1e-6, atol=1e-6)
+    assert inst.material_override == wpt.INVALID and inst.acceleration == wpt.ACCEL_BVH
+
+
+def test_gltf_scene_import(tmp_path):
+    doc, binary, pixels = gltf_document(images_dir=tmp_path)
+    host = load(tmp_path, sponza_manifest(write_gltf(tmp_path / "scene.gltf", doc, binary)))
+    d = host.desc()
+    # Three triangle primitives: the quad, the textured triangle and the
+    # 32-bit indexed one (its trailing partial triangle dropped); the line
+    # and non-indexed primitives are skipped.
+    assert d.mesh_count == 3 and d.mesh_triangle_count == 4
+    assert [(d.meshes[i].first_triangle, d.meshes[i].triangle_count) for i in range(3)] == [(0, 2), (2, 1), (3, 1)]
+    quad0, quad1, tri, tri32 = (d.mesh_triangles[i] for i in range(4))
+    assert list(quad0.v1) == [1, 0, -1] and list(quad1.v2) == [-1, 0, 1]
+    assert quad0.flags == wpt.TRI_HAS_NORMALS | wpt.TRI_HAS_TEXCOORDS | wpt.TRI_HAS_VERTEX_TANGENTS
+    assert list(quad0.uv1) == [2, 1] and list(quad0.uv2) == [2, -1]  # v flipped to 1 - v
+    assert list(quad0.vertex_tangent0) == [1, 0, 0] and list(quad0.vertex_tangent_sign) == [1, 1, 1]
+    # Texture axis u runs along +x and v (flipped) along -z on this quad.
+    assert np.allclose(quad0.tangent, [1, 0, 0]) and np.allclose(quad0.bitangent, [0, 0, -1])
+    assert tri.flags == wpt.TRI_HAS_TEXCOORDS | wpt.TRI_HAS_TEXCOORDS1
+    assert np.allclose(tri.uv1_1, [0.2, 0.1]) and list(tri32.v0) == list(tri.v2)
+    for uv, tan, bit in ((("uv0", "uv1", "uv2"), "tangent", "bitangent"),
+                         (("uv1_0", "uv1_1", "uv1_2"), "tangent1", "bitangent1")):
+        # The frame solves e_k = du_k T + dv_k B for both edges.
+        p = [np.array(getattr(tri, k), np.float64) for k in ("v0", "v1", "v2")]
+        t = [np.array(getattr(tri, k), np.float64) for k in uv]
+        duv = np.array([t[1] - t[0], t[2] - t[0]])
+        tb = np.linalg.solve(duv, np.array([p[1] - p[0], p[2] - p[0]]))
+        assert np.allclose(getattr(tri, tan), tb[0] / np.linalg.norm(tb[0]), atol=1e-6)
+        assert np.allclose(getattr(tri, bit), tb[1] / np.linalg.norm(tb[1]), atol=1e-6)
+    assert (quad0.material, tri.material, tri32.material) == (1, 4, 0)
+    # Instances of the default scene (scene 1): root (mesh 0), its child
+    # (both assets of mesh 1); node 5 of scene 0 is not instanced.
+    root, child, spot, sun, bulb = expected_worlds()
+    assert d.instance_count == 3 and [d.instances[i].mesh for i in range(3)] == [0, 1, 2]
+    check_instance(d.instances[0], root)
+    check_instance(d.instances[1], child)
+    check_instance(d.instances[2], child)
+    # Lights in node order: the spot under the root, the sun, the bulb.
+    assert d.punctual_light_count == 3
+    s, dl, p = (d.punctual_lights[i] for i in range(3))
+    assert (s.type, dl.type, p.type) == (wpt.LIGHT_SPOT, wpt.LIGHT_DIRECTIONAL, wpt.LIGHT_POINT)
+    assert np.allclose(s.position, spot[:3, 3], atol=1e-6)
+    assert np.allclose(s.direction, -spot[:3, 2] / np.linalg.norm(spot[:3, 2]), atol=1e-6)
+    assert list(s.intensity) == [20, 20, 20] and s.range == 0
+    assert s.inner_cone_cos == np.float32(np.cos(np.float32(0.2)))
+    assert np.isclose(s.outer_cone_cos, np.cos(0.6), rtol=1e-6)
+    assert np.allclose(dl.direction, -sun[:3, 2], atol=1e-6)
+    assert np.allclose(dl.intensity, np.float32([1.0, 0.9, 0.8]) * np.float32(3.0))
+    assert list(p.position) == [-1, 1, 0] and list(p.intensity) == [8, 4, 2] and p.range == 5
+    assert p.inner_cone_cos == 1.0 and np.isclose(p.outer_cone_cos, np.cos(np.pi / 4), rtol=1e-6)
+    # Images in file order: a data URI, a buffer view and an external file.
+    assert d.texture_count == 3
+    for i, px in enumerate(pixels):
+        t = d.textures[i]
+        got = np.frombuffer(C.string_at(t.rgba, t.width * t.height * 4), np.uint8).reshape(t.height, t.width, 4)
+        assert np.array_equal(got, px)
+
+
+def test_gltf_materials(tmp_path):
+    doc, binary, _ = gltf_document()
+    host = load(tmp_path, sponza_manifest(write_gltf(tmp_path / "scene.gltf", doc, binary)))
+    d = host.desc()
+    assert d.material_count == 6
+    m = [d.materials[i] for i in range(6)]
+    e = [d.material_ext[i] for i in range(6)]
+    # 0: the default Lambertian (gltf_loader.cpp:347).
+    assert m[0].type == wpt.DIFFUSE and np.allclose(m[0].albedo, 0.7) and m[0].ior == 1.0
+    assert e[0].gltf_pbr_workflow == 0 and e[0].albedo_texture.texture == wpt.INVALID
+    # 1: metallic-roughness, masked, with a transformed base colour texture.
+    assert m[1].type == wpt.DIFFUSE and e[1].gltf_pbr_workflow == 1
+    assert np.allclose(m[1].albedo, [0.8, 0.6, 0.4]) and np.isclose(m[1].opacity, 0.9)
+    assert np.isclose(m[1].metallic, 0.25) and np.isclose(m[1].roughness, 0.6) and m[1].ior == 1.5
+    assert m[1].alpha_masked == 1 and np.isclose(m[1].alpha_cutoff, 0.3) and np.isclose(e[1].normal_scale, 0.7)
+    b = e[1].albedo_texture
+    assert (b.texture, b.texcoord, b.has_transform) == (0, 1, 1)
+    assert np.allclose(b.offset, [0.1, 0.2]) and list(b.scale) == [2, 3] and np.isclose(b.rotation, 0.5)
+    assert e[1].metallic_roughness_texture.texture == 1 and e[1].normal_texture.texture == 2
+    assert e[1].metallic_roughness_texture.has_transform == 0
+    # 2: emissive, textured emission on the second set.
+    assert m[2].type == wpt.EMISSIVE and list(m[2].emission) == [4, 3, 2] and e[2].gltf_pbr_workflow == 0
+    assert (e[2].emissive_texture.texture, e[2].emissive_texture.texcoord) == (1, 1)
+    assert m[2].roughness == 1.0 and m[2].metallic == 1.0
+    # 3: transmission -> smooth dielectric with ior and Beer attenuation.
+    assert m[3].type == wpt.DIELECTRIC and np.isclose(m[3].ior, 1.33)
+    assert np.allclose(m[3].attenuation_color, [0.9, 0.5, 0.2]) and m[3].attenuation_distance == 2.5
+    assert np.isclose(m[3].volume_density, 0.4) and e[3].thickness_texture.texture == 0
+    # 4: specular-glossiness.
+    assert m[4].type == wpt.SPECULAR_GLOSSINESS and e[4].specular_glossiness_workflow == 1
+    assert e[4].gltf_pbr_workflow == 0 and np.allclose(m[4].albedo, [0.5, 0.4, 0.3])
+    assert np.isclose(m[4].opacity, 0.8) and np.allclose(m[4].specular, [0.2, 0.3, 0.4])
+    assert np.isclose(m[4].glossiness, 0.6) and e[4].albedo_texture.texture == 2
+    assert (e[4].specular_glossiness_texture.texture, e[4].specular_glossiness_texture.texcoord) == (0, 1)
+    # 5: clearcoat and specular extensions on the metallic-roughness model.
+    assert m[5].type == wpt.DIFFUSE and e[5].gltf_pbr_workflow == 1
+    assert np.isclose(e[5].clearcoat, 0.8) and np.isclose(e[5].clearcoat_roughness, 0.1)
+    assert e[5].clearcoat_texture.texture == 1 and e[5].clearcoat_normal_texture.texture == 2
+    assert e[5].clearcoat_normal_scale == 0.5 and np.isclose(e[5].specular_weight, 0.7)
+    assert np.allclose(e[5].specular_color, [1.0, 0.9, 0.8]) and e[5].specular_weight_texture.texture == 0
+
+
+def test_gltf_binary_and_external_buffers_import_identically(tmp_path):
+    doc, binary, _ = gltf_document()
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    (tmp_path / "c").mkdir()
+    variants = [write_gltf(tmp_path / "a" / "scene.gltf", doc, binary),
+                write_gltf(tmp_path / "b" / "scene.gltf", doc, binary, external_buffer=True),
+                write_glb(tmp_path / "c" / "scene.glb", doc, binary)]
+    hosts = [load(tmp_path, sponza_manifest(v)) for v in variants]
+    got = [arrays(h.desc()) for h in hosts]
+    assert got[0] == got[1] == got[2]
+
+
+def test_sponza_assembly_scale_and_top_light(tmp_path):
+    doc, binary, _ = gltf_document()
+    file = write_gltf(tmp_path / "scene.gltf", doc, binary)
+    plain_host = load(tmp_path, sponza_manifest(file))  # owns the arrays behind the description
+    plain = plain_host.desc()
+    host = load(tmp_path, sponza_manifest(file, scale=0.5, top_light=True, ignore_occlusion=True))
+    d = host.desc()
+    root, child, spot, *_ = expected_worlds()
+    check_instance(d.instances[0], root, scale=0.5)
+    check_instance(d.instances[1], child, scale=0.5)
+    assert np.allclose(d.punctual_lights[0].position, 0.5 * spot[:3, 3], atol=1e-6)
+    assert list(d.punctual_lights[0].direction) == list(plain.punctual_lights[0].direction)
+    # The 8 x 8 quad at y = 9.2 with emission 80 after the file's materials.
+    assert d.static_triangle_count == 2 and d.material_count == 7
+    assert d.materials[6].type == wpt.EMISSIVE and list(d.materials[6].emission) == [80, 80, 80]
+    assert all(d.static_triangles[i].material == 6 for i in range(2))
+    assert d.static_triangles[0].v0[1] == np.float32(9.2)
+    assert d.ignore_direct_light_occlusion == 1 and plain.ignore_direct_light_occlusion == 0
+    assert d.black_background == 0
+
+
+@pytest.mark.parametrize("dielectric", [False, True])
+def test_cornell_room_around_gltf(tmp_path, dielectric):
+    doc, binary, _ = gltf_document()
+    file = write_gltf(tmp_path / "scene.gltf", doc, binary)
+    manifest = {"preset": "cornell", "object": {"source": "gltf", "file": file, "scale": 0.5,
+                                                "translation": [0.0, -0.5, 0.25], "dielectric": dielectric},
+                "room": {"remove_backdrop": True}}
+    host = load(tmp_path, manifest)
+    d = host.desc()
+    root, child, spot, sun, bulb = expected_worlds()
+    # Mesh 0's instance (the "backdrop") is removed; the rest are scaled and moved.
+    assert d.instance_count == 2 and [d.instances[i].mesh for i in range(2)] == [1, 2]
+    check_instance(d.instances[0], child, scale=0.5, offset=(0.0, -0.5, 0.25))
+    assert np.allclose(d.punctual_lights[2].position, 0.5 * bulb[:3, 3] + [0.0, -0.5, 0.25], atol=1e-6)
+    # Five room materials, then the file's (default first).
+    assert d.material_count == 11 and d.black_background == 1
+    assert [d.mesh_triangles[i].material for i in range(4)] == [6, 6, 9, 5]
+    assert d.materials[4].type == wpt.EMISSIVE
+    pbr, pbr_ext = d.materials[6], d.material_ext[6]
+    if dielectric:
+        assert pbr.type == wpt.DIELECTRIC and pbr.ior == np.float32(1.52) and list(pbr.albedo) == [1, 1, 1]
+        assert pbr_ext.dispersion == np.float32(0.02) and pbr_ext.albedo_texture.texture == wpt.INVALID
+        assert list(d.materials[7].emission) == [0, 0, 0]
+        assert d.materials[4].emission[0] == 32  # the crystal light
+    else:
+        assert pbr.type == wpt.DIFFUSE and pbr_ext.albedo_texture.texture == 0 and pbr_ext.dispersion == 0
+        assert d.materials[4].emission[0] == 16
+    assert d.texture_count == 3
+
+
+def test_prism_preset_keeps_its_own_material():
+    # The procedural prism's material is not converted by the dielectric
+    # switch the prism preset sets (scene_host.cpp:251-282).
+    host = wpt.HostScene("prism")
+    d = host.desc()
+    assert d.material_count == 6 and d.materials[5].type == wpt.DIELECTRIC
+    assert np.allclose(d.materials[5].albedo, [0.98, 0.99, 1.0]) and d.material_ext[5].dispersion == np.float32(0.035)
+    assert d.materials[4].emission[0] == 32
+
+
+def mutate(doc, fn):
+    doc = json.loads(json.dumps(doc))
+    fn(doc)
+    return doc
+
+
+BAD = {
+    "accessor_index": lambda d: d["meshes"][0]["primitives"][0]["attributes"].update(POSITION=99),
+    "beyond_view": lambda d: d["accessors"][0].update(count=100),
+    "int_positions": lambda d: d["accessors"][0].update(componentType=5123),
+    "vec2_positions": lambda d: d["accessors"][0].update(type="VEC2"),
+    "float_indices": lambda d: d["accessors"][4].update(componentType=5126),
+    "index_range": lambda d: d["accessors"][0].update(count=2),  # indices 2 and 3 past the positions
+    "spec_gloss": lambda d: d["materials"][3]["extensions"]["KHR_materials_pbrSpecularGlossiness"].update(
+        glossinessFactor=2.0),
+    "clearcoat": lambda d: d["materials"][4]["extensions"]["KHR_materials_clearcoat"].update(clearcoatFactor=-1),
+    "light_type": lambda d: d["extensions"]["KHR_lights_punctual"]["lights"][0].update(type="area"),
+    "light_index": lambda d: d["nodes"][4]["extensions"]["KHR_lights_punctual"].update(light=7),
+    "spot_cone": lambda d: d["extensions"]["KHR_lights_punctual"]["lights"][1]["spot"].update(outerConeAngle=0.1),
+    "cycle": lambda d: d["nodes"][1].update(children=[0]),
+    "jpeg_image": lambda d: d["images"][0].update(uri="data:image/jpeg;base64,/9j/4AAQSkZJRgABAQ=="),
+    "missing_image": lambda d: d["images"][2].update(uri="nowhere.png"),
+    "bad_base64": lambda d: d["images"][0].update(uri="data:image/png;base64,@@@"),
+    "no_json": None,
+}
+
+
+@pytest.mark.parametrize("case", sorted(BAD))
+def test_gltf_rejections(tmp_path, case):
+    doc, binary, _ = gltf_document()
+    if case == "no_json":
+        (tmp_path / "scene.gltf").write_text("{ not json")
+        file = str(tmp_path / "scene.gltf")
+    else:
+        file = write_gltf(tmp_path / "scene.gltf", mutate(doc, BAD[case]), binary)
+    with pytest.raises(wpt.WptError):
+        load(tmp_path, sponza_manifest(file))
+
+
+def test_gltf_without_instances_is_rejected(tmp_path):
+    doc, binary, _ = gltf_document()
+    doc = mutate(doc, lambda d: d.update(scene=0, scenes=[{"nodes": [3]}]))
+    with pytest.raises(wpt.WptError):
+        load(tmp_path, sponza_manifest(write_gltf(tmp_path / "scene.gltf", doc, binary)))

@@ -9,6 +9,7 @@
 #include "checkpoint.hpp"
 #include "config.hpp"
 #include "ledger.hpp"
+#include "image_decode.hpp"
 #include "loaders.hpp"
 #include "render_driver.hpp"
 #include "scene_factory.hpp"
@@ -145,6 +146,19 @@ WPT_API int wpt_host_load_hdr(const char* filename, float* rgb, uint64_t capacit
     if (rgb) {
         if (capacity < pixels.size()) return invalid("HDR output buffer too small");
         std::memcpy(rgb, pixels.data(), sizeof(float) * pixels.size());
+    }
+    return WPT_OK;
+}
+
+WPT_API int wpt_host_decode_image(const uint8_t* data, uint64_t size, uint8_t* rgba, uint64_t capacity,
+    uint32_t* width, uint32_t* height) {
+    if (!data || !width || !height) return invalid("null argument");
+    std::vector<uint8_t> pixels;
+    std::string error;
+    if (!wpt::decodeImage(data, static_cast<size_t>(size), pixels, *width, *height, &error)) return invalid(error);
+    if (rgba) {
+        if (capacity < pixels.size()) return invalid("image output buffer too small");
+        std::memcpy(rgba, pixels.data(), pixels.size());
     }
     return WPT_OK;
 }

@@ -1,0 +1,32 @@
+// Image decoding for glTF textures: PNG files to 8-bit RGBA, rows top to
+// bottom, as the reference's loadTexture asks stb_image for them
+// (src/scene/gltf_loader.cpp:273-305, stbi_load_from_memory with
+// STBI_rgb_alpha).
+//
+// PNG (ISO/IEC 15948): zlib/deflate streams (RFC 1950/1951), the five
+// scanline filters, Adam7 interlacing, every colour type and bit depth.
+// Conversions to RGBA8 are stb_image's: grey g -> (g, g, g, 255) with
+// 1/2/4-bit greys scaled by 255/3/17, grey+alpha -> (g, g, g, a), palette
+// entries with tRNS alphas (255 without), a tRNS colour key turns matching
+// pixels' alpha to 0 (keys of sub-byte greys scaled like the pixels),
+// 16-bit samples keep their high byte.  Chunk CRCs and the zlib Adler-32
+// are not verified, and extra image data after the last row is ignored, as
+// in stb_image.  JPEG and other formats are reported as undecodable.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace wpt {
+
+bool decodeImage(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, uint32_t& width, uint32_t& height,
+    std::string* error);
+
+// zlib stream (RFC 1950 header, deflate body) to bytes; expectedSize is a
+// capacity hint.
+bool inflateZlib(const uint8_t* data, size_t size, std::vector<uint8_t>& out, size_t expectedSize,
+    std::string* error);
+
+}  // namespace wpt

@@ -21,6 +21,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/wpt.h"
@@ -34,6 +35,29 @@ struct ObjScene {
 };
 
 bool loadObjScene(const std::string& filename, ObjScene& out, std::string* error);
+
+// glTF 2.0 (src/scene/gltf_loader.cpp:307-503): one mesh asset per indexed
+// triangle primitive with FLOAT positions, normals, tangents and two
+// texture-coordinate sets (v flipped to 1 - v) and face tangent frames per
+// set; instances and KHR_lights_punctual lights from the default scene's
+// node trees (matrix or TRS, composed in double precision); material 0 the
+// reference's default (Lambertian 0.7) and the file's materials from 1 with
+// the metallic-roughness, specular-glossiness, transmission/volume, ior,
+// specular and clearcoat fields and their texture bindings; images decoded
+// to RGBA8 in file order (texture bindings name images).  Malformed input is
+// an error (the reference exits or throws).
+struct GltfScene {
+    std::vector<std::vector<wpt_triangle>> meshes;
+    std::vector<wpt_instance> instances;  // mesh = index into meshes
+    std::vector<wpt_material> materials;
+    std::vector<wpt_material_ext> materialExt;
+    std::vector<std::vector<uint8_t>> textureData;
+    std::vector<std::pair<uint32_t, uint32_t>> textureSizes;  // width, height
+    std::vector<wpt_punctual_light> punctualLights;
+    uint32_t specularGlossinessMaterials = 0;
+};
+
+bool loadGltfScene(const std::string& filename, GltfScene& out, std::string* error);
 
 bool loadRadianceHdr(const std::string& filename, std::vector<float>& rgb, uint32_t& width, uint32_t& height,
     std::string* error);

@@ -341,6 +341,8 @@ wpt_scene_desc HostScene::desc() const {
     d.material_count = static_cast<uint32_t>(materials.size());
     d.black_background = blackBackground ? 1u : 0u;
     d.ignore_direct_light_occlusion = ignoreDirectLightOcclusion ? 1u : 0u;
+    d.punctual_lights = punctualLights.empty() ? nullptr : punctualLights.data();
+    d.punctual_light_count = static_cast<uint32_t>(punctualLights.size());
     d.textures = textures.empty() ? nullptr : textures.data();
     d.texture_count = static_cast<uint32_t>(textures.size());
     d.material_ext = materialExt.empty() ? nullptr : materialExt.data();
@@ -503,42 +505,144 @@ wpt_transform objectTransform(const SceneConfig& c) {
     return t;
 }
 
-// The Cornell room around an OBJ object (assembleCornellScene's OBJ branch,
-// scene_host.cpp:242-355): one instance per OBJ mesh, object materials after
-// the room's five, optionally turned into dispersive glass.
-bool cornellObjScene(HostScene& s, const SceneConfig& c, std::string* error) {
-    ObjScene obj;
-    if (!loadObjScene(c.objectFilename, obj, error)) return false;
-    std::vector<float> dispersion(obj.materials.size(), 0.0f);
-    if (c.convertObjectMaterialsToDielectric) {
-        for (size_t i = 0; i < obj.materials.size(); ++i) {  // scene_host.cpp:273-282
-            wpt_material& m = obj.materials[i];
-            m.type = WPT_MATERIAL_DIELECTRIC;
-            set3(m.albedo, 1.0f, 1.0f, 1.0f);
-            set3(m.emission, 0.0f, 0.0f, 0.0f);
-            m.roughness = 0.0f;
-            m.ior = 1.52f;
-            dispersion[i] = 0.02f;
+void scaleInstance(wpt_instance& inst, float scale, const float translation[3]) {
+    // Vec3 * s and Vec3 / s (= Vec3 * (1 / s)) on the basis, s * t + offset
+    // on the translation (scene_host.cpp:299-307, 358-366).
+    float inverse = 1.0f / scale;
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 3; ++k) {
+            inst.transform.local_to_world[c][k] = scale * inst.transform.local_to_world[c][k];
+            inst.transform.world_to_local[c][k] = inst.transform.world_to_local[c][k] * inverse;
         }
-    }
-    for (std::vector<wpt_triangle>& mesh : obj.meshes) {
+    for (int k = 0; k < 3; ++k) inst.transform.translation[k] = scale * inst.transform.translation[k] + translation[k];
+}
+
+void adoptTextures(HostScene& s, GltfScene& g) {
+    s.textureData = std::move(g.textureData);
+    s.textures.clear();
+    for (size_t i = 0; i < s.textureData.size(); ++i)
+        s.textures.push_back(wpt_texture{g.textureSizes[i].first, g.textureSizes[i].second, s.textureData[i].data()});
+}
+
+// The Cornell room around its object (assembleCornellScene,
+// scene_host.cpp:242-350): the procedural prism, the meshes of an OBJ file
+// (one instance each) or a glTF scene (its node instances, scaled and
+// moved, without mesh 0's when the backdrop is removed, and its punctual
+// lights).  Object materials follow the room's five, optionally turned into
+// dispersive glass; the prism keeps its own material.
+bool cornellScene(HostScene& s, const SceneConfig& c, std::string* error) {
+    std::vector<wpt_material> objectMaterials;
+    std::vector<wpt_material_ext> objectExt;
+    const float offset[3] = {c.objectTranslation.x, c.objectTranslation.y, c.objectTranslation.z};
+    auto addMesh = [&](std::vector<wpt_triangle>& mesh) {
         uint32_t first = static_cast<uint32_t>(s.meshTriangles.size());
         for (wpt_triangle& t : mesh) {
             t.material = c.objectMaterialOverride >= 0 ? static_cast<uint32_t>(c.objectMaterialOverride) : t.material + 5;
             s.meshTriangles.push_back(t);
         }
         s.meshes.push_back(wpt_mesh{first, static_cast<uint32_t>(mesh.size())});
-        wpt_instance inst{};
-        inst.mesh = static_cast<uint32_t>(s.meshes.size() - 1);
-        inst.material_override = WPT_INVALID_INDEX;
-        inst.acceleration = WPT_ACCEL_BVH;
-        inst.transform = objectTransform(c);
+    };
+    if (c.objectSource == CornellObjectSource::ProceduralPrism) {
+        std::vector<wpt_triangle> prism = causticPrism();
+        addMesh(prism);
+        wpt_material glass = makeMaterial(WPT_MATERIAL_DIELECTRIC, 0.98f, 0.99f, 1.0f);
+        glass.ior = 1.52f;
+        s.materials.push_back(glass);
+    } else if (c.objectSource == CornellObjectSource::Gltf) {
+        GltfScene g;
+        if (!loadGltfScene(c.objectFilename, g, error)) return false;
+        for (std::vector<wpt_triangle>& mesh : g.meshes) addMesh(mesh);
+        for (wpt_instance inst : g.instances) {
+            if (c.removeBackdrop && inst.mesh == 0) continue;
+            scaleInstance(inst, c.objectScale, offset);
+            s.instances.push_back(inst);
+        }
+        for (wpt_punctual_light light : g.punctualLights) {
+            for (int k = 0; k < 3; ++k) light.position[k] = c.objectScale * light.position[k] + offset[k];
+            s.punctualLights.push_back(light);
+        }
+        objectMaterials = std::move(g.materials);
+        objectExt = std::move(g.materialExt);
+        adoptTextures(s, g);
+    } else {
+        ObjScene obj;
+        if (!loadObjScene(c.objectFilename, obj, error)) return false;
+        for (std::vector<wpt_triangle>& mesh : obj.meshes) addMesh(mesh);
+        objectMaterials = std::move(obj.materials);
+        objectExt.assign(objectMaterials.size(), plainMaterialExt());
+    }
+    if (c.convertObjectMaterialsToDielectric) {
+        for (size_t i = 0; i < objectMaterials.size(); ++i) {  // scene_host.cpp:273-282
+            wpt_material& m = objectMaterials[i];
+            m.type = WPT_MATERIAL_DIELECTRIC;
+            set3(m.albedo, 1.0f, 1.0f, 1.0f);
+            set3(m.emission, 0.0f, 0.0f, 0.0f);
+            m.roughness = 0.0f;
+            m.ior = 1.52f;
+            objectExt[i].dispersion = 0.02f;
+            objectExt[i].albedo_texture.texture = WPT_INVALID_INDEX;
+        }
+    }
+    if (c.objectSource != CornellObjectSource::Gltf) {
+        for (uint32_t mesh = 0; mesh < s.meshes.size(); ++mesh) {
+            wpt_instance inst{};
+            inst.mesh = mesh;
+            inst.material_override = WPT_INVALID_INDEX;
+            inst.acceleration = WPT_ACCEL_BVH;
+            inst.transform = objectTransform(c);
+            s.instances.push_back(inst);
+        }
+    }
+    bool prism = c.objectSource == CornellObjectSource::ProceduralPrism;
+    s.materials.insert(s.materials.end(), objectMaterials.begin(), objectMaterials.end());
+    // The light follows the dielectric switch first, then the profile
+    // (scene_host.cpp:322-326).
+    referenceCornellRoom(s, c, c.convertObjectMaterialsToDielectric, c.lightProfile == CornellLightProfile::Prism);
+    s.materialExt.assign(5, plainMaterialExt());
+    if (prism) {
+        // The prism splits photons by channel (scene_host.cpp:251-255).
+        s.materialExt.push_back(plainMaterialExt());
+        s.materialExt.back().dispersion = 0.035f;
+    }
+    s.materialExt.insert(s.materialExt.end(), objectExt.begin(), objectExt.end());
+    return true;
+}
+
+// A glTF scene under the sky (assembleGltfScene, scene_host.cpp:351-385):
+// its instances and punctual lights scaled by the scene scale, and an
+// optional 8 x 8 emissive quad above it.
+bool gltfScene(HostScene& s, const SceneConfig& c, std::string* error) {
+    GltfScene g;
+    if (!loadGltfScene(c.gltfFilename, g, error)) return false;
+    if (g.meshes.empty() || g.instances.empty()) {
+        if (error) *error = "glTF scene contains no renderable mesh instances";
+        return false;
+    }
+    for (const std::vector<wpt_triangle>& mesh : g.meshes) {
+        s.meshes.push_back(wpt_mesh{static_cast<uint32_t>(s.meshTriangles.size()), static_cast<uint32_t>(mesh.size())});
+        s.meshTriangles.insert(s.meshTriangles.end(), mesh.begin(), mesh.end());
+    }
+    const float zero[3] = {0.0f, 0.0f, 0.0f};
+    for (wpt_instance inst : g.instances) {
+        scaleInstance(inst, c.gltfScale, zero);
         s.instances.push_back(inst);
     }
-    s.materials = obj.materials;
-    referenceCornellRoom(s, c, c.convertObjectMaterialsToDielectric, c.lightProfile == CornellLightProfile::Prism);
-    s.materialExt.assign(s.materials.size(), plainMaterialExt());
-    for (size_t i = 0; i < dispersion.size(); ++i) s.materialExt[5 + i].dispersion = dispersion[i];
+    for (wpt_punctual_light light : g.punctualLights) {
+        for (float& v : light.position) v = c.gltfScale * v;
+        s.punctualLights.push_back(light);
+    }
+    s.materials = std::move(g.materials);
+    s.materialExt = std::move(g.materialExt);
+    adoptTextures(s, g);
+    if (c.addSponzaTopLight) {
+        uint32_t light = static_cast<uint32_t>(s.materials.size());
+        s.materials.push_back(makeMaterial(WPT_MATERIAL_EMISSIVE, 1.0f, 1.0f, 1.0f, 80.0f, 80.0f, 80.0f));
+        s.materialExt.push_back(plainMaterialExt());
+        const float a[3] = {-4.0f, 9.2f, -4.0f}, b[3] = {4.0f, 9.2f, -4.0f}, cc[3] = {4.0f, 9.2f, 4.0f},
+                    d[3] = {-4.0f, 9.2f, 4.0f};
+        addQuad(s.staticTriangles, a, b, cc, d, light);
+    }
+    s.ignoreDirectLightOcclusion = c.ignoreSponzaLightOcclusion;
     return true;
 }
 
@@ -598,44 +702,19 @@ bool buildPreset(const SceneConfig& config, HostScene& out, std::string* error) 
     case ScenePreset::RtWeek:
         sphereFieldScene(out, config, false);
         return true;
-    case ScenePreset::Prism: {
-        out.meshTriangles = causticPrism();
-        out.meshes.push_back(wpt_mesh{0, static_cast<uint32_t>(out.meshTriangles.size())});
-        wpt_material prism = makeMaterial(WPT_MATERIAL_DIELECTRIC, 0.98f, 0.99f, 1.0f);
-        prism.ior = 1.52f;
-        out.materials.push_back(prism);
-        for (wpt_triangle& t : out.meshTriangles)
-            t.material = config.objectMaterialOverride >= 0 ? static_cast<uint32_t>(config.objectMaterialOverride) : t.material + 5;
-        wpt_instance inst{};
-        inst.mesh = 0;
-        inst.material_override = WPT_INVALID_INDEX;
-        inst.acceleration = WPT_ACCEL_BVH;
-        inst.transform = makeScaledTransform(config.objectTranslation.x, config.objectTranslation.y,
-            config.objectTranslation.z, config.objectScale);
-        if (config.lightProfile == CornellLightProfile::Prism) rotateY(inst.transform, -20.0f);
-        out.instances.push_back(inst);
-        // The light follows the dielectric switch first, then the profile
-        // (scene_host.cpp:322-326).
-        referenceCornellRoom(out, config, config.convertObjectMaterialsToDielectric,
-            config.lightProfile == CornellLightProfile::Prism);
-        // The prism splits photons by channel (scene_host.cpp:251-255).
-        out.materialExt.assign(out.materials.size(), plainMaterialExt());
-        out.materialExt[5].dispersion = 0.035f;
-        return true;
-    }
     case ScenePreset::Cornell:
     case ScenePreset::Hurricane:
+    case ScenePreset::Prism:
     case ScenePreset::Crystal:
-        if (config.objectSource != CornellObjectSource::Obj) break;
-        return cornellObjScene(out, config, error);
+        return cornellScene(out, config, error);
+    case ScenePreset::Sponza:
+        return gltfScene(out, config, error);
     case ScenePreset::Deer:
         return objScene(out, config, error);
     default:
         break;
     }
-    if (error)
-        *error = std::string("scene preset '") + presetName(config.preset) +
-            "' needs glTF or other external assets, which this renderer does not load";
+    if (error) *error = std::string("unsupported scene preset '") + presetName(config.preset) + "'";
     return false;
 }
 }  // namespace

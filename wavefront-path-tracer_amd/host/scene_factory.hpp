@@ -20,6 +20,7 @@ struct HostScene {
     std::vector<wpt_mesh> meshes;
     std::vector<wpt_instance> instances;
     std::vector<wpt_material> materials;
+    std::vector<wpt_punctual_light> punctualLights;
     bool blackBackground = false;
     bool ignoreDirectLightOcclusion = false;
     // Textured scenes: RGBA8 texels of each texture (textures[i].rgba points

@@ -307,6 +307,7 @@ def library() -> C.CDLL:
         "wpt_host_scene_fingerprint": ([C.c_char_p, C.POINTER(u64)], C.c_int),
         "wpt_renderer_load_framebuffer": ([P, P, u32], C.c_int),
         "wpt_host_load_hdr": ([C.c_char_p, P, u64, C.POINTER(u32), C.POINTER(u32)], C.c_int),
+        "wpt_host_decode_image": ([P, u64, P, u64, C.POINTER(u32), C.POINTER(u32)], C.c_int),
         "wpt_host_render_checkpoint_write": ([C.c_char_p, P, u32, u32, u32, u64], C.c_int),
         "wpt_host_render_checkpoint_read": ([C.c_char_p, u32, u32, u64, P, C.POINTER(u32)], C.c_int),
         "wpt_ledger_create": ([C.POINTER(LedgerJob), P, u32, C.POINTER(P)], C.c_int),
@@ -893,6 +894,17 @@ def load_hdr(filename) -> np.ndarray:
     out = np.zeros((h.value, w.value, 3), np.float32)
     check(library().wpt_host_load_hdr(str(filename).encode(), out.ctypes.data, out.size, C.byref(w), C.byref(h)),
           "load_hdr")
+    return out
+
+
+def decode_image(data: bytes) -> np.ndarray:
+    """An encoded image (PNG) as uint8 (height, width, 4) RGBA."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    w, h = C.c_uint32(), C.c_uint32()
+    check(library().wpt_host_decode_image(buf.ctypes.data, buf.size, None, 0, C.byref(w), C.byref(h)), "decode_image")
+    out = np.zeros((h.value, w.value, 4), np.uint8)
+    check(library().wpt_host_decode_image(buf.ctypes.data, buf.size, out.ctypes.data, out.size, C.byref(w),
+                                          C.byref(h)), "decode_image")
     return out
 
 
