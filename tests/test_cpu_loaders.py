@@ -4,6 +4,7 @@ scanlines, rejections), environment maps attached through manifests, OBJ/MTL
 parsing (fans, negative and partial indices, materials, generated normals)
 and the OBJ presets (crystal room, deer over a ground sphere) assembled as
 src/scene/scene_host.cpp:242-418 does."""
+import ctypes as C
 import json
 import os
 import struct
@@ -149,6 +150,7 @@ map_Kd -s 1 1 1 lamp.jpg
 def write_obj(tmp_path, obj=OBJ, mtl=MTL):
     (tmp_path / "parts.obj").write_text(obj)
     (tmp_path / "parts.mtl").write_text(mtl)
+    (tmp_path / "lamp.jpg").write_bytes(b"\xff\xd8\xff\xe0" + bytes(60))  # JPEG: not decoded, Kd kept
     return str(tmp_path / "parts.obj")
 
 
@@ -193,6 +195,30 @@ def test_obj_preset_over_ground_sphere(tmp_path):
     flat = flat_host.desc()
     assert flat.mesh_count == 0 and flat.flat_triangle_count == 6
     assert list(flat.flat_triangles[0].v1) == [2.5, -0.5, 0.0]
+
+
+def test_obj_png_textures(tmp_path):
+    # obj_loader.cpp:69-104: map_Kd images decoded to RGBA8, one texture per
+    # distinct file shared by the materials naming it; the JPEG keeps Kd.
+    from gltf_fixtures import checker_png
+    png, px = checker_png(4, 9)
+    (tmp_path / "wood.png").write_bytes(png)
+    mtl = MTL + "newmtl red2\nKd 0.5 0.5 0.5\nmap_Kd wood.png\nnewmtl red3\nmap_Kd wood.png\n"
+    mtl = mtl.replace("newmtl red\nKd 0.8 0.1 0.1\n", "newmtl red\nKd 0.8 0.1 0.1\nmap_Kd wood.png\n")
+    write_obj(tmp_path, mtl=mtl)
+    p = tmp_path / "deer.json"
+    p.write_text(json.dumps({"preset": "deer", "object": {"source": "obj", "file": "parts.obj"}}))
+    host = wpt.HostScene(str(p))
+    d = host.desc()
+    assert d.texture_count == 1 and (d.textures[0].width, d.textures[0].height) == (4, 4)
+    got = np.frombuffer(C.string_at(d.textures[0].rgba, 64), np.uint8).reshape(4, 4, 4)
+    assert np.array_equal(got, px)
+    # Materials: ground, OBJ default, red, glow, red2, red3.
+    assert [d.material_ext[i].albedo_texture.texture for i in range(6)] == [wpt.INVALID, wpt.INVALID, 0,
+                                                                             wpt.INVALID, 0, 0]
+    (tmp_path / "wood.png").unlink()
+    with pytest.raises(wpt.WptError, match="texture"):
+        wpt.HostScene(str(p))
 
 
 @pytest.mark.parametrize("bad", ["f 1 2 9\n", "f 1 0 2\n", "f 1/x 2 3\n"])

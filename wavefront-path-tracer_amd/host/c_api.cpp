@@ -43,13 +43,10 @@ WPT_API int wpt_host_scene_create(const char* name, uint32_t seed, uint32_t freq
     std::string n(name);
     if (n.size() > 5 && n.compare(n.size() - 5, 5, ".json") == 0) {
         std::string error;
-        if (!wpt::loadSceneManifest(name, config, &error)) {
-            std::cerr << error << '\n';
-            return WPT_ERROR_INVALID_ARGUMENT;
-        }
+        if (!wpt::loadSceneManifest(name, config, &error)) return invalid(error);
     } else {
         wpt::ScenePreset preset;
-        if (!wpt::parseScenePreset(name, preset)) return WPT_ERROR_INVALID_ARGUMENT;
+        if (!wpt::parseScenePreset(name, preset)) return invalid(std::string("unknown scene preset: ") + name);
         config = wpt::scenePreset(preset);
     }
     if (seed) config.seed = seed;
@@ -58,9 +55,8 @@ WPT_API int wpt_host_scene_create(const char* name, uint32_t seed, uint32_t freq
     s->config = config;
     std::string error;
     if (!wpt::createSceneFromConfig(config, s->scene, &error)) {
-        std::cerr << error << '\n';
         delete s;
-        return WPT_ERROR_UNSUPPORTED;
+        return wpt::setLastError(WPT_ERROR_UNSUPPORTED, error);
     }
     *out = s;
     return WPT_OK;
@@ -72,10 +68,7 @@ WPT_API int wpt_host_scene_fingerprint(const char* name, uint64_t* out) {
     std::string n(name);
     if (n.size() > 5 && n.compare(n.size() - 5, 5, ".json") == 0) {
         std::string error;
-        if (!wpt::loadSceneManifest(name, config, &error)) {
-            std::cerr << error << '\n';
-            return WPT_ERROR_INVALID_ARGUMENT;
-        }
+        if (!wpt::loadSceneManifest(name, config, &error)) return invalid(error);
     } else {
         wpt::ScenePreset preset;
         if (!wpt::parseScenePreset(name, preset)) return WPT_ERROR_INVALID_ARGUMENT;

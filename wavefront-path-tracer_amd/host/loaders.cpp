@@ -1,9 +1,13 @@
 #include "loaders.hpp"
 
+#include "image_decode.hpp"
+
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iostream>
+#include <iterator>
 #include <sstream>
 #include <unordered_map>
 
@@ -183,6 +187,34 @@ bool loadObjScene(const std::string& filename, ObjScene& out, std::string* error
     for (const auto& entry : mtl) {
         out.materials.push_back(objMaterial(entry.second.kd, entry.second.ke, entry.second.ni));
         out.albedoMaps.push_back(entry.second.mapKd);
+    }
+    // Albedo textures, one per distinct file name (obj_loader.cpp:69-104).
+    out.albedoTexture.assign(out.materials.size(), WPT_INVALID_INDEX);
+    std::unordered_map<std::string, uint32_t> textureIndex;
+    for (size_t i = 0; i < out.albedoMaps.size(); ++i) {
+        const std::string& name = out.albedoMaps[i];
+        if (name.empty()) continue;
+        auto known = textureIndex.find(name);
+        if (known != textureIndex.end()) {
+            out.albedoTexture[i] = known->second;
+            continue;
+        }
+        std::string path = directoryOf(filename) + name;
+        std::ifstream image(path, std::ios::binary);
+        if (!image) return fail(error, "failed to load texture " + path);
+        std::vector<uint8_t> encoded((std::istreambuf_iterator<char>(image)), std::istreambuf_iterator<char>());
+        std::vector<uint8_t> rgba;
+        uint32_t w = 0, h = 0;
+        std::string why;
+        if (!decodeImage(encoded.data(), encoded.size(), rgba, w, h, &why)) {
+            std::cerr << "Texture " << path << " not decoded (" << why << "); the material keeps its Kd\n";
+            continue;
+        }
+        uint32_t index = static_cast<uint32_t>(out.textureData.size());
+        out.textureData.push_back(std::move(rgba));
+        out.textureSizes.push_back({w, h});
+        textureIndex.emplace(name, index);
+        out.albedoTexture[i] = index;
     }
 
     // Vertex normals where the file gives none: the normalized sum of the

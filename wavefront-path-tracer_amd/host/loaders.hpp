@@ -9,8 +9,10 @@
 // three are given, else the normalized sum of the unnormalized face normals
 // around the vertex; texture coordinates when all three are given.
 // Polygons are split as fans from their first vertex.  Image textures
-// (map_Kd) are recorded by name but not decoded: this build has no image
-// codec, so such materials keep their Kd.
+// (map_Kd, relative to the OBJ file, shared between materials naming the
+// same file) are decoded to RGBA8 albedo textures; an unreadable file is an
+// error, an image in a format this build does not decode (JPEG) leaves the
+// material with its Kd and a warning.
 //
 // HDR follows stb_image's Radiance decoder, which the reference's
 // loadEnvironmentMap uses (src/scene/environment.cpp:66-82): "#?RADIANCE" or
@@ -32,6 +34,11 @@ struct ObjScene {
     std::vector<std::vector<wpt_triangle>> meshes;
     std::vector<wpt_material> materials;   // [0] default, then the MTL's
     std::vector<std::string> albedoMaps;   // per material; empty: none
+    // Decoded map_Kd images (one per distinct file name) and each
+    // material's texture index (WPT_INVALID_INDEX: none).
+    std::vector<std::vector<uint8_t>> textureData;
+    std::vector<std::pair<uint32_t, uint32_t>> textureSizes;  // width, height
+    std::vector<uint32_t> albedoTexture;
 };
 
 bool loadObjScene(const std::string& filename, ObjScene& out, std::string* error);

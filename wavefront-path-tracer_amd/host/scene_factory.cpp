@@ -517,11 +517,19 @@ void scaleInstance(wpt_instance& inst, float scale, const float translation[3]) 
     for (int k = 0; k < 3; ++k) inst.transform.translation[k] = scale * inst.transform.translation[k] + translation[k];
 }
 
-void adoptTextures(HostScene& s, GltfScene& g) {
-    s.textureData = std::move(g.textureData);
+void adoptTextures(HostScene& s, std::vector<std::vector<uint8_t>>& data,
+    const std::vector<std::pair<uint32_t, uint32_t>>& sizes) {
+    s.textureData = std::move(data);
     s.textures.clear();
     for (size_t i = 0; i < s.textureData.size(); ++i)
-        s.textures.push_back(wpt_texture{g.textureSizes[i].first, g.textureSizes[i].second, s.textureData[i].data()});
+        s.textures.push_back(wpt_texture{sizes[i].first, sizes[i].second, s.textureData[i].data()});
+}
+
+// Material extensions of OBJ materials: their albedo textures.
+std::vector<wpt_material_ext> objMaterialExt(const ObjScene& obj) {
+    std::vector<wpt_material_ext> ext(obj.materials.size(), plainMaterialExt());
+    for (size_t i = 0; i < ext.size(); ++i) ext[i].albedo_texture.texture = obj.albedoTexture[i];
+    return ext;
 }
 
 // The Cornell room around its object (assembleCornellScene,
@@ -563,13 +571,14 @@ bool cornellScene(HostScene& s, const SceneConfig& c, std::string* error) {
         }
         objectMaterials = std::move(g.materials);
         objectExt = std::move(g.materialExt);
-        adoptTextures(s, g);
+        adoptTextures(s, g.textureData, g.textureSizes);
     } else {
         ObjScene obj;
         if (!loadObjScene(c.objectFilename, obj, error)) return false;
         for (std::vector<wpt_triangle>& mesh : obj.meshes) addMesh(mesh);
+        objectExt = objMaterialExt(obj);
         objectMaterials = std::move(obj.materials);
-        objectExt.assign(objectMaterials.size(), plainMaterialExt());
+        adoptTextures(s, obj.textureData, obj.textureSizes);
     }
     if (c.convertObjectMaterialsToDielectric) {
         for (size_t i = 0; i < objectMaterials.size(); ++i) {  // scene_host.cpp:273-282
@@ -633,7 +642,7 @@ bool gltfScene(HostScene& s, const SceneConfig& c, std::string* error) {
     }
     s.materials = std::move(g.materials);
     s.materialExt = std::move(g.materialExt);
-    adoptTextures(s, g);
+    adoptTextures(s, g.textureData, g.textureSizes);
     if (c.addSponzaTopLight) {
         uint32_t light = static_cast<uint32_t>(s.materials.size());
         s.materials.push_back(makeMaterial(WPT_MATERIAL_EMISSIVE, 1.0f, 1.0f, 1.0f, 80.0f, 80.0f, 80.0f));
@@ -655,6 +664,12 @@ bool objScene(HostScene& s, const SceneConfig& c, std::string* error) {
     s.spheres.push_back(sphere(0.0f, -100.5f, 0.0f, 100.0f, 0, WPT_ACCEL_DIRECT));
     s.materials.push_back(makeMaterial(WPT_MATERIAL_DIFFUSE, 0.75f, 0.75f, 0.75f));
     s.materials.insert(s.materials.end(), obj.materials.begin(), obj.materials.end());
+    if (!obj.textureData.empty()) {
+        s.materialExt.assign(1, plainMaterialExt());
+        std::vector<wpt_material_ext> ext = objMaterialExt(obj);
+        s.materialExt.insert(s.materialExt.end(), ext.begin(), ext.end());
+        adoptTextures(s, obj.textureData, obj.textureSizes);
+    }
     std::vector<wpt_triangle> all;
     for (const std::vector<wpt_triangle>& mesh : obj.meshes) all.insert(all.end(), mesh.begin(), mesh.end());
     for (wpt_triangle& t : all) t.material += 1;
