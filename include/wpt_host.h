@@ -64,8 +64,8 @@ WPT_API int wpt_host_load_hdr(const char* filename, float* rgb, uint64_t capacit
 /* Decodes an image file held in memory to 8-bit RGBA, rows top to bottom,
  * as the reference's glTF texture import asks stb_image for it
  * (loadTexture, src/scene/gltf_loader.cpp:273-305, STBI_rgb_alpha).  PNG
- * of every colour type, bit depth and interlacing; other formats (JPEG
- * among them) fail.  With rgba NULL only the size is returned; else
+ * of every colour type, bit depth and interlacing; baseline and progressive
+ * JPEG; other formats fail.  With rgba NULL only the size is returned; else
  * capacity (bytes) must hold width*height*4. */
 WPT_API int wpt_host_decode_image(const uint8_t* data, uint64_t size, uint8_t* rgba, uint64_t capacity,
                                   uint32_t* width, uint32_t* height);

@@ -448,3 +448,143 @@ def test_gltf_without_instances_is_rejected(tmp_path):
     doc = mutate(doc, lambda d: d.update(scene=0, scenes=[{"nodes": [3]}]))
     with pytest.raises(wpt.WptError):
         load(tmp_path, sponza_manifest(write_gltf(tmp_path / "scene.gltf", doc, binary)))
+
+
+# ---- JPEG -------------------------------------------------------------------
+
+def pil_jpeg(px, mode="RGB", **options):
+    import io
+
+    from PIL import Image
+    buf = io.BytesIO()
+    Image.fromarray(np.asarray(px, np.uint8), mode).save(buf, "JPEG", **options)
+    return buf.getvalue()
+
+
+def smooth_image(w, h, channels, seed):
+    # Gradients plus noise: every coefficient band is exercised.
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = [(xx * (3 + c) + yy * (5 - c)) % 256 for c in range(channels)]
+    px = np.stack(base, -1) + rng.integers(-40, 40, (h, w, channels))
+    return np.clip(px, 0, 255).astype(np.uint8)
+
+
+JPEG_CASES = [
+    dict(mode="RGB", subsampling=0, quality=90),
+    dict(mode="RGB", subsampling=1, quality=75),
+    dict(mode="RGB", subsampling=2, quality=50),
+    dict(mode="RGB", subsampling=2, quality=95, progressive=True),
+    dict(mode="RGB", subsampling=0, quality=30, progressive=True),
+    dict(mode="RGB", subsampling=2, quality=80, restart_marker_blocks=3),
+    dict(mode="RGB", subsampling=1, quality=85, progressive=True, restart_marker_rows=1),
+    dict(mode="L", quality=70),
+    dict(mode="L", quality=60, progressive=True),
+    dict(mode="CMYK", quality=80),
+]
+
+
+@needs_stb
+@pytest.mark.parametrize("w,h", [(1, 1), (7, 3), (17, 9), (64, 48), (33, 65)])
+@pytest.mark.parametrize("case", range(len(JPEG_CASES)))
+def test_jpeg_matches_stb(w, h, case):
+    options = dict(JPEG_CASES[case])
+    mode = options.pop("mode")
+    px = smooth_image(w, h, {"RGB": 3, "L": 1, "CMYK": 4}[mode], seed=case * 7 + w)
+    if mode == "L":
+        px = px[..., 0]
+    data = pil_jpeg(px, mode, **options)
+    got = wpt.decode_image(data)
+    want = stb_decode(data)
+    assert want is not None and got.shape == (h, w, 4)
+    assert np.array_equal(got, want), f"max |diff| {np.abs(got.astype(int) - want).max()}"
+
+
+def test_jpeg_decodes_close_to_the_source():
+    # Independent of stb: a high-quality 4:4:4 encode decodes near its source.
+    px = smooth_image(40, 24, 3, seed=5)
+    got = wpt.decode_image(pil_jpeg(px, "RGB", quality=100, subsampling=0))
+    assert got.shape == (24, 40, 4) and (got[..., 3] == 255).all()
+    assert np.abs(got[..., :3].astype(int) - px).mean() < 2.0
+
+
+REFERENCE_JPEGS = sorted(
+    os.path.join(d, f) for d in ("/root/reference/assets", "/root/reference/assets/sponza") if os.path.isdir(d)
+    for f in os.listdir(d) if f.lower().endswith(".jpg"))[:12]
+
+
+@needs_stb
+@pytest.mark.skipif(not REFERENCE_JPEGS, reason="reference assets not present")
+def test_reference_asset_jpegs_match_stb():
+    # The reference's own textures (the deer's UV grid, Sponza's maps).
+    for path in REFERENCE_JPEGS:
+        data = open(path, "rb").read()
+        assert np.array_equal(wpt.decode_image(data), stb_decode(data)), path
+
+
+def test_jpeg_rejections():
+    good = pil_jpeg(smooth_image(16, 16, 3, 1), "RGB", quality=80)
+    bad = [good[:2], good[:200], b"\xff\xd8\xff\xc3" + bytes(40)]
+    for data in bad:
+        with pytest.raises(wpt.WptError):
+            wpt.decode_image(data)
+
+
+# ---- the reference's own assets (only where its tree is present) ------------
+
+REF_ASSETS = "/root/reference/assets"
+
+
+def stb_hdr(data: bytes):
+    lib = C.CDLL(STB)
+    lib.wpt_ref_stbi_rgbf.restype = C.c_longlong
+    lib.wpt_ref_stbi_rgbf.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_longlong, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]
+    w, h = C.c_int(), C.c_int()
+    assert lib.wpt_ref_stbi_rgbf(data, len(data), None, 0, C.byref(w), C.byref(h)) > 0
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    lib.wpt_ref_stbi_rgbf(data, len(data), out.ctypes.data, out.size, C.byref(w), C.byref(h))
+    return out
+
+
+@needs_stb
+@pytest.mark.skipif(not os.path.exists(f"{REF_ASSETS}/sponza/spruit_sunrise_4k.hdr"), reason="reference assets absent")
+def test_reference_hdr_panorama_matches_stb():
+    path = f"{REF_ASSETS}/sponza/spruit_sunrise_4k.hdr"
+    assert np.array_equal(wpt.load_hdr(path), stb_hdr(open(path, "rb").read()))
+
+
+def gltf_counts(path):
+    """(triangle primitives, triangles, images) counted from the file's JSON."""
+    import struct
+    raw = open(path, "rb").read()
+    if raw[:4] == b"glTF":
+        n = struct.unpack_from("<I", raw, 12)[0]
+        doc = json.loads(raw[20:20 + n])
+    else:
+        doc = json.loads(raw)
+    prims = [p for m in doc.get("meshes", []) for p in m["primitives"]
+             if p.get("mode", 4) == 4 and "POSITION" in p["attributes"] and "indices" in p]
+    tris = sum(doc["accessors"][p["indices"]]["count"] // 3 for p in prims)
+    return len(prims), tris, len(doc.get("images", []))
+
+
+@needs_stb
+@pytest.mark.skipif(not os.path.exists(f"{REF_ASSETS}/scenes/sponza.json"), reason="reference assets absent")
+@pytest.mark.parametrize("name,asset", [("sponza", "sponza/Sponza.gltf"), ("hurricane", "GlassHurricaneCandleHolder.glb")])
+def test_reference_gltf_scenes_import(name, asset):
+    # The reference's manifests with its assets: every triangle primitive,
+    # triangle and image arrives, and every texture is stb_image's decode.
+    host = wpt.HostScene(f"{REF_ASSETS}/scenes/{name}.json")
+    d = host.desc()
+    prims, tris, images = gltf_counts(f"{REF_ASSETS}/{asset}")
+    assert d.mesh_count == prims and d.mesh_triangle_count == tris and d.texture_count == images
+    assert d.instance_count > 0
+    doc_dir = os.path.dirname(f"{REF_ASSETS}/{asset}")
+    if asset.endswith(".gltf"):
+        doc = json.load(open(f"{REF_ASSETS}/{asset}"))
+        for i, image in enumerate(doc["images"][:6]):
+            data = open(os.path.join(doc_dir, image["uri"]), "rb").read()
+            t = d.textures[i]
+            got = np.frombuffer(C.string_at(t.rgba, t.width * t.height * 4), np.uint8).reshape(t.height, t.width, 4)
+            assert np.array_equal(got, stb_decode(data)), image["uri"]

@@ -1,8 +1,10 @@
-// PNG decoding to RGBA8 (see image_decode.hpp for the conversions).
+// PNG and JPEG decoding to RGBA8 (see image_decode.hpp for the conversions).
 #include "image_decode.hpp"
 
+#include <algorithm>
 #include <array>
 #include <cstring>
+#include <iterator>
 
 namespace wpt {
 namespace {
@@ -470,14 +472,706 @@ bool decodePng(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, uin
 
 }  // namespace
 
+// ---- JPEG -------------------------------------------------------------------
+// Baseline and progressive Huffman JPEG (ITU T.81), 8-bit samples, 1, 3 or
+// 4 components, with stb_image's arithmetic after entropy decoding: its
+// integer IDCT (the IJG "islow" factorisation, constants scaled by 4096),
+// its centred "fancy" 2x upsampling (triangle filters with the same
+// rounding, nearest neighbour for other ratios), its reduced-precision
+// YCbCr->RGB (constants x4096 << 8, the green Cb term truncated to 16
+// fractional bits) and its CMYK/YCCK handling under the Adobe marker.
+namespace {
+
+constexpr uint8_t kDezigzag[64 + 15] = {0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48,
+    41, 34, 27, 20, 13, 6, 7, 14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52,
+    45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63,
+    // runs past the end of a corrupt block land on the last coefficient
+    63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63};
+
+struct JpegHuffman {
+    bool present = false;
+    uint16_t fast[1 << 9] = {};  // (length << 8 | symbol) + 1 for codes of <= 9 bits
+    int maxcode[18] = {};        // largest code of each length, left-aligned to 16 bits (exclusive bound)
+    int valptr[17] = {};
+    int mincode[17] = {};
+    uint8_t symbols[256] = {};
+
+    bool build(const uint8_t counts[16], const uint8_t* values, int total) {
+        std::memcpy(symbols, values, static_cast<size_t>(total));
+        std::fill(std::begin(fast), std::end(fast), 0);
+        int code = 0, k = 0;
+        for (int len = 1; len <= 16; ++len) {
+            valptr[len] = k;
+            mincode[len] = code;
+            for (int i = 0; i < counts[len - 1]; ++i, ++k, ++code) {
+                if (len <= 9) {
+                    int first = code << (9 - len);
+                    for (int f = 0; f < (1 << (9 - len)); ++f)
+                        fast[first + f] = static_cast<uint16_t>(((len << 8) | symbols[k]) + 1);
+                }
+            }
+            if (code - 1 >= (1 << len) && counts[len - 1]) return false;  // over-subscribed
+            maxcode[len] = code << (16 - len);
+            code <<= 1;
+        }
+        maxcode[17] = 0x7fffffff;
+        present = true;
+        return true;
+    }
+};
+
+struct JpegComponent {
+    int id = 0, h = 1, v = 1, tq = 0, hd = 0, ha = 0;
+    int x = 0, y = 0, w2 = 0, h2 = 0;  // sample extent; padded plane size
+    int dcPred = 0;
+    std::vector<uint8_t> plane;
+    std::vector<int16_t> coeff;  // progressive: 64 per block, natural order
+    int coeffW = 0;              // blocks per coefficient row
+};
+
+class JpegDecoder {
+public:
+    JpegDecoder(const uint8_t* data, size_t size) : d_(data), n_(size) {}
+
+    bool decode(std::vector<uint8_t>& rgba, uint32_t& width, uint32_t& height, std::string* error);
+
+private:
+    // --- byte and marker access
+    int byte() { return pos_ < n_ ? d_[pos_++] : -1; }
+    int be16() {
+        int a = byte(), b = byte();
+        return (a < 0 || b < 0) ? -1 : (a << 8) | b;
+    }
+    // --- entropy-coded bits (MSB first, 0xFF00 stuffing; a marker ends the data)
+    void fill() {
+        while (bits_ <= 24) {
+            int c = 0;
+            if (!hitMarker_) {
+                c = byte();
+                if (c < 0) {
+                    c = 0;
+                } else if (c == 0xff) {
+                    int next = byte();
+                    while (next == 0xff) next = byte();  // fill bytes
+                    if (next != 0) {
+                        marker_ = next;
+                        hitMarker_ = true;
+                        c = 0;
+                    }
+                }
+            }
+            buffer_ |= static_cast<uint32_t>(c) << (24 - bits_);
+            bits_ += 8;
+        }
+    }
+    int bit() {
+        fill();
+        int b = static_cast<int>(buffer_ >> 31);
+        buffer_ <<= 1;
+        --bits_;
+        return b;
+    }
+    int getBits(int n) {
+        if (n == 0) return 0;
+        fill();
+        int v = static_cast<int>(buffer_ >> (32 - n));
+        buffer_ <<= n;
+        bits_ -= n;
+        return v;
+    }
+    int receiveExtend(int n) {
+        if (n == 0) return 0;
+        int v = getBits(n);
+        return v < (1 << (n - 1)) ? v - (1 << n) + 1 : v;
+    }
+    int huffDecode(const JpegHuffman& h) {
+        fill();
+        uint16_t entry = h.fast[buffer_ >> (32 - 9)];
+        if (entry) {
+            int len = (entry - 1) >> 8;
+            buffer_ <<= len;
+            bits_ -= len;
+            return (entry - 1) & 0xff;
+        }
+        uint32_t top = buffer_ >> 16;
+        int len = 10;
+        while (len <= 16 && static_cast<int>(top) >= h.maxcode[len]) ++len;
+        if (len > 16) return -1;
+        int code = static_cast<int>(top >> (16 - len));
+        int index = h.valptr[len] + code - h.mincode[len];
+        if (index < 0 || index >= 256) return -1;
+        buffer_ <<= len;
+        bits_ -= len;
+        return h.symbols[index];
+    }
+    void resetEntropy() {
+        buffer_ = 0;
+        bits_ = 0;
+        hitMarker_ = false;
+        marker_ = -1;
+        eobRun_ = 0;
+        for (JpegComponent& c : comps_) c.dcPred = 0;
+    }
+
+    bool readTables(int marker, std::string* error);
+    bool readFrame(std::string* error);
+    bool readScan(std::string* error);
+    bool decodeScan(std::string* error);
+    bool decodeBlock(JpegComponent& c, int16_t* out, std::string* error);
+    bool progressiveBlock(JpegComponent& c, int16_t* data, std::string* error);
+    void idct(const int16_t* in, uint8_t* out, int stride);
+    void output(std::vector<uint8_t>& rgba);
+
+    const uint8_t* d_;
+    size_t n_, pos_ = 0;
+    uint32_t buffer_ = 0;
+    int bits_ = 0;
+    bool hitMarker_ = false;
+    int marker_ = -1;
+    JpegHuffman dc_[4], ac_[4];
+    uint16_t quant_[4][64] = {};  // natural order
+    bool quantPresent_[4] = {};
+    std::vector<JpegComponent> comps_;
+    int width_ = 0, height_ = 0, hmax_ = 1, vmax_ = 1, mcux_ = 0, mcuy_ = 0;
+    bool progressive_ = false, jfif_ = false;
+    int adobeTransform_ = -1;
+    int restartInterval_ = 0;
+    std::vector<int> scan_;  // component indices of the current scan
+    int ss_ = 0, se_ = 63, ah_ = 0, al_ = 0, eobRun_ = 0;
+};
+
+bool JpegDecoder::readTables(int marker, std::string* error) {
+    int length = be16();
+    if (length < 2 || pos_ + static_cast<size_t>(length - 2) > n_) return fail(error, "JPEG: truncated segment");
+    size_t end = pos_ + static_cast<size_t>(length - 2);
+    if (marker == 0xdb) {  // DQT
+        while (pos_ < end) {
+            int pq = byte(), precision = pq >> 4, table = pq & 15;
+            if (precision > 1 || table > 3) return fail(error, "JPEG: bad DQT");
+            for (int i = 0; i < 64; ++i)
+                quant_[table][kDezigzag[i]] = static_cast<uint16_t>(precision ? be16() : byte());
+            quantPresent_[table] = true;
+        }
+    } else if (marker == 0xc4) {  // DHT
+        while (pos_ < end) {
+            int tc = byte(), kind = tc >> 4, table = tc & 15;
+            if (kind > 1 || table > 3) return fail(error, "JPEG: bad DHT");
+            uint8_t counts[16];
+            int total = 0;
+            for (int i = 0; i < 16; ++i) {
+                counts[i] = static_cast<uint8_t>(byte());
+                total += counts[i];
+            }
+            if (total > 256 || pos_ + static_cast<size_t>(total) > end) return fail(error, "JPEG: bad DHT");
+            JpegHuffman& h = kind ? ac_[table] : dc_[table];
+            if (!h.build(counts, d_ + pos_, total)) return fail(error, "JPEG: bad code lengths");
+            pos_ += static_cast<size_t>(total);
+        }
+    } else if (marker == 0xdd) {  // DRI
+        if (length != 4) return fail(error, "JPEG: bad DRI length");
+        restartInterval_ = be16();
+    } else if (marker == 0xe0 && length >= 7) {  // APP0: JFIF
+        if (std::memcmp(d_ + pos_, "JFIF\0", 5) == 0) jfif_ = true;
+    } else if (marker == 0xee && length >= 14) {  // APP14: Adobe
+        if (std::memcmp(d_ + pos_, "Adobe", 6) == 0) adobeTransform_ = d_[pos_ + 11];
+    }
+    pos_ = end;
+    return true;
+}
+
+bool JpegDecoder::readFrame(std::string* error) {
+    int length = be16();
+    int precision = byte();
+    height_ = be16();
+    width_ = be16();
+    int count = byte();
+    if (length < 8 || precision != 8) return fail(error, "JPEG: only 8-bit samples are supported");
+    if (height_ <= 0 || width_ <= 0) return fail(error, "JPEG: zero-sized image");
+    if (height_ > (1 << 24) || width_ > (1 << 24)) return fail(error, "JPEG: image too large");
+    if (count != 1 && count != 3 && count != 4) return fail(error, "JPEG: bad component count");
+    if (length != 8 + 3 * count) return fail(error, "JPEG: bad SOF length");
+    comps_.resize(static_cast<size_t>(count));
+    for (JpegComponent& c : comps_) {
+        c.id = byte();
+        int hv = byte();
+        c.h = hv >> 4;
+        c.v = hv & 15;
+        c.tq = byte();
+        if (c.h < 1 || c.h > 4 || c.v < 1 || c.v > 4 || c.tq < 0 || c.tq > 3) return fail(error, "JPEG: bad component");
+        hmax_ = std::max(hmax_, c.h);
+        vmax_ = std::max(vmax_, c.v);
+    }
+    for (const JpegComponent& c : comps_)  // stb_image's check on sampling ratios
+        if (hmax_ % c.h != 0 || vmax_ % c.v != 0) return fail(error, "JPEG: bad sampling factors");
+    mcux_ = (width_ + hmax_ * 8 - 1) / (hmax_ * 8);
+    mcuy_ = (height_ + vmax_ * 8 - 1) / (vmax_ * 8);
+    for (JpegComponent& c : comps_) {
+        c.x = (width_ * c.h + hmax_ - 1) / hmax_;
+        c.y = (height_ * c.v + vmax_ - 1) / vmax_;
+        c.w2 = mcux_ * c.h * 8;
+        c.h2 = mcuy_ * c.v * 8;
+        c.plane.assign(static_cast<size_t>(c.w2) * c.h2, 0);
+        if (progressive_) {
+            c.coeffW = c.w2 / 8;
+            c.coeff.assign(static_cast<size_t>(c.w2) * c.h2, 0);
+        }
+    }
+    return true;
+}
+
+bool JpegDecoder::readScan(std::string* error) {
+    int length = be16();
+    int count = byte();
+    if (count < 1 || count > 4 || count > static_cast<int>(comps_.size()) || length != 6 + 2 * count)
+        return fail(error, "JPEG: bad SOS");
+    scan_.clear();
+    for (int i = 0; i < count; ++i) {
+        int id = byte(), tables = byte();
+        int which = -1;
+        for (size_t k = 0; k < comps_.size(); ++k)
+            if (comps_[k].id == id) which = static_cast<int>(k);
+        if (which < 0) return fail(error, "JPEG: scan names an unknown component");
+        comps_[which].hd = tables >> 4;
+        comps_[which].ha = tables & 15;
+        if (comps_[which].hd > 3 || comps_[which].ha > 3) return fail(error, "JPEG: bad table selector");
+        scan_.push_back(which);
+    }
+    ss_ = byte();
+    se_ = byte();
+    int a = byte();
+    ah_ = a >> 4;
+    al_ = a & 15;
+    if (progressive_) {
+        if (ss_ > 63 || se_ > 63 || ss_ > se_ || ah_ > 13 || al_ > 13) return fail(error, "JPEG: bad progressive scan");
+        if (ss_ > 0 && count != 1) return fail(error, "JPEG: interleaved AC scan");
+        if (ss_ == 0 && se_ != 0) return fail(error, "JPEG: DC and AC in one progressive scan");
+    } else {
+        if (ss_ != 0 || ah_ != 0 || al_ != 0) return fail(error, "JPEG: bad baseline scan");
+        se_ = 63;
+    }
+    return true;
+}
+
+bool JpegDecoder::decodeBlock(JpegComponent& c, int16_t* data, std::string* error) {
+    const JpegHuffman& dc = dc_[c.hd];
+    const JpegHuffman& ac = ac_[c.ha];
+    if (!dc.present || !ac.present) return fail(error, "JPEG: missing Huffman table");
+    const uint16_t* q = quant_[c.tq];
+    std::memset(data, 0, 64 * sizeof(int16_t));
+    int t = huffDecode(dc);
+    if (t < 0 || t > 15) return fail(error, "JPEG: bad Huffman code");
+    c.dcPred += receiveExtend(t);
+    data[0] = static_cast<int16_t>(c.dcPred * q[0]);
+    for (int k = 1; k < 64;) {
+        int rs = huffDecode(ac);
+        if (rs < 0) return fail(error, "JPEG: bad Huffman code");
+        int s = rs & 15, r = rs >> 4;
+        if (s == 0) {
+            if (rs != 0xf0) break;  // end of block
+            k += 16;
+        } else {
+            k += r;
+            int zig = kDezigzag[k++];
+            data[zig] = static_cast<int16_t>(receiveExtend(s) * q[zig]);
+        }
+    }
+    return true;
+}
+
+// One block of a progressive scan (T.81 G.1.2): DC first/refine, AC
+// first/refine with end-of-band runs; coefficients stay unquantized.
+bool JpegDecoder::progressiveBlock(JpegComponent& c, int16_t* data, std::string* error) {
+    if (ss_ == 0) {
+        if (ah_ == 0) {
+            if (!dc_[c.hd].present) return fail(error, "JPEG: missing Huffman table");
+            int t = huffDecode(dc_[c.hd]);
+            if (t < 0 || t > 15) return fail(error, "JPEG: bad Huffman code");
+            c.dcPred += receiveExtend(t);
+            data[0] = static_cast<int16_t>(c.dcPred * (1 << al_));
+        } else if (bit()) {
+            data[0] = static_cast<int16_t>(data[0] + (1 << al_));
+        }
+        return true;
+    }
+    const JpegHuffman& ac = ac_[c.ha];
+    if (!ac.present) return fail(error, "JPEG: missing Huffman table");
+    if (ah_ == 0) {
+        if (eobRun_) {
+            --eobRun_;
+            return true;
+        }
+        for (int k = ss_; k <= se_;) {
+            int rs = huffDecode(ac);
+            if (rs < 0) return fail(error, "JPEG: bad Huffman code");
+            int s = rs & 15, r = rs >> 4;
+            if (s == 0) {
+                if (r < 15) {
+                    eobRun_ = (1 << r) - 1 + getBits(r);
+                    break;
+                }
+                k += 16;
+            } else {
+                k += r;
+                int zig = kDezigzag[k++];
+                data[zig] = static_cast<int16_t>(receiveExtend(s) * (1 << al_));
+            }
+        }
+        return true;
+    }
+    // AC refinement: one correction bit per nonzero coefficient passed,
+    // new coefficients of magnitude 1 << Al placed after r zero ones.
+    const int16_t p1 = static_cast<int16_t>(1 << al_);
+    auto refine = [&](int16_t& coef) {
+        if (bit() && (coef & p1) == 0) coef = static_cast<int16_t>(coef > 0 ? coef + p1 : coef - p1);
+    };
+    int k = ss_;
+    if (eobRun_ == 0) {
+        for (; k <= se_;) {
+            int rs = huffDecode(ac);
+            if (rs < 0) return fail(error, "JPEG: bad Huffman code");
+            int s = rs & 15, r = rs >> 4;
+            int value = 0;
+            if (s == 0) {
+                if (r < 15) {
+                    eobRun_ = (1 << r) + getBits(r);
+                    break;
+                }
+            } else {
+                if (s != 1) return fail(error, "JPEG: bad Huffman code");
+                value = bit() ? p1 : -p1;
+            }
+            while (k <= se_) {
+                int16_t& coef = data[kDezigzag[k++]];
+                if (coef != 0) {
+                    refine(coef);
+                } else {
+                    if (r == 0) {
+                        if (value) coef = static_cast<int16_t>(value);
+                        break;
+                    }
+                    --r;
+                }
+            }
+        }
+    }
+    if (eobRun_ > 0) {
+        for (; k <= se_; ++k) {
+            int16_t& coef = data[kDezigzag[k]];
+            if (coef != 0) refine(coef);
+        }
+        --eobRun_;
+    }
+    return true;
+}
+
+bool JpegDecoder::decodeScan(std::string* error) {
+    resetEntropy();
+    int todo = restartInterval_ ? restartInterval_ : 0x7fffffff;
+    auto restart = [&]() -> bool {  // true: continue decoding
+        if (--todo > 0) return true;
+        fill();
+        if (!(hitMarker_ && marker_ >= 0xd0 && marker_ <= 0xd7)) return false;
+        resetEntropy();
+        todo = restartInterval_ ? restartInterval_ : 0x7fffffff;
+        return true;
+    };
+    int16_t block[64];
+    if (scan_.size() == 1) {
+        // Non-interleaved: the component's own blocks in raster order.
+        JpegComponent& c = comps_[scan_[0]];
+        int bw = (c.x + 7) / 8, bh = (c.y + 7) / 8;
+        for (int by = 0; by < bh; ++by)
+            for (int bx = 0; bx < bw; ++bx) {
+                if (progressive_) {
+                    if (!progressiveBlock(c, &c.coeff[(static_cast<size_t>(by) * c.coeffW + bx) * 64], error)) return false;
+                } else {
+                    if (!decodeBlock(c, block, error)) return false;
+                    idct(block, &c.plane[static_cast<size_t>(by) * 8 * c.w2 + bx * 8], c.w2);
+                }
+                if (!restart()) return true;
+            }
+        return true;
+    }
+    for (int my = 0; my < mcuy_; ++my)
+        for (int mx = 0; mx < mcux_; ++mx) {
+            for (int which : scan_) {
+                JpegComponent& c = comps_[which];
+                for (int v = 0; v < c.v; ++v)
+                    for (int h = 0; h < c.h; ++h) {
+                        int bx = mx * c.h + h, by = my * c.v + v;
+                        if (progressive_) {
+                            if (!progressiveBlock(c, &c.coeff[(static_cast<size_t>(by) * c.coeffW + bx) * 64], error))
+                                return false;
+                        } else {
+                            if (!decodeBlock(c, block, error)) return false;
+                            idct(block, &c.plane[static_cast<size_t>(by) * 8 * c.w2 + bx * 8], c.w2);
+                        }
+                    }
+            }
+            if (!restart()) return true;
+        }
+    return true;
+}
+
+inline uint8_t clampByte(int x) { return static_cast<uint8_t>(x < 0 ? 0 : (x > 255 ? 255 : x)); }
+
+// The IJG islow IDCT as stb_image computes it: columns keep two extra bits
+// ((x + 512) >> 10), rows remove 17 bits with rounding and the +128 level
+// shift folded in.
+void JpegDecoder::idct(const int16_t* in, uint8_t* out, int stride) {
+    auto f2f = [](double x) { return static_cast<int>(x * 4096 + 0.5); };
+    static const int c0 = f2f(0.5411961f), c1 = f2f(-1.847759065f), c2 = f2f(0.765366865f), c3 = f2f(1.175875602f),
+                     c4 = f2f(0.298631336f), c5 = f2f(2.053119869f), c6 = f2f(3.072711026f), c7 = f2f(1.501321110f),
+                     c8 = f2f(-0.899976223f), c9 = f2f(-2.562915447f), c10 = f2f(-1.961570560f),
+                     c11 = f2f(-0.390180644f);
+    // One 1-D pass over s[0..7]: even part (x0..x3) and odd part (t0..t3).
+    auto pass = [&](const int s[8], int x[4], int t[4]) {
+        int p1 = (s[2] + s[6]) * c0;
+        int e2 = p1 + s[6] * c1, e3 = p1 + s[2] * c2;
+        int e0 = (s[0] + s[4]) * 4096, e1 = (s[0] - s[4]) * 4096;
+        x[0] = e0 + e3;
+        x[3] = e0 - e3;
+        x[1] = e1 + e2;
+        x[2] = e1 - e2;
+        int o0 = s[7], o1 = s[5], o2 = s[3], o3 = s[1];
+        int q3 = o0 + o2, q4 = o1 + o3, q1 = o0 + o3, q2 = o1 + o2;
+        int q5 = (q3 + q4) * c3;
+        o0 *= c4;
+        o1 *= c5;
+        o2 *= c6;
+        o3 *= c7;
+        q1 = q5 + q1 * c8;
+        q2 = q5 + q2 * c9;
+        q3 *= c10;
+        q4 *= c11;
+        t[3] = o3 + q1 + q4;
+        t[2] = o2 + q2 + q3;
+        t[1] = o1 + q2 + q4;
+        t[0] = o0 + q1 + q3;
+    };
+    int tmp[64];
+    for (int col = 0; col < 8; ++col) {
+        int s[8], x[4], t[4];
+        for (int k = 0; k < 8; ++k) s[k] = in[k * 8 + col];
+        pass(s, x, t);
+        for (int& v : x) v += 512;
+        tmp[0 * 8 + col] = (x[0] + t[3]) >> 10;
+        tmp[7 * 8 + col] = (x[0] - t[3]) >> 10;
+        tmp[1 * 8 + col] = (x[1] + t[2]) >> 10;
+        tmp[6 * 8 + col] = (x[1] - t[2]) >> 10;
+        tmp[2 * 8 + col] = (x[2] + t[1]) >> 10;
+        tmp[5 * 8 + col] = (x[2] - t[1]) >> 10;
+        tmp[3 * 8 + col] = (x[3] + t[0]) >> 10;
+        tmp[4 * 8 + col] = (x[3] - t[0]) >> 10;
+    }
+    for (int row = 0; row < 8; ++row) {
+        int x[4], t[4];
+        pass(&tmp[row * 8], x, t);
+        for (int& v : x) v += 65536 + (128 << 17);
+        uint8_t* o = out + static_cast<size_t>(row) * stride;
+        o[0] = clampByte((x[0] + t[3]) >> 17);
+        o[7] = clampByte((x[0] - t[3]) >> 17);
+        o[1] = clampByte((x[1] + t[2]) >> 17);
+        o[6] = clampByte((x[1] - t[2]) >> 17);
+        o[2] = clampByte((x[2] + t[1]) >> 17);
+        o[5] = clampByte((x[2] - t[1]) >> 17);
+        o[3] = clampByte((x[3] + t[0]) >> 17);
+        o[4] = clampByte((x[3] - t[0]) >> 17);
+    }
+}
+
+void JpegDecoder::output(std::vector<uint8_t>& rgba) {
+    const int n = static_cast<int>(comps_.size());
+    struct Resample {
+        int hs, vs, ystep, wLores, ypos;
+        const uint8_t* line0;
+        const uint8_t* line1;
+        std::vector<uint8_t> buffer;
+    };
+    std::vector<Resample> rs(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) {
+        Resample& r = rs[k];
+        r.hs = hmax_ / comps_[k].h;
+        r.vs = vmax_ / comps_[k].v;
+        r.ystep = r.vs >> 1;
+        r.wLores = (width_ + r.hs - 1) / r.hs;
+        r.ypos = 0;
+        r.line0 = r.line1 = comps_[k].plane.data();
+        r.buffer.assign(static_cast<size_t>(width_) + 3 + 8, 0);
+    }
+    bool isRgb = n == 3 && ((comps_[0].id == 'R' && comps_[1].id == 'G' && comps_[2].id == 'B') ||
+                            (adobeTransform_ == 0 && !jfif_));
+    rgba.assign(static_cast<size_t>(width_) * height_ * 4, 0);
+    const uint8_t* row[4] = {};
+    for (int j = 0; j < height_; ++j) {
+        for (int k = 0; k < n; ++k) {
+            Resample& r = rs[k];
+            bool bottom = r.ystep >= (r.vs >> 1);
+            const uint8_t* nearRow = bottom ? r.line1 : r.line0;
+            const uint8_t* farRow = bottom ? r.line0 : r.line1;
+            uint8_t* out = r.buffer.data();
+            int w = r.wLores;
+            if (r.hs == 1 && r.vs == 1) {
+                row[k] = nearRow;
+            } else if (r.hs == 1 && r.vs == 2) {
+                for (int i = 0; i < w; ++i) out[i] = static_cast<uint8_t>((3 * nearRow[i] + farRow[i] + 2) >> 2);
+                row[k] = out;
+            } else if (r.hs == 2 && r.vs == 1) {
+                if (w == 1) {
+                    out[0] = out[1] = nearRow[0];
+                } else {
+                    out[0] = nearRow[0];
+                    out[1] = static_cast<uint8_t>((nearRow[0] * 3 + nearRow[1] + 2) >> 2);
+                    int i = 1;
+                    for (; i < w - 1; ++i) {
+                        int m = 3 * nearRow[i] + 2;
+                        out[i * 2] = static_cast<uint8_t>((m + nearRow[i - 1]) >> 2);
+                        out[i * 2 + 1] = static_cast<uint8_t>((m + nearRow[i + 1]) >> 2);
+                    }
+                    out[i * 2] = static_cast<uint8_t>((nearRow[w - 2] * 3 + nearRow[w - 1] + 2) >> 2);
+                    out[i * 2 + 1] = nearRow[w - 1];
+                }
+                row[k] = out;
+            } else if (r.hs == 2 && r.vs == 2) {
+                if (w == 1) {
+                    out[0] = out[1] = static_cast<uint8_t>((3 * nearRow[0] + farRow[0] + 2) >> 2);
+                } else {
+                    int t1 = 3 * nearRow[0] + farRow[0];
+                    out[0] = static_cast<uint8_t>((t1 + 2) >> 2);
+                    for (int i = 1; i < w; ++i) {
+                        int t0 = t1;
+                        t1 = 3 * nearRow[i] + farRow[i];
+                        out[i * 2 - 1] = static_cast<uint8_t>((3 * t0 + t1 + 8) >> 4);
+                        out[i * 2] = static_cast<uint8_t>((3 * t1 + t0 + 8) >> 4);
+                    }
+                    out[w * 2 - 1] = static_cast<uint8_t>((t1 + 2) >> 2);
+                }
+                row[k] = out;
+            } else {
+                for (int i = 0; i < w; ++i)
+                    for (int h = 0; h < r.hs; ++h) out[i * r.hs + h] = nearRow[i];
+                row[k] = out;
+            }
+            if (++r.ystep >= r.vs) {
+                r.ystep = 0;
+                r.line0 = r.line1;
+                if (++r.ypos < comps_[k].y) r.line1 += comps_[k].w2;
+            }
+        }
+        uint8_t* o = &rgba[static_cast<size_t>(j) * width_ * 4];
+        auto ycc = [](int y, int cb, int cr, uint8_t* px) {
+            auto fixed = [](float x) { return static_cast<int>(x * 4096.0f + 0.5f) << 8; };
+            int yf = (y << 20) + (1 << 19);
+            cr -= 128;
+            cb -= 128;
+            int r = yf + cr * fixed(1.40200f);
+            int g = yf + (cr * -fixed(0.71414f)) + static_cast<int>(static_cast<unsigned>(cb * -fixed(0.34414f)) & 0xffff0000u);
+            int b = yf + cb * fixed(1.77200f);
+            px[0] = clampByte(r >> 20);
+            px[1] = clampByte(g >> 20);
+            px[2] = clampByte(b >> 20);
+            px[3] = 255;
+        };
+        auto blinn = [](int x, int y) {
+            unsigned t = static_cast<unsigned>(x * y + 128);
+            return static_cast<uint8_t>((t + (t >> 8)) >> 8);
+        };
+        for (int i = 0; i < width_; ++i, o += 4) {
+            if (n == 1) {
+                o[0] = o[1] = o[2] = row[0][i];
+                o[3] = 255;
+            } else if (n == 3) {
+                if (isRgb) {
+                    o[0] = row[0][i];
+                    o[1] = row[1][i];
+                    o[2] = row[2][i];
+                    o[3] = 255;
+                } else {
+                    ycc(row[0][i], row[1][i], row[2][i], o);
+                }
+            } else if (adobeTransform_ == 0) {  // CMYK
+                int m = row[3][i];
+                o[0] = blinn(row[0][i], m);
+                o[1] = blinn(row[1][i], m);
+                o[2] = blinn(row[2][i], m);
+                o[3] = 255;
+            } else if (adobeTransform_ == 2) {  // YCCK
+                ycc(row[0][i], row[1][i], row[2][i], o);
+                int m = row[3][i];
+                o[0] = blinn(255 - o[0], m);
+                o[1] = blinn(255 - o[1], m);
+                o[2] = blinn(255 - o[2], m);
+            } else {  // YCbCr and an ignored fourth channel
+                ycc(row[0][i], row[1][i], row[2][i], o);
+            }
+        }
+    }
+}
+
+bool JpegDecoder::decode(std::vector<uint8_t>& rgba, uint32_t& width, uint32_t& height, std::string* error) {
+    if (byte() != 0xff || byte() != 0xd8) return fail(error, "JPEG: missing SOI");
+    bool sawFrame = false, sawScan = false;
+    int marker = -1;
+    for (;;) {
+        if (marker < 0) {  // next marker: skip to 0xFF, then past fill bytes
+            int c = byte();
+            while (c >= 0 && c != 0xff) c = byte();
+            while (c == 0xff) c = byte();
+            if (c < 0) break;
+            marker = c;
+        }
+        int m = marker;
+        marker = -1;
+        if (m == 0xd9) break;  // EOI
+        if (m >= 0xd0 && m <= 0xd7) continue;  // stray restart marker
+        if (m == 0xc0 || m == 0xc1 || m == 0xc2) {
+            if (sawFrame) return fail(error, "JPEG: more than one frame");
+            progressive_ = m == 0xc2;
+            if (!readFrame(error)) return false;
+            sawFrame = true;
+        } else if ((m >= 0xc3 && m <= 0xcf && m != 0xc4 && m != 0xc8 && m != 0xcc)) {
+            return fail(error, "JPEG: lossless, hierarchical and arithmetic-coded JPEG are not supported");
+        } else if (m == 0xda) {
+            if (!sawFrame) return fail(error, "JPEG: scan before frame");
+            if (!readScan(error) || !decodeScan(error)) return false;
+            sawScan = true;
+            if (hitMarker_) marker = marker_;
+        } else if (m == 0xdc) {
+            return fail(error, "JPEG: DNL is not supported");
+        } else {
+            if (!readTables(m, error)) return false;
+        }
+    }
+    if (!sawFrame || !sawScan) return fail(error, "JPEG: no image data");
+    if (progressive_) {
+        for (JpegComponent& c : comps_) {
+            int bw = (c.x + 7) / 8, bh = (c.y + 7) / 8;
+            for (int by = 0; by < bh; ++by)
+                for (int bx = 0; bx < bw; ++bx) {
+                    int16_t* data = &c.coeff[(static_cast<size_t>(by) * c.coeffW + bx) * 64];
+                    for (int i = 0; i < 64; ++i) data[i] = static_cast<int16_t>(data[i] * quant_[c.tq][i]);
+                    idct(data, &c.plane[static_cast<size_t>(by) * 8 * c.w2 + bx * 8], c.w2);
+                }
+        }
+    }
+    output(rgba);
+    width = static_cast<uint32_t>(width_);
+    height = static_cast<uint32_t>(height_);
+    return true;
+}
+
+}  // namespace
+
 bool decodeImage(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, uint32_t& width, uint32_t& height,
     std::string* error) {
     static const uint8_t pngSignature[8] = {137, 80, 78, 71, 13, 10, 26, 10};
     if (size >= 8 && std::memcmp(data, pngSignature, 8) == 0)
         return decodePng(data, size, rgba, width, height, error);
-    if (size >= 3 && data[0] == 0xff && data[1] == 0xd8 && data[2] == 0xff)
-        return fail(error, "JPEG images are not decoded by this build; convert the texture to PNG");
-    return fail(error, "unknown image format (PNG expected)");
+    if (size >= 3 && data[0] == 0xff && data[1] == 0xd8 && data[2] == 0xff) {
+        JpegDecoder jpeg(data, size);
+        return jpeg.decode(rgba, width, height, error);
+    }
+    return fail(error, "unknown image format (PNG or JPEG expected)");
 }
 
 }  // namespace wpt

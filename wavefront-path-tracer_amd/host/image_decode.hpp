@@ -1,4 +1,4 @@
-// Image decoding for glTF textures: PNG files to 8-bit RGBA, rows top to
+// Image decoding for glTF and OBJ textures: PNG and JPEG files to 8-bit RGBA, rows top to
 // bottom, as the reference's loadTexture asks stb_image for them
 // (src/scene/gltf_loader.cpp:273-305, stbi_load_from_memory with
 // STBI_rgb_alpha).
@@ -11,7 +11,13 @@
 // pixels' alpha to 0 (keys of sub-byte greys scaled like the pixels),
 // 16-bit samples keep their high byte.  Chunk CRCs and the zlib Adler-32
 // are not verified, and extra image data after the last row is ignored, as
-// in stb_image.  JPEG and other formats are reported as undecodable.
+// in stb_image.
+//
+// JPEG (ITU T.81): baseline and progressive Huffman coding, 8-bit samples,
+// greyscale, YCbCr/RGB and CMYK/YCCK, any sampling factors, restart
+// intervals; the pixels are stb_image's: its integer IDCT, its centred 2x
+// upsampling filters and its fixed-point YCbCr conversion.  Lossless,
+// hierarchical, arithmetic-coded and 12-bit JPEG fail, as in stb_image.
 #pragma once
 
 #include <cstddef>
