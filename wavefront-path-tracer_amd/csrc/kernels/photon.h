@@ -92,9 +92,32 @@ WPT_HD Vec3 gatherCaustics(const PhotonGridView& g, const Vec3& position, const 
         lo[a] = c[a] - cellRadius > omin ? c[a] - cellRadius : omin;
         hi[a] = c[a] + cellRadius < omax ? c[a] + cellRadius : omax;
     }
-    for (int z = lo[2]; z <= hi[2]; ++z)
-        for (int y = lo[1]; y <= hi[1]; ++y)
+    // Cells whose box lies beyond the search radius hold no photon that
+    // passes the distance test below, so skipping them leaves the photons
+    // kept, and their order, unchanged.  The boxes are widened by 1% of a
+    // cell plus a few ulps of the grid's coordinates, and the radius by a
+    // relative 1e-4, so that rounding never skips a cell it should visit.
+    const float res = static_cast<float>(g.resolution);
+    const float mins[3] = {g.minimum.x, g.minimum.y, g.minimum.z};
+    const float sizes[3] = {extent.x / res, extent.y / res, extent.z / res};
+    const float query[3] = {position.x, position.y, position.z};
+    float margin[3];
+    for (int a = 0; a < 3; ++a) margin[a] = 0.01f * sizes[a] + 4e-6f * (fabsf(mins[a]) + fabsf(sizes[a] * res));
+    const float cullSquared = searchRadiusSquared * 1.0001f;
+    auto gap = [&](int a, int i) {
+        float lower = mins[a] + sizes[a] * static_cast<float>(i) - margin[a];
+        float upper = mins[a] + sizes[a] * static_cast<float>(i + 1) + margin[a];
+        return query[a] < lower ? lower - query[a] : (query[a] > upper ? query[a] - upper : 0.0f);
+    };
+    for (int z = lo[2]; z <= hi[2]; ++z) {
+        float gz = gap(2, z);
+        for (int y = lo[1]; y <= hi[1]; ++y) {
+            float gy = gap(1, y);
+            float gyz = gy * gy + gz * gz;
+            if (gyz > cullSquared) continue;
             for (int x = lo[0]; x <= hi[0]; ++x) {
+                float gx = gap(0, x);
+                if (gx * gx + gyz > cullSquared) continue;
                 uint32_t cell = (static_cast<uint32_t>(z) * g.resolution + static_cast<uint32_t>(y)) * g.resolution +
                     static_cast<uint32_t>(x);
                 const CellRange range = g.cells[cell];
@@ -125,6 +148,8 @@ WPT_HD Vec3 gatherCaustics(const PhotonGridView& g, const Vec3& position, const 
                     }
                 }
             }
+        }
+    }
     if (count == 0) return Vec3(0.0f, 0.0f, 0.0f);
     float gatherSquared = searchRadiusSquared;
     if (count >= kMinimum) {
