@@ -588,3 +588,51 @@ def test_reference_gltf_scenes_import(name, asset):
             t = d.textures[i]
             got = np.frombuffer(C.string_at(t.rgba, t.width * t.height * 4), np.uint8).reshape(t.height, t.width, 4)
             assert np.array_equal(got, stb_decode(data)), image["uri"]
+
+
+@pytest.mark.parametrize("kind", ["png", "jpeg", "progressive"])
+def test_corrupt_images_fail_cleanly(kind):
+    # Mutated and truncated files either decode to a full image or raise;
+    # none may crash the process or return a short buffer.
+    rng = np.random.default_rng({"png": 1, "jpeg": 2, "progressive": 3}[kind])
+    if kind == "png":
+        base = encode_png(rng.integers(0, 256, (20, 17, 4)), 6, 8, level=9)
+    else:
+        base = pil_jpeg(smooth_image(41, 33, 3, 4), "RGB", quality=80, subsampling=2,
+                        progressive=kind == "progressive")
+    for it in range(300):
+        data = bytearray(base)
+        if it % 4 == 3:
+            data = data[:int(rng.integers(1, len(data)))]
+        for _ in range(int(rng.integers(1, 8))):
+            at = int(rng.integers(0, len(data)))
+            data[at] = int(rng.integers(0, 256)) if it % 2 else data[at] ^ (1 << int(rng.integers(0, 8)))
+        try:
+            img = wpt.decode_image(bytes(data))
+        except wpt.WptError:
+            continue
+        assert img.ndim == 3 and img.shape[2] == 4 and img.size > 0
+
+
+def test_corrupt_gltf_fails_cleanly(tmp_path):
+    # Random indices, counts, offsets and strides in accessors, buffer
+    # views, primitives, nodes and textures: each file imports or raises.
+    doc, binary, _ = gltf_document()
+    rng = np.random.default_rng(17)
+    fields = [("accessors", ["bufferView", "count", "byteOffset", "componentType"]),
+              ("bufferViews", ["byteOffset", "byteLength", "byteStride", "buffer"]),
+              ("nodes", ["mesh", "children"]), ("textures", ["source"]), ("images", ["bufferView"])]
+    for it in range(120):
+        d = json.loads(json.dumps(doc))
+        for _ in range(int(rng.integers(1, 4))):
+            key, names = fields[int(rng.integers(0, len(fields)))]
+            item = d[key][int(rng.integers(0, len(d[key])))]
+            name = names[int(rng.integers(0, len(names)))]
+            value = int(rng.choice([-1, 0, 1, 2, 3, 7, 64, 255, 5126, 1 << 20, 1 << 31]))
+            item[name] = [value] if name == "children" else value
+        file = write_gltf(tmp_path / f"s{it}.gltf", d, binary)
+        try:
+            host = load(tmp_path, sponza_manifest(file))
+        except wpt.WptError:
+            continue
+        assert host.desc().mesh_count > 0

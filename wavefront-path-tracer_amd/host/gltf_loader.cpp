@@ -109,7 +109,20 @@ double numberOr(const Json& object, const std::string& key, double fallback) {
 
 int intOr(const Json& object, const std::string& key, int fallback) {
     const Json& v = member(object, key);
-    return v.isNumber() ? static_cast<int>(v.number()) : fallback;
+    if (!v.isNumber()) return fallback;
+    double x = v.number();
+    return (x >= -2147483648.0 && x <= 2147483647.0) ? static_cast<int>(x) : -1;  // out of range: invalid index
+}
+
+// A byte count, offset or element count: absent -> 0; negative, fractional
+// or beyond 2^48 is an error.
+size_t sizeField(const Json& object, const std::string& key) {
+    const Json& v = member(object, key);
+    if (!v.isNumber()) return 0;
+    double x = v.number();
+    if (!(x >= 0.0 && x <= 281474976710656.0) || x != static_cast<double>(static_cast<uint64_t>(x)))
+        throw GltfError("glTF: bad " + key);
+    return static_cast<size_t>(x);
 }
 
 std::vector<double> numbers(const Json& object, const std::string& key) {
@@ -180,15 +193,17 @@ AccessorView accessor(const Document& doc, int index) {
     int size = componentSize(out.componentType);
     if (out.components == 0 || size == 0) throw GltfError("glTF: bad accessor type");
     size_t elementBytes = static_cast<size_t>(out.components) * size;
-    size_t byteStride = static_cast<size_t>(intOr(view, "byteStride", 0));
+    size_t byteStride = sizeField(view, "byteStride");
     out.stride = byteStride == 0 ? elementBytes : byteStride;
     if (out.stride % size != 0) throw GltfError("glTF: bad byteStride");
-    out.count = static_cast<size_t>(numberOr(a, "count", 0.0));
-    size_t viewOffset = static_cast<size_t>(numberOr(view, "byteOffset", 0.0));
-    size_t viewLength = static_cast<size_t>(numberOr(view, "byteLength", 0.0));
-    size_t offset = static_cast<size_t>(numberOr(a, "byteOffset", 0.0));
+    out.count = sizeField(a, "count");
+    size_t viewOffset = sizeField(view, "byteOffset");
+    size_t viewLength = sizeField(view, "byteLength");
+    size_t offset = sizeField(a, "byteOffset");
     const std::vector<uint8_t>& data = doc.buffers[buffer];
     if (viewOffset + viewLength > data.size()) throw GltfError("glTF: bufferView beyond its buffer");
+    if (out.count > viewLength || out.stride > viewLength || offset > viewLength)  // bounds each term before the product below
+        throw GltfError("glTF: accessor beyond its bufferView");
     if (out.count > 0 && offset + out.stride * (out.count - 1) + elementBytes > viewLength)
         throw GltfError("glTF: accessor beyond its bufferView");
     out.base = data.data() + viewOffset + offset;
@@ -591,7 +606,7 @@ void parseDocument(const std::string& filename, Document& doc) {
         } else {
             throw GltfError("glTF: buffer without data");
         }
-        size_t byteLength = static_cast<size_t>(numberOr(buffers[i], "byteLength", 0.0));
+        size_t byteLength = sizeField(buffers[i], "byteLength");
         if (data.size() < byteLength) throw GltfError("glTF: buffer shorter than its byteLength");
         doc.buffers.push_back(std::move(data));
     }
@@ -629,8 +644,8 @@ bool loadGltfScene(const std::string& filename, GltfScene& out, std::string* err
                 loadUri(uri, directory, encoded);
             } else if (view >= 0 && static_cast<size_t>(view) < views.size()) {
                 int buffer = intOr(views[view], "buffer", -1);
-                size_t offset = static_cast<size_t>(numberOr(views[view], "byteOffset", 0.0));
-                size_t length = static_cast<size_t>(numberOr(views[view], "byteLength", 0.0));
+                size_t offset = sizeField(views[view], "byteOffset");
+                size_t length = sizeField(views[view], "byteLength");
                 if (buffer < 0 || static_cast<size_t>(buffer) >= doc.buffers.size() ||
                     offset + length > doc.buffers[buffer].size())
                     throw GltfError("glTF: bad image bufferView");

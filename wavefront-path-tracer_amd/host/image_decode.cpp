@@ -14,6 +14,9 @@ bool fail(std::string* error, const std::string& message) {
     return false;
 }
 
+// Largest decoded image: 2^28 pixels (1 GiB of RGBA8, a 16384^2 texture).
+constexpr uint64_t kMaxPixels = 1ull << 28;
+
 // LSB-first bit reader over a byte range; reads past the end return zero
 // bits and are detected by overrun().
 class BitReader {
@@ -165,7 +168,7 @@ bool dynamicTables(BitReader& in, Huffman& lit, Huffman& dist, std::string* erro
     for (int i = 0; i < hclen; ++i) codeLengths[order[i]] = static_cast<uint8_t>(in.take(3));
     Huffman lengthCode;
     if (!lengthCode.build(codeLengths, 19)) return fail(error, "PNG: bad code lengths");
-    uint8_t lengths[286 + 32] = {};
+    uint8_t lengths[288 + 32] = {};  // HLIT <= 288, HDIST <= 32
     int n = 0;
     while (n < hlit + hdist) {
         int sym = lengthCode.decode(in);
@@ -379,7 +382,9 @@ bool decodePng(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, uin
             p.depth = body[8];
             p.color = body[9];
             if (p.width == 0 || p.height == 0) return fail(error, "PNG: zero-sized image");
-            if (p.width > (1u << 24) || p.height > (1u << 24)) return fail(error, "PNG: image too large");
+            if (p.width > (1u << 24) || p.height > (1u << 24) ||
+                static_cast<uint64_t>(p.width) * p.height > kMaxPixels)
+                return fail(error, "PNG: image too large");
             bool depthOk = false;
             switch (p.color) {
             case 0: depthOk = p.depth == 1 || p.depth == 2 || p.depth == 4 || p.depth == 8 || p.depth == 16; p.channels = 1; break;
@@ -687,7 +692,8 @@ bool JpegDecoder::readFrame(std::string* error) {
     int count = byte();
     if (length < 8 || precision != 8) return fail(error, "JPEG: only 8-bit samples are supported");
     if (height_ <= 0 || width_ <= 0) return fail(error, "JPEG: zero-sized image");
-    if (height_ > (1 << 24) || width_ > (1 << 24)) return fail(error, "JPEG: image too large");
+    if (height_ > (1 << 24) || width_ > (1 << 24) || static_cast<uint64_t>(width_) * height_ > kMaxPixels)
+        return fail(error, "JPEG: image too large");
     if (count != 1 && count != 3 && count != 4) return fail(error, "JPEG: bad component count");
     if (length != 8 + 3 * count) return fail(error, "JPEG: bad SOF length");
     comps_.resize(static_cast<size_t>(count));
@@ -914,28 +920,29 @@ bool JpegDecoder::decodeScan(std::string* error) {
 }
 
 inline uint8_t clampByte(int x) { return static_cast<uint8_t>(x < 0 ? 0 : (x > 255 ? 255 : x)); }
+inline uint8_t clampWide(int64_t x) { return static_cast<uint8_t>(x < 0 ? 0 : (x > 255 ? 255 : x)); }
 
 // The IJG islow IDCT as stb_image computes it: columns keep two extra bits
 // ((x + 512) >> 10), rows remove 17 bits with rounding and the +128 level
 // shift folded in.
 void JpegDecoder::idct(const int16_t* in, uint8_t* out, int stride) {
-    auto f2f = [](double x) { return static_cast<int>(x * 4096 + 0.5); };
-    static const int c0 = f2f(0.5411961f), c1 = f2f(-1.847759065f), c2 = f2f(0.765366865f), c3 = f2f(1.175875602f),
+    auto f2f = [](double x) { return static_cast<int64_t>(static_cast<int>(x * 4096 + 0.5)); };
+    static const int64_t c0 = f2f(0.5411961f), c1 = f2f(-1.847759065f), c2 = f2f(0.765366865f), c3 = f2f(1.175875602f),
                      c4 = f2f(0.298631336f), c5 = f2f(2.053119869f), c6 = f2f(3.072711026f), c7 = f2f(1.501321110f),
                      c8 = f2f(-0.899976223f), c9 = f2f(-2.562915447f), c10 = f2f(-1.961570560f),
                      c11 = f2f(-0.390180644f);
     // One 1-D pass over s[0..7]: even part (x0..x3) and odd part (t0..t3).
-    auto pass = [&](const int s[8], int x[4], int t[4]) {
-        int p1 = (s[2] + s[6]) * c0;
-        int e2 = p1 + s[6] * c1, e3 = p1 + s[2] * c2;
-        int e0 = (s[0] + s[4]) * 4096, e1 = (s[0] - s[4]) * 4096;
+    auto pass = [&](const int64_t s[8], int64_t x[4], int64_t t[4]) {
+        int64_t p1 = (s[2] + s[6]) * c0;
+        int64_t e2 = p1 + s[6] * c1, e3 = p1 + s[2] * c2;
+        int64_t e0 = (s[0] + s[4]) * 4096, e1 = (s[0] - s[4]) * 4096;
         x[0] = e0 + e3;
         x[3] = e0 - e3;
         x[1] = e1 + e2;
         x[2] = e1 - e2;
-        int o0 = s[7], o1 = s[5], o2 = s[3], o3 = s[1];
-        int q3 = o0 + o2, q4 = o1 + o3, q1 = o0 + o3, q2 = o1 + o2;
-        int q5 = (q3 + q4) * c3;
+        int64_t o0 = s[7], o1 = s[5], o2 = s[3], o3 = s[1];
+        int64_t q3 = o0 + o2, q4 = o1 + o3, q1 = o0 + o3, q2 = o1 + o2;
+        int64_t q5 = (q3 + q4) * c3;
         o0 *= c4;
         o1 *= c5;
         o2 *= c6;
@@ -949,12 +956,12 @@ void JpegDecoder::idct(const int16_t* in, uint8_t* out, int stride) {
         t[1] = o1 + q2 + q4;
         t[0] = o0 + q1 + q3;
     };
-    int tmp[64];
+    int64_t tmp[64];
     for (int col = 0; col < 8; ++col) {
-        int s[8], x[4], t[4];
+        int64_t s[8], x[4], t[4];
         for (int k = 0; k < 8; ++k) s[k] = in[k * 8 + col];
         pass(s, x, t);
-        for (int& v : x) v += 512;
+        for (int64_t& v : x) v += 512;
         tmp[0 * 8 + col] = (x[0] + t[3]) >> 10;
         tmp[7 * 8 + col] = (x[0] - t[3]) >> 10;
         tmp[1 * 8 + col] = (x[1] + t[2]) >> 10;
@@ -965,18 +972,18 @@ void JpegDecoder::idct(const int16_t* in, uint8_t* out, int stride) {
         tmp[4 * 8 + col] = (x[3] - t[0]) >> 10;
     }
     for (int row = 0; row < 8; ++row) {
-        int x[4], t[4];
+        int64_t x[4], t[4];
         pass(&tmp[row * 8], x, t);
-        for (int& v : x) v += 65536 + (128 << 17);
+        for (int64_t& v : x) v += 65536 + (128 << 17);
         uint8_t* o = out + static_cast<size_t>(row) * stride;
-        o[0] = clampByte((x[0] + t[3]) >> 17);
-        o[7] = clampByte((x[0] - t[3]) >> 17);
-        o[1] = clampByte((x[1] + t[2]) >> 17);
-        o[6] = clampByte((x[1] - t[2]) >> 17);
-        o[2] = clampByte((x[2] + t[1]) >> 17);
-        o[5] = clampByte((x[2] - t[1]) >> 17);
-        o[3] = clampByte((x[3] + t[0]) >> 17);
-        o[4] = clampByte((x[3] - t[0]) >> 17);
+        o[0] = clampWide((x[0] + t[3]) >> 17);
+        o[7] = clampWide((x[0] - t[3]) >> 17);
+        o[1] = clampWide((x[1] + t[2]) >> 17);
+        o[6] = clampWide((x[1] - t[2]) >> 17);
+        o[2] = clampWide((x[2] + t[1]) >> 17);
+        o[5] = clampWide((x[2] - t[1]) >> 17);
+        o[3] = clampWide((x[3] + t[0]) >> 17);
+        o[4] = clampWide((x[3] - t[0]) >> 17);
     }
 }
 
@@ -1162,8 +1169,21 @@ bool JpegDecoder::decode(std::vector<uint8_t>& rgba, uint32_t& width, uint32_t& 
 
 }  // namespace
 
+bool decodeImageUnchecked(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, uint32_t& width,
+    uint32_t& height, std::string* error);
+
 bool decodeImage(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, uint32_t& width, uint32_t& height,
     std::string* error) {
+    try {
+        return decodeImageUnchecked(data, size, rgba, width, height, error);
+    } catch (const std::exception& e) {  // allocation failure on a corrupt header
+        rgba.clear();
+        return fail(error, std::string("image decoding failed: ") + e.what());
+    }
+}
+
+bool decodeImageUnchecked(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, uint32_t& width,
+    uint32_t& height, std::string* error) {
     static const uint8_t pngSignature[8] = {137, 80, 78, 71, 13, 10, 26, 10};
     if (size >= 8 && std::memcmp(data, pngSignature, 8) == 0)
         return decodePng(data, size, rgba, width, height, error);
