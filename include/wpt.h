@@ -549,7 +549,9 @@ WPT_API int wpt_diff_get_layout(const wpt_diff* diff, wpt_diff_layout* layout);
  * parameters at d_params (device); writes the mean image to d_image
  * (device) and keeps every path's radiance for the adjoint. */
 WPT_API int wpt_diff_forward(wpt_diff* diff, const float* d_params, uint32_t sample_start, float* d_image);
-/* Path-replay adjoint of the last forward (same d_params, unchanged since):
+/* Path-replay adjoint of the last forward (same d_params, unchanged since;
+ * an adjoint after a wpt_diff_adam step on those parameters is rejected until
+ * a new forward has run):
  * d_param_grad (device, parameter_count floats) = dLoss/dparams given
  * d_image_grad = dLoss/dimage (device). */
 WPT_API int wpt_diff_adjoint(wpt_diff* diff, const float* d_params, const float* d_image_grad, float* d_param_grad);

@@ -636,3 +636,127 @@ def test_corrupt_gltf_fails_cleanly(tmp_path):
         except wpt.WptError:
             continue
         assert host.desc().mesh_count > 0
+
+
+# ---- decoder hardening: regressions and a sanitizer run ---------------------
+
+def jpeg_segments(data: bytes):
+    """(offset, marker, payload) of every marker segment before the scan."""
+    out, at = [], 2
+    while at + 4 <= len(data) and data[at] == 0xFF:
+        marker = data[at + 1]
+        length = (data[at + 2] << 8) | data[at + 3]
+        out.append((at, marker, data[at + 4:at + 2 + length]))
+        if marker == 0xDA:
+            break
+        at += 2 + length
+    return out
+
+
+def oversubscribed_dht(codes=200) -> bytes:
+    # Table class 0 id 0 with `codes` codes of length 1: only two exist.
+    counts = bytes([codes] + [0] * 15)
+    body = bytes([0x00]) + counts + bytes(range(codes))
+    return b"\xff\xc4" + (len(body) + 2).to_bytes(2, "big") + body
+
+
+def test_jpeg_oversubscribed_huffman_table_is_rejected():
+    good = pil_jpeg(smooth_image(16, 16, 3, 1), "RGB", quality=80)
+    # The bad table replaces the first DHT segment; the file is otherwise intact.
+    at, marker, payload = next(s for s in jpeg_segments(good) if s[1] == 0xC4)
+    bad = good[:at] + oversubscribed_dht() + good[at + 4 + len(payload):]
+    with pytest.raises(wpt.WptError):
+        wpt.decode_image(bad)
+    # Over-subscription at every length up to the fast-table limit and past it.
+    for length in range(1, 12):
+        counts = [0] * 16
+        counts[length - 1] = (1 << length) + 1
+        values = bytes(range(counts[length - 1] % 256)) if counts[length - 1] <= 256 else bytes(256)
+        counts[length - 1] = min(counts[length - 1], 255)
+        if counts[length - 1] <= (1 << length):
+            continue
+        body = bytes([0x00]) + bytes(counts) + values[:counts[length - 1]]
+        seg = b"\xff\xc4" + (len(body) + 2).to_bytes(2, "big") + body
+        with pytest.raises(wpt.WptError):
+            wpt.decode_image(good[:at] + seg + good[at + 4 + len(payload):])
+
+
+def png_with_stream(samples, stream: bytes) -> bytes:
+    from gltf_fixtures import chunk
+    import struct
+    h, w = samples.shape[:2]
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, 0, 0, 0, 0)
+    return b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", stream) + chunk(b"IEND", b"")
+
+
+def test_png_truncated_dynamic_stream_is_rejected():
+    import zlib
+    # Mostly zero bytes, Huffman-only: the all-zero code is a literal, so a
+    # reader that runs out of bits would keep producing zeros.
+    rng = np.random.default_rng(5)
+    samples = np.where(rng.random((64, 64, 1)) < 0.9, 0, rng.integers(1, 256, (64, 64, 1))).astype(np.uint8)
+    raw = b"".join(b"\x00" + samples[y, :, 0].tobytes() for y in range(64))
+    co = zlib.compressobj(9, zlib.DEFLATED, 15, 9, zlib.Z_HUFFMAN_ONLY)
+    stream = co.compress(raw) + co.flush()
+    assert np.array_equal(wpt.decode_image(png_with_stream(samples, stream))[..., 0], samples[..., 0])
+    for cut in (len(stream) // 4, len(stream) // 2, len(stream) - 6):
+        with pytest.raises(wpt.WptError):
+            wpt.decode_image(png_with_stream(samples, stream[:cut]))
+
+
+def test_png_extra_image_data_is_ignored():
+    import zlib
+    samples = np.random.default_rng(6).integers(0, 256, (9, 11, 1)).astype(np.uint8)
+    raw = b"".join(b"\x00" + samples[y, :, 0].tobytes() for y in range(9))
+    # Rows beyond the header's height: decoded as far as the image goes, as
+    # stb_image does.
+    stream = zlib.compress(raw + bytes(5000), 6)
+    img = wpt.decode_image(png_with_stream(samples, stream))
+    assert np.array_equal(img[..., 0], samples[..., 0])
+    if os.path.exists(STB):
+        assert np.array_equal(img, stb_decode(png_with_stream(samples, stream)))
+
+
+def test_corrupt_images_under_sanitizers(tmp_path):
+    # The host decoder built on the CPU with AddressSanitizer and UBSan runs
+    # over mutated and truncated PNG, baseline and progressive JPEG files,
+    # plus the over-subscribed table; an out-of-bounds access aborts it.
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    host = os.path.join(ROOT, "wavefront-path-tracer_amd", "host")
+    exe = tmp_path / "asan_decode"
+    build = subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
+                            "-fno-sanitize-recover=all", "-I", host,
+                            os.path.join(ROOT, "tests", "asan_decode_main.cpp"),
+                            os.path.join(host, "image_decode.cpp"), "-o", str(exe)],
+                           capture_output=True, text=True)
+    if build.returncode != 0 and "sanitize" in build.stderr:
+        pytest.skip("sanitizer runtime unavailable: " + build.stderr[-200:])
+    assert build.returncode == 0, build.stderr[-2000:]
+    rng = np.random.default_rng(11)
+    bases = [encode_png(rng.integers(0, 256, (20, 17, 4)), 6, 8, level=9),
+             encode_png(rng.integers(0, 256, (9, 13, 1)), 0, 8, interlace=True, level=1),
+             pil_jpeg(smooth_image(41, 33, 3, 4), "RGB", quality=80, subsampling=2),
+             pil_jpeg(smooth_image(41, 33, 3, 4), "RGB", quality=80, subsampling=2, progressive=True)]
+    files = []
+    good = bases[2]
+    at, _, payload = next(s for s in jpeg_segments(good) if s[1] == 0xC4)
+    special = [good[:at] + oversubscribed_dht(codes) + good[at + 4 + len(payload):] for codes in (3, 17, 200, 255)]
+    for i, data in enumerate(special):
+        files.append(tmp_path / f"special{i}")
+        files[-1].write_bytes(data)
+    for b, base in enumerate(bases):
+        for it in range(250):
+            data = bytearray(base)
+            if it % 4 == 3:
+                data = data[:int(rng.integers(1, len(data)))]
+            for _ in range(int(rng.integers(1, 8))):
+                k = int(rng.integers(0, len(data)))
+                data[k] = int(rng.integers(0, 256)) if it % 2 else data[k] ^ (1 << int(rng.integers(0, 8)))
+            files.append(tmp_path / f"m{b}_{it}")
+            files[-1].write_bytes(bytes(data))
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0", UBSAN_OPTIONS="halt_on_error=1")
+    run = subprocess.run([str(exe)] + [str(f) for f in files], capture_output=True, text=True, env=env, timeout=600)
+    assert run.returncode == 0, (run.stdout + run.stderr)[-3000:]

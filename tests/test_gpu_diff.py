@@ -126,3 +126,19 @@ def test_adam_steps_reduce_the_loss(torch_cuda):
     assert losses[-1] < 0.5 * losses[0], losses
     lay = r.layout
     assert float(p.min()) >= 0.0 and float(p[lay.texture_floats:lay.texture_floats + lay.roughness_count].min()) >= 0.02
+
+
+def test_adjoint_after_adam_needs_a_new_forward(torch_cuda):
+    # wpt_diff_adam changes the parameters in place: the kept path radiance
+    # then belongs to the old values, and an adjoint must not replay it.
+    import torch
+    host, scene, cam, r, params = setup(32, 2)
+    img = r.forward(params)
+    g = 2.0 * (img - 0.5 * img) / img.numel()
+    grad = r.adjoint(params, g)
+    m, v = torch.zeros_like(params), torch.zeros_like(params)
+    r.adam(params, grad, m, v, 1, 0.01)
+    with pytest.raises(wpt.WptError):
+        r.adjoint(params, g)
+    img = r.forward(params)
+    r.adjoint(params, 2.0 * (img - 0.5 * img) / img.numel())

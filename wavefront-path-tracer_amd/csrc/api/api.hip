@@ -1290,7 +1290,9 @@ struct wpt_diff {
     DiffContext ctx{};
     uint32_t forwardBlocks = 0, adjointBlocks = 0;
     uint32_t textureFloats = 0, roughnessCount = 0, emissionCount = 0;
-    const float* forwardParams = nullptr;  // parameters of the kept path radiance
+    // Parameters of the kept path radiance; cleared when wpt_diff_adam
+    // changes them, so an adjoint cannot replay paths of older values.
+    const float* forwardParams = nullptr;
     wpt_diff_stats stats{};
     ~wpt_diff() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -1483,6 +1485,8 @@ WPT_API int wpt_diff_adam(wpt_diff* d, float* params, const float* grad, float* 
     a.roughnessRange[1] = 1.0f;
     a.emissionRange[0] = 0.0f;
     a.emissionRange[1] = std::numeric_limits<float>::infinity();
+    // The kept paths belong to the old values from here on.
+    if (params == d->forwardParams) d->forwardParams = nullptr;
     WPT_HIP(enqueueAdam(a, d->stream));
     WPT_HIP(hipStreamSynchronize(d->stream));
     return WPT_OK;

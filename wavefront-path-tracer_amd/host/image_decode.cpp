@@ -136,24 +136,38 @@ constexpr uint16_t kDistBase[30] = {1, 2, 3, 4, 5, 7, 9, 13, 17, 25, 33, 49, 65,
 constexpr uint8_t kDistExtra[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11,
     12, 12, 13, 13};
 
-bool inflateBlock(BitReader& in, const Huffman& lit, const Huffman& dist, std::vector<uint8_t>& out,
+// Outcome of one compressed block.
+enum class BlockEnd { kFailed, kEndOfBlock, kLimit };
+
+// Decodes one compressed block.  `limit` bounds the inflated size (the
+// image's filtered bytes): decoding stops once that many bytes exist, since
+// data after the last row is ignored, so no stream can grow the output
+// further.
+BlockEnd inflateBlock(BitReader& in, const Huffman& lit, const Huffman& dist, std::vector<uint8_t>& out, size_t limit,
     std::string* error) {
+    auto failed = [&](const char* message) {
+        fail(error, message);
+        return BlockEnd::kFailed;
+    };
     for (;;) {
+        if (out.size() >= limit) return BlockEnd::kLimit;
         int sym = lit.decode(in);
-        if (sym < 0) return fail(error, "PNG: bad Huffman code");
+        if (sym < 0) return failed("PNG: bad Huffman code");
+        if (in.overrun()) return failed("PNG: truncated zlib stream");
         if (sym < 256) {
             out.push_back(static_cast<uint8_t>(sym));
             continue;
         }
-        if (sym == 256) return true;
+        if (sym == 256) return BlockEnd::kEndOfBlock;
         sym -= 257;
-        if (sym >= 29) return fail(error, "PNG: bad length symbol");
+        if (sym >= 29) return failed("PNG: bad length symbol");
         size_t length = kLengthBase[sym] + in.take(kLengthExtra[sym]);
         int d = dist.decode(in);
-        if (d < 0 || d >= 30) return fail(error, "PNG: bad distance code");
+        if (d < 0 || d >= 30) return failed("PNG: bad distance code");
         size_t distance = kDistBase[d] + in.take(kDistExtra[d]);
-        if (distance > out.size()) return fail(error, "PNG: distance before the start of the data");
-        if (in.overrun()) return fail(error, "PNG: truncated zlib stream");
+        if (distance > out.size()) return failed("PNG: distance before the start of the data");
+        if (in.overrun()) return failed("PNG: truncated zlib stream");
+        length = std::min(length, limit - out.size());
         size_t from = out.size() - distance;
         for (size_t k = 0; k < length; ++k) out.push_back(out[from + k]);
     }
@@ -201,7 +215,9 @@ bool dynamicTables(BitReader& in, Huffman& lit, Huffman& dist, std::string* erro
 bool inflateZlib(const uint8_t* data, size_t size, std::vector<uint8_t>& out, size_t expectedSize,
     std::string* error) {
     out.clear();
-    out.reserve(expectedSize);
+    // Deflate expands at most 1032:1, so a tiny stream cannot claim a huge
+    // reservation through its header alone.
+    out.reserve(std::min(expectedSize, size * 1032));
     if (size < 2) return fail(error, "PNG: truncated zlib stream");
     uint32_t cmf = data[0], flg = data[1];
     if ((cmf * 256 + flg) % 31 != 0) return fail(error, "PNG: bad zlib header");
@@ -218,8 +234,10 @@ bool inflateZlib(const uint8_t* data, size_t size, std::vector<uint8_t>& out, si
             if ((len ^ 0xffffu) != nlen) return fail(error, "PNG: corrupt stored block");
             size_t at = in.bytePosition();
             if (at + len > size - 2) return fail(error, "PNG: truncated zlib stream");
-            out.insert(out.end(), data + 2 + at, data + 2 + at + len);
+            size_t keep = std::min<size_t>(len, expectedSize - std::min(expectedSize, out.size()));
+            out.insert(out.end(), data + 2 + at, data + 2 + at + keep);
             in.seekByte(at + len);
+            if (out.size() >= expectedSize) return true;
         } else if (type == 1) {
             uint8_t lengths[288 + 32];
             std::memset(lengths, 8, 144);
@@ -229,9 +247,14 @@ bool inflateZlib(const uint8_t* data, size_t size, std::vector<uint8_t>& out, si
             std::memset(lengths + 288, 5, 32);
             lit.build(lengths, 288);
             dist.build(lengths + 288, 32);
-            if (!inflateBlock(in, lit, dist, out, error)) return false;
+            BlockEnd end = inflateBlock(in, lit, dist, out, expectedSize, error);
+            if (end == BlockEnd::kFailed) return false;
+            if (end == BlockEnd::kLimit) return true;
         } else if (type == 2) {
-            if (!dynamicTables(in, lit, dist, error) || !inflateBlock(in, lit, dist, out, error)) return false;
+            if (!dynamicTables(in, lit, dist, error)) return false;
+            BlockEnd end = inflateBlock(in, lit, dist, out, expectedSize, error);
+            if (end == BlockEnd::kFailed) return false;
+            if (end == BlockEnd::kLimit) return true;
         } else {
             return fail(error, "PNG: bad block type");
         }
@@ -509,13 +532,16 @@ struct JpegHuffman {
             valptr[len] = k;
             mincode[len] = code;
             for (int i = 0; i < counts[len - 1]; ++i, ++k, ++code) {
+                // A code of `len` bits must stay below 1 << len; a table with
+                // more codes than that is over-subscribed, and its fast-table
+                // span would run past the end of `fast`.
+                if (code >= (1 << len)) return false;
                 if (len <= 9) {
                     int first = code << (9 - len);
                     for (int f = 0; f < (1 << (9 - len)); ++f)
                         fast[first + f] = static_cast<uint16_t>(((len << 8) | symbols[k]) + 1);
                 }
             }
-            if (code - 1 >= (1 << len) && counts[len - 1]) return false;  // over-subscribed
             maxcode[len] = code << (16 - len);
             code <<= 1;
         }

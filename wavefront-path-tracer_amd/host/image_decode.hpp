@@ -30,8 +30,10 @@ namespace wpt {
 bool decodeImage(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, uint32_t& width, uint32_t& height,
     std::string* error);
 
-// zlib stream (RFC 1950 header, deflate body) to bytes; expectedSize is a
-// capacity hint.
+// zlib stream (RFC 1950 header, deflate body) to bytes.  Decoding stops
+// once expectedSize bytes exist (the rest of the stream is not read), so the
+// output never exceeds expectedSize; a shorter output is the caller's to
+// reject.
 bool inflateZlib(const uint8_t* data, size_t size, std::vector<uint8_t>& out, size_t expectedSize,
     std::string* error);
 
