@@ -2935,6 +2935,15 @@ void oracle_elementary(uint32_t op, const float* x, const float* y, uint32_t n, 
 
 float oracle_dielectric_reflectance(float cosine, float eta) { return dielectricReflectance<float>(cosine, eta); }
 
+void oracle_offset_ray_origin(const float* position, const float* normal, const float* direction, float hit_distance,
+    float* out) {
+    V3<float> r = offsetRayOrigin(V3<float>{position[0], position[1], position[2]},
+        V3<float>{normal[0], normal[1], normal[2]}, V3<float>{direction[0], direction[1], direction[2]}, hit_distance);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
 void oracle_eval_specgloss(const float f0[3], const float diffuse[3], float roughness, const float n[3], const float wo[3],
     const float wi[3], float value[3], float* pdf) {
     SG<float> s{vec<float>(diffuse), vec<float>(f0), roughness};

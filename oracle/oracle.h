@@ -133,6 +133,11 @@ void oracle_elementary(uint32_t op, const float* x, const float* y, uint32_t n, 
 
 /* Reference BSDF helpers for known-answer tests. */
 float oracle_dielectric_reflectance(float cosine, float eta);
+/* offsetRayOrigin (ray.cuh:10-21): the continuation / shadow ray origin
+ * pushed off a surface point along the geometric normal, to the side of
+ * `direction`. */
+void oracle_offset_ray_origin(const float* position, const float* normal, const float* direction, float hit_distance,
+                              float* out);
 void oracle_eval_specgloss(const float f0[3], const float diffuse[3], float roughness, const float n[3],
                            const float wo[3], const float wi[3], float value[3], float* pdf);
 

@@ -80,6 +80,7 @@ def library():
             ("oracle_reference_rays", [u32, u32, P, u32], u32),
             ("oracle_rng_stream", [u32, u32, u32, u32, P], C.c_uint64),
             ("oracle_dielectric_reflectance", [C.c_float, C.c_float], C.c_float),
+            ("oracle_offset_ray_origin", [P, P, P, C.c_float, P], None),
             ("oracle_elementary", [u32, P, P, u32, P], None),
             ("oracle_eval_specgloss", [P, P, C.c_float, P, P, P, P, P], None),
         ]:
@@ -163,6 +164,13 @@ def gltf_reflection_normal(normal, geometric, view):
     out = np.zeros(3, np.float32)
     n, g, v = f32(normal), f32(geometric), f32(view)
     library().oracle_gltf_reflection_normal(n.ctypes.data, g.ctypes.data, v.ctypes.data, out.ctypes.data)
+    return out
+
+
+def offset_ray_origin(position, normal, direction, hit_distance=0.0):
+    out = np.zeros(3, np.float32)
+    p, n, d = f32(position), f32(normal), f32(direction)
+    library().oracle_offset_ray_origin(p.ctypes.data, n.ctypes.data, d.ctypes.data, hit_distance, out.ctypes.data)
     return out
 
 
