@@ -426,6 +426,16 @@ WPT_API int wpt_trace_occluded(const wpt_scene* scene, const float* d_rays6, con
 WPT_API int wpt_compact(const uint32_t* d_items, const uint8_t* d_flags, uint32_t n, uint32_t* d_out,
                         uint32_t capacity, uint32_t* d_count, void* stream);
 
+/* wpt_compact without any device allocation, for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same contract and
+ * output as wpt_compact for n <= 16777216 items (larger inputs are rejected);
+ * its scratch is a fixed 256 KiB device array that each GPU holds until the
+ * process ends, so calls that overlap on one GPU race.  Asynchronous: the
+ * work is queued on `stream` and nothing waits for it. */
+WPT_API int wpt_compact_noalloc(const uint32_t* d_items, const uint8_t* d_flags, uint32_t n, uint32_t* d_out,
+                                uint32_t capacity, uint32_t* d_count, void* stream);
+
 /* ---- renderer: reference RenderDriver / RenderTaskRenderer ------------ */
 WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, const wpt_render_config* config,
                                 uint64_t job_fingerprint, wpt_renderer** out);

@@ -257,3 +257,13 @@ def test_procedural_scene_shapes():
     metals = [c1.materials[c1.spheres[i].material].roughness for i in range(1, 485)
               if c1.materials[c1.spheres[i].material].type == wpt.METAL]
     assert max(metals) <= 0.5
+
+
+def test_noalloc_compaction_rejects_oversized_input_before_the_device():
+    # The size limit of the fixed scratch is checked on the host, before any
+    # device call, so it holds without a GPU.
+    lib = wpt.library()
+    dummy = C.c_void_p(1)
+    status = lib.wpt_compact_noalloc(dummy, dummy, (1 << 24) + 1, dummy, 0, dummy, None)
+    assert status == 1 and b"16777216" in lib.wpt_last_error()
+    assert lib.wpt_compact_noalloc(dummy, dummy, 4, dummy, 4, None, None) == 1  # null count

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -586,14 +587,63 @@ WPT_API int wpt_compact(const uint32_t* items, const uint8_t* flags, uint32_t n,
         return fail(WPT_ERROR_INVALID_ARGUMENT, "null compaction buffer");
     if (n >= (1u << 30)) return fail(WPT_ERROR_INVALID_ARGUMENT, "compaction input too large");
     uint32_t words = compactScratchWords(n);
-    void* scratch = nullptr;
-    WPT_HIP(hipMalloc(&scratch, sizeof(uint32_t) * words));
+    void* temp = nullptr;
+    WPT_HIP(hipMalloc(&temp, sizeof(uint32_t) * words));
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipError_t e = launchCompact(items, flags, n, out, capacity, count, static_cast<uint32_t*>(scratch), words, st);
+    hipError_t e = launchCompact(items, flags, n, out, capacity, count, static_cast<uint32_t*>(temp), words, st);
     hipError_t e2 = hipStreamSynchronize(st);
-    (void)hipFree(scratch);
+    (void)hipFree(temp);
     if (e != hipSuccess) return hipFail(e, "launchCompact");
     if (e2 != hipSuccess) return hipFail(e2, "hipStreamSynchronize");
+    return WPT_OK;
+}
+
+// ---- compaction without allocation ---------------------------------------
+// The scratch of wpt_compact_noalloc (chunk ticket + one look-back flag per
+// 256-item chunk) lives in a fixed device array instead of a per-call
+// allocation.  Every device holds its own copy, at its own address; host
+// code reaches it only through the address hipGetSymbolAddress reports for
+// the current device, never through the array's name.
+constexpr uint32_t kNoAllocMaxItems = 1u << 24;
+constexpr uint32_t kNoAllocScratchWords = 1u + kNoAllocMaxItems / 256u;
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+uint32_t* gScratchAddress[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device.
+int currentScratch(uint32_t** out) {
+    int device = 0;
+    WPT_HIP(hipGetDevice(&device));
+    if (device < 0 || device >= kMaxDevices) return fail(WPT_ERROR_UNSUPPORTED, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchAddress[device]) {
+        void* p = nullptr;
+        WPT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        gScratchAddress[device] = static_cast<uint32_t*>(p);
+    }
+    *out = gScratchAddress[device];
+    return WPT_OK;
+}
+
+}  // namespace
+
+WPT_API int wpt_compact_noalloc(const uint32_t* items, const uint8_t* flags, uint32_t n, uint32_t* out,
+    uint32_t capacity, uint32_t* count, void* stream) {
+    if (!count || (n && (!items || !flags)) || (capacity && !out))
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "null compaction buffer");
+    if (n > kNoAllocMaxItems)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "compaction input exceeds the 16777216 items of the fixed scratch");
+    uint32_t words = compactScratchWords(n);
+    if (words > kNoAllocScratchWords) return fail(WPT_ERROR_INVALID_ARGUMENT, "compaction scratch too small");
+    uint32_t* base = nullptr;
+    int status = currentScratch(&base);
+    if (status != WPT_OK) return status;
+    WPT_HIP(launchCompact(items, flags, n, out, capacity, count, base, words, static_cast<hipStream_t>(stream)));
     return WPT_OK;
 }
 

@@ -275,6 +275,7 @@ def library() -> C.CDLL:
         "wpt_trace_closest": ([P, P, u32, P, P], C.c_int),
         "wpt_trace_occluded": ([P, P, P, u32, P, P], C.c_int),
         "wpt_compact": ([P, P, u32, P, u32, P, P], C.c_int),
+        "wpt_compact_noalloc": ([P, P, u32, P, u32, P, P], C.c_int),
         "wpt_renderer_create": ([P, C.POINTER(Camera), C.POINTER(RenderConfig), u64, C.POINTER(P)], C.c_int),
         "wpt_renderer_destroy": ([P], C.c_int),
         "wpt_renderer_reset": ([P], C.c_int),
