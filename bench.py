@@ -61,17 +61,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, camera, threads, sample_spp):
+def cpu_baseline(scene, camera, threads, sample_spp, runs=5):
     """Oracle FP32 render of `sample_spp` samples of the full frame on the
-    host cores; returns the rate, the sample description and the work counts."""
+    host cores, `runs` times; returns the median rate, the sample
+    description and the work counts."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     oracle = pyoracle.Oracle(scene)
     cfg = wpt.render_config(WIDTH, HEIGHT, spp=sample_spp, max_depth=DEPTH, rr_start=RR_START, clamp=CLAMP)
-    t0 = time.perf_counter()
-    _, st = oracle.render(camera, cfg, threads=threads)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        _, st = oracle.render(camera, cfg, threads=threads)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     queries = st.closest_queries + st.shadow_queries
     return {
         "value": round(queries / dt / 1e6, 3),
@@ -79,9 +83,75 @@ def cpu_baseline(scene, camera, threads, sample_spp):
         "cores": threads,
         "kind": "port",
         "sample": f"{WIDTH}x{HEIGHT} x {sample_spp} spp of the same scene/camera/seeds ({st.paths} paths, "
-                  f"{queries} queries, {dt:.2f} s), FP32 scalar restatement, {threads} std::threads",
+                  f"{queries} queries), median of {runs} runs ({dt:.2f} s; {min(times):.2f}-{max(times):.2f} s), "
+                  f"FP32 scalar restatement, {threads} std::threads",
         "msamples_per_s": round(st.paths / dt / 1e6, 4),
     }, st
+
+
+def primary_rays(camera, width, height):
+    """Camera rays through the pixel centres (SoA float32, 6 x w*h), row 0 at
+    the top: position + normalize(f + r*x + u*y) with
+    x = (2 (i + 0.5) / w - 1) * aspect * tan(fov/2), y = (1 - 2 (j + 0.5) / h) * tan(fov/2)."""
+    p, f, r, u = (np.array(list(v), np.float64) for v in (camera.position, camera.forward, camera.right, camera.up))
+    t = np.tan(camera.vertical_fov * 0.5)
+    j, i = np.mgrid[0:height, 0:width]
+    x = (2.0 * (i.reshape(-1) + 0.5) / width - 1.0) * camera.aspect_ratio * t
+    y = (1.0 - 2.0 * (j.reshape(-1) + 0.5) / height) * t
+    d = f[None, :] + r[None, :] * x[:, None] + u[None, :] * y[:, None]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.empty((6, width * height), np.float32)
+    rays[0:3] = p[:, None]
+    rays[3:6] = d.T
+    return rays
+
+
+def measure_traversal(scene, camera, threads, reps=5):
+    """Closest-hit traversal alone (wpt_trace_closest against the oracle's
+    traceRay restatement on the host cores) on two ray sets of the headline
+    scene: the reference acceleration test's rays (8 fixed + 2048 from
+    mt19937 seed 12345, tests/scene_acceleration_tests.cu:184-201) and one
+    primary wavefront of the 1920x1080 frame.  Median of `reps` runs each;
+    hit records compared bit for bit."""
+    import concurrent.futures
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    oracle = pyoracle.Oracle(scene)
+    out = {}
+    sets = {"reference_rays_2056": pyoracle.reference_rays(12345, 2048),
+            "primary_wavefront_1080p": primary_rays(camera, WIDTH, HEIGHT)}
+    pool = concurrent.futures.ThreadPoolExecutor(threads)
+    for name, rays in sets.items():
+        n = rays.shape[1]
+        d_rays = torch.from_numpy(rays).cuda()
+        hits = scene.trace_closest(d_rays)
+        torch.cuda.synchronize()
+        gpu = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            hits = scene.trace_closest(d_rays)
+            b.record()
+            b.synchronize()
+            gpu.append(a.elapsed_time(b) / 1000.0)
+        gpu_hits = wpt.hits_to_numpy(hits)
+        chunks = np.array_split(np.arange(n), threads)
+        cpu = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            parts = list(pool.map(lambda c: oracle.trace(np.ascontiguousarray(rays[:, c])), chunks))
+            cpu.append(time.perf_counter() - t0)
+        cpu_hits = np.concatenate(parts)
+        g, c = float(np.median(gpu)), float(np.median(cpu))
+        out[name] = {"rays": int(n), "gpu_mrays_per_s": round(n / g / 1e6, 2), "gpu_ms": round(g * 1000.0, 4),
+                     "cpu_mrays_per_s": round(n / c / 1e6, 3), "cpu_ms": round(c * 1000.0, 2), "cpu_threads": threads,
+                     "hits_identical": gpu_hits.tobytes() == cpu_hits.tobytes(),
+                     "hit_fraction": round(float((gpu_hits["kind"] != 0).mean()), 4)}
+        log(f"traversal {name}: {out[name]}")
+    pool.shutdown()
+    return out
 
 
 def measure_cornell(threads, frames=20):
@@ -125,17 +195,44 @@ def measure_cornell(threads, frames=20):
     return out
 
 
-def batch_workload(n_tasks, seed=2024):
+def terrain_scene():
+    """The shading-heavy terrain preset at full size, BLASes built on the
+    device, and its render configuration (caustics with `photons` spectral
+    photons, 1920x1080, depth 10, Russian roulette from 3, PCG32)."""
+    host = wpt.HostScene("terrain")
+    desc = wpt.SceneDesc.from_buffer_copy(host.desc())
+    desc.bvh_builder = wpt.BVH_DEVICE_PLOC
+    scene = wpt.Scene(desc, device=True, owner=host)
+    return host, scene
+
+
+def terrain_config(photons, spp=64, mode="--regenerate"):
+    status, rc, _, _ = wpt.parse_command_line(
+        ["--scene", "terrain", "--caustics", "--caustic-photons", str(photons), "--resolution",
+         f"{WIDTH}x{HEIGHT}", "--spp", str(spp), "--max-depth", "10", "--rr-depth", "3", "--rng", "pcg32", mode])
+    assert status == wpt.OK
+    return rc
+
+
+def batch_workload(n_tasks, photons, seed=2024):
     """Batching workload: four resident scenes -- the Cornell box, the sphere
-    field with its 1,003,520-triangle icosphere, the dielectric prism room
-    (standing in for the shading-heavy scene, whose textures, environment and
-    photon map are not built) and a sphere-field variant whose icosphere has
-    200,000 triangles (frequency 100) -- and `n_tasks` square tiles of side
-    {32, 64, 128, 256, 512} at seeded positions of a 1920x1080 frame, with
-    spp uniform in [1, 64], mixed across the scenes."""
-    specs = [("cornell-box", 0, 8), ("sphere-field", 0, 12), ("prism", 0, 10), ("sphere-field", 100, 12)]
-    hosts = [wpt.HostScene(name, icosphere_frequency=freq) for name, freq, _ in specs]
-    scenes = [wpt.Scene(h.desc(), device=True, owner=h) for h in hosts]
+    field with its 1,003,520-triangle icosphere, the shading-heavy terrain
+    (2,006,680 triangles, 16 x 4096^2 textures, HDR sky, `photons` caustic
+    photons in its own photon map) and a sphere-field variant whose
+    icosphere has 200,000 triangles (frequency 100) -- and `n_tasks` square
+    tiles of side {32, 64, 128, 256, 512} at seeded positions of a 1920x1080
+    frame, with spp uniform in [1, 64], mixed across the scenes."""
+    specs = [("cornell-box", 0, 8), ("sphere-field", 0, 12), ("terrain", 0, 10), ("sphere-field", 100, 12)]
+    hosts, scenes = [], []
+    for name, freq, _ in specs:
+        if name == "terrain":
+            h, sc = terrain_scene()
+        else:
+            h = wpt.HostScene(name, icosphere_frequency=freq)
+            sc = wpt.Scene(h.desc(), device=True, owner=h)
+        hosts.append(h)
+        scenes.append(sc)
+    configs = [None, None, terrain_config(photons), None]
     cams = [h.camera(WIDTH, HEIGHT) for h in hosts]
     rng = np.random.default_rng(seed)
     tasks = []
@@ -146,38 +243,25 @@ def batch_workload(n_tasks, seed=2024):
         spp = int(rng.integers(1, 65))
         tasks.append(wpt.batch_task(s, cams[s], WIDTH, HEIGHT, x, y, side, side, 0, spp, max_depth=specs[s][2],
                                     rr_start=RR_START, rng=wpt.RNG_PCG32, clamp=CLAMP))
-    return hosts, scenes, tasks
+    return hosts, scenes, configs, tasks
 
 
-def measure_batch(n_tasks):
-    hosts, scenes, tasks = batch_workload(n_tasks)
-    samples = sum(t.tile_width * t.tile_height * t.sample_count for t in tasks)
-    batch = wpt.Batch(scenes)
-    # Autotune on the first 32 tasks at up to 4 spp, then render everything.
-    probe = [wpt.batch_task(t.scene, t.camera, t.width, t.height, t.x, t.y, t.tile_width, t.tile_height, 0,
-                            min(4, t.sample_count), t.max_depth, t.rr_start_depth, t.rng_strategy, t.indirect_clamp)
-             for t in tasks[:32]]
-    ta = time.perf_counter()
-    chosen = batch.autotune(probe)
-    tune_s = time.perf_counter() - ta
-    log(f"batch: autotuned in {tune_s:.1f} s -> block {chosen.block_size}, grid {chosen.blocks}")
-    batch.render(tasks[:8])  # warm-up
-    tb = time.perf_counter()
-    outs, lat = batch.render(tasks, latencies=True)
-    batch_s = time.perf_counter() - tb
-    bst = batch.stats()
-    log(f"batch: {n_tasks} tasks, {samples / 1e6:.1f} M samples in {batch_s * 1000:.1f} ms "
-        f"(device {bst.milliseconds:.1f} ms)")
-
-    # The same tasks one by one through wpt_render_task, as the reference's
-    # RenderTaskRenderer renders them: one renderer per scene and tile size,
-    # its queues sized to the tile.
+def tasks_one_by_one(hosts, scenes, configs, tasks, mode, outs):
+    """The batch's tasks through wpt_render_task one at a time, as the
+    reference's RenderTaskRenderer renders them: one renderer per scene and
+    tile size (its queues sized to the tile; a caustic scene's renderer
+    builds its own photon map), all created before the clock starts.
+    Returns the wall time, per-task completion times and whether every task
+    equals the batch's output."""
     renderers = {}
     for t in tasks:
         key = (t.scene, t.tile_width)
         if key not in renderers:
-            cfg = wpt.render_config(WIDTH, HEIGHT, spp=64, max_depth=t.max_depth, rr_start=RR_START, clamp=CLAMP,
-                                    tile=(t.tile_width, t.tile_height))
+            if configs[t.scene] is not None:
+                cfg = wpt.RenderConfig.from_buffer_copy(configs[t.scene])
+            else:
+                cfg = wpt.render_config(WIDTH, HEIGHT, spp=64, max_depth=t.max_depth, rr_start=RR_START, clamp=CLAMP)
+            cfg.tiled, cfg.tile_width, cfg.tile_height, cfg.mode = 1, t.tile_width, t.tile_height, mode
             renderers[key] = wpt.Renderer(scenes[t.scene], hosts[t.scene].camera(WIDTH, HEIGHT), cfg)
     for (_, side), r in renderers.items():
         r.render_task(wpt.Task(0, 0, 0, 1, 0, 0, side, side))
@@ -189,28 +273,65 @@ def measure_batch(n_tasks):
             wpt.Task(0, 0, t.sample_start, t.sample_count, t.x, t.y, t.tile_width, t.tile_height))
         done.append((time.perf_counter() - tp) * 1000.0)
         identical &= bool(np.array_equal(out, outs[i]))
-    plain_s = time.perf_counter() - tp
+    wall = time.perf_counter() - tp
     for r in renderers.values():
         r.close()
-    lat_plain = np.array(done)
+    return wall, np.array(done), identical
+
+
+def measure_batch(n_tasks, photons):
+    hosts, scenes, configs, tasks = batch_workload(n_tasks, photons)
+    samples = sum(t.tile_width * t.tile_height * t.sample_count for t in tasks)
+    batch = wpt.Batch(scenes)
+    stored, photon_ms = batch.set_caustics(2, configs[2])
+    # Autotune on the first 32 tasks at up to 4 spp, then render everything.
+    probe = [wpt.batch_task(t.scene, t.camera, t.width, t.height, t.x, t.y, t.tile_width, t.tile_height, 0,
+                            min(4, t.sample_count), t.max_depth, t.rr_start_depth, t.rng_strategy, t.indirect_clamp)
+             for t in tasks[:32]]
+    ta = time.perf_counter()
+    chosen = batch.autotune(probe)
+    tune_s = time.perf_counter() - ta
+    log(f"batch: terrain photon map {stored} photons stored in {photon_ms:.1f} ms; autotuned in {tune_s:.1f} s "
+        f"-> block {chosen.block_size}, grid {chosen.blocks}")
+    batch.render(tasks[:8])  # warm-up
+    tb = time.perf_counter()
+    outs, lat = batch.render(tasks, latencies=True)
+    batch_s = time.perf_counter() - tb
+    bst = batch.stats()
+    batch.close()
+    log(f"batch: {n_tasks} tasks, {samples / 1e6:.1f} M samples in {batch_s * 1000:.1f} ms "
+        f"(device {bst.milliseconds:.1f} ms)")
+    # Tasks one by one with the same kernel (regenerating scheduler), then
+    # with the staged hipGraph wavefront.
+    regen_s, regen_done, regen_same = tasks_one_by_one(hosts, scenes, configs, tasks, wpt.MODE_REGENERATE, outs)
+    log(f"batch: one by one, regenerating kernel: {regen_s * 1000:.1f} ms")
+    graph_s, graph_done, graph_same = tasks_one_by_one(hosts, scenes, configs, tasks, wpt.MODE_GRAPH, outs)
+    log(f"batch: one by one, staged wavefront: {graph_s * 1000:.1f} ms")
+
+    def pct(a):
+        return [round(float(np.percentile(a, 50)), 2), round(float(np.percentile(a, 99)), 2)]
+
     result = {
-        "workload": f"{n_tasks} tasks over 4 resident scenes (cornell-box, sphere-field 1.0M-tri icosphere, prism, "
-                    f"sphere-field 200k-tri icosphere); square tiles {{32..512}}, spp U[1,64], seed 2024",
+        "workload": f"{n_tasks} tasks over 4 resident scenes (cornell-box, sphere-field 1.0M-tri icosphere, terrain "
+                    f"2.0M tris + 16 4096^2 textures + HDR sky + {photons} caustic photons, sphere-field 200k-tri "
+                    f"icosphere); square tiles {{32..512}}, spp U[1,64], seed 2024",
         "samples": int(samples),
         "batched_msamples_per_s": round(samples / batch_s / 1e6, 2),
-        "plain_msamples_per_s": round(samples / plain_s / 1e6, 2),
-        "speedup": round(plain_s / batch_s, 3),
         "batched_ms": round(batch_s * 1000.0, 2),
-        "plain_ms": round(plain_s * 1000.0, 2),
-        "batched_latency_ms_p50_p99": [round(float(np.percentile(lat, 50)), 2), round(float(np.percentile(lat, 99)), 2)],
-        "plain_latency_ms_p50_p99": [round(float(np.percentile(lat_plain, 50)), 2),
-                                     round(float(np.percentile(lat_plain, 99)), 2)],
+        "batched_latency_ms_p50_p99": pct(lat),
+        "one_by_one_same_kernel": {
+            "msamples_per_s": round(samples / regen_s / 1e6, 2), "ms": round(regen_s * 1000.0, 2),
+            "latency_ms_p50_p99": pct(regen_done), "speedup_of_batch": round(regen_s / batch_s, 3),
+            "identical": regen_same},
+        "one_by_one_staged": {
+            "msamples_per_s": round(samples / graph_s / 1e6, 2), "ms": round(graph_s * 1000.0, 2),
+            "latency_ms_p50_p99": pct(graph_done), "speedup_of_batch": round(graph_s / batch_s, 3),
+            "identical": graph_same},
+        "terrain_photons_stored": int(stored),
         "autotuned": {"block_size": int(chosen.block_size), "blocks": int(chosen.blocks),
                       "tuning_s": round(tune_s, 2)},
-        "identical_to_plain": identical,
     }
     log(f"batch vs plain: {result}")
-    batch.close()
     return result
 
 
@@ -308,20 +429,12 @@ def measure_terrain(spp, photons, frames):
     spectral photon map, 1920x1080, depth 10, regenerating scheduler; BLASes
     built on the device."""
     t0 = time.perf_counter()
-    host = wpt.HostScene("terrain")
-    t1 = time.perf_counter()
-    desc = wpt.SceneDesc.from_buffer_copy(host.desc())
-    desc.bvh_builder = wpt.BVH_DEVICE_PLOC
-    scene = wpt.Scene(desc, device=True, owner=host)
+    host, scene = terrain_scene()
     t2 = time.perf_counter()
     info = scene.info()
-    log(f"terrain: generated in {t1 - t0:.2f} s, scene build + upload {t2 - t1:.2f} s "
+    log(f"terrain: generated, built and uploaded in {t2 - t0:.2f} s "
         f"(device BLAS build {info.blas_build_milliseconds:.1f} ms)")
-    status, rc, _, _ = wpt.parse_command_line(
-        ["--scene", "terrain", "--caustics", "--caustic-photons", str(photons), "--resolution",
-         f"{WIDTH}x{HEIGHT}", "--spp", str(spp), "--max-depth", "10", "--rr-depth", "3", "--rng", "pcg32",
-         "--regenerate"])
-    assert status == wpt.OK
+    rc = terrain_config(photons, spp)
     r = wpt.Renderer(scene, host.camera(WIDTH, HEIGHT), rc)
     photon = r.stats()
     r.reset()
@@ -369,6 +482,8 @@ def main():
     ap.add_argument("--skip-diff", action="store_true", help="do not measure the differentiable renderer")
     ap.add_argument("--skip-terrain", action="store_true", help="do not measure the shading-heavy caustic workload")
     ap.add_argument("--skip-cornell", action="store_true", help="do not measure the small Cornell box case")
+    ap.add_argument("--skip-traversal", action="store_true",
+                    help="do not measure closest-hit traversal alone on the reference and primary ray sets")
     ap.add_argument("--terrain-spp", type=int, default=64)
     ap.add_argument("--terrain-photons", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--diff-steps", type=int, default=200, help="Adam steps of the inverse-rendering workload")
@@ -381,12 +496,11 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is a design, "
               f"not a measured configuration", file=sys.stderr)
         sys.exit(2)
-    if args.calibration_copy_mib or not args.skip_diff:
-        # torch ships its own HIP runtime beside the one libwpt links; it
-        # must initialise the device before libwpt's first device call.
-        import torch
-        torch.zeros(1, device="cuda:0")
-        torch.cuda.synchronize()
+    # torch ships its own HIP runtime beside the one libwpt links; it must
+    # initialise the device before libwpt's first device call.
+    import torch
+    torch.zeros(1, device="cuda:0")
+    torch.cuda.synchronize()
     if wpt.device_count() == 0:
         print("bench.py needs a gfx950 device", file=sys.stderr)
         sys.exit(1)
@@ -416,6 +530,8 @@ def main():
     mode = modes[args.mode]
     cfg = wpt.render_config(WIDTH, HEIGHT, spp=args.spp, max_depth=DEPTH, rr_start=RR_START, clamp=CLAMP, mode=mode)
     renderer = wpt.Renderer(scene, camera, cfg)
+    launch = renderer.launch_info()
+    log(f"launch: {launch}")
 
     for i in range(args.warmup):
         renderer.reset()
@@ -490,7 +606,11 @@ def main():
         log(f"cpu baseline: {cpu}")
     else:
         # Work counts for the byte model from one sample of the frame.
-        _, work = cpu_baseline(scene, camera, threads, 1)
+        _, work = cpu_baseline(scene, camera, threads, 1, runs=1)
+
+    traversal = None
+    if not args.skip_traversal:
+        traversal = measure_traversal(scene, camera, threads)
 
     cornell = None
     if not args.skip_cornell:
@@ -498,7 +618,7 @@ def main():
 
     batch = None
     if not args.skip_batch:
-        batch = measure_batch(args.batch_tasks)
+        batch = measure_batch(args.batch_tasks, args.terrain_photons)
 
     diff = None
     if not args.skip_diff:
@@ -537,6 +657,9 @@ def main():
             "bytes_per_closest_query": round(per_query, 1),
             "avg_launch_ms": round(trace_ms / max(1, trace_launches), 4),
             "launches": trace_launches,
+            "occupancy": {k: launch[k] for k in ("block_size", "blocks", "blocks_per_cu", "waves_per_simd",
+                                                 "registers", "scratch_bytes", "lds_bytes_per_block",
+                                                 "lds_bytes_per_cu")},
         }
 
     line = {
@@ -562,6 +685,7 @@ def main():
         "device_ms_per_frame": round(float(np.mean(device_ms)), 2),
         "other_schedulers": others,
         "plain_small": cornell,
+        "traversal": traversal,
         "batch": batch,
         "diff": diff,
         "shading_heavy": terrain,

@@ -467,6 +467,23 @@ WPT_API int wpt_renderer_last_stats(const wpt_renderer* renderer, wpt_stats* sta
 WPT_API int wpt_renderer_profile_kernels(wpt_renderer* renderer, uint32_t first_sample, uint32_t count,
                                          double* kernel_ms, uint32_t* launches);
 
+/* Launch shape and resources of the renderer's dominant kernel (the
+ * regenerating kernel, the cooperative persistent kernel, or the staged
+ * closest-hit trace), as the occupancy calculation sees them. */
+typedef struct wpt_launch_info {
+    uint32_t mode;                 /* WPT_MODE_* of the renderer */
+    uint32_t block_size;           /* threads per workgroup */
+    uint32_t blocks;               /* workgroups of a persistent launch (0: one item per thread) */
+    uint32_t blocks_per_cu;        /* resident workgroups per compute unit */
+    uint32_t compute_units;
+    uint32_t registers;            /* VGPRs per lane */
+    uint32_t scratch_bytes;        /* private (spill) bytes per lane */
+    uint32_t lds_bytes_per_block;  /* static + dynamic LDS of one workgroup */
+    uint32_t lds_bytes_per_cu;     /* LDS available on a compute unit */
+    uint32_t waves_per_simd;       /* resident waves per SIMD at blocks_per_cu */
+} wpt_launch_info;
+WPT_API int wpt_renderer_launch_info(const wpt_renderer* renderer, wpt_launch_info* info);
+
 /* ---- batch: many tasks of several scenes in one persistent launch -------
  * No reference counterpart: the reference renders its tasks one at a time
  * (RenderTaskRenderer::render, render_task.cu:26-114); a batch takes the
@@ -500,6 +517,17 @@ typedef struct wpt_batch_stats {
 WPT_API int wpt_batch_create(wpt_scene* const* scenes, uint32_t scene_count, const wpt_batch_config* config,
                              wpt_batch** out);
 WPT_API int wpt_batch_destroy(wpt_batch* batch);
+/* Caustics of one batch scene (reference RenderDriver::buildPhotonMap,
+ * render_driver.cu:180-234, with the photon fields of RenderConfig,
+ * config.hpp:18-22): builds the scene's photon map from the caustic_* fields
+ * and rng_strategy of `config`, exactly as wpt_renderer_create does for a
+ * renderer with that configuration; with caustic_gather set, every task of
+ * that scene then gathers it at diffuse vertices, so its sums equal
+ * wpt_render_task's on such a renderer.  caustic_photons 0 removes the map.
+ * photons_stored and build_ms (may be NULL) receive the photons kept in the
+ * grid and the build's device time. */
+WPT_API int wpt_batch_set_caustics(wpt_batch* batch, uint32_t scene, const wpt_render_config* config,
+                                   uint32_t* photons_stored, double* build_ms);
 /* Renders every task in one launch.  outs[i] (host memory) receives task i's
  * unnormalized tile sums, tile_width*tile_height*3 floats row-major, bit
  * identical to wpt_render_task on the same task.  task_latency_ms, when not

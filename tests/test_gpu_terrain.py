@@ -92,3 +92,59 @@ def test_full_terrain_schedulers_agree(torch_cuda):
     assert np.all(np.isfinite(images[0])) and images[0].mean() > 0
     assert stored[0] == stored[1] > 1000
     assert np.array_equal(images[0], images[1])
+
+
+def test_mixed_batch_with_caustic_terrain(torch_cuda, oracle_mod, tmp_path):
+    # A batch holding a small caustic terrain beside two other scenes: the
+    # terrain's photon map is built for the batch from the same caustic
+    # settings a renderer takes, and every task equals wpt_render_task on a
+    # renderer with those settings bit for bit; small terrain tasks equal the
+    # oracle's caustic render.
+    manifest = small_manifest(tmp_path, seed=7)
+    terrain_host = wpt.HostScene(manifest)
+    others = [wpt.HostScene("cornell-box"), wpt.HostScene("sphere-field", icosphere_frequency=8)]
+    hosts = [others[0], terrain_host, others[1]]
+    scenes = [wpt.Scene(h.desc(), device=True, owner=h) for h in hosts]
+    w, h = 96, 64
+    rc, _ = config(manifest, w, h, 8, 30000)
+    batch = wpt.Batch(scenes, block_size=256)
+    stored, ms = batch.set_caustics(1, rc)
+    assert stored > 0
+    rng = np.random.default_rng(11)
+    tasks = []
+    for i in range(18):
+        s = i % 3
+        tw, th = int(rng.choice([16, 32, 48])), int(rng.choice([16, 24, 32]))
+        x, y = int(rng.integers(0, w - tw + 1)), int(rng.integers(0, h - th + 1))
+        tasks.append(wpt.batch_task(s, hosts[s].camera(w, h), w, h, x, y, tw, th, int(rng.integers(0, 3)),
+                                    int(rng.integers(1, 5)), max_depth=10, rr_start=3, rng=wpt.RNG_PCG32,
+                                    clamp=rc.indirect_clamp))
+    outs = batch.render(tasks)
+    st = batch.stats()
+    closest = shadow = 0
+    for t, out in zip(tasks, outs):
+        cfg = wpt.RenderConfig.from_buffer_copy(rc) if t.scene == 1 else wpt.render_config(
+            w, h, spp=8, max_depth=10, rr_start=3, rng=wpt.RNG_PCG32, clamp=rc.indirect_clamp)
+        cfg.tiled, cfg.tile_width, cfg.tile_height = 1, t.tile_width, t.tile_height
+        r = wpt.Renderer(scenes[t.scene], t.camera, cfg)
+        if t.scene == 1:
+            assert r.stats().caustic_photons_stored == stored
+        want = r.render_task(wpt.Task(0, 0, t.sample_start, t.sample_count, t.x, t.y, t.tile_width, t.tile_height))
+        q = r.stats()
+        closest += q.closest_queries
+        shadow += q.shadow_queries
+        r.close()
+        assert np.array_equal(out, want), f"task on scene {t.scene}: max |diff| {np.abs(out - want).max()}"
+    assert (st.closest_queries, st.shadow_queries) == (closest, shadow)
+    # Terrain tasks against the oracle with the same photon map settings.
+    oracle = oracle_mod.Oracle(scenes[1])
+    for t, out in zip(tasks, outs):
+        if t.scene != 1:
+            continue
+        fb = np.zeros((h, w, 3), np.float32)
+        want, ost = oracle.render(t.camera, rc, tile=(t.x, t.y, t.tile_width, t.tile_height),
+                                  sample_start=t.sample_start, sample_count=t.sample_count, framebuffer=fb)
+        assert ost.caustic_photons_stored == stored
+        tile = want[t.y:t.y + t.tile_height, t.x:t.x + t.tile_width]
+        assert np.array_equal(out, tile)
+    batch.close()

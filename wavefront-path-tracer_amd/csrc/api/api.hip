@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -598,6 +599,87 @@ WPT_API int wpt_compact(const uint32_t* items, const uint8_t* flags, uint32_t n,
     return WPT_OK;
 }
 
+// ---- caustic photon maps -----------------------------------------------
+namespace {
+
+// Caustic settings of a render configuration (config.hpp:18-22).
+int validateCaustics(const wpt_render_config& c) {
+    if (c.caustic_photons > 0x7fffffffu) return fail(WPT_ERROR_INVALID_ARGUMENT, "too many caustic photons");
+    if (!(c.caustic_gather_radius >= 0.0f) || !std::isfinite(c.caustic_gather_radius) ||
+        c.caustic_grid_resolution > 1024)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "invalid caustic gather radius or grid resolution");
+    if (c.rng_strategy > WPT_RNG_PCG32) return fail(WPT_ERROR_INVALID_ARGUMENT, "unknown RNG strategy");
+    return WPT_OK;
+}
+
+struct PhotonMapResult {
+    const PhotonGridView* view;  // device copy of the gather's view
+    double milliseconds;         // trace + grid build
+    uint32_t stored;             // photons kept inside the grid
+};
+
+// Builds the caustic photon map of `scene` under the caustic fields of `c`
+// (RenderDriver::buildPhotonMap, render_driver.cu:180-234) into buffers of
+// `arena`, on `stream`; waits for it.  `spill` holds the traversal stacks of
+// `spillThreads` threads (at least 256).
+int buildPhotonMap(const DevScene& scene, const wpt_render_config& c, DeviceArena& arena, uint2* spill,
+    uint32_t spillThreads, hipStream_t stream, hipEvent_t start, hipEvent_t end, PhotonMapResult* out) {
+    PhotonSettings ps{c.caustic_photons, c.caustic_max_depth ? c.caustic_max_depth : 8u, 0x13579bdfu, c.rng_strategy,
+        c.caustic_spectral ? 1u : 0u};
+    PhotonGridView view{};
+    view.resolution = c.caustic_grid_resolution ? c.caustic_grid_resolution : 64u;
+    view.radius = c.caustic_gather_radius > 0.0f ? c.caustic_gather_radius : 0.06f;
+    bool box = c.caustic_grid_min[0] < c.caustic_grid_max[0] && c.caustic_grid_min[1] < c.caustic_grid_max[1] &&
+        c.caustic_grid_min[2] < c.caustic_grid_max[2];
+    view.minimum = box ? Vec3(c.caustic_grid_min[0], c.caustic_grid_min[1], c.caustic_grid_min[2])
+                       : Vec3(-2.5f, -1.0f, -2.5f);
+    view.maximum = box ? Vec3(c.caustic_grid_max[0], c.caustic_grid_max[1], c.caustic_grid_max[2])
+                       : Vec3(2.5f, 3.0f, 2.5f);
+    uint32_t cells = view.resolution * view.resolution * view.resolution;
+    PhotonMapBuffers pb{};
+    size_t n = c.caustic_photons;
+    void* p = nullptr;
+    WPT_HIP(arena.alloc(sizeof(Photon) * n, &p));
+    pb.photons = static_cast<Photon*>(p);
+    WPT_HIP(arena.alloc(sizeof(Photon) * n, &p));
+    pb.sorted = static_cast<Photon*>(p);
+    WPT_HIP(arena.alloc(sizeof(uint64_t) * n, &p));
+    pb.keys = static_cast<uint64_t*>(p);
+    WPT_HIP(arena.alloc(sizeof(uint64_t) * n, &p));
+    pb.sortedKeys = static_cast<uint64_t*>(p);
+    WPT_HIP(photonSortBytes(c.caustic_photons, cells, &pb.tempBytes));
+    WPT_HIP(arena.alloc(pb.tempBytes > 0 ? pb.tempBytes : 1, &p));
+    pb.temp = p;
+    WPT_HIP(arena.alloc(sizeof(CellRange) * cells, &p));
+    pb.cells = static_cast<CellRange*>(p);
+    WPT_HIP(arena.alloc(sizeof(uint32_t) * 7, &p));
+    pb.stored = static_cast<uint32_t*>(p);
+    pb.occupied = pb.stored + 1;
+    view.photons = pb.sorted;
+    view.cells = pb.cells;
+    WPT_HIP(hipEventRecord(start, stream));
+    WPT_HIP(enqueuePhotonMap(scene, ps, pb, view, spill, spillThreads, stream));
+    WPT_HIP(hipEventRecord(end, stream));
+    WPT_HIP(hipEventSynchronize(end));
+    float ms = 0.0f;
+    WPT_HIP(hipEventElapsedTime(&ms, start, end));
+    uint32_t counters[7];
+    WPT_HIP(hipMemcpy(counters, pb.stored, sizeof counters, hipMemcpyDeviceToHost));
+    for (int a = 0; a < 3; ++a) {
+        view.occupiedMin[a] = counters[1 + a];
+        view.occupiedMax[a] = counters[4 + a];
+    }
+    // The gather's view, with the occupied cell box, once the map exists.
+    WPT_HIP(arena.alloc(sizeof(PhotonGridView), &p));
+    WPT_HIP(hipMemcpy(p, &view, sizeof(PhotonGridView), hipMemcpyHostToDevice));
+    out->view = static_cast<const PhotonGridView*>(p);
+    out->milliseconds = ms;
+    out->stored = counters[0];
+    return WPT_OK;
+}
+
+}  // namespace
+
 // ---- compaction without allocation ---------------------------------------
 // The scratch of wpt_compact_noalloc (chunk ticket + one look-back flag per
 // 256-item chunk) lives in a fixed device array instead of a per-call
@@ -682,10 +764,8 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     // Caustic photons gathered at diffuse vertices need the full-feature
     // kernels (render_driver.cu:180-234 builds the map per driver).
     if (c.caustic_photons) {
-        if (c.caustic_photons > 0x7fffffffu) return cleanup(fail(WPT_ERROR_INVALID_ARGUMENT, "too many caustic photons"));
-        if (!(c.caustic_gather_radius >= 0.0f) || !std::isfinite(c.caustic_gather_radius) ||
-            c.caustic_grid_resolution > 1024)
-            return cleanup(fail(WPT_ERROR_INVALID_ARGUMENT, "invalid caustic gather radius or grid resolution"));
+        int vs = validateCaustics(c);
+        if (vs != WPT_OK) return cleanup(vs);
         if (c.caustic_gather) ctx.scene.fullFeatures = 1;
     }
     ctx.camera.position = Vec3(camera->position[0], camera->position[1], camera->position[2]);
@@ -795,56 +875,12 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     r->regen.spill = ctx.spill;
     r->regen.spillThreads = ctx.spillThreads;
     if (c.caustic_photons) {
-        PhotonSettings ps{c.caustic_photons, c.caustic_max_depth ? c.caustic_max_depth : 8u, 0x13579bdfu,
-            c.rng_strategy, c.caustic_spectral ? 1u : 0u};
-        PhotonGridView view{};
-        view.resolution = c.caustic_grid_resolution ? c.caustic_grid_resolution : 64u;
-        view.radius = c.caustic_gather_radius > 0.0f ? c.caustic_gather_radius : 0.06f;
-        bool box = c.caustic_grid_min[0] < c.caustic_grid_max[0] && c.caustic_grid_min[1] < c.caustic_grid_max[1] &&
-            c.caustic_grid_min[2] < c.caustic_grid_max[2];
-        view.minimum = box ? Vec3(c.caustic_grid_min[0], c.caustic_grid_min[1], c.caustic_grid_min[2])
-                           : Vec3(-2.5f, -1.0f, -2.5f);
-        view.maximum = box ? Vec3(c.caustic_grid_max[0], c.caustic_grid_max[1], c.caustic_grid_max[2])
-                           : Vec3(2.5f, 3.0f, 2.5f);
-        uint32_t cells = view.resolution * view.resolution * view.resolution;
-        PhotonMapBuffers pb{};
-        size_t n = c.caustic_photons;
-        R_HIP(r->arena.alloc(sizeof(Photon) * n, &p));
-        pb.photons = static_cast<Photon*>(p);
-        R_HIP(r->arena.alloc(sizeof(Photon) * n, &p));
-        pb.sorted = static_cast<Photon*>(p);
-        R_HIP(r->arena.alloc(sizeof(uint64_t) * n, &p));
-        pb.keys = static_cast<uint64_t*>(p);
-        R_HIP(r->arena.alloc(sizeof(uint64_t) * n, &p));
-        pb.sortedKeys = static_cast<uint64_t*>(p);
-        R_HIP(photonSortBytes(c.caustic_photons, cells, &pb.tempBytes));
-        R_HIP(r->arena.alloc(pb.tempBytes > 0 ? pb.tempBytes : 1, &p));
-        pb.temp = p;
-        R_HIP(r->arena.alloc(sizeof(CellRange) * cells, &p));
-        pb.cells = static_cast<CellRange*>(p);
-        R_HIP(r->arena.alloc(sizeof(uint32_t) * 7, &p));
-        pb.stored = static_cast<uint32_t*>(p);
-        pb.occupied = pb.stored + 1;
-        view.photons = pb.sorted;
-        view.cells = pb.cells;
-        R_HIP(hipEventRecord(r->start, r->stream));
-        R_HIP(enqueuePhotonMap(ctx.scene, ps, pb, view, ctx.spill, ctx.spillThreads, r->stream));
-        R_HIP(hipEventRecord(r->end, r->stream));
-        R_HIP(hipEventSynchronize(r->end));
-        float ms = 0.0f;
-        R_HIP(hipEventElapsedTime(&ms, r->start, r->end));
-        r->photonMs = ms;
-        uint32_t counters[7];
-        R_HIP(hipMemcpy(counters, pb.stored, sizeof counters, hipMemcpyDeviceToHost));
-        r->photonsStored = counters[0];
-        for (int a = 0; a < 3; ++a) {
-            view.occupiedMin[a] = counters[1 + a];
-            view.occupiedMax[a] = counters[4 + a];
-        }
-        // The gather's view, with the occupied cell box, once the map exists.
-        R_HIP(r->arena.alloc(sizeof(PhotonGridView), &p));
-        R_HIP(hipMemcpy(p, &view, sizeof(PhotonGridView), hipMemcpyHostToDevice));
-        if (c.caustic_gather) ctx.shade.photons = static_cast<const PhotonGridView*>(p);
+        PhotonMapResult map{};
+        int ps = buildPhotonMap(ctx.scene, c, r->arena, ctx.spill, ctx.spillThreads, r->stream, r->start, r->end, &map);
+        if (ps != WPT_OK) return cleanup(ps);
+        r->photonMs = map.milliseconds;
+        r->photonsStored = map.stored;
+        if (c.caustic_gather) ctx.shade.photons = map.view;
     }
     R_HIP(hipDeviceSynchronize());
 #undef R_HIP
@@ -1018,6 +1054,42 @@ WPT_API int wpt_renderer_profile_kernels(wpt_renderer* r, uint32_t first, uint32
     return WPT_OK;
 }
 
+WPT_API int wpt_renderer_launch_info(const wpt_renderer* r, wpt_launch_info* info) {
+    if (!r || !info) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    const wpt_render_config& c = r->config;
+    const LaunchContext& ctx = r->ctx;
+    bool full = ctx.scene.fullFeatures != 0;
+    *info = wpt_launch_info{};
+    info->mode = c.mode;
+    info->block_size = c.block_size;
+    hipFuncAttributes attr{};
+    int perCu = 0;
+    size_t lds = stackLdsBytes(c.block_size);
+    if (c.mode == WPT_MODE_REGENERATE) {
+        WPT_HIP(batchAttributes(full, &attr));
+        WPT_HIP(batchOccupancy(c.block_size, full, &perCu));
+        info->blocks = r->regenBlocks;
+    } else {
+        Occupancy occ{};
+        WPT_HIP(kernelOccupancy(c.block_size, full, &occ));
+        WPT_HIP(kernelAttributes(c.mode == WPT_MODE_PERSISTENT, full, &attr));
+        perCu = c.mode == WPT_MODE_PERSISTENT ? occ.persistent : occ.trace;
+        info->blocks = c.mode == WPT_MODE_PERSISTENT ? ctx.persistentBlocks : 0u;
+    }
+    int device = 0, cus = 0, ldsCu = 0;
+    WPT_HIP(hipGetDevice(&device));
+    WPT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    WPT_HIP(hipDeviceGetAttribute(&ldsCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
+    info->blocks_per_cu = static_cast<uint32_t>(perCu);
+    info->compute_units = static_cast<uint32_t>(cus);
+    info->registers = static_cast<uint32_t>(attr.numRegs);
+    info->scratch_bytes = static_cast<uint32_t>(attr.localSizeBytes);
+    info->lds_bytes_per_block = static_cast<uint32_t>(attr.sharedSizeBytes + lds);
+    info->lds_bytes_per_cu = static_cast<uint32_t>(ldsCu);
+    info->waves_per_simd = static_cast<uint32_t>(perCu) * ((c.block_size + 63u) / 64u) / 4u;
+    return WPT_OK;
+}
+
 WPT_API int wpt_renderer_last_stats(const wpt_renderer* r, wpt_stats* stats) {
     if (!r || !stats) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
     WavefrontCounters c;
@@ -1066,6 +1138,10 @@ struct wpt_batch {
     int cus = 0;
     int wallClockKhz = 0;
     wpt_batch_stats last{};
+    // Per scene: the caustic photon map its tasks gather at diffuse vertices
+    // (null: none) and the buffers behind it.
+    std::vector<const PhotonGridView*> photons;
+    std::vector<std::unique_ptr<DeviceArena>> photonArena;
     ~wpt_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : {static_cast<void*>(scenesDev), static_cast<void*>(counters), static_cast<void*>(tasks),
@@ -1137,7 +1213,8 @@ int stageBatch(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, std::vecto
         d.camera.up = Vec3(t.camera.up[0], t.camera.up[1], t.camera.up[2]);
         d.camera.verticalFov = t.camera.vertical_fov;
         d.camera.aspectRatio = t.camera.aspect_ratio;
-        d.shade = ShadeParams{t.max_depth, t.rr_start_depth, t.rng_strategy, t.indirect_clamp, 0u, 0u};
+        d.shade = ShadeParams{t.max_depth, t.rr_start_depth, t.rng_strategy, t.indirect_clamp, 0u, 0u,
+            b->photons[t.scene]};
         d.tile = RenderTile{t.x, t.y, t.tile_width, t.tile_height, 0u};
         d.scene = t.scene;
         d.width = t.width;
@@ -1168,19 +1245,24 @@ int stageBatch(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, std::vecto
     return WPT_OK;
 }
 
+// Traversal spill columns for `threads` threads.
+int ensureBatchSpill(wpt_batch* b, uint32_t threads) {
+    if (threads <= b->spillThreads) return WPT_OK;
+    if (b->spill) WPT_HIP(hipFree(b->spill));
+    b->spill = nullptr;
+    b->spillThreads = 0;
+    WPT_HIP(hipMalloc(reinterpret_cast<void**>(&b->spill), sizeof(uint2) * kSpillEntriesTotal * static_cast<size_t>(threads)));
+    b->spillThreads = threads;
+    return WPT_OK;
+}
+
 // One launch over the staged tasks under configuration `c`; device time in *ms.
 int runBatch(wpt_batch* b, uint32_t n, const wpt_batch_config& c, float* ms) {
     uint32_t blocks = 0;
     int s = batchGrid(b, c, &blocks);
     if (s != WPT_OK) return s;
-    uint32_t threads = blocks * c.block_size;
-    if (threads > b->spillThreads) {
-        if (b->spill) WPT_HIP(hipFree(b->spill));
-        b->spill = nullptr;
-        b->spillThreads = 0;
-        WPT_HIP(hipMalloc(reinterpret_cast<void**>(&b->spill), sizeof(uint2) * kSpillEntriesTotal * static_cast<size_t>(threads)));
-        b->spillThreads = threads;
-    }
+    s = ensureBatchSpill(b, blocks * c.block_size);
+    if (s != WPT_OK) return s;
     BatchContext ctx{};
     ctx.scenes = b->scenesDev;
     ctx.tasks = b->tasks;
@@ -1221,6 +1303,8 @@ WPT_API int wpt_batch_create(wpt_scene* const* scenes, uint32_t sceneCount, cons
     wpt_batch* b = new (std::nothrow) wpt_batch();
     if (!b) return fail(WPT_ERROR_OUT_OF_MEMORY, "host allocation failed");
     b->scenes.assign(scenes, scenes + sceneCount);
+    b->photons.assign(sceneCount, nullptr);
+    b->photonArena.resize(sceneCount);
     for (uint32_t i = 0; i < sceneCount; ++i) b->full = b->full || scenes[i]->dev.fullFeatures;
     b->config = *config;
     auto cleanup = [&](int status) {
@@ -1248,6 +1332,42 @@ WPT_API int wpt_batch_create(wpt_scene* const* scenes, uint32_t sceneCount, cons
     int s = batchGrid(b, b->config, &blocks);
     if (s != WPT_OK) return cleanup(s);
     *out = b;
+    return WPT_OK;
+}
+
+WPT_API int wpt_batch_set_caustics(wpt_batch* b, uint32_t scene, const wpt_render_config* config,
+    uint32_t* photonsStored, double* buildMs) {
+    if (!b || !config) return fail(WPT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (scene >= b->scenes.size()) return fail(WPT_ERROR_INVALID_ARGUMENT, "scene index out of range");
+    int s = validateCaustics(*config);
+    if (s != WPT_OK) return s;
+    WPT_HIP(hipStreamSynchronize(b->stream));
+    b->photons[scene] = nullptr;
+    b->photonArena[scene].reset();
+    uint32_t stored = 0;
+    double ms = 0.0;
+    if (config->caustic_photons) {
+        uint32_t blocks = 0;
+        s = batchGrid(b, b->config, &blocks);
+        if (s != WPT_OK) return s;
+        s = ensureBatchSpill(b, std::max(blocks * b->config.block_size, 256u));
+        if (s != WPT_OK) return s;
+        auto arena = std::make_unique<DeviceArena>();
+        PhotonMapResult map{};
+        s = buildPhotonMap(b->scenes[scene]->dev, *config, *arena, b->spill, b->spillThreads, b->stream, b->start,
+            b->end, &map);
+        if (s != WPT_OK) return s;
+        b->photonArena[scene] = std::move(arena);
+        if (config->caustic_gather) b->photons[scene] = map.view;
+        stored = map.stored;
+        ms = map.milliseconds;
+    }
+    // Gathering needs the full-feature kernel.
+    bool full = false;
+    for (size_t i = 0; i < b->scenes.size(); ++i) full = full || b->scenes[i]->dev.fullFeatures || b->photons[i];
+    b->full = full;
+    if (photonsStored) *photonsStored = stored;
+    if (buildMs) *buildMs = ms;
     return WPT_OK;
 }
 

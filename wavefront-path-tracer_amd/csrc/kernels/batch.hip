@@ -47,8 +47,18 @@ __global__ void __launch_bounds__(512) kBatchReset(BatchContext ctx) {
     }
 }
 
+// Scenes without environment maps, punctual lights and texture bindings
+// (the kernel without `Full`) test occlusion only for the area-light sample,
+// and that sample's contribution is the last radiance a bounce adds.  Their
+// shading therefore defers the shadow ray: the lane traces it in the next
+// loop iteration, through the same traversal call as the closest-hit
+// queries of the other lanes, and adds the contribution if nothing blocks
+// it -- the same additions in the same order as testing it inline.  Full
+// scenes, whose shading adds radiance after several occlusion tests, trace
+// those inline.
 template <bool Full>
-__global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY kBatch(BatchContext ctx) {
+__global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY(Full) kBatch(BatchContext ctx) {
+    constexpr bool kDefer = !Full;
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const uint32_t n = ctx.taskCount;
     const uint32_t waveId = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -62,6 +72,9 @@ __global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY kBatch(BatchContext c
     uint32_t x = 0, y = 0, samplesDone = 0, local = 0;
     Vec3 sum(0.0f, 0.0f, 0.0f);
     PathState path;
+    ShadowRequest pending;   // deferred shadow ray of the last shading step
+    pending.valid = 0;
+    bool pathEnded = false;  // the path ended at its last shading step
     unsigned long long closest = 0, shadow = 0;
 
     while (true) {
@@ -111,14 +124,33 @@ __global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY kBatch(BatchContext c
         }
         if (busy) {
             const DevScene& scene = ctx.scenes[T.scene];
-            ++closest;
-            HitRecord h = traceClosest<Full>(scene, path.ray, stack);
-            ShadeOutcome outcome;
-            InlineOcclusion<LdsStack> occ{stack, 0u};
-            shadePath<Full>(scene, path, h, T.shade, occ, outcome);
-            shadow += occ.queries;
-            if (!outcome.alive) {
-                sum += outcome.isDebug ? outcome.debugColor : path.radiance;
+            const bool shadowQuery = kDefer && pending.valid;
+            HitRecord h = traceQuery<Full, kDefer ? Query::kEither : Query::kClosest>(scene,
+                shadowQuery ? Ray(pending.origin, pending.direction) : path.ray, pending.tmax, shadowQuery, stack);
+            bool ended;
+            if (shadowQuery) {
+                ++shadow;
+                if (h.kind == kHitNone) path.radiance += pending.contribution;
+                pending.valid = 0;
+                ended = pathEnded;
+            } else {
+                ++closest;
+                ShadeOutcome outcome;
+                if constexpr (kDefer) {
+                    DeferOcclusion occ;
+                    shadePath<Full>(scene, path, h, T.shade, occ, outcome);
+                    pending = outcome.shadow;
+                } else {
+                    InlineOcclusion<LdsStack> occ{stack, 0u};
+                    shadePath<Full>(scene, path, h, T.shade, occ, outcome);
+                    shadow += occ.queries;
+                }
+                if (outcome.isDebug) path.radiance = outcome.debugColor;
+                pathEnded = !outcome.alive;
+                ended = pathEnded && !(kDefer && pending.valid);
+            }
+            if (ended) {
+                sum += path.radiance;
                 ++samplesDone;
                 if (samplesDone < T.sampleCount) {
                     path = primaryPath(T.camera, x, y, T.width, T.height, T.sampleStart + samplesDone,
@@ -162,6 +194,11 @@ hipError_t batchOccupancy(uint32_t blockSize, bool full, int* blocksPerCu) {
                       stackLdsBytes(blockSize))
                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch<false>,
                       static_cast<int>(blockSize), stackLdsBytes(blockSize));
+}
+
+hipError_t batchAttributes(bool full, hipFuncAttributes* out) {
+    return full ? hipFuncGetAttributes(out, reinterpret_cast<const void*>(kBatch<true>))
+                : hipFuncGetAttributes(out, reinterpret_cast<const void*>(kBatch<false>));
 }
 
 }  // namespace wpt

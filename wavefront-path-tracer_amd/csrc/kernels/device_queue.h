@@ -13,11 +13,20 @@ constexpr uint32_t kSpillEntries = kMaxStackDepth - kLdsStackEntries;
 
 // Occupancy target of the traversal kernels (waves per SIMD), 0: compiler's
 // choice.  The register budget it implies must not force scratch spills.
-#if defined(WPT_BATCH_WAVES_PER_EU) && WPT_BATCH_WAVES_PER_EU > 0
-#define WPT_BATCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WPT_BATCH_WAVES_PER_EU)))
-#else
-#define WPT_BATCH_OCCUPANCY
+// The batch kernel has its own targets with (Full) and without the full
+// feature set; a target of 1 leaves the choice to the compiler.
+#ifndef WPT_BATCH_WAVES_PER_EU
+#define WPT_BATCH_WAVES_PER_EU 0
 #endif
+#ifndef WPT_BATCH_LEAN_WAVES_PER_EU
+#define WPT_BATCH_LEAN_WAVES_PER_EU 0
+#endif
+constexpr int batchWavesPerEu(bool full) {
+    return (full ? WPT_BATCH_WAVES_PER_EU : WPT_BATCH_LEAN_WAVES_PER_EU) > 0
+        ? (full ? WPT_BATCH_WAVES_PER_EU : WPT_BATCH_LEAN_WAVES_PER_EU)
+        : 1;
+}
+#define WPT_BATCH_OCCUPANCY(full) __attribute__((amdgpu_waves_per_eu(batchWavesPerEu(full))))
 #if defined(WPT_TRACE_WAVES_PER_EU) && WPT_TRACE_WAVES_PER_EU > 0
 #define WPT_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WPT_TRACE_WAVES_PER_EU)))
 #else

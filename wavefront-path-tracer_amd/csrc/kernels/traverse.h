@@ -200,15 +200,42 @@ WPT_HD bool popLive(Stack& stack, uint32_t base, float closestT, uint32_t& cur) 
     return false;
 }
 
-// Closest triangle of one BVH (BLAS in local space, or the flat-triangle BVH).
+// Query kinds.  A closest-hit query keeps the nearest hit in the tie order
+// above; an any-hit query (shadow rays) stops at the first blocker closer
+// than its distance limit.  kEither picks one of the two per lane at run
+// time, so that lanes of one wave running different kinds of query step
+// through the same traversal loop together.
+enum class Query { kClosest, kAny, kEither };
+
+// Offers a candidate at distance t to the query.  Any-hit queries accept
+// anything strictly closer than their limit (held in best.t); closest-hit
+// queries apply the tie order.
+WPT_HD bool wanted(bool any, float t, uint32_t rank, uint32_t instance, uint32_t prim, const HitRecord& best) {
+    return any ? t < best.t : nearer(t, rank, instance, prim, best);
+}
+
+WPT_HD void record(HitRecord& best, float t, float u, float v, uint32_t kind, uint32_t instance, uint32_t prim,
+    uint32_t material, uint32_t rank) {
+    best.t = t;
+    best.u = u;
+    best.v = v;
+    best.kind = kind;
+    best.instance = instance;
+    best.prim = prim;
+    best.material = material;
+    best.rank = rank;
+}
+
+// Triangles of one BVH (the flat-triangle BVH, world space).  Returns true
+// when an any-hit query found its blocker.
 template <bool Full, class Stack>
-WPT_HD void closestInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, const float4* geometry,
-    const TriShade* tris, uint32_t materialOverride, uint32_t kind, uint32_t rank, uint32_t instance, HitRecord& best,
+WPT_HD bool queryBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, const float4* geometry, const TriShade* tris,
+    uint32_t materialOverride, uint32_t kind, uint32_t rank, uint32_t instance, bool any, HitRecord& best,
     Stack& stack) {
     float nearT;
     if (!slab(r, bvh.rootMin[0], bvh.rootMin[1], bvh.rootMin[2], bvh.rootMax[0], bvh.rootMax[1], bvh.rootMax[2],
             best.t, nearT))
-        return;
+        return false;
     uint32_t base = stack.size;
     uint32_t cur = bvh.root;
     while (true) {
@@ -219,17 +246,14 @@ WPT_HD void closestInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, con
             float4 c = geometry[3 * slot + 2];
             float t, u, v;
             if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) &&
-                nearer(t, rank, instance, wbits(a), best)) {
+                wanted(any, t, rank, instance, wbits(a), best)) {
                 uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
                 if (!cutAway<Full>(s, material, tris, wbits(a), u, v)) {
-                    best.t = t;
-                    best.u = u;
-                    best.v = v;
-                    best.kind = kind;
-                    best.instance = instance;
-                    best.prim = wbits(a);
-                    best.material = material;
-                    best.rank = rank;
+                    record(best, t, u, v, kind, instance, wbits(a), material, rank);
+                    if (any) {
+                        stack.truncate(base);
+                        return true;
+                    }
                 }
             }
         } else {
@@ -252,89 +276,25 @@ WPT_HD void closestInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, con
                 continue;
             }
         }
-        if (!popLive(stack, base, best.t, cur)) return;
+        if (!popLive(stack, base, best.t, cur)) return false;
     }
 }
 
-// First triangle of one BVH closer than maxDistance (shadow rays).
-template <bool Full, class Stack>
-WPT_HD bool anyInBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, const float4* geometry,
-    const TriShade* tris, uint32_t materialOverride, float maxDistance, Stack& stack) {
-    float nearT;
-    if (!slab(r, bvh.rootMin[0], bvh.rootMin[1], bvh.rootMin[2], bvh.rootMax[0], bvh.rootMax[1], bvh.rootMax[2],
-            maxDistance, nearT))
-        return false;
-    uint32_t base = stack.size;
-    uint32_t cur = bvh.root;
-    while (true) {
-        if (cur & kLeafBit) {
-            uint32_t slot = cur & ~kLeafBit;
-            float4 a = geometry[3 * slot + 0];
-            float4 b = geometry[3 * slot + 1];
-            float4 c = geometry[3 * slot + 2];
-            float t, u, v;
-            if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < maxDistance) {
-                uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
-                if (!cutAway<Full>(s, material, tris, wbits(a), u, v)) {
-                    stack.truncate(base);
-                    return true;
-                }
-            }
-        } else {
-            const BvhNode& n = bvh.nodes[cur];
-            float tl, tr;
-            bool hl = slab(r, n.lminx, n.lminy, n.lminz, n.lmaxx, n.lmaxy, n.lmaxz, maxDistance, tl);
-            bool hr = slab(r, n.rminx, n.rminy, n.rminz, n.rmaxx, n.rmaxy, n.rmaxz, maxDistance, tr);
-            if (hl && hr) {
-                bool leftNear = tl < tr;
-                stack.push(leftNear ? n.right : n.left, leftNear ? tr : tl);
-                cur = leftNear ? n.left : n.right;
-                continue;
-            }
-            if (hl) {
-                cur = n.left;
-                continue;
-            }
-            if (hr) {
-                cur = n.right;
-                continue;
-            }
-        }
-        if (!popLive(stack, base, maxDistance, cur)) return false;
-    }
-}
-
-// Brute-force closest over original-order triangles (direct instances,
-// static triangles, flat triangles without a BVH).
+// Brute force over original-order triangles (direct instances, static
+// triangles, flat triangles without a BVH).  Returns true when an any-hit
+// query found its blocker.
 template <bool Full>
-WPT_HD void closestBrute(const DevScene& s, const Ray& ray, const TriShade* tris, uint32_t count,
-    uint32_t materialOverride, uint32_t kind, uint32_t rank, uint32_t instance, HitRecord& best) {
+WPT_HD bool queryBrute(const DevScene& s, const Ray& ray, const TriShade* tris, uint32_t count,
+    uint32_t materialOverride, uint32_t kind, uint32_t rank, uint32_t instance, bool any, HitRecord& best) {
     for (uint32_t i = 0; i < count; ++i) {
         const TriShade& tri = tris[i];
         float t, u, v;
-        if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !nearer(t, rank, instance, i, best)) continue;
+        if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !wanted(any, t, rank, instance, i, best))
+            continue;
         uint32_t material = materialOverride == kInvalid ? tri.material : materialOverride;
         if (cutAway<Full>(s, material, tris, i, u, v)) continue;
-        best.t = t;
-        best.u = u;
-        best.v = v;
-        best.kind = kind;
-        best.instance = instance;
-        best.prim = i;
-        best.material = material;
-        best.rank = rank;
-    }
-}
-
-template <bool Full>
-WPT_HD bool anyBrute(const DevScene& s, const Ray& ray, const TriShade* tris, uint32_t count,
-    uint32_t materialOverride, float maxDistance) {
-    for (uint32_t i = 0; i < count; ++i) {
-        const TriShade& tri = tris[i];
-        float t, u, v;
-        if (!intersectTriangle(ray, tri.v0, tri.v1, tri.v2, t, u, v) || !(t < maxDistance)) continue;
-        uint32_t material = materialOverride == kInvalid ? tri.material : materialOverride;
-        if (!cutAway<Full>(s, material, tris, i, u, v)) return true;
+        record(best, t, u, v, kind, instance, i, material, rank);
+        if (any) return true;
     }
     return false;
 }
@@ -346,52 +306,48 @@ WPT_HD Ray toLocal(const InstanceTransform& tr, const Ray& ray) {
     return local;
 }
 
-// Sphere leaf: nearer root wins; on an exact tie the lower sphere index wins
-// (reference device_traversal.cuh:789-800).
-WPT_HD void sphereLeaf(const DevScene& s, const Ray& ray, uint32_t index, HitRecord& best) {
+// Sphere candidate: nearer root wins; on an exact tie the lower sphere index
+// wins (reference device_traversal.cuh:789-800).  Returns true when an
+// any-hit query found its blocker.
+WPT_HD bool querySphere(const DevScene& s, const Ray& ray, uint32_t index, bool any, HitRecord& best) {
     float4 sp = s.spheres[index];
     float t;
-    if (!intersectSphere(ray, xyz(sp), sp.w, t) || !nearer(t, kRankSphere, kInvalid, index, best)) return;
-    best.t = t;
-    best.u = best.v = 0.0f;
-    best.kind = kHitSphere;
-    best.instance = kInvalid;
-    best.prim = index;
-    best.material = s.sphereMaterial[index];
-    best.rank = kRankSphere;
+    if (!intersectSphere(ray, xyz(sp), sp.w, t) || !wanted(any, t, kRankSphere, kInvalid, index, best)) return false;
+    record(best, t, 0.0f, 0.0f, kHitSphere, kInvalid, index, s.sphereMaterial[index], kRankSphere);
+    return any;
 }
 
-template <bool Full, class Stack>
-WPT_HD void instanceClosest(const DevScene& s, const Ray& ray, uint32_t index, bool useBvh, HitRecord& best,
-    Stack& stack) {
-    const Instance& inst = s.instances[index];
-    const Blas& blas = s.blases[inst.mesh];
-    Ray local = toLocal(inst.transform, ray);
-    if (useBvh && blas.bvh.primCount != 0) {
-        RayInv ri = makeRayInv(local);
-        closestInBvh<Full>(s, ri, blas.bvh, blas.geometry, blas.triangles, inst.materialOverride, kHitInstance,
-            useBvh ? kRankTlasInstance : kRankDirectInstance, index, best, stack);
-    } else {
-        closestBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride, kHitInstance,
-            useBvh ? kRankTlasInstance : kRankDirectInstance, index, best);
-    }
-}
-
-// Closest hit in the reference's category order: direct spheres, direct
-// instances, static triangles, TLAS, flat triangles.
+// One query in the reference's category order: direct spheres, direct
+// instances, static triangles, TLAS, flat triangles (traceRay /
+// isOccluded, device_traversal.cuh:1035-1184).  `maxDistance` bounds
+// any-hit queries.  The result's kind is kHitNone when nothing was found;
+// for an any-hit query any other kind means the ray is blocked.
 //
 // The TLAS and the BLASes of its instance leaves are walked by one loop: on
 // an instance leaf whose BLAS root box is hit, the loop switches its node
 // array, ray and stack floor to the BLAS; when the BLAS level is exhausted
 // it switches back and resumes popping the TLAS.  The visit order is that of
 // the nested walk (each BLAS finishes before the TLAS resumes), but lanes at
-// different levels now run the same node-visit code in the same iteration.
-template <bool Full = true, class Stack>
-WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
+// different levels -- and, for kEither, lanes running different kinds of
+// query -- run the same node-visit code in the same iteration.
+template <bool Full, Query Q, class Stack>
+WPT_HD HitRecord traceQuery(const DevScene& s, const Ray& ray, float maxDistance, bool anyHit, Stack& stack) {
+    const bool any = Q == Query::kAny || (Q == Query::kEither && anyHit);
     HitRecord best = emptyHit();
-    for (uint32_t i = 0; i < s.directSphereCount; ++i) sphereLeaf(s, ray, s.directSpheres[i], best);
-    for (uint32_t i = 0; i < s.directInstanceCount; ++i) instanceClosest<Full>(s, ray, s.directInstances[i], false, best, stack);
-    closestBrute<Full>(s, ray, s.staticTriangles, s.staticCount, kInvalid, kHitStatic, kRankStatic, kInvalid, best);
+    if (any) best.t = maxDistance;
+    for (uint32_t i = 0; i < s.directSphereCount; ++i)
+        if (querySphere(s, ray, s.directSpheres[i], any, best)) return best;
+    for (uint32_t i = 0; i < s.directInstanceCount; ++i) {
+        uint32_t index = s.directInstances[i];
+        const Instance& inst = s.instances[index];
+        const Blas& blas = s.blases[inst.mesh];
+        if (queryBrute<Full>(s, toLocal(inst.transform, ray), blas.triangles, blas.triangleCount, inst.materialOverride,
+                kHitInstance, kRankDirectInstance, index, any, best))
+            return best;
+    }
+    if (queryBrute<Full>(s, ray, s.staticTriangles, s.staticCount, kInvalid, kHitStatic, kRankStatic, kInvalid, any,
+            best))
+        return best;
     if (s.tlas.primCount != 0) {
         RayInv r = makeRayInv(ray);
         float nearT;
@@ -413,24 +369,24 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
                         float4 c = geometry[3 * slot + 2];
                         float t, u, v;
                         if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) &&
-                            nearer(t, kRankTlasInstance, instance, wbits(a), best)) {
+                            wanted(any, t, kRankTlasInstance, instance, wbits(a), best)) {
                             uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
                             if (s.materialCutout[material] == kCutNever ||
                                 !cutAway<Full>(s, material, s.blases[s.instances[instance].mesh].triangles, wbits(a), u, v)) {
-                                best.t = t;
-                                best.u = u;
-                                best.v = v;
-                                best.kind = kHitInstance;
-                                best.instance = instance;
-                                best.prim = wbits(a);
-                                best.material = material;
-                                best.rank = kRankTlasInstance;
+                                record(best, t, u, v, kHitInstance, instance, wbits(a), material, kRankTlasInstance);
+                                if (any) {
+                                    stack.truncate(base);
+                                    return best;
+                                }
                             }
                         }
                     } else {
                         uint32_t leaf = s.tlasLeaves[slot];
                         if (leaf & kTlasSphereBit) {
-                            sphereLeaf(s, ray, leaf & ~kTlasSphereBit, best);
+                            if (querySphere(s, ray, leaf & ~kTlasSphereBit, any, best)) {
+                                stack.truncate(base);
+                                return best;
+                            }
                         } else {
                             const Instance& inst = s.instances[leaf];
                             const Blas& blas = s.blases[inst.mesh];
@@ -448,9 +404,10 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
                                     cur = blas.bvh.root;
                                     continue;
                                 }
-                            } else {
-                                closestBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride,
-                                    kHitInstance, kRankTlasInstance, leaf, best);
+                            } else if (queryBrute<Full>(s, local, blas.triangles, blas.triangleCount,
+                                           inst.materialOverride, kHitInstance, kRankTlasInstance, leaf, any, best)) {
+                                stack.truncate(base);
+                                return best;
                             }
                         }
                     }
@@ -487,127 +444,24 @@ WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
     if (s.flatCount != 0) {
         if (s.flat.primCount != 0) {
             RayInv r = makeRayInv(ray);
-            closestInBvh<Full>(s, r, s.flat, s.flatGeometry, s.flatTriangles, kInvalid, kHitFlat, kRankFlat, kInvalid,
+            queryBvh<Full>(s, r, s.flat, s.flatGeometry, s.flatTriangles, kInvalid, kHitFlat, kRankFlat, kInvalid, any,
                 best, stack);
         } else {
-            closestBrute<Full>(s, ray, s.flatTriangles, s.flatCount, kInvalid, kHitFlat, kRankFlat, kInvalid, best);
+            queryBrute<Full>(s, ray, s.flatTriangles, s.flatCount, kInvalid, kHitFlat, kRankFlat, kInvalid, any, best);
         }
     }
     return best;
 }
 
+template <bool Full = true, class Stack>
+WPT_HD HitRecord traceClosest(const DevScene& s, const Ray& ray, Stack& stack) {
+    return traceQuery<Full, Query::kClosest>(s, ray, kNoHit, false, stack);
+}
+
 // Shadow query: any blocker with t < maxDistance.
 template <bool Full = true, class Stack>
 WPT_HD bool traceOccluded(const DevScene& s, const Ray& ray, float maxDistance, Stack& stack) {
-    for (uint32_t i = 0; i < s.directSphereCount; ++i) {
-        float4 sp = s.spheres[s.directSpheres[i]];
-        float t;
-        if (intersectSphere(ray, xyz(sp), sp.w, t) && t < maxDistance) return true;
-    }
-    for (uint32_t i = 0; i < s.directInstanceCount; ++i) {
-        const Instance& inst = s.instances[s.directInstances[i]];
-        const Blas& blas = s.blases[inst.mesh];
-        Ray local = toLocal(inst.transform, ray);
-        if (anyBrute<Full>(s, local, blas.triangles, blas.triangleCount, inst.materialOverride, maxDistance))
-            return true;
-    }
-    if (anyBrute<Full>(s, ray, s.staticTriangles, s.staticCount, kInvalid, maxDistance)) return true;
-    if (s.tlas.primCount != 0) {
-        RayInv r = makeRayInv(ray);
-        float nearT;
-        if (slab(r, s.tlas.rootMin[0], s.tlas.rootMin[1], s.tlas.rootMin[2], s.tlas.rootMax[0], s.tlas.rootMax[1],
-                s.tlas.rootMax[2], maxDistance, nearT)) {
-            // One loop over the TLAS and its instances' BLASes, as in
-            // traceClosest; any blocker ends the query.
-            const uint32_t base = stack.size;
-            uint32_t blasBase = base;
-            uint32_t cur = s.tlas.root;
-            const BvhNode* nodes = s.tlas.nodes;
-            const float4* geometry = nullptr;
-            uint32_t instance = kInvalid;
-            uint32_t materialOverride = kInvalid;
-            while (true) {
-                if (cur & kLeafBit) {
-                    uint32_t slot = cur & ~kLeafBit;
-                    if (instance != kInvalid) {
-                        float4 a = geometry[3 * slot + 0];
-                        float4 b = geometry[3 * slot + 1];
-                        float4 c = geometry[3 * slot + 2];
-                        float t, u, v;
-                        if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) && t < maxDistance) {
-                            uint32_t material = materialOverride == kInvalid ? wbits(b) : materialOverride;
-                            if (s.materialCutout[material] == kCutNever ||
-                                !cutAway<Full>(s, material, s.blases[s.instances[instance].mesh].triangles, wbits(a), u, v)) {
-                                stack.truncate(base);
-                                return true;
-                            }
-                        }
-                    } else {
-                        uint32_t leaf = s.tlasLeaves[slot];
-                        if (leaf & kTlasSphereBit) {
-                            float4 sp = s.spheres[leaf & ~kTlasSphereBit];
-                            float t;
-                            if (intersectSphere(ray, xyz(sp), sp.w, t) && t < maxDistance) {
-                                stack.truncate(base);
-                                return true;
-                            }
-                        } else {
-                            const Instance& inst = s.instances[leaf];
-                            const Blas& blas = s.blases[inst.mesh];
-                            RayInv rl = makeRayInv(toLocal(inst.transform, ray));
-                            if (slab(rl, blas.bvh.rootMin[0], blas.bvh.rootMin[1], blas.bvh.rootMin[2],
-                                    blas.bvh.rootMax[0], blas.bvh.rootMax[1], blas.bvh.rootMax[2], maxDistance,
-                                    nearT)) {
-                                r = rl;
-                                instance = leaf;
-                                nodes = blas.bvh.nodes;
-                                geometry = blas.geometry;
-                                materialOverride = inst.materialOverride;
-                                blasBase = stack.size;
-                                cur = blas.bvh.root;
-                                continue;
-                            }
-                        }
-                    }
-                } else {
-                    const BvhNode& n = nodes[cur];
-                    float tl, tr;
-                    bool hl = slab(r, n.lminx, n.lminy, n.lminz, n.lmaxx, n.lmaxy, n.lmaxz, maxDistance, tl);
-                    bool hr = slab(r, n.rminx, n.rminy, n.rminz, n.rmaxx, n.rmaxy, n.rmaxz, maxDistance, tr);
-                    if (hl && hr) {
-                        bool leftNear = tl < tr;
-                        stack.push(leftNear ? n.right : n.left, leftNear ? tr : tl);
-                        cur = leftNear ? n.left : n.right;
-                        continue;
-                    }
-                    if (hl) {
-                        cur = n.left;
-                        continue;
-                    }
-                    if (hr) {
-                        cur = n.right;
-                        continue;
-                    }
-                }
-                if (instance != kInvalid) {
-                    if (popLive(stack, blasBase, maxDistance, cur)) continue;
-                    instance = kInvalid;
-                    nodes = s.tlas.nodes;
-                    r = makeRayInv(ray);
-                }
-                if (!popLive(stack, base, maxDistance, cur)) break;
-            }
-        }
-    }
-    if (s.flatCount != 0) {
-        if (s.flat.primCount != 0) {
-            RayInv r = makeRayInv(ray);
-            if (anyInBvh<Full>(s, r, s.flat, s.flatGeometry, s.flatTriangles, kInvalid, maxDistance, stack)) return true;
-        } else if (anyBrute<Full>(s, ray, s.flatTriangles, s.flatCount, kInvalid, maxDistance)) {
-            return true;
-        }
-    }
-    return false;
+    return traceQuery<Full, Query::kAny>(s, ray, maxDistance, true, stack).kind != kHitNone;
 }
 
 }  // namespace wpt

@@ -196,6 +196,10 @@ hipError_t enqueuePhotonMap(const DevScene& scene, const PhotonSettings& setting
 // consumer on `blocks` x `blockSize` threads.
 hipError_t enqueueBatch(const BatchContext& ctx, uint32_t blocks, uint32_t blockSize, hipStream_t stream);
 hipError_t batchOccupancy(uint32_t blockSize, bool full, int* blocksPerCu);
+// Compiled resources (registers, scratch, static LDS) of the batch kernel and
+// of the staged trace or cooperative persistent kernel.
+hipError_t batchAttributes(bool full, hipFuncAttributes* out);
+hipError_t kernelAttributes(bool persistent, bool full, hipFuncAttributes* out);
 
 // Standalone traversal for the C ABI; `spill` holds
 // (kMaxStackDepth - kLdsStackEntries) x abiTraceThreads() entries.

@@ -551,6 +551,14 @@ hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out) {
     return e;
 }
 
+hipError_t kernelAttributes(bool persistent, bool full, hipFuncAttributes* out) {
+    const void* k = persistent ? (full ? reinterpret_cast<const void*>(kPersistent<true>)
+                                       : reinterpret_cast<const void*>(kPersistent<false>))
+                               : (full ? reinterpret_cast<const void*>(kTrace<true>)
+                                       : reinterpret_cast<const void*>(kTrace<false>));
+    return hipFuncGetAttributes(out, k);
+}
+
 hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream) {
     hipError_t e = launchPersistent(ctx, stream);
     if (e != hipSuccess) return e;

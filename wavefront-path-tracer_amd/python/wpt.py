@@ -203,6 +203,12 @@ class Stats(C.Structure):
                 ("caustic_build_milliseconds", C.c_double)]
 
 
+class LaunchInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("mode", "block_size", "blocks", "blocks_per_cu", "compute_units",
+                                          "registers", "scratch_bytes", "lds_bytes_per_block", "lds_bytes_per_cu",
+                                          "waves_per_simd")]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [(name, C.c_uint32) for name in (
         "sphere_count", "direct_sphere_count", "instance_count", "direct_instance_count", "tlas_leaf_count",
@@ -285,8 +291,10 @@ def library() -> C.CDLL:
         "wpt_render_task": ([P, C.POINTER(Task), P], C.c_int),
         "wpt_renderer_last_stats": ([P, C.POINTER(Stats)], C.c_int),
         "wpt_renderer_profile_kernels": ([P, u32, u32, P, P], C.c_int),
+        "wpt_renderer_launch_info": ([P, C.POINTER(LaunchInfo)], C.c_int),
         "wpt_batch_create": ([C.POINTER(P), u32, C.POINTER(BatchConfig), C.POINTER(P)], C.c_int),
         "wpt_batch_destroy": ([P], C.c_int),
+        "wpt_batch_set_caustics": ([P, u32, C.POINTER(RenderConfig), C.POINTER(u32), C.POINTER(C.c_double)], C.c_int),
         "wpt_batch_render": ([P, C.POINTER(BatchTask), u32, C.POINTER(P), P], C.c_int),
         "wpt_batch_last_stats": ([P, C.POINTER(BatchStats)], C.c_int),
         "wpt_batch_autotune": ([P, C.POINTER(BatchTask), u32, C.POINTER(BatchConfig)], C.c_int),
@@ -721,6 +729,12 @@ class Renderer:
               "wpt_renderer_profile_kernels")
         return ms, launches
 
+    def launch_info(self) -> dict:
+        """Launch shape and compiled resources of the dominant kernel."""
+        info = LaunchInfo()
+        check(library().wpt_renderer_launch_info(self._h, C.byref(info)), "wpt_renderer_launch_info")
+        return {n: int(getattr(info, n)) for n, _ in LaunchInfo._fields_}
+
     def stats(self) -> Stats:
         s = Stats()
         check(library().wpt_renderer_last_stats(self._h, C.byref(s)), "wpt_renderer_last_stats")
@@ -755,6 +769,16 @@ class Batch:
         cfg = BatchConfig(block_size, blocks)
         check(library().wpt_batch_create(handles, len(self._scenes), C.byref(cfg), C.byref(self._h)),
               "wpt_batch_create")
+
+    def set_caustics(self, scene_index: int, config: RenderConfig):
+        """Builds scene `scene_index`'s caustic photon map from the caustic
+        fields of a render configuration (wpt_batch_set_caustics); returns
+        (photons stored, build ms)."""
+        stored = C.c_uint32()
+        ms = C.c_double()
+        check(library().wpt_batch_set_caustics(self._h, scene_index, C.byref(config), C.byref(stored), C.byref(ms)),
+              "wpt_batch_set_caustics")
+        return int(stored.value), float(ms.value)
 
     @staticmethod
     def _tasks(tasks):
