@@ -473,6 +473,8 @@ def main():
     ap.add_argument("--spp", type=int, default=SPP)
     ap.add_argument("--mode", choices=["regenerate", "graph", "persistent"], default="regenerate",
                     help="scheduler of the timed frames; the other two are timed beside it")
+    ap.add_argument("--samples-per-pass", type=int, default=0,
+                    help="samples per pass of the staged and persistent schedulers (0: the library's default)")
     ap.add_argument("--skip-persistent", action="store_true", help="do not time the other schedulers beside")
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--cpu-spp", type=int, default=8, help="samples per pixel of the CPU baseline sample")
@@ -528,7 +530,8 @@ def main():
     camera = host.camera(WIDTH, HEIGHT)
     modes = {"regenerate": wpt.MODE_REGENERATE, "graph": wpt.MODE_GRAPH, "persistent": wpt.MODE_PERSISTENT}
     mode = modes[args.mode]
-    cfg = wpt.render_config(WIDTH, HEIGHT, spp=args.spp, max_depth=DEPTH, rr_start=RR_START, clamp=CLAMP, mode=mode)
+    cfg = wpt.render_config(WIDTH, HEIGHT, spp=args.spp, max_depth=DEPTH, rr_start=RR_START, clamp=CLAMP, mode=mode,
+                            samples_per_pass=args.samples_per_pass)
     renderer = wpt.Renderer(scene, camera, cfg)
     launch = renderer.launch_info()
     log(f"launch: {launch}")
@@ -580,7 +583,7 @@ def main():
             if m == mode:
                 continue
             ocfg = wpt.render_config(WIDTH, HEIGHT, spp=args.spp, max_depth=DEPTH, rr_start=RR_START, clamp=CLAMP,
-                                     mode=m)
+                                     mode=m, samples_per_pass=args.samples_per_pass)
             orr = wpt.Renderer(scene, camera, ocfg)
             orr.reset()
             orr.render_samples(0, args.spp)

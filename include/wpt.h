@@ -26,7 +26,7 @@ extern "C" {
 
 #define WPT_API __attribute__((visibility("default")))
 
-#define WPT_ABI_VERSION 1
+#define WPT_ABI_VERSION 2
 
 /* Status codes. */
 #define WPT_OK 0
@@ -260,10 +260,22 @@ typedef struct wpt_scene_desc {
      * device builder needs a device scene (wpt_scene_create). */
     uint32_t bvh_builder;
     uint32_t bvh_search_radius;
+    /* Accelerator of flat_triangles (built when the scene has no meshes;
+     * reference ObjAccelerationPolicy, scene.cu:241-294): WPT_FLAT_*. */
+    uint32_t flat_accelerator;
 } wpt_scene_desc;
 
 #define WPT_BVH_BUILDER_HOST_SAH 0u
 #define WPT_BVH_BUILDER_DEVICE_PLOC 1u
+
+/* Flat-triangle accelerators.  Every one gives the same hits (exact
+ * distance ties are ordered by category, instance and primitive). */
+#define WPT_FLAT_SAH_BVH2 0u           /* binned SAH, one triangle per leaf (the nexus_bvh2 policy's tree kind) */
+#define WPT_FLAT_MEDIAN_BVH2 1u        /* object median on the longest centroid axis (median_bvh, bvh.cpp:49-109) */
+#define WPT_FLAT_SPATIAL_SPLIT_BVH2 2u /* spatial splits, triangles referenced on both sides of a cut
+                                          (spatial_split_bvh, bvh_sbvh.cpp:94-219) */
+#define WPT_FLAT_WIDE8 3u              /* eight-wide nodes collapsed from the SAH tree (nexus_bvh8,
+                                          device_traversal.cuh:893-963) */
 
 /* Reference Camera (include/scene/camera.cuh:7-27). */
 typedef struct wpt_camera {
@@ -308,6 +320,11 @@ typedef struct wpt_render_config {
     uint32_t caustic_grid_resolution;
     float caustic_grid_min[3];
     float caustic_grid_max[3];
+    /* Samples the staged and cooperative persistent schedulers render per
+     * pass of a tile (0: 8; at most 32; never more than samples_per_pixel).
+     * Each pixel still adds its samples to the framebuffer in sample order,
+     * so images do not depend on it. */
+    uint32_t samples_per_pass;
 } wpt_render_config;
 
 /* Reference RenderTask / RenderTaskId / DistributedTile
@@ -378,6 +395,10 @@ typedef struct wpt_scene_info {
     uint32_t bvh_builder;           /* WPT_BVH_BUILDER_* of the mesh BLASes */
     uint32_t max_blas_depth;        /* longest root-to-leaf path of any BLAS */
     double blas_build_milliseconds; /* BLAS construction (host wall or device) */
+    uint32_t flat_accelerator;      /* WPT_FLAT_* of the flat-triangle tree */
+    uint32_t flat_leaf_slots;       /* leaf slots (above the triangle count after spatial splits) */
+    uint32_t flat_wide_node_count;  /* eight-wide nodes (WPT_FLAT_WIDE8) */
+    uint32_t max_stack_depth;       /* deepest traversal stack any query can need */
 } wpt_scene_info;
 WPT_API int wpt_scene_info_get(const wpt_scene* scene, wpt_scene_info* info);
 /* which: 0 direct spheres, 1 direct instances, 2 TLAS leaves packed as
@@ -462,8 +483,8 @@ WPT_API int wpt_renderer_last_stats(const wpt_renderer* renderer, wpt_stats* sta
  * HIP events on the renderer's stream (no graph), accumulating device time
  * per kernel into kernel_ms[7] and launch counts into launches[7]: 0 primary
  * rays, 1 queue reset, 2 closest-hit trace, 3 shade, 4 persistent, 5 shadow
- * rays, 6 queue compaction (unused by the renderer, whose shade kernel
- * appends continuations itself). */
+ * rays, 6 pass resolve (a pass's samples added to the framebuffer in
+ * sample order). */
 WPT_API int wpt_renderer_profile_kernels(wpt_renderer* renderer, uint32_t first_sample, uint32_t count,
                                          double* kernel_ms, uint32_t* launches);
 

@@ -29,7 +29,7 @@ def test_every_declared_symbol_is_exported(lib):
     assert len(names) >= 30
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.wpt_abi_version() == 1
+    assert lib.wpt_abi_version() == 2
 
 
 def test_device_count_without_gpu_does_not_fail(lib):

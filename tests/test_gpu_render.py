@@ -151,3 +151,33 @@ def test_closed_form_scenes(torch_cuda, oracle_mod, case):
         assert np.array_equal(mean.reshape(-1, 3), np.tile(want, (32 * 32, 1)))
     else:
         assert np.all(np.abs(mean.mean(axis=(0, 1)) - 0.6) < 0.004)
+
+
+@pytest.mark.parametrize("mode", [wpt.MODE_GRAPH, wpt.MODE_PERSISTENT])
+def test_samples_per_pass_leave_images_unchanged(torch_cuda, oracle_mod, mode):
+    # Passes of 1, 3, 8 and 32 samples (partial last passes included), tiled
+    # and whole-frame: each pixel still adds its samples in sample order, so
+    # every image equals the oracle's; tasks over sample ranges agree too.
+    host = wpt.HostScene("cornell-box")
+    scene = wpt.Scene(host.desc(), device=True, owner=host)
+    cam = host.camera(64, 48)
+    want = None
+    for per_pass in (1, 3, 8, 32):
+        for tile in (None, (24, 20)):
+            cfg = wpt.render_config(64, 48, spp=7, max_depth=8, rr_start=3, mode=mode, tile=tile,
+                                    samples_per_pass=per_pass)
+            r = wpt.Renderer(scene, cam, cfg)
+            img, n = r.render_frame()
+            st = r.stats()
+            part = r.render_task(wpt.Task(0, 0, 2, 5, 8, 4, 24, 20))
+            r.close()
+            if want is None:
+                want, ost = oracle_mod.Oracle(scene).render(cam, cfg)
+                fb = np.zeros((48, 64, 3), np.float32)
+                task_want, _ = oracle_mod.Oracle(scene).render(cam, cfg, tile=(8, 4, 24, 20), sample_start=2,
+                                                               sample_count=5, framebuffer=fb)
+                task_want = task_want[4:24, 8:32]
+            assert n == 7
+            assert np.array_equal(img, want), f"{per_pass} per pass, tile {tile}: max |diff| {np.abs(img - want).max()}"
+            assert (st.closest_queries, st.shadow_queries) == (ost.closest_queries, ost.shadow_queries)
+            assert np.array_equal(part, task_want)

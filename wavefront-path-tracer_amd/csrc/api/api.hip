@@ -125,7 +125,9 @@ struct wpt_renderer {
     hipEvent_t start = nullptr, end = nullptr;
     LaunchContext ctx{};
     DeviceArena arena;
-    uint32_t currentTile = 0xffffffffu;
+    RenderTile uploaded{};  // tile and pass size last copied to the device
+    bool tileValid = false;
+    uint32_t tileCapacity = 0;  // pixels of the largest tile
     double lastMs = 0.0;
     uint64_t lastSamples = 0;
     double photonMs = 0.0;       // caustic photon map build
@@ -181,13 +183,17 @@ int validateConfig(const wpt_render_config& c) {
     uint64_t cap = c.tiled ? static_cast<uint64_t>(c.tile_width) * c.tile_height : pixels;
     if (pixels > std::numeric_limits<uint32_t>::max() || cap >= (1ull << 30))
         return fail(WPT_ERROR_INVALID_ARGUMENT, "Render dimensions exceed the supported queue index range");
+    // Staged and persistent paths keep their sample slot above the pixel index.
+    if (c.mode != WPT_MODE_REGENERATE && pixels > kPixelMask + 1ull)
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "Render dimensions exceed 2^26 pixels for the wavefront schedulers");
     return WPT_OK;
 }
 
 std::vector<RenderTile> makeTiles(uint32_t width, uint32_t height, uint32_t tw, uint32_t th) {
     std::vector<RenderTile> tiles;
     for (uint32_t y = 0; y < height; y += th)
-        for (uint32_t x = 0; x < width; x += tw) tiles.push_back(RenderTile{x, y, std::min(tw, width - x), std::min(th, height - y), 0});
+        for (uint32_t x = 0; x < width; x += tw)
+            tiles.push_back(RenderTile{x, y, std::min(tw, width - x), std::min(th, height - y), 0u, 1u});
     return tiles;
 }
 
@@ -207,11 +213,16 @@ int ensureGraph(wpt_renderer* r) {
     return WPT_OK;
 }
 
-int launchTileSample(wpt_renderer* r, uint32_t tileIndex, const RenderTile& tile) {
-    if (r->currentTile != tileIndex || r->tiles.size() != 1) {
+// One pass (tile.samples samples) of a tile by the staged or persistent
+// scheduler; the tile descriptor is copied to the device when it changes.
+int launchTilePass(wpt_renderer* r, const RenderTile& tile) {
+    const RenderTile& u = r->uploaded;
+    if (!r->tileValid || u.x != tile.x || u.y != tile.y || u.width != tile.width || u.height != tile.height ||
+        u.advanceSample != tile.advanceSample || u.samples != tile.samples) {
         hipLaunchKernelGGL(kSetTile, dim3(1), dim3(1), 0, r->stream, const_cast<RenderTile*>(r->ctx.tile), tile);
         WPT_HIP(hipGetLastError());
-        r->currentTile = tileIndex;
+        r->uploaded = tile;
+        r->tileValid = true;
     }
     if (r->config.mode == WPT_MODE_GRAPH) {
         WPT_HIP(hipGraphLaunch(r->exec, r->stream));
@@ -363,6 +374,7 @@ static int createScene(const wpt_scene_desc* desc, wpt_scene** out, bool upload)
         d.blasCount = static_cast<uint32_t>(blases.size());
         check(a.upload(h.flat.nodes, &nodes), "flat nodes");
         d.flat = deviceBvhView(h.flat, nodes);
+        check(a.upload(h.flatWide, &d.flatWide), "flat wide nodes");
         check(a.upload(h.flatGeometry, &f4), "flat geometry");
         d.flatGeometry = f4;
         check(a.upload(h.flatTriangles, &d.flatTriangles), "flat triangles");
@@ -444,6 +456,10 @@ WPT_API int wpt_scene_info_get(const wpt_scene* s, wpt_scene_info* info) {
     info->static_triangle_count = static_cast<uint32_t>(h.staticTriangles.size());
     info->flat_triangle_count = static_cast<uint32_t>(h.flatTriangles.size());
     info->flat_node_count = h.flat.primCount() ? 2 * h.flat.primCount() - 1 : 0;
+    info->flat_accelerator = h.flatAccelerator;
+    info->flat_leaf_slots = h.flat.primCount();
+    info->flat_wide_node_count = static_cast<uint32_t>(h.flatWide.size());
+    info->max_stack_depth = h.maxStackDepth;
     info->material_count = static_cast<uint32_t>(h.materials.size());
     return WPT_OK;
 }
@@ -742,8 +758,18 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     r->config = *config;
     r->fingerprint = fingerprint;
     const wpt_render_config& c = r->config;
-    uint32_t cap = c.tiled ? c.tile_width * c.tile_height : c.width * c.height;
+    uint32_t tilePixels = c.tiled ? c.tile_width * c.tile_height : c.width * c.height;
+    // Samples per pass of the staged and cooperative schedulers.
+    uint32_t slots = c.mode == WPT_MODE_REGENERATE ? 1u : (c.samples_per_pass ? c.samples_per_pass : 8u);
+    slots = std::max(1u, std::min(std::min(slots, kMaxSlots), c.samples_per_pixel));
+    uint64_t cap64 = static_cast<uint64_t>(tilePixels) * slots;
+    if (cap64 >= (1ull << 31)) {
+        delete r;
+        return fail(WPT_ERROR_INVALID_ARGUMENT, "tile pixels x samples per pass out of range");
+    }
+    uint32_t cap = static_cast<uint32_t>(cap64);
     r->capacity = cap;
+    r->tileCapacity = tilePixels;
     r->tiles = c.tiled ? makeTiles(c.width, c.height, c.tile_width, c.tile_height)
                        : makeTiles(c.width, c.height, c.width, c.height);
     auto cleanup = [&](int status) {
@@ -784,8 +810,10 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     ctx.height = c.height;
     ctx.blockSize = c.block_size;
     ctx.gridBlocks = (cap + c.block_size - 1) / c.block_size;
+    ctx.resolveBlocks = (tilePixels + c.block_size - 1) / c.block_size;
     ctx.profileQueues = 1;
     ctx.capacity = cap;
+    ctx.slots = slots;
     // Trace, shade and shadow kernels run one item per thread over
     // gridBlocks workgroups; the persistent kernel runs a resident grid.
     Occupancy occ{};
@@ -841,6 +869,10 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     }
     R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height, &p));
     ctx.framebuffer = static_cast<float*>(p);
+    if (c.mode != WPT_MODE_REGENERATE) {
+        R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height * slots, &p));
+        ctx.slotRadiance = static_cast<float*>(p);
+    }
     R_HIP(r->arena.alloc(sizeof(WavefrontCounters), &p));
     ctx.counters = static_cast<WavefrontCounters*>(p);
     R_HIP(hipMemset(p, 0, sizeof(WavefrontCounters)));
@@ -929,11 +961,12 @@ WPT_API int wpt_renderer_render_samples(wpt_renderer* r, uint32_t first, uint32_
     hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, first);
     WPT_HIP(hipGetLastError());
     WPT_HIP(hipEventRecord(r->start, r->stream));
-    for (uint32_t s = 0; s < count; ++s) {
+    for (uint32_t s = 0; s < count; s += r->ctx.slots) {
         for (uint32_t t = 0; t < r->tiles.size(); ++t) {
             RenderTile tile = r->tiles[t];
             tile.advanceSample = t + 1 == r->tiles.size() ? 1u : 0u;
-            status = launchTileSample(r, t, tile);
+            tile.samples = std::min(r->ctx.slots, count - s);
+            status = launchTilePass(r, tile);
             if (status != WPT_OK) return status;
         }
     }
@@ -986,12 +1019,12 @@ WPT_API int wpt_render_task(wpt_renderer* r, const wpt_task* task, float* hostTi
         return fail(WPT_ERROR_INVALID_TASK, "Render task fingerprint does not match the prepared renderer");
     if (task->width == 0 || task->height == 0 || task->x >= c.width || task->y >= c.height ||
         task->width > c.width - task->x || task->height > c.height - task->y ||
-        static_cast<uint64_t>(task->width) * task->height > r->capacity)
+        static_cast<uint64_t>(task->width) * task->height > r->tileCapacity)
         return fail(WPT_ERROR_INVALID_TASK, "Render task tile is outside the prepared renderer bounds");
     if (task->sample_count == 0 || task->sample_start > c.samples_per_pixel ||
         task->sample_count > c.samples_per_pixel - task->sample_start)
         return fail(WPT_ERROR_INVALID_TASK, "Render task sample range is outside the prepared renderer bounds");
-    RenderTile tile{task->x, task->y, task->width, task->height, 1u};
+    RenderTile tile{task->x, task->y, task->width, task->height, 1u, 1u};
     WPT_HIP(hipEventRecord(r->start, r->stream));
     if (c.mode == WPT_MODE_REGENERATE) {
         // Sums written over the tile: the same values as clearing it first.
@@ -1006,12 +1039,11 @@ WPT_API int wpt_render_task(wpt_renderer* r, const wpt_task* task, float* hostTi
         hipLaunchKernelGGL(kClearStats, dim3(1), dim3(1), 0, r->stream, r->ctx.counters);
         hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, task->sample_start);
         WPT_HIP(hipGetLastError());
-        r->currentTile = 0xffffffffu;  // no tile uploaded: the first sample uploads the task's tile
-        for (uint32_t s = 0; s < task->sample_count; ++s) {
-            status = launchTileSample(r, 0xfffffffeu, tile);
+        for (uint32_t s = 0; s < task->sample_count; s += r->ctx.slots) {
+            tile.samples = std::min(r->ctx.slots, task->sample_count - s);
+            status = launchTilePass(r, tile);
             if (status != WPT_OK) return status;
         }
-        r->currentTile = 0xffffffffu;
     }
     WPT_HIP(hipEventRecord(r->end, r->stream));
     WPT_HIP(hipStreamSynchronize(r->stream));
@@ -1041,16 +1073,17 @@ WPT_API int wpt_renderer_profile_kernels(wpt_renderer* r, uint32_t first, uint32
     hipLaunchKernelGGL(kSetSample, dim3(1), dim3(1), 0, r->stream, r->ctx.counters, first);
     WPT_HIP(hipGetLastError());
     bool persistent = r->config.mode == WPT_MODE_PERSISTENT;
-    for (uint32_t s = 0; s < count; ++s) {
+    for (uint32_t s = 0; s < count; s += r->ctx.slots) {
         for (uint32_t t = 0; t < r->tiles.size(); ++t) {
             RenderTile tile = r->tiles[t];
             tile.advanceSample = t + 1 == r->tiles.size() ? 1u : 0u;
+            tile.samples = std::min(r->ctx.slots, count - s);
             hipLaunchKernelGGL(kSetTile, dim3(1), dim3(1), 0, r->stream, const_cast<RenderTile*>(r->ctx.tile), tile);
             WPT_HIP(hipGetLastError());
             WPT_HIP(profileSample(r->ctx, r->stream, persistent, kernelMs, launches));
         }
     }
-    r->currentTile = 0xffffffffu;
+    r->tileValid = false;
     return WPT_OK;
 }
 
@@ -1215,7 +1248,7 @@ int stageBatch(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, std::vecto
         d.camera.aspectRatio = t.camera.aspect_ratio;
         d.shade = ShadeParams{t.max_depth, t.rr_start_depth, t.rng_strategy, t.indirect_clamp, 0u, 0u,
             b->photons[t.scene]};
-        d.tile = RenderTile{t.x, t.y, t.tile_width, t.tile_height, 0u};
+        d.tile = RenderTile{t.x, t.y, t.tile_width, t.tile_height, 0u, 1u};
         d.scene = t.scene;
         d.width = t.width;
         d.height = t.height;

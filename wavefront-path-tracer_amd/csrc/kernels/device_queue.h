@@ -65,6 +65,9 @@ struct LdsStack {
         n = e.x;
         t = __uint_as_float(e.y);
     }
+    // Entry k below the top, for k < 8 (always in the LDS ring).
+    __device__ uint2 peek(uint32_t k) const { return ring[((size - 1u - k) & (kLdsStackEntries - 1u)) * stride]; }
+    __device__ void poke(uint32_t k, uint2 e) { ring[((size - 1u - k) & (kLdsStackEntries - 1u)) * stride] = e; }
     __device__ void truncate(uint32_t base) {
         while (size > base) {
             uint32_t n;
@@ -74,6 +77,7 @@ struct LdsStack {
     }
 };
 static_assert((kLdsStackEntries & (kLdsStackEntries - 1u)) == 0, "LDS stack depth must be a power of two");
+static_assert(kLdsStackEntries >= 8, "wide nodes reorder their children within the LDS part of the stack");
 
 __device__ __forceinline__ LdsStack makeStack(uint2* spill, uint32_t spillThreads) {
     LdsStack s;

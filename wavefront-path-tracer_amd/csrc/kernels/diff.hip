@@ -28,7 +28,7 @@ __device__ __forceinline__ void pathPixel(const DiffContext& c, uint32_t i, uint
     tilePixel(frame, i - sample * pixels, x, y);
 }
 
-__global__ void __launch_bounds__(512) kDiffForward(DiffContext c) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) kDiffForward(DiffContext c) {
     LdsStack stack = makeStack(c.spill, c.spillThreads);
     const uint32_t pixels = c.width * c.height;
     const uint32_t count = pixels * c.spp;
@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(512) kDiffForward(DiffContext c) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         uint32_t sample, x, y;
         pathPixel(c, i, sample, x, y);
-        Vec3 L = diffPath<false>(c.scene, c.layout, c.params, c.camera, x, y, c.width, c.height,
+        Vec3 L = diffInlinePath<false>(c.scene, c.layout, c.params, c.camera, x, y, c.width, c.height,
             c.sampleStart + sample, c.rngStrategy, c.maxDepth, stack, zero, zero, none);
         size_t slot = static_cast<size_t>(sample) * pixels + y * c.width + x;
         c.pathRadiance[slot] = L.x;
@@ -92,7 +92,7 @@ struct AdjointSink {
     }
 };
 
-__global__ void __launch_bounds__(512) kDiffAdjoint(DiffContext c) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) kDiffAdjoint(DiffContext c) {
     __shared__ float ldsRoughness[kDiffLdsSlots];
     __shared__ float ldsEmission[kDiffLdsSlots];
     for (uint32_t k = threadIdx.x; k < kDiffLdsSlots; k += blockDim.x) {
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(512) kDiffAdjoint(DiffContext c) {
             c.pathRadiance[2 * static_cast<size_t>(count) + slot]);
         Vec3 delta(c.imageGrad[3u * p] * invSpp, c.imageGrad[3u * p + 1] * invSpp, c.imageGrad[3u * p + 2] * invSpp);
         if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) continue;
-        diffPath<true>(c.scene, c.layout, c.params, c.camera, x, y, c.width, c.height, c.sampleStart + sample,
+        diffInlinePath<true>(c.scene, c.layout, c.params, c.camera, x, y, c.width, c.height, c.sampleStart + sample,
             c.rngStrategy, c.maxDepth, stack, delta, total, sink);
     }
     __syncthreads();

@@ -186,6 +186,12 @@ struct PrivateStack {
         t = tmin[size];
     }
     WPT_HD void truncate(uint32_t base) { size = base; }
+    // Entry k below the top (k < 8).
+    WPT_HD uint2 peek(uint32_t k) const { return make_uint2(node[size - 1 - k], bitsOf(tmin[size - 1 - k])); }
+    WPT_HD void poke(uint32_t k, uint2 e) {
+        node[size - 1 - k] = e.x;
+        tmin[size - 1 - k] = fromBits32(e.y);
+    }
 };
 
 // Pops until an entry still closer than closestT; false when the traversal
@@ -274,6 +280,61 @@ WPT_HD bool queryBvh(const DevScene& s, const RayInv& r, const Bvh& bvh, const f
             if (hr) {
                 cur = n.right;
                 continue;
+            }
+        }
+        if (!popLive(stack, base, best.t, cur)) return false;
+    }
+}
+
+// Triangles under eight-wide nodes (the flat-triangle accelerator
+// WPT_FLAT_WIDE8).  A node's children that the ray reaches are pushed so that
+// the nearest ends on top (insertion from the top of the stack, at most
+// eight entries, all in the stack's LDS part), then the walk pops as for
+// binary nodes.  Returns true when an any-hit query found its blocker.
+template <bool Full, class Stack>
+WPT_HD bool queryWide(const DevScene& s, const RayInv& r, const Bvh& bvh, const WideNode* nodes, const float4* geometry,
+    const TriShade* tris, uint32_t kind, uint32_t rank, bool any, HitRecord& best, Stack& stack) {
+    float nearT;
+    if (!slab(r, bvh.rootMin[0], bvh.rootMin[1], bvh.rootMin[2], bvh.rootMax[0], bvh.rootMax[1], bvh.rootMax[2],
+            best.t, nearT))
+        return false;
+    uint32_t base = stack.size;
+    uint32_t cur = bvh.primCount > 1 ? 0u : (kLeafBit | 0u);
+    while (true) {
+        if (cur & kLeafBit) {
+            uint32_t slot = cur & ~kLeafBit;
+            float4 a = geometry[3 * slot + 0];
+            float4 b = geometry[3 * slot + 1];
+            float4 c = geometry[3 * slot + 2];
+            float t, u, v;
+            if (intersectTriangle(r.ray, xyz(a), xyz(b), xyz(c), t, u, v) &&
+                wanted(any, t, rank, kInvalid, wbits(a), best)) {
+                uint32_t material = wbits(b);
+                if (!cutAway<Full>(s, material, tris, wbits(a), u, v)) {
+                    record(best, t, u, v, kind, kInvalid, wbits(a), material, rank);
+                    if (any) {
+                        stack.truncate(base);
+                        return true;
+                    }
+                }
+            }
+        } else {
+            const WideNode& n = nodes[cur];
+            uint32_t pushed = 0;
+            for (uint32_t k = 0; k < kWideChildren; ++k) {
+                uint32_t child = n.child[k];
+                if (child == kInvalid) break;
+                float t;
+                if (!slab(r, n.minX[k], n.minY[k], n.minZ[k], n.maxX[k], n.maxY[k], n.maxZ[k], best.t, t)) continue;
+                stack.push(child, t);
+                // Keep the pushed entries ordered, nearest on top.
+                for (uint32_t j = 0; j < pushed; ++j) {
+                    uint2 top = stack.peek(j), below = stack.peek(j + 1);
+                    if (!(fromBits32(top.y) > fromBits32(below.y))) break;
+                    stack.poke(j, below);
+                    stack.poke(j + 1, top);
+                }
+                ++pushed;
             }
         }
         if (!popLive(stack, base, best.t, cur)) return false;
@@ -442,7 +503,11 @@ WPT_HD HitRecord traceQuery(const DevScene& s, const Ray& ray, float maxDistance
         }
     }
     if (s.flatCount != 0) {
-        if (s.flat.primCount != 0) {
+        if (s.flatWide) {
+            RayInv r = makeRayInv(ray);
+            queryWide<Full>(s, r, s.flat, s.flatWide, s.flatGeometry, s.flatTriangles, kHitFlat, kRankFlat, any, best,
+                stack);
+        } else if (s.flat.primCount != 0) {
             RayInv r = makeRayInv(ray);
             queryBvh<Full>(s, r, s.flat, s.flatGeometry, s.flatTriangles, kInvalid, kHitFlat, kRankFlat, kInvalid, any,
                 best, stack);

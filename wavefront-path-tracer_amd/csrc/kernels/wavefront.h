@@ -63,9 +63,17 @@ struct HitBuffer {
     uint32_t* material;
 };
 
+// A tile and, for the staged and persistent schedulers, the number of
+// consecutive samples one pass renders together (1 .. LaunchContext.slots).
 struct RenderTile {
     uint32_t x, y, width, height, advanceSample;
+    uint32_t samples;
 };
+
+// Paths of one pass carry their sample slot in the pixel word's top bits.
+constexpr uint32_t kSlotShift = 26;
+constexpr uint32_t kPixelMask = (1u << kSlotShift) - 1u;
+constexpr uint32_t kMaxSlots = 32;
 
 // Device counters shared by the launches of one renderer.
 struct WavefrontCounters {
@@ -95,14 +103,21 @@ struct LaunchContext {
     ShadowQueue shadow;
     HitBuffer hits;
     float* framebuffer;      // width*height*3
+    // Staged and persistent passes render `slots` samples of a tile at once:
+    // a finished path stores its radiance in its sample's plane of
+    // slotRadiance (slots x width*height*3), and the pass ends by adding the
+    // planes to the framebuffer in sample order.
+    float* slotRadiance;
+    uint32_t slots;
     uint32_t width, height;
     WavefrontCounters* counters;
     const RenderTile* tile;  // device copy of the current tile
     uint2* spill;            // (kMaxStackDepth - kLdsStackEntries) x spillThreads, one column per thread index
     uint32_t spillThreads;
-    uint32_t capacity;         // tile pixels: the most items any queue holds
+    uint32_t capacity;         // tile pixels x slots: the most items any queue holds
     uint32_t blockSize;
     uint32_t gridBlocks;       // ceil(capacity / blockSize): one item per thread
+    uint32_t resolveBlocks;    // ceil(tile pixels / blockSize)
     uint32_t persistentBlocks;
     uint32_t profileQueues;
 };
@@ -111,10 +126,12 @@ struct LaunchContext {
 inline size_t stackLdsBytes(uint32_t blockSize) { return sizeof(uint2) * kLdsStackEntries * blockSize; }
 
 // Staged wavefront: records primary + maxDepth x (reset, trace, shade,
-// shadow) + advance on `stream` (captured once into a hipGraph by the
+// shadow) + resolve + advance for one pass of the current tile (its
+// `samples` samples) on `stream` (captured once into a hipGraph by the
 // caller).
 hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream);
-// Persistent cooperative kernel for one tile sample, then advance.
+// Persistent cooperative kernel for one pass of the current tile, then
+// resolve and advance.
 hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream);
 // Resident workgroups per CU of each kernel at `blockSize`.
 struct Occupancy {
@@ -123,10 +140,9 @@ struct Occupancy {
 // Occupancy of the kernels serving scenes with (full) or without
 // environment maps, punctual lights and texture bindings.
 hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out);
-// One sample of the current tile with every launch bracketed by events:
+// One pass of the current tile with every launch bracketed by events:
 // ms/launches accumulate per kernel (0 primary, 1 reset, 2 trace, 3 shade,
-// 4 persistent, 5 shadow, 6 compact: unused by the renderer, whose shade
-// kernel appends continuations itself).  Synchronizes the stream.
+// 4 persistent, 5 shadow, 6 resolve).  Synchronizes the stream.
 constexpr int kProfileSlots = 7;
 hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool persistent, double ms[kProfileSlots],
     uint32_t launches[kProfileSlots]);

@@ -163,17 +163,63 @@ __device__ __forceinline__ void storePath(const PathQueue& q, uint32_t i, const 
     sf(q, kEtaScale, i, p.etaScale);
 }
 
+// Paths of a pass: tile.samples consecutive samples of every tile pixel.
+__device__ __forceinline__ uint32_t passPaths(const LaunchContext& ctx, const RenderTile& tile) {
+    uint32_t count = tile.width * tile.height * tile.samples;
+    return count < ctx.capacity ? count : ctx.capacity;
+}
+
+// Fresh path g of a pass: sample slot g / pixels of tile pixel g % pixels.
+__device__ __forceinline__ PathState passPath(const LaunchContext& ctx, const RenderTile& tile, uint32_t sample,
+    uint32_t g) {
+    uint32_t pixels = tile.width * tile.height;
+    uint32_t slot = g / pixels;
+    uint32_t x, y;
+    tilePixel(tile, g - slot * pixels, x, y);
+    PathState p = primaryPath(ctx.camera, x, y, ctx.width, ctx.height, sample + slot, ctx.shade.rngStrategy);
+    p.pixel |= slot << kSlotShift;
+    return p;
+}
+
+// A finished path's radiance, stored in its sample's plane.
+__device__ __forceinline__ void finishPath(const LaunchContext& ctx, uint32_t tagged, const Vec3& v) {
+    size_t i = 3 * (static_cast<size_t>(tagged >> kSlotShift) * ctx.width * ctx.height + (tagged & kPixelMask));
+    ctx.slotRadiance[i] = v.x;
+    ctx.slotRadiance[i + 1] = v.y;
+    ctx.slotRadiance[i + 2] = v.z;
+}
+
 __global__ void __launch_bounds__(512) kPrimary(LaunchContext ctx) {
     RenderTile tile = *ctx.tile;
-    uint32_t pixels = tile.width * tile.height;
-    uint32_t count = pixels < ctx.capacity ? pixels : ctx.capacity;
+    uint32_t count = passPaths(ctx, tile);
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < kSegments) ctx.counters->count[0][g] = primarySegmentCount(count, ctx.blockSize, g);
     if (g >= count) return;
+    storePath(ctx.queue[0], primarySlot(g, ctx.blockSize, ctx.queue[0].segmentCapacity),
+        passPath(ctx, tile, ctx.counters->sampleIndex, g));
+}
+
+// End of a pass: each tile pixel adds its samples' radiance in sample order
+// (the additions a one-sample-per-pass schedule makes), then the sample
+// counter advances past the pass.
+__global__ void __launch_bounds__(512) kResolve(LaunchContext ctx) {
+    RenderTile tile = *ctx.tile;
+    uint32_t pixels = tile.width * tile.height;
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= pixels) return;
     uint32_t x, y;
     tilePixel(tile, g, x, y);
-    PathState p = primaryPath(ctx.camera, x, y, ctx.width, ctx.height, ctx.counters->sampleIndex, ctx.shade.rngStrategy);
-    storePath(ctx.queue[0], primarySlot(g, ctx.blockSize, ctx.queue[0].segmentCapacity), p);
+    size_t pixel = static_cast<size_t>(y) * ctx.width + x;
+    size_t plane = 3 * static_cast<size_t>(ctx.width) * ctx.height;
+    float* fb = ctx.framebuffer + 3 * pixel;
+    Vec3 sum(fb[0], fb[1], fb[2]);
+    for (uint32_t k = 0; k < tile.samples; ++k) {
+        const float* r = ctx.slotRadiance + k * plane + 3 * pixel;
+        sum += Vec3(r[0], r[1], r[2]);
+    }
+    fb[0] = sum.x;
+    fb[1] = sum.y;
+    fb[2] = sum.z;
 }
 
 __global__ void kReset(LaunchContext ctx, uint32_t nextSel) {
@@ -286,7 +332,7 @@ __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
         storeShadow(ctx.shadow, seg * ctx.shadow.segmentCapacity + s, outcome.shadow, alive ? slot : kShadowTerminated,
             path);
     } else if (valid && !alive) {
-        accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
+        finishPath(ctx, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
     }
 }
 
@@ -324,12 +370,12 @@ __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadow(LaunchContext
         Vec3 r(__uint_as_float(sfield(q, kSRx)[i]), __uint_as_float(sfield(q, kSRy)[i]),
             __uint_as_float(sfield(q, kSRz)[i]));
         if (!occluded) r += c;
-        accumulate(ctx.framebuffer, sfield(q, kSPixel)[i], r);
+        finishPath(ctx, sfield(q, kSPixel)[i], r);
     }
 }
 
 __global__ void kAdvance(LaunchContext ctx) {
-    if (ctx.tile->advanceSample) ++ctx.counters->sampleIndex;
+    if (ctx.tile->advanceSample) ctx.counters->sampleIndex += ctx.tile->samples;
 }
 
 // Persistent kernel: one grid-wide barrier per bounce.  The resident grid
@@ -345,17 +391,13 @@ __global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
     cg::grid_group grid = cg::this_grid();
     WavefrontCounters* counters = ctx.counters;
     RenderTile tile = *ctx.tile;
-    uint32_t pixels = tile.width * tile.height;
-    uint32_t count0 = pixels < ctx.capacity ? pixels : ctx.capacity;
+    uint32_t count0 = passPaths(ctx, tile);
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t sample = loadAgent(&counters->sampleIndex);
-    for (uint32_t g = gid; g < count0; g += stride) {
-        uint32_t x, y;
-        tilePixel(tile, g, x, y);
+    for (uint32_t g = gid; g < count0; g += stride)
         storePath(ctx.queue[0], primarySlot(g, ctx.blockSize, ctx.queue[0].segmentCapacity),
-            primaryPath(ctx.camera, x, y, ctx.width, ctx.height, sample, ctx.shade.rngStrategy));
-    }
+            passPath(ctx, tile, sample, g));
     if (gid < kSegments) {
         storeAgent(&counters->pcount[0][gid], primarySegmentCount(count0, ctx.blockSize, gid));
         storeAgent(&counters->pcount[1][gid], 0u);
@@ -397,10 +439,34 @@ __global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
             outcome.alive = false;
             if (valid) {
                 path = loadPath(q, segmentSlot(v, q.segmentCapacity, g));
-                HitRecord h = traceClosest<Full>(ctx.scene, path.ray, stack);
-                InlineOcclusion<LdsStack> occ{stack, 0u};
-                shadePath<Full>(ctx.scene, path, h, ctx.shade, occ, outcome);
-                shadowQueries += occ.queries;
+                if constexpr (Full) {
+                    HitRecord h = traceClosest<Full>(ctx.scene, path.ray, stack);
+                    InlineOcclusion<LdsStack> occ{stack, 0u};
+                    shadePath<Full>(ctx.scene, path, h, ctx.shade, occ, outcome);
+                    shadowQueries += occ.queries;
+                } else {
+                    // Without the full feature set the area-light sample is
+                    // the bounce's only occlusion test and its last radiance
+                    // addition: shading defers it, and the same traversal
+                    // call then resolves it (as in the regenerating kernel).
+                    DeferOcclusion occ;
+                    bool shadowQuery = false;
+                    Ray ray = path.ray;
+                    float tmax = kNoHit;
+                    while (true) {
+                        HitRecord h = traceQuery<Full, Query::kEither>(ctx.scene, ray, tmax, shadowQuery, stack);
+                        if (shadowQuery) {
+                            ++shadowQueries;
+                            if (h.kind == kHitNone) path.radiance += outcome.shadow.contribution;
+                            break;
+                        }
+                        shadePath<Full>(ctx.scene, path, h, ctx.shade, occ, outcome);
+                        if (!outcome.shadow.valid) break;
+                        shadowQuery = true;
+                        ray = Ray(outcome.shadow.origin, outcome.shadow.direction);
+                        tmax = outcome.shadow.tmax;
+                    }
+                }
             }
             bool alive = valid && outcome.alive;
             uint32_t a, unused;
@@ -408,7 +474,7 @@ __global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
             if (alive)
                 storePath(next, seg * next.segmentCapacity + a, path);
             else if (valid)
-                accumulate(ctx.framebuffer, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
+                finishPath(ctx, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
         }
         if (shadowQueries) atomicAdd(&counters->shadowQueries, static_cast<unsigned long long>(shadowQueries));
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
@@ -512,6 +578,8 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
             });
     }
     if (e != hipSuccess) return e;
+    e = timer(6, [&] { hipLaunchKernelGGL(kResolve, dim3(ctx.resolveBlocks), block, 0, stream, ctx); });
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
     return hipGetLastError();
 }
@@ -562,6 +630,7 @@ hipError_t kernelAttributes(bool persistent, bool full, hipFuncAttributes* out) 
 hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream) {
     hipError_t e = launchPersistent(ctx, stream);
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kResolve, dim3(ctx.resolveBlocks), dim3(ctx.blockSize), 0, stream, ctx);
     hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
     return hipGetLastError();
 }
@@ -591,6 +660,8 @@ hipError_t profileSample(const LaunchContext& ctx, hipStream_t stream, bool pers
         hipError_t launchError = hipSuccess;
         e = timer(4, [&] { launchError = launchPersistent(ctx, stream); });
         if (e == hipSuccess) e = launchError;
+        if (e == hipSuccess)
+            e = timer(6, [&] { hipLaunchKernelGGL(kResolve, dim3(ctx.resolveBlocks), dim3(ctx.blockSize), 0, stream, ctx); });
         if (e == hipSuccess) {
             hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
             e = hipGetLastError();

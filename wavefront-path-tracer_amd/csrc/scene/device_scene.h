@@ -37,6 +37,18 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "BvhNode must be one 64 B chunk");
 
+// Eight-wide node of the flat-triangle accelerator WPT_FLAT_WIDE8: the
+// bounds of up to eight children as coordinate arrays, then the children
+// (interior node index, kLeafBit|slot, or kInvalid for an unused child).
+constexpr uint32_t kWideChildren = 8;
+struct alignas(16) WideNode {
+    float minX[kWideChildren], minY[kWideChildren], minZ[kWideChildren];
+    float maxX[kWideChildren], maxY[kWideChildren], maxZ[kWideChildren];
+    uint32_t child[kWideChildren];
+    uint32_t pad[kWideChildren];
+};
+static_assert(sizeof(WideNode) == 256, "WideNode must be 256 B");
+
 struct Bvh {
     const BvhNode* nodes;
     uint32_t nodeCount;  // interior nodes
@@ -240,6 +252,7 @@ struct DevScene {
     uint32_t instanceCount;
     uint32_t blasCount;
     Bvh flat;
+    const WideNode* flatWide;  // non-null: the flat triangles use eight-wide nodes
     const float4* flatGeometry;
     const TriShade* flatTriangles;
     uint32_t flatCount;

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <future>
 #include <limits>
 #include <thread>
@@ -33,6 +34,7 @@ struct BuildContext {
     std::vector<uint32_t> idx;
     std::vector<BvhNode>* nodes;
     unsigned parallelDepth;
+    bool median = false;  // object median on the longest centroid axis instead of binned SAH
 };
 
 inline float area(const Aabb& b) {
@@ -81,6 +83,19 @@ uint32_t split(BuildContext& c, uint32_t begin, uint32_t end) {
             cmin[a] = std::fmin(cmin[a], v[a]);
             cmax[a] = std::fmax(cmax[a], v[a]);
         }
+    }
+    if (c.median) {
+        // Median BVH (the reference's host BVH, bvh.cpp:49-109, leaves of
+        // one primitive here): the centroid bounds' longest axis, split at
+        // the median centroid, ties kept in primitive order.
+        float ex = cmax[0] - cmin[0], ey = cmax[1] - cmin[1], ez = cmax[2] - cmin[2];
+        int axis = ex >= ey && ex >= ez ? 0 : (ey >= ez ? 1 : 2);
+        uint32_t mid = begin + n / 2;
+        std::nth_element(c.idx.begin() + begin, c.idx.begin() + mid, c.idx.begin() + end, [&](uint32_t a, uint32_t b) {
+            float ca = centroid(c, axis, a), cb = centroid(c, axis, b);
+            return ca < cb || (ca == cb && a < b);
+        });
+        return mid;
     }
     const std::vector<Aabb>& bounds = *c.bounds;
     float bestCost = INFINITY;
@@ -353,10 +368,11 @@ Aabb triangleBounds(const Vec3& a, const Vec3& b, const Vec3& c) {
 }
 
 uint32_t stackDepth(const HostSceneData& s) {
-    return std::max(s.tlas.depth + s.maxBlasDepth + 2, s.flat.depth + 1);
+    uint32_t flat = s.flatWide.empty() ? s.flat.depth + 1 : s.flatWideStack;
+    return std::max(s.tlas.depth + s.maxBlasDepth + 2, flat);
 }
 
-HostBvh buildBvh2(const std::vector<Aabb>& primBounds, unsigned threads) {
+HostBvh buildBvh2(const std::vector<Aabb>& primBounds, unsigned threads, bool median) {
     HostBvh bvh;
     bvh.primBounds = primBounds;
     uint32_t n = static_cast<uint32_t>(primBounds.size());
@@ -376,6 +392,7 @@ HostBvh buildBvh2(const std::vector<Aabb>& primBounds, unsigned threads) {
     }
     bvh.nodes.resize(n - 1);
     c.nodes = &bvh.nodes;
+    c.median = median;
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     unsigned depth = 0;
     while ((1u << depth) < threads && depth < 6) ++depth;
@@ -398,7 +415,7 @@ std::vector<wpt_bvh2_node> exportBvh2(const HostBvh& bvh) {
         return n + (n - 2 - pointer);
     };
     for (uint32_t slot = 0; slot < n; ++slot) {
-        const Aabb& b = bvh.primBounds[bvh.order[slot]];
+        const Aabb& b = bvh.slotBounds.empty() ? bvh.primBounds[bvh.order[slot]] : bvh.slotBounds[slot];
         wpt_bvh2_node& node = out[slot];
         std::memcpy(node.bmin, b.mn, sizeof node.bmin);
         std::memcpy(node.bmax, b.mx, sizeof node.bmax);
@@ -418,6 +435,430 @@ std::vector<wpt_bvh2_node> exportBvh2(const HostBvh& bvh) {
         node.right = nxbIndex(src.right);
     }
     return out;
+}
+
+
+namespace {
+
+// ---- spatial-split BVH ----------------------------------------------------
+// Spatial splits (Stich, Friedrich and Dietrich 2009, the method behind the
+// reference's SpatialSplitBvh policy, bvh_sbvh.cpp:94-219): besides object
+// splits, a node may cut space with a plane; a triangle crossing it is
+// referenced on both sides, each reference bounded by the part of the
+// triangle inside its side.  Leaves hold one reference, so the tree keeps
+// the single-primitive leaf format; a triangle may sit in several leaves.
+
+struct SplitRef {
+    uint32_t prim;
+    Aabb box;  // bounds of the triangle's part inside this reference's region
+};
+
+struct SplitNode {
+    Aabb box[2];
+    int32_t child[2];  // >= 0: node, < 0: -(reference index) - 1
+};
+
+struct SpatialBuilder {
+    const std::vector<TriShade>* tris;
+    std::vector<SplitRef> leaves;  // one per leaf, in creation order
+    std::vector<SplitNode> nodes;
+    size_t budget;                 // most references the build may create
+    size_t refs = 0;
+    float rootArea = 0.0f;
+};
+
+constexpr int kSpatialBins = 16;
+constexpr uint32_t kSpatialMaxDepth = 28;  // deeper nodes take object splits only
+constexpr float kSpatialOverlap = 1.0e-5f; // spatial splits only where object splits overlap this much (x root area)
+
+inline Aabb unionBox(const Aabb& a, const Aabb& b) {
+    Aabb r = a;
+    grow(r, b);
+    return r;
+}
+
+inline Aabb intersectBox(const Aabb& a, const Aabb& b) {
+    Aabb r;
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] = std::fmax(a.mn[k], b.mn[k]);
+        r.mx[k] = std::fmin(a.mx[k], b.mx[k]);
+    }
+    return r;
+}
+
+inline bool emptyOrInverted(const Aabb& b) { return !(b.mn[0] <= b.mx[0] && b.mn[1] <= b.mx[1] && b.mn[2] <= b.mx[2]); }
+
+// Bounds of triangle (a, b, c) clipped to `region` (Sutherland-Hodgman
+// against the six planes), intersected with the region to absorb rounding.
+Aabb clipTriangle(const Vec3& a, const Vec3& b, const Vec3& c, const Aabb& region) {
+    float poly[9][3] = {{a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}};
+    int n = 3;
+    for (int axis = 0; axis < 3 && n > 0; ++axis) {
+        for (int side = 0; side < 2 && n > 0; ++side) {
+            float plane = side == 0 ? region.mn[axis] : region.mx[axis];
+            float out[9][3];
+            int m = 0;
+            for (int i = 0; i < n; ++i) {
+                const float* p = poly[i];
+                const float* q = poly[(i + 1) % n];
+                float dp = side == 0 ? p[axis] - plane : plane - p[axis];
+                float dq = side == 0 ? q[axis] - plane : plane - q[axis];
+                if (dp >= 0.0f && m < 9) std::memcpy(out[m++], p, sizeof(float) * 3);
+                if ((dp < 0.0f) != (dq < 0.0f) && m < 9) {
+                    float t = dp / (dp - dq);
+                    for (int k = 0; k < 3; ++k) out[m][k] = p[k] + (q[k] - p[k]) * t;
+                    out[m][axis] = plane;
+                    ++m;
+                }
+            }
+            n = m;
+            std::memcpy(poly, out, sizeof(float) * 3 * static_cast<size_t>(m));
+        }
+    }
+    Aabb r = emptyBox();
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            r.mn[k] = std::fmin(r.mn[k], poly[i][k]);
+            r.mx[k] = std::fmax(r.mx[k], poly[i][k]);
+        }
+    r = intersectBox(r, region);
+    if (emptyOrInverted(r)) return r;
+    // Clipped vertices are rounded: widen by a few ulps so the box surely
+    // holds the triangle's part (overlap with the neighbour is harmless).
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] -= 4.0e-7f * std::fmax(1.0f, std::fabs(r.mn[k]));
+        r.mx[k] += 4.0e-7f * std::fmax(1.0f, std::fabs(r.mx[k]));
+    }
+    return r;
+}
+
+struct SplitChoice {
+    float cost = INFINITY;
+    int axis = -1;
+    bool spatial = false;
+    float plane = 0.0f;  // spatial: the cutting plane; object: centroid threshold
+};
+
+// Object split: binned SAH over reference centroids.
+SplitChoice objectSplit(const std::vector<SplitRef>& refs, const Aabb& centroids) {
+    SplitChoice best;
+    for (int axis = 0; axis < 3; ++axis) {
+        float lo = centroids.mn[axis], extent = centroids.mx[axis] - lo;
+        if (!(extent > 0.0f)) continue;
+        Aabb box[kBins];
+        uint32_t count[kBins] = {};
+        for (int b = 0; b < kBins; ++b) box[b] = emptyBox();
+        float scale = static_cast<float>(kBins) * (1.0f - 1.0e-5f) / extent;
+        for (const SplitRef& r : refs) {
+            float cc = (r.box.mn[axis] + r.box.mx[axis]) * 0.5f;
+            int b = std::min(std::max(static_cast<int>((cc - lo) * scale), 0), kBins - 1);
+            ++count[b];
+            grow(box[b], r.box);
+        }
+        float rightArea[kBins];
+        uint32_t rightCount[kBins];
+        Aabb acc = emptyBox();
+        uint32_t cnt = 0;
+        for (int b = kBins - 1; b > 0; --b) {
+            grow(acc, box[b]);
+            cnt += count[b];
+            rightArea[b] = area(acc);
+            rightCount[b] = cnt;
+        }
+        acc = emptyBox();
+        cnt = 0;
+        for (int b = 0; b < kBins - 1; ++b) {
+            grow(acc, box[b]);
+            cnt += count[b];
+            if (cnt == 0 || rightCount[b + 1] == 0) continue;
+            float cost = area(acc) * static_cast<float>(cnt) + rightArea[b + 1] * static_cast<float>(rightCount[b + 1]);
+            if (cost < best.cost) {
+                best.cost = cost;
+                best.axis = axis;
+                best.spatial = false;
+                best.plane = lo + static_cast<float>(b + 1) / scale;
+            }
+        }
+    }
+    return best;
+}
+
+// Spatial split: planes at bin boundaries of the node box; each reference
+// is clipped into every bin it overlaps.
+SplitChoice spatialSplit(const SpatialBuilder& sb, const std::vector<SplitRef>& refs, const Aabb& nodeBox) {
+    SplitChoice best;
+    for (int axis = 0; axis < 3; ++axis) {
+        float lo = nodeBox.mn[axis], extent = nodeBox.mx[axis] - lo;
+        if (!(extent > 0.0f)) continue;
+        float width = extent / static_cast<float>(kSpatialBins);
+        Aabb box[kSpatialBins];
+        uint32_t enter[kSpatialBins] = {}, exit[kSpatialBins] = {};
+        for (int b = 0; b < kSpatialBins; ++b) box[b] = emptyBox();
+        for (const SplitRef& r : refs) {
+            const TriShade& t = (*sb.tris)[r.prim];
+            int first = std::min(std::max(static_cast<int>((r.box.mn[axis] - lo) / width), 0), kSpatialBins - 1);
+            int last = std::min(std::max(static_cast<int>((r.box.mx[axis] - lo) / width), first), kSpatialBins - 1);
+            for (int b = first; b <= last; ++b) {
+                Aabb slab = r.box;
+                slab.mn[axis] = std::fmax(slab.mn[axis], lo + width * static_cast<float>(b));
+                slab.mx[axis] = std::fmin(slab.mx[axis], b == kSpatialBins - 1 ? nodeBox.mx[axis] : lo + width * static_cast<float>(b + 1));
+                Aabb part = clipTriangle(t.v0, t.v1, t.v2, slab);
+                if (!emptyOrInverted(part)) grow(box[b], part);
+            }
+            ++enter[first];
+            ++exit[last];
+        }
+        float rightArea[kSpatialBins];
+        uint32_t rightCount[kSpatialBins];
+        Aabb acc = emptyBox();
+        uint32_t cnt = 0;
+        for (int b = kSpatialBins - 1; b > 0; --b) {
+            grow(acc, box[b]);
+            cnt += exit[b];
+            rightArea[b] = area(acc);
+            rightCount[b] = cnt;
+        }
+        acc = emptyBox();
+        cnt = 0;
+        for (int b = 0; b < kSpatialBins - 1; ++b) {
+            grow(acc, box[b]);
+            cnt += enter[b];
+            if (cnt == 0 || rightCount[b + 1] == 0) continue;
+            float cost = area(acc) * static_cast<float>(cnt) + rightArea[b + 1] * static_cast<float>(rightCount[b + 1]);
+            if (cost < best.cost) {
+                best.cost = cost;
+                best.axis = axis;
+                best.spatial = true;
+                best.plane = lo + width * static_cast<float>(b + 1);
+            }
+        }
+    }
+    return best;
+}
+
+// Builds the subtree over `refs`; returns its child pointer (node index, or
+// -(leaf index) - 1) and writes its bounds.
+int32_t buildSpatial(SpatialBuilder& sb, std::vector<SplitRef>& refs, uint32_t depth, Aabb& bounds) {
+    bounds = emptyBox();
+    Aabb centroids = emptyBox();
+    for (const SplitRef& r : refs) {
+        grow(bounds, r.box);
+        Aabb c;
+        for (int k = 0; k < 3; ++k) c.mn[k] = c.mx[k] = (r.box.mn[k] + r.box.mx[k]) * 0.5f;
+        grow(centroids, c);
+    }
+    if (refs.size() == 1) {
+        sb.leaves.push_back(refs[0]);
+        return -static_cast<int32_t>(sb.leaves.size());
+    }
+    std::vector<SplitRef> left, right;
+    SplitChoice obj = depth < kSpatialMaxDepth ? objectSplit(refs, centroids) : SplitChoice{};
+    SplitChoice best = obj;
+    if (obj.axis >= 0 && depth < kSpatialMaxDepth && sb.refs < sb.budget) {
+        // Spatial splits pay only where the object split's children overlap.
+        Aabb l = emptyBox(), r = emptyBox();
+        for (const SplitRef& ref : refs) {
+            float cc = (ref.box.mn[obj.axis] + ref.box.mx[obj.axis]) * 0.5f;
+            grow(cc < obj.plane ? l : r, ref.box);
+        }
+        Aabb overlap = intersectBox(l, r);
+        if (!emptyOrInverted(overlap) && area(overlap) > kSpatialOverlap * sb.rootArea) {
+            SplitChoice sp = spatialSplit(sb, refs, bounds);
+            if (sp.cost < best.cost) best = sp;
+        }
+    }
+    if (best.axis >= 0 && best.spatial) {
+        for (const SplitRef& ref : refs) {
+            const TriShade& t = (*sb.tris)[ref.prim];
+            if (ref.box.mn[best.axis] < best.plane) {
+                Aabb region = ref.box;
+                region.mx[best.axis] = std::fmin(region.mx[best.axis], best.plane);
+                Aabb part = clipTriangle(t.v0, t.v1, t.v2, region);
+                if (!emptyOrInverted(part)) left.push_back(SplitRef{ref.prim, part});
+            }
+            if (ref.box.mx[best.axis] > best.plane) {
+                Aabb region = ref.box;
+                region.mn[best.axis] = std::fmax(region.mn[best.axis], best.plane);
+                Aabb part = clipTriangle(t.v0, t.v1, t.v2, region);
+                if (!emptyOrInverted(part)) right.push_back(SplitRef{ref.prim, part});
+            }
+        }
+        if (left.empty() || right.empty() || left.size() + right.size() < refs.size()) {
+            left.clear();
+            right.clear();
+        } else {
+            sb.refs += left.size() + right.size() - refs.size();
+        }
+    } else if (best.axis >= 0) {
+        for (const SplitRef& ref : refs) {
+            float cc = (ref.box.mn[best.axis] + ref.box.mx[best.axis]) * 0.5f;
+            (cc < best.plane ? left : right).push_back(ref);
+        }
+        if (left.empty() || right.empty()) {
+            left.clear();
+            right.clear();
+        }
+    }
+    if (left.empty()) {
+        // Median split on the longest centroid axis (also below the spatial
+        // depth limit: halving bounds the depth).
+        float ex = centroids.mx[0] - centroids.mn[0], ey = centroids.mx[1] - centroids.mn[1],
+              ez = centroids.mx[2] - centroids.mn[2];
+        int axis = ex >= ey && ex >= ez ? 0 : (ey >= ez ? 1 : 2);
+        size_t mid = refs.size() / 2;
+        std::nth_element(refs.begin(), refs.begin() + static_cast<std::ptrdiff_t>(mid), refs.end(),
+            [axis](const SplitRef& a, const SplitRef& b) {
+                float ca = a.box.mn[axis] + a.box.mx[axis], cb = b.box.mn[axis] + b.box.mx[axis];
+                return ca < cb || (ca == cb && a.prim < b.prim);
+            });
+        left.assign(refs.begin(), refs.begin() + static_cast<std::ptrdiff_t>(mid));
+        right.assign(refs.begin() + static_cast<std::ptrdiff_t>(mid), refs.end());
+    }
+    std::vector<SplitRef>().swap(refs);
+    int32_t index = static_cast<int32_t>(sb.nodes.size());
+    sb.nodes.push_back(SplitNode{});
+    Aabb lb, rb;
+    int32_t lc = buildSpatial(sb, left, depth + 1, lb);
+    int32_t rc = buildSpatial(sb, right, depth + 1, rb);
+    SplitNode& node = sb.nodes[static_cast<size_t>(index)];
+    node.box[0] = lb;
+    node.box[1] = rb;
+    node.child[0] = lc;
+    node.child[1] = rc;
+    return index;
+}
+
+void setChild(BvhNode& n, bool right, const Aabb& b, uint32_t pointer) {
+    float* v = right ? &n.rminx : &n.lminx;
+    if (!right) {
+        n.lminx = b.mn[0]; n.lminy = b.mn[1]; n.lminz = b.mn[2];
+        n.lmaxx = b.mx[0]; n.lmaxy = b.mx[1]; n.lmaxz = b.mx[2];
+        n.left = pointer;
+    } else {
+        n.rminx = b.mn[0]; n.rminy = b.mn[1]; n.rminz = b.mn[2];
+        n.rmaxx = b.mx[0]; n.rmaxy = b.mx[1]; n.rmaxz = b.mx[2];
+        n.right = pointer;
+    }
+    (void)v;
+}
+
+}  // namespace
+
+HostBvh buildSpatialSplitBvh2(const std::vector<TriShade>& tris) {
+    HostBvh bvh;
+    uint32_t n = static_cast<uint32_t>(tris.size());
+    for (const TriShade& t : tris) bvh.primBounds.push_back(triangleBounds(t.v0, t.v1, t.v2));
+    if (n == 0) return bvh;
+    SpatialBuilder sb;
+    sb.tris = &tris;
+    sb.budget = static_cast<size_t>(n) * 2;  // at most one extra reference per triangle
+    sb.refs = n;
+    std::vector<SplitRef> refs(n);
+    Aabb all = emptyBox();
+    for (uint32_t i = 0; i < n; ++i) {
+        refs[i] = SplitRef{i, bvh.primBounds[i]};
+        grow(all, bvh.primBounds[i]);
+    }
+    sb.rootArea = area(all);
+    Aabb rootBounds;
+    int32_t root = buildSpatial(sb, refs, 0, rootBounds);
+    // Renumber depth first: a subtree over m leaves owns m - 1 consecutive
+    // interior nodes, leaves take slots in the order the walk meets them.
+    uint32_t leaves = static_cast<uint32_t>(sb.leaves.size());
+    bvh.nodes.resize(leaves > 1 ? leaves - 1 : 0);
+    bvh.order.reserve(leaves);
+    bvh.slotBounds.reserve(leaves);
+    uint32_t nextNode = 0;
+    std::function<uint32_t(int32_t, uint32_t&)> emit = [&](int32_t ptr, uint32_t& depth) -> uint32_t {
+        if (ptr < 0) {
+            const SplitRef& r = sb.leaves[static_cast<size_t>(-ptr - 1)];
+            uint32_t slot = static_cast<uint32_t>(bvh.order.size());
+            bvh.order.push_back(r.prim);
+            bvh.slotBounds.push_back(r.box);
+            depth = 0;
+            return kLeafBit | slot;
+        }
+        uint32_t self = nextNode++;
+        const SplitNode sn = sb.nodes[static_cast<size_t>(ptr)];
+        uint32_t dl = 0, dr = 0;
+        uint32_t l = emit(sn.child[0], dl);
+        uint32_t r = emit(sn.child[1], dr);
+        BvhNode& node = bvh.nodes[self];
+        setChild(node, false, sn.box[0], l);
+        setChild(node, true, sn.box[1], r);
+        node.pad0 = node.pad1 = 0;
+        depth = 1 + std::max(dl, dr);
+        return self;
+    };
+    uint32_t depth = 0;
+    uint32_t rootPointer = emit(root, depth);
+    bvh.root = rootPointer;
+    bvh.rootBounds = rootBounds;
+    bvh.depth = depth;
+    return bvh;
+}
+
+std::vector<WideNode> collapseWide(const HostBvh& bvh, uint32_t* stackNeed) {
+    std::vector<WideNode> wide;
+    *stackNeed = 1;
+    if (bvh.primCount() < 2) return wide;
+    struct Item {
+        uint32_t pointer;
+        Aabb box;
+    };
+    // Children of binary node i.
+    auto children = [&](uint32_t i, Item out[2]) {
+        const BvhNode& n = bvh.nodes[i];
+        out[0] = Item{n.left, nodeChildBounds(n, false)};
+        out[1] = Item{n.right, nodeChildBounds(n, true)};
+    };
+    std::function<uint32_t(uint32_t, uint32_t)> build = [&](uint32_t binary, uint32_t stackSoFar) -> uint32_t {
+        Item items[kWideChildren];
+        int count = 0;
+        Item two[2];
+        children(binary, two);
+        items[count++] = two[0];
+        items[count++] = two[1];
+        // Open the interior child of largest area until eight children.
+        while (count < static_cast<int>(kWideChildren)) {
+            int pick = -1;
+            float best = -1.0f;
+            for (int k = 0; k < count; ++k) {
+                if (items[k].pointer & kLeafBit) continue;
+                float a = area(items[k].box);
+                if (a > best) {
+                    best = a;
+                    pick = k;
+                }
+            }
+            if (pick < 0) break;
+            children(items[pick].pointer, two);
+            items[pick] = two[0];
+            items[count++] = two[1];
+        }
+        uint32_t self = static_cast<uint32_t>(wide.size());
+        wide.push_back(WideNode{});
+        uint32_t need = stackSoFar + static_cast<uint32_t>(count) - 1u;
+        *stackNeed = std::max(*stackNeed, need + 1u);
+        uint32_t ptrs[kWideChildren];
+        for (int k = 0; k < count; ++k)
+            ptrs[k] = (items[k].pointer & kLeafBit) ? items[k].pointer : build(items[k].pointer, need);
+        WideNode& w = wide[self];
+        for (uint32_t k = 0; k < kWideChildren; ++k) {
+            bool used = k < static_cast<uint32_t>(count);
+            const Aabb b = used ? items[k].box : emptyBox();
+            w.minX[k] = b.mn[0];
+            w.minY[k] = b.mn[1];
+            w.minZ[k] = b.mn[2];
+            w.maxX[k] = b.mx[0];
+            w.maxY[k] = b.mx[1];
+            w.maxZ[k] = b.mx[2];
+            w.child[k] = used ? ptrs[k] : kInvalid;
+        }
+        return self;
+    };
+    build(bvh.root, 0);
+    return wide;
 }
 
 // Environment importance distribution, reference buildEnvironmentDistribution
@@ -676,6 +1117,7 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
     }
     if (desc.bvh_builder > WPT_BVH_BUILDER_DEVICE_PLOC) return "unknown BVH builder";
     if (desc.bvh_search_radius > 64) return "BVH search radius above 64";
+    if (desc.flat_accelerator > WPT_FLAT_WIDE8) return "unknown flat-triangle accelerator";
     out.bvhBuilder = desc.bvh_builder;
     out.bvhSearchRadius = desc.bvh_search_radius ? desc.bvh_search_radius : 16u;
     uint32_t maxBlasDepth = 0;
@@ -787,13 +1229,21 @@ std::string buildHostScene(const wpt_scene_desc& desc, HostSceneData& out, unsig
     for (uint32_t slot = 0; slot < out.tlas.primCount(); ++slot)
         out.tlasLeavesSlot.push_back(out.tlasLeavesOriginal[out.tlas.order[slot]]);
 
-    // Flat triangles get a BVH only when the scene has no meshes.
+    // Flat triangles get a BVH only when the scene has no meshes, built by
+    // the requested method (reference uploadDirectAcceleration,
+    // scene.cu:241-294).
     if (desc.mesh_count == 0 && !out.flatTriangles.empty()) {
-        std::vector<Aabb> bounds;
-        bounds.reserve(out.flatTriangles.size());
-        for (const TriShade& t : out.flatTriangles) bounds.push_back(triangleBounds(t.v0, t.v1, t.v2));
-        out.flat = buildBvh2(bounds, threads);
+        out.flatAccelerator = desc.flat_accelerator;
+        if (out.flatAccelerator == WPT_FLAT_SPATIAL_SPLIT_BVH2) {
+            out.flat = buildSpatialSplitBvh2(out.flatTriangles);
+        } else {
+            std::vector<Aabb> bounds;
+            bounds.reserve(out.flatTriangles.size());
+            for (const TriShade& t : out.flatTriangles) bounds.push_back(triangleBounds(t.v0, t.v1, t.v2));
+            out.flat = buildBvh2(bounds, threads, out.flatAccelerator == WPT_FLAT_MEDIAN_BVH2);
+        }
         out.flatGeometry = slotGeometry(out.flatTriangles, out.flat);
+        if (out.flatAccelerator == WPT_FLAT_WIDE8) out.flatWide = collapseWide(out.flat, &out.flatWideStack);
     }
     out.maxStackDepth = stackDepth(out);
     return std::string();

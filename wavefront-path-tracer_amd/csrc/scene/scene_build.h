@@ -22,6 +22,7 @@ struct HostBvh {
     std::vector<BvhNode> nodes;
     std::vector<uint32_t> order;
     std::vector<Aabb> primBounds;  // original primitive bounds (for export)
+    std::vector<Aabb> slotBounds;  // per-slot bounds when they differ from primBounds (spatial splits)
     uint32_t root = 0;
     Aabb rootBounds{};
     uint32_t depth = 0;  // longest root-to-leaf edge count
@@ -31,7 +32,21 @@ struct HostBvh {
 // Binned-SAH build over primitive bounds.  The node numbering is fixed by the
 // primitive partition alone (a subtree over n primitives owns n-1 consecutive
 // interior nodes), so the result is identical for any thread count.
-HostBvh buildBvh2(const std::vector<Aabb>& primBounds, unsigned threads = 0);
+// `median` builds the median-split tree instead (object median on the
+// longest centroid axis).
+HostBvh buildBvh2(const std::vector<Aabb>& primBounds, unsigned threads = 0, bool median = false);
+
+// Spatial-split BVH over triangles: triangles crossing a chosen plane are
+// referenced on both sides, bounded by their clipped parts, so a triangle
+// can occupy several leaf slots (order[slot] names it; slotBounds holds the
+// clipped bounds).
+HostBvh buildSpatialSplitBvh2(const std::vector<TriShade>& tris);
+
+// Eight-wide nodes collapsed from a binary tree: each wide node takes the
+// children of a binary node and keeps opening its largest interior child
+// until it has eight.  Leaves keep the binary tree's slots.  *stackNeed
+// receives the deepest traversal stack the wide walk can need.
+std::vector<WideNode> collapseWide(const HostBvh& bvh, uint32_t* stackNeed);
 
 // Export in the reference node convention (NXB::BVH2: own bounds per node,
 // leaf iff left == 0xffffffff, root = count - 1).
@@ -62,6 +77,9 @@ struct HostSceneData {
     std::vector<TriShade> flatTriangles;
     HostBvh flat;
     std::vector<float4> flatGeometry;
+    uint32_t flatAccelerator = 0;    // WPT_FLAT_*
+    std::vector<WideNode> flatWide;  // WPT_FLAT_WIDE8: nodes over flat's slots
+    uint32_t flatWideStack = 0;
     std::vector<Material> materials;
     std::vector<uint8_t> materialCutout;
     std::vector<TriangleLight> lights;

@@ -355,6 +355,7 @@ wpt_scene_desc HostScene::desc() const {
     }
     d.use_normal_maps = useNormalMaps ? 1u : 0u;
     d.normal_map_minimum_cosine = normalMapMinimumCosine;
+    d.flat_accelerator = flatAccelerator;
     return d;
 }
 
@@ -690,6 +691,12 @@ bool objScene(HostScene& s, const SceneConfig& c, std::string* error) {
                 for (int k = 0; k < 3; ++k) v[k] = c.objectScale * v[k] + tr[k];
         }
         s.flatTriangles = std::move(all);
+        // The reference's flat accelerators (scene.cu:241-294): NexusBVH's
+        // BVH2 and BVH8, the median BVH and the spatial-split BVH.
+        s.flatAccelerator = c.acceleration == ObjAccelerationPolicy::NexusBvh8 ? WPT_FLAT_WIDE8
+            : c.acceleration == ObjAccelerationPolicy::MedianBvh ? WPT_FLAT_MEDIAN_BVH2
+            : c.acceleration == ObjAccelerationPolicy::SpatialSplitBvh ? WPT_FLAT_SPATIAL_SPLIT_BVH2
+                                                                       : WPT_FLAT_SAH_BVH2;
     }
     return true;
 }

@@ -34,6 +34,7 @@ struct HostScene {
     float environmentIntensity = 1.0f, environmentRotationDegrees = 0.0f;
     bool useNormalMaps = true;
     float normalMapMinimumCosine = 0.0f;
+    uint32_t flatAccelerator = WPT_FLAT_SAH_BVH2;  // tree over flatTriangles
 
     wpt_scene_desc desc() const;
 };

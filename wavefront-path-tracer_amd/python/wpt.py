@@ -148,7 +148,7 @@ class SceneDesc(C.Structure):
                 ("textures", C.POINTER(Texture)), ("texture_count", C.c_uint32),
                 ("material_ext", C.POINTER(MaterialExt)), ("use_normal_maps", C.c_uint32),
                 ("normal_map_minimum_cosine", C.c_float), ("bvh_builder", C.c_uint32),
-                ("bvh_search_radius", C.c_uint32)]
+                ("bvh_search_radius", C.c_uint32), ("flat_accelerator", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -166,7 +166,7 @@ class RenderConfig(C.Structure):
                 ("caustic_photons", C.c_uint32), ("caustic_max_depth", C.c_uint32),
                 ("caustic_gather_radius", C.c_float), ("caustic_gather", C.c_uint32),
                 ("caustic_spectral", C.c_uint32), ("caustic_grid_resolution", C.c_uint32),
-                ("caustic_grid_min", F3), ("caustic_grid_max", F3)]
+                ("caustic_grid_min", F3), ("caustic_grid_max", F3), ("samples_per_pass", C.c_uint32)]
 
 
 class Task(C.Structure):
@@ -183,6 +183,7 @@ class LedgerJob(C.Structure):
 
 LEDGER_ACCEPTED, LEDGER_DUPLICATE, LEDGER_REJECTED = 0, 1, 2
 BVH_HOST_SAH, BVH_DEVICE_PLOC = 0, 1
+FLAT_SAH_BVH2, FLAT_MEDIAN_BVH2, FLAT_SPATIAL_SPLIT_BVH2, FLAT_WIDE8 = 0, 1, 2, 3
 TASK_PENDING, TASK_LEASED, TASK_COMPLETED = 0, 1, 2
 
 
@@ -214,7 +215,8 @@ class SceneInfo(C.Structure):
         "sphere_count", "direct_sphere_count", "instance_count", "direct_instance_count", "tlas_leaf_count",
         "tlas_node_count", "blas_count", "light_count", "static_triangle_count", "flat_triangle_count",
         "flat_node_count", "material_count", "bvh_builder", "max_blas_depth")] + [
-        ("blas_build_milliseconds", C.c_double)]
+        ("blas_build_milliseconds", C.c_double)] + [(name, C.c_uint32) for name in (
+        "flat_accelerator", "flat_leaf_slots", "flat_wide_node_count", "max_stack_depth")]
 
 
 class BatchTask(C.Structure):
@@ -418,6 +420,7 @@ class SceneBuilder:
         self.spheres: List[Sphere] = []
         self.static_triangles: List[Triangle] = []
         self.flat_triangles: List[Triangle] = []
+        self.flat_accelerator = FLAT_SAH_BVH2
         self.mesh_triangles: List[Triangle] = []
         self.meshes: List[Mesh] = []
         self.instances: List[Instance] = []
@@ -487,6 +490,7 @@ class SceneBuilder:
         d.spheres, d.sphere_count = self._array(Sphere, self.spheres)
         d.static_triangles, d.static_triangle_count = self._array(Triangle, self.static_triangles)
         d.flat_triangles, d.flat_triangle_count = self._array(Triangle, self.flat_triangles)
+        d.flat_accelerator = self.flat_accelerator
         d.mesh_triangles, d.mesh_triangle_count = self._array(Triangle, self.mesh_triangles)
         d.meshes, d.mesh_count = self._array(Mesh, self.meshes)
         d.instances, d.instance_count = self._array(Instance, self.instances)
@@ -661,7 +665,7 @@ def compact(items, flags, capacity):
 def render_config(width, height, spp=1, max_depth=8, rr_start=3, block_size=256, rng=RNG_PCG32, clamp=10.0,
                   mode=MODE_GRAPH, tile=None, persistent_blocks=0, intersection_debug=False,
                   profile_queues=True, caustic_photons=0, caustic_max_depth=8, caustic_radius=0.06,
-                  caustic_gather=True, caustic_spectral=False, caustic_grid=None) -> RenderConfig:
+                  caustic_gather=True, caustic_spectral=False, caustic_grid=None, samples_per_pass=0) -> RenderConfig:
     rc = RenderConfig()
     rc.width, rc.height = width, height
     rc.tiled = 1 if tile else 0
@@ -675,6 +679,7 @@ def render_config(width, height, spp=1, max_depth=8, rr_start=3, block_size=256,
     rc.caustic_photons, rc.caustic_max_depth, rc.caustic_gather_radius = caustic_photons, caustic_max_depth, caustic_radius
     rc.caustic_gather = 1 if caustic_gather else 0
     rc.caustic_spectral = 1 if caustic_spectral else 0
+    rc.samples_per_pass = samples_per_pass
     if caustic_grid is not None:  # (resolution, (min xyz), (max xyz))
         rc.caustic_grid_resolution = caustic_grid[0]
         rc.caustic_grid_min, rc.caustic_grid_max = vec3(caustic_grid[1]), vec3(caustic_grid[2])
