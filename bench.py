@@ -154,6 +154,91 @@ def measure_traversal(scene, camera, threads, reps=5):
     return out
 
 
+def measure_flat_accelerators(reps=5):
+    """Flat-triangle accelerators (the reference's nexus_bvh2, nexus_bvh8,
+    median_bvh and spatial_split_bvh policies, src/scene/scene.cu:241-294)
+    on the sphere field's 1,003,520-triangle icosphere uploaded as flat
+    triangles: host build time, and closest-hit rates on a 1920x1080
+    pinhole wavefront aimed at the mesh and on 2M scattered rays (origins
+    inside the mesh's box, uniform directions).  Every accelerator must give
+    the same hit records, bit for bit."""
+    import ctypes
+    import torch
+
+    host = wpt.HostScene("sphere-field")
+    src = host.desc()
+    n = src.mesh_triangle_count
+    raw = (ctypes.c_uint8 * (n * ctypes.sizeof(wpt.Triangle))).from_address(
+        ctypes.addressof(src.mesh_triangles.contents))
+    verts = np.frombuffer(raw, np.uint8).reshape(n, -1)[:, :36].copy().view(np.float32).reshape(n, 3, 3)
+    lo, hi = verts.min(axis=(0, 1)).astype(np.float64), verts.max(axis=(0, 1)).astype(np.float64)
+    centre, extent = 0.5 * (lo + hi), float(np.max(hi - lo))
+    # Pinhole wavefront from outside the box, looking at its centre.
+    eye = centre + np.array([0.35, 0.25, 1.0]) * 1.2 * extent
+    f = centre - eye
+    f /= np.linalg.norm(f)
+    r = np.cross(f, [0.0, 1.0, 0.0])
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    t = np.tan(np.radians(40.0) * 0.5)
+    j, i = np.mgrid[0:HEIGHT, 0:WIDTH]
+    x = ((2.0 * (i.reshape(-1) + 0.5) / WIDTH - 1.0) * (WIDTH / HEIGHT) * t)[:, None]
+    y = ((1.0 - 2.0 * (j.reshape(-1) + 0.5) / HEIGHT) * t)[:, None]
+    d = f[None, :] + r[None, :] * x + u[None, :] * y
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    camera_rays = np.empty((6, WIDTH * HEIGHT), np.float32)
+    camera_rays[0:3] = eye[:, None]
+    camera_rays[3:6] = d.T
+    rng = np.random.default_rng(12345)
+    m = 2 * 1024 * 1024
+    scattered = np.empty((6, m), np.float32)
+    scattered[0:3] = (lo[:, None] + (hi - lo)[:, None] * rng.random((3, m)))
+    v = rng.normal(size=(3, m))
+    scattered[3:6] = v / np.linalg.norm(v, axis=0)
+    sets = {"camera_1080p": torch.from_numpy(camera_rays).cuda(), "scattered_2m": torch.from_numpy(scattered).cuda()}
+
+    names = {wpt.FLAT_SAH_BVH2: "sah_bvh2", wpt.FLAT_MEDIAN_BVH2: "median_bvh2",
+             wpt.FLAT_SPATIAL_SPLIT_BVH2: "spatial_split_bvh2", wpt.FLAT_WIDE8: "wide8"}
+    out, first = {}, {}
+    identical = True
+    for accel, name in names.items():
+        desc = wpt.SceneDesc()
+        desc.flat_triangles, desc.flat_triangle_count = src.mesh_triangles, n
+        desc.materials, desc.material_count = src.materials, src.material_count
+        desc.flat_accelerator = accel
+        t0 = time.perf_counter()
+        scene = wpt.Scene(desc, device=True, owner=host)
+        build_s = time.perf_counter() - t0
+        info = scene.info()
+        row = {"build_and_upload_s": round(build_s, 2), "leaf_slots": int(info.flat_leaf_slots),
+               "wide_nodes": int(info.flat_wide_node_count), "max_stack_depth": int(info.max_stack_depth)}
+        for set_name, rays in sets.items():
+            hits = scene.trace_closest(rays)
+            torch.cuda.synchronize()
+            times = []
+            for _ in range(reps):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                hits = scene.trace_closest(rays)
+                b.record()
+                b.synchronize()
+                times.append(a.elapsed_time(b) / 1000.0)
+            h = wpt.hits_to_numpy(hits)
+            if set_name in first:
+                identical = identical and h.tobytes() == first[set_name]
+            else:
+                first[set_name] = h.tobytes()
+            g = float(np.median(times))
+            row[set_name] = {"mrays_per_s": round(rays.shape[1] / g / 1e6, 1), "ms": round(g * 1000.0, 3),
+                             "hit_fraction": round(float((h["kind"] != 0).mean()), 4)}
+        scene.close()
+        out[name] = row
+        log(f"flat accelerator {name}: {row}")
+    out["hits_identical"] = identical
+    out["triangles"] = int(n)
+    return out
+
+
 def measure_cornell(threads, frames=20):
     """The small plain case: the unit Cornell box (12 triangles, emissive
     quad, Lambertian and dielectric spheres), 256x256, 16 spp, depth 8,
@@ -417,44 +502,104 @@ def measure_diff(steps):
                          err(params[t + rc:], truth[t + rc:])],
         },
     }
+    result["roughness_observability"] = roughness_observability(r, truth, target, lay)
     log(f"diff: {result}")
     r.close()
     return result
+
+
+def roughness_observability(r, truth, target, lay, seeds=8):
+    """Why Adam leaves the roughness where it is: the loss change from
+    moving every roughness by +-0.1 from the truth (other parameters at the
+    truth, common random numbers), beside the same for a 0.05 texture shift
+    and a 5% emission change, and the signal-to-noise ratio of the
+    roughness gradient at truth + 0.1 over `seeds` independent sample sets
+    (|mean| / standard deviation, averaged over the slots)."""
+    import torch
+    t, rc = lay.texture_floats, lay.roughness_count
+    start = 1 << 24
+
+    def loss(p, seed=0):
+        img = r.forward(p, sample_start=start + seed * r.spp)
+        return float(((img.double() - target.double()) ** 2).mean())
+
+    def moved(section, amount, scale=False):
+        p = truth.clone()
+        sl = {"texture": slice(0, t), "roughness": slice(t, t + rc), "emission": slice(t + rc, None)}[section]
+        p[sl] = p[sl] * (1.0 + amount) if scale else (p[sl] + amount).clamp(0.01, 1.0)
+        return p
+
+    base = loss(truth)
+    out = {"loss_at_truth": base,
+           "roughness_minus_0.1": loss(moved("roughness", -0.1)) - base,
+           "roughness_plus_0.1": loss(moved("roughness", 0.1)) - base,
+           "texture_plus_0.05": loss(moved("texture", 0.05)) - base,
+           "emission_times_1.05": loss(moved("emission", 0.05, scale=True)) - base}
+    p = moved("roughness", 0.1)
+    grads = []
+    for k in range(seeds):
+        img = r.forward(p, sample_start=start + (k + 1) * r.spp)
+        g = r.adjoint(p, 2.0 * (img - target) / img.numel())
+        grads.append(g[t:t + rc].double().cpu().numpy())
+    g = np.array(grads)
+    seen = np.abs(g).max(axis=0) > 0  # slots some path reaches
+    snr = np.abs(g.mean(axis=0)[seen]) / np.maximum(g.std(axis=0, ddof=1)[seen], 1e-30)
+    out["roughness_slots_seen"] = int(seen.sum())
+    out["roughness_gradient_snr_mean"] = float(snr.mean())
+    # Above the truth the loss grows with roughness: the mean gradient is positive.
+    out["roughness_gradient_positive_fraction"] = float((g.mean(axis=0)[seen] > 0).mean())
+    return out
 
 
 def measure_terrain(spp, photons, frames):
     """Shading-heavy workload with caustics: the terrain preset at full size
     (2,006,680 triangles, 16 x 4096^2 RGBA8 textures, 512 emissive
     triangles, a 2048 x 1024 HDR sky with a sun, dielectric crystals) with a
-    spectral photon map, 1920x1080, depth 10, regenerating scheduler; BLASes
-    built on the device."""
+    spectral photon map, 1920x1080, depth 10, each scheduler timed on the
+    same frame (images compared); BLASes built on the device."""
     t0 = time.perf_counter()
     host, scene = terrain_scene()
     t2 = time.perf_counter()
     info = scene.info()
     log(f"terrain: generated, built and uploaded in {t2 - t0:.2f} s "
         f"(device BLAS build {info.blas_build_milliseconds:.1f} ms)")
-    rc = terrain_config(photons, spp)
-    r = wpt.Renderer(scene, host.camera(WIDTH, HEIGHT), rc)
-    photon = r.stats()
-    r.reset()
-    r.render_samples(0, spp)  # warmup
-    tp = time.perf_counter()
-    for _ in range(frames):
+    camera = host.camera(WIDTH, HEIGHT)
+    modes = {"graph": wpt.MODE_GRAPH, "persistent": wpt.MODE_PERSISTENT, "regenerate": wpt.MODE_REGENERATE}
+    runs, first = {}, None
+    photon = None
+    for name, mode in modes.items():
+        rc = terrain_config(photons, spp)
+        rc.mode = mode
+        r = wpt.Renderer(scene, camera, rc)
+        if photon is None:
+            photon = r.stats()
         r.reset()
-        r.render_samples(0, spp)
-    wall = (time.perf_counter() - tp) / frames
-    st = r.stats()
-    img, n = r.framebuffer()
-    r.close()
+        r.render_samples(0, spp)  # warmup
+        tp = time.perf_counter()
+        for _ in range(frames):
+            r.reset()
+            r.render_samples(0, spp)
+        wall = (time.perf_counter() - tp) / frames
+        st = r.stats()
+        img, n = r.framebuffer()
+        r.close()
+        if first is None:
+            first = (img, n, st)
+        runs[name] = {"ms_per_frame": round(wall * 1000.0, 2),
+                      "mrays_per_s": round((st.closest_queries + st.shadow_queries) / wall / 1e6, 2),
+                      "image_identical": bool(np.array_equal(img, first[0]))}
+        log(f"terrain {name}: {runs[name]}")
+    best = min(runs, key=lambda k: runs[k]["ms_per_frame"])
+    img, n, st = first
     out = {
         "workload": f"terrain 1920x1080 {spp}spp depth10 rr3: 2,006,680 triangles (terrain + 4 rock meshes "
                     f"+ foliage + crystals + 512 lamp triangles, 162 instances), 16 x 4096^2 RGBA8 textures, "
                     f"2048x1024 HDR sky, {photons} spectral caustic photons",
-        "mrays_per_s": round((st.closest_queries + st.shadow_queries) / wall / 1e6, 2),
-        "msamples_per_s": round(WIDTH * HEIGHT * spp / wall / 1e6, 2),
-        "ms_per_frame": round(wall * 1000.0, 2),
-        "device_ms_per_frame": round(st.trace_milliseconds, 2),
+        "scheduler": best,
+        "mrays_per_s": runs[best]["mrays_per_s"],
+        "msamples_per_s": round(WIDTH * HEIGHT * spp / (runs[best]["ms_per_frame"] / 1000.0) / 1e6, 2),
+        "ms_per_frame": runs[best]["ms_per_frame"],
+        "schedulers": runs,
         "queries_per_path": round((st.closest_queries + st.shadow_queries) / (WIDTH * HEIGHT * spp), 3),
         "photon_map_ms": round(photon.caustic_build_milliseconds, 2),
         "photons_stored": int(photon.caustic_photons_stored),
@@ -471,19 +616,22 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=SPP)
-    ap.add_argument("--mode", choices=["regenerate", "graph", "persistent"], default="regenerate",
+    ap.add_argument("--mode", choices=["regenerate", "graph", "persistent"], default="graph",
                     help="scheduler of the timed frames; the other two are timed beside it")
     ap.add_argument("--samples-per-pass", type=int, default=0,
                     help="samples per pass of the staged and persistent schedulers (0: the library's default)")
     ap.add_argument("--skip-persistent", action="store_true", help="do not time the other schedulers beside")
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--cpu-spp", type=int, default=8, help="samples per pixel of the CPU baseline sample")
-    ap.add_argument("--profile-spp", type=int, default=4, help="samples rendered with per-kernel events")
+    ap.add_argument("--profile-spp", type=int, default=0,
+                    help="samples rendered with per-kernel events (0: the whole frame, so launches have their real size)")
     ap.add_argument("--skip-batch", action="store_true", help="do not measure batched vs one-by-one tasks")
     ap.add_argument("--batch-tasks", type=int, default=256, help="tasks of the batching workload")
     ap.add_argument("--skip-diff", action="store_true", help="do not measure the differentiable renderer")
     ap.add_argument("--skip-terrain", action="store_true", help="do not measure the shading-heavy caustic workload")
     ap.add_argument("--skip-cornell", action="store_true", help="do not measure the small Cornell box case")
+    ap.add_argument("--skip-flat", action="store_true",
+                    help="do not measure the flat-triangle accelerators")
     ap.add_argument("--skip-traversal", action="store_true",
                     help="do not measure closest-hit traversal alone on the reference and primary ray sets")
     ap.add_argument("--terrain-spp", type=int, default=64)
@@ -560,21 +708,27 @@ def main():
     # Device time of the dominant kernel.  The regenerating scheduler is one
     # kBatch launch per frame: its HIP-event time is the frames' own.  The
     # staged scheduler's kernels are timed one by one over a few samples.
+    breakdown = None
     if mode == wpt.MODE_REGENERATE:
         trace_ms = float(np.sum(device_ms))
         trace_launches = args.steps
         prof_closest, prof_shadow = stats.closest_queries * args.steps, stats.shadow_queries * args.steps
     else:
-        kernel_ms, launches = renderer.profile_kernels(0, args.profile_spp)
+        profile_spp = args.profile_spp or args.spp
+        kernel_ms, launches = renderer.profile_kernels(0, profile_spp)
         prof_stats = renderer.stats()
         trace_slot = 2 if mode == wpt.MODE_GRAPH else 4
         trace_ms = kernel_ms[trace_slot]
         trace_launches = int(launches[trace_slot])
         prof_closest, prof_shadow = prof_stats.closest_queries, prof_stats.shadow_queries
-        log(f"kernels over {args.profile_spp} spp: primary {kernel_ms[0]:.2f} ms, reset {kernel_ms[1]:.2f} ms, "
+        log(f"kernels over {profile_spp} spp: primary {kernel_ms[0]:.2f} ms, reset {kernel_ms[1]:.2f} ms, "
             f"trace {kernel_ms[2]:.2f} ms ({int(launches[2])} launches), shade {kernel_ms[3]:.2f} ms, "
             f"persistent {kernel_ms[4]:.2f} ms ({int(launches[4])} launches), shadow {kernel_ms[5]:.2f} ms "
             f"({int(launches[5])} launches)")
+        names = ["primary", "reset", "trace", "shade", "persistent", "shadow", "resolve"]
+        breakdown = {"spp": profile_spp,
+                     "ms": {n: round(float(kernel_ms[k]), 3) for k, n in enumerate(names) if launches[k]},
+                     "launches": {n: int(launches[k]) for k, n in enumerate(names) if launches[k]}}
     renderer.close()
 
     others = {}
@@ -614,6 +768,10 @@ def main():
     traversal = None
     if not args.skip_traversal:
         traversal = measure_traversal(scene, camera, threads)
+
+    flat = None
+    if not args.skip_flat:
+        flat = measure_flat_accelerators()
 
     cornell = None
     if not args.skip_cornell:
@@ -660,6 +818,7 @@ def main():
             "bytes_per_closest_query": round(per_query, 1),
             "avg_launch_ms": round(trace_ms / max(1, trace_launches), 4),
             "launches": trace_launches,
+            "kernel_breakdown": breakdown,
             "occupancy": {k: launch[k] for k in ("block_size", "blocks", "blocks_per_cu", "waves_per_simd",
                                                  "registers", "scratch_bytes", "lds_bytes_per_block",
                                                  "lds_bytes_per_cu")},
@@ -689,6 +848,7 @@ def main():
         "other_schedulers": others,
         "plain_small": cornell,
         "traversal": traversal,
+        "flat_accelerators": flat,
         "batch": batch,
         "diff": diff,
         "shading_heavy": terrain,

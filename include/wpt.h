@@ -321,9 +321,11 @@ typedef struct wpt_render_config {
     float caustic_grid_min[3];
     float caustic_grid_max[3];
     /* Samples the staged and cooperative persistent schedulers render per
-     * pass of a tile (0: 8; at most 32; never more than samples_per_pixel).
-     * Each pixel still adds its samples to the framebuffer in sample order,
-     * so images do not depend on it. */
+     * pass of a tile (0: as many as keep a pass within 2^27 paths, 64 at
+     * 1920x1080; at most 64; never more than samples_per_pixel).  Path
+     * memory grows with it, about 260 B per path of a pass.  Each pixel
+     * still adds its samples to the framebuffer in sample order, so images
+     * do not depend on it. */
     uint32_t samples_per_pass;
 } wpt_render_config;
 

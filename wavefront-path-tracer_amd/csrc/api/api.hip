@@ -703,6 +703,13 @@ int buildPhotonMap(const DevScene& scene, const wpt_render_config& c, DeviceAren
 // code reaches it only through the address hipGetSymbolAddress reports for
 // the current device, never through the array's name.
 constexpr uint32_t kNoAllocMaxItems = 1u << 24;
+// Default paths per pass of the staged and cooperative schedulers (64
+// samples of a 1920x1080 frame): about 260 B each of queues, hit and shadow
+// records and sample planes, 35 GB of the device's 288 GB.
+constexpr uint64_t kDefaultPassPaths = 1ull << 27;
+// Staged kernels launch at most this many times the trace kernel's resident
+// workgroups (the dispatcher still balances the workgroups).
+constexpr uint32_t kStagedGridFactor = 4;
 constexpr uint32_t kNoAllocScratchWords = 1u + kNoAllocMaxItems / 256u;
 static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
 
@@ -759,8 +766,11 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     r->fingerprint = fingerprint;
     const wpt_render_config& c = r->config;
     uint32_t tilePixels = c.tiled ? c.tile_width * c.tile_height : c.width * c.height;
-    // Samples per pass of the staged and cooperative schedulers.
-    uint32_t slots = c.mode == WPT_MODE_REGENERATE ? 1u : (c.samples_per_pass ? c.samples_per_pass : 8u);
+    // Samples per pass of the staged and cooperative schedulers: by default
+    // as many as keep a pass within kDefaultPassPaths paths (64 at 1080p).
+    uint32_t slots = c.samples_per_pass ? c.samples_per_pass
+                                        : static_cast<uint32_t>(std::max<uint64_t>(1u, kDefaultPassPaths / tilePixels));
+    if (c.mode == WPT_MODE_REGENERATE) slots = 1u;
     slots = std::max(1u, std::min(std::min(slots, kMaxSlots), c.samples_per_pixel));
     uint64_t cap64 = static_cast<uint64_t>(tilePixels) * slots;
     if (cap64 >= (1ull << 31)) {
@@ -809,13 +819,15 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     ctx.width = c.width;
     ctx.height = c.height;
     ctx.blockSize = c.block_size;
-    ctx.gridBlocks = (cap + c.block_size - 1) / c.block_size;
+    ctx.gridBlocks = (cap + c.block_size - 1) / c.block_size;  // capped below by the resident grid
     ctx.resolveBlocks = (tilePixels + c.block_size - 1) / c.block_size;
     ctx.profileQueues = 1;
     ctx.capacity = cap;
     ctx.slots = slots;
-    // Trace, shade and shadow kernels run one item per thread over
-    // gridBlocks workgroups; the persistent kernel runs a resident grid.
+    // Primary, trace, shade and shadow kernels walk their items in grid
+    // stride over gridBlocks workgroups, at most kStagedGridFactor times the
+    // trace kernel's resident grid (which bounds the traversal spill area);
+    // the persistent kernel runs a resident grid.
     Occupancy occ{};
     R_HIP(kernelOccupancy(c.block_size, ctx.scene.fullFeatures != 0, &occ));
     int device = 0, cus = 0, coop = 0;
@@ -824,6 +836,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     if (occ.trace <= 0 || occ.shade <= 0 || occ.shadow <= 0)
         return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the wavefront kernels"));
     uint32_t cu = static_cast<uint32_t>(cus);
+    ctx.gridBlocks = std::max(1u, std::min(ctx.gridBlocks, kStagedGridFactor * static_cast<uint32_t>(occ.trace) * cu));
     ctx.persistentBlocks = 1;
     if (c.mode == WPT_MODE_PERSISTENT) {
         if (occ.persistent <= 0) return cleanup(fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the persistent kernel"));
@@ -834,9 +847,15 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
         ctx.persistentBlocks = std::max(1u, std::min(blocks, ctx.gridBlocks));
     }
     // Segment capacity: workgroup b appends to segment b % kSegments, so a
-    // segment receives at most (its workgroups) x (items per workgroup).
+    // segment receives at most (its workgroups) x (items per workgroup); a
+    // fresh pass fills segment s with the paths of virtual workgroups
+    // s, s + kSegments, ... (primarySlot).
     uint32_t B = c.block_size;
-    uint64_t stagedSegment = static_cast<uint64_t>((ctx.gridBlocks + kSegments - 1) / kSegments) * B;
+    uint64_t stagedRounds = (static_cast<uint64_t>(cap) + static_cast<uint64_t>(ctx.gridBlocks) * B - 1) /
+        (static_cast<uint64_t>(ctx.gridBlocks) * B);
+    uint64_t virtualBlocks = (static_cast<uint64_t>(cap) + B - 1) / B;
+    uint64_t stagedSegment = std::max(static_cast<uint64_t>((ctx.gridBlocks + kSegments - 1) / kSegments) * stagedRounds,
+                                      (virtualBlocks + kSegments - 1) / kSegments) * B;
     uint32_t P = ctx.persistentBlocks;
     uint64_t rounds = (static_cast<uint64_t>(cap) + static_cast<uint64_t>(P) * B - 1) / (static_cast<uint64_t>(P) * B);
     uint64_t persistentSegment = static_cast<uint64_t>((P + kSegments - 1) / kSegments) * rounds * B;

@@ -16,11 +16,16 @@
 #include <cmath>
 #include <cstring>
 
+// WPT_HD: inlined everywhere.  WPT_CALL: a real call on the device, for a
+// large body reached from many call sites of one kernel (each inlined copy
+// would add its full length to the kernel's code).
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define WPT_HD __host__ __device__ __forceinline__
+#define WPT_CALL __host__ __device__ inline __attribute__((noinline))
 #else
 #define WPT_HD inline
+#define WPT_CALL inline
 #endif
 
 namespace wpt {

@@ -269,12 +269,16 @@ struct DeferOcclusion {
     WPT_HD bool blocked(const DevScene&, const Ray&, float) { return false; }
 };
 
+// Shadow and escape rays traced where shading draws them.  The full-feature
+// shading reaches this from about ten sites; as a call it exists once in the
+// kernel's code instead of once per site (kBatch<true>: 1.08 MB of code
+// inlined, 0.45 MB with the call; the terrain frame 928 -> 780 ms).
 template <class Stack>
 struct InlineOcclusion {
     static constexpr bool kInline = true;
     Stack& stack;
     uint32_t queries;
-    WPT_HD bool blocked(const DevScene& s, const Ray& ray, float tmax) {
+    WPT_CALL bool blocked(const DevScene& s, const Ray& ray, float tmax) {
         ++queries;
         return traceOccluded(s, ray, tmax, stack);
     }

@@ -5,15 +5,16 @@
 // mesh instance, TLAS), :1035-1184 (isOccluded, traceRay).
 //
 // The node layout differs from the reference (64 B nodes holding both child
-// boxes, see device_scene.h) but every traversal decision is the same:
-//  * the reference pushes far then near and immediately pops near, whose
-//    re-test cannot fail (closestT is unchanged), so near is taken directly;
-//  * a popped entry is re-tested against the current closestT; the slab
-//    test's tmax part cannot change, so the re-test is tmin < closestT with
-//    the tmin stored at push time (bit-identical to recomputing it);
-//  * leaf children are visited exactly when the reference would pop them.
-// With the same BVH and IEEE FP32 arithmetic in the same order, hit
-// identities and distances are bit-identical to the reference algorithm.
+// boxes, or 256 B eight-wide nodes for flat triangles, see device_scene.h),
+// and one loop serves closest-hit and any-hit queries.  The primitive tests
+// are the reference's, in the same FP32 operation order, so each candidate's
+// distance and barycentrics are bit-identical to the reference's.  The
+// nearest hit is the candidate first in (distance, category, instance,
+// primitive) order.  That is the reference's result whenever the nearest
+// distance is held by one primitive, and across categories on exact ties;
+// on an exact tie inside one tree (a ray through a shared edge) the
+// reference keeps whichever triangle its tree visits first, while this
+// order keeps the lower index, whatever the tree.
 #pragma once
 
 #include "../common/vec.h"

@@ -73,7 +73,7 @@ struct RenderTile {
 // Paths of one pass carry their sample slot in the pixel word's top bits.
 constexpr uint32_t kSlotShift = 26;
 constexpr uint32_t kPixelMask = (1u << kSlotShift) - 1u;
-constexpr uint32_t kMaxSlots = 32;
+constexpr uint32_t kMaxSlots = 64;  // slot < 64: six bits above the 26 pixel bits
 
 // Device counters shared by the launches of one renderer.
 struct WavefrontCounters {

@@ -192,11 +192,12 @@ __device__ __forceinline__ void finishPath(const LaunchContext& ctx, uint32_t ta
 __global__ void __launch_bounds__(512) kPrimary(LaunchContext ctx) {
     RenderTile tile = *ctx.tile;
     uint32_t count = passPaths(ctx, tile);
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < kSegments) ctx.counters->count[0][g] = primarySegmentCount(count, ctx.blockSize, g);
-    if (g >= count) return;
-    storePath(ctx.queue[0], primarySlot(g, ctx.blockSize, ctx.queue[0].segmentCapacity),
-        passPath(ctx, tile, ctx.counters->sampleIndex, g));
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < kSegments) ctx.counters->count[0][gid] = primarySegmentCount(count, ctx.blockSize, gid);
+    uint32_t sample = ctx.counters->sampleIndex;
+    for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x)
+        storePath(ctx.queue[0], primarySlot(g, ctx.blockSize, ctx.queue[0].segmentCapacity),
+            passPath(ctx, tile, sample, g));
 }
 
 // End of a pass: each tile pixel adds its samples' radiance in sample order
@@ -230,29 +231,32 @@ __global__ void kReset(LaunchContext ctx, uint32_t nextSel) {
     }
 }
 
-// One ray per thread; the hardware dispatcher balances workgroups.
+// Rays in grid stride: the grid (at most a few times the resident
+// workgroups) bounds the traversal spill area, and the dispatcher balances
+// the workgroups.
 template <bool Full>
 __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kTrace(LaunchContext ctx, uint32_t bounce, uint32_t sel) {
     SegmentView v = loadSegments(ctx.counters->count[sel]);
     uint32_t count = v.start[kSegments];
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g == 0) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) {
         atomicAdd(&ctx.counters->closestQueries, static_cast<unsigned long long>(count));
         if (ctx.profileQueues && bounce < 64) ctx.counters->queueCounts[bounce] = count;
     }
-    if (g >= count) return;
-    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const PathQueue& q = ctx.queue[sel];
-    uint32_t i = segmentSlot(v, q.segmentCapacity, g);
-    Ray ray = loadRay(q, i);
-    HitRecord h = traceClosest<Full>(ctx.scene, ray, stack);
-    ctx.hits.t[i] = h.t;
-    ctx.hits.u[i] = h.u;
-    ctx.hits.v[i] = h.v;
-    ctx.hits.kind[i] = h.kind;
-    ctx.hits.instance[i] = h.instance;
-    ctx.hits.prim[i] = h.prim;
-    ctx.hits.material[i] = h.material;
+    for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x) {
+        LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+        uint32_t i = segmentSlot(v, q.segmentCapacity, g);
+        Ray ray = loadRay(q, i);
+        HitRecord h = traceClosest<Full>(ctx.scene, ray, stack);
+        ctx.hits.t[i] = h.t;
+        ctx.hits.u[i] = h.u;
+        ctx.hits.v[i] = h.v;
+        ctx.hits.kind[i] = h.kind;
+        ctx.hits.instance[i] = h.instance;
+        ctx.hits.prim[i] = h.prim;
+        ctx.hits.material[i] = h.material;
+    }
 }
 
 __device__ __forceinline__ HitRecord loadHit(const HitBuffer& hits, uint32_t i) {
@@ -286,66 +290,61 @@ __device__ __forceinline__ void storeShadow(const ShadowQueue& q, uint32_t s, co
     sfield(q, kSRz)[s] = __float_as_uint(path.radiance.z);
 }
 
-// Staged shading, one path per thread: continuing paths are appended to the
-// workgroup's segment of the next queue, shadow requests to its segment of
-// the shadow queue; finished paths without a pending shadow ray reach the
-// framebuffer.  With `Inline` (scenes with an environment map or punctual
-// lights, whose shading adds radiance after the area-light NEE) every
-// shadow ray is traced here and the shadow queue stays empty.
+// Staged shading in workgroup-uniform grid stride: continuing paths are
+// appended to the workgroup's segment of the next queue, shadow requests to
+// its segment of the shadow queue; finished paths without a pending shadow
+// ray reach the framebuffer.  With `Inline` (scenes with an environment map
+// or punctual lights, whose shading adds radiance after the area-light NEE)
+// every shadow ray is traced here and the shadow queue stays empty.
 template <bool Inline, bool Full>
 __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
     __shared__ AppendShared sh;
     SegmentView v = loadSegments(ctx.counters->count[sel]);
     uint32_t count = v.start[kSegments];
-    if (blockIdx.x * blockDim.x >= count) return;  // whole workgroup
     // Queue descriptors by value: a dynamic index into the kernel argument
     // makes the compiler copy the whole argument to scratch.
     const PathQueue q = sel ? ctx.queue[1] : ctx.queue[0];
     const PathQueue next = sel ? ctx.queue[0] : ctx.queue[1];
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = g < count;
-    PathState path;
-    ShadeOutcome outcome;
-    outcome.alive = false;
-    outcome.shadow.valid = 0;
-    if (valid) {
-        uint32_t i = segmentSlot(v, q.segmentCapacity, g);
-        path = loadPath(q, i);
-        if constexpr (Inline) {
-            LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
-            InlineOcclusion<LdsStack> occ{stack, 0u};
-            shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
-            if (occ.queries) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(occ.queries));
-        } else {
-            DeferOcclusion occ;
-            shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
+    const uint32_t seg = blockIdx.x % kSegments;
+    for (uint32_t first = blockIdx.x * blockDim.x; first < count; first += gridDim.x * blockDim.x) {
+        uint32_t g = first + threadIdx.x;
+        bool valid = g < count;
+        PathState path;
+        ShadeOutcome outcome;
+        outcome.alive = false;
+        outcome.shadow.valid = 0;
+        if (valid) {
+            uint32_t i = segmentSlot(v, q.segmentCapacity, g);
+            path = loadPath(q, i);
+            if constexpr (Inline) {
+                LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+                InlineOcclusion<LdsStack> occ{stack, 0u};
+                shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
+                if (occ.queries) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(occ.queries));
+            } else {
+                DeferOcclusion occ;
+                shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
+            }
         }
-    }
-    bool alive = valid && outcome.alive;
-    bool wantsShadow = valid && outcome.shadow.valid;
-    uint32_t seg = blockIdx.x % kSegments;
-    uint32_t a, s;
-    blockAppend(alive, wantsShadow, &ctx.counters->count[sel ^ 1u][seg], &ctx.counters->shadowCount[seg], sh, a, s);
-    uint32_t slot = seg * next.segmentCapacity + a;
-    if (alive) storePath(next, slot, path);
-    if (wantsShadow) {
-        storeShadow(ctx.shadow, seg * ctx.shadow.segmentCapacity + s, outcome.shadow, alive ? slot : kShadowTerminated,
-            path);
-    } else if (valid && !alive) {
-        finishPath(ctx, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
+        bool alive = valid && outcome.alive;
+        bool wantsShadow = valid && outcome.shadow.valid;
+        uint32_t a, s;
+        blockAppend(alive, wantsShadow, &ctx.counters->count[sel ^ 1u][seg], &ctx.counters->shadowCount[seg], sh, a, s);
+        uint32_t slot = seg * next.segmentCapacity + a;
+        if (alive) storePath(next, slot, path);
+        if (wantsShadow) {
+            storeShadow(ctx.shadow, seg * ctx.shadow.segmentCapacity + s, outcome.shadow,
+                alive ? slot : kShadowTerminated, path);
+        } else if (valid && !alive) {
+            finishPath(ctx, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
+        }
     }
 }
 
-// Resolves the bounce's shadow requests, one per thread: unoccluded
-// contributions are added to the continuing path's radiance, and finished
-// paths reach the pixel.
+// One shadow request: the contribution joins the continuing path's
+// radiance, or the finished path's pixel, unless the ray is blocked.
 template <bool Full>
-__global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadow(LaunchContext ctx, uint32_t sel) {
-    SegmentView v = loadSegments(ctx.counters->shadowCount);
-    uint32_t count = v.start[kSegments];
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g == 0) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(count));
-    if (g >= count) return;
+__device__ __forceinline__ void resolveShadow(const LaunchContext& ctx, const SegmentView& v, uint32_t sel, uint32_t g) {
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const ShadowQueue& q = ctx.shadow;
     const PathQueue& next = ctx.queue[sel ^ 1u];
@@ -372,6 +371,19 @@ __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadow(LaunchContext
         if (!occluded) r += c;
         finishPath(ctx, sfield(q, kSPixel)[i], r);
     }
+}
+
+// Resolves the bounce's shadow requests in grid stride: unoccluded
+// contributions are added to the continuing path's radiance, and finished
+// paths reach the pixel.
+template <bool Full>
+__global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadow(LaunchContext ctx, uint32_t sel) {
+    SegmentView v = loadSegments(ctx.counters->shadowCount);
+    uint32_t count = v.start[kSegments];
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(count));
+    for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x)
+        resolveShadow<Full>(ctx, v, sel, g);
 }
 
 __global__ void kAdvance(LaunchContext ctx) {

@@ -458,18 +458,23 @@ struct SplitNode {
     int32_t child[2];  // >= 0: node, < 0: -(reference index) - 1
 };
 
-struct SpatialBuilder {
+// Shared inputs of a spatial-split build.
+struct SpatialParams {
     const std::vector<TriShade>* tris;
-    std::vector<SplitRef> leaves;  // one per leaf, in creation order
-    std::vector<SplitNode> nodes;
-    size_t budget;                 // most references the build may create
-    size_t refs = 0;
     float rootArea = 0.0f;
+};
+
+// A built subtree: its leaves in creation order and its interior nodes.
+struct SpatialTree {
+    std::vector<SplitRef> leaves;
+    std::vector<SplitNode> nodes;
 };
 
 constexpr int kSpatialBins = 16;
 constexpr uint32_t kSpatialMaxDepth = 28;  // deeper nodes take object splits only
 constexpr float kSpatialOverlap = 1.0e-5f; // spatial splits only where object splits overlap this much (x root area)
+constexpr uint32_t kSpatialParallelDepth = 5;  // subtrees above this depth build on their own threads
+constexpr size_t kSpatialParallelMin = 4096;   // ... when they hold at least this many references
 
 inline Aabb unionBox(const Aabb& a, const Aabb& b) {
     Aabb r = a;
@@ -585,7 +590,7 @@ SplitChoice objectSplit(const std::vector<SplitRef>& refs, const Aabb& centroids
 
 // Spatial split: planes at bin boundaries of the node box; each reference
 // is clipped into every bin it overlaps.
-SplitChoice spatialSplit(const SpatialBuilder& sb, const std::vector<SplitRef>& refs, const Aabb& nodeBox) {
+SplitChoice spatialSplit(const SpatialParams& sp, const std::vector<SplitRef>& refs, const Aabb& nodeBox) {
     SplitChoice best;
     for (int axis = 0; axis < 3; ++axis) {
         float lo = nodeBox.mn[axis], extent = nodeBox.mx[axis] - lo;
@@ -595,7 +600,7 @@ SplitChoice spatialSplit(const SpatialBuilder& sb, const std::vector<SplitRef>& 
         uint32_t enter[kSpatialBins] = {}, exit[kSpatialBins] = {};
         for (int b = 0; b < kSpatialBins; ++b) box[b] = emptyBox();
         for (const SplitRef& r : refs) {
-            const TriShade& t = (*sb.tris)[r.prim];
+            const TriShade& t = (*sp.tris)[r.prim];
             int first = std::min(std::max(static_cast<int>((r.box.mn[axis] - lo) / width), 0), kSpatialBins - 1);
             int last = std::min(std::max(static_cast<int>((r.box.mx[axis] - lo) / width), first), kSpatialBins - 1);
             for (int b = first; b <= last; ++b) {
@@ -636,9 +641,25 @@ SplitChoice spatialSplit(const SpatialBuilder& sb, const std::vector<SplitRef>& 
     return best;
 }
 
-// Builds the subtree over `refs`; returns its child pointer (node index, or
-// -(leaf index) - 1) and writes its bounds.
-int32_t buildSpatial(SpatialBuilder& sb, std::vector<SplitRef>& refs, uint32_t depth, Aabb& bounds) {
+// Appends subtree `sub` to `out`; returns `pointer` (a pointer into `sub`)
+// translated into `out`.
+int32_t graftSubtree(SpatialTree& out, const SpatialTree& sub, int32_t pointer) {
+    int32_t nodeOffset = static_cast<int32_t>(out.nodes.size());
+    int32_t leafOffset = static_cast<int32_t>(out.leaves.size());
+    auto shift = [&](int32_t p) { return p >= 0 ? p + nodeOffset : p - leafOffset; };
+    out.leaves.insert(out.leaves.end(), sub.leaves.begin(), sub.leaves.end());
+    for (const SplitNode& n : sub.nodes) out.nodes.push_back(SplitNode{{n.box[0], n.box[1]}, {shift(n.child[0]), shift(n.child[1])}});
+    return shift(pointer);
+}
+
+// Builds the subtree over `refs` into `out`; returns its child pointer
+// (node index, or -(leaf index) - 1) and writes its bounds.  `budget` is
+// the most references the subtree may hold: a spatial split that would
+// exceed it is not taken, and the children share what is left in
+// proportion to their sizes, so the tree does not depend on the order in
+// which subtrees are built (the upper levels build in parallel).
+int32_t buildSpatial(const SpatialParams& sp, SpatialTree& out, std::vector<SplitRef>& refs, uint32_t depth,
+    size_t budget, Aabb& bounds) {
     bounds = emptyBox();
     Aabb centroids = emptyBox();
     for (const SplitRef& r : refs) {
@@ -648,13 +669,13 @@ int32_t buildSpatial(SpatialBuilder& sb, std::vector<SplitRef>& refs, uint32_t d
         grow(centroids, c);
     }
     if (refs.size() == 1) {
-        sb.leaves.push_back(refs[0]);
-        return -static_cast<int32_t>(sb.leaves.size());
+        out.leaves.push_back(refs[0]);
+        return -static_cast<int32_t>(out.leaves.size());
     }
     std::vector<SplitRef> left, right;
     SplitChoice obj = depth < kSpatialMaxDepth ? objectSplit(refs, centroids) : SplitChoice{};
     SplitChoice best = obj;
-    if (obj.axis >= 0 && depth < kSpatialMaxDepth && sb.refs < sb.budget) {
+    if (obj.axis >= 0 && depth < kSpatialMaxDepth && refs.size() < budget) {
         // Spatial splits pay only where the object split's children overlap.
         Aabb l = emptyBox(), r = emptyBox();
         for (const SplitRef& ref : refs) {
@@ -662,14 +683,14 @@ int32_t buildSpatial(SpatialBuilder& sb, std::vector<SplitRef>& refs, uint32_t d
             grow(cc < obj.plane ? l : r, ref.box);
         }
         Aabb overlap = intersectBox(l, r);
-        if (!emptyOrInverted(overlap) && area(overlap) > kSpatialOverlap * sb.rootArea) {
-            SplitChoice sp = spatialSplit(sb, refs, bounds);
-            if (sp.cost < best.cost) best = sp;
+        if (!emptyOrInverted(overlap) && area(overlap) > kSpatialOverlap * sp.rootArea) {
+            SplitChoice sc = spatialSplit(sp, refs, bounds);
+            if (sc.cost < best.cost) best = sc;
         }
     }
     if (best.axis >= 0 && best.spatial) {
         for (const SplitRef& ref : refs) {
-            const TriShade& t = (*sb.tris)[ref.prim];
+            const TriShade& t = (*sp.tris)[ref.prim];
             if (ref.box.mn[best.axis] < best.plane) {
                 Aabb region = ref.box;
                 region.mx[best.axis] = std::fmin(region.mx[best.axis], best.plane);
@@ -683,13 +704,14 @@ int32_t buildSpatial(SpatialBuilder& sb, std::vector<SplitRef>& refs, uint32_t d
                 if (!emptyOrInverted(part)) right.push_back(SplitRef{ref.prim, part});
             }
         }
-        if (left.empty() || right.empty() || left.size() + right.size() < refs.size()) {
+        if (left.empty() || right.empty() || left.size() + right.size() < refs.size() ||
+            left.size() + right.size() > budget) {
             left.clear();
             right.clear();
-        } else {
-            sb.refs += left.size() + right.size() - refs.size();
+            best = obj;
         }
-    } else if (best.axis >= 0) {
+    }
+    if (left.empty() && best.axis >= 0 && !best.spatial) {
         for (const SplitRef& ref : refs) {
             float cc = (ref.box.mn[best.axis] + ref.box.mx[best.axis]) * 0.5f;
             (cc < best.plane ? left : right).push_back(ref);
@@ -715,12 +737,26 @@ int32_t buildSpatial(SpatialBuilder& sb, std::vector<SplitRef>& refs, uint32_t d
         right.assign(refs.begin() + static_cast<std::ptrdiff_t>(mid), refs.end());
     }
     std::vector<SplitRef>().swap(refs);
-    int32_t index = static_cast<int32_t>(sb.nodes.size());
-    sb.nodes.push_back(SplitNode{});
+    size_t spare = budget - left.size() - right.size();
+    size_t leftSpare = static_cast<size_t>(static_cast<double>(spare) * static_cast<double>(left.size()) /
+        static_cast<double>(left.size() + right.size()));
+    size_t leftBudget = left.size() + leftSpare, rightBudget = right.size() + (spare - leftSpare);
+    int32_t index = static_cast<int32_t>(out.nodes.size());
+    out.nodes.push_back(SplitNode{});
     Aabb lb, rb;
-    int32_t lc = buildSpatial(sb, left, depth + 1, lb);
-    int32_t rc = buildSpatial(sb, right, depth + 1, rb);
-    SplitNode& node = sb.nodes[static_cast<size_t>(index)];
+    int32_t lc, rc;
+    if (depth < kSpatialParallelDepth && left.size() + right.size() >= kSpatialParallelMin) {
+        SpatialTree lt, rt;
+        std::thread worker([&] { lc = buildSpatial(sp, lt, left, depth + 1, leftBudget, lb); });
+        rc = buildSpatial(sp, rt, right, depth + 1, rightBudget, rb);
+        worker.join();
+        lc = graftSubtree(out, lt, lc);
+        rc = graftSubtree(out, rt, rc);
+    } else {
+        lc = buildSpatial(sp, out, left, depth + 1, leftBudget, lb);
+        rc = buildSpatial(sp, out, right, depth + 1, rightBudget, rb);
+    }
+    SplitNode& node = out.nodes[static_cast<size_t>(index)];
     node.box[0] = lb;
     node.box[1] = rb;
     node.child[0] = lc;
@@ -749,29 +785,29 @@ HostBvh buildSpatialSplitBvh2(const std::vector<TriShade>& tris) {
     uint32_t n = static_cast<uint32_t>(tris.size());
     for (const TriShade& t : tris) bvh.primBounds.push_back(triangleBounds(t.v0, t.v1, t.v2));
     if (n == 0) return bvh;
-    SpatialBuilder sb;
-    sb.tris = &tris;
-    sb.budget = static_cast<size_t>(n) * 2;  // at most one extra reference per triangle
-    sb.refs = n;
+    SpatialParams sp;
+    sp.tris = &tris;
+    SpatialTree tree;
     std::vector<SplitRef> refs(n);
     Aabb all = emptyBox();
     for (uint32_t i = 0; i < n; ++i) {
         refs[i] = SplitRef{i, bvh.primBounds[i]};
         grow(all, bvh.primBounds[i]);
     }
-    sb.rootArea = area(all);
+    sp.rootArea = area(all);
     Aabb rootBounds;
-    int32_t root = buildSpatial(sb, refs, 0, rootBounds);
+    // At most one extra reference per triangle.
+    int32_t root = buildSpatial(sp, tree, refs, 0, static_cast<size_t>(n) * 2, rootBounds);
     // Renumber depth first: a subtree over m leaves owns m - 1 consecutive
     // interior nodes, leaves take slots in the order the walk meets them.
-    uint32_t leaves = static_cast<uint32_t>(sb.leaves.size());
+    uint32_t leaves = static_cast<uint32_t>(tree.leaves.size());
     bvh.nodes.resize(leaves > 1 ? leaves - 1 : 0);
     bvh.order.reserve(leaves);
     bvh.slotBounds.reserve(leaves);
     uint32_t nextNode = 0;
     std::function<uint32_t(int32_t, uint32_t&)> emit = [&](int32_t ptr, uint32_t& depth) -> uint32_t {
         if (ptr < 0) {
-            const SplitRef& r = sb.leaves[static_cast<size_t>(-ptr - 1)];
+            const SplitRef& r = tree.leaves[static_cast<size_t>(-ptr - 1)];
             uint32_t slot = static_cast<uint32_t>(bvh.order.size());
             bvh.order.push_back(r.prim);
             bvh.slotBounds.push_back(r.box);
@@ -779,7 +815,7 @@ HostBvh buildSpatialSplitBvh2(const std::vector<TriShade>& tris) {
             return kLeafBit | slot;
         }
         uint32_t self = nextNode++;
-        const SplitNode sn = sb.nodes[static_cast<size_t>(ptr)];
+        const SplitNode sn = tree.nodes[static_cast<size_t>(ptr)];
         uint32_t dl = 0, dr = 0;
         uint32_t l = emit(sn.child[0], dl);
         uint32_t r = emit(sn.child[1], dr);
