@@ -50,11 +50,19 @@ HBM_PEAK_GBS = 8000.0
 # (center, radius, material), a mesh-instance record (transforms + ids).
 NODE_B, TRI_B, SPHERE_B, INSTANCE_B = 32, 36, 20, 96
 RAY_IN_B, HIT_OUT_B = 24, 28
-# HBM bytes per launch of the headline kernel on this workload, from
+# HBM bytes per launch of the dominant kernel on this workload, from
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a 256 MiB copy kernel
-# (profiles/pmc_traffic_kBatch_sphere_field_1080p_64spp.md): one launch = one
-# 64-sample frame of the regenerating scheduler.
-MEASURED_TRAFFIC = {("kBatch", 64): 38.86e9}
+# (FETCH_SIZE reads 131086 KiB for it: x2.000; WRITE_SIZE 262144 KiB: x1.000),
+# with the counters that say what bounds it.  kTrace: average of the 12
+# launches (bounces) of one 64-sample pass of the staged scheduler
+# (profiles/pmc_kTrace_sphere_field_1080p_64spp.md); kBatch: one 64-sample
+# frame of the regenerating scheduler
+# (profiles/pmc_kBatch_sphere_field_1080p_64spp.md).
+MEASURED_TRAFFIC = {("kTrace", 64): 47.69e9 / 12, ("kBatch", 64): 39.21e9}
+MEASURED_COUNTERS = {
+    ("kTrace", 64): {"l2_hit": 0.500, "valu_busy": 0.457, "waves_per_cu": 11.96, "wait_any": 0.669},
+    ("kBatch", 64): {"l2_hit": 0.615, "valu_busy": 0.303, "waves_per_cu": 7.87, "wait_any": 0.672},
+}
 
 
 def log(msg):
@@ -503,9 +511,36 @@ def measure_diff(steps):
         },
     }
     result["roughness_observability"] = roughness_observability(r, truth, target, lay)
+    result["decorrelated_loss"] = decorrelated_adam(r, truth, target, lay, steps, spp)
     log(f"diff: {result}")
     r.close()
     return result
+
+
+def decorrelated_adam(r, truth, target, lay, steps, spp):
+    """The same Adam run with the gradient of (X1 - T)(X2 - T) for two
+    independent 16-spp renders X1, X2 (the adjoint of X1's paths with X2's
+    residual) instead of (X - T)^2.  The squared loss of one noisy render
+    adds the render's variance, which falls as roughness grows, so its
+    expected gradient pulls roughness up; the product of independent
+    residuals is an unbiased estimate of the squared error of the mean."""
+    import torch
+    params = torch.tensor(diff_parameters(lay, 0xA, False), device="cuda:0")
+    m = torch.zeros_like(params)
+    v = torch.zeros_like(params)
+    t0 = time.perf_counter()
+    for step in range(1, steps + 1):
+        other = r.forward(params, sample_start=(1 << 22) + step * spp)
+        img = r.forward(params, sample_start=step * spp)
+        grad = r.adjoint(params, 2.0 * (other - target) / img.numel())
+        r.adam(params, grad, m, v, step, 0.01)
+    wall = time.perf_counter() - t0
+    t, rc = lay.texture_floats, lay.roughness_count
+    err = lambda a, b: float((a - b).abs().mean())
+    return {"adam_steps": steps, "wall_ms_per_iter": round(wall * 1000.0 / steps, 3),
+            "param_mae_end": {"texture": err(params[:t], truth[:t]),
+                              "roughness": err(params[t:t + rc], truth[t:t + rc]),
+                              "emission": err(params[t + rc:], truth[t + rc:])}}
 
 
 def roughness_observability(r, truth, target, lay, seeds=8):
@@ -815,6 +850,11 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "kernel": kernel,
             "traffic": MEASURED_TRAFFIC.get((kernel, args.spp)),
+            "hbm_gbs_measured": None if (kernel, args.spp) not in MEASURED_TRAFFIC else
+                round(MEASURED_TRAFFIC[(kernel, args.spp)] / (trace_ms / max(1, trace_launches) / 1000.0) / 1e9, 1),
+            "limiter": "latency of dependent node and triangle fetches served mostly by L2/MALL "
+                       "(algorithmic bytes count every node visit; the measured HBM traffic is a small part of them)",
+            "counters": MEASURED_COUNTERS.get((kernel, args.spp)),
             "bytes_per_closest_query": round(per_query, 1),
             "avg_launch_ms": round(trace_ms / max(1, trace_launches), 4),
             "launches": trace_launches,
