@@ -709,7 +709,13 @@ constexpr uint32_t kNoAllocMaxItems = 1u << 24;
 constexpr uint64_t kDefaultPassPaths = 1ull << 27;
 // Staged kernels launch at most this many times the trace kernel's resident
 // workgroups (the dispatcher still balances the workgroups).
-constexpr uint32_t kStagedGridFactor = 4;
+#ifndef WPT_STAGED_GRID_FACTOR
+#define WPT_STAGED_GRID_FACTOR 4
+#endif
+constexpr uint32_t kStagedGridFactor = WPT_STAGED_GRID_FACTOR;
+// Most radiance events per vertex that event shading stores (scenes with
+// more directional lights trace their shadow rays inline).
+constexpr uint32_t kMaxVertexEvents = 12;
 constexpr uint32_t kNoAllocScratchWords = 1u + kNoAllocMaxItems / 256u;
 static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
 
@@ -766,10 +772,23 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     r->fingerprint = fingerprint;
     const wpt_render_config& c = r->config;
     uint32_t tilePixels = c.tiled ? c.tile_width * c.tile_height : c.width * c.height;
+    // Scenes whose shading adds radiance after an occlusion test (environment,
+    // punctual lights, caustics) shade through ordered events in the staged
+    // scheduler, at most `vertexEvents` per vertex: the area-light sample, every
+    // directional light, one positional light, the environment sample, then
+    // two escape tests (dielectric, glTF delta lobes) or the caustic gather.
+    const DevScene& sceneView = scene->dev;
+    bool inlineShadows = shadesWithInlineShadows(sceneView) || (c.caustic_photons && c.caustic_gather);
+    uint32_t vertexEvents = (sceneView.lightCount ? 1u : 0u) + sceneView.directionalCount +
+        (sceneView.punctualCount > sceneView.directionalCount ? 1u : 0u) + (sceneView.environment.texels ? 1u : 0u) + 2u;
+    bool eventShading = c.mode == WPT_MODE_GRAPH && inlineShadows && vertexEvents <= kMaxVertexEvents;
     // Samples per pass of the staged and cooperative schedulers: by default
-    // as many as keep a pass within kDefaultPassPaths paths (64 at 1080p).
+    // as many as keep a pass within kDefaultPassPaths paths (64 at 1080p;
+    // half that with event shading, whose events take about 200 B more per
+    // path).
+    uint64_t passPaths = eventShading ? kDefaultPassPaths / 2 : kDefaultPassPaths;
     uint32_t slots = c.samples_per_pass ? c.samples_per_pass
-                                        : static_cast<uint32_t>(std::max<uint64_t>(1u, kDefaultPassPaths / tilePixels));
+                                        : static_cast<uint32_t>(std::max<uint64_t>(1u, passPaths / tilePixels));
     if (c.mode == WPT_MODE_REGENERATE) slots = 1u;
     slots = std::max(1u, std::min(std::min(slots, kMaxSlots), c.samples_per_pixel));
     uint64_t cap64 = static_cast<uint64_t>(tilePixels) * slots;
@@ -881,10 +900,19 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
         ctx.hits.instance = u + qcap;
         ctx.hits.prim = u + 2 * static_cast<size_t>(qcap);
         ctx.hits.material = u + 3 * static_cast<size_t>(qcap);
-        R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kShadowFieldCount) * qcap, &p));
-        ctx.shadow.words = static_cast<uint32_t*>(p);
-        ctx.shadow.capacity = static_cast<uint32_t>(qcap);
-        ctx.shadow.segmentCapacity = static_cast<uint32_t>(segment);
+        if (!inlineShadows) {  // deferred area-light shadow rays
+            R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kShadowFieldCount) * qcap, &p));
+            ctx.shadow.words = static_cast<uint32_t*>(p);
+            ctx.shadow.capacity = static_cast<uint32_t>(qcap);
+            ctx.shadow.segmentCapacity = static_cast<uint32_t>(segment);
+        }
+        if (eventShading) {
+            size_t words = static_cast<size_t>(kEventHeaderFields + vertexEvents * kEventItemFields) * qcap;
+            R_HIP(r->arena.alloc(sizeof(uint32_t) * words, &p));
+            ctx.events.words = static_cast<uint32_t*>(p);
+            ctx.events.capacity = static_cast<uint32_t>(qcap);
+            ctx.events.maxEvents = vertexEvents;
+        }
     }
     R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height, &p));
     ctx.framebuffer = static_cast<float*>(p);

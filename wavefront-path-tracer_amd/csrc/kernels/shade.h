@@ -50,13 +50,19 @@ struct ShadeParams {
 
 WPT_HD float rand01(PathState& p, uint32_t strategy) { return rngNextFloat(p.rng, strategy); }
 
-WPT_HD void addPathRadiance(PathState& path, Vec3 contribution, float clamp, bool emissionOrBackground) {
+// The indirect clamp of addPathRadiance: it depends on the contribution and
+// the path's depth only, so a deferred addition can apply it up front.
+WPT_HD Vec3 clampContribution(const PathState& path, Vec3 contribution, float clamp, bool emissionOrBackground) {
     bool indirect = path.depth > (emissionOrBackground ? 1u : 0u);
     if (indirect && clamp > 0.0f) {
         float magnitude = fabsf(contribution.x) / 3.0f + fabsf(contribution.y) / 3.0f + fabsf(contribution.z) / 3.0f;
         if (magnitude > clamp) contribution *= clamp / magnitude;
     }
-    path.radiance += contribution;
+    return contribution;
+}
+
+WPT_HD void addPathRadiance(PathState& path, Vec3 contribution, float clamp, bool emissionOrBackground) {
+    path.radiance += clampContribution(path, contribution, clamp, emissionOrBackground);
 }
 
 WPT_HD float powerHeuristic(float a, float b) {
@@ -266,8 +272,100 @@ struct ShadowRequest {
 
 struct DeferOcclusion {
     static constexpr bool kInline = false;
+    static constexpr bool kEvents = false;
     WPT_HD bool blocked(const DevScene&, const Ray&, float) { return false; }
+    WPT_HD uint32_t record(const Ray&, float, const Vec3&, bool) { return 0; }
+    WPT_HD void add(PathState& path, const Vec3& c) { path.radiance += c; }
+    WPT_HD void terminateIfVisible(uint32_t) {}
 };
+
+// Ordered radiance events of one vertex, for staged shading that traces no
+// ray itself.  Until the vertex's first occlusion-dependent addition, every
+// addition goes to the path's radiance at once; from it on, each addition
+// (the clamped contribution, with the ray and distance that must be
+// unblocked for it, or none) is stored in order in the event buffer at the
+// path's queue slot.  A later pass traces the rays and adds the visible
+// contributions in the stored order, so the path's radiance sees the same
+// floating-point additions as shading that traces inline.  An event can also
+// end the path when its ray escapes (dielectric and glTF delta lobes toward
+// the environment).
+enum EventField : uint32_t {
+    kEvCount, kEvTarget, kEvPixel, kEvRx, kEvRy, kEvRz,  // per slot: events, continuation, pixel, radiance before them
+    kEventHeaderFields,
+};
+enum EventItemField : uint32_t {
+    kEiOx, kEiOy, kEiOz, kEiDx, kEiDy, kEiDz, kEiTmax, kEiCx, kEiCy, kEiCz, kEiFlags,
+    kEventItemFields,
+};
+constexpr uint32_t kEventTraced = 1u;      // the ray must be unblocked for the addition
+constexpr uint32_t kEventTerminates = 2u;  // an unblocked ray ends the path
+
+struct EventBuffer {
+    uint32_t* words;      // (kEventHeaderFields + maxEvents * kEventItemFields) x capacity
+    uint32_t capacity;
+    uint32_t maxEvents;
+};
+
+WPT_HD uint32_t* eventWord(const EventBuffer& b, uint32_t field, uint32_t slot) {
+    return b.words + static_cast<size_t>(field) * b.capacity + slot;
+}
+WPT_HD uint32_t* eventItemWord(const EventBuffer& b, uint32_t event, uint32_t field, uint32_t slot) {
+    return eventWord(b, kEventHeaderFields + event * kEventItemFields + field, slot);
+}
+
+struct EventOcclusion {
+    static constexpr bool kInline = false;
+    static constexpr bool kEvents = true;
+    EventBuffer buffer;
+    uint32_t slot;
+    uint32_t count;
+    WPT_HD bool blocked(const DevScene&, const Ray&, float) { return false; }
+    // Stores event `count` (bounded by the buffer's maxEvents, which the
+    // renderer sizes from the scene's lights and materials) and returns its
+    // index.
+    WPT_HD uint32_t record(const Ray& ray, float tmax, const Vec3& c, bool traced) {
+        uint32_t e = count++;
+        if (e >= buffer.maxEvents) return e;
+        *eventItemWord(buffer, e, kEiOx, slot) = bitsOf(ray.origin.x);
+        *eventItemWord(buffer, e, kEiOy, slot) = bitsOf(ray.origin.y);
+        *eventItemWord(buffer, e, kEiOz, slot) = bitsOf(ray.origin.z);
+        *eventItemWord(buffer, e, kEiDx, slot) = bitsOf(ray.direction.x);
+        *eventItemWord(buffer, e, kEiDy, slot) = bitsOf(ray.direction.y);
+        *eventItemWord(buffer, e, kEiDz, slot) = bitsOf(ray.direction.z);
+        *eventItemWord(buffer, e, kEiTmax, slot) = bitsOf(tmax);
+        *eventItemWord(buffer, e, kEiCx, slot) = bitsOf(c.x);
+        *eventItemWord(buffer, e, kEiCy, slot) = bitsOf(c.y);
+        *eventItemWord(buffer, e, kEiCz, slot) = bitsOf(c.z);
+        *eventItemWord(buffer, e, kEiFlags, slot) = traced ? kEventTraced : 0u;
+        return e;
+    }
+    WPT_HD void add(PathState& path, const Vec3& c) {
+        if (count == 0)
+            path.radiance += c;
+        else
+            record(Ray(Vec3(0.0f, 0.0f, 0.0f), Vec3(0.0f, 0.0f, 1.0f)), 0.0f, c, false);
+    }
+    WPT_HD void terminateIfVisible(uint32_t e) {
+        if (e < count && e < buffer.maxEvents) *eventItemWord(buffer, e, kEiFlags, slot) = kEventTraced | kEventTerminates;
+    }
+};
+
+// A vertex's radiance addition: at once, or (event shading) as the next
+// event, traced unless `traced` is false.  With inline or deferred
+// occlusion the caller has already tested the ray.
+template <class Occluder>
+WPT_HD void emitRadiance(Occluder& occ, PathState& path, const Vec3& contribution, float clamp, const Ray& ray,
+    float tmax, bool traced) {
+    if constexpr (Occluder::kEvents) {
+        Vec3 c = clampContribution(path, contribution, clamp, false);
+        if (traced)
+            occ.record(ray, tmax, c, true);
+        else
+            occ.add(path, c);
+    } else {
+        addPathRadiance(path, contribution, clamp, false);
+    }
+}
 
 // Shadow and escape rays traced where shading draws them.  The full-feature
 // shading reaches this from about ten sites; as a call it exists once in the
@@ -276,12 +374,16 @@ struct DeferOcclusion {
 template <class Stack>
 struct InlineOcclusion {
     static constexpr bool kInline = true;
+    static constexpr bool kEvents = false;
     Stack& stack;
     uint32_t queries;
     WPT_CALL bool blocked(const DevScene& s, const Ray& ray, float tmax) {
         ++queries;
         return traceOccluded(s, ray, tmax, stack);
     }
+    WPT_HD uint32_t record(const Ray&, float, const Vec3&, bool) { return 0; }
+    WPT_HD void add(PathState& path, const Vec3& c) { path.radiance += c; }
+    WPT_HD void terminateIfVisible(uint32_t) {}
 };
 
 // ---- environment sampling and MIS (device_traversal.cuh:403-425, 571-587,
@@ -386,6 +488,11 @@ WPT_HD void sampleDirectLight(const DevScene& s, PathState& path, const SurfaceH
     float misWeight = powerHeuristic(lightPdf, bsdfPdf);
     Vec3 directLighting = hadamard(bsdf, lightMaterial.emission) * (geometryTerm * misWeight);
     Vec3 contribution = hadamard(path.throughput, directLighting);
+    if constexpr (Occluder::kEvents) {
+        emitRadiance(occ, path, contribution, sp.clamp, shadow, shadowDistance - kRayMinDistance,
+            !s.ignoreDirectLightOcclusion);
+        return;
+    }
     if (Occluder::kInline || s.ignoreDirectLightOcclusion) {
         addPathRadiance(path, contribution, sp.clamp, false);
         return;
@@ -459,11 +566,13 @@ WPT_HD void addPunctualLight(const DevScene& s, PathState& path, const SurfaceHi
         if (!(shadowDistance > kRayMinDistance)) return;
         shadow.direction = segment / shadowDistance;
     }
-    if (!s.ignoreDirectLightOcclusion && occ.blocked(s, shadow, shadowDistance - kRayMinDistance)) return;
+    bool traced = !s.ignoreDirectLightOcclusion;
+    if (traced && occ.blocked(s, shadow, shadowDistance - kRayMinDistance)) return;
     float ignoredPdf;
     Vec3 bsdf = bsdfAt.evalCos(s, -path.ray.direction, light.direction, ignoredPdf);
     Vec3 direct = hadamard(bsdf, light.incident) * inverseSelectionPdf;
-    addPathRadiance(path, hadamard(path.throughput, direct), sp.clamp, false);
+    emitRadiance(occ, path, hadamard(path.throughput, direct), sp.clamp, shadow, shadowDistance - kRayMinDistance,
+        traced);
 }
 
 // Every directional light, then one positional light chosen uniformly (one
@@ -494,12 +603,13 @@ WPT_HD void sampleDirectEnvironment(const DevScene& s, PathState& path, const Su
         !(light.pdf > 0.0f))
         return;
     Ray shadow(offsetRayOrigin(hit.position, hit.geometricNormal, light.direction, hit.t), light.direction);
-    if (!s.ignoreDirectLightOcclusion && occ.blocked(s, shadow, 1.0e30f)) return;
+    bool traced = !s.ignoreDirectLightOcclusion;
+    if (traced && occ.blocked(s, shadow, 1.0e30f)) return;
     float bsdfPdf;
     Vec3 bsdf = bsdfAt.evalCos(s, -path.ray.direction, light.direction, bsdfPdf);
     float weight = powerHeuristic(light.pdf, bsdfPdf);
     Vec3 direct = hadamard(bsdf, environmentRadiance(m, light.direction)) * (weight / light.pdf);
-    addPathRadiance(path, hadamard(path.throughput, direct), sp.clamp, false);
+    emitRadiance(occ, path, hadamard(path.throughput, direct), sp.clamp, shadow, 1.0e30f, traced);
 }
 
 // ---- continuations
@@ -555,22 +665,34 @@ WPT_HD Vec3 continueDielectric(const DevScene& s, PathState& path, const Surface
         shadingNormal, m.ior);
     float etaSquared = boundary.eta * boundary.eta;
     bool reflectionEscapes = false, transmissionEscapes = false;
+    uint32_t reflectionEvent = kInvalid, transmissionEvent = kInvalid;  // event shading
     if (Full && s.environment.texels && path.depth + 1 < sp.maxDepth) {
         if (boundary.reflectance > 0.0f) {
             Vec3 direction = boundary.reflection;
             Ray ray(offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t), direction);
-            reflectionEscapes = !occ.blocked(s, ray, 1.0e30f);
-            if (reflectionEscapes)
-                addPathRadiance(path, hadamard(path.throughput, environmentRadiance(s.environment, direction)) *
-                    boundary.reflectance, sp.clamp, false);
+            if constexpr (Occluder::kEvents) {
+                reflectionEvent = occ.record(ray, 1.0e30f, clampContribution(path, hadamard(path.throughput,
+                    environmentRadiance(s.environment, direction)) * boundary.reflectance, sp.clamp, false), true);
+            } else {
+                reflectionEscapes = !occ.blocked(s, ray, 1.0e30f);
+                if (reflectionEscapes)
+                    addPathRadiance(path, hadamard(path.throughput, environmentRadiance(s.environment, direction)) *
+                        boundary.reflectance, sp.clamp, false);
+            }
         }
         if (boundary.reflectance < 1.0f) {
             Vec3 direction = boundary.transmission;
             Ray ray(offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t), direction);
-            transmissionEscapes = !occ.blocked(s, ray, 1.0e30f);
-            if (transmissionEscapes)
-                addPathRadiance(path, hadamard(path.throughput, environmentRadiance(s.environment, direction)) *
-                    ((1.0f - boundary.reflectance) * etaSquared), sp.clamp, false);
+            if constexpr (Occluder::kEvents) {
+                transmissionEvent = occ.record(ray, 1.0e30f, clampContribution(path, hadamard(path.throughput,
+                    environmentRadiance(s.environment, direction)) * ((1.0f - boundary.reflectance) * etaSquared),
+                    sp.clamp, false), true);
+            } else {
+                transmissionEscapes = !occ.blocked(s, ray, 1.0e30f);
+                if (transmissionEscapes)
+                    addPathRadiance(path, hadamard(path.throughput, environmentRadiance(s.environment, direction)) *
+                        ((1.0f - boundary.reflectance) * etaSquared), sp.clamp, false);
+            }
         }
     }
     bool reflected = boundary.reflectance >= 1.0f || boundary.reflectance > rand01(path, sp.rngStrategy);
@@ -585,7 +707,10 @@ WPT_HD Vec3 continueDielectric(const DevScene& s, PathState& path, const Surface
             path.mediumDensity = 1.0f;
         }
     }
-    if (reflected ? reflectionEscapes : transmissionEscapes) path.active = 0;
+    if constexpr (Occluder::kEvents)
+        occ.terminateIfVisible(reflected ? reflectionEvent : transmissionEvent);
+    else if (reflected ? reflectionEscapes : transmissionEscapes)
+        path.active = 0;
     path.specularBounce = 1;
     path.previousBsdfPdf = 0.0f;
     // Per-component selects keep the interface in registers.
@@ -610,6 +735,7 @@ WPT_HD Vec3 continueGltf(const DevScene& s, PathState& path, const SurfaceHit& h
     bool specular = !coat && choice < weights.coat + weights.specular;
     bool delta = (coat && g.coatRoughness <= 0.0f) || (specular && g.roughness <= 0.0f);
     bool baseEscapes = false, coatEscapes = false;
+    uint32_t baseEvent = kInvalid, coatEvent = kInvalid;  // event shading
     Vec3 baseCoefficient = pbrBaseFresnel(g, dot(pbrSpecularNormal(g), view)) * (1.0f - weights.coat);
     if (s.environment.texels && path.depth + 1 < sp.maxDepth) {
         for (int lobe = 0; lobe < 2; ++lobe) {
@@ -621,6 +747,16 @@ WPT_HD Vec3 continueGltf(const DevScene& s, PathState& path, const SurfaceHit& h
             Vec3 direction = pbrSampleReflection(normal, view, 0.0f, 0.0f, 0.0f);
             if (dot(normal, view) <= 0.0f || dot(direction, hit.geometricNormal) <= 0.0f) continue;
             Ray ray(offsetRayOrigin(hit.position, hit.geometricNormal, direction, hit.t), direction);
+            if constexpr (Occluder::kEvents) {
+                Vec3 coefficient = isCoat ? Vec3(weights.coat, weights.coat, weights.coat) : baseCoefficient;
+                uint32_t e = occ.record(ray, 1.0e30f, clampContribution(path, hadamard(path.throughput,
+                    hadamard(coefficient, environmentRadiance(s.environment, direction))), sp.clamp, false), true);
+                if (isCoat)
+                    coatEvent = e;
+                else
+                    baseEvent = e;
+                continue;
+            }
             bool escapes = !occ.blocked(s, ray, 1.0e30f);
             if (isCoat)
                 coatEscapes = escapes;
@@ -658,7 +794,10 @@ WPT_HD Vec3 continueGltf(const DevScene& s, PathState& path, const SurfaceHit& h
         path.throughput = hadamard(path.throughput, coefficient) / mass;
         path.specularBounce = 1;
         path.previousBsdfPdf = 0.0f;
-        if (coat ? coatEscapes : baseEscapes) path.active = 0;
+        if constexpr (Occluder::kEvents)
+            occ.terminateIfVisible(coat ? coatEvent : baseEvent);
+        else if (coat ? coatEscapes : baseEscapes)
+            path.active = 0;
     } else {
         float pdf;
         Vec3 value = evaluateGltfCos(g, view, direction, diffuseContinuationPdf(s, g.normal, direction), pdf);
@@ -767,7 +906,8 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
         sampleDirectEnvironment(s, path, hit, shadingNormal, bsdf, sp, occ);
         if (Full && sp.photons) {  // device_path_shading.cuh:99-102
             Vec3 caustic = gatherCaustics(*sp.photons, hit.position, shadingNormal);
-            addPathRadiance(path, hadamard(path.throughput, hadamard(albedo, caustic)), sp.clamp, false);
+            emitRadiance(occ, path, hadamard(path.throughput, hadamard(albedo, caustic)), sp.clamp, path.ray, 0.0f,
+                false);
         }
         direction = continueDiffuse<Full>(s, path, hit, shadingNormal, albedo, sp.rngStrategy);
         break;

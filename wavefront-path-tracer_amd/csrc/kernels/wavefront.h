@@ -17,7 +17,7 @@ enum QueueField : uint32_t {
     kTx, kTy, kTz,                 // throughput
     kRx, kRy, kRz,                 // radiance
     kPixel,
-    kDepthFlags,                   // depth | specularBounce << 31
+    kDepthFlags,                   // depth | ended << 30 | specularBounce << 31
     kRngLo, kRngHi,
     kMedium, kMediumDensity, kPrevPdf, kEtaScale,
     kQueueFieldCount
@@ -102,6 +102,11 @@ struct LaunchContext {
     PathQueue queue[2];
     ShadowQueue shadow;
     HitBuffer hits;
+    // Staged scenes whose shading adds radiance after an occlusion test
+    // (environment, punctual lights, caustics): each shaded path's ordered
+    // radiance events, indexed by its slot in the queue being shaded
+    // (words null when the scene does not need them).
+    EventBuffer events;
     float* framebuffer;      // width*height*3
     // Staged and persistent passes render `slots` samples of a tile at once:
     // a finished path stores its radiance in its sample's plane of
@@ -116,7 +121,7 @@ struct LaunchContext {
     uint32_t spillThreads;
     uint32_t capacity;         // tile pixels x slots: the most items any queue holds
     uint32_t blockSize;
-    uint32_t gridBlocks;       // ceil(capacity / blockSize): one item per thread
+    uint32_t gridBlocks;       // staged kernels' grid (grid stride over their items)
     uint32_t resolveBlocks;    // ceil(tile pixels / blockSize)
     uint32_t persistentBlocks;
     uint32_t profileQueues;

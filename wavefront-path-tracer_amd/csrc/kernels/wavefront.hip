@@ -106,6 +106,11 @@ __device__ uint32_t orderedSlots(bool alive, uint32_t chunk, uint32_t* flags, Co
     return sh.base + sh.total;
 }
 
+// A queued path whose continuation an escaping event ended after it was
+// appended (event shading): traced with its bounce, then skipped.
+constexpr uint32_t kEndedBit = 0x40000000u;
+constexpr uint32_t kDepthMask = 0x3fffffffu;
+
 __device__ __forceinline__ uint32_t* field(const PathQueue& q, uint32_t f) {
     return q.words + static_cast<size_t>(f) * q.capacity;
 }
@@ -129,14 +134,14 @@ __device__ __forceinline__ PathState loadPath(const PathQueue& q, uint32_t i) {
     p.radiance = Vec3(qf(q, kRx, i), qf(q, kRy, i), qf(q, kRz, i));
     p.pixel = qu(q, kPixel, i);
     uint32_t df = qu(q, kDepthFlags, i);
-    p.depth = df & 0x7fffffffu;
+    p.depth = df & kDepthMask;
     p.specularBounce = df >> 31;
     p.rng = static_cast<uint64_t>(qu(q, kRngLo, i)) | (static_cast<uint64_t>(qu(q, kRngHi, i)) << 32);
     p.mediumMaterial = qu(q, kMedium, i);
     p.mediumDensity = qf(q, kMediumDensity, i);
     p.previousBsdfPdf = qf(q, kPrevPdf, i);
     p.etaScale = qf(q, kEtaScale, i);
-    p.active = 1;
+    p.active = (df & kEndedBit) ? 0u : 1u;
     return p;
 }
 
@@ -154,7 +159,7 @@ __device__ __forceinline__ void storePath(const PathQueue& q, uint32_t i, const 
     sf(q, kRy, i, p.radiance.y);
     sf(q, kRz, i, p.radiance.z);
     su(q, kPixel, i, p.pixel);
-    su(q, kDepthFlags, i, (p.depth & 0x7fffffffu) | (p.specularBounce ? 0x80000000u : 0u));
+    su(q, kDepthFlags, i, (p.depth & kDepthMask) | (p.specularBounce ? 0x80000000u : 0u));
     su(q, kRngLo, i, static_cast<uint32_t>(p.rng));
     su(q, kRngHi, i, static_cast<uint32_t>(p.rng >> 32));
     su(q, kMedium, i, p.mediumMaterial);
@@ -339,6 +344,124 @@ __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
             finishPath(ctx, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
         }
     }
+}
+
+// Staged shading that traces no ray (scenes whose shading adds radiance
+// after an occlusion test): each path's occlusion-dependent additions become
+// ordered events at its slot (EventOcclusion), continuing paths are appended
+// to the workgroup's segment of the next queue, and kShadowEvents then
+// traces the events' rays and completes the radiance.  Paths without events
+// finish or continue here as in kShade.
+template <bool Full>
+__global__ void __launch_bounds__(512) kShadeEvents(LaunchContext ctx, uint32_t sel) {
+    __shared__ AppendShared sh;
+    SegmentView v = loadSegments(ctx.counters->count[sel]);
+    uint32_t count = v.start[kSegments];
+    const PathQueue q = sel ? ctx.queue[1] : ctx.queue[0];
+    const PathQueue next = sel ? ctx.queue[0] : ctx.queue[1];
+    const uint32_t seg = blockIdx.x % kSegments;
+    for (uint32_t first = blockIdx.x * blockDim.x; first < count; first += gridDim.x * blockDim.x) {
+        uint32_t g = first + threadIdx.x;
+        bool valid = g < count;
+        uint32_t i = 0, events = 0;
+        PathState path;
+        ShadeOutcome outcome;
+        outcome.alive = false;
+        outcome.isDebug = false;
+        if (valid) {
+            i = segmentSlot(v, q.segmentCapacity, g);
+            path = loadPath(q, i);
+            if (path.active) {
+                EventOcclusion occ{ctx.events, i, 0u};
+                shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
+                events = occ.count;
+            } else {
+                valid = false;  // ended by an event of the previous bounce
+            }
+            *eventWord(ctx.events, kEvCount, i) = events;
+        }
+        bool alive = valid && outcome.alive;
+        uint32_t a, s;
+        blockAppend(alive, false, &ctx.counters->count[sel ^ 1u][seg], &ctx.counters->shadowCount[seg], sh, a, s);
+        uint32_t slot = seg * next.segmentCapacity + a;
+        if (alive) storePath(next, slot, path);
+        if (events) {
+            *eventWord(ctx.events, kEvTarget, i) = alive ? slot : kShadowTerminated;
+            *eventWord(ctx.events, kEvPixel, i) = path.pixel;
+            *eventWord(ctx.events, kEvRx, i) = __float_as_uint(path.radiance.x);
+            *eventWord(ctx.events, kEvRy, i) = __float_as_uint(path.radiance.y);
+            *eventWord(ctx.events, kEvRz, i) = __float_as_uint(path.radiance.z);
+        } else if (valid && !alive) {
+            finishPath(ctx, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
+        }
+    }
+}
+
+// The events of one shaded path, in order: each traced ray is tested, the
+// visible contributions are added to the radiance the path had before its
+// first event, and the result goes to the continuing path or, when the path
+// ended (or an escaping event ends it), to its pixel.
+template <bool Full>
+__device__ __forceinline__ uint32_t resolveEvents(const LaunchContext& ctx, const SegmentView& v, uint32_t sel,
+    uint32_t g) {
+    const PathQueue& q = ctx.queue[sel];
+    const PathQueue& next = ctx.queue[sel ^ 1u];
+    uint32_t i = segmentSlot(v, q.segmentCapacity, g);
+    uint32_t n = *eventWord(ctx.events, kEvCount, i);
+    if (n == 0) return 0;
+    if (n > ctx.events.maxEvents) n = ctx.events.maxEvents;
+    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+    Vec3 r(__uint_as_float(*eventWord(ctx.events, kEvRx, i)), __uint_as_float(*eventWord(ctx.events, kEvRy, i)),
+        __uint_as_float(*eventWord(ctx.events, kEvRz, i)));
+    bool ends = false;
+    uint32_t traced = 0;
+    for (uint32_t e = 0; e < n; ++e) {
+        uint32_t flags = *eventItemWord(ctx.events, e, kEiFlags, i);
+        bool visible = true;
+        if (flags & kEventTraced) {
+            Ray ray(Vec3(__uint_as_float(*eventItemWord(ctx.events, e, kEiOx, i)),
+                        __uint_as_float(*eventItemWord(ctx.events, e, kEiOy, i)),
+                        __uint_as_float(*eventItemWord(ctx.events, e, kEiOz, i))),
+                Vec3(__uint_as_float(*eventItemWord(ctx.events, e, kEiDx, i)),
+                    __uint_as_float(*eventItemWord(ctx.events, e, kEiDy, i)),
+                    __uint_as_float(*eventItemWord(ctx.events, e, kEiDz, i))));
+            visible = !traceOccluded<Full>(ctx.scene, ray, __uint_as_float(*eventItemWord(ctx.events, e, kEiTmax, i)),
+                stack);
+            ++traced;
+        }
+        if (visible) {
+            r += Vec3(__uint_as_float(*eventItemWord(ctx.events, e, kEiCx, i)),
+                __uint_as_float(*eventItemWord(ctx.events, e, kEiCy, i)),
+                __uint_as_float(*eventItemWord(ctx.events, e, kEiCz, i)));
+            if (flags & kEventTerminates) ends = true;
+        }
+    }
+    uint32_t target = *eventWord(ctx.events, kEvTarget, i);
+    if (target != kShadowTerminated && !ends) {
+        sf(next, kRx, target, r.x);
+        sf(next, kRy, target, r.y);
+        sf(next, kRz, target, r.z);
+    } else {
+        finishPath(ctx, *eventWord(ctx.events, kEvPixel, i), r);
+        if (target != kShadowTerminated) {
+            su(next, kDepthFlags, target, qu(next, kDepthFlags, target) | kEndedBit);
+            // The next bounce traces the ended path with the others; it is
+            // not a query of the path tracer.
+            atomicAdd(&ctx.counters->closestQueries, ~0ull);
+        }
+    }
+    return traced;
+}
+
+template <bool Full>
+__global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadowEvents(LaunchContext ctx, uint32_t sel) {
+    SegmentView v = loadSegments(ctx.counters->count[sel]);
+    uint32_t count = v.start[kSegments];
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long traced = 0;
+    for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x) traced += resolveEvents<Full>(ctx, v, sel, g);
+    for (int offset = 32; offset > 0; offset >>= 1) traced += __shfl_xor(traced, offset, 64);
+    if (__lane_id() == 0 && traced) atomicAdd(&ctx.counters->shadowQueries, traced);
 }
 
 // One shadow request: the contribution joins the continuing path's
@@ -561,6 +684,9 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
     // deferred shadow ray would reorder the sum.
     const bool inlineShadows = shadesWithInlineShadows(ctx.scene) || ctx.shade.photons != nullptr;
     const bool full = ctx.scene.fullFeatures != 0;
+    // Such scenes shade without tracing when the renderer holds an event
+    // buffer (the usual case), and trace inline otherwise.
+    const bool eventShading = inlineShadows && full && ctx.events.words != nullptr;
     hipError_t e = timer(0, [&] { hipLaunchKernelGGL(kPrimary, grid, block, 0, stream, ctx); });
     for (uint32_t bounce = 0; e == hipSuccess && bounce < ctx.shade.maxDepth; ++bounce) {
         uint32_t sel = bounce & 1u;
@@ -574,13 +700,17 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
             });
         if (e == hipSuccess)
             e = timer(3, [&] {
-                if (inlineShadows)
+                if (eventShading)
+                    hipLaunchKernelGGL(kShadeEvents<true>, grid, block, 0, stream, ctx, sel);
+                else if (inlineShadows)
                     hipLaunchKernelGGL((kShade<true, true>), grid, block, lds, stream, ctx, sel);
                 else if (full)
                     hipLaunchKernelGGL((kShade<false, true>), grid, block, 0, stream, ctx, sel);
                 else
                     hipLaunchKernelGGL((kShade<false, false>), grid, block, 0, stream, ctx, sel);
             });
+        if (e == hipSuccess && eventShading)
+            e = timer(5, [&] { hipLaunchKernelGGL(kShadowEvents<true>, grid, block, lds, stream, ctx, sel); });
         if (e == hipSuccess && !inlineShadows && ctx.scene.lightCount != 0 && !ctx.scene.ignoreDirectLightOcclusion)
             e = timer(5, [&] {
                 if (full)
