@@ -375,7 +375,7 @@ def tasks_one_by_one(hosts, scenes, configs, tasks, mode, outs):
 def measure_batch(n_tasks, photons):
     hosts, scenes, configs, tasks = batch_workload(n_tasks, photons)
     samples = sum(t.tile_width * t.tile_height * t.sample_count for t in tasks)
-    batch = wpt.Batch(scenes)
+    batch = wpt.Batch(scenes)  # staged passes
     stored, photon_ms = batch.set_caustics(2, configs[2])
     # Autotune on the first 32 tasks at up to 4 spp, then render everything.
     probe = [wpt.batch_task(t.scene, t.camera, t.width, t.height, t.x, t.y, t.tile_width, t.tile_height, 0,
@@ -385,15 +385,30 @@ def measure_batch(n_tasks, photons):
     chosen = batch.autotune(probe)
     tune_s = time.perf_counter() - ta
     log(f"batch: terrain photon map {stored} photons stored in {photon_ms:.1f} ms; autotuned in {tune_s:.1f} s "
-        f"-> block {chosen.block_size}, grid {chosen.blocks}")
+        f"-> scheduler {chosen.scheduler}, block {chosen.block_size}, grid {chosen.blocks}")
     batch.render(tasks[:8])  # warm-up
     tb = time.perf_counter()
     outs, lat = batch.render(tasks, latencies=True)
     batch_s = time.perf_counter() - tb
     bst = batch.stats()
     batch.close()
-    log(f"batch: {n_tasks} tasks, {samples / 1e6:.1f} M samples in {batch_s * 1000:.1f} ms "
+    scheduler = {wpt.BATCH_STAGED: "staged", wpt.BATCH_REGENERATE: "regenerate"}[chosen.scheduler]
+    log(f"batch ({scheduler}): {n_tasks} tasks, {samples / 1e6:.1f} M samples in {batch_s * 1000:.1f} ms "
         f"(device {bst.milliseconds:.1f} ms)")
+    # The same batch with the other scheduler: one persistent launch over
+    # every task (regenerating kernel), autotuned on the same probe.
+    other_scheduler = wpt.BATCH_REGENERATE if chosen.scheduler == wpt.BATCH_STAGED else wpt.BATCH_STAGED
+    other = wpt.Batch(scenes, scheduler=other_scheduler)
+    other.set_caustics(2, configs[2])
+    other.autotune(probe)
+    other.render(tasks[:8])
+    to = time.perf_counter()
+    other_outs, other_lat = other.render(tasks, latencies=True)
+    other_s = time.perf_counter() - to
+    other.close()
+    other_same = all(np.array_equal(a, b) for a, b in zip(outs, other_outs))
+    log(f"batch ({'regenerate' if other_scheduler == wpt.BATCH_REGENERATE else 'staged'}): {other_s * 1000:.1f} ms, "
+        f"identical {other_same}")
     # Tasks one by one with the same kernel (regenerating scheduler), then
     # with the staged hipGraph wavefront.
     regen_s, regen_done, regen_same = tasks_one_by_one(hosts, scenes, configs, tasks, wpt.MODE_REGENERATE, outs)
@@ -412,6 +427,11 @@ def measure_batch(n_tasks, photons):
         "batched_msamples_per_s": round(samples / batch_s / 1e6, 2),
         "batched_ms": round(batch_s * 1000.0, 2),
         "batched_latency_ms_p50_p99": pct(lat),
+        "batched_scheduler": scheduler,
+        "batched_other_scheduler": {
+            "scheduler": "regenerate" if other_scheduler == wpt.BATCH_REGENERATE else "staged",
+            "ms": round(other_s * 1000.0, 2), "msamples_per_s": round(samples / other_s / 1e6, 2),
+            "latency_ms_p50_p99": pct(other_lat), "identical": other_same},
         "one_by_one_same_kernel": {
             "msamples_per_s": round(samples / regen_s / 1e6, 2), "ms": round(regen_s * 1000.0, 2),
             "latency_ms_p50_p99": pct(regen_done), "speedup_of_batch": round(regen_s / batch_s, 3),
@@ -421,7 +441,7 @@ def measure_batch(n_tasks, photons):
             "latency_ms_p50_p99": pct(graph_done), "speedup_of_batch": round(graph_s / batch_s, 3),
             "identical": graph_same},
         "terrain_photons_stored": int(stored),
-        "autotuned": {"block_size": int(chosen.block_size), "blocks": int(chosen.blocks),
+        "autotuned": {"block_size": int(chosen.block_size), "blocks": int(chosen.blocks), "scheduler": scheduler,
                       "tuning_s": round(tune_s, 2)},
     }
     log(f"batch vs plain: {result}")

@@ -526,7 +526,16 @@ typedef struct wpt_batch_task {
 typedef struct wpt_batch_config {
     uint32_t block_size;       /* multiple of 64, at most 512 */
     uint32_t blocks;           /* persistent grid; 0 = every resident workgroup */
+    /* WPT_BATCH_STAGED (0): the tasks of each scene (and camera, frame and
+     * path parameters) run as staged wavefront passes of up to 2^26 paths
+     * over many tasks at once; WPT_BATCH_REGENERATE: one persistent launch
+     * over every task (per-task latency from the kernel's clock).  Outputs
+     * are identical. */
+    uint32_t scheduler;
 } wpt_batch_config;
+
+#define WPT_BATCH_STAGED 0u
+#define WPT_BATCH_REGENERATE 1u
 
 typedef struct wpt_batch_stats {
     uint64_t closest_queries;
@@ -551,18 +560,20 @@ WPT_API int wpt_batch_destroy(wpt_batch* batch);
  * grid and the build's device time. */
 WPT_API int wpt_batch_set_caustics(wpt_batch* batch, uint32_t scene, const wpt_render_config* config,
                                    uint32_t* photons_stored, double* build_ms);
-/* Renders every task in one launch.  outs[i] (host memory) receives task i's
- * unnormalized tile sums, tile_width*tile_height*3 floats row-major, bit
- * identical to wpt_render_task on the same task.  task_latency_ms, when not
- * NULL, receives per task the time from the launch's start to the end of
- * its last pixel. */
+/* Renders every task (staged passes, or one persistent launch).  outs[i]
+ * (host memory) receives task i's unnormalized tile sums,
+ * tile_width*tile_height*3 floats row-major, bit identical to
+ * wpt_render_task on the same task.  task_latency_ms, when not NULL,
+ * receives per task the time from the render's start to the end of its last
+ * samples. */
 WPT_API int wpt_batch_render(wpt_batch* batch, const wpt_batch_task* tasks, uint32_t n, float* const* outs,
                              double* task_latency_ms);
 WPT_API int wpt_batch_last_stats(const wpt_batch* batch, wpt_batch_stats* stats);
-/* Autotuner: renders the tasks under each candidate workgroup size (64, 128,
- * 256, 512) and persistent grid size (1/4, 1/2, 3/4 and all of the resident
- * workgroups), keeps the fastest configuration for later renders and
- * reports it in *chosen. */
+/* Autotuner within the batch's scheduler: renders the tasks under each
+ * candidate configuration -- staged: pass workgroups of 128, 256 or 512
+ * lanes; persistent: workgroups of 64, 128, 256 or 512 lanes times a grid of
+ * 1/4, 1/2, 3/4 or all of the resident workgroups -- keeps the fastest for
+ * later renders and reports it in *chosen. */
 WPT_API int wpt_batch_autotune(wpt_batch* batch, const wpt_batch_task* tasks, uint32_t n, wpt_batch_config* chosen);
 
 /* ---- differentiable rendering: forward + path-replay adjoint -----------

@@ -1,5 +1,6 @@
 """GPU tests of the batch consumer (wpt_batch_*): uneven tasks of several
-resident scenes in one persistent launch.  Each task's sums must equal
+resident scenes, as staged passes over many tasks or in one persistent
+launch.  Each task's sums must equal
 wpt_render_task's on the same task bit for bit (the reference renders tasks
 one by one, src/renderer/render_task.cu:26-114), and small tasks the
 oracle's; exact-once coverage follows tests/distributed_task_tests.cpp:40-51."""
@@ -11,6 +12,8 @@ import wpt
 pytestmark = pytest.mark.gpu
 
 FRAME = (160, 96)
+SCHEDULERS = pytest.mark.parametrize("scheduler", [wpt.BATCH_STAGED, wpt.BATCH_REGENERATE],
+                                     ids=["staged", "regenerate"])
 
 
 def make_scenes():
@@ -48,10 +51,11 @@ def render_one_by_one(scenes, task):
     return out, queries
 
 
-def test_batch_equals_tasks_one_by_one(torch_cuda):
+@SCHEDULERS
+def test_batch_equals_tasks_one_by_one(torch_cuda, scheduler):
     hosts, scenes = make_scenes()
     tasks = random_tasks(hosts, 24, seed=7)
-    batch = wpt.Batch(scenes, block_size=256)
+    batch = wpt.Batch(scenes, block_size=256, scheduler=scheduler)
     outs, lat = batch.render(tasks, latencies=True)
     st = batch.stats()
     closest = shadow = 0
@@ -65,10 +69,11 @@ def test_batch_equals_tasks_one_by_one(torch_cuda):
     assert np.all(lat > 0) and lat.max() <= st.milliseconds * 1.05 + 0.05
 
 
-def test_batch_small_tasks_match_oracle(torch_cuda, oracle_mod):
+@SCHEDULERS
+def test_batch_small_tasks_match_oracle(torch_cuda, oracle_mod, scheduler):
     hosts, scenes = make_scenes()
     tasks = random_tasks(hosts, 6, seed=11, max_spp=3)
-    outs = wpt.Batch(scenes).render(tasks)
+    outs = wpt.Batch(scenes, scheduler=scheduler).render(tasks)
     for i, t in enumerate(tasks):
         cfg = wpt.render_config(t.width, t.height, spp=t.sample_start + t.sample_count, max_depth=t.max_depth,
                                 rr_start=t.rr_start_depth, rng=t.rng_strategy, clamp=t.indirect_clamp)
@@ -77,7 +82,8 @@ def test_batch_small_tasks_match_oracle(torch_cuda, oracle_mod):
         assert np.array_equal(outs[i], fb[t.y:t.y + t.tile_height, t.x:t.x + t.tile_width]), f"task {i}"
 
 
-def test_batch_covers_a_frame_exactly_once(torch_cuda):
+@SCHEDULERS
+def test_batch_covers_a_frame_exactly_once(torch_cuda, scheduler):
     # A frame cut into uneven tiles and sample ranges sums to the full-frame
     # render (same per-pixel addition order within each task's range).
     hosts, scenes = make_scenes()
@@ -87,7 +93,7 @@ def test_batch_covers_a_frame_exactly_once(torch_cuda):
     for ty, th in ((0, 40), (40, 56)):
         for tx, tw in ((0, 64), (64, 96)):
             tasks.append(wpt.batch_task(0, cam, w, h, tx, ty, tw, th, 0, 4))
-    outs = wpt.Batch(scenes[:1]).render(tasks)
+    outs = wpt.Batch(scenes[:1], scheduler=scheduler).render(tasks)
     frame = np.zeros((h, w, 3), np.float32)
     cover = np.zeros((h, w), int)
     for t, o in zip(tasks, outs):
@@ -99,23 +105,28 @@ def test_batch_covers_a_frame_exactly_once(torch_cuda):
     assert n == 4 and np.array_equal(frame, full)
 
 
-def test_batch_autotune_keeps_results(torch_cuda):
+@SCHEDULERS
+def test_batch_autotune_keeps_results(torch_cuda, scheduler):
     hosts, scenes = make_scenes()
     tasks = random_tasks(hosts, 8, seed=3, max_spp=2)
-    batch = wpt.Batch(scenes)
+    batch = wpt.Batch(scenes, scheduler=scheduler)
     before = batch.render(tasks)
     chosen = batch.autotune(tasks)
-    assert chosen.block_size in (64, 128, 256, 512) and chosen.blocks > 0
+    assert chosen.block_size in (64, 128, 256, 512) and chosen.scheduler == scheduler
     after = batch.render(tasks)
     st = batch.stats()
-    assert (st.block_size, st.blocks) == (chosen.block_size, chosen.blocks)
+    if chosen.scheduler == wpt.BATCH_REGENERATE:
+        assert chosen.blocks > 0 and (st.block_size, st.blocks) == (chosen.block_size, chosen.blocks)
+    else:
+        assert st.block_size == chosen.block_size and st.blocks > 0
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
 
 
-def test_batch_edge_cases(torch_cuda):
+@SCHEDULERS
+def test_batch_edge_cases(torch_cuda, scheduler):
     hosts, scenes = make_scenes()
-    batch = wpt.Batch(scenes)
+    batch = wpt.Batch(scenes, scheduler=scheduler)
     assert batch.render([]) == []
     w, h = FRAME
     cam = hosts[0].camera(w, h)

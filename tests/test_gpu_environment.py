@@ -132,7 +132,8 @@ def test_small_map_scenes_match_oracle(torch_cuda, oracle_mod, case):
         assert np.array_equal(img, want), (case, mode)
 
 
-def test_batch_with_map_scenes_equals_render_task(torch_cuda):
+@pytest.mark.parametrize("scheduler", [wpt.BATCH_STAGED, wpt.BATCH_REGENERATE], ids=["staged", "regenerate"])
+def test_batch_with_map_scenes_equals_render_task(torch_cuda, scheduler):
     srcs = [PresetWithMap("cornell-box", random_map(), 1.0, 30.0),
             PresetWithMap("rtweek", random_map(64, 32, 4), 0.5, 120.0, 8)]
     dev = [wpt.Scene(s.desc(), device=True, owner=s) for s in srcs]
@@ -142,7 +143,7 @@ def test_batch_with_map_scenes_equals_render_task(torch_cuda):
         for (x, y) in [(0, 0), (32, 16), (64, 48)]:
             tasks.append(wpt.batch_task(i, s.camera(w, h), w, h, x, y, 32, 16, 1, 3, max_depth=8, rr_start=3,
                                         clamp=10.0))
-    outs = wpt.Batch(dev).render(tasks)
+    outs = wpt.Batch(dev, scheduler=scheduler).render(tasks)
     for t, out in zip(tasks, outs):
         cfg = wpt.render_config(w, h, spp=4, max_depth=8, rr_start=3, tile=(32, 16), clamp=10.0)
         r = wpt.Renderer(dev[t.scene], srcs[t.scene].camera(w, h), cfg)

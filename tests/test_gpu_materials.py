@@ -44,14 +44,15 @@ def test_textured_room_debug_views(torch_cuda, oracle_mod, debug):
     assert np.array_equal(img, want)
 
 
-def test_textured_batch_equals_render_task(torch_cuda):
+@pytest.mark.parametrize("scheduler", [wpt.BATCH_STAGED, wpt.BATCH_REGENERATE], ids=["staged", "regenerate"])
+def test_textured_batch_equals_render_task(torch_cuda, scheduler):
     rooms = [scenes.textured_room(), scenes.textured_room(with_env=False, with_lights=False)]
     dev = [wpt.Scene(b.desc(), device=True, owner=b) for b in rooms]
     w, h = 96, 64
     cam = scenes.room_camera(w / h)
     tasks = [wpt.batch_task(i, cam, w, h, x, y, 32, 32, 1, 2, max_depth=8, rr_start=3, clamp=10.0)
              for i in range(2) for (x, y) in [(0, 0), (32, 32), (64, 0)]]
-    outs = wpt.Batch(dev).render(tasks)
+    outs = wpt.Batch(dev, scheduler=scheduler).render(tasks)
     for t, out in zip(tasks, outs):
         cfg = wpt.render_config(w, h, spp=3, max_depth=8, rr_start=3, tile=(32, 32), clamp=10.0)
         want = wpt.Renderer(dev[t.scene], cam, cfg).render_task(

@@ -94,7 +94,8 @@ def test_full_terrain_schedulers_agree(torch_cuda):
     assert np.array_equal(images[0], images[1])
 
 
-def test_mixed_batch_with_caustic_terrain(torch_cuda, oracle_mod, tmp_path):
+@pytest.mark.parametrize("scheduler", [wpt.BATCH_STAGED, wpt.BATCH_REGENERATE], ids=["staged", "regenerate"])
+def test_mixed_batch_with_caustic_terrain(torch_cuda, oracle_mod, tmp_path, scheduler):
     # A batch holding a small caustic terrain beside two other scenes: the
     # terrain's photon map is built for the batch from the same caustic
     # settings a renderer takes, and every task equals wpt_render_task on a
@@ -107,7 +108,7 @@ def test_mixed_batch_with_caustic_terrain(torch_cuda, oracle_mod, tmp_path):
     scenes = [wpt.Scene(h.desc(), device=True, owner=h) for h in hosts]
     w, h = 96, 64
     rc, _ = config(manifest, w, h, 8, 30000)
-    batch = wpt.Batch(scenes, block_size=256)
+    batch = wpt.Batch(scenes, block_size=256, scheduler=scheduler)
     stored, ms = batch.set_caustics(1, rc)
     assert stored > 0
     rng = np.random.default_rng(11)

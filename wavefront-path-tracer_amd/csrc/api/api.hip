@@ -716,6 +716,16 @@ constexpr uint32_t kStagedGridFactor = WPT_STAGED_GRID_FACTOR;
 // Most radiance events per vertex that event shading stores (scenes with
 // more directional lights trace their shadow rays inline).
 constexpr uint32_t kMaxVertexEvents = 12;
+
+// Radiance events one vertex of `s` can store: the area-light sample, every
+// directional light, one positional light, the environment sample, then two
+// escape tests (dielectric, glTF delta lobes) or the caustic gather.
+uint32_t vertexEventsOf(const DevScene& s) {
+    return (s.lightCount ? 1u : 0u) + s.directionalCount + (s.punctualCount > s.directionalCount ? 1u : 0u) +
+        (s.environment.texels ? 1u : 0u) + 2u;
+}
+
+uint32_t eventWordsPerColumn(uint32_t maxEvents) { return kEventHeaderFields + maxEvents * kEventItemFields; }
 constexpr uint32_t kNoAllocScratchWords = 1u + kNoAllocMaxItems / 256u;
 static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
 
@@ -779,8 +789,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     // two escape tests (dielectric, glTF delta lobes) or the caustic gather.
     const DevScene& sceneView = scene->dev;
     bool inlineShadows = shadesWithInlineShadows(sceneView) || (c.caustic_photons && c.caustic_gather);
-    uint32_t vertexEvents = (sceneView.lightCount ? 1u : 0u) + sceneView.directionalCount +
-        (sceneView.punctualCount > sceneView.directionalCount ? 1u : 0u) + (sceneView.environment.texels ? 1u : 0u) + 2u;
+    uint32_t vertexEvents = vertexEventsOf(sceneView);
     bool eventShading = c.mode == WPT_MODE_GRAPH && inlineShadows && vertexEvents <= kMaxVertexEvents;
     // Samples per pass of the staged and cooperative schedulers: by default
     // as many as keep a pass within kDefaultPassPaths paths (64 at 1080p;
@@ -907,12 +916,20 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
             ctx.shadow.segmentCapacity = static_cast<uint32_t>(segment);
         }
         if (eventShading) {
-            size_t words = static_cast<size_t>(kEventHeaderFields + vertexEvents * kEventItemFields) * qcap;
+            size_t words = static_cast<size_t>(eventWordsPerColumn(vertexEvents)) * qcap;
             R_HIP(r->arena.alloc(sizeof(uint32_t) * words, &p));
             ctx.events.words = static_cast<uint32_t*>(p);
             ctx.events.capacity = static_cast<uint32_t>(qcap);
             ctx.events.maxEvents = vertexEvents;
         }
+    }
+    if (c.mode == WPT_MODE_PERSISTENT && ctx.scene.fullFeatures && vertexEvents <= kMaxVertexEvents) {
+        // One event column per thread of the cooperative grid.
+        uint32_t threads = ctx.persistentBlocks * c.block_size;
+        R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(eventWordsPerColumn(vertexEvents)) * threads, &p));
+        ctx.events.words = static_cast<uint32_t*>(p);
+        ctx.events.capacity = threads;
+        ctx.events.maxEvents = vertexEvents;
     }
     R_HIP(r->arena.alloc(sizeof(float) * 3 * static_cast<size_t>(c.width) * c.height, &p));
     ctx.framebuffer = static_cast<float*>(p);
@@ -946,6 +963,14 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
         R_HIP(hipMemset(p, 0, sizeof(BatchCounters)));
         b.out = ctx.framebuffer;
         b.full = ctx.scene.fullFeatures;
+        if (b.full && vertexEvents <= kMaxVertexEvents) {  // one event column per thread of the launch
+            uint32_t threads = r->regenBlocks * c.block_size;
+            R_HIP(r->arena.alloc(sizeof(uint32_t) * static_cast<size_t>(eventWordsPerColumn(vertexEvents)) * threads,
+                &p));
+            b.events.words = static_cast<uint32_t*>(p);
+            b.events.capacity = threads;
+            b.events.maxEvents = vertexEvents;
+        }
     }
     // One spill column per thread index of any grid.
     ctx.spillThreads = std::max(std::max(ctx.gridBlocks, ctx.persistentBlocks), r->regenBlocks) * c.block_size;
@@ -1214,6 +1239,8 @@ struct wpt_batch {
     size_t outCapacity = 0;  // floats
     uint2* spill = nullptr;
     uint32_t spillThreads = 0;
+    uint32_t* events = nullptr;  // full-feature kernel: one event column per thread
+    uint32_t eventThreads = 0, eventMax = 0;
     bool full = false;  // some scene needs the full-feature kernel
     int cus = 0;
     int wallClockKhz = 0;
@@ -1222,10 +1249,18 @@ struct wpt_batch {
     // (null: none) and the buffers behind it.
     std::vector<const PhotonGridView*> photons;
     std::vector<std::unique_ptr<DeviceArena>> photonArena;
+    // Staged scheduler: wavefront buffers for passes of `stagedCapacity`
+    // paths, shared by every scene's passes (allocated on first use).
+    std::unique_ptr<DeviceArena> stagedArena;
+    LaunchContext staged{};
+    uint32_t stagedCapacity = 0, stagedBlock = 0, stagedEvents = 0;
+    PassPiece* pieces = nullptr;
+    size_t pieceCapacity = 0;
     ~wpt_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : {static_cast<void*>(scenesDev), static_cast<void*>(counters), static_cast<void*>(tasks),
-                 static_cast<void*>(state), static_cast<void*>(out), static_cast<void*>(spill)})
+                 static_cast<void*>(state), static_cast<void*>(out), static_cast<void*>(spill),
+                 static_cast<void*>(events), static_cast<void*>(pieces)})
             if (p) (void)hipFree(p);
         if (start) (void)hipEventDestroy(start);
         if (end) (void)hipEventDestroy(end);
@@ -1238,6 +1273,7 @@ namespace {
 int validateBatchConfig(const wpt_batch_config& c) {
     if (c.block_size == 0 || c.block_size > 512 || c.block_size % 64 != 0)
         return fail(WPT_ERROR_INVALID_ARGUMENT, "Block size must be a multiple of 64 between 64 and 512");
+    if (c.scheduler > WPT_BATCH_REGENERATE) return fail(WPT_ERROR_INVALID_ARGUMENT, "unknown batch scheduler");
     return WPT_OK;
 }
 
@@ -1336,12 +1372,31 @@ int ensureBatchSpill(wpt_batch* b, uint32_t threads) {
     return WPT_OK;
 }
 
+// Event columns for `threads` threads when a scene needs the full-feature
+// kernel (none when a scene would store more than kMaxVertexEvents).
+int ensureBatchEvents(wpt_batch* b, uint32_t threads) {
+    uint32_t maxEvents = 0;
+    for (const wpt_scene* s : b->scenes) maxEvents = std::max(maxEvents, vertexEventsOf(s->dev));
+    if (!b->full || maxEvents > kMaxVertexEvents) return WPT_OK;
+    if (threads <= b->eventThreads && maxEvents <= b->eventMax) return WPT_OK;
+    if (b->events) WPT_HIP(hipFree(b->events));
+    b->events = nullptr;
+    b->eventThreads = b->eventMax = 0;
+    WPT_HIP(hipMalloc(reinterpret_cast<void**>(&b->events),
+        sizeof(uint32_t) * static_cast<size_t>(eventWordsPerColumn(maxEvents)) * threads));
+    b->eventThreads = threads;
+    b->eventMax = maxEvents;
+    return WPT_OK;
+}
+
 // One launch over the staged tasks under configuration `c`; device time in *ms.
 int runBatch(wpt_batch* b, uint32_t n, const wpt_batch_config& c, float* ms) {
     uint32_t blocks = 0;
     int s = batchGrid(b, c, &blocks);
     if (s != WPT_OK) return s;
     s = ensureBatchSpill(b, blocks * c.block_size);
+    if (s != WPT_OK) return s;
+    s = ensureBatchEvents(b, blocks * c.block_size);
     if (s != WPT_OK) return s;
     BatchContext ctx{};
     ctx.scenes = b->scenesDev;
@@ -1353,6 +1408,7 @@ int runBatch(wpt_batch* b, uint32_t n, const wpt_batch_config& c, float* ms) {
     ctx.spill = b->spill;
     ctx.spillThreads = b->spillThreads;
     ctx.full = b->full ? 1u : 0u;
+    if (b->full && b->events) ctx.events = EventBuffer{b->events, b->eventThreads, b->eventMax};
     WPT_HIP(hipEventRecord(b->start, b->stream));
     WPT_HIP(enqueueBatch(ctx, blocks, c.block_size, b->stream));
     WPT_HIP(hipEventRecord(b->end, b->stream));
@@ -1360,6 +1416,230 @@ int runBatch(wpt_batch* b, uint32_t n, const wpt_batch_config& c, float* ms) {
     WPT_HIP(hipEventElapsedTime(ms, b->start, b->end));
     b->last.block_size = c.block_size;
     b->last.blocks = blocks;
+    return WPT_OK;
+}
+
+
+// ---- staged batches
+// Paths per staged batch pass (16 samples of a 1920x1080 tile).
+constexpr uint32_t kStagedBatchPaths = 1u << 25;
+
+// Allocates the shared wavefront buffers of staged batch passes: queues, hit
+// records, deferred shadow requests, radiance events (when a scene needs
+// them), per-path radiance, counters and the traversal spill area.
+int ensureStagedBatch(wpt_batch* b, uint32_t blockSize) {
+    uint32_t maxEvents = 0;
+    bool anyEvents = false, full = false;
+    for (size_t i = 0; i < b->scenes.size(); ++i) {
+        const DevScene& d = b->scenes[i]->dev;
+        full = full || d.fullFeatures || b->photons[i];
+        if (shadesWithInlineShadows(d) || b->photons[i]) {
+            anyEvents = true;
+            maxEvents = std::max(maxEvents, vertexEventsOf(d));
+        }
+    }
+    if (maxEvents > kMaxVertexEvents) maxEvents = 0;  // such scenes trace inline
+    uint32_t wantEvents = anyEvents ? maxEvents : 0u;
+    if (b->stagedArena && b->stagedBlock == blockSize && b->stagedEvents >= wantEvents) return WPT_OK;
+    b->stagedArena.reset(new (std::nothrow) DeviceArena());
+    if (!b->stagedArena) return fail(WPT_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    b->staged = LaunchContext{};
+    LaunchContext& ctx = b->staged;
+    DeviceArena& arena = *b->stagedArena;
+    const uint32_t cap = kStagedBatchPaths, B = blockSize;
+    Occupancy lean{}, rich{};
+    WPT_HIP(kernelOccupancy(B, false, &lean));
+    if (full) WPT_HIP(kernelOccupancy(B, true, &rich));
+    int trace = std::max(lean.trace, full ? rich.trace : 0);
+    if (trace <= 0 || lean.shade <= 0 || (full && rich.shade <= 0))
+        return fail(WPT_ERROR_UNSUPPORTED, "block size cannot launch the wavefront kernels");
+    ctx.blockSize = B;
+    ctx.capacity = cap;
+    ctx.gridBlocks = std::max(1u, std::min((cap + B - 1) / B, kStagedGridFactor * static_cast<uint32_t>(trace) *
+        static_cast<uint32_t>(b->cus)));
+    uint64_t rounds = (static_cast<uint64_t>(cap) + static_cast<uint64_t>(ctx.gridBlocks) * B - 1) /
+        (static_cast<uint64_t>(ctx.gridBlocks) * B);
+    uint64_t virtualBlocks = (static_cast<uint64_t>(cap) + B - 1) / B;
+    uint64_t segment = std::max(static_cast<uint64_t>((ctx.gridBlocks + kSegments - 1) / kSegments) * rounds,
+                                (virtualBlocks + kSegments - 1) / kSegments) * B;
+    uint64_t qcap = segment * kSegments;
+    if (qcap >= (1ull << 31)) return fail(WPT_ERROR_INVALID_ARGUMENT, "queue capacity out of range");
+    void* p = nullptr;
+    for (int q = 0; q < 2; ++q) {
+        WPT_HIP(arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kQueueFieldCount) * qcap, &p));
+        ctx.queue[q].words = static_cast<uint32_t*>(p);
+        ctx.queue[q].capacity = static_cast<uint32_t>(qcap);
+        ctx.queue[q].segmentCapacity = static_cast<uint32_t>(segment);
+    }
+    WPT_HIP(arena.alloc(sizeof(float) * 3 * static_cast<size_t>(qcap) + sizeof(uint32_t) * 4 * static_cast<size_t>(qcap), &p));
+    float* f = static_cast<float*>(p);
+    ctx.hits.t = f;
+    ctx.hits.u = f + qcap;
+    ctx.hits.v = f + 2 * static_cast<size_t>(qcap);
+    uint32_t* u = reinterpret_cast<uint32_t*>(f + 3 * static_cast<size_t>(qcap));
+    ctx.hits.kind = u;
+    ctx.hits.instance = u + qcap;
+    ctx.hits.prim = u + 2 * static_cast<size_t>(qcap);
+    ctx.hits.material = u + 3 * static_cast<size_t>(qcap);
+    WPT_HIP(arena.alloc(sizeof(uint32_t) * static_cast<size_t>(kShadowFieldCount) * qcap, &p));
+    ctx.shadow.words = static_cast<uint32_t*>(p);
+    ctx.shadow.capacity = static_cast<uint32_t>(qcap);
+    ctx.shadow.segmentCapacity = static_cast<uint32_t>(segment);
+    if (wantEvents) {
+        WPT_HIP(arena.alloc(sizeof(uint32_t) * static_cast<size_t>(eventWordsPerColumn(wantEvents)) * qcap, &p));
+        ctx.events.words = static_cast<uint32_t*>(p);
+        ctx.events.capacity = static_cast<uint32_t>(qcap);
+        ctx.events.maxEvents = wantEvents;
+    }
+    WPT_HIP(arena.alloc(sizeof(float) * 3 * static_cast<size_t>(cap), &p));
+    ctx.slotRadiance = static_cast<float*>(p);
+    WPT_HIP(arena.alloc(sizeof(WavefrontCounters), &p));
+    ctx.counters = static_cast<WavefrontCounters*>(p);
+    WPT_HIP(hipMemset(p, 0, sizeof(WavefrontCounters)));
+    ctx.spillThreads = ctx.gridBlocks * B;
+    WPT_HIP(arena.alloc(sizeof(uint2) * kSpillEntriesTotal * static_cast<size_t>(ctx.spillThreads), &p));
+    ctx.spill = static_cast<uint2*>(p);
+    ctx.profileQueues = 0;
+    ctx.slots = 1;
+    b->stagedCapacity = cap;
+    b->stagedBlock = B;
+    b->stagedEvents = wantEvents;
+    return WPT_OK;
+}
+
+// Tasks that one staged pass can share: same scene, camera, frame and path
+// parameters.
+bool sameStagedGroup(const wpt_batch_task& a, const wpt_batch_task& c) {
+    return a.scene == c.scene && std::memcmp(&a.camera, &c.camera, sizeof a.camera) == 0 && a.width == c.width &&
+        a.height == c.height && a.max_depth == c.max_depth && a.rr_start_depth == c.rr_start_depth &&
+        a.rng_strategy == c.rng_strategy && bitsOf(a.indirect_clamp) == bitsOf(c.indirect_clamp);
+}
+
+// Every task through staged passes: the tasks of one group fill passes of up
+// to stagedCapacity paths (a task's samples split across passes in order),
+// each pass ends by adding its pieces' samples to the tasks' sums.  *ms is
+// the device time; latencyMs (optional) the end of each task's last pass.
+int runStagedBatch(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, const std::vector<size_t>& offsets,
+    float* ms, double* latencyMs) {
+    int s = ensureStagedBatch(b, b->config.block_size);
+    if (s != WPT_OK) return s;
+    const uint32_t cap = b->stagedCapacity;
+    for (uint32_t i = 0; i < n; ++i)
+        if (tasks[i].sample_count && static_cast<uint64_t>(tasks[i].tile_width) * tasks[i].tile_height > cap)
+            return fail(WPT_ERROR_UNSUPPORTED, "Batch task " + std::to_string(i) +
+                ": tile larger than a staged pass; use the regenerating batch scheduler");
+    // Groups in order of their first task; passes in group order.
+    std::vector<int> group(n, -1);
+    std::vector<uint32_t> leaders;
+    for (uint32_t i = 0; i < n; ++i) {
+        for (size_t g = 0; g < leaders.size() && group[i] < 0; ++g)
+            if (sameStagedGroup(tasks[leaders[g]], tasks[i])) group[i] = static_cast<int>(g);
+        if (group[i] < 0) {
+            group[i] = static_cast<int>(leaders.size());
+            leaders.push_back(i);
+        }
+    }
+    struct Pass {
+        uint32_t leader;
+        std::vector<PassPiece> pieces;
+        std::vector<uint32_t> finishing;  // tasks whose last samples are in this pass
+        uint32_t paths = 0, pixels = 0;
+    };
+    std::vector<Pass> passes;
+    for (size_t g = 0; g < leaders.size(); ++g) {
+        Pass cur{leaders[g], {}, {}, 0, 0};
+        for (uint32_t i = 0; i < n; ++i) {
+            if (group[i] != static_cast<int>(g)) continue;
+            const wpt_batch_task& t = tasks[i];
+            uint32_t pixels = t.tile_width * t.tile_height, done = 0;
+            while (done < t.sample_count) {
+                uint32_t fit = (cap - cur.paths) / pixels;
+                if (fit == 0) {
+                    passes.push_back(std::move(cur));
+                    cur = Pass{leaders[g], {}, {}, 0, 0};
+                    continue;
+                }
+                uint32_t k = std::min(fit, t.sample_count - done);
+                PassPiece piece{};
+                piece.tile = RenderTile{t.x, t.y, t.tile_width, t.tile_height, 0u, 1u};
+                piece.sampleStart = t.sample_start + done;
+                piece.samples = k;
+                piece.pathBase = cur.paths;
+                piece.pixelBase = cur.pixels;
+                piece.outStride = t.tile_width;
+                piece.out = offsets[i];
+                cur.pieces.push_back(piece);
+                cur.paths += k * pixels;
+                cur.pixels += pixels;
+                done += k;
+                if (done == t.sample_count) cur.finishing.push_back(i);
+            }
+        }
+        if (!cur.pieces.empty()) passes.push_back(std::move(cur));
+    }
+    size_t most = 1;
+    for (const Pass& pass : passes) most = std::max(most, pass.pieces.size());
+    s = growDevice(&b->pieces, most, &b->pieceCapacity);
+    if (s != WPT_OK) return s;
+    std::vector<hipEvent_t> ends(passes.size(), nullptr);
+    auto release = [&] {
+        for (hipEvent_t e : ends)
+            if (e) (void)hipEventDestroy(e);
+    };
+    WPT_HIP(hipMemsetAsync(b->staged.counters, 0, sizeof(WavefrontCounters), b->stream));
+    WPT_HIP(hipEventRecord(b->start, b->stream));
+    for (size_t k = 0; k < passes.size(); ++k) {
+        const Pass& pass = passes[k];
+        const wpt_batch_task& t = tasks[pass.leader];
+        hipError_t e = hipMemcpyAsync(b->pieces, pass.pieces.data(), sizeof(PassPiece) * pass.pieces.size(),
+            hipMemcpyHostToDevice, b->stream);
+        LaunchContext ctx = b->staged;
+        ctx.scene = b->scenes[t.scene]->dev;
+        const PhotonGridView* photons = b->photons[t.scene];
+        if (photons) ctx.scene.fullFeatures = 1;  // the gather runs in the full-feature kernels
+        if (!(shadesWithInlineShadows(ctx.scene) || photons) || vertexEventsOf(ctx.scene) > kMaxVertexEvents)
+            ctx.events = EventBuffer{};
+        ctx.camera.position = Vec3(t.camera.position[0], t.camera.position[1], t.camera.position[2]);
+        ctx.camera.forward = Vec3(t.camera.forward[0], t.camera.forward[1], t.camera.forward[2]);
+        ctx.camera.right = Vec3(t.camera.right[0], t.camera.right[1], t.camera.right[2]);
+        ctx.camera.up = Vec3(t.camera.up[0], t.camera.up[1], t.camera.up[2]);
+        ctx.camera.verticalFov = t.camera.vertical_fov;
+        ctx.camera.aspectRatio = t.camera.aspect_ratio;
+        ctx.shade = ShadeParams{t.max_depth, t.rr_start_depth, t.rng_strategy, t.indirect_clamp, 0u, 0u, photons};
+        ctx.width = t.width;
+        ctx.height = t.height;
+        ctx.pieces = b->pieces;
+        ctx.pieceCount = static_cast<uint32_t>(pass.pieces.size());
+        ctx.passPaths = pass.paths;
+        ctx.passPixels = pass.pixels;
+        ctx.batchOut = b->out;
+        if (e == hipSuccess) e = enqueueStagedBatchPass(ctx, b->stream);
+        if (e == hipSuccess) e = hipEventCreate(&ends[k]);
+        if (e == hipSuccess) e = hipEventRecord(ends[k], b->stream);
+        if (e != hipSuccess) {
+            release();
+            return hipFail(e, "staged batch pass");
+        }
+    }
+    hipError_t e = hipEventRecord(b->end, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, b->start, b->end);
+    for (size_t k = 0; e == hipSuccess && latencyMs && k < passes.size(); ++k) {
+        float at = 0.0f;
+        e = hipEventElapsedTime(&at, b->start, ends[k]);
+        for (uint32_t i : passes[k].finishing) latencyMs[i] = at;
+    }
+    release();
+    if (e != hipSuccess) return hipFail(e, "staged batch");
+    if (latencyMs)
+        for (uint32_t i = 0; i < n; ++i)
+            if (tasks[i].sample_count == 0) latencyMs[i] = 0.0;
+    WavefrontCounters counters{};
+    WPT_HIP(hipMemcpy(&counters, b->staged.counters, sizeof counters, hipMemcpyDeviceToHost));
+    b->last.closest_queries = counters.closestQueries;
+    b->last.shadow_queries = counters.shadowQueries;
+    b->last.block_size = b->config.block_size;
+    b->last.blocks = b->staged.gridBlocks;
     return WPT_OK;
 }
 
@@ -1466,12 +1746,20 @@ WPT_API int wpt_batch_render(wpt_batch* b, const wpt_batch_task* tasks, uint32_t
     int s = stageBatch(b, tasks, n, offsets, &samples);
     if (s != WPT_OK) return s;
     float ms = 0.0f;
-    s = runBatch(b, n, b->config, &ms);
-    if (s != WPT_OK) return s;
+    const bool staged = b->config.scheduler == WPT_BATCH_STAGED;
+    if (staged) {
+        s = runStagedBatch(b, tasks, n, offsets, &ms, latencyMs);
+        if (s != WPT_OK) return s;
+    } else {
+        s = runBatch(b, n, b->config, &ms);
+        if (s != WPT_OK) return s;
+    }
     BatchCounters counters{};
-    WPT_HIP(hipMemcpy(&counters, b->counters, sizeof counters, hipMemcpyDeviceToHost));
-    b->last.closest_queries = counters.closestQueries;
-    b->last.shadow_queries = counters.shadowQueries;
+    if (!staged) {
+        WPT_HIP(hipMemcpy(&counters, b->counters, sizeof counters, hipMemcpyDeviceToHost));
+        b->last.closest_queries = counters.closestQueries;
+        b->last.shadow_queries = counters.shadowQueries;
+    }
     b->last.samples = samples;
     b->last.milliseconds = ms;
     size_t total = 0;
@@ -1480,7 +1768,7 @@ WPT_API int wpt_batch_render(wpt_batch* b, const wpt_batch_task* tasks, uint32_t
     if (total) WPT_HIP(hipMemcpy(host.data(), b->out, sizeof(float) * total, hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < n; ++i)
         std::memcpy(outs[i], host.data() + offsets[i], sizeof(float) * 3ull * tasks[i].tile_width * tasks[i].tile_height);
-    if (latencyMs && n) {
+    if (latencyMs && n && !staged) {
         std::vector<BatchTaskState> state(n);
         WPT_HIP(hipMemcpy(state.data(), b->state, sizeof(BatchTaskState) * n, hipMemcpyDeviceToHost));
         double khz = b->wallClockKhz > 0 ? static_cast<double>(b->wallClockKhz) : 1.0;
@@ -1506,19 +1794,37 @@ WPT_API int wpt_batch_autotune(wpt_batch* b, const wpt_batch_task* tasks, uint32
     if (s != WPT_OK) return s;
     wpt_batch_config best = b->config;
     float bestMs = std::numeric_limits<float>::infinity();
-    for (uint32_t block : {64u, 128u, 256u, 512u}) {
-        int perCu = 0;
-        WPT_HIP(batchOccupancy(block, b->full, &perCu));
-        if (perCu <= 0) continue;
-        uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(b->cus);
-        for (uint32_t quarter = 1; quarter <= 4; ++quarter) {
-            wpt_batch_config c{block, std::max(1u, resident * quarter / 4u)};
+    if (b->config.scheduler == WPT_BATCH_STAGED) {
+        // Staged passes: the workgroup size of the pass kernels.
+        for (uint32_t block : {128u, 256u, 512u}) {
+            wpt_batch_config c{block, 0u, WPT_BATCH_STAGED};
+            wpt_batch_config saved = b->config;
+            b->config = c;
             float ms = 0.0f;
-            s = runBatch(b, n, c, &ms);
+            s = runStagedBatch(b, tasks, n, offsets, &ms, nullptr);
+            b->config = saved;
             if (s != WPT_OK) return s;
             if (ms < bestMs) {
                 bestMs = ms;
                 best = c;
+            }
+        }
+    } else {
+        // One persistent launch: workgroup size and grid.
+        for (uint32_t block : {64u, 128u, 256u, 512u}) {
+            int perCu = 0;
+            WPT_HIP(batchOccupancy(block, b->full, &perCu));
+            if (perCu <= 0) continue;
+            uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(b->cus);
+            for (uint32_t quarter = 1; quarter <= 4; ++quarter) {
+                wpt_batch_config c{block, std::max(1u, resident * quarter / 4u), WPT_BATCH_REGENERATE};
+                float ms = 0.0f;
+                s = runBatch(b, n, c, &ms);
+                if (s != WPT_OK) return s;
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    best = c;
+                }
             }
         }
     }

@@ -47,21 +47,26 @@ __global__ void __launch_bounds__(512) kBatchReset(BatchContext ctx) {
     }
 }
 
-// Scenes without environment maps, punctual lights and texture bindings
-// (the kernel without `Full`) test occlusion only for the area-light sample,
-// and that sample's contribution is the last radiance a bounce adds.  Their
-// shading therefore defers the shadow ray: the lane traces it in the next
-// loop iteration, through the same traversal call as the closest-hit
-// queries of the other lanes, and adds the contribution if nothing blocks
-// it -- the same additions in the same order as testing it inline.  Full
-// scenes, whose shading adds radiance after several occlusion tests, trace
-// those inline.
-template <bool Full>
+// Every lane runs one traversal per loop iteration, through one call site:
+// the closest hit of its path, or a shadow ray its last shading step left.
+// - Scenes without environment maps, punctual lights and texture bindings
+//   (the kernel without `Full`) test occlusion only for the area-light
+//   sample, and that sample's contribution is the last radiance a bounce
+//   adds: the shading step returns it as one deferred request.
+// - Full scenes with `Events` shade through ordered radiance events
+//   (EventOcclusion) stored in the lane's column of the event buffer; the
+//   lane resolves them in order in the following iterations, one traced ray
+//   each, adding the visible contributions to the path's radiance and ending
+//   the path when an escaping event says so.
+// Both add the same contributions in the same order as tracing inline, which
+// Full scenes without an event buffer do.
+template <bool Full, bool Events>
 __global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY(Full) kBatch(BatchContext ctx) {
     constexpr bool kDefer = !Full;
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     const uint32_t n = ctx.taskCount;
-    const uint32_t waveId = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t waveId = lane >> 6;
     const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     if (__lane_id() == 0) atomicMin(&ctx.counters->startClock, static_cast<unsigned long long>(wall_clock64()));
     // Waves start spread evenly over the task list.
@@ -75,7 +80,24 @@ __global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY(Full) kBatch(BatchCon
     ShadowRequest pending;   // deferred shadow ray of the last shading step
     pending.valid = 0;
     bool pathEnded = false;  // the path ended at its last shading step
+    uint32_t eventCount = 0, eventNext = 0;  // events of the last shading step, and the next to resolve
+    bool eventEnds = false;                  // an escaping event ended the path
     unsigned long long closest = 0, shadow = 0;
+
+    // Adds the untraced events from eventNext on, up to the next traced one;
+    // when none is left, returns whether the path has ended.
+    auto drainEvents = [&]() -> bool {
+        while (eventNext < eventCount) {
+            if (*eventItemWord(ctx.events, eventNext, kEiFlags, lane) & kEventTraced) return false;
+            path.radiance += Vec3(__uint_as_float(*eventItemWord(ctx.events, eventNext, kEiCx, lane)),
+                __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiCy, lane)),
+                __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiCz, lane)));
+            ++eventNext;
+        }
+        eventCount = 0;
+        eventNext = 0;
+        return pathEnded || eventEnds;
+    };
 
     while (true) {
         const BatchTask& T = ctx.tasks[task];
@@ -124,15 +146,48 @@ __global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY(Full) kBatch(BatchCon
         }
         if (busy) {
             const DevScene& scene = ctx.scenes[T.scene];
-            const bool shadowQuery = kDefer && pending.valid;
-            HitRecord h = traceQuery<Full, kDefer ? Query::kEither : Query::kClosest>(scene,
-                shadowQuery ? Ray(pending.origin, pending.direction) : path.ray, pending.tmax, shadowQuery, stack);
+            bool shadowQuery = false;
+            Ray queryRay = path.ray;
+            float queryTmax = 0.0f;
+            if constexpr (kDefer) {
+                if (pending.valid) {
+                    shadowQuery = true;
+                    queryRay = Ray(pending.origin, pending.direction);
+                    queryTmax = pending.tmax;
+                }
+            } else if constexpr (Events) {
+                if (eventNext < eventCount) {  // a traced event (untraced ones are drained already)
+                    shadowQuery = true;
+                    queryRay = Ray(Vec3(__uint_as_float(*eventItemWord(ctx.events, eventNext, kEiOx, lane)),
+                                       __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiOy, lane)),
+                                       __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiOz, lane))),
+                        Vec3(__uint_as_float(*eventItemWord(ctx.events, eventNext, kEiDx, lane)),
+                            __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiDy, lane)),
+                            __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiDz, lane))));
+                    queryTmax = __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiTmax, lane));
+                }
+            }
+            HitRecord h = traceQuery<Full, (kDefer || Events) ? Query::kEither : Query::kClosest>(scene, queryRay,
+                queryTmax, shadowQuery, stack);
             bool ended;
             if (shadowQuery) {
                 ++shadow;
-                if (h.kind == kHitNone) path.radiance += pending.contribution;
-                pending.valid = 0;
-                ended = pathEnded;
+                if constexpr (kDefer) {
+                    if (h.kind == kHitNone) path.radiance += pending.contribution;
+                    pending.valid = 0;
+                    ended = pathEnded;
+                } else {
+                    if (h.kind == kHitNone) {
+                        path.radiance += Vec3(__uint_as_float(*eventItemWord(ctx.events, eventNext, kEiCx, lane)),
+                            __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiCy, lane)),
+                            __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiCz, lane)));
+                        if (*eventItemWord(ctx.events, eventNext, kEiFlags, lane) & kEventTerminates) eventEnds = true;
+                    }
+                    ++eventNext;
+                    ended = drainEvents();
+                    // Events still pending: the path waits for the next iteration.
+                    if (eventCount != 0) ended = false;
+                }
             } else {
                 ++closest;
                 ShadeOutcome outcome;
@@ -140,6 +195,12 @@ __global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY(Full) kBatch(BatchCon
                     DeferOcclusion occ;
                     shadePath<Full>(scene, path, h, T.shade, occ, outcome);
                     pending = outcome.shadow;
+                } else if constexpr (Events) {
+                    EventOcclusion occ{ctx.events, lane, 0u};
+                    shadePath<Full>(scene, path, h, T.shade, occ, outcome);
+                    eventCount = occ.count < ctx.events.maxEvents ? occ.count : ctx.events.maxEvents;
+                    eventNext = 0;
+                    eventEnds = false;
                 } else {
                     InlineOcclusion<LdsStack> occ{stack, 0u};
                     shadePath<Full>(scene, path, h, T.shade, occ, outcome);
@@ -147,7 +208,12 @@ __global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY(Full) kBatch(BatchCon
                 }
                 if (outcome.isDebug) path.radiance = outcome.debugColor;
                 pathEnded = !outcome.alive;
-                ended = pathEnded && !(kDefer && pending.valid);
+                if constexpr (Events) {
+                    ended = drainEvents();
+                    if (eventCount != 0) ended = false;
+                } else {
+                    ended = pathEnded && !(kDefer && pending.valid);
+                }
             }
             if (ended) {
                 sum += path.radiance;
@@ -182,23 +248,26 @@ hipError_t enqueueBatch(const BatchContext& ctx, uint32_t blocks, uint32_t block
     hipLaunchKernelGGL(kBatchReset, dim3(resetBlocks > 0 ? resetBlocks : 1u), dim3(256), 0, stream, ctx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ctx.taskCount == 0) return e;
-    if (ctx.full)
-        hipLaunchKernelGGL(kBatch<true>, dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream, ctx);
+    if (ctx.full && ctx.events.words)
+        hipLaunchKernelGGL((kBatch<true, true>), dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream, ctx);
+    else if (ctx.full)
+        hipLaunchKernelGGL((kBatch<true, false>), dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream, ctx);
     else
-        hipLaunchKernelGGL(kBatch<false>, dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream, ctx);
+        hipLaunchKernelGGL((kBatch<false, false>), dim3(blocks), dim3(blockSize), stackLdsBytes(blockSize), stream,
+            ctx);
     return hipGetLastError();
 }
 
 hipError_t batchOccupancy(uint32_t blockSize, bool full, int* blocksPerCu) {
-    return full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch<true>, static_cast<int>(blockSize),
-                      stackLdsBytes(blockSize))
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch<false>,
+    return full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch<true, true>,
+                      static_cast<int>(blockSize), stackLdsBytes(blockSize))
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCu, kBatch<false, false>,
                       static_cast<int>(blockSize), stackLdsBytes(blockSize));
 }
 
 hipError_t batchAttributes(bool full, hipFuncAttributes* out) {
-    return full ? hipFuncGetAttributes(out, reinterpret_cast<const void*>(kBatch<true>))
-                : hipFuncGetAttributes(out, reinterpret_cast<const void*>(kBatch<false>));
+    return full ? hipFuncGetAttributes(out, reinterpret_cast<const void*>(kBatch<true, true>))
+                : hipFuncGetAttributes(out, reinterpret_cast<const void*>(kBatch<false, false>));
 }
 
 }  // namespace wpt

@@ -75,6 +75,21 @@ constexpr uint32_t kSlotShift = 26;
 constexpr uint32_t kPixelMask = (1u << kSlotShift) - 1u;
 constexpr uint32_t kMaxSlots = 64;  // slot < 64: six bits above the 26 pixel bits
 
+// Part of a batch task rendered by one staged batch pass: `samples`
+// consecutive samples from `sampleStart` of every pixel of the task's tile.
+// The pass's paths are enumerated piece by piece (pathBase), and a path's
+// radiance lands at its index in the pass; the resolve then adds each tile
+// pixel's samples, in order, to the task's output.
+struct PassPiece {
+    RenderTile tile;
+    uint32_t sampleStart, samples;
+    uint32_t pathBase;        // first path of the piece in the pass
+    uint32_t pixelBase;       // first tile pixel of the piece among the pass's pieces
+    uint32_t outStride;       // pixels per output row
+    uint32_t pad;
+    unsigned long long out;   // float offset of the tile's first pixel in the output
+};
+
 // Device counters shared by the launches of one renderer.
 struct WavefrontCounters {
     uint32_t count[2][kSegments];    // staged mode: items per segment of queue 0 / 1
@@ -107,6 +122,13 @@ struct LaunchContext {
     // radiance events, indexed by its slot in the queue being shaded
     // (words null when the scene does not need them).
     EventBuffer events;
+    // Staged batch pass (pieces non-null): the paths of `pieceCount` pieces
+    // of one scene's tasks (same camera, frame and path parameters), `passPaths`
+    // in all, `passPixels` tile pixels; radiance per path in slotRadiance,
+    // resolved into `batchOut`.
+    const PassPiece* pieces;
+    uint32_t pieceCount, passPaths, passPixels;
+    float* batchOut;
     float* framebuffer;      // width*height*3
     // Staged and persistent passes render `slots` samples of a tile at once:
     // a finished path stores its radiance in its sample's plane of
@@ -135,6 +157,9 @@ inline size_t stackLdsBytes(uint32_t blockSize) { return sizeof(uint2) * kLdsSta
 // `samples` samples) on `stream` (captured once into a hipGraph by the
 // caller).
 hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream);
+// One staged batch pass (ctx.pieces set): primary, maxDepth x (reset, trace,
+// shade, shadow), then the pieces' resolve into ctx.batchOut.
+hipError_t enqueueStagedBatchPass(const LaunchContext& ctx, hipStream_t stream);
 // Persistent cooperative kernel for one pass of the current tile, then
 // resolve and advance.
 hipError_t enqueuePersistentSample(const LaunchContext& ctx, hipStream_t stream);
@@ -192,6 +217,9 @@ struct BatchContext {
     uint2* spill;
     uint32_t spillThreads;
     uint32_t full;  // some scene has an environment map, punctual lights or texture bindings
+    // Full scenes: radiance events of each lane's last shading step, one
+    // column per thread index of the launch (words null: trace inline).
+    EventBuffer events;
 };
 
 // Caustic photon map (photon.hip, photon_trace.h): the renderer's buffers, and

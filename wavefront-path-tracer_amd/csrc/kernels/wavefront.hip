@@ -168,10 +168,41 @@ __device__ __forceinline__ void storePath(const PathQueue& q, uint32_t i, const 
     sf(q, kEtaScale, i, p.etaScale);
 }
 
-// Paths of a pass: tile.samples consecutive samples of every tile pixel.
+// Paths of a pass: tile.samples consecutive samples of every tile pixel (or
+// the batch pass's paths).
 __device__ __forceinline__ uint32_t passPaths(const LaunchContext& ctx, const RenderTile& tile) {
-    uint32_t count = tile.width * tile.height * tile.samples;
+    uint32_t count = ctx.pieces ? ctx.passPaths : tile.width * tile.height * tile.samples;
     return count < ctx.capacity ? count : ctx.capacity;
+}
+
+// The piece whose range [base, next base) holds `index` (base = pathBase or
+// pixelBase of consecutive pieces, the first at 0).
+template <bool Pixels>
+__device__ __forceinline__ uint32_t findPiece(const LaunchContext& ctx, uint32_t index) {
+    uint32_t lo = 0, hi = ctx.pieceCount;  // invariant: base[lo] <= index < base[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        uint32_t base = Pixels ? ctx.pieces[mid].pixelBase : ctx.pieces[mid].pathBase;
+        if (base <= index)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Fresh path g of a batch pass: sample (g - pathBase) / pixels of its piece's
+// tile pixel; its pixel word is g, the radiance slot of the pass.
+__device__ __forceinline__ PathState batchPassPath(const LaunchContext& ctx, uint32_t g) {
+    const PassPiece& piece = ctx.pieces[findPiece<false>(ctx, g)];
+    uint32_t pixels = piece.tile.width * piece.tile.height;
+    uint32_t local = g - piece.pathBase;
+    uint32_t slot = local / pixels;
+    uint32_t x, y;
+    tilePixel(piece.tile, local - slot * pixels, x, y);
+    PathState p = primaryPath(ctx.camera, x, y, ctx.width, ctx.height, piece.sampleStart + slot, ctx.shade.rngStrategy);
+    p.pixel = g;
+    return p;
 }
 
 // Fresh path g of a pass: sample slot g / pixels of tile pixel g % pixels.
@@ -186,23 +217,46 @@ __device__ __forceinline__ PathState passPath(const LaunchContext& ctx, const Re
     return p;
 }
 
-// A finished path's radiance, stored in its sample's plane.
+// A finished path's radiance, stored in its sample's plane (batch pass: at
+// the path's index).
 __device__ __forceinline__ void finishPath(const LaunchContext& ctx, uint32_t tagged, const Vec3& v) {
-    size_t i = 3 * (static_cast<size_t>(tagged >> kSlotShift) * ctx.width * ctx.height + (tagged & kPixelMask));
+    size_t i = ctx.pieces ? 3 * static_cast<size_t>(tagged)
+                          : 3 * (static_cast<size_t>(tagged >> kSlotShift) * ctx.width * ctx.height + (tagged & kPixelMask));
     ctx.slotRadiance[i] = v.x;
     ctx.slotRadiance[i + 1] = v.y;
     ctx.slotRadiance[i + 2] = v.z;
 }
 
 __global__ void __launch_bounds__(512) kPrimary(LaunchContext ctx) {
-    RenderTile tile = *ctx.tile;
+    RenderTile tile = ctx.pieces ? RenderTile{} : *ctx.tile;
     uint32_t count = passPaths(ctx, tile);
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < kSegments) ctx.counters->count[0][gid] = primarySegmentCount(count, ctx.blockSize, gid);
     uint32_t sample = ctx.counters->sampleIndex;
     for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x)
         storePath(ctx.queue[0], primarySlot(g, ctx.blockSize, ctx.queue[0].segmentCapacity),
-            passPath(ctx, tile, sample, g));
+            ctx.pieces ? batchPassPath(ctx, g) : passPath(ctx, tile, sample, g));
+}
+
+// End of a batch pass: each piece's tile pixel adds its samples' radiance,
+// in sample order, to the task's output.
+__global__ void __launch_bounds__(512) kResolvePieces(LaunchContext ctx) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ctx.passPixels) return;
+    const PassPiece& piece = ctx.pieces[findPiece<true>(ctx, t)];
+    uint32_t pixels = piece.tile.width * piece.tile.height;
+    uint32_t p = t - piece.pixelBase;
+    uint32_t x, y;
+    tilePixel(piece.tile, p, x, y);
+    float* o = ctx.batchOut + piece.out + 3ull * ((y - piece.tile.y) * piece.outStride + (x - piece.tile.x));
+    Vec3 sum(o[0], o[1], o[2]);
+    for (uint32_t k = 0; k < piece.samples; ++k) {
+        const float* r = ctx.slotRadiance + 3 * (static_cast<size_t>(piece.pathBase) + static_cast<size_t>(k) * pixels + p);
+        sum += Vec3(r[0], r[1], r[2]);
+    }
+    o[0] = sum.x;
+    o[1] = sum.y;
+    o[2] = sum.z;
 }
 
 // End of a pass: each tile pixel adds its samples' radiance in sample order
@@ -344,6 +398,18 @@ __global__ void __launch_bounds__(512) kShade(LaunchContext ctx, uint32_t sel) {
             finishPath(ctx, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
         }
     }
+}
+
+__device__ __forceinline__ Vec3 eventContribution(const EventBuffer& b, uint32_t e, uint32_t slot) {
+    return Vec3(__uint_as_float(*eventItemWord(b, e, kEiCx, slot)), __uint_as_float(*eventItemWord(b, e, kEiCy, slot)),
+        __uint_as_float(*eventItemWord(b, e, kEiCz, slot)));
+}
+
+__device__ __forceinline__ Ray eventRay(const EventBuffer& b, uint32_t e, uint32_t slot) {
+    return Ray(Vec3(__uint_as_float(*eventItemWord(b, e, kEiOx, slot)), __uint_as_float(*eventItemWord(b, e, kEiOy, slot)),
+                   __uint_as_float(*eventItemWord(b, e, kEiOz, slot))),
+        Vec3(__uint_as_float(*eventItemWord(b, e, kEiDx, slot)), __uint_as_float(*eventItemWord(b, e, kEiDy, slot)),
+            __uint_as_float(*eventItemWord(b, e, kEiDz, slot))));
 }
 
 // Staged shading that traces no ray (scenes whose shading adds radiance
@@ -520,7 +586,7 @@ __global__ void kAdvance(LaunchContext ctx) {
 // appended to the workgroup's segment of the other queue.  Bounce b reads
 // pcount[b % 3], appends to pcount[(b + 1) % 3] and clears pcount[(b + 2) % 3],
 // which no workgroup touches again before the barrier that ends bounce b.
-template <bool Full>
+template <bool Full, bool Events>
 __global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
     __shared__ AppendShared sh;
     cg::grid_group grid = cg::this_grid();
@@ -574,7 +640,42 @@ __global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
             outcome.alive = false;
             if (valid) {
                 path = loadPath(q, segmentSlot(v, q.segmentCapacity, g));
-                if constexpr (Full) {
+                if constexpr (Full && Events) {
+                    // Shading stores its occlusion-dependent additions as
+                    // ordered events in this thread's column; the same
+                    // traversal call then resolves them one by one.
+                    bool shadowQuery = false;
+                    Ray ray = path.ray;
+                    float tmax = kNoHit;
+                    uint32_t eventCount = 0, eventNext = 0;
+                    bool eventEnds = false;
+                    while (true) {
+                        HitRecord h = traceQuery<Full, Query::kEither>(ctx.scene, ray, tmax, shadowQuery, stack);
+                        if (shadowQuery) {
+                            ++shadowQueries;
+                            if (h.kind == kHitNone) {
+                                path.radiance += eventContribution(ctx.events, eventNext, gid);
+                                if (*eventItemWord(ctx.events, eventNext, kEiFlags, gid) & kEventTerminates)
+                                    eventEnds = true;
+                            }
+                            ++eventNext;
+                        } else {
+                            EventOcclusion occ{ctx.events, gid, 0u};
+                            shadePath<Full>(ctx.scene, path, h, ctx.shade, occ, outcome);
+                            eventCount = occ.count < ctx.events.maxEvents ? occ.count : ctx.events.maxEvents;
+                        }
+                        while (eventNext < eventCount &&
+                            !(*eventItemWord(ctx.events, eventNext, kEiFlags, gid) & kEventTraced)) {
+                            path.radiance += eventContribution(ctx.events, eventNext, gid);
+                            ++eventNext;
+                        }
+                        if (eventNext >= eventCount) break;
+                        shadowQuery = true;
+                        ray = eventRay(ctx.events, eventNext, gid);
+                        tmax = __uint_as_float(*eventItemWord(ctx.events, eventNext, kEiTmax, gid));
+                    }
+                    if (eventEnds) outcome.alive = false;
+                } else if constexpr (Full) {
                     HitRecord h = traceClosest<Full>(ctx.scene, path.ray, stack);
                     InlineOcclusion<LdsStack> occ{stack, 0u};
                     shadePath<Full>(ctx.scene, path, h, ctx.shade, occ, outcome);
@@ -720,6 +821,11 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
             });
     }
     if (e != hipSuccess) return e;
+    if (ctx.pieces) {
+        uint32_t blocks = (ctx.passPixels + ctx.blockSize - 1) / ctx.blockSize;
+        e = timer(6, [&] { hipLaunchKernelGGL(kResolvePieces, dim3(blocks > 0 ? blocks : 1u), block, 0, stream, ctx); });
+        return e != hipSuccess ? e : hipGetLastError();
+    }
     e = timer(6, [&] { hipLaunchKernelGGL(kResolve, dim3(ctx.resolveBlocks), block, 0, stream, ctx); });
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kAdvance, dim3(1), dim3(1), 0, stream, ctx);
@@ -729,13 +835,21 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
 hipError_t launchPersistent(const LaunchContext& ctx, hipStream_t stream) {
     LaunchContext local = ctx;
     void* args[] = {&local};
-    const void* kernel = ctx.scene.fullFeatures ? reinterpret_cast<const void*>(kPersistent<true>)
-                                                : reinterpret_cast<const void*>(kPersistent<false>);
+    const void* kernel = !ctx.scene.fullFeatures ? reinterpret_cast<const void*>(kPersistent<false, false>)
+        : ctx.events.words                       ? reinterpret_cast<const void*>(kPersistent<true, true>)
+                                                 : reinterpret_cast<const void*>(kPersistent<true, false>);
     return hipLaunchCooperativeKernel(kernel, dim3(ctx.persistentBlocks),
         dim3(ctx.blockSize), args, stackLdsBytes(ctx.blockSize), stream);
 }
 
 }  // namespace
+
+hipError_t enqueueStagedBatchPass(const LaunchContext& ctx, hipStream_t stream) {
+    return stagedSample(ctx, stream, [](int, auto&& launch) {
+        launch();
+        return hipGetLastError();
+    });
+}
 
 hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream) {
     return stagedSample(ctx, stream, [](int, auto&& launch) {
@@ -756,14 +870,20 @@ hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out) {
         e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<true>, b, lds)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<false>, b, lds);
     if (e == hipSuccess)
-        e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<true>, b, lds)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<false>, b, lds);
+        e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<true, false>, b, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<false, false>, b, lds);
+    if (e == hipSuccess && full) {
+        // The cooperative grid must fit whichever full-feature variant runs.
+        int events = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&events, kPersistent<true, true>, b, lds);
+        if (events < out->persistent) out->persistent = events;
+    }
     return e;
 }
 
 hipError_t kernelAttributes(bool persistent, bool full, hipFuncAttributes* out) {
-    const void* k = persistent ? (full ? reinterpret_cast<const void*>(kPersistent<true>)
-                                       : reinterpret_cast<const void*>(kPersistent<false>))
+    const void* k = persistent ? (full ? reinterpret_cast<const void*>(kPersistent<true, true>)
+                                       : reinterpret_cast<const void*>(kPersistent<false, false>))
                                : (full ? reinterpret_cast<const void*>(kTrace<true>)
                                        : reinterpret_cast<const void*>(kTrace<false>));
     return hipFuncGetAttributes(out, k);

@@ -226,8 +226,11 @@ class BatchTask(C.Structure):
                 ("rr_start_depth", C.c_uint32), ("rng_strategy", C.c_uint32), ("indirect_clamp", C.c_float)]
 
 
+BATCH_STAGED, BATCH_REGENERATE = 0, 1
+
+
 class BatchConfig(C.Structure):
-    _fields_ = [("block_size", C.c_uint32), ("blocks", C.c_uint32)]
+    _fields_ = [("block_size", C.c_uint32), ("blocks", C.c_uint32), ("scheduler", C.c_uint32)]
 
 
 class BatchStats(C.Structure):
@@ -764,14 +767,16 @@ def batch_task(scene_index, camera, width, height, x, y, tile_w, tile_h, sample_
 
 
 class Batch:
-    """Many tasks of several device scenes rendered by one persistent launch
-    (wpt_batch_*).  The scenes must outlive the batch."""
+    """Many tasks of several device scenes rendered together (wpt_batch_*):
+    staged wavefront passes over many tasks (BATCH_STAGED) or one persistent
+    launch (BATCH_REGENERATE).  The scenes must outlive the batch."""
 
-    def __init__(self, scenes: Sequence["Scene"], block_size: int = 256, blocks: int = 0):
+    def __init__(self, scenes: Sequence["Scene"], block_size: int = 256, blocks: int = 0,
+                 scheduler: int = BATCH_STAGED):
         self._scenes = list(scenes)
         handles = (C.c_void_p * len(self._scenes))(*[sc.handle for sc in self._scenes])
         self._h = C.c_void_p()
-        cfg = BatchConfig(block_size, blocks)
+        cfg = BatchConfig(block_size, blocks, scheduler)
         check(library().wpt_batch_create(handles, len(self._scenes), C.byref(cfg), C.byref(self._h)),
               "wpt_batch_create")
 
