@@ -622,6 +622,7 @@ def measure_terrain(spp, photons, frames):
     modes = {"graph": wpt.MODE_GRAPH, "persistent": wpt.MODE_PERSISTENT, "regenerate": wpt.MODE_REGENERATE}
     runs, first = {}, None
     photon = None
+    breakdown = None
     for name, mode in modes.items():
         rc = terrain_config(photons, spp)
         rc.mode = mode
@@ -637,6 +638,11 @@ def measure_terrain(spp, photons, frames):
         wall = (time.perf_counter() - tp) / frames
         st = r.stats()
         img, n = r.framebuffer()
+        if mode == wpt.MODE_GRAPH:
+            # Per-kernel device time of one more frame (events around each launch).
+            k_ms, k_launches = r.profile_kernels(0, spp)
+            names = ["primary", "reset", "trace", "shade", "persistent", "shadow", "resolve"]
+            breakdown = {n_: round(float(k_ms[k]), 2) for k, n_ in enumerate(names) if k_launches[k]}
         r.close()
         if first is None:
             first = (img, n, st)
@@ -655,6 +661,7 @@ def measure_terrain(spp, photons, frames):
         "msamples_per_s": round(WIDTH * HEIGHT * spp / (runs[best]["ms_per_frame"] / 1000.0) / 1e6, 2),
         "ms_per_frame": runs[best]["ms_per_frame"],
         "schedulers": runs,
+        "staged_kernel_ms_per_frame": breakdown,
         "queries_per_path": round((st.closest_queries + st.shadow_queries) / (WIDTH * HEIGHT * spp), 3),
         "photon_map_ms": round(photon.caustic_build_milliseconds, 2),
         "photons_stored": int(photon.caustic_photons_stored),

@@ -96,6 +96,30 @@ def test_punctual_lights_match_oracle(torch_cuda, oracle_mod, mode, with_map):
 
 
 @pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("directional", [7, 10])
+def test_many_directional_lights_match_oracle(torch_cuda, oracle_mod, mode, directional):
+    # Every directional light is a radiance event of a diffuse vertex: with
+    # the area light, the positional light, the map and two escape slots, 7
+    # lights fill the 12 events a vertex may store; 10 exceed it, and such
+    # scenes trace their shadow rays inline.  Both must give the oracle's
+    # image and counts.
+    rng = np.random.default_rng(directional)
+    lights = [wpt.make_punctual_light(wpt.LIGHT_DIRECTIONAL, direction=(rng.uniform(-1, 1), -1.0, rng.uniform(-1, 1)),
+                                      intensity=tuple(rng.uniform(0.05, 0.2, 3))) for _ in range(directional)]
+    lights.append(wpt.make_punctual_light(wpt.LIGHT_POINT, (0.4, 0.8, 0.6), intensity=(0.4, 0.3, 0.2)))
+    src = PresetWithMap("cornell-box", random_map(), 0.5, 20.0, lights=lights)
+    scene = wpt.Scene(src.desc(), device=True, owner=src)
+    cam = src.camera(64, 48)
+    cfg = wpt.render_config(64, 48, spp=2, max_depth=6, rr_start=3, mode=mode, clamp=10.0)
+    r = wpt.Renderer(scene, cam, cfg)
+    img, _ = r.render_frame()
+    want, ost = oracle_mod.Oracle(scene).render(cam, cfg)
+    assert np.array_equal(img, want), f"max |diff| {np.abs(img - want).max()}"
+    st = r.stats()
+    assert st.closest_queries == ost.closest_queries and st.shadow_queries == ost.shadow_queries
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_tiled_render_task_with_map(torch_cuda, oracle_mod, mode):
     src = PresetWithMap("rtweek", random_map(48, 24, 9), 1.3, 10.0, 8)
     scene = wpt.Scene(src.desc(), device=True, owner=src)
