@@ -606,7 +606,7 @@ def roughness_observability(r, truth, target, lay, seeds=8):
     return out
 
 
-def measure_terrain(spp, photons, frames):
+def measure_terrain(spp, photons, frames, schedulers=("graph", "persistent", "regenerate")):
     """Shading-heavy workload with caustics: the terrain preset at full size
     (2,006,680 triangles, 16 x 4096^2 RGBA8 textures, 512 emissive
     triangles, a 2048 x 1024 HDR sky with a sun, dielectric crystals) with a
@@ -624,6 +624,8 @@ def measure_terrain(spp, photons, frames):
     photon = None
     breakdown = None
     for name, mode in modes.items():
+        if name not in schedulers:
+            continue
         rc = terrain_config(photons, spp)
         rc.mode = mode
         r = wpt.Renderer(scene, camera, rc)
@@ -698,6 +700,8 @@ def main():
                     help="do not measure closest-hit traversal alone on the reference and primary ray sets")
     ap.add_argument("--terrain-spp", type=int, default=64)
     ap.add_argument("--terrain-photons", type=int, default=4 * 1024 * 1024)
+    ap.add_argument("--terrain-schedulers", default="graph,persistent,regenerate",
+                    help="comma-separated schedulers timed on the terrain")
     ap.add_argument("--diff-steps", type=int, default=200, help="Adam steps of the inverse-rendering workload")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB three times: the reference kernel of "
@@ -849,7 +853,7 @@ def main():
 
     terrain = None
     if not args.skip_terrain:
-        terrain = measure_terrain(args.terrain_spp, args.terrain_photons, 1)
+        terrain = measure_terrain(args.terrain_spp, args.terrain_photons, 1, args.terrain_schedulers.split(","))
 
     roofline = None
     if work is not None and trace_ms > 0:

@@ -857,7 +857,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
     // trace kernel's resident grid (which bounds the traversal spill area);
     // the persistent kernel runs a resident grid.
     Occupancy occ{};
-    R_HIP(kernelOccupancy(c.block_size, ctx.scene.fullFeatures != 0, &occ));
+    R_HIP(kernelOccupancy(c.block_size, ctx.scene.fullFeatures != 0, &occ, vertexEvents <= kMaxVertexEvents));
     int device = 0, cus = 0, coop = 0;
     R_HIP(hipGetDevice(&device));
     R_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -1176,8 +1176,9 @@ WPT_API int wpt_renderer_launch_info(const wpt_renderer* r, wpt_launch_info* inf
         info->blocks = r->regenBlocks;
     } else {
         Occupancy occ{};
-        WPT_HIP(kernelOccupancy(c.block_size, full, &occ));
-        WPT_HIP(kernelAttributes(c.mode == WPT_MODE_PERSISTENT, full, &attr));
+        bool events = ctx.events.words != nullptr;
+        WPT_HIP(kernelOccupancy(c.block_size, full, &occ, events));
+        WPT_HIP(kernelAttributes(c.mode == WPT_MODE_PERSISTENT, full, &attr, events));
         perCu = c.mode == WPT_MODE_PERSISTENT ? occ.persistent : occ.trace;
         info->blocks = c.mode == WPT_MODE_PERSISTENT ? ctx.persistentBlocks : 0u;
     }

@@ -32,6 +32,22 @@ constexpr int batchWavesPerEu(bool full) {
 #else
 #define WPT_TRACE_OCCUPANCY
 #endif
+// Occupancy target of the cooperative kernel with the full feature set, 0:
+// compiler's choice (249 VGPRs, 2 waves per SIMD).
+#ifndef WPT_PERSISTENT_WAVES_PER_EU
+#define WPT_PERSISTENT_WAVES_PER_EU 0
+#endif
+constexpr int persistentWavesPerEu(bool full) {
+    return full && WPT_PERSISTENT_WAVES_PER_EU > 0 ? WPT_PERSISTENT_WAVES_PER_EU : 1;
+}
+#define WPT_PERSISTENT_OCCUPANCY(full) __attribute__((amdgpu_waves_per_eu(persistentWavesPerEu(full))))
+// Occupancy target of the full-feature event shading kernel, 0: compiler's
+// choice (206 VGPRs, 2 waves per SIMD).
+#if defined(WPT_SHADE_WAVES_PER_EU) && WPT_SHADE_WAVES_PER_EU > 0
+#define WPT_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WPT_SHADE_WAVES_PER_EU)))
+#else
+#define WPT_SHADE_OCCUPANCY
+#endif
 
 __device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);

@@ -169,7 +169,9 @@ struct Occupancy {
 };
 // Occupancy of the kernels serving scenes with (full) or without
 // environment maps, punctual lights and texture bindings.
-hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out);
+// `events`: the full-feature cooperative kernel resolves radiance events
+// (an event buffer exists), else it traces inline.
+hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out, bool events = true);
 // One pass of the current tile with every launch bracketed by events:
 // ms/launches accumulate per kernel (0 primary, 1 reset, 2 trace, 3 shade,
 // 4 persistent, 5 shadow, 6 resolve).  Synchronizes the stream.
@@ -248,7 +250,7 @@ hipError_t batchOccupancy(uint32_t blockSize, bool full, int* blocksPerCu);
 // Compiled resources (registers, scratch, static LDS) of the batch kernel and
 // of the staged trace or cooperative persistent kernel.
 hipError_t batchAttributes(bool full, hipFuncAttributes* out);
-hipError_t kernelAttributes(bool persistent, bool full, hipFuncAttributes* out);
+hipError_t kernelAttributes(bool persistent, bool full, hipFuncAttributes* out, bool events = true);
 
 // Standalone traversal for the C ABI; `spill` holds
 // (kMaxStackDepth - kLdsStackEntries) x abiTraceThreads() entries.

@@ -419,7 +419,7 @@ __device__ __forceinline__ Ray eventRay(const EventBuffer& b, uint32_t e, uint32
 // traces the events' rays and completes the radiance.  Paths without events
 // finish or continue here as in kShade.
 template <bool Full>
-__global__ void __launch_bounds__(512) kShadeEvents(LaunchContext ctx, uint32_t sel) {
+__global__ void __launch_bounds__(512) WPT_SHADE_OCCUPANCY kShadeEvents(LaunchContext ctx, uint32_t sel) {
     __shared__ AppendShared sh;
     SegmentView v = loadSegments(ctx.counters->count[sel]);
     uint32_t count = v.start[kSegments];
@@ -587,7 +587,7 @@ __global__ void kAdvance(LaunchContext ctx) {
 // pcount[b % 3], appends to pcount[(b + 1) % 3] and clears pcount[(b + 2) % 3],
 // which no workgroup touches again before the barrier that ends bounce b.
 template <bool Full, bool Events>
-__global__ void __launch_bounds__(512) kPersistent(LaunchContext ctx) {
+__global__ void __launch_bounds__(512) WPT_PERSISTENT_OCCUPANCY(Full && Events) kPersistent(LaunchContext ctx) {
     __shared__ AppendShared sh;
     cg::grid_group grid = cg::this_grid();
     WavefrontCounters* counters = ctx.counters;
@@ -858,7 +858,7 @@ hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream) {
     });
 }
 
-hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out) {
+hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out, bool events) {
     size_t lds = stackLdsBytes(blockSize);
     int b = static_cast<int>(blockSize);
     hipError_t e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace<true>, b, lds)
@@ -869,20 +869,17 @@ hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out) {
     if (e == hipSuccess)
         e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<true>, b, lds)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<false>, b, lds);
+    // The cooperative grid must fit the variant that runs.
     if (e == hipSuccess)
-        e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<true, false>, b, lds)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<false, false>, b, lds);
-    if (e == hipSuccess && full) {
-        // The cooperative grid must fit whichever full-feature variant runs.
-        int events = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&events, kPersistent<true, true>, b, lds);
-        if (events < out->persistent) out->persistent = events;
-    }
+        e = !full  ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<false, false>, b, lds)
+            : events ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<true, true>, b, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<true, false>, b, lds);
     return e;
 }
 
-hipError_t kernelAttributes(bool persistent, bool full, hipFuncAttributes* out) {
-    const void* k = persistent ? (full ? reinterpret_cast<const void*>(kPersistent<true, true>)
+hipError_t kernelAttributes(bool persistent, bool full, hipFuncAttributes* out, bool events) {
+    const void* k = persistent ? (full ? (events ? reinterpret_cast<const void*>(kPersistent<true, true>)
+                                                 : reinterpret_cast<const void*>(kPersistent<true, false>))
                                        : reinterpret_cast<const void*>(kPersistent<false, false>))
                                : (full ? reinterpret_cast<const void*>(kTrace<true>)
                                        : reinterpret_cast<const void*>(kTrace<false>));
