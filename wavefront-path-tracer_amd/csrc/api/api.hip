@@ -1422,8 +1422,13 @@ int runBatch(wpt_batch* b, uint32_t n, const wpt_batch_config& c, float* ms) {
 
 
 // ---- staged batches
-// Paths per staged batch pass (16 samples of a 1920x1080 tile).
-constexpr uint32_t kStagedBatchPaths = 1u << 25;
+// Paths per staged batch pass (2^26: 32 samples of a 1920x1080 tile; about
+// 31 GB of queues, records and events when a scene needs events; the 256-task
+// benchmark batch: 2.72 s at 2^25, 2.59 s at 2^26).
+#ifndef WPT_STAGED_BATCH_PATHS_LOG2
+#define WPT_STAGED_BATCH_PATHS_LOG2 26
+#endif
+constexpr uint32_t kStagedBatchPaths = 1u << WPT_STAGED_BATCH_PATHS_LOG2;
 
 // Allocates the shared wavefront buffers of staged batch passes: queues, hit
 // records, deferred shadow requests, radiance events (when a scene needs
