@@ -485,8 +485,9 @@ WPT_API int wpt_renderer_last_stats(const wpt_renderer* renderer, wpt_stats* sta
  * HIP events on the renderer's stream (no graph), accumulating device time
  * per kernel into kernel_ms[7] and launch counts into launches[7]: 0 primary
  * rays, 1 queue reset, 2 closest-hit trace, 3 shade, 4 persistent, 5 shadow
- * rays, 6 pass resolve (a pass's samples added to the framebuffer in
- * sample order). */
+ * rays (with radiance events: the event rays and the pass adding the
+ * unblocked contributions), 6 pass resolve (a pass's samples added to the
+ * framebuffer in sample order). */
 WPT_API int wpt_renderer_profile_kernels(wpt_renderer* renderer, uint32_t first_sample, uint32_t count,
                                          double* kernel_ms, uint32_t* launches);
 
@@ -532,6 +533,9 @@ typedef struct wpt_batch_config {
      * over every task (per-task latency from the kernel's clock).  Outputs
      * are identical. */
     uint32_t scheduler;
+    /* Staged scheduler: paths per pass (0: 2^26; at most 2^26).  A task's
+     * tile must not hold more pixels than a pass holds paths. */
+    uint32_t pass_paths;
 } wpt_batch_config;
 
 #define WPT_BATCH_STAGED 0u

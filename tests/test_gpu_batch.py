@@ -139,3 +139,22 @@ def test_batch_edge_cases(torch_cuda, scheduler):
                 wpt.batch_task(0, cam, w, h, 0, 0, 8, 8, 0, 1, clamp=-1.0)):
         with pytest.raises(wpt.WptError):
             batch.render([bad])
+
+
+def test_staged_batch_small_passes(torch_cuda):
+    # Passes of 3000 paths split tasks of up to 1536 pixels into pieces of a
+    # few samples across passes; the sums must not depend on the pass size.
+    hosts, scenes = make_scenes()
+    tasks = random_tasks(hosts, 16, seed=21, max_spp=5)
+    want = wpt.Batch(scenes, scheduler=wpt.BATCH_STAGED).render(tasks)
+    small = wpt.Batch(scenes, scheduler=wpt.BATCH_STAGED, pass_paths=3000)
+    outs = small.render(tasks)
+    for i, (a, b) in enumerate(zip(outs, want)):
+        assert np.array_equal(a, b), f"task {i}"
+    assert small.autotune(tasks).pass_paths == 3000
+    w, h = FRAME
+    big = wpt.batch_task(0, hosts[0].camera(w, h), w, h, 0, 0, 64, 64, 0, 1)  # 4096 pixels > 3000
+    with pytest.raises(wpt.WptError):
+        small.render([big])
+    with pytest.raises(wpt.WptError):
+        wpt.Batch(scenes, scheduler=wpt.BATCH_STAGED, pass_paths=(1 << 26) + 1)

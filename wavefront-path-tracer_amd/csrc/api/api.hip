@@ -666,13 +666,13 @@ int buildPhotonMap(const DevScene& scene, const wpt_render_config& c, DeviceAren
     WPT_HIP(photonSortBytes(c.caustic_photons, cells, &pb.tempBytes));
     WPT_HIP(arena.alloc(pb.tempBytes > 0 ? pb.tempBytes : 1, &p));
     pb.temp = p;
-    WPT_HIP(arena.alloc(sizeof(CellRange) * cells, &p));
-    pb.cells = static_cast<CellRange*>(p);
+    WPT_HIP(arena.alloc(sizeof(uint32_t) * (static_cast<size_t>(cells) + 1), &p));
+    pb.cellStart = static_cast<uint32_t*>(p);
     WPT_HIP(arena.alloc(sizeof(uint32_t) * 7, &p));
     pb.stored = static_cast<uint32_t*>(p);
     pb.occupied = pb.stored + 1;
     view.photons = pb.sorted;
-    view.cells = pb.cells;
+    view.cellStart = pb.cellStart;
     WPT_HIP(hipEventRecord(start, stream));
     WPT_HIP(enqueuePhotonMap(scene, ps, pb, view, spill, spillThreads, stream));
     WPT_HIP(hipEventRecord(end, stream));
@@ -726,6 +726,21 @@ uint32_t vertexEventsOf(const DevScene& s) {
 }
 
 uint32_t eventWordsPerColumn(uint32_t maxEvents) { return kEventHeaderFields + maxEvents * kEventItemFields; }
+
+// The staged event ray list: a queue segment's paths append at most
+// maxEvents rays each to the same segment of the list.
+hipError_t allocEventRays(DeviceArena& arena, uint64_t segment, uint32_t maxEvents, EventBuffer* events) {
+    uint64_t raySegment = segment * maxEvents;
+    if (raySegment >= (1ull << 32) / kSegments) return hipErrorInvalidValue;
+    size_t entries = static_cast<size_t>(raySegment) * kSegments;
+    void* p = nullptr;
+    hipError_t e = arena.alloc((sizeof(uint32_t) + sizeof(uint8_t)) * entries, &p);
+    if (e != hipSuccess) return e;
+    events->raySlot = static_cast<uint32_t*>(p);
+    events->rayEvent = reinterpret_cast<uint8_t*>(events->raySlot + entries);
+    events->raySegment = static_cast<uint32_t>(raySegment);
+    return hipSuccess;
+}
 constexpr uint32_t kNoAllocScratchWords = 1u + kNoAllocMaxItems / 256u;
 static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
 
@@ -921,6 +936,7 @@ WPT_API int wpt_renderer_create(wpt_scene* scene, const wpt_camera* camera, cons
             ctx.events.words = static_cast<uint32_t*>(p);
             ctx.events.capacity = static_cast<uint32_t>(qcap);
             ctx.events.maxEvents = vertexEvents;
+            R_HIP(allocEventRays(r->arena, segment, vertexEvents, &ctx.events));
         }
     }
     if (c.mode == WPT_MODE_PERSISTENT && ctx.scene.fullFeatures && vertexEvents <= kMaxVertexEvents) {
@@ -1275,6 +1291,7 @@ int validateBatchConfig(const wpt_batch_config& c) {
     if (c.block_size == 0 || c.block_size > 512 || c.block_size % 64 != 0)
         return fail(WPT_ERROR_INVALID_ARGUMENT, "Block size must be a multiple of 64 between 64 and 512");
     if (c.scheduler > WPT_BATCH_REGENERATE) return fail(WPT_ERROR_INVALID_ARGUMENT, "unknown batch scheduler");
+    if (c.pass_paths > (1u << 26)) return fail(WPT_ERROR_INVALID_ARGUMENT, "staged batch passes hold at most 2^26 paths");
     return WPT_OK;
 }
 
@@ -1433,7 +1450,7 @@ constexpr uint32_t kStagedBatchPaths = 1u << WPT_STAGED_BATCH_PATHS_LOG2;
 // Allocates the shared wavefront buffers of staged batch passes: queues, hit
 // records, deferred shadow requests, radiance events (when a scene needs
 // them), per-path radiance, counters and the traversal spill area.
-int ensureStagedBatch(wpt_batch* b, uint32_t blockSize) {
+int ensureStagedBatch(wpt_batch* b, uint32_t blockSize, uint32_t paths) {
     uint32_t maxEvents = 0;
     bool anyEvents = false, full = false;
     for (size_t i = 0; i < b->scenes.size(); ++i) {
@@ -1446,13 +1463,15 @@ int ensureStagedBatch(wpt_batch* b, uint32_t blockSize) {
     }
     if (maxEvents > kMaxVertexEvents) maxEvents = 0;  // such scenes trace inline
     uint32_t wantEvents = anyEvents ? maxEvents : 0u;
-    if (b->stagedArena && b->stagedBlock == blockSize && b->stagedEvents >= wantEvents) return WPT_OK;
+    if (b->stagedArena && b->stagedBlock == blockSize && b->stagedEvents >= wantEvents && b->stagedCapacity >= paths)
+        return WPT_OK;
+    b->stagedArena.reset();  // free the old buffers first
     b->stagedArena.reset(new (std::nothrow) DeviceArena());
     if (!b->stagedArena) return fail(WPT_ERROR_OUT_OF_MEMORY, "host allocation failed");
     b->staged = LaunchContext{};
     LaunchContext& ctx = b->staged;
     DeviceArena& arena = *b->stagedArena;
-    const uint32_t cap = kStagedBatchPaths, B = blockSize;
+    const uint32_t cap = std::max(paths, 1u), B = blockSize;
     Occupancy lean{}, rich{};
     WPT_HIP(kernelOccupancy(B, false, &lean));
     if (full) WPT_HIP(kernelOccupancy(B, true, &rich));
@@ -1496,6 +1515,7 @@ int ensureStagedBatch(wpt_batch* b, uint32_t blockSize) {
         ctx.events.words = static_cast<uint32_t*>(p);
         ctx.events.capacity = static_cast<uint32_t>(qcap);
         ctx.events.maxEvents = wantEvents;
+        WPT_HIP(allocEventRays(arena, segment, wantEvents, &ctx.events));
     }
     WPT_HIP(arena.alloc(sizeof(float) * 3 * static_cast<size_t>(cap), &p));
     ctx.slotRadiance = static_cast<float*>(p);
@@ -1527,13 +1547,18 @@ bool sameStagedGroup(const wpt_batch_task& a, const wpt_batch_task& c) {
 // the device time; latencyMs (optional) the end of each task's last pass.
 int runStagedBatch(wpt_batch* b, const wpt_batch_task* tasks, uint32_t n, const std::vector<size_t>& offsets,
     float* ms, double* latencyMs) {
-    int s = ensureStagedBatch(b, b->config.block_size);
-    if (s != WPT_OK) return s;
-    const uint32_t cap = b->stagedCapacity;
+    const uint32_t passLimit = b->config.pass_paths ? b->config.pass_paths : kStagedBatchPaths;
     for (uint32_t i = 0; i < n; ++i)
-        if (tasks[i].sample_count && static_cast<uint64_t>(tasks[i].tile_width) * tasks[i].tile_height > cap)
+        if (tasks[i].sample_count && static_cast<uint64_t>(tasks[i].tile_width) * tasks[i].tile_height > passLimit)
             return fail(WPT_ERROR_UNSUPPORTED, "Batch task " + std::to_string(i) +
                 ": tile larger than a staged pass; use the regenerating batch scheduler");
+    // Passes hold at most passLimit paths; a smaller batch needs no more
+    // than its own paths.
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n; ++i) total += static_cast<uint64_t>(tasks[i].tile_width) * tasks[i].tile_height * tasks[i].sample_count;
+    int s = ensureStagedBatch(b, b->config.block_size, static_cast<uint32_t>(std::min<uint64_t>(total, passLimit)));
+    if (s != WPT_OK) return s;
+    const uint32_t cap = std::min(b->stagedCapacity, passLimit);
     // Groups in order of their first task; passes in group order.
     std::vector<int> group(n, -1);
     std::vector<uint32_t> leaders;
@@ -1803,7 +1828,7 @@ WPT_API int wpt_batch_autotune(wpt_batch* b, const wpt_batch_task* tasks, uint32
     if (b->config.scheduler == WPT_BATCH_STAGED) {
         // Staged passes: the workgroup size of the pass kernels.
         for (uint32_t block : {128u, 256u, 512u}) {
-            wpt_batch_config c{block, 0u, WPT_BATCH_STAGED};
+            wpt_batch_config c{block, 0u, WPT_BATCH_STAGED, b->config.pass_paths};
             wpt_batch_config saved = b->config;
             b->config = c;
             float ms = 0.0f;
@@ -1823,7 +1848,7 @@ WPT_API int wpt_batch_autotune(wpt_batch* b, const wpt_batch_task* tasks, uint32
             if (perCu <= 0) continue;
             uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(b->cus);
             for (uint32_t quarter = 1; quarter <= 4; ++quarter) {
-                wpt_batch_config c{block, std::max(1u, resident * quarter / 4u), WPT_BATCH_REGENERATE};
+                wpt_batch_config c{block, std::max(1u, resident * quarter / 4u), WPT_BATCH_REGENERATE, b->config.pass_paths};
                 float ms = 0.0f;
                 s = runBatch(b, n, c, &ms);
                 if (s != WPT_OK) return s;

@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(512) WPT_BATCH_OCCUPANCY(Full) kBatch(BatchCon
                     shadePath<Full>(scene, path, h, T.shade, occ, outcome);
                     pending = outcome.shadow;
                 } else if constexpr (Events) {
-                    EventOcclusion occ{ctx.events, lane, 0u};
+                    EventOcclusion occ{ctx.events, lane, 0u, 0u};
                     shadePath<Full>(scene, path, h, T.shade, occ, outcome);
                     eventCount = occ.count < ctx.events.maxEvents ? occ.count : ctx.events.maxEvents;
                     eventNext = 0;

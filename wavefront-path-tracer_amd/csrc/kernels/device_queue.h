@@ -197,6 +197,45 @@ __device__ __forceinline__ void blockAppend(bool a0, bool a1, uint32_t* counter0
     __syncthreads();
 }
 
+// Workgroup-aggregated append of continuing paths (kind 0, one item per
+// `alive` lane) and of traced events (kind 1, bit e of `mask` for event e <
+// `events`): one atomic per kind per workgroup.  Every thread of the
+// workgroup must call it.  index0 is an alive lane's index within its
+// segment; the wave's event entries start at waveBase1, grouped by event
+// index and in lane order within a group (what the lanes' own ballots over
+// e < events reproduce).
+__device__ __forceinline__ void blockAppendEvents(bool alive, uint32_t mask, uint32_t events, uint32_t* counter0,
+    uint32_t* counter1, AppendShared& sh, uint32_t& index0, uint32_t& waveBase1) {
+    unsigned long long m0 = __ballot(alive);
+    uint32_t r0 = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m0 >> 32),
+        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m0), 0u));
+    uint32_t total1 = 0;
+    for (uint32_t e = 0; e < events; ++e) total1 += static_cast<uint32_t>(__popcll(__ballot((mask >> e) & 1u)));
+    uint32_t w = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        sh.wave[w][0] = static_cast<uint32_t>(__popcll(m0));
+        sh.wave[w][1] = total1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t waves = blockDim.x >> 6;
+        uint32_t t0 = 0, t1 = 0;
+        for (uint32_t k = 0; k < waves; ++k) {
+            uint32_t c0 = sh.wave[k][0], c1 = sh.wave[k][1];
+            sh.wave[k][0] = t0;
+            sh.wave[k][1] = t1;
+            t0 += c0;
+            t1 += c1;
+        }
+        sh.base[0] = t0 ? atomicAdd(counter0, t0) : 0u;
+        sh.base[1] = t1 ? atomicAdd(counter1, t1) : 0u;
+    }
+    __syncthreads();
+    index0 = sh.base[0] + sh.wave[w][0] + r0;
+    waveBase1 = sh.base[1] + sh.wave[w][1];
+    __syncthreads();
+}
+
 // Tile-local index -> pixel.  Rows are walked in strips of 8 and each strip
 // column-major, so a wave's 64 consecutive paths cover an 8x8 pixel block.
 __device__ __forceinline__ void tilePixel(const RenderTile& tile, uint32_t i, uint32_t& x, uint32_t& y) {

@@ -28,17 +28,14 @@ struct Photon {
     uint32_t pad;
 };
 
-// The photons of one cell: sorted[start .. end).
-struct alignas(8) CellRange {
-    uint32_t start, end;
-};
-
 // Grid over [minimum, maximum] with resolution^3 cells; photons sorted by
-// cell.  occupiedMin/Max bound the cell coordinates of every non-empty cell
-// (min > max: no photon stored); cells outside that box are empty.
+// cell, cell c holding photons[cellStart[c] .. cellStart[c + 1]) (cellStart
+// has resolution^3 + 1 entries).  occupiedMin/Max bound the cell
+// coordinates of every non-empty cell (min > max: no photon stored); cells
+// outside that box are empty.
 struct PhotonGridView {
     const Photon* photons;
-    const CellRange* cells;
+    const uint32_t* cellStart;
     uint32_t resolution;
     float radius;  // gather radius floor
     Vec3 minimum;
@@ -109,43 +106,50 @@ WPT_HD Vec3 gatherCaustics(const PhotonGridView& g, const Vec3& position, const 
         float upper = mins[a] + sizes[a] * static_cast<float>(i + 1) + margin[a];
         return query[a] < lower ? lower - query[a] : (query[a] > upper ? query[a] - upper : 0.0f);
     };
+    // The cells of one row (z, y) that the test keeps form one run of x (the
+    // gap grows away from the query's cell), and their photons one run of
+    // the sorted order: a row is a single loop over photons.
     for (int z = lo[2]; z <= hi[2]; ++z) {
         float gz = gap(2, z);
         for (int y = lo[1]; y <= hi[1]; ++y) {
             float gy = gap(1, y);
             float gyz = gy * gy + gz * gz;
             if (gyz > cullSquared) continue;
+            int first = hi[0] + 1, last = lo[0] - 1;
             for (int x = lo[0]; x <= hi[0]; ++x) {
                 float gx = gap(0, x);
                 if (gx * gx + gyz > cullSquared) continue;
-                uint32_t cell = (static_cast<uint32_t>(z) * g.resolution + static_cast<uint32_t>(y)) * g.resolution +
-                    static_cast<uint32_t>(x);
-                const CellRange range = g.cells[cell];
-                for (uint32_t i = range.start; i < range.end; ++i) {
-                    const Photon& photon = g.photons[i];
-                    Vec3 d = photon.position - position;
-                    float distanceSquared = dot(d, d);
-                    float cosine = dot(normal, photon.direction);
-                    if (distanceSquared > searchRadiusSquared || cosine <= 0.0f) continue;
-                    if (count < kNearest) {
-                        distances[count] = distanceSquared;
-                        indices[count] = i;
-                        if (distanceSquared > farthestSquared) {
-                            farthestSquared = distanceSquared;
-                            farthest = count;
-                        }
-                        ++count;
-                    } else if (distanceSquared < farthestSquared) {
-                        distances[farthest] = distanceSquared;
-                        indices[farthest] = i;
-                        farthest = 0;
-                        farthestSquared = distances[0];
-                        for (uint32_t k = 1; k < kNearest; ++k)
-                            if (distances[k] > farthestSquared) {
-                                farthestSquared = distances[k];
-                                farthest = k;
-                            }
+                first = x < first ? x : first;
+                last = x;
+            }
+            if (first > last) continue;
+            uint32_t row = (static_cast<uint32_t>(z) * g.resolution + static_cast<uint32_t>(y)) * g.resolution;
+            uint32_t begin = g.cellStart[row + static_cast<uint32_t>(first)];
+            uint32_t end = g.cellStart[row + static_cast<uint32_t>(last) + 1u];
+            for (uint32_t i = begin; i < end; ++i) {
+                const Photon& photon = g.photons[i];
+                Vec3 d = photon.position - position;
+                float distanceSquared = dot(d, d);
+                float cosine = dot(normal, photon.direction);
+                if (distanceSquared > searchRadiusSquared || cosine <= 0.0f) continue;
+                if (count < kNearest) {
+                    distances[count] = distanceSquared;
+                    indices[count] = i;
+                    if (distanceSquared > farthestSquared) {
+                        farthestSquared = distanceSquared;
+                        farthest = count;
                     }
+                    ++count;
+                } else if (distanceSquared < farthestSquared) {
+                    distances[farthest] = distanceSquared;
+                    indices[farthest] = i;
+                    farthest = 0;
+                    farthestSquared = distances[0];
+                    for (uint32_t k = 1; k < kNearest; ++k)
+                        if (distances[k] > farthestSquared) {
+                            farthestSquared = distances[k];
+                            farthest = k;
+                        }
                 }
             }
         }

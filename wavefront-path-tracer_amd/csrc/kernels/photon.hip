@@ -81,17 +81,27 @@ __global__ void kFillWords(uint32_t* p, uint32_t n, uint32_t value) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = value;
 }
 
-// Cell ranges of the sorted keys, and the photons in sorted order.
-__global__ void kPhotonRanges(PhotonSettings ps, uint32_t cells, const uint64_t* sortedKeys, const Photon* photons,
-    Photon* sorted, CellRange* ranges) {
+// The photons in sorted order.
+__global__ void kPhotonSorted(PhotonSettings ps, const uint64_t* sortedKeys, const Photon* photons, Photon* sorted) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= ps.count) return;
-    uint64_t key = sortedKeys[p];
-    uint32_t cell = static_cast<uint32_t>(key >> 32);
-    sorted[p] = photons[static_cast<uint32_t>(key)];
-    if (cell >= cells) return;
-    if (p == 0 || static_cast<uint32_t>(sortedKeys[p - 1] >> 32) != cell) ranges[cell].start = p;
-    if (p + 1 == ps.count || static_cast<uint32_t>(sortedKeys[p + 1] >> 32) != cell) ranges[cell].end = p + 1;
+    if (p < ps.count) sorted[p] = photons[static_cast<uint32_t>(sortedKeys[p])];
+}
+
+// cellStart[c]: the sorted keys of cells below c (a lower bound of c << 32;
+// c == cells counts every stored photon).
+__global__ void kPhotonCellStarts(PhotonSettings ps, uint32_t cells, const uint64_t* sortedKeys, uint32_t* cellStart) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > cells) return;
+    const uint64_t key = static_cast<uint64_t>(c) << 32;
+    uint32_t lo = 0, hi = ps.count;  // first index whose key >= key
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (sortedKeys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    cellStart[c] = lo;
 }
 
 uint32_t keyBits(uint32_t cells) {
@@ -125,11 +135,9 @@ hipError_t enqueuePhotonMap(const DevScene& scene, const PhotonSettings& ps, con
     hipError_t e = hipcub::DeviceRadixSort::SortKeys(b.temp, bytes, b.keys, b.sortedKeys, static_cast<int>(ps.count),
         0, static_cast<int>(keyBits(cells)), stream);
     if (e != hipSuccess) return e;
-    uint32_t fillBlocks = (2 * cells + 255) / 256 < 4096 ? (2 * cells + 255) / 256 : 4096;
-    hipLaunchKernelGGL(kFillWords, dim3(fillBlocks), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(b.cells),
-        2 * cells, 0u);
-    hipLaunchKernelGGL(kPhotonRanges, dim3(blocks), dim3(256), 0, stream, ps, cells, b.sortedKeys, b.photons,
-        b.sorted, b.cells);
+    hipLaunchKernelGGL(kPhotonSorted, dim3(blocks), dim3(256), 0, stream, ps, b.sortedKeys, b.photons, b.sorted);
+    hipLaunchKernelGGL(kPhotonCellStarts, dim3(cells / 256 + 1), dim3(256), 0, stream, ps, cells, b.sortedKeys,
+        b.cellStart);
     return hipGetLastError();
 }
 

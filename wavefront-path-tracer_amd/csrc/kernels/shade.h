@@ -299,11 +299,18 @@ enum EventItemField : uint32_t {
 };
 constexpr uint32_t kEventTraced = 1u;      // the ray must be unblocked for the addition
 constexpr uint32_t kEventTerminates = 2u;  // an unblocked ray ends the path
+constexpr uint32_t kEventBlocked = 4u;     // set by the pass that traces the ray, when blocked
 
 struct EventBuffer {
     uint32_t* words;      // (kEventHeaderFields + maxEvents * kEventItemFields) x capacity
     uint32_t capacity;
     uint32_t maxEvents;
+    // Staged shading: the traced events of a bounce, appended per workgroup
+    // segment of raySegment entries; entry k names event rayEvent[k] of
+    // slot raySlot[k].
+    uint32_t* raySlot;
+    uint8_t* rayEvent;
+    uint32_t raySegment;
 };
 
 WPT_HD uint32_t* eventWord(const EventBuffer& b, uint32_t field, uint32_t slot) {
@@ -319,6 +326,7 @@ struct EventOcclusion {
     EventBuffer buffer;
     uint32_t slot;
     uint32_t count;
+    uint32_t traced;  // bit e: stored event e has a ray to trace
     WPT_HD bool blocked(const DevScene&, const Ray&, float) { return false; }
     // Stores event `count` (bounded by the buffer's maxEvents, which the
     // renderer sizes from the scene's lights and materials) and returns its
@@ -337,6 +345,7 @@ struct EventOcclusion {
         *eventItemWord(buffer, e, kEiCy, slot) = bitsOf(c.y);
         *eventItemWord(buffer, e, kEiCz, slot) = bitsOf(c.z);
         *eventItemWord(buffer, e, kEiFlags, slot) = traced ? kEventTraced : 0u;
+        if (traced) this->traced |= 1u << e;
         return e;
     }
     WPT_HD void add(PathState& path, const Vec3& c) {

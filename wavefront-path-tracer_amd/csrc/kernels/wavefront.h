@@ -225,8 +225,8 @@ struct BatchContext {
 };
 
 // Caustic photon map (photon.hip, photon_trace.h): the renderer's buffers, and
-// the build: trace, key, sort, cell ranges.  `view` names the buffers the
-// gather reads (sorted photons and cell ranges) and the grid box.
+// the build: trace, key, sort, cell starts.  `view` names the buffers the
+// gather reads (sorted photons and cell starts) and the grid box.
 struct PhotonMapBuffers {
     Photon* photons;  // emission order
     Photon* sorted;   // cell order
@@ -234,7 +234,7 @@ struct PhotonMapBuffers {
     uint64_t* sortedKeys;
     void* temp;
     size_t tempBytes;
-    CellRange* cells;
+    uint32_t* cellStart;  // resolution^3 + 1 entries
     uint32_t* stored;    // photons kept inside the grid
     uint32_t* occupied;  // min x, y, z then max x, y, z of the non-empty cells
 };

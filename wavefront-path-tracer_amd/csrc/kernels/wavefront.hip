@@ -415,9 +415,10 @@ __device__ __forceinline__ Ray eventRay(const EventBuffer& b, uint32_t e, uint32
 // Staged shading that traces no ray (scenes whose shading adds radiance
 // after an occlusion test): each path's occlusion-dependent additions become
 // ordered events at its slot (EventOcclusion), continuing paths are appended
-// to the workgroup's segment of the next queue, and kShadowEvents then
-// traces the events' rays and completes the radiance.  Paths without events
-// finish or continue here as in kShade.
+// to the workgroup's segment of the next queue and the traced events to its
+// segment of the event ray list; kShadowEvents then traces the list and
+// kResolveEvents completes the radiance.  Paths without events finish or
+// continue here as in kShade.
 template <bool Full>
 __global__ void __launch_bounds__(512) WPT_SHADE_OCCUPANCY kShadeEvents(LaunchContext ctx, uint32_t sel) {
     __shared__ AppendShared sh;
@@ -426,10 +427,11 @@ __global__ void __launch_bounds__(512) WPT_SHADE_OCCUPANCY kShadeEvents(LaunchCo
     const PathQueue q = sel ? ctx.queue[1] : ctx.queue[0];
     const PathQueue next = sel ? ctx.queue[0] : ctx.queue[1];
     const uint32_t seg = blockIdx.x % kSegments;
+    const uint32_t maxEvents = ctx.events.maxEvents;
     for (uint32_t first = blockIdx.x * blockDim.x; first < count; first += gridDim.x * blockDim.x) {
         uint32_t g = first + threadIdx.x;
         bool valid = g < count;
-        uint32_t i = 0, events = 0;
+        uint32_t i = 0, events = 0, traced = 0;
         PathState path;
         ShadeOutcome outcome;
         outcome.alive = false;
@@ -438,17 +440,19 @@ __global__ void __launch_bounds__(512) WPT_SHADE_OCCUPANCY kShadeEvents(LaunchCo
             i = segmentSlot(v, q.segmentCapacity, g);
             path = loadPath(q, i);
             if (path.active) {
-                EventOcclusion occ{ctx.events, i, 0u};
+                EventOcclusion occ{ctx.events, i, 0u, 0u};
                 shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
                 events = occ.count;
+                traced = occ.traced;
             } else {
                 valid = false;  // ended by an event of the previous bounce
             }
             *eventWord(ctx.events, kEvCount, i) = events;
         }
         bool alive = valid && outcome.alive;
-        uint32_t a, s;
-        blockAppend(alive, false, &ctx.counters->count[sel ^ 1u][seg], &ctx.counters->shadowCount[seg], sh, a, s);
+        uint32_t a, at;
+        blockAppendEvents(alive, traced, maxEvents, &ctx.counters->count[sel ^ 1u][seg],
+            &ctx.counters->shadowCount[seg], sh, a, at);
         uint32_t slot = seg * next.segmentCapacity + a;
         if (alive) storePath(next, slot, path);
         if (events) {
@@ -460,45 +464,60 @@ __global__ void __launch_bounds__(512) WPT_SHADE_OCCUPANCY kShadeEvents(LaunchCo
         } else if (valid && !alive) {
             finishPath(ctx, path.pixel, outcome.isDebug ? outcome.debugColor : path.radiance);
         }
+        // The wave's event rays, grouped by event index: rays of one kind
+        // (the area-light sample, the sky sample, ...) sit side by side.
+        const size_t rayBase = static_cast<size_t>(seg) * ctx.events.raySegment;
+        for (uint32_t e = 0; e < maxEvents; ++e) {
+            bool has = (traced >> e) & 1u;
+            unsigned long long m = __ballot(has);
+            if (has) {
+                uint32_t k = at + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+                ctx.events.raySlot[rayBase + k] = i;
+                ctx.events.rayEvent[rayBase + k] = static_cast<uint8_t>(e);
+            }
+            at += static_cast<uint32_t>(__popcll(m));
+        }
     }
 }
 
-// The events of one shaded path, in order: each traced ray is tested, the
-// visible contributions are added to the radiance the path had before its
-// first event, and the result goes to the continuing path or, when the path
-// ended (or an escaping event ends it), to its pixel.
+// The bounce's event rays, one per thread: a blocked ray marks its event.
 template <bool Full>
-__device__ __forceinline__ uint32_t resolveEvents(const LaunchContext& ctx, const SegmentView& v, uint32_t sel,
-    uint32_t g) {
+__global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadowEvents(LaunchContext ctx) {
+    SegmentView v = loadSegments(ctx.counters->shadowCount);
+    uint32_t count = v.start[kSegments];
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x) {
+        uint32_t k = segmentSlot(v, ctx.events.raySegment, g);
+        uint32_t i = ctx.events.raySlot[k], e = ctx.events.rayEvent[k];
+        LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+        if (traceOccluded<Full>(ctx.scene, eventRay(ctx.events, e, i),
+                __uint_as_float(*eventItemWord(ctx.events, e, kEiTmax, i)), stack)) {
+            uint32_t* flags = eventItemWord(ctx.events, e, kEiFlags, i);
+            *flags |= kEventBlocked;
+        }
+    }
+    if (gid == 0 && count) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(count));
+}
+
+// The events of one shaded path, in order: the contributions whose ray was
+// not blocked are added to the radiance the path had before its first
+// event, and the result goes to the continuing path or, when the path ended
+// (or an escaping event ends it), to its pixel.
+__device__ __forceinline__ void resolveEvents(const LaunchContext& ctx, const SegmentView& v, uint32_t sel, uint32_t g) {
     const PathQueue& q = ctx.queue[sel];
     const PathQueue& next = ctx.queue[sel ^ 1u];
     uint32_t i = segmentSlot(v, q.segmentCapacity, g);
     uint32_t n = *eventWord(ctx.events, kEvCount, i);
-    if (n == 0) return 0;
+    if (n == 0) return;
     if (n > ctx.events.maxEvents) n = ctx.events.maxEvents;
-    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
     Vec3 r(__uint_as_float(*eventWord(ctx.events, kEvRx, i)), __uint_as_float(*eventWord(ctx.events, kEvRy, i)),
         __uint_as_float(*eventWord(ctx.events, kEvRz, i)));
     bool ends = false;
-    uint32_t traced = 0;
     for (uint32_t e = 0; e < n; ++e) {
         uint32_t flags = *eventItemWord(ctx.events, e, kEiFlags, i);
-        bool visible = true;
-        if (flags & kEventTraced) {
-            Ray ray(Vec3(__uint_as_float(*eventItemWord(ctx.events, e, kEiOx, i)),
-                        __uint_as_float(*eventItemWord(ctx.events, e, kEiOy, i)),
-                        __uint_as_float(*eventItemWord(ctx.events, e, kEiOz, i))),
-                Vec3(__uint_as_float(*eventItemWord(ctx.events, e, kEiDx, i)),
-                    __uint_as_float(*eventItemWord(ctx.events, e, kEiDy, i)),
-                    __uint_as_float(*eventItemWord(ctx.events, e, kEiDz, i))));
-            visible = !traceOccluded<Full>(ctx.scene, ray, __uint_as_float(*eventItemWord(ctx.events, e, kEiTmax, i)),
-                stack);
-            ++traced;
-        }
-        if (visible) {
-            r += Vec3(__uint_as_float(*eventItemWord(ctx.events, e, kEiCx, i)),
-                __uint_as_float(*eventItemWord(ctx.events, e, kEiCy, i)),
-                __uint_as_float(*eventItemWord(ctx.events, e, kEiCz, i)));
+        if (!(flags & kEventBlocked)) {
+            r += eventContribution(ctx.events, e, i);
             if (flags & kEventTerminates) ends = true;
         }
     }
@@ -516,18 +535,13 @@ __device__ __forceinline__ uint32_t resolveEvents(const LaunchContext& ctx, cons
             atomicAdd(&ctx.counters->closestQueries, ~0ull);
         }
     }
-    return traced;
 }
 
-template <bool Full>
-__global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadowEvents(LaunchContext ctx, uint32_t sel) {
+__global__ void __launch_bounds__(512) kResolveEvents(LaunchContext ctx, uint32_t sel) {
     SegmentView v = loadSegments(ctx.counters->count[sel]);
     uint32_t count = v.start[kSegments];
-    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long traced = 0;
-    for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x) traced += resolveEvents<Full>(ctx, v, sel, g);
-    for (int offset = 32; offset > 0; offset >>= 1) traced += __shfl_xor(traced, offset, 64);
-    if (__lane_id() == 0 && traced) atomicAdd(&ctx.counters->shadowQueries, traced);
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < count; g += gridDim.x * blockDim.x)
+        resolveEvents(ctx, v, sel, g);
 }
 
 // One shadow request: the contribution joins the continuing path's
@@ -660,7 +674,7 @@ __global__ void __launch_bounds__(512) WPT_PERSISTENT_OCCUPANCY(Full && Events) 
                             }
                             ++eventNext;
                         } else {
-                            EventOcclusion occ{ctx.events, gid, 0u};
+                            EventOcclusion occ{ctx.events, gid, 0u, 0u};
                             shadePath<Full>(ctx.scene, path, h, ctx.shade, occ, outcome);
                             eventCount = occ.count < ctx.events.maxEvents ? occ.count : ctx.events.maxEvents;
                         }
@@ -811,7 +825,9 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
                     hipLaunchKernelGGL((kShade<false, false>), grid, block, 0, stream, ctx, sel);
             });
         if (e == hipSuccess && eventShading)
-            e = timer(5, [&] { hipLaunchKernelGGL(kShadowEvents<true>, grid, block, lds, stream, ctx, sel); });
+            e = timer(5, [&] { hipLaunchKernelGGL(kShadowEvents<true>, grid, block, lds, stream, ctx); });
+        if (e == hipSuccess && eventShading)
+            e = timer(5, [&] { hipLaunchKernelGGL(kResolveEvents, grid, block, 0, stream, ctx, sel); });
         if (e == hipSuccess && !inlineShadows && ctx.scene.lightCount != 0 && !ctx.scene.ignoreDirectLightOcclusion)
             e = timer(5, [&] {
                 if (full)

@@ -230,7 +230,8 @@ BATCH_STAGED, BATCH_REGENERATE = 0, 1
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("block_size", C.c_uint32), ("blocks", C.c_uint32), ("scheduler", C.c_uint32)]
+    _fields_ = [("block_size", C.c_uint32), ("blocks", C.c_uint32), ("scheduler", C.c_uint32),
+                ("pass_paths", C.c_uint32)]
 
 
 class BatchStats(C.Structure):
@@ -772,11 +773,11 @@ class Batch:
     launch (BATCH_REGENERATE).  The scenes must outlive the batch."""
 
     def __init__(self, scenes: Sequence["Scene"], block_size: int = 256, blocks: int = 0,
-                 scheduler: int = BATCH_STAGED):
+                 scheduler: int = BATCH_STAGED, pass_paths: int = 0):
         self._scenes = list(scenes)
         handles = (C.c_void_p * len(self._scenes))(*[sc.handle for sc in self._scenes])
         self._h = C.c_void_p()
-        cfg = BatchConfig(block_size, blocks, scheduler)
+        cfg = BatchConfig(block_size, blocks, scheduler, pass_paths)
         check(library().wpt_batch_create(handles, len(self._scenes), C.byref(cfg), C.byref(self._h)),
               "wpt_batch_create")
 
