@@ -80,6 +80,35 @@ WPT_HD Vec3 gatherCaustics(const PhotonGridView& g, const Vec3& position, const 
     uint32_t indices[kNearest];
     uint32_t count = 0, farthest = 0;
     float farthestSquared = 0.0f;
+    // Once the slots are full, the largest distance of each group of eight
+    // slots and its first slot (in registers: constant indices only).  The
+    // farthest slot is then the first group maximum's slot, the slot a
+    // rescan of all 64 for the first strictly larger distance finds, and a
+    // replacement reads only its own group.
+    constexpr uint32_t kGroups = 8, kGroupSize = kNearest / kGroups;
+    float groupMax[kGroups];
+    uint32_t groupArg[kGroups];
+    auto scanGroup = [&](uint32_t group, float& m, uint32_t& a) {
+        a = group * kGroupSize;
+        m = distances[a];
+        for (uint32_t j = 1; j < kGroupSize; ++j) {
+            float x = distances[group * kGroupSize + j];
+            if (x > m) {
+                m = x;
+                a = group * kGroupSize + j;
+            }
+        }
+    };
+    auto firstMaximum = [&] {
+        farthestSquared = groupMax[0];
+        farthest = groupArg[0];
+#pragma unroll
+        for (uint32_t k = 1; k < kGroups; ++k)
+            if (groupMax[k] > farthestSquared) {
+                farthestSquared = groupMax[k];
+                farthest = groupArg[k];
+            }
+    };
     // The search cube clipped to the occupied cells: the cells left out are
     // empty, so the photons visited, and their order, are unchanged.
     int lo[3], hi[3];
@@ -126,30 +155,51 @@ WPT_HD Vec3 gatherCaustics(const PhotonGridView& g, const Vec3& position, const 
             uint32_t row = (static_cast<uint32_t>(z) * g.resolution + static_cast<uint32_t>(y)) * g.resolution;
             uint32_t begin = g.cellStart[row + static_cast<uint32_t>(first)];
             uint32_t end = g.cellStart[row + static_cast<uint32_t>(last) + 1u];
-            for (uint32_t i = begin; i < end; ++i) {
-                const Photon& photon = g.photons[i];
-                Vec3 d = photon.position - position;
-                float distanceSquared = dot(d, d);
-                float cosine = dot(normal, photon.direction);
-                if (distanceSquared > searchRadiusSquared || cosine <= 0.0f) continue;
-                if (count < kNearest) {
-                    distances[count] = distanceSquared;
-                    indices[count] = i;
-                    if (distanceSquared > farthestSquared) {
-                        farthestSquared = distanceSquared;
-                        farthest = count;
-                    }
-                    ++count;
-                } else if (distanceSquared < farthestSquared) {
-                    distances[farthest] = distanceSquared;
-                    indices[farthest] = i;
-                    farthest = 0;
-                    farthestSquared = distances[0];
-                    for (uint32_t k = 1; k < kNearest; ++k)
-                        if (distances[k] > farthestSquared) {
-                            farthestSquared = distances[k];
-                            farthest = k;
+            // Photons in groups of kBatch: the group's loads are issued
+            // together (indices past the run repeat its last photon and are
+            // not visited), then the photons are visited in order.
+            constexpr uint32_t kBatch = 8;
+            for (uint32_t i = begin; i < end; i += kBatch) {
+                Vec3 at[kBatch], from[kBatch];
+#pragma unroll
+                for (uint32_t k = 0; k < kBatch; ++k) {
+                    const Photon& photon = g.photons[i + k < end ? i + k : end - 1u];
+                    at[k] = photon.position;
+                    from[k] = photon.direction;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kBatch; ++k) {
+                    if (i + k >= end) break;
+                    Vec3 d = at[k] - position;
+                    float distanceSquared = dot(d, d);
+                    float cosine = dot(normal, from[k]);
+                    if (distanceSquared > searchRadiusSquared || cosine <= 0.0f) continue;
+                    if (count < kNearest) {
+                        distances[count] = distanceSquared;
+                        indices[count] = i + k;
+                        if (distanceSquared > farthestSquared) {
+                            farthestSquared = distanceSquared;
+                            farthest = count;
                         }
+                        if (++count == kNearest) {
+#pragma unroll
+                            for (uint32_t q = 0; q < kGroups; ++q) scanGroup(q, groupMax[q], groupArg[q]);
+                        }
+                    } else if (distanceSquared < farthestSquared) {
+                        distances[farthest] = distanceSquared;
+                        indices[farthest] = i + k;
+                        uint32_t group = farthest / kGroupSize;
+                        float m;
+                        uint32_t a;
+                        scanGroup(group, m, a);
+#pragma unroll
+                        for (uint32_t q = 0; q < kGroups; ++q)
+                            if (q == group) {
+                                groupMax[q] = m;
+                                groupArg[q] = a;
+                            }
+                        firstMaximum();
+                    }
                 }
             }
         }
