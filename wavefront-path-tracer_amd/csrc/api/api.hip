@@ -727,18 +727,21 @@ uint32_t vertexEventsOf(const DevScene& s) {
 
 uint32_t eventWordsPerColumn(uint32_t maxEvents) { return kEventHeaderFields + maxEvents * kEventItemFields; }
 
-// The staged event ray list: a queue segment's paths append at most
-// maxEvents rays each to the same segment of the list.
+// The staged event ray and gather lists: a queue segment's paths append at
+// most maxEvents rays and one gather each to the same segment of the lists.
 hipError_t allocEventRays(DeviceArena& arena, uint64_t segment, uint32_t maxEvents, EventBuffer* events) {
     uint64_t raySegment = segment * maxEvents;
     if (raySegment >= (1ull << 32) / kSegments) return hipErrorInvalidValue;
-    size_t entries = static_cast<size_t>(raySegment) * kSegments;
+    size_t rays = static_cast<size_t>(raySegment) * kSegments, gathers = static_cast<size_t>(segment) * kSegments;
     void* p = nullptr;
-    hipError_t e = arena.alloc((sizeof(uint32_t) + sizeof(uint8_t)) * entries, &p);
+    hipError_t e = arena.alloc((sizeof(uint32_t) + sizeof(uint8_t)) * (rays + gathers), &p);
     if (e != hipSuccess) return e;
     events->raySlot = static_cast<uint32_t*>(p);
-    events->rayEvent = reinterpret_cast<uint8_t*>(events->raySlot + entries);
+    events->gatherSlot = events->raySlot + rays;
+    events->rayEvent = reinterpret_cast<uint8_t*>(events->gatherSlot + gathers);
+    events->gatherEvent = events->rayEvent + rays;
     events->raySegment = static_cast<uint32_t>(raySegment);
+    events->gatherSegment = static_cast<uint32_t>(segment);
     return hipSuccess;
 }
 constexpr uint32_t kNoAllocScratchWords = 1u + kNoAllocMaxItems / 256u;

@@ -48,6 +48,13 @@ constexpr int persistentWavesPerEu(bool full) {
 #else
 #define WPT_SHADE_OCCUPANCY
 #endif
+// Occupancy target of the deferred caustic gather kernel, 0: compiler's
+// choice.
+#if defined(WPT_GATHER_WAVES_PER_EU) && WPT_GATHER_WAVES_PER_EU > 0
+#define WPT_GATHER_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WPT_GATHER_WAVES_PER_EU)))
+#else
+#define WPT_GATHER_OCCUPANCY
+#endif
 
 __device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -161,8 +168,8 @@ WPT_HD uint32_t primarySegmentCount(uint32_t total, uint32_t B, uint32_t s) {
 // segment.
 constexpr uint32_t kMaxWaves = 16;  // 1024-lane workgroups
 struct AppendShared {
-    uint32_t wave[kMaxWaves][2];
-    uint32_t base[2];
+    uint32_t wave[kMaxWaves][3];
+    uint32_t base[3];
 };
 
 __device__ __forceinline__ void blockAppend(bool a0, bool a1, uint32_t* counter0, uint32_t* counter1, AppendShared& sh,
@@ -198,41 +205,47 @@ __device__ __forceinline__ void blockAppend(bool a0, bool a1, uint32_t* counter0
 }
 
 // Workgroup-aggregated append of continuing paths (kind 0, one item per
-// `alive` lane) and of traced events (kind 1, bit e of `mask` for event e <
-// `events`): one atomic per kind per workgroup.  Every thread of the
-// workgroup must call it.  index0 is an alive lane's index within its
-// segment; the wave's event entries start at waveBase1, grouped by event
-// index and in lane order within a group (what the lanes' own ballots over
-// e < events reproduce).
-__device__ __forceinline__ void blockAppendEvents(bool alive, uint32_t mask, uint32_t events, uint32_t* counter0,
-    uint32_t* counter1, AppendShared& sh, uint32_t& index0, uint32_t& waveBase1) {
-    unsigned long long m0 = __ballot(alive);
+// `alive` lane), traced events (kind 1, bit e of `mask` for event e <
+// `events`) and deferred gathers (kind 2, one item per `gather` lane): one
+// atomic per kind per workgroup.  Every thread of the workgroup must call
+// it.  index0 and index2 are a lane's indices within its segments; the
+// wave's event entries start at waveBase1, grouped by event index and in
+// lane order within a group (what the lanes' own ballots over e < events
+// reproduce).
+__device__ __forceinline__ void blockAppendEvents(bool alive, uint32_t mask, uint32_t events, bool gather,
+    uint32_t* counter0, uint32_t* counter1, uint32_t* counter2, AppendShared& sh, uint32_t& index0,
+    uint32_t& waveBase1, uint32_t& index2) {
+    unsigned long long m0 = __ballot(alive), m2 = __ballot(gather);
     uint32_t r0 = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m0 >> 32),
         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m0), 0u));
+    uint32_t r2 = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m2 >> 32),
+        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m2), 0u));
     uint32_t total1 = 0;
     for (uint32_t e = 0; e < events; ++e) total1 += static_cast<uint32_t>(__popcll(__ballot((mask >> e) & 1u)));
     uint32_t w = threadIdx.x >> 6;
     if (__lane_id() == 0) {
         sh.wave[w][0] = static_cast<uint32_t>(__popcll(m0));
         sh.wave[w][1] = total1;
+        sh.wave[w][2] = static_cast<uint32_t>(__popcll(m2));
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t waves = blockDim.x >> 6;
-        uint32_t t0 = 0, t1 = 0;
-        for (uint32_t k = 0; k < waves; ++k) {
-            uint32_t c0 = sh.wave[k][0], c1 = sh.wave[k][1];
-            sh.wave[k][0] = t0;
-            sh.wave[k][1] = t1;
-            t0 += c0;
-            t1 += c1;
-        }
-        sh.base[0] = t0 ? atomicAdd(counter0, t0) : 0u;
-        sh.base[1] = t1 ? atomicAdd(counter1, t1) : 0u;
+        uint32_t t[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < waves; ++k)
+            for (uint32_t c = 0; c < 3; ++c) {
+                uint32_t n = sh.wave[k][c];
+                sh.wave[k][c] = t[c];
+                t[c] += n;
+            }
+        sh.base[0] = t[0] ? atomicAdd(counter0, t[0]) : 0u;
+        sh.base[1] = t[1] ? atomicAdd(counter1, t[1]) : 0u;
+        sh.base[2] = t[2] ? atomicAdd(counter2, t[2]) : 0u;
     }
     __syncthreads();
     index0 = sh.base[0] + sh.wave[w][0] + r0;
     waveBase1 = sh.base[1] + sh.wave[w][1];
+    index2 = sh.base[2] + sh.wave[w][2] + r2;
     __syncthreads();
 }
 

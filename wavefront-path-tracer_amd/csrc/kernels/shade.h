@@ -52,12 +52,16 @@ WPT_HD float rand01(PathState& p, uint32_t strategy) { return rngNextFloat(p.rng
 
 // The indirect clamp of addPathRadiance: it depends on the contribution and
 // the path's depth only, so a deferred addition can apply it up front.
+// Scales the contribution down to mean magnitude `clamp` (> 0) if above it.
+WPT_HD Vec3 clampMagnitude(Vec3 contribution, float clamp) {
+    float magnitude = fabsf(contribution.x) / 3.0f + fabsf(contribution.y) / 3.0f + fabsf(contribution.z) / 3.0f;
+    if (magnitude > clamp) contribution *= clamp / magnitude;
+    return contribution;
+}
+
 WPT_HD Vec3 clampContribution(const PathState& path, Vec3 contribution, float clamp, bool emissionOrBackground) {
     bool indirect = path.depth > (emissionOrBackground ? 1u : 0u);
-    if (indirect && clamp > 0.0f) {
-        float magnitude = fabsf(contribution.x) / 3.0f + fabsf(contribution.y) / 3.0f + fabsf(contribution.z) / 3.0f;
-        if (magnitude > clamp) contribution *= clamp / magnitude;
-    }
+    if (indirect && clamp > 0.0f) contribution = clampMagnitude(contribution, clamp);
     return contribution;
 }
 
@@ -273,6 +277,7 @@ struct ShadowRequest {
 struct DeferOcclusion {
     static constexpr bool kInline = false;
     static constexpr bool kEvents = false;
+    static constexpr bool kDeferGather = false;
     WPT_HD bool blocked(const DevScene&, const Ray&, float) { return false; }
     WPT_HD uint32_t record(const Ray&, float, const Vec3&, bool) { return 0; }
     WPT_HD void add(PathState& path, const Vec3& c) { path.radiance += c; }
@@ -291,6 +296,7 @@ struct DeferOcclusion {
 // the environment).
 enum EventField : uint32_t {
     kEvCount, kEvTarget, kEvPixel, kEvRx, kEvRy, kEvRz,  // per slot: events, continuation, pixel, radiance before them
+    kEvGx, kEvGy, kEvGz,                                 // the throughput at a deferred caustic gather
     kEventHeaderFields,
 };
 enum EventItemField : uint32_t {
@@ -307,10 +313,14 @@ struct EventBuffer {
     uint32_t maxEvents;
     // Staged shading: the traced events of a bounce, appended per workgroup
     // segment of raySegment entries; entry k names event rayEvent[k] of
-    // slot raySlot[k].
+    // slot raySlot[k].  The deferred caustic gathers likewise, at most one
+    // per slot, in segments of gatherSegment entries.
     uint32_t* raySlot;
     uint8_t* rayEvent;
     uint32_t raySegment;
+    uint32_t* gatherSlot;
+    uint8_t* gatherEvent;
+    uint32_t gatherSegment;
 };
 
 WPT_HD uint32_t* eventWord(const EventBuffer& b, uint32_t field, uint32_t slot) {
@@ -323,6 +333,7 @@ WPT_HD uint32_t* eventItemWord(const EventBuffer& b, uint32_t event, uint32_t fi
 struct EventOcclusion {
     static constexpr bool kInline = false;
     static constexpr bool kEvents = true;
+    static constexpr bool kDeferGather = false;
     EventBuffer buffer;
     uint32_t slot;
     uint32_t count;
@@ -359,6 +370,37 @@ struct EventOcclusion {
     }
 };
 
+// Staged event shading that also leaves the caustic gather to a pass of
+// its own: the gather becomes an event whose contribution that pass fills
+// in (from the query point, normal, albedo and clamp stored in the event's
+// ray, contribution and distance fields and the throughput in the slot's
+// header), so the shading kernel carries no gather code or heap.
+struct StagedEventOcclusion : EventOcclusion {
+    static constexpr bool kDeferGather = true;
+    uint32_t gather;  // event index + 1 of the vertex's gather, 0: none
+    WPT_HD void recordGather(const PathState& path, const Vec3& position, const Vec3& normal, const Vec3& albedo,
+        float clamp) {
+        uint32_t e = count++;
+        if (e >= buffer.maxEvents) return;
+        *eventItemWord(buffer, e, kEiOx, slot) = bitsOf(position.x);
+        *eventItemWord(buffer, e, kEiOy, slot) = bitsOf(position.y);
+        *eventItemWord(buffer, e, kEiOz, slot) = bitsOf(position.z);
+        *eventItemWord(buffer, e, kEiDx, slot) = bitsOf(normal.x);
+        *eventItemWord(buffer, e, kEiDy, slot) = bitsOf(normal.y);
+        *eventItemWord(buffer, e, kEiDz, slot) = bitsOf(normal.z);
+        // clampContribution's test, made here where the depth is known.
+        *eventItemWord(buffer, e, kEiTmax, slot) = bitsOf(path.depth > 0u && clamp > 0.0f ? clamp : 0.0f);
+        *eventItemWord(buffer, e, kEiCx, slot) = bitsOf(albedo.x);
+        *eventItemWord(buffer, e, kEiCy, slot) = bitsOf(albedo.y);
+        *eventItemWord(buffer, e, kEiCz, slot) = bitsOf(albedo.z);
+        *eventItemWord(buffer, e, kEiFlags, slot) = 0u;
+        *eventWord(buffer, kEvGx, slot) = bitsOf(path.throughput.x);
+        *eventWord(buffer, kEvGy, slot) = bitsOf(path.throughput.y);
+        *eventWord(buffer, kEvGz, slot) = bitsOf(path.throughput.z);
+        gather = e + 1u;
+    }
+};
+
 // A vertex's radiance addition: at once, or (event shading) as the next
 // event, traced unless `traced` is false.  With inline or deferred
 // occlusion the caller has already tested the ray.
@@ -384,6 +426,7 @@ template <class Stack>
 struct InlineOcclusion {
     static constexpr bool kInline = true;
     static constexpr bool kEvents = false;
+    static constexpr bool kDeferGather = false;
     Stack& stack;
     uint32_t queries;
     WPT_CALL bool blocked(const DevScene& s, const Ray& ray, float tmax) {
@@ -914,9 +957,13 @@ WPT_HD bool shadeBounce(const DevScene& s, PathState& path, const HitRecord& h, 
         sampleDirectPunctualLight(s, path, hit, shadingNormal, bsdf, sp, occ);
         sampleDirectEnvironment(s, path, hit, shadingNormal, bsdf, sp, occ);
         if (Full && sp.photons) {  // device_path_shading.cuh:99-102
-            Vec3 caustic = gatherCaustics(*sp.photons, hit.position, shadingNormal);
-            emitRadiance(occ, path, hadamard(path.throughput, hadamard(albedo, caustic)), sp.clamp, path.ray, 0.0f,
-                false);
+            if constexpr (Occluder::kDeferGather) {
+                occ.recordGather(path, hit.position, shadingNormal, albedo, sp.clamp);
+            } else {
+                Vec3 caustic = gatherCaustics(*sp.photons, hit.position, shadingNormal);
+                emitRadiance(occ, path, hadamard(path.throughput, hadamard(albedo, caustic)), sp.clamp, path.ray,
+                    0.0f, false);
+            }
         }
         direction = continueDiffuse<Full>(s, path, hit, shadingNormal, albedo, sp.rngStrategy);
         break;

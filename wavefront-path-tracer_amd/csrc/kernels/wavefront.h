@@ -93,7 +93,8 @@ struct PassPiece {
 // Device counters shared by the launches of one renderer.
 struct WavefrontCounters {
     uint32_t count[2][kSegments];    // staged mode: items per segment of queue 0 / 1
-    uint32_t shadowCount[kSegments];  // shadow requests per segment of the current bounce
+    uint32_t shadowCount[kSegments];  // shadow requests (event rays) per segment of the current bounce
+    uint32_t gatherCount[kSegments];  // deferred caustic gathers per segment of the current bounce
     uint32_t pcount[3][kSegments];   // persistent kernel: queues rotate through three count sets
     uint32_t sampleIndex;
     uint32_t pad;

@@ -287,6 +287,7 @@ __global__ void kReset(LaunchContext ctx, uint32_t nextSel) {
     if (s < kSegments) {
         ctx.counters->count[nextSel][s] = 0;
         ctx.counters->shadowCount[s] = 0;
+        ctx.counters->gatherCount[s] = 0;
     }
 }
 
@@ -431,7 +432,7 @@ __global__ void __launch_bounds__(512) WPT_SHADE_OCCUPANCY kShadeEvents(LaunchCo
     for (uint32_t first = blockIdx.x * blockDim.x; first < count; first += gridDim.x * blockDim.x) {
         uint32_t g = first + threadIdx.x;
         bool valid = g < count;
-        uint32_t i = 0, events = 0, traced = 0;
+        uint32_t i = 0, events = 0, traced = 0, gather = 0;
         PathState path;
         ShadeOutcome outcome;
         outcome.alive = false;
@@ -440,21 +441,27 @@ __global__ void __launch_bounds__(512) WPT_SHADE_OCCUPANCY kShadeEvents(LaunchCo
             i = segmentSlot(v, q.segmentCapacity, g);
             path = loadPath(q, i);
             if (path.active) {
-                EventOcclusion occ{ctx.events, i, 0u, 0u};
+                StagedEventOcclusion occ{{ctx.events, i, 0u, 0u}, 0u};
                 shadePath<Full>(ctx.scene, path, loadHit(ctx.hits, i), ctx.shade, occ, outcome);
                 events = occ.count;
                 traced = occ.traced;
+                gather = occ.gather;
             } else {
                 valid = false;  // ended by an event of the previous bounce
             }
             *eventWord(ctx.events, kEvCount, i) = events;
         }
         bool alive = valid && outcome.alive;
-        uint32_t a, at;
-        blockAppendEvents(alive, traced, maxEvents, &ctx.counters->count[sel ^ 1u][seg],
-            &ctx.counters->shadowCount[seg], sh, a, at);
+        uint32_t a, at, gi;
+        blockAppendEvents(alive, traced, maxEvents, gather != 0u, &ctx.counters->count[sel ^ 1u][seg],
+            &ctx.counters->shadowCount[seg], &ctx.counters->gatherCount[seg], sh, a, at, gi);
         uint32_t slot = seg * next.segmentCapacity + a;
         if (alive) storePath(next, slot, path);
+        if (gather) {
+            size_t k = static_cast<size_t>(seg) * ctx.events.gatherSegment + gi;
+            ctx.events.gatherSlot[k] = i;
+            ctx.events.gatherEvent[k] = static_cast<uint8_t>(gather - 1u);
+        }
         if (events) {
             *eventWord(ctx.events, kEvTarget, i) = alive ? slot : kShadowTerminated;
             *eventWord(ctx.events, kEvPixel, i) = path.pixel;
@@ -478,6 +485,30 @@ __global__ void __launch_bounds__(512) WPT_SHADE_OCCUPANCY kShadeEvents(LaunchCo
             }
             at += static_cast<uint32_t>(__popcll(m));
         }
+    }
+}
+
+// The bounce's deferred caustic gathers, one per thread: each fills in its
+// event's contribution, throughput x (albedo x irradiance), clamped as
+// shading clamps it (device_path_shading.cuh:99-102).
+__global__ void __launch_bounds__(512) WPT_GATHER_OCCUPANCY kGatherEvents(LaunchContext ctx) {
+    SegmentView v = loadSegments(ctx.counters->gatherCount);
+    uint32_t count = v.start[kSegments];
+    const EventBuffer& b = ctx.events;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < count; g += gridDim.x * blockDim.x) {
+        uint32_t k = segmentSlot(v, b.gatherSegment, g);
+        uint32_t i = b.gatherSlot[k], e = b.gatherEvent[k];
+        Ray at = eventRay(b, e, i);  // query point and shading normal
+        Vec3 albedo = eventContribution(b, e, i);
+        float clamp = __uint_as_float(*eventItemWord(b, e, kEiTmax, i));
+        Vec3 throughput(__uint_as_float(*eventWord(b, kEvGx, i)), __uint_as_float(*eventWord(b, kEvGy, i)),
+            __uint_as_float(*eventWord(b, kEvGz, i)));
+        Vec3 caustic = gatherCaustics(*ctx.shade.photons, at.origin, at.direction);
+        Vec3 c = hadamard(throughput, hadamard(albedo, caustic));
+        if (clamp > 0.0f) c = clampMagnitude(c, clamp);
+        *eventItemWord(b, e, kEiCx, i) = __float_as_uint(c.x);
+        *eventItemWord(b, e, kEiCy, i) = __float_as_uint(c.y);
+        *eventItemWord(b, e, kEiCz, i) = __float_as_uint(c.z);
     }
 }
 
@@ -824,6 +855,8 @@ hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& ti
                 else
                     hipLaunchKernelGGL((kShade<false, false>), grid, block, 0, stream, ctx, sel);
             });
+        if (e == hipSuccess && eventShading && ctx.shade.photons)
+            e = timer(3, [&] { hipLaunchKernelGGL(kGatherEvents, grid, block, 0, stream, ctx); });
         if (e == hipSuccess && eventShading)
             e = timer(5, [&] { hipLaunchKernelGGL(kShadowEvents<true>, grid, block, lds, stream, ctx); });
         if (e == hipSuccess && eventShading)
