@@ -1188,7 +1188,7 @@ WPT_API int wpt_renderer_launch_info(const wpt_renderer* r, wpt_launch_info* inf
     info->block_size = c.block_size;
     hipFuncAttributes attr{};
     int perCu = 0;
-    size_t lds = stackLdsBytes(c.block_size);
+    size_t lds = c.mode == WPT_MODE_GRAPH ? traversalLdsBytes(c.block_size) : stackLdsBytes(c.block_size);
     if (c.mode == WPT_MODE_REGENERATE) {
         WPT_HIP(batchAttributes(full, &attr));
         WPT_HIP(batchOccupancy(c.block_size, full, &perCu));

@@ -153,6 +153,24 @@ struct LaunchContext {
 // LDS bytes of the traversal stack for a workgroup of `blockSize` lanes.
 inline size_t stackLdsBytes(uint32_t blockSize) { return sizeof(uint2) * kLdsStackEntries * blockSize; }
 
+// Most waves per SIMD of the staged traversal kernels (0: as many as their
+// registers and LDS stacks allow).  More waves than this thrash the caches
+// the node and triangle fetches hit (a fifth wave cost kTrace and the event
+// shadow rays 10-30%).  The cap is held by padding each workgroup's LDS
+// request, given the CU's 160 KiB.
+#ifndef WPT_TRACE_MAX_WAVES
+#define WPT_TRACE_MAX_WAVES 4
+#endif
+constexpr size_t kLdsPerCu = 160u * 1024u;
+inline size_t traversalLdsBytes(uint32_t blockSize) {
+    size_t stack = stackLdsBytes(blockSize);
+    if (WPT_TRACE_MAX_WAVES <= 0) return stack;
+    size_t waves = blockSize / 64u;                    // per workgroup
+    size_t allowed = 4u * WPT_TRACE_MAX_WAVES / waves;  // workgroups per CU
+    if (allowed == 0 || (kLdsPerCu / stack) <= allowed) return stack;
+    return kLdsPerCu / allowed;
+}
+
 // Staged wavefront: records primary + maxDepth x (reset, trace, shade,
 // shadow) + resolve + advance for one pass of the current tile (its
 // `samples` samples) on `stream` (captured once into a hipGraph by the

@@ -291,6 +291,18 @@ __global__ void kReset(LaunchContext ctx, uint32_t nextSel) {
     }
 }
 
+// Lanes of a wave that wait before they start new rays together (1: each
+// starts its next ray as soon as the last one ends; 64: the wave's rays
+// start and end together).
+#ifndef WPT_TRACE_REFILL
+#define WPT_TRACE_REFILL 16
+#endif
+constexpr int kTraceRefill = WPT_TRACE_REFILL;
+#ifndef WPT_INTERLEAVE_LEAN
+#define WPT_INTERLEAVE_LEAN 0
+#endif
+constexpr bool kInterleaveLean = WPT_INTERLEAVE_LEAN != 0;
+
 // Rays in grid stride: the grid (at most a few times the resident
 // workgroups) bounds the traversal spill area, and the dispatcher balances
 // the workgroups.
@@ -304,18 +316,60 @@ __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kTrace(LaunchContext 
         if (ctx.profileQueues && bounce < 64) ctx.counters->queueCounts[bounce] = count;
     }
     const PathQueue& q = ctx.queue[sel];
-    for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x) {
-        LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
-        uint32_t i = segmentSlot(v, q.segmentCapacity, g);
-        Ray ray = loadRay(q, i);
-        HitRecord h = traceClosest<Full>(ctx.scene, ray, stack);
-        ctx.hits.t[i] = h.t;
-        ctx.hits.u[i] = h.u;
-        ctx.hits.v[i] = h.v;
-        ctx.hits.kind[i] = h.kind;
-        ctx.hits.instance[i] = h.instance;
-        ctx.hits.prim[i] = h.prim;
-        ctx.hits.material[i] = h.material;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+    if constexpr (!kInterleaveLean && !Full) {
+        for (uint32_t g = gid; g < count; g += stride) {
+            uint32_t i = segmentSlot(v, q.segmentCapacity, g);
+            HitRecord h = traceClosest<Full>(ctx.scene, loadRay(q, i), stack);
+            ctx.hits.t[i] = h.t;
+            ctx.hits.u[i] = h.u;
+            ctx.hits.v[i] = h.v;
+            ctx.hits.kind[i] = h.kind;
+            ctx.hits.instance[i] = h.instance;
+            ctx.hits.prim[i] = h.prim;
+            ctx.hits.material[i] = h.material;
+        }
+        return;
+    }
+    // One loop over node visits, not one per ray: lanes whose rays ended
+    // store their hits and start their next rays (prefetched) while the
+    // other lanes of the wave walk on, once at least kTraceRefill lanes (or
+    // all that are left) wait, instead of idling until every ray of the
+    // wave has ended.
+    TlasWalk<Full, Query::kClosest> w;
+    Walk state = Walk::kDone;
+    bool have = false;  // the lane holds a ray
+    uint32_t g = gid, i = 0;
+    uint32_t nextSlot = g < count ? segmentSlot(v, q.segmentCapacity, g) : 0u;
+    Ray next = g < count ? loadRay(q, nextSlot) : Ray();
+    while (true) {
+        bool walking = state == Walk::kTlas;
+        unsigned long long waiting = __ballot(!walking), left = __ballot(true);
+        if (walking || (__popcll(waiting) < kTraceRefill && waiting != left)) {
+            if (walking) state = w.step(ctx.scene, stack);
+            continue;
+        }
+        if (have) {  // the ray ended: its hit
+            if (state == Walk::kAfter) w.finish(ctx.scene, stack);
+            ctx.hits.t[i] = w.best.t;
+            ctx.hits.u[i] = w.best.u;
+            ctx.hits.v[i] = w.best.v;
+            ctx.hits.kind[i] = w.best.kind;
+            ctx.hits.instance[i] = w.best.instance;
+            ctx.hits.prim[i] = w.best.prim;
+            ctx.hits.material[i] = w.best.material;
+            g += stride;
+        }
+        if (g >= count) break;
+        i = nextSlot;
+        Ray ray = next;
+        if (g + stride < count) {
+            nextSlot = segmentSlot(v, q.segmentCapacity, g + stride);
+            next = loadRay(q, nextSlot);
+        }
+        state = w.start(ctx.scene, ray, kNoHit, false, stack);
+        have = true;
     }
 }
 
@@ -518,17 +572,58 @@ __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kShadowEvents(LaunchC
     SegmentView v = loadSegments(ctx.counters->shadowCount);
     uint32_t count = v.start[kSegments];
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t g = gid; g < count; g += gridDim.x * blockDim.x) {
-        uint32_t k = segmentSlot(v, ctx.events.raySegment, g);
-        uint32_t i = ctx.events.raySlot[k], e = ctx.events.rayEvent[k];
-        LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
-        if (traceOccluded<Full>(ctx.scene, eventRay(ctx.events, e, i),
-                __uint_as_float(*eventItemWord(ctx.events, e, kEiTmax, i)), stack)) {
-            uint32_t* flags = eventItemWord(ctx.events, e, kEiFlags, i);
-            *flags |= kEventBlocked;
-        }
-    }
     if (gid == 0 && count) atomicAdd(&ctx.counters->shadowQueries, static_cast<unsigned long long>(count));
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const EventBuffer& b = ctx.events;
+    LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
+    auto entry = [&](uint32_t at, uint32_t& slot, uint32_t& event) {
+        uint32_t k = segmentSlot(v, b.raySegment, at);
+        slot = b.raySlot[k];
+        event = b.rayEvent[k];
+    };
+    // The walks of a lane's rays interleave with the other lanes' as in
+    // kTrace.  Prefetched: the next ray (its slot, event, ray and distance)
+    // and the list entry after it.
+    uint32_t i1 = 0, e1 = 0, i2 = 0, e2 = 0;
+    Ray ray1;
+    float tmax1 = 0.0f;
+    if (gid < count) {
+        entry(gid, i1, e1);
+        ray1 = eventRay(b, e1, i1);
+        tmax1 = __uint_as_float(*eventItemWord(b, e1, kEiTmax, i1));
+    }
+    if (gid + stride < count) entry(gid + stride, i2, e2);
+    TlasWalk<Full, Query::kAny> w;
+    Walk state = Walk::kDone;
+    bool have = false;  // the lane holds a ray
+    uint32_t g = gid, i = 0, e = 0;
+    while (true) {
+        bool walking = state == Walk::kTlas;
+        unsigned long long waiting = __ballot(!walking), left = __ballot(true);
+        if (walking || (__popcll(waiting) < kTraceRefill && waiting != left)) {
+            if (walking) state = w.step(ctx.scene, stack);
+            continue;
+        }
+        if (have) {  // the ray ended: a blocked ray marks its event
+            if (state == Walk::kAfter) w.finish(ctx.scene, stack);
+            if (w.best.kind != kHitNone) *eventItemWord(b, e, kEiFlags, i) |= kEventBlocked;
+            g += stride;
+        }
+        if (g >= count) break;
+        i = i1;
+        e = e1;
+        Ray ray = ray1;
+        float tmax = tmax1;
+        if (g + stride < count) {
+            i1 = i2;
+            e1 = e2;
+            ray1 = eventRay(b, e1, i1);
+            tmax1 = __uint_as_float(*eventItemWord(b, e1, kEiTmax, i1));
+            if (g + 2u * stride < count) entry(g + 2u * stride, i2, e2);
+        }
+        state = w.start(ctx.scene, ray, tmax, true, stack);
+        have = true;
+    }
 }
 
 // The events of one shaded path, in order: the contributions whose ray was
@@ -824,7 +919,7 @@ namespace {
 // One staged sample; `timer` (optional) brackets every launch with events.
 template <class Timer>
 hipError_t stagedSample(const LaunchContext& ctx, hipStream_t stream, Timer&& timer) {
-    size_t lds = stackLdsBytes(ctx.blockSize);
+    size_t lds = traversalLdsBytes(ctx.blockSize);
     dim3 grid(ctx.gridBlocks), block(ctx.blockSize);
     // The caustic gather adds radiance after the area light's, so a
     // deferred shadow ray would reorder the sum.
@@ -908,16 +1003,16 @@ hipError_t enqueueStagedSample(const LaunchContext& ctx, hipStream_t stream) {
 }
 
 hipError_t kernelOccupancy(uint32_t blockSize, bool full, Occupancy* out, bool events) {
-    size_t lds = stackLdsBytes(blockSize);
+    size_t lds = stackLdsBytes(blockSize), traversal = traversalLdsBytes(blockSize);
     int b = static_cast<int>(blockSize);
-    hipError_t e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace<true>, b, lds)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace<false>, b, lds);
+    hipError_t e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace<true>, b, traversal)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->trace, kTrace<false>, b, traversal);
     if (e == hipSuccess)
         e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade<true, true>, b, lds)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shade, kShade<false, false>, b, 0);
     if (e == hipSuccess)
-        e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<true>, b, lds)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<false>, b, lds);
+        e = full ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<true>, b, traversal)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->shadow, kShadow<false>, b, traversal);
     // The cooperative grid must fit the variant that runs.
     if (e == hipSuccess)
         e = !full  ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&out->persistent, kPersistent<false, false>, b, lds)
