@@ -154,10 +154,11 @@ struct LaunchContext {
 inline size_t stackLdsBytes(uint32_t blockSize) { return sizeof(uint2) * kLdsStackEntries * blockSize; }
 
 // Most waves per SIMD of the staged traversal kernels (0: as many as their
-// registers and LDS stacks allow).  More waves than this thrash the caches
-// the node and triangle fetches hit (a fifth wave cost kTrace and the event
-// shadow rays 10-30%).  The cap is held by padding each workgroup's LDS
-// request, given the CU's 160 KiB.
+// registers and LDS stacks allow), so that their occupancy does not drift
+// with their register count: a fifth wave never made them faster, and cost
+// the closest-hit pass 10-14% where registers were squeezed for it.  The
+// cap is held by padding each workgroup's LDS request, given the CU's
+// 160 KiB.
 #ifndef WPT_TRACE_MAX_WAVES
 #define WPT_TRACE_MAX_WAVES 4
 #endif
