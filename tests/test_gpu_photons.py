@@ -45,6 +45,24 @@ def test_caustic_render_matches_oracle(torch_cuda, oracle_mod, preset, spectral,
     assert st.closest_queries == ost.closest_queries and st.shadow_queries == ost.shadow_queries
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_dense_gathers_match_oracle(torch_cuda, oracle_mod, mode):
+    # 400k photons and a 0.1 radius (0.4 search radius) in the unit box: a
+    # gather keeps 64 of thousands of candidates, so the farthest kept
+    # photon is replaced many times per query (the device tracks it by
+    # groups of eight slots, the oracle rescans all 64).
+    host = wpt.HostScene("cornell-box")
+    scene = wpt.Scene(host.desc(), device=True, owner=host)
+    cam = host.camera(40, 30)
+    cfg = wpt.render_config(40, 30, spp=2, max_depth=6, rr_start=3, clamp=10.0, caustic_photons=400000,
+                            caustic_radius=0.1, caustic_grid=(32, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), mode=mode)
+    r = wpt.Renderer(scene, cam, cfg)
+    img, _ = r.render_frame()
+    want, ost = oracle_mod.Oracle(scene).render(cam, cfg)
+    assert r.stats().caustic_photons_stored == ost.caustic_photons_stored > 5000
+    assert np.array_equal(img, want), f"max |diff| {np.abs(img - want).max()}"
+
+
 def test_caustics_brighten_and_tracing_alone_changes_nothing(torch_cuda):
     host = wpt.HostScene("prism")
     scene = wpt.Scene(host.desc(), device=True, owner=host)
