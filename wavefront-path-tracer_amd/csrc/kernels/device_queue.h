@@ -32,15 +32,21 @@ constexpr int batchWavesPerEu(bool full) {
 #else
 #define WPT_TRACE_OCCUPANCY
 #endif
-// Occupancy target of the cooperative kernel with the full feature set, 0:
-// compiler's choice (249 VGPRs, 2 waves per SIMD).
+// Occupancy targets of the cooperative kernel with the full feature set and
+// radiance events, and without the full feature set; 0: the compiler's
+// choice (249 and 142 VGPRs, 2 and 3 waves per SIMD).
 #ifndef WPT_PERSISTENT_WAVES_PER_EU
 #define WPT_PERSISTENT_WAVES_PER_EU 0
 #endif
-constexpr int persistentWavesPerEu(bool full) {
-    return full && WPT_PERSISTENT_WAVES_PER_EU > 0 ? WPT_PERSISTENT_WAVES_PER_EU : 1;
+#ifndef WPT_PERSISTENT_LEAN_WAVES_PER_EU
+#define WPT_PERSISTENT_LEAN_WAVES_PER_EU 0
+#endif
+constexpr int persistentWavesPerEu(bool full, bool events) {
+    return full ? (events && WPT_PERSISTENT_WAVES_PER_EU > 0 ? WPT_PERSISTENT_WAVES_PER_EU : 1)
+                : (WPT_PERSISTENT_LEAN_WAVES_PER_EU > 0 ? WPT_PERSISTENT_LEAN_WAVES_PER_EU : 1);
 }
-#define WPT_PERSISTENT_OCCUPANCY(full) __attribute__((amdgpu_waves_per_eu(persistentWavesPerEu(full))))
+#define WPT_PERSISTENT_OCCUPANCY(full, events) \
+    __attribute__((amdgpu_waves_per_eu(persistentWavesPerEu(full, events))))
 // Occupancy target of the full-feature event shading kernel, 0: compiler's
 // choice (206 VGPRs, 2 waves per SIMD).
 #if defined(WPT_SHADE_WAVES_PER_EU) && WPT_SHADE_WAVES_PER_EU > 0

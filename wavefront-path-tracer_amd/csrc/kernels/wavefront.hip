@@ -727,7 +727,7 @@ __global__ void kAdvance(LaunchContext ctx) {
 // pcount[b % 3], appends to pcount[(b + 1) % 3] and clears pcount[(b + 2) % 3],
 // which no workgroup touches again before the barrier that ends bounce b.
 template <bool Full, bool Events>
-__global__ void __launch_bounds__(512) WPT_PERSISTENT_OCCUPANCY(Full && Events) kPersistent(LaunchContext ctx) {
+__global__ void __launch_bounds__(512) WPT_PERSISTENT_OCCUPANCY(Full, Events) kPersistent(LaunchContext ctx) {
     __shared__ AppendShared sh;
     cg::grid_group grid = cg::this_grid();
     WavefrontCounters* counters = ctx.counters;
