@@ -322,8 +322,10 @@ typedef struct wpt_render_config {
     float caustic_grid_max[3];
     /* Samples the staged and cooperative persistent schedulers render per
      * pass of a tile (0: as many as keep a pass within 2^27 paths, 64 at
-     * 1920x1080; at most 64; never more than samples_per_pixel).  Path
-     * memory grows with it, about 260 B per path of a pass.  Each pixel
+     * 1920x1080, half that for scenes shaded with radiance events; at most
+     * 64; never more than samples_per_pixel).  Path memory grows with it,
+     * about 260 B per path of a pass (about 435 B with radiance events for
+     * a scene whose vertices store up to 4 of them).  Each pixel
      * still adds its samples to the framebuffer in sample order, so images
      * do not depend on it. */
     uint32_t samples_per_pass;
