@@ -710,7 +710,7 @@ constexpr uint64_t kDefaultPassPaths = 1ull << 27;
 // Staged kernels launch at most this many times the trace kernel's resident
 // workgroups (the dispatcher still balances the workgroups).
 #ifndef WPT_STAGED_GRID_FACTOR
-#define WPT_STAGED_GRID_FACTOR 4
+#define WPT_STAGED_GRID_FACTOR 8
 #endif
 constexpr uint32_t kStagedGridFactor = WPT_STAGED_GRID_FACTOR;
 // Most radiance events per vertex that event shading stores (scenes with
