@@ -298,10 +298,6 @@ __global__ void kReset(LaunchContext ctx, uint32_t nextSel) {
 #define WPT_TRACE_REFILL 16
 #endif
 constexpr int kTraceRefill = WPT_TRACE_REFILL;
-#ifndef WPT_INTERLEAVE_LEAN
-#define WPT_INTERLEAVE_LEAN 0
-#endif
-constexpr bool kInterleaveLean = WPT_INTERLEAVE_LEAN != 0;
 
 // Rays in grid stride: the grid (at most a few times the resident
 // workgroups) bounds the traversal spill area, and the dispatcher balances
@@ -318,7 +314,7 @@ __global__ void __launch_bounds__(512) WPT_TRACE_OCCUPANCY kTrace(LaunchContext 
     const PathQueue& q = ctx.queue[sel];
     const uint32_t stride = gridDim.x * blockDim.x;
     LdsStack stack = makeStack(ctx.spill, ctx.spillThreads);
-    if constexpr (!kInterleaveLean && !Full) {
+    if (bounce == 0) {  // camera rays: coherent, their walks end nearly together
         for (uint32_t g = gid; g < count; g += stride) {
             uint32_t i = segmentSlot(v, q.segmentCapacity, g);
             HitRecord h = traceClosest<Full>(ctx.scene, loadRay(q, i), stack);
