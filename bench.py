@@ -58,10 +58,10 @@ RAY_IN_B, HIT_OUT_B = 24, 28
 # (profiles/pmc_kTrace_sphere_field_1080p_64spp.md); kBatch: one 64-sample
 # frame of the regenerating scheduler
 # (profiles/pmc_kBatch_sphere_field_1080p_64spp.md).
-MEASURED_TRAFFIC = {("kTrace", 64): 47.69e9 / 12, ("kBatch", 64): 39.21e9}
+MEASURED_TRAFFIC = {("kTrace", 64): 46.84e9 / 12, ("kBatch", 64): 39.21e9}
 MEASURED_COUNTERS = {
-    ("kTrace", 64): {"l2_hit": 0.500, "valu_busy": 0.457, "waves_per_cu": 11.96, "wait_any": 0.669,
-                     "l1_hit": 0.902, "ta_busy": 0.512, "l1_to_l2_latency_cycles": 307},
+    ("kTrace", 64): {"l2_hit": 0.515, "valu_busy": 0.757, "waves_per_cu": 12.14, "wait_any": 0.526,
+                     "lanes_active": 0.257, "l1_hit": 0.904, "ta_busy": 0.485, "l1_to_l2_latency_cycles": 306},
     ("kBatch", 64): {"l2_hit": 0.615, "valu_busy": 0.303, "waves_per_cu": 7.87, "wait_any": 0.672},
 }
 
@@ -884,9 +884,9 @@ def main():
             "traffic": MEASURED_TRAFFIC.get((kernel, args.spp)),
             "hbm_gbs_measured": None if (kernel, args.spp) not in MEASURED_TRAFFIC else
                 round(MEASURED_TRAFFIC[(kernel, args.spp)] / (trace_ms / max(1, trace_launches) / 1000.0) / 1e9, 1),
-            "limiter": "latency of dependent node and triangle fetches: L1 serves 90% and L2 half of the rest "
-                       "(307 cycles each), the address path is half busy; the algorithmic bytes count every node "
-                       "visit and the measured HBM traffic is a small part of them",
+            "limiter": "chains of dependent node and triangle fetches (L1 serves 90%, L2 half of the rest at "
+                       "306 cycles) with a quarter of the lanes active: VALU busy 76%, the address path half busy; "
+                       "the algorithmic bytes count every node visit and the measured HBM traffic is a small part",
             "counters": MEASURED_COUNTERS.get((kernel, args.spp)),
             "bytes_per_closest_query": round(per_query, 1),
             "avg_launch_ms": round(trace_ms / max(1, trace_launches), 4),
